@@ -1,0 +1,283 @@
+"""Seeded synthetic input decks in the reference's own file formats.
+
+Writes a deck directory laid out like the reference examples
+(bin/examples/*/input): ``input/CONTROL``, ``input/GEOMETRY``,
+``input/PHONON``, a boundary-condition file, a DATABASE band file and a 2D
+COMSOL ``.mphtxt`` mesh.  The formats follow the readers they feed:
+
+* CONTROL / GEOMETRY / PHONON keyword scan: src/BTEMain.cpp:97-315
+* band DATABASE rows ``v tau C lattice_ratio heat_ratio``: src/BTEBand/BTEBand.cpp:64-80
+* boundary-condition file: src/BTEBoundaryCondition/BTEBoundaryCondition.cpp:9-33
+* COMSOL 2D mesh: src/BTEMesh/ReadinMesh.cpp:245-384 (edge entity indices are
+  0-based in the file and become 1-based BC indices; domain indices are 1-based)
+
+Mesh coordinates are written in units of the domain edge (0..1) and scaled by
+ScaleX/ScaleY, as the reader does (ReadinMesh.cpp:287-294).
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# edge entity ids used by the generators (BC index = entity + 1)
+BOTTOM, RIGHT, TOP, LEFT, PORE = 0, 1, 2, 3, 4
+
+
+@dataclass
+class Mesh2D:
+    nodes: np.ndarray          # (N, 2) float64, unit-square coordinates
+    tris: np.ndarray           # (M, 3) int
+    edges: np.ndarray          # (E, 2) int, boundary edges
+    edge_entity: np.ndarray    # (E,) int, 0-based entity id
+
+
+@dataclass
+class Bands:
+    v: np.ndarray
+    tau: np.ndarray
+    C: np.ndarray
+    lr: np.ndarray
+    hr: np.ndarray
+
+
+@dataclass
+class DeckSpec:
+    mesh: Mesh2D
+    bands: Bands
+    bcs: list                      # [(index, type, value), ...]
+    scale: float = 1e-7
+    material_dim: int = 2
+    angle_method: int = 1
+    ntheta: int = 1
+    nphi: int = 8
+    order: str = "2"
+    method: str = "Synthetic"
+    matrix: str = "LU"
+    max_iter: int = 10000
+    tol_temp: float = 1e-8
+    tol_flux: float = 1e-4
+    tref: float = 300.0
+    uniform_heat: float = 0.0
+    extra_geometry: dict = field(default_factory=dict)
+
+
+# ----------------------------------------------------------------- meshes
+
+def square_split_mesh(n: int) -> Mesh2D:
+    """n x n squares on the unit square, each cut along the same diagonal."""
+    xs = np.linspace(0.0, 1.0, n + 1)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+
+    def nid(i, j):
+        return i * (n + 1) + j
+
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)
+            tris.append((a, b, c))
+            tris.append((a, c, d))
+    edges, ent = [], []
+    for k in range(n):
+        edges.append((nid(k, 0), nid(k + 1, 0))); ent.append(BOTTOM)
+        edges.append((nid(n, k), nid(n, k + 1))); ent.append(RIGHT)
+        edges.append((nid(k + 1, n), nid(k, n))); ent.append(TOP)
+        edges.append((nid(0, k + 1), nid(0, k))); ent.append(LEFT)
+    return Mesh2D(nodes, np.asarray(tris, np.int64), np.asarray(edges, np.int64), np.asarray(ent, np.int64))
+
+
+def porous_delaunay_mesh(n_target: int, pore_radius: float, seed: int) -> Mesh2D:
+    """Seeded Delaunay mesh of the unit square minus a centred circular pore.
+
+    All points (square boundary, pore circle, seeded interior points) are
+    triangulated together and triangles whose centroid lies in the pore are
+    dropped, which keeps the triangulation Delaunay and therefore every
+    direction's upwind graph acyclic.  Top and bottom edges carry the same x
+    positions so that they can be paired periodically.
+    """
+    from scipy.spatial import Delaunay
+
+    rng = np.random.default_rng(seed)
+    r = pore_radius
+    area = 1.0 - math.pi * r * r
+    # a Delaunay triangulation of N points has about 2N triangles
+    n_pts = max(int(n_target / 2), 16)
+    h = math.sqrt(area / n_pts)
+    nb = max(int(round(1.0 / h)), 4)
+    s = np.linspace(0.0, 1.0, nb + 1)
+    pts = [np.stack([s, np.zeros_like(s)], 1),
+           np.stack([s, np.ones_like(s)], 1),
+           np.stack([np.zeros(nb - 1), s[1:-1]], 1),
+           np.stack([np.ones(nb - 1), s[1:-1]], 1)]
+    n_c = 0
+    if r > 0:
+        n_c = max(int(round(2 * math.pi * r / h)), 8)
+        th = 2 * math.pi * np.arange(n_c) / n_c
+        pts.append(np.stack([0.5 + r * np.cos(th), 0.5 + r * np.sin(th)], 1))
+    # seeded interior points, rejected near walls and the pore
+    need = n_pts - sum(len(p) for p in pts)
+    inner = []
+    margin = 0.45 * h
+    while need > 0:
+        cand = rng.random((2 * need + 16, 2))
+        ok = (cand[:, 0] > margin) & (cand[:, 0] < 1 - margin) & (cand[:, 1] > margin) & (cand[:, 1] < 1 - margin)
+        if r > 0:
+            ok &= np.hypot(cand[:, 0] - 0.5, cand[:, 1] - 0.5) > r + margin
+        cand = cand[ok][:need]
+        inner.append(cand)
+        need -= len(cand)
+    pts.append(np.concatenate(inner, 0) if inner else np.zeros((0, 2)))
+    P = np.concatenate(pts, 0)
+    tri = Delaunay(P).simplices.astype(np.int64)
+    cen = P[tri].mean(axis=1)
+    if r > 0:
+        tri = tri[np.hypot(cen[:, 0] - 0.5, cen[:, 1] - 0.5) > r]
+    # drop unused points, renumber
+    used = np.unique(tri)
+    remap = -np.ones(len(P), np.int64)
+    remap[used] = np.arange(len(used))
+    P = P[used]
+    tri = remap[tri]
+    # boundary edges = edges used by exactly one triangle
+    e = np.concatenate([tri[:, [1, 2]], tri[:, [2, 0]], tri[:, [0, 1]]], 0)
+    es = np.sort(e, 1)
+    _, inv, cnt = np.unique(es, axis=0, return_inverse=True, return_counts=True)
+    bmask = cnt[inv.ravel()] == 1
+    bedges = e[bmask]
+    mid = P[bedges].mean(axis=1)
+    tol = 1e-9
+    ent = np.full(len(bedges), -1, np.int64)
+    ent[np.abs(mid[:, 1]) < tol] = BOTTOM
+    ent[np.abs(mid[:, 0] - 1) < tol] = RIGHT
+    ent[np.abs(mid[:, 1] - 1) < tol] = TOP
+    ent[np.abs(mid[:, 0]) < tol] = LEFT
+    ent[(ent < 0) & (np.hypot(mid[:, 0] - 0.5, mid[:, 1] - 0.5) < r + h)] = PORE
+    if (ent < 0).any():
+        raise RuntimeError("porous_delaunay_mesh: unclassified boundary edge")
+    return Mesh2D(P, tri, bedges, ent)
+
+
+# ------------------------------------------------------------------ bands
+
+def gray_band(v=6400.0, tau=15.6e-12, C=1.66e6) -> Bands:
+    one = np.ones(1)
+    return Bands(v * one, tau * one, C * one, one.copy(), one.copy())
+
+
+def debye_bands(nb: int, seed: int, omega_max=1e14, vmax=6400.0, mfp_min=1e-9, mfp_max=1e-5,
+                T=300.0, c_total=1.66e6, sigma=0.1) -> Bands:
+    """Isotropic Debye-like dispersion on nb bands.
+
+    omega_b = (b - 1/2) omega_max / nb, v_b = vmax cos(pi omega_b / (2 omega_max));
+    mean free paths log-uniform from mfp_max (lowest band) down to mfp_min
+    (highest band), tau_b = MFP_b / v_b times a seeded LogNormal(0, sigma)
+    factor; C_b follows the Debye weight omega^2 x^2 e^x / (e^x - 1)^2 at T and
+    sums to c_total.  lattice_ratio_b = (C/tau)_b / sum(C/tau) and
+    heat_ratio_b = C_b / sum(C).
+    """
+    rng = np.random.default_rng(seed)
+    b = np.arange(nb) + 0.5
+    w = b * omega_max / nb
+    v = vmax * np.cos(np.pi * w / (2 * omega_max))
+    mfp = mfp_max * (mfp_min / mfp_max) ** (b / nb)
+    tau = mfp / v * rng.lognormal(0.0, sigma, nb)
+    hbar, kb = 1.054571817e-34, 1.380649e-23
+    x = hbar * w / (kb * T)
+    wt = w ** 2 * x ** 2 * np.exp(x) / np.expm1(x) ** 2
+    C = c_total * wt / wt.sum()
+    lr = (C / tau) / (C / tau).sum()
+    hr = C / C.sum()
+    return Bands(v, tau, C, lr, hr)
+
+
+# ------------------------------------------------------------------ writers
+
+def _f(x) -> str:
+    """Shortest round-trip decimal text of a float."""
+    return repr(float(x))
+
+
+def write_mphtxt(path: str, m: Mesh2D) -> None:
+    """COMSOL text mesh, 2D: vtx/edg/tri element types."""
+    out = []
+    out += ["# Created by COMSOL Multiphysics.", "", "# Major & minor version", "0 1 ",
+            "1 # number of tags", "# Tags", "5 mesh1 ", "1 # number of types", "# Types", "3 obj ", "",
+            "# --------- Object 0 ----------", "", "0 0 1 ", "4 Mesh # class", "4 # version",
+            "2 # sdim", f"{len(m.nodes)} # number of mesh vertices", "0 # lowest mesh vertex index", "",
+            "# Mesh vertex coordinates"]
+    out += [f"{_f(x)} {_f(y)} " for x, y in m.nodes.tolist()]
+    out += ["", "2 # number of element types", "", "# Type #0", "", "3 edg # type name", "", "",
+            "2 # number of vertices per element", f"{len(m.edges)} # number of elements", "# Elements"]
+    out += [f"{a} {b} " for a, b in m.edges.tolist()]
+    out += ["", f"{len(m.edges)} # number of geometric entity indices", "# Geometric entity indices"]
+    out += [f"{e} " for e in m.edge_entity.tolist()]
+    out += ["", "# Type #1", "", "3 tri # type name", "", "",
+            "3 # number of vertices per element", f"{len(m.tris)} # number of elements", "# Elements"]
+    out += [f"{a} {b} {c} " for a, b, c in m.tris.tolist()]
+    out += ["", f"{len(m.tris)} # number of geometric entity indices", "# Geometric entity indices"]
+    out += ["1 "] * len(m.tris)
+    out.append("")
+    with open(path, "w") as f:
+        f.write("\n".join(out))
+
+
+def write_deck(root: str, spec: DeckSpec) -> str:
+    """Write a complete deck under ``root``; returns ``root``."""
+    inp = os.path.join(root, "input")
+    os.makedirs(inp, exist_ok=True)
+    write_mphtxt(os.path.join(inp, "mesh.mphtxt"), spec.mesh)
+    bd = spec.bands
+    with open(os.path.join(inp, "bands.dat"), "w") as f:
+        for k in range(len(bd.v)):
+            f.write(" ".join(_f(a[k]) for a in (bd.v, bd.tau, bd.C, bd.lr, bd.hr)) + "\n")
+    with open(os.path.join(inp, "bc.dat"), "w") as f:
+        f.write(f"{len(spec.bcs)} #Number of boundaries\n\n")
+        f.write("BoundaryIndex BoundaryType BoundaryValue\n")
+        for idx, typ, val in spec.bcs:
+            f.write(f"{int(idx)} {int(typ)} {_f(val)}\n")
+    with open(os.path.join(inp, "CONTROL"), "w") as f:
+        f.write(f"State=\nSteady\nOrder=\n{spec.order}\nIterativeScheme=\n{spec.method}\n"
+                f"MatrixSolver=\n{spec.matrix}\nMaxNumIter=\n{spec.max_iter}\n"
+                f"ResidualTemp=\n{_f(spec.tol_temp)}\nResidualFlux=\n{_f(spec.tol_flux)}\n"
+                f"Tref=\n{_f(spec.tref)}\n")
+    with open(os.path.join(inp, "GEOMETRY"), "w") as f:
+        f.write(f"GeometryDimension=\n2\nScaleX=\n{_f(spec.scale)}\nScaleY=\n{_f(spec.scale)}\n"
+                f"BCFile=\ninput/bc.dat\nMeshType=\nCOMSOL\nMeshFile=\ninput/mesh.mphtxt\n")
+        if spec.uniform_heat:
+            f.write(f"UniformHeat=\n{_f(spec.uniform_heat)}\n")
+    with open(os.path.join(inp, "PHONON"), "w") as f:
+        f.write(f"MaterialNumber=\n1\nBandNumber=\n{len(bd.v)}\n\n"
+                f"MaterialFile 1= DATABASE\ninput/bands.dat\n\n\n"
+                f"GeometryMaterialType=\n1 1\n\n"
+                f"MaterialDimension=\n{spec.material_dim}\nAngle_method=\n{spec.angle_method}\n"
+                f"Ntheta=\n{spec.ntheta}\nNphi=\n{spec.nphi}\n")
+    return root
+
+
+# ------------------------------------------------------- named workloads
+
+def gray_square_spec(n=40, **kw) -> DeckSpec:
+    """2D gray film: n x n split squares, left 301 K / right 299 K, top and
+    bottom specular, 1 band (v 6400 m/s, tau 15.6 ps, C 1.66e6), CADOM
+    in-plane directions (4 * nphi)."""
+    bcs = [(BOTTOM + 1, 3, 0.0), (RIGHT + 1, 1, 299.0), (TOP + 1, 3, 0.0), (LEFT + 1, 1, 301.0)]
+    base = dict(mesh=square_split_mesh(n), bands=gray_band(), bcs=bcs, scale=1e-7,
+                material_dim=2, angle_method=1, nphi=8, tol_temp=1e-8)
+    base.update(kw)
+    return DeckSpec(**base)
+
+
+def porous_spec(n_target=50000, nbands=40, nphi=16, pore_radius=0.3, seed=1, scale=5e-7, **kw) -> DeckSpec:
+    """2D nanoporous film: unit square (scaled) minus a centred circular pore
+    (diffuse), left/right isothermal at 301/299 K, top/bottom periodic,
+    Debye-like bands, CADOM in-plane directions (4 * nphi)."""
+    bcs = [(BOTTOM + 1, -(TOP + 1), 0.0), (RIGHT + 1, 1, 299.0), (TOP + 1, -(BOTTOM + 1), 0.0),
+           (LEFT + 1, 1, 301.0), (PORE + 1, 2, 0.0)]
+    base = dict(mesh=porous_delaunay_mesh(n_target, pore_radius, seed), bands=debye_bands(nbands, seed),
+                bcs=bcs, scale=scale, material_dim=2, angle_method=1, nphi=nphi, tol_temp=1e-8)
+    base.update(kw)
+    return DeckSpec(**base)

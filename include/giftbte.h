@@ -1,0 +1,150 @@
+/*
+ * giftbte.h - C ABI of libgiftbte_hip.so, the MI355X solver for GiftBTE's
+ * steady StaticBTESynthetic path (Order 2, IterativeScheme Synthetic,
+ * MatrixSolver LU) and its StaticFourier warm start.
+ *
+ * The reference exposes this path as C++ classes with no FFI:
+ *   StaticFourier(BTEMesh*, BTEBoundaryCondition*, BTEBand*, int, int) + _solve()
+ *       includes/StaticFourier/StaticFourier.h:91-102
+ *   StaticBTESynthetic(BTEMesh*, BTEBoundaryCondition*, BTEBand*, BTEAngle*, int, int)
+ *       + solve(int Use_Backup, int max_iter, int USE_limiter, double tolT, double tolQ)
+ *       includes/StaticBTESynthetic/StaticBTESynthetic.h:177,221
+ *   driven by SolutionAll::_Fourier_Solver / _BTE_Solver
+ *       src/Solution/Solution.cpp:236-414
+ * The entry points below replace those calls one for one.  Every function
+ * returns GBTE_OK or an error code; gbte_last_error() gives the message.
+ * Nothing here calls exit(): the reference's exit() paths
+ * (e.g. src/BTEMesh/BTEMesh.cpp:868-879, StaticBTESynthetic.cpp:2719-2723)
+ * become status codes.  One handle per host thread; no MPI.
+ */
+#ifndef GIFTBTE_H
+#define GIFTBTE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GBTE_OK = 0,
+    GBTE_BAD_INPUT = 1,   /* deck / mesh / argument error                        */
+    GBTE_DIVERGED = 2,    /* error grew 10 times in a row                       */
+    GBTE_NAN = 3,         /* NaN / Inf in a residual                            */
+    GBTE_HIP_ERROR = 4,   /* HIP runtime failure, or no GPU                     */
+    GBTE_UNSUPPORTED = 5  /* a reference feature outside this build             */
+};
+
+/* Precision of the transport sweep. */
+enum { GBTE_FP64 = 0, GBTE_FP32_SWEEP = 1 };
+
+/* Read-only view of the flattened problem tables, laid out exactly as the
+ * reference's StaticBTESynthetic constructor flattens BTEMesh/BTEBand/BTEAngle
+ * (src/StaticBTESynthetic/StaticBTESynthetic.cpp:224-626): 6 face slots per
+ * cell, 4 vertices per face slot, 8 vertices per cell, boundaries with two
+ * (cell, face) sides, unique faces with two sides.  Owned by the problem. */
+typedef struct gbte_tables {
+    int dim;              /* geometry dimension (2 or 3)                       */
+    int n_cells, n_nodes, n_bounds, n_faces, n_bands, n_dirs, n_matters;
+    int material_dim;     /* MaterialDimension (2 or 3)                        */
+    double Lx, Ly, Lz;
+    /* cells */
+    const int *face_size;          /* [n_cells]                   */
+    const int *face_neighbor;      /* [n_cells*6]  -1 on boundary */
+    const int *face_bound;         /* [n_cells*6]  -1 if interior */
+    const int *face_index;         /* [n_cells*6]  unique face id */
+    const double *face_area;       /* [n_cells*6]                 */
+    const double *face_nx, *face_ny, *face_nz;   /* [n_cells*6]   */
+    const double *face_cx, *face_cy, *face_cz;   /* [n_cells*6]   */
+    const int *face_nvert;         /* [n_cells*6]                 */
+    const int *face_vert;          /* [n_cells*6*4]               */
+    const int *cell_nvert;         /* [n_cells]                   */
+    const int *cell_vert;          /* [n_cells*8]                 */
+    const double *volume, *cx, *cy, *cz, *heat_source;   /* [n_cells] */
+    const int *matter;             /* [n_cells]                   */
+    const int *nbr_ptr, *nbr_idx;  /* same-matter vertex neighbours, CSR */
+    const double *node_x, *node_y, *node_z;              /* [n_nodes] */
+    /* boundaries */
+    const int *bound_cell, *bound_face;    /* [n_bounds*2], -1 if absent */
+    const int *bound_type;                 /* [n_bounds]                */
+    const double *bound_temp;              /* [n_bounds] deck value     */
+    const int *bound_connect;              /* [n_bounds] periodic pair  */
+    /* unique faces */
+    const int *fface_cell, *fface_face;    /* [n_faces*2]               */
+    const int *fface_bound;                /* [n_faces]                 */
+    /* materials, bands, directions */
+    const double *kappa_bulk, *capacity_bulk;           /* [n_matters]          */
+    const double *vg, *tau, *cap, *lr, *hr;             /* [n_matters*n_bands] */
+    const double *dir_x, *dir_y, *dir_z;                /* [n_dirs]             */
+    const double *weight;          /* [n_dirs] angle weight / solid angle */
+    const double *initial_temp;    /* [n_cells] Tref unless a file sets it */
+    /* CONTROL values from the deck */
+    int max_iter;
+    double tol_temp, tol_flux, tref;
+} gbte_tables;
+
+typedef struct gbte_problem gbte_problem;
+typedef struct gbte_handle gbte_handle;
+
+typedef struct gbte_options {
+    int precision;        /* GBTE_FP64 or GBTE_FP32_SWEEP                       */
+    int device;           /* HIP device ordinal                                 */
+    int verbose;          /* print the reference's per-iteration log lines      */
+    double macro_rtol;    /* relative residual of the FP64 synthetic solve (0: default) */
+} gbte_options;
+
+typedef struct gbte_stats {
+    int iterations;       /* outer iterations run                               */
+    int macro_passes;     /* synthetic (macro) solves, summed                   */
+    int fourier_passes;   /* Fourier warm-start passes                          */
+    int converged;        /* 1 if tolT and tolQ were met                        */
+    double err_temp, err_flux;   /* last errors (StaticBTESynthetic.cpp:2565-2611) */
+    double solve_ms;      /* wall time of the outer loop                        */
+    double sweep_ms;      /* device time of the transport sweeps (HIP events)   */
+} gbte_stats;
+
+/* ---- problem construction (replaces BTEMain.cpp:87-362: decks, BTEBand,
+ *      BTEBoundaryCondition, DistributeMesh, BTEAngle) ---------------------- */
+
+/* Read input/CONTROL, input/GEOMETRY, input/PHONON and the files they name,
+ * relative to deck_dir, and build the tables.  Replaces the deck scan in
+ * src/BTEMain.cpp:87-315 and the object set-up in :327-362. */
+int gbte_problem_load(const char *deck_dir, gbte_problem **out);
+/* Pointer view of the problem's tables (valid until gbte_problem_free). */
+int gbte_problem_tables(const gbte_problem *p, gbte_tables *out);
+void gbte_problem_free(gbte_problem *p);
+
+/* ---- solver (replaces StaticFourier / StaticBTESynthetic) --------------- */
+
+/* Allocate device state for a problem.  Replaces the two solver
+ * constructors (StaticFourier.cpp:82-268, StaticBTESynthetic.cpp:224-626). */
+int gbte_create(const gbte_problem *p, const gbte_options *opts, gbte_handle **out);
+/* Fourier warm start, StaticFourier::_solve (StaticFourier.cpp:934-976),
+ * from T = initial temperature, as SolutionAll::_Fourier_Solver
+ * (Solution.cpp:236-260).  Leaves T and q in the handle. */
+int gbte_fourier_solve(gbte_handle *h, gbte_stats *stats);
+/* Overwrite the starting T and q (Solution.cpp:268-273).  Any pointer may be
+ * NULL to keep the current field. */
+int gbte_set_initial(gbte_handle *h, const double *T, const double *qx, const double *qy, const double *qz);
+/* StaticBTESynthetic::solve(0, max_iter, 0, tolT, tolQ)
+ * (StaticBTESyntheticSolver.cpp:12-247). */
+int gbte_solve(gbte_handle *h, int max_iter, double tol_temp, double tol_flux, gbte_stats *stats);
+/* Output fields (caller-owned, n_cells doubles each; T_band n_bands*n_cells;
+ * NULL skips a field): macro (lattice) temperature, energy-equivalent
+ * temperature (what Solution.cpp:408-413 copies back), heat flux, band
+ * temperatures.  StaticBTESynthetic.cpp:2613-2653. */
+int gbte_get_fields(gbte_handle *h, double *T_lattice, double *T_energy, double *qx, double *qy, double *qz,
+                    double *T_band);
+/* Effective conductivity sum_i q_axis,i V_i / (A_perp dT), with A_perp the
+ * domain extent normal to the axis (L_y for axis 0 in 2D).  No reference
+ * function; the MATLAB check bin/examples/cross-plane/verify_kappa_crossplane.m:8
+ * computes mean(q) L / dT, which equals this on full equal-area domains. */
+int gbte_keff(gbte_handle *h, int axis, double delta_T, double *keff);
+const char *gbte_last_error(const gbte_handle *h);
+const char *gbte_problem_error(void);
+void gbte_destroy(gbte_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GIFTBTE_H */

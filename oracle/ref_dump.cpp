@@ -1,0 +1,397 @@
+// Dump driver for the compiled reference (test infrastructure only).
+//
+// Links against the reference GiftBTE objects built by oracle/ref.mk and
+// replays the steady flow of src/BTEMain.cpp:15-421 at np=1 for the
+// Order=2 / IterativeScheme=Synthetic / MatrixSolver=LU branch
+// (src/Solution/Solution.cpp:262-414), then writes every field the GPU
+// solver is compared against as .npy files at full FP64 precision.  The
+// reference's own .dat outputs carry 6 significant digits only
+// (src/StaticBTESynthetic/StaticBTESynthetic.cpp:2613-2653).
+//
+// usage: ref_dump <deck_dir> <out_dir> <max_iter> <tolT> <tolQ> [stage_band stage_dir]
+//   deck_dir  a COPY of a deck (the reference writes its .dat files into the
+//             working directory on every iteration)
+//   out_dir   where the .npy files go
+//   max_iter  outer-iteration cap handed to StaticBTESynthetic::solve
+//   tolT/tolQ stop thresholds (0 0 runs exactly max_iter iterations)
+//   stage_*   optional: also dump every per-system stage of iteration 0 for
+//             system (band, direction)
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include <fstream>
+#include <sstream>
+#include <iostream>
+#include <unistd.h>
+#include <sys/stat.h>
+
+#include "StaticBTESynthetic/StaticBTESynthetic.h"
+#include "StaticFourier/StaticFourier.h"
+#include "BTEMesh/Distribute.h"
+#include "Solution/Solution.h"
+
+namespace {
+
+std::string g_out;
+
+void write_npy(const std::string &name, const char *descr, size_t elem,
+               const void *data, const std::vector<size_t> &shape) {
+    std::string sh = "(";
+    size_t n = 1;
+    for (size_t k = 0; k < shape.size(); ++k) {
+        sh += std::to_string(shape[k]);
+        sh += (shape.size() == 1) ? "," : (k + 1 < shape.size() ? ", " : "");
+        n *= shape[k];
+    }
+    sh += ")";
+    std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + sh + ", }";
+    size_t total = 10 + hdr.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    hdr.append(pad, ' ');
+    hdr.push_back('\n');
+    std::string path = g_out + "/" + name + ".npy";
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) { perror(path.c_str()); exit(2); }
+    const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    fwrite(magic, 1, 8, f);
+    uint16_t hl = (uint16_t)hdr.size();
+    fwrite(&hl, 2, 1, f);
+    fwrite(hdr.data(), 1, hdr.size(), f);
+    if (n) fwrite(data, elem, n, f);
+    fclose(f);
+}
+void dump_d(const std::string &name, const double *p, std::vector<size_t> shape) { write_npy(name, "<f8", 8, p, shape); }
+void dump_i(const std::string &name, const int *p, std::vector<size_t> shape) { write_npy(name, "<i4", 4, p, shape); }
+void dump_d(const std::string &name, const std::vector<double> &v) { dump_d(name, v.data(), {v.size()}); }
+void dump_i(const std::string &name, const std::vector<int> &v) { dump_i(name, v.data(), {v.size()}); }
+
+// Keyword scan with the reference's semantics (src/BTEMain.cpp:97-193):
+// a line that contains the keyword and no '!' makes the next stream token the
+// value.
+struct Deck {
+    std::string order = "2", method = "Sequential", matrix = "LU", state = "Steady";
+    double tref = 300, errT = 1e-6, errQ = 1e-4;
+    int maxIter = 10000;
+    int dimGeo = -1;
+    double Lx = 1, Ly = 1, Lz = 1, uniformHeat = 0;
+    std::string bcFile, meshType, meshFile, heatType, heatFile, multiscale;
+    int dimMat = 3, angleMethod = 2, ntheta = 4, nphi = 4;
+};
+
+bool has_kw(const std::string &s, const char *kw) {
+    return s.find(kw) != std::string::npos && s.find('!') == std::string::npos;
+}
+
+Deck parse_deck() {
+    Deck d;
+    {
+        std::ifstream f("input/CONTROL");
+        std::string s;
+        while (std::getline(f, s)) {
+            if (has_kw(s, "State")) f >> d.state;
+            if (has_kw(s, "Order")) f >> d.order;
+            if (has_kw(s, "IterativeScheme")) f >> d.method;
+            if (has_kw(s, "MatrixSolver")) f >> d.matrix;
+            if (has_kw(s, "Tref")) f >> d.tref;
+            if (has_kw(s, "MaxNumIter")) f >> d.maxIter;
+            if (has_kw(s, "ResidualTemp")) f >> d.errT;
+            if (has_kw(s, "ResidualFlux")) f >> d.errQ;
+            if (has_kw(s, "Multiscale")) f >> d.multiscale;
+        }
+    }
+    {
+        std::ifstream f("input/GEOMETRY");
+        std::string s;
+        while (std::getline(f, s)) {
+            if (has_kw(s, "GeometryDimension")) f >> d.dimGeo;
+            if (has_kw(s, "ScaleX")) f >> d.Lx;
+            if (has_kw(s, "ScaleY")) f >> d.Ly;
+            if (has_kw(s, "ScaleZ")) f >> d.Lz;
+            if (has_kw(s, "UniformHeat")) f >> d.uniformHeat;
+            if (has_kw(s, "BCFile")) f >> d.bcFile;
+            if (has_kw(s, "MeshType")) f >> d.meshType;
+            if (has_kw(s, "MeshFile")) f >> d.meshFile;
+            if (has_kw(s, "HeatType")) f >> d.heatType;
+            if (has_kw(s, "HeatFile")) f >> d.heatFile;
+        }
+    }
+    {
+        std::ifstream f("input/PHONON");
+        std::string s;
+        while (std::getline(f, s)) {
+            if (has_kw(s, "MaterialDimension")) f >> d.dimMat;
+            if (has_kw(s, "Angle_method")) f >> d.angleMethod;
+            if (has_kw(s, "Ntheta")) f >> d.ntheta;
+            if (has_kw(s, "Nphi")) f >> d.nphi;
+        }
+    }
+    return d;
+}
+
+// Flattened solver tables built by the reference constructor
+// (src/StaticBTESynthetic/StaticBTESynthetic.cpp:224-626).
+void dump_tables(StaticBTESynthetic &s, const std::string &pfx) {
+    size_t nc = s.numCell, nb = s.numBound, nf = s.numFace, nn = s.numNode;
+    dump_i(pfx + "elementFaceSize", s.elementFaceSize, {nc});
+    dump_i(pfx + "elementFaceNeighbor", s.elementFaceNeighobr, {nc, 6});
+    dump_i(pfx + "elementFaceBound", s.elementFaceBound, {nc, 6});
+    dump_i(pfx + "elementFaceIndex", s.elementFaceIndex, {nc, 6});
+    dump_d(pfx + "elementFaceArea", s.elementFaceArea, {nc, 6});
+    dump_d(pfx + "elementFaceNormX", s.elementFaceNormX, {nc, 6});
+    dump_d(pfx + "elementFaceNormY", s.elementFaceNormY, {nc, 6});
+    dump_d(pfx + "elementFaceNormZ", s.elementFaceNormZ, {nc, 6});
+    dump_d(pfx + "elementFaceCenterX", s.elementFaceCenterX, {nc, 6});
+    dump_d(pfx + "elementFaceCenterY", s.elementFaceCenterY, {nc, 6});
+    dump_d(pfx + "elementFaceCenterZ", s.elementFaceCenterZ, {nc, 6});
+    dump_i(pfx + "elementVertexesSize", s.elementVertexesSize, {nc});
+    dump_d(pfx + "elementVolume", s.elementVolume, {nc});
+    dump_d(pfx + "elementCenterX", s.elementCenterX, {nc});
+    dump_d(pfx + "elementCenterY", s.elementCenterY, {nc});
+    dump_d(pfx + "elementCenterZ", s.elementCenterZ, {nc});
+    dump_d(pfx + "elementHeatSource", s.elementHeatSource, {nc});
+    dump_i(pfx + "matter", s.matter, {nc});
+    std::vector<int> fv(nc * 6 * 4, -1), fvs(nc * 6, 0), cv(nc * 8, -1);
+    for (size_t i = 0; i < nc; ++i) {
+        for (int j = 0; j < s.elementVertexesSize[i]; ++j) cv[i * 8 + j] = s.elementVertexes[i * 8 + j];
+        for (int j = 0; j < s.elementFaceSize[i]; ++j) {
+            fvs[i * 6 + j] = s.elementFaceVertexesSize[i * 6 + j];
+            for (int k = 0; k < fvs[i * 6 + j]; ++k) fv[(i * 6 + j) * 4 + k] = s.elementFaceVertexes[(i * 6 + j) * 4 + k];
+        }
+    }
+    dump_i(pfx + "elementVertexes", cv.data(), {nc, 8});
+    dump_i(pfx + "elementFaceVertexesSize", fvs.data(), {nc, 6});
+    dump_i(pfx + "elementFaceVertexes", fv.data(), {nc, 6, 4});
+    // vertex-neighbour list (same matter), CSR
+    std::vector<int> nptr(nc + 1, 0), nidx;
+    for (size_t i = 0; i < nc; ++i) {
+        for (int c : s.elementNeighborList[i]) nidx.push_back(c);
+        nptr[i + 1] = (int)nidx.size();
+    }
+    dump_i(pfx + "neighborPtr", nptr);
+    dump_i(pfx + "neighborIdx", nidx);
+    dump_d(pfx + "nodeX", s.nodeX, {nn});
+    dump_d(pfx + "nodeY", s.nodeY, {nn});
+    dump_d(pfx + "nodeZ", s.nodeZ, {nn});
+    std::vector<int> bc(nb * 2), bf(nb * 2);
+    for (size_t i = 0; i < nb; ++i)
+        for (int k = 0; k < 2; ++k) { bc[i * 2 + k] = s.boundaryCell[i][k]; bf[i * 2 + k] = s.boundaryFace[i][k]; }
+    dump_i(pfx + "boundaryCell", bc.data(), {nb, 2});
+    dump_i(pfx + "boundaryFace", bf.data(), {nb, 2});
+    dump_i(pfx + "boundaryType", s.boundaryType, {nb});
+    dump_d(pfx + "boundaryThermal", s.boundaryThermal, {nb});
+    dump_i(pfx + "boundaryConnect", s.boundaryConnect, {nb});
+    std::vector<int> fc(nf * 2), ff(nf * 2);
+    for (size_t i = 0; i < nf; ++i)
+        for (int k = 0; k < 2; ++k) { fc[i * 2 + k] = s.faceCell[i][k]; ff[i * 2 + k] = s.faceFace[i][k]; }
+    dump_i(pfx + "faceCell", fc.data(), {nf, 2});
+    dump_i(pfx + "faceFace", ff.data(), {nf, 2});
+    dump_i(pfx + "faceBound", s.faceBound, {nf});
+    size_t nm = s.numofMatter, nB = s.numBand, nD = s.numDirection;
+    dump_d(pfx + "kappaBulk", s.kappaBulk, {nm});
+    dump_d(pfx + "capacityBulk", s.capacityBulk, {nm});
+    dump_d(pfx + "directionX", s.directionX, {nD});
+    dump_d(pfx + "directionY", s.directionY, {nD});
+    dump_d(pfx + "directionZ", s.directionZ, {nD});
+    std::vector<double> w(nD), vg(nm * nB), tau(nm * nB), cap(nm * nB), lr(nm * nB), hr(nm * nB);
+    for (size_t d = 0; d < nD; ++d) w[d] = s.modeWeight[0][0][d];
+    for (size_t m = 0; m < nm; ++m)
+        for (size_t b = 0; b < nB; ++b) {
+            // |v| recovered from the stored v*s products of the first direction
+            double vx = s.groupVelocityX[m][b][0], vy = s.groupVelocityY[m][b][0], vz = s.groupVelocityZ[m][b][0];
+            double sx = s.directionX[0], sy = s.directionY[0], sz = s.directionZ[0];
+            double ss = sx * sx + sy * sy + sz * sz;
+            vg[m * nB + b] = (vx * sx + vy * sy + vz * sz) / ss;
+            tau[m * nB + b] = s.relaxationTime[m][b][0];
+            cap[m * nB + b] = s.heatCapacity[m][b][0];
+            lr[m * nB + b] = s.latticeRatio[m][b][0];
+            hr[m * nB + b] = s.heatRatio[m][b][0];
+        }
+    dump_d(pfx + "modeWeight", w);
+    dump_d(pfx + "groupVelocity", vg.data(), {nm, nB});
+    dump_d(pfx + "relaxationTime", tau.data(), {nm, nB});
+    dump_d(pfx + "heatCapacity", cap.data(), {nm, nB});
+    dump_d(pfx + "latticeRatio", lr.data(), {nm, nB});
+    dump_d(pfx + "heatRatio", hr.data(), {nm, nB});
+    int dims[8] = {s.numCell, s.numBound, s.numFace, s.numNode, s.numBand, s.numDirection, s.numofMatter, s.dimension};
+    dump_i(pfx + "sizes", dims, {8});
+    double scale[3] = {s.L_x, s.L_y, s.L_z};
+    dump_d(pfx + "scale", scale, {3});
+}
+
+void dump_cellmatrix(StaticBTESynthetic &s, const std::string &name) {
+    int dim = s.dimension;
+    std::vector<double> cm;
+    for (int i = 0; i < s.numCell; ++i) {
+        size_t n = s.elementNeighborList[i].size();
+        bool set = (dim == 2) ? n > 2 : n > 3;
+        for (int r = 0; r < dim; ++r)
+            for (size_t m = 0; m < n; ++m) cm.push_back(set ? s.CellMatrix[i][r][m] : 0.0);
+    }
+    dump_d(name, cm);
+}
+
+void init_solver_state(StaticBTESynthetic &solver, DistributeMesh &dm, SolutionAll &sa) {
+    // Solution.cpp:268-285 (hand-over from the Fourier warm start)
+    for (size_t i = 0; i < dm.volumeElementsBTE.size(); ++i) {
+        solver.temperature[i] = sa.solution.Temperature[dm.elementProjection[i]];
+        solver.heatFluxXGlobal[i] = sa.solution.heatFluxX[dm.elementProjection[i]];
+        solver.heatFluxYGlobal[i] = sa.solution.heatFluxY[dm.elementProjection[i]];
+        solver.heatFluxZGlobal[i] = sa.solution.heatFluxZ[dm.elementProjection[i]];
+    }
+    for (size_t ib = 0; ib < dm.BTEMeshes.Boundaries.size(); ++ib) {
+        int ie = dm.BTEMeshes.Boundaries[ib].cellindex[0];
+        int jface = dm.BTEMeshes.Boundaries[ib].faceindex[0];
+        Cell &fc = dm.FourierMeshes.Elements[dm.elementProjection[ie]];
+        if (fc.faces[jface].bound == -1) {
+            double d1 = get_distance(fc.center, fc.faces[jface].center);
+            double d2 = get_distance(dm.FourierMeshes.Elements[fc.faces[jface].neighbor].center, fc.faces[jface].center);
+            double t = sa.solution.Temperature[dm.elementProjection[ie]] / d1 +
+                       sa.solution.Temperature[fc.faces[jface].neighbor] / d2;
+            solver.boundaryThermal[ib] = t / (1.0 / d1 + 1.0 / d2);
+        }
+    }
+}
+
+void stage_dump(StaticBTESynthetic &s, int sb, int sd) {
+    // Setup part of StaticBTESyntheticSolver.cpp:15-62, then iteration 0 of
+    // the per-system loop (:78-127) for system (sb, sd), one stage at a time.
+    s._set_face_matrix();
+    s._set_cell_matrix_larger();
+    s._set_matrix("LU");
+    s.errorIncreaseTime = 0;
+    s._set_initial(0);
+    for (int d = 0; d < s.numDirectionLocal; d++)
+        for (int b = 0; b < s.numBandLocal; ++b) s._get_coefficient(b, d);
+    s._get_coefficient_macro();
+    s._get_face_matrix();
+    s._get_CellMatrix_larger();
+    dump_cellmatrix(s, "cellMatrix");
+    size_t nc = s.numCell, nb = s.numBound, nB = s.numBand, nD = s.numDirection;
+    for (int d = 0; d < s.numDirectionLocal; d++)
+        for (int b = 0; b < s.numBandLocal; ++b) {
+            s._get_gradient_larger(0, b, d);
+            s._get_bound_ee(b, d);
+            MPI_Allgather(s.eboundLocal + b * nb * 2, nb * 2, MPI_DOUBLE,
+                          s.ebound + nb * 2 * d + nD * nb * 2 * b, nb * 2, MPI_DOUBLE, MPI_COMM_WORLD);
+        }
+    s._set_bound_ee_1();
+    dump_d("stage_ebound_init", s.ebound, {nB, nD, nb, 2});
+    s.copy();
+    dump_d("stage_e_old", s.energyDensity[sb][sd], {nc});
+    dump_d("stage_T_old", s.temperatureOld, {nc});
+    s._get_gradient_larger(0, sb, sd);
+    dump_d("stage_gradX", s.gradientX, {nc});
+    dump_d("stage_gradY", s.gradientY, {nc});
+    dump_d("stage_gradZ", s.gradientZ, {nc});
+    s._get_Re(sb, sd);
+    dump_d("stage_Re", s.Re, {nc});
+    s.sol1 = s.cg[sb][sd].solve(s.Re1);
+    for (size_t i = 0; i < nc; ++i) s.energyDensity[sb][sd][i] += s.sol1(i);
+    dump_d("stage_e_new", s.energyDensity[sb][sd], {nc});
+    s._get_bound_ee(sb, sd);
+    dump_d("stage_ebound_sys", s.ebound + (sb * nD + sd) * nb * 2, {nb, 2});
+    s._set_vertex_energydensity(sb, sd);
+    dump_d("stage_e_vertex", s.energyDensityVertex, {(size_t)s.numNode * s.numofMatter});
+    s._get_face_gradient(sb, sd);
+    size_t nf = s.numFace;
+    dump_d("stage_gradFaceX", s.gradientFaceX, {nf, 2});
+    dump_d("stage_gradFaceY", s.gradientFaceY, {nf, 2});
+    dump_d("stage_gradFaceZ", s.gradientFaceZ, {nf, 2});
+    s._get_non_fourier(sb, sd);
+    dump_d("stage_ReMacro", s.ReMacroLocal, {nc});
+    s._recover_temperature(sb, sd);
+    s._get_total_energy(sb, sd);
+    s._get_heat_flux(sb, sd);
+    dump_d("stage_T_sys", s.temperatureLocal, {nc});
+    dump_d("stage_E_sys", s.totalEnergyLocal, {nc});
+    dump_d("stage_qx_sys", s.heatFluxXLocal, {nc});
+    dump_d("stage_qy_sys", s.heatFluxYLocal, {nc});
+    dump_d("stage_qz_sys", s.heatFluxZLocal, {nc});
+    int sys[2] = {sb, sd};
+    dump_i("stage_system", sys, {2});
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    MPI_Init(&argc, &argv);
+    if (argc < 6) {
+        fprintf(stderr, "usage: ref_dump <deck_dir> <out_dir> <max_iter> <tolT> <tolQ> [stage_band stage_dir]\n");
+        return 2;
+    }
+    char cwd0[4096];
+    if (!getcwd(cwd0, sizeof cwd0)) return 2;
+    std::string out = argv[2];
+    if (out[0] != '/') out = std::string(cwd0) + "/" + out;
+    g_out = out;
+    mkdir(g_out.c_str(), 0755);
+    if (chdir(argv[1]) != 0) { perror(argv[1]); return 2; }
+    int maxIter = atoi(argv[3]);
+    double tolT = atof(argv[4]), tolQ = atof(argv[5]);
+    int sb = -1, sd = -1;
+    if (argc >= 8) { sb = atoi(argv[6]); sd = atoi(argv[7]); }
+
+    Deck deck = parse_deck();
+    if (deck.order != "2" || deck.method != "Synthetic" || deck.matrix != "LU") {
+        fprintf(stderr, "ref_dump covers Order=2 Synthetic LU only\n");
+        return 3;
+    }
+    std::ifstream bandFile("input/PHONON");
+    BTEBand *bands = new BTEBand(bandFile, deck.dimMat);
+    bandFile.close();
+    std::ifstream bcFile(deck.bcFile);
+    BTEBoundaryCondition *bcs = new BTEBoundaryCondition(bcFile, bands);
+    bcFile.close();
+    std::ifstream heatfile(deck.heatFile), inittemp(""), geofile(deck.meshFile);
+    DistributeMesh *dm = new DistributeMesh(deck.dimGeo, geofile, deck.Lx, deck.Ly, deck.Lz, deck.meshType, bands, bcs,
+                                            heatfile, deck.uniformHeat, deck.heatType, inittemp, deck.tref);
+    BTEAngle *angles = new BTEAngle(deck.ntheta, deck.nphi, deck.dimMat, deck.angleMethod);
+
+    SolutionAll sa(dm, bcs, bands, angles, 1, 0);
+    sa._set_initial(dm, bands, angles);
+    size_t ncF = dm->volumeElementsFourier.size();
+    // 2D Fourier runs never write heatFluxZ (StaticFourier.cpp:615-710); pin
+    // it to zero so the hand-over below is defined.
+    for (size_t i = 0; i < ncF; ++i) sa.solution.heatFluxZ[i] = 0.0;
+    sa._Fourier_Solver(dm, bcs, bands, 1, 0);
+    if (dm->FourierMeshes.Dimension != 3)
+        for (size_t i = 0; i < ncF; ++i) sa.solution.heatFluxZ[i] = 0.0;
+    dump_d("fourier_T", sa.solution.Temperature, {ncF});
+    dump_d("fourier_qx", sa.solution.heatFluxX, {ncF});
+    dump_d("fourier_qy", sa.solution.heatFluxY, {ncF});
+    dump_d("fourier_qz", sa.solution.heatFluxZ, {ncF});
+
+    std::ifstream heatfile2(deck.heatFile);
+    dm->_build_BTEMesh(deck.dimGeo, deck.Lx, deck.Ly, deck.Lz, bands, bcs, heatfile2, deck.uniformHeat, deck.multiscale);
+
+    if (sb >= 0) {
+        StaticBTESynthetic st(&dm->BTEMeshes, bcs, bands, angles, 1, 0);
+        init_solver_state(st, *dm, sa);
+        st.error_old = 0;
+        stage_dump(st, sb, sd);
+    }
+
+    StaticBTESynthetic solver(&dm->BTEMeshes, bcs, bands, angles, 1, 0);
+    init_solver_state(solver, *dm, sa);
+    solver.error_old = 0;
+    solver.errorOld = 0;
+    solver.errorOld_heat = 0;
+    dump_tables(solver, "");
+    solver.solve(0, maxIter, 0, tolT, tolQ);
+    size_t nc = solver.numCell, nB = solver.numBand, nD = solver.numDirection, nb = solver.numBound;
+    dump_d("T_lattice", solver.temperature, {nc});
+    dump_d("T_energy", solver.totalEnergy, {nc});
+    dump_d("qx", solver.heatFluxXGlobal, {nc});
+    dump_d("qy", solver.heatFluxYGlobal, {nc});
+    dump_d("qz", solver.heatFluxZGlobal, {nc});
+    dump_d("T_band", solver.temperatureband, {nB, nc});
+    dump_d("ebound", solver.ebound, {nB, nD, nb, 2});
+    dump_d("T_old", solver.temperatureOld, {nc});
+    double errs[2] = {solver.error, solver.error_heat};
+    dump_d("final_errors", errs, {2});
+    MPI_Finalize();
+    return 0;
+}
