@@ -1,0 +1,153 @@
+"""giftbte_amd - MI355X solver for GiftBTE's steady StaticBTESynthetic path.
+
+Python mirror of the reference's driver flow for this path
+(src/BTEMain.cpp:327-393 -> SolutionAll::_Fourier_Solver / _BTE_Solver,
+src/Solution/Solution.cpp:236-414) over the C ABI of libgiftbte_hip.so
+(include/giftbte.h).  Everything numerical runs in the HIP library; this
+module only moves arguments and results.  There is no CPU fallback: if the
+library is missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["Problem", "BTESolver", "SolveStats", "run_deck", "GBTEError"]
+
+
+class GBTEError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[gbte {code}] {msg}")
+        self.code = code
+
+
+@dataclass
+class SolveStats:
+    iterations: int
+    macro_passes: int
+    fourier_passes: int
+    converged: bool
+    err_temp: float
+    err_flux: float
+    solve_ms: float
+    sweep_ms: float
+    transport_ms: float
+    macro_ms: float
+
+
+class Problem:
+    """Deck + mesh + tables (replaces BTEMain.cpp:87-362)."""
+
+    def __init__(self, deck_dir: str):
+        self.lib = _lib.load()
+        self._h = C.c_void_p()
+        rc = self.lib.gbte_problem_load(deck_dir.encode(), C.byref(self._h))
+        if rc != 0:
+            raise GBTEError(rc, self.lib.gbte_problem_error().decode())
+        self.tables = _lib.Tables()
+        self.lib.gbte_problem_tables(self._h, C.byref(self.tables))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def array(self, name: str) -> np.ndarray:
+        return self.tables.array(name)
+
+    def close(self):
+        if self._h:
+            self.lib.gbte_problem_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BTESolver:
+    """StaticFourier + StaticBTESynthetic on one GPU.
+
+    ``fourier()`` is StaticFourier::_solve (StaticFourier.cpp:934-976);
+    ``solve()`` is StaticBTESynthetic::solve (StaticBTESyntheticSolver.cpp:12-247);
+    ``fields()`` returns what StaticBTESynthetic::_print_out writes
+    (StaticBTESynthetic.cpp:2613-2653).
+    """
+
+    def __init__(self, problem: Problem, precision: str = "fp64", device: int = 0, verbose: bool = False,
+                 macro_rtol: float = 0.0):
+        self.problem = problem
+        self.lib = problem.lib
+        t = problem.tables
+        self.nc, self.nb = t.n_cells, t.n_bands
+        opts = _lib.Options(_lib.GBTE_FP64 if precision == "fp64" else _lib.GBTE_FP32_SWEEP, device, int(verbose),
+                            macro_rtol)
+        self._h = C.c_void_p()
+        rc = self.lib.gbte_create(problem.handle, C.byref(opts), C.byref(self._h))
+        if rc != 0:
+            msg = self.lib.gbte_last_error(self._h).decode() if self._h else "create failed"
+            self.close()
+            raise GBTEError(rc, msg)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise GBTEError(rc, self.lib.gbte_last_error(self._h).decode())
+
+    def fourier(self) -> int:
+        st = _lib.Stats()
+        self._check(self.lib.gbte_fourier_solve(self._h, C.byref(st)))
+        return st.fourier_passes
+
+    def set_initial(self, T=None, qx=None, qy=None, qz=None):
+        conv = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (T, qx, qy, qz)]
+        self._check(self.lib.gbte_set_initial(self._h, *[_lib.dptr(a) for a in conv]))
+
+    def solve(self, max_iter: int, tol_temp: float, tol_flux: float) -> SolveStats:
+        st = _lib.Stats()
+        rc = self.lib.gbte_solve(self._h, int(max_iter), float(tol_temp), float(tol_flux), C.byref(st))
+        self._check(rc)
+        return SolveStats(st.iterations, st.macro_passes, st.fourier_passes, bool(st.converged), st.err_temp,
+                          st.err_flux, st.solve_ms, st.sweep_ms, st.transport_ms, st.macro_ms)
+
+    def fields(self) -> dict:
+        nc, nb = self.nc, self.nb
+        out = {k: np.zeros(nc) for k in ("T_lattice", "T_energy", "qx", "qy", "qz")}
+        out["T_band"] = np.zeros(nb * nc)
+        self._check(self.lib.gbte_get_fields(self._h, *[_lib.dptr(out[k]) for k in
+                                                        ("T_lattice", "T_energy", "qx", "qy", "qz", "T_band")]))
+        out["T_band"] = out["T_band"].reshape(nb, nc)
+        return out
+
+    def keff(self, axis: int, delta_t: float) -> float:
+        v = C.c_double()
+        self._check(self.lib.gbte_keff(self._h, axis, delta_t, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.gbte_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_deck(deck_dir: str, max_iter: int | None = None, tol_temp: float | None = None,
+             tol_flux: float | None = None, precision: str = "fp64", verbose: bool = False):
+    """The reference's steady flow on one deck: Fourier warm start, then the
+    synthetic BTE solve (BTEMain.cpp:372-393).  Returns (problem, solver, stats)."""
+    p = Problem(deck_dir)
+    s = BTESolver(p, precision=precision, verbose=verbose)
+    s.fourier()
+    t = p.tables
+    st = s.solve(t.max_iter if max_iter is None else max_iter, t.tol_temp if tol_temp is None else tol_temp,
+                 t.tol_flux if tol_flux is None else tol_flux)
+    return p, s, st

@@ -77,7 +77,8 @@ class Options(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("macro_passes", C.c_int), ("fourier_passes", C.c_int),
                 ("converged", C.c_int), ("err_temp", C.c_double), ("err_flux", C.c_double),
-                ("solve_ms", C.c_double), ("sweep_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("sweep_ms", C.c_double),
+                ("transport_ms", C.c_double), ("macro_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -4,10 +4,6 @@
 
 #include "problem.hpp"
 
-struct gbte_problem {
-    gbte::Problem p;
-};
-
 namespace {
 thread_local std::string g_problem_error;
 }

@@ -1,0 +1,81 @@
+// C ABI: solver entry points (declared in include/giftbte.h).
+#include <cstdio>
+#include <memory>
+#include <string>
+
+#include "solver.hpp"
+
+struct gbte_handle {
+    std::unique_ptr<gbte::Solver> s;
+    std::string err;
+};
+
+namespace {
+
+int finish(gbte_handle *h, int rc) {
+    if (rc != GBTE_OK && h->s) h->err = h->s->error();
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gbte_create(const gbte_problem *p, const gbte_options *opts, gbte_handle **out) {
+    if (!p || !out) return GBTE_BAD_INPUT;
+    *out = nullptr;
+    gbte_options o{};
+    if (opts) o = *opts;
+    if (o.precision != GBTE_FP64) {
+        auto *h = new gbte_handle();
+        h->err = "only the FP64 sweep is available in this build";
+        *out = h;
+        return GBTE_UNSUPPORTED;
+    }
+    auto *h = new gbte_handle();
+    h->s.reset(new gbte::Solver(p->p, o));
+    *out = h;
+    if (!h->s->ok()) {
+        h->err = h->s->error();
+        return h->err.find("no HIP device") != std::string::npos || h->err.find("HIP error") != std::string::npos
+                   ? GBTE_HIP_ERROR
+                   : GBTE_UNSUPPORTED;
+    }
+    return GBTE_OK;
+}
+
+int gbte_fourier_solve(gbte_handle *h, gbte_stats *stats) {
+    if (!h || !h->s || !h->s->ok()) return GBTE_HIP_ERROR;
+    return finish(h, h->s->fourier(stats));
+}
+
+int gbte_set_initial(gbte_handle *h, const double *T, const double *qx, const double *qy, const double *qz) {
+    if (!h || !h->s || !h->s->ok()) return GBTE_HIP_ERROR;
+    return finish(h, h->s->set_initial(T, qx, qy, qz));
+}
+
+int gbte_solve(gbte_handle *h, int max_iter, double tol_temp, double tol_flux, gbte_stats *stats) {
+    if (!h || !h->s || !h->s->ok()) return GBTE_HIP_ERROR;
+    if (max_iter < 0) {
+        h->err = "max_iter must be non-negative";
+        return GBTE_BAD_INPUT;
+    }
+    return finish(h, h->s->solve(max_iter, tol_temp, tol_flux, stats));
+}
+
+int gbte_get_fields(gbte_handle *h, double *T_lattice, double *T_energy, double *qx, double *qy, double *qz,
+                    double *T_band) {
+    if (!h || !h->s || !h->s->ok()) return GBTE_HIP_ERROR;
+    return finish(h, h->s->get_fields(T_lattice, T_energy, qx, qy, qz, T_band));
+}
+
+int gbte_keff(gbte_handle *h, int axis, double delta_T, double *keff) {
+    if (!h || !h->s || !keff) return GBTE_BAD_INPUT;
+    return finish(h, h->s->keff(axis, delta_T, keff));
+}
+
+const char *gbte_last_error(const gbte_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+void gbte_destroy(gbte_handle *h) { delete h; }
+
+}  // extern "C"

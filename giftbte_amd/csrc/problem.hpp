@@ -63,3 +63,8 @@ std::string load_problem(const std::string &deck_dir, Problem &p);
 std::vector<double> gauss_legendre(double a, double b, int n);
 
 }  // namespace gbte
+
+// Opaque C handle of include/giftbte.h.
+struct gbte_problem {
+    gbte::Problem p;
+};

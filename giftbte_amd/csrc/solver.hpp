@@ -1,0 +1,115 @@
+// Device-resident solver for the steady StaticBTESynthetic path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "giftbte.h"
+#include "problem.hpp"
+
+namespace gbte {
+
+// Owning device buffer.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Compressed sparse rows of the synthetic (macro) and Fourier matrices,
+// stored negated so that they are symmetric positive definite.
+struct DevCSR {
+    int n = 0;
+    DBuf<int> ptr, col;
+    DBuf<double> val, dinv;
+};
+
+class Solver {
+   public:
+    Solver(const Problem &p, const gbte_options &o);
+    ~Solver();
+    bool ok() const { return err_.empty(); }
+    const std::string &error() const { return err_; }
+
+    int fourier(gbte_stats *st);
+    int set_initial(const double *T, const double *qx, const double *qy, const double *qz);
+    int solve(int max_iter, double tolT, double tolQ, gbte_stats *st);
+    int get_fields(double *TL, double *TE, double *qx, double *qy, double *qz, double *Tb);
+    int keff(int axis, double dT, double *out);
+
+    void set_error(const std::string &e) { err_ = e; }
+
+   private:
+    template <typename T>
+    bool upload(DBuf<T> &b, const std::vector<T> &h);
+    template <typename T>
+    bool alloc(DBuf<T> &b, size_t n);
+    bool check(hipError_t e, const char *what);
+    bool build();
+    bool build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m);
+    // macro / Fourier linear solve of (-K) x = -rhs, warm-started from x
+    bool cg(const DevCSR &K, double *d_rhs, double *d_x, int *iters);
+    bool reduce_host(const double *d_partials, int n, std::vector<double> &out, int stride);
+    void launch_vertex_T(const double *T, double *TV);
+    void launch_rhs(bool macro, const double *bval, double *rhs);
+    bool inner_error(const double *a, const double *b, double *out);
+
+    const Problem &P;
+    gbte_options opt{};
+    std::string err_;
+    hipStream_t s_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev2_ = nullptr;
+
+    int nc = 0, nb = 0, ne = 0, nn = 0, nB = 0, nD = 0, nf = 0, dim = 2, cpw = 1, nlev = 0;
+
+    // mode tables
+    DBuf<double> d_vg, d_tau, d_cap, d_lr, d_hr, d_sx, d_sy, d_sz, d_w;
+    // cell / slot geometry
+    DBuf<int> d_fsize, d_snb, d_sent;
+    DBuf<double> d_vol, d_Q, d_snx, d_sny, d_snz, d_sarea, d_srx, d_sry, d_srz, d_sqx, d_sqy, d_sqz;
+    // level schedule
+    DBuf<int> d_item_c, d_item_d;
+    std::vector<int> lev_ptr;
+    // gradient stencil
+    DBuf<int> d_gptr, d_gidx, d_gok;
+    DBuf<double> d_gP;
+    // IDW stencils (energy density: cells or boundary entries; temperature: cells)
+    DBuf<int> d_vptr, d_vsrc, d_tptr, d_tsrc;
+    DBuf<double> d_vdist, d_tdist;
+    // face-gradient / non-Fourier slot data
+    DBuf<int> d_fkind, d_fother, d_fv0, d_fv1;
+    DBuf<double> d_fan, d_faa, d_fn0x, d_fn0y, d_fn0z, d_fmx, d_fmy, d_fm2;
+    // boundary entries
+    std::vector<int> h_ent_list;
+    DBuf<int> d_ent_list, d_be_cell, d_be_type, d_be_conn, d_be_spec;
+    DBuf<double> d_be_nx, d_be_ny, d_be_nz, d_be_tw, d_be_dT, d_be_tov;
+    // macro / Fourier right-hand side slot data (two sets)
+    DBuf<int> d_mbent[2], d_mnonorth[2], d_mv0[2], d_mv1[2];
+    DBuf<double> d_mbcoef[2], d_mkA[2], d_ma2x[2], d_ma2y[2], d_mmx[2], d_mmy[2], d_mm2[2];
+    DBuf<double> d_fourier_bval;
+    DevCSR Kmacro, Kfour;
+    // state
+    DBuf<double> d_e, d_grad, d_eb, d_eV, d_T, d_Told, d_T1, d_TV, d_E, d_qx, d_qy, d_qz, d_qxo, d_qyo, d_qzo,
+        d_RM, d_Tb, d_tmac, d_rhs;
+    // CG / reduction scratch
+    DBuf<double> d_r, d_z, d_p, d_Ap, d_part, d_part2;
+    double *h_pin = nullptr;  // pinned host scratch
+    int red_blocks = 0;
+    bool have_state = false;
+    double errInner_ = 0, errTold_ = 0, errQold_ = 0;
+    int errInc_ = 0;
+    int fourier_passes_ = 0;
+};
+
+}  // namespace gbte

@@ -100,7 +100,9 @@ typedef struct gbte_stats {
     int converged;        /* 1 if tolT and tolQ were met                        */
     double err_temp, err_flux;   /* last errors (StaticBTESynthetic.cpp:2565-2611) */
     double solve_ms;      /* wall time of the outer loop                        */
-    double sweep_ms;      /* device time of the transport sweeps (HIP events)   */
+    double sweep_ms;      /* device time of the upwind sweeps (HIP events)      */
+    double transport_ms;  /* device time of sweep + vertex + face/moment kernels */
+    double macro_ms;      /* wall time of the synthetic (macro) solves          */
 } gbte_stats;
 
 /* ---- problem construction (replaces BTEMain.cpp:87-362: decks, BTEBand,

@@ -25,6 +25,7 @@
 // iteration 0; closed forms (LSQ gradients of linear fields, quadrature
 // identities, equilibrium T_hot = T_cold, linearity in dT, mirror symmetry).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -982,6 +983,54 @@ int oracle_stages(const gbte_tables *t, int band, int dir, double *grad, double 
     o.non_fourier(band, dir);
     std::copy(o.ReMacro.begin(), o.ReMacro.end(), remacro);
     return 0;
+}
+
+// CPU-baseline sample: wall time of the per-system transport work of one
+// outer iteration (gradient, _get_Re + exact sweep, _get_bound_ee,
+// _set_vertex_energydensity, _get_face_gradient, _get_non_fourier, moments;
+// StaticBTESyntheticSolver.cpp:79-127) over the first n_sys systems in the
+// reference's (direction, band) order, starting from e = C T.  Set-up and the
+// synthetic solve are excluded; boundary values start at equilibrium with
+// the cells, which changes no operation count.  Returns seconds, or -1.
+double oracle_transport_sample(const gbte_tables *t, const double *T0, int n_sys) {
+    Oracle o(*t);
+    int nc = t->n_cells, nb = t->n_bounds, nB = t->n_bands, nD = t->n_dirs, nf = t->n_faces;
+    o.e.assign((size_t)nB * nD * nc, 0.0);
+    o.ebound.assign((size_t)nB * nD * nb * 2, 0.0);
+    o.eboundLocal.assign((size_t)nB * nb * 2, 0.0);
+    o.T.assign(T0, T0 + nc); o.Told = o.T;
+    o.E.assign(nc, 0.0); o.qx.assign(nc, 0.0); o.qy.assign(nc, 0.0); o.qz.assign(nc, 0.0);
+    o.Tband.assign((size_t)nB * nc, 0.0); o.ReMacro.assign(nc, 0.0);
+    o.gradX.assign(nc, 0); o.gradY.assign(nc, 0); o.gradZ.assign(nc, 0);
+    o.eV.assign((size_t)t->n_nodes * t->n_matters, 0.0);
+    o.gfx.assign((size_t)nf * 2, 0.0); o.gfy.assign((size_t)nf * 2, 0.0); o.gfz.assign((size_t)nf * 2, 0.0);
+    for (int b = 0; b < nB; ++b)
+        for (int d = 0; d < nD; ++d)
+            for (int i = 0; i < nc; ++i) o.E_(b, d)[i] = o.T[i] * o.cap(t->matter[i], b);
+    o.cell_matrix();
+    // boundary values at equilibrium with the cells (timing only)
+    for (int b = 0; b < nB; ++b)
+        for (int d = 0; d < nD; ++d)
+            for (int ib = 0; ib < nb; ++ib)
+                for (int ic = 0; ic < 2; ++ic) {
+                    int c = t->bound_cell[ib * 2 + ic];
+                    if (c >= 0) o.EB(b, d, ib, ic) = o.T[c] * o.cap(t->matter[c], b);
+                }
+    std::vector<int> order, indeg;
+    std::vector<double> Re(nc), diag(nc);
+    auto t0 = std::chrono::steady_clock::now();
+    int k = 0;
+    for (int d = 0; d < nD && k < n_sys; d++)
+        for (int b = 0; b < nB && k < n_sys; ++b, ++k) {
+            o.gradient(b, d);
+            if (!o.sweep(b, d, order, indeg, Re, diag)) return -1;
+            o.bound_ee(b, d);
+            o.vertex_e(b, d);
+            if (!o.face_gradient(b, d)) return -1;
+            o.non_fourier(b, d);
+            o.moments(b, d);
+        }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 }  // extern "C"

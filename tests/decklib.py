@@ -1,0 +1,40 @@
+"""Named test decks (seeded) shared by the CPU and GPU tests."""
+from __future__ import annotations
+
+import os
+
+from giftbte_amd import decks
+
+# name -> (spec factory, run settings)
+DECKS = {
+    # 2D gray film, 40x40 split squares, 32 in-plane directions, specular
+    # top/bottom, left 301 K / right 299 K, converged to the deck tolerance.
+    "gray40": (lambda: decks.gray_square_spec(40), dict(max_iter=500, tol_temp=1e-8, tol_flux=1e-4)),
+    # small porous film: periodic top/bottom, diffuse pore, 4 bands x 16 dirs
+    "porous_b4": (lambda: decks.porous_spec(n_target=2000, nbands=4, nphi=4, seed=3),
+                  dict(max_iter=25, tol_temp=0.0, tol_flux=0.0)),
+    # 40 bands x 64 directions on a coarse porous mesh (one cell per wavefront)
+    "porous_b40": (lambda: decks.porous_spec(n_target=400, nbands=40, nphi=16, seed=5),
+                   dict(max_iter=4, tol_temp=0.0, tol_flux=0.0)),
+    # gray film with a uniform heat source and cold walls on all sides
+    "gray_heat": (lambda: decks.gray_square_spec(
+        12, uniform_heat=1e17, bcs=[(1, 1, 300.0), (2, 1, 300.0), (3, 1, 300.0), (4, 1, 300.0)]),
+                  dict(max_iter=6, tol_temp=0.0, tol_flux=0.0)),
+    # 20 bands (three cells per wavefront), diffuse walls on top/bottom
+    "square_b20_diffuse": (lambda: decks.DeckSpec(
+        mesh=decks.square_split_mesh(10), bands=decks.debye_bands(20, 7),
+        bcs=[(1, 2, 0.0), (2, 1, 299.0), (3, 2, 0.0), (4, 1, 301.0)], scale=3e-7, nphi=4),
+                           dict(max_iter=5, tol_temp=0.0, tol_flux=0.0)),
+}
+
+
+def make(name: str, root) -> str:
+    d = os.path.join(str(root), name)
+    if not os.path.exists(os.path.join(d, "input", "CONTROL")):
+        spec, _ = DECKS[name]
+        decks.write_deck(d, spec())
+    return d
+
+
+def settings(name: str) -> dict:
+    return dict(DECKS[name][1])
