@@ -114,7 +114,9 @@ def main():
                        "updates_per_step": U, "parallelism": "single GPU"},
             "breakdown_ms_per_step": {"transport": st.transport_ms / a.steps, "sweep": st.sweep_ms / a.steps,
                                       "macro": st.macro_ms / a.steps,
-                                      "macro_passes_per_step": st.macro_passes / a.steps},
+                                      "macro_passes_per_step": st.macro_passes / a.steps,
+                                      "cg_iters_per_step": st.macro_cg_iters / a.steps,
+                                      "sweep_levels": st.sweep_levels},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                          "kernel": "transport step (k_sweep levels + k_vertex_e + k_face_moments)",

@@ -37,6 +37,8 @@ class SolveStats:
     sweep_ms: float
     transport_ms: float
     macro_ms: float
+    sweep_levels: int
+    macro_cg_iters: int
 
 
 class Problem:
@@ -112,7 +114,8 @@ class BTESolver:
         rc = self.lib.gbte_solve(self._h, int(max_iter), float(tol_temp), float(tol_flux), C.byref(st))
         self._check(rc)
         return SolveStats(st.iterations, st.macro_passes, st.fourier_passes, bool(st.converged), st.err_temp,
-                          st.err_flux, st.solve_ms, st.sweep_ms, st.transport_ms, st.macro_ms)
+                          st.err_flux, st.solve_ms, st.sweep_ms, st.transport_ms, st.macro_ms,
+                          st.sweep_levels, st.macro_cg_iters)
 
     def fields(self) -> dict:
         nc, nb = self.nc, self.nb

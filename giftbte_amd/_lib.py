@@ -78,7 +78,8 @@ class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("macro_passes", C.c_int), ("fourier_passes", C.c_int),
                 ("converged", C.c_int), ("err_temp", C.c_double), ("err_flux", C.c_double),
                 ("solve_ms", C.c_double), ("sweep_ms", C.c_double),
-                ("transport_ms", C.c_double), ("macro_ms", C.c_double)]
+                ("transport_ms", C.c_double), ("macro_ms", C.c_double), ("sweep_levels", C.c_int),
+                ("macro_cg_iters", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

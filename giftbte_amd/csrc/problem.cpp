@@ -650,8 +650,8 @@ std::string load_problem(const std::string &dir, Problem &P) {
     Deck &d = P.deck;
     std::string err = parse_deck(dir, d);
     if (!err.empty()) return err;
-    if (d.state != "Steady") return "only the steady solver is supported (State=" + d.state + ")";
-    if (d.dim != 2) return "GeometryDimension " + std::to_string(d.dim) + " is not supported by this deck reader (2D only)";
+    if (d.state != "Steady") return "State=" + d.state + " is not supported: this build is the steady solver";
+    if (d.dim != 2) return "GeometryDimension " + std::to_string(d.dim) + " decks are not supported by this deck reader (2D only)";
     if (!d.multiscale.empty()) return "Multiscale Fourier/BTE region files are not supported";
     Bands B;
     err = parse_bands(dir, d.material_dim, B);

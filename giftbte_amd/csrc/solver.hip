@@ -1225,6 +1225,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     bool conv = false;
     double errT = 0, errQ = 0;
     double sweep_ms = 0, transport_ms = 0, macro_ms = 0;
+    long cg_total = 0;
     int fmw = blocks_for(blocks_for(nc, cpw), kWaves);
     int vw = blocks_for(blocks_for((long)nn * nD, cpw), kWaves);
     for (nt = 0; nt < max_iter; ++nt) {
@@ -1256,7 +1257,9 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
             launch_vertex_T(d_T.p, d_TV.p);
             launch_rhs(true, d_tmac.p, d_rhs.p);
-            if (!cg(Kmacro, d_rhs.p, d_T.p, nullptr)) return GBTE_HIP_ERROR;
+            int cgit = 0;
+            if (!cg(Kmacro, d_rhs.p, d_T.p, &cgit)) return GBTE_HIP_ERROR;
+            cg_total += cgit;
             if (!inner_error(d_T1.p, d_T.p, &inner)) return GBTE_HIP_ERROR;
             if (inner > errInner_) inc++;
             errInner_ = inner;
@@ -1312,6 +1315,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         st->sweep_ms = sweep_ms;
         st->transport_ms = transport_ms;
         st->macro_ms = macro_ms;
+        st->sweep_levels = nlev;
+        st->macro_cg_iters = (int)cg_total;
     }
     return status;
 }

@@ -90,6 +90,26 @@ def square_split_mesh(n: int) -> Mesh2D:
     return Mesh2D(nodes, np.asarray(tris, np.int64), np.asarray(edges, np.int64), np.asarray(ent, np.int64))
 
 
+def square_quad_mesh(n: int) -> Mesh2D:
+    """n x n quadrilaterals on the unit square, COMSOL vertex order
+    (v0, v1, v2, v3) = (i,j), (i+1,j), (i,j+1), (i+1,j+1)."""
+    xs = np.linspace(0.0, 1.0, n + 1)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+
+    def nid(i, j):
+        return i * (n + 1) + j
+
+    quads = [(nid(i, j), nid(i + 1, j), nid(i, j + 1), nid(i + 1, j + 1)) for i in range(n) for j in range(n)]
+    edges, ent = [], []
+    for k in range(n):
+        edges.append((nid(k, 0), nid(k + 1, 0))); ent.append(BOTTOM)
+        edges.append((nid(n, k), nid(n, k + 1))); ent.append(RIGHT)
+        edges.append((nid(k + 1, n), nid(k, n))); ent.append(TOP)
+        edges.append((nid(0, k + 1), nid(0, k))); ent.append(LEFT)
+    return Mesh2D(nodes, np.asarray(quads, np.int64), np.asarray(edges, np.int64), np.asarray(ent, np.int64))
+
+
 def porous_delaunay_mesh(n_target: int, pore_radius: float, seed: int) -> Mesh2D:
     """Seeded Delaunay mesh of the unit square minus a centred circular pore.
 
@@ -215,9 +235,11 @@ def write_mphtxt(path: str, m: Mesh2D) -> None:
     out += [f"{a} {b} " for a, b in m.edges.tolist()]
     out += ["", f"{len(m.edges)} # number of geometric entity indices", "# Geometric entity indices"]
     out += [f"{e} " for e in m.edge_entity.tolist()]
-    out += ["", "# Type #1", "", "3 tri # type name", "", "",
-            "3 # number of vertices per element", f"{len(m.tris)} # number of elements", "# Elements"]
-    out += [f"{a} {b} {c} " for a, b, c in m.tris.tolist()]
+    nv = m.tris.shape[1]
+    tname = "tri" if nv == 3 else "quad"
+    out += ["", "# Type #1", "", f"{len(tname)} {tname} # type name", "", "",
+            f"{nv} # number of vertices per element", f"{len(m.tris)} # number of elements", "# Elements"]
+    out += [" ".join(str(v) for v in row) + " " for row in m.tris.tolist()]
     out += ["", f"{len(m.tris)} # number of geometric entity indices", "# Geometric entity indices"]
     out += ["1 "] * len(m.tris)
     out.append("")
@@ -225,11 +247,44 @@ def write_mphtxt(path: str, m: Mesh2D) -> None:
         f.write("\n".join(out))
 
 
-def write_deck(root: str, spec: DeckSpec) -> str:
+def write_msh(path: str, m: Mesh2D) -> None:
+    """Gmsh 4.1 text mesh, 2D, in the subset src/BTEMesh/ReadinMesh.cpp:387-532
+    reads: one node block, one line block per boundary entity (entity tag =
+    BC index), one triangle block (entity tag 1 -> region 0)."""
+    n = len(m.nodes)
+    lines = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$Nodes", f"1 {n} 1 {n}", f"2 1 0 {n}"]
+    lines += [str(i + 1) for i in range(n)]
+    lines += [f"{_f(x)} {_f(y)} 0" for x, y in m.nodes.tolist()]
+    lines.append("$EndNodes")
+    ents = sorted(set(m.edge_entity.tolist()))
+    nel = len(m.edges) + len(m.tris)
+    lines += ["$Elements", f"{len(ents) + 1} {nel} 1 {nel}"]
+    tag = 1
+    for e in ents:
+        sel = m.edges[m.edge_entity == e]
+        lines.append(f"1 {e + 1} 1 {len(sel)}")
+        for a, b in sel.tolist():
+            lines.append(f"{tag} {a + 1} {b + 1}")
+            tag += 1
+    if m.tris.shape[1] != 3:
+        raise ValueError("write_msh writes triangle meshes")
+    lines.append(f"2 1 2 {len(m.tris)}")
+    for a, b, c in m.tris.tolist():
+        lines.append(f"{tag} {a + 1} {b + 1} {c + 1}")
+        tag += 1
+    lines.append("$EndElements")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_deck(root: str, spec: DeckSpec, mesh_format: str = "COMSOL") -> str:
     """Write a complete deck under ``root``; returns ``root``."""
     inp = os.path.join(root, "input")
     os.makedirs(inp, exist_ok=True)
-    write_mphtxt(os.path.join(inp, "mesh.mphtxt"), spec.mesh)
+    if mesh_format == "MSH":
+        write_msh(os.path.join(inp, "mesh.msh"), spec.mesh)
+    else:
+        write_mphtxt(os.path.join(inp, "mesh.mphtxt"), spec.mesh)
     bd = spec.bands
     with open(os.path.join(inp, "bands.dat"), "w") as f:
         for k in range(len(bd.v)):
@@ -246,7 +301,8 @@ def write_deck(root: str, spec: DeckSpec) -> str:
                 f"Tref=\n{_f(spec.tref)}\n")
     with open(os.path.join(inp, "GEOMETRY"), "w") as f:
         f.write(f"GeometryDimension=\n2\nScaleX=\n{_f(spec.scale)}\nScaleY=\n{_f(spec.scale)}\n"
-                f"BCFile=\ninput/bc.dat\nMeshType=\nCOMSOL\nMeshFile=\ninput/mesh.mphtxt\n")
+                f"BCFile=\ninput/bc.dat\nMeshType=\n{mesh_format}\nMeshFile=\n"
+                f"input/mesh.{'msh' if mesh_format == 'MSH' else 'mphtxt'}\n")
         if spec.uniform_heat:
             f.write(f"UniformHeat=\n{_f(spec.uniform_heat)}\n")
     with open(os.path.join(inp, "PHONON"), "w") as f:
@@ -279,5 +335,22 @@ def porous_spec(n_target=50000, nbands=40, nphi=16, pore_radius=0.3, seed=1, sca
            (LEFT + 1, 1, 301.0), (PORE + 1, 2, 0.0)]
     base = dict(mesh=porous_delaunay_mesh(n_target, pore_radius, seed), bands=debye_bands(nbands, seed),
                 bcs=bcs, scale=scale, material_dim=2, angle_method=1, nphi=nphi, tol_temp=1e-8)
+    base.update(kw)
+    return DeckSpec(**base)
+
+
+def read_band_table(path: str) -> Bands:
+    """DATABASE band rows: v tau C lattice_ratio heat_ratio."""
+    a = np.loadtxt(path, ndmin=2)
+    return Bands(a[:, 0].copy(), a[:, 1].copy(), a[:, 2].copy(), a[:, 3].copy(), a[:, 4].copy())
+
+
+def crossplane_spec(band_file: str, length: float = 1e-7, n: int = 100, **kw) -> DeckSpec:
+    """Cross-plane film of thickness ``length`` (the reference's
+    bin/examples/cross-plane decks): n x n quads, bottom 301 K and top 300 K
+    isothermal, left/right specular, 3D Gauss directions (Ntheta 4, Nphi 4)."""
+    bcs = [(BOTTOM + 1, 1, 301.0), (RIGHT + 1, 3, 0.0), (TOP + 1, 1, 300.0), (LEFT + 1, 3, 0.0)]
+    base = dict(mesh=square_quad_mesh(n), bands=read_band_table(band_file), bcs=bcs, scale=length,
+                material_dim=3, angle_method=2, ntheta=4, nphi=4, tol_temp=1e-8, tol_flux=1e-2, max_iter=100000)
     base.update(kw)
     return DeckSpec(**base)

@@ -103,6 +103,8 @@ typedef struct gbte_stats {
     double sweep_ms;      /* device time of the upwind sweeps (HIP events)      */
     double transport_ms;  /* device time of sweep + vertex + face/moment kernels */
     double macro_ms;      /* wall time of the synthetic (macro) solves          */
+    int sweep_levels;     /* upwind levels (sweep launches) per outer iteration */
+    int macro_cg_iters;   /* PCG iterations, summed over synthetic solves       */
 } gbte_stats;
 
 /* ---- problem construction (replaces BTEMain.cpp:87-362: decks, BTEBand,

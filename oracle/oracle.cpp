@@ -1033,4 +1033,18 @@ double oracle_transport_sample(const gbte_tables *t, const double *T0, int n_sys
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// LSQ cell gradient of an arbitrary cell field with the stencil and matrices
+// of _get_CellMatrix_larger / _get_gradient_larger (StaticBTESynthetic.cpp:
+// 1000-1064, 1243-1340): g = [gx(nc), gy(nc), gz(nc)].
+int oracle_lsq_gradient(const gbte_tables *t, const double *field, double *g) {
+    Oracle o(*t);
+    int nc = t->n_cells;
+    o.cell_matrix();
+    o.e.assign(field, field + nc);  // system (0, 0) storage
+    o.gradX.assign(nc, 0); o.gradY.assign(nc, 0); o.gradZ.assign(nc, 0);
+    o.gradient(0, 0);
+    for (int i = 0; i < nc; ++i) { g[i] = o.gradX[i]; g[nc + i] = o.gradY[i]; g[2 * nc + i] = o.gradZ[i]; }
+    return 0;
+}
+
 }  // extern "C"

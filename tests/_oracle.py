@@ -23,6 +23,7 @@ def lib():
         L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _pd, _pd, _pd, _pd, _pd, _pd,
                                  C.POINTER(C.c_int), _pd]
         L.oracle_stages.argtypes = [C.c_void_p, C.c_int, C.c_int, _pd, _pd, _pd, _pd, _pd, _pd, _pd]
+        L.oracle_lsq_gradient.argtypes = [C.c_void_p, _pd, _pd]
         _lib = L
     return _lib
 
@@ -69,6 +70,14 @@ def stages(tables, band, direction):
     if rc != 0:
         raise RuntimeError(lib().oracle_last_error().decode())
     return out
+
+
+def lsq_gradient(tables, field):
+    nc = tables.n_cells
+    f = np.ascontiguousarray(field, np.float64)
+    g = np.zeros(3 * nc)
+    lib().oracle_lsq_gradient(C.byref(tables), _p(f), _p(g))
+    return g.reshape(3, nc)
 
 
 def rel_l2(a, b):

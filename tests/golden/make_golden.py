@@ -1,0 +1,88 @@
+"""Write the golden fixtures from the compiled reference (oracle/_ref/ref_dump).
+
+Run from the repo root after `make -f oracle/ref.mk` (needs /root/reference):
+
+    python tests/golden/make_golden.py
+
+For every deck below, the seeded deck is written by giftbte_amd.decks, the
+reference's steady path is run at np=1 by ref_dump (which links the
+reference's own objects), and its full-precision outputs are stored as a
+compressed .npz: the flattened solver tables, the Fourier warm start, the
+fields after the run, the iteration / synthetic-pass / Fourier-pass counts
+parsed from the reference's log, and every per-system stage of iteration 0
+for one system.  These are data (inputs and expected outputs), not code.
+"""
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+
+TABLES = ["elementFaceSize", "elementFaceNeighbor", "elementFaceBound", "elementFaceIndex", "elementFaceArea",
+          "elementFaceNormX", "elementFaceNormY", "elementFaceNormZ", "elementFaceCenterX", "elementFaceCenterY",
+          "elementFaceCenterZ", "elementVertexesSize", "elementVolume", "elementCenterX", "elementCenterY",
+          "elementCenterZ", "elementHeatSource", "matter", "elementVertexes", "elementFaceVertexesSize",
+          "elementFaceVertexes", "neighborPtr", "neighborIdx", "nodeX", "nodeY", "nodeZ", "boundaryCell",
+          "boundaryFace", "boundaryType", "boundaryThermal", "boundaryConnect", "faceCell", "faceFace", "faceBound",
+          "kappaBulk", "capacityBulk", "directionX", "directionY", "directionZ", "modeWeight", "relaxationTime",
+          "heatCapacity", "latticeRatio", "heatRatio", "sizes", "scale", "cellMatrix"]
+FIELDS = ["fourier_T", "fourier_qx", "fourier_qy", "T_lattice", "T_energy", "qx", "qy", "T_band", "final_errors"]
+STAGES = ["stage_system", "stage_ebound_init", "stage_gradX", "stage_gradY", "stage_e_new", "stage_ebound_sys",
+          "stage_e_vertex", "stage_gradFaceX", "stage_gradFaceY", "stage_ReMacro", "stage_T_sys", "stage_E_sys",
+          "stage_qx_sys", "stage_qy_sys"]
+
+
+def golden_decks():
+    import decklib
+    from giftbte_amd import decks
+    return {
+        # name: (deck factory, mesh format, max_iter, tolT, tolQ, stage system)
+        "gray40": (decklib.DECKS["gray40"][0], "COMSOL", 500, 1e-8, 1e-4, (0, 3)),
+        "porous_b4": (decklib.DECKS["porous_b4"][0], "COMSOL", 25, 0.0, 0.0, (1, 2)),
+        "msh_gray6": (lambda: decks.gray_square_spec(6), "MSH", 5, 0.0, 0.0, (0, 1)),
+        "quad_cross8": (lambda: decks.crossplane_spec(os.path.join(HERE, "Si_15.dat"), n=8), "COMSOL", 3, 0.0, 0.0,
+                        (4, 17)),
+    }
+
+
+def run_one(name, factory, fmt, max_iter, tolT, tolQ, stage):
+    from giftbte_amd import decks
+    td = tempfile.mkdtemp()
+    try:
+        deck = decks.write_deck(os.path.join(td, name), factory(), mesh_format=fmt)
+        out = os.path.join(td, "out")
+        log = subprocess.run([REF_DUMP, deck, out, str(max_iter), repr(tolT), repr(tolQ), str(stage[0]),
+                              str(stage[1])], cwd=deck, capture_output=True, text=True, check=True).stdout
+        data = {}
+        for k in TABLES + FIELDS + STAGES:
+            data[k] = np.load(os.path.join(out, k + ".npy"))
+        data["count_iterations"] = np.int64(len(re.findall(r"Iteration #", log)))
+        data["count_macro_passes"] = np.int64(len(re.findall(r"internal residual", log)))
+        data["count_fourier_passes"] = np.int64(len(re.findall(r"iterations residual", log)))
+        data["run"] = np.array([max_iter, tolT, tolQ])
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **data)
+        print(name, "iterations", data["count_iterations"], "macro", data["count_macro_passes"], "fourier",
+              data["count_fourier_passes"])
+    finally:
+        shutil.rmtree(td, ignore_errors=True)
+
+
+def main():
+    if not os.path.exists(REF_DUMP):
+        sys.exit("oracle/_ref/ref_dump is missing: make -f oracle/ref.mk")
+    for name, args in golden_decks().items():
+        run_one(name, *args)
+
+
+if __name__ == "__main__":
+    main()

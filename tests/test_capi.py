@@ -1,0 +1,83 @@
+"""The C ABI: every entry point of include/giftbte.h is exported, problem
+construction reports errors as status codes, and solver calls fail loudly
+(never fall back to the CPU) when no GPU is present."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from giftbte_amd import _lib, decks, Problem, BTESolver, GBTEError
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "giftbte.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gbte_[a-z_]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = _lib.load()
+    names = declared()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_lib.EXPORTED) == names
+
+
+def test_library_does_not_link_the_oracle():
+    with open(_lib.LIB_PATH, "rb") as f:
+        blob = f.read()
+    assert b"oracle_run" not in blob and b"liboracle" not in blob
+
+
+def test_missing_deck_is_bad_input(tmp_path):
+    with pytest.raises(GBTEError) as e:
+        Problem(str(tmp_path / "nowhere"))
+    assert e.value.code == _lib.GBTE_BAD_INPUT
+    assert "CONTROL" in str(e.value)
+
+
+def test_unsupported_features_are_reported(tmp_path):
+    d = decks.write_deck(str(tmp_path / "t"), decks.gray_square_spec(3))
+    with open(os.path.join(d, "input", "CONTROL"), "a") as f:
+        f.write("State=\nTransient\n")
+    with pytest.raises(GBTEError) as e:
+        Problem(d)
+    assert e.value.code == _lib.GBTE_UNSUPPORTED
+
+
+def test_unclosed_boundary_is_bad_input(tmp_path):
+    spec = decks.gray_square_spec(3, bcs=[(1, 3, 0.0), (2, 1, 299.0), (4, 1, 301.0)])
+    d = decks.write_deck(str(tmp_path / "open"), spec)
+    with pytest.raises(GBTEError) as e:
+        Problem(d)
+    assert e.value.code == _lib.GBTE_BAD_INPUT
+    assert "neither boundary nor inner face" in str(e.value)
+
+
+def test_tables_view(tmp_path):
+    p = Problem(decks.write_deck(str(tmp_path / "v"), decks.gray_square_spec(5, nphi=2)))
+    t = p.tables
+    assert (t.n_cells, t.n_dirs, t.n_bands, t.dim) == (50, 8, 1, 2)
+    assert t.max_iter == 10000 and t.tol_temp == 1e-8
+
+
+def test_solver_without_gpu_fails_loudly(tmp_path):
+    """On a host without a HIP device gbte_create returns GBTE_HIP_ERROR; there
+    is no CPU path to fall back to."""
+    lib = _lib.load()
+    p = Problem(decks.write_deck(str(tmp_path / "g"), decks.gray_square_spec(3, nphi=2)))
+    h = C.c_void_p()
+    opts = _lib.Options(0, 0, 0, 0.0)
+    rc = lib.gbte_create(p.handle, C.byref(opts), C.byref(h))
+    msg = lib.gbte_last_error(h).decode()
+    lib.gbte_destroy(h)
+    if rc == 0:
+        pytest.skip("a HIP device is present; covered by the gpu tests")
+    assert rc == _lib.GBTE_HIP_ERROR and "HIP device" in msg
+    with pytest.raises(GBTEError):
+        BTESolver(p)

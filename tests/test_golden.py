@@ -1,0 +1,131 @@
+"""Pin the deck/geometry layer and the CPU oracle against the compiled
+reference's full-precision dumps (tests/golden/*.npz, written by
+tests/golden/make_golden.py from oracle/_ref/ref_dump at np=1)."""
+import os
+
+import numpy as np
+import pytest
+
+import _oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+import sys  # noqa: E402
+
+sys.path.insert(0, GOLD)
+import make_golden  # noqa: E402
+
+NAMES = list(make_golden.golden_decks())
+
+# product table -> reference dump name
+PAIRS = {"face_size": "elementFaceSize", "face_neighbor": "elementFaceNeighbor", "face_bound": "elementFaceBound",
+         "face_index": "elementFaceIndex", "face_area": "elementFaceArea", "face_nx": "elementFaceNormX",
+         "face_ny": "elementFaceNormY", "face_nz": "elementFaceNormZ", "face_cx": "elementFaceCenterX",
+         "face_cy": "elementFaceCenterY", "face_cz": "elementFaceCenterZ", "volume": "elementVolume",
+         "cx": "elementCenterX", "cy": "elementCenterY", "cz": "elementCenterZ", "heat_source": "elementHeatSource",
+         "matter": "matter", "cell_nvert": "elementVertexesSize", "cell_vert": "elementVertexes",
+         "face_nvert": "elementFaceVertexesSize", "face_vert": "elementFaceVertexes", "nbr_ptr": "neighborPtr",
+         "nbr_idx": "neighborIdx", "node_x": "nodeX", "node_y": "nodeY", "node_z": "nodeZ",
+         "bound_cell": "boundaryCell", "bound_face": "boundaryFace", "bound_type": "boundaryType",
+         "bound_temp": "boundaryThermal", "bound_connect": "boundaryConnect", "fface_cell": "faceCell",
+         "fface_face": "faceFace", "fface_bound": "faceBound", "kappa_bulk": "kappaBulk",
+         "capacity_bulk": "capacityBulk", "dir_x": "directionX", "dir_y": "directionY", "dir_z": "directionZ",
+         "weight": "modeWeight", "tau": "relaxationTime", "cap": "heatCapacity", "lr": "latticeRatio",
+         "hr": "heatRatio"}
+
+
+@pytest.fixture(scope="module")
+def loaded(tmp_path_factory):
+    from giftbte_amd import Problem, decks
+    root = tmp_path_factory.mktemp("golden_decks")
+    out = {}
+    for name, (factory, fmt, *_rest) in make_golden.golden_decks().items():
+        d = decks.write_deck(os.path.join(str(root), name), factory(), mesh_format=fmt)
+        out[name] = (Problem(d), dict(np.load(os.path.join(GOLD, name + ".npz"))))
+    return out
+
+
+def _masked(name, a, t):
+    """Zero the padding slots the reference leaves uninitialised."""
+    fs = t.array("face_size")
+    if a.ndim >= 2 and a.shape[0] == t.n_cells and a.shape[1] == 6:
+        m = np.arange(6)[None, :] < fs[:, None]
+        if a.ndim == 3:
+            m = m[:, :, None] & (np.arange(4)[None, None, :] < t.array("face_nvert")[:, :, None])
+        a = np.where(m, a, 0)
+    if name == "cell_vert":
+        a = np.where(np.arange(8)[None, :] < t.array("cell_nvert")[:, None], a, 0)
+    return a
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_tables_bitwise(loaded, name):
+    p, g = loaded[name]
+    t = p.tables
+    assert list(g["sizes"][:6]) == [t.n_cells, t.n_bounds, t.n_faces, t.n_nodes, t.n_bands, t.n_dirs]
+    for ours, ref in PAIRS.items():
+        a = _masked(ours, p.array(ours), t)
+        b = _masked(ours, g[ref].reshape(a.shape), t)
+        assert np.array_equal(a, b), ours
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_run_matches_reference(loaded, name):
+    p, g = loaded[name]
+    max_iter, tolT, tolQ = g["run"]
+    f, info = _oracle.run(p.tables, int(max_iter), tolT, tolQ)
+    assert info["iterations"] == int(g["count_iterations"])
+    assert info["macro_passes"] == int(g["count_macro_passes"])
+    assert info["fourier_passes"] == int(g["count_fourier_passes"])
+    assert _oracle.rel_l2(f["T_lattice"], g["T_lattice"]) < 1e-12
+    assert _oracle.rel_l2(f["T_energy"], g["T_energy"]) < 1e-12
+    assert _oracle.rel_l2(f["T_band"], g["T_band"]) < 1e-12
+    q = np.concatenate([f["qx"], f["qy"]])
+    qr = np.concatenate([g["qx"], g["qy"]])
+    assert _oracle.rel_l2(q, qr) < 1e-9
+    assert np.allclose([info["err_temp"], info["err_flux"]], g["final_errors"], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_fourier_matches_reference(loaded, name):
+    p, g = loaded[name]
+    passes, f = _oracle.fourier(p.tables)
+    assert passes == int(g["count_fourier_passes"])
+    assert _oracle.rel_l2(f["T"], g["fourier_T"]) < 1e-13
+    q = np.concatenate([f["qx"], f["qy"]])
+    assert _oracle.rel_l2(q, np.concatenate([g["fourier_qx"], g["fourier_qy"]])) < 1e-8
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_stages_match_reference(loaded, name):
+    """Every per-system stage of iteration 0 (gradient, residual + solve,
+    outgoing boundary values, vertex values, face gradients, non-Fourier
+    residual) for one system, and the boundary values after the set-up."""
+    p, g = loaded[name]
+    t = p.tables
+    band, direction = (int(v) for v in g["stage_system"])
+    s = _oracle.stages(t, band, direction)
+    nc, nf = t.n_cells, t.n_faces
+    # entries without a cell on that side are never written by the reference
+    valid = np.broadcast_to((g["boundaryCell"] >= 0)[None, None], g["stage_ebound_init"].shape)
+    assert _oracle.rel_l2(s["ebound_init"][valid.ravel()], g["stage_ebound_init"][valid]) < 1e-13
+    grad = s["grad"].reshape(3, nc)
+    gs = np.linalg.norm(np.concatenate([g["stage_gradX"], g["stage_gradY"]]))
+    assert np.linalg.norm(np.concatenate([grad[0] - g["stage_gradX"], grad[1] - g["stage_gradY"]])) <= 1e-9 * gs + 1e-300
+    assert _oracle.rel_l2(s["e_new"], g["stage_e_new"]) < 1e-13
+    vs = (g["boundaryCell"] >= 0).ravel()
+    assert _oracle.rel_l2(s["ebound_sys"][vs], g["stage_ebound_sys"].ravel()[vs]) < 1e-13
+    assert _oracle.rel_l2(s["e_vertex"], g["stage_e_vertex"]) < 1e-13
+    gf = s["gface"].reshape(3, nf * 2)
+    # faces the reference never writes (specular boundaries) hold stale values there
+    bound = g["faceBound"]
+    btype = g["boundaryType"]
+    skip = np.zeros(nf, bool)
+    skip[bound >= 0] = btype[bound[bound >= 0]] == 3
+    # boundary faces have one side only; the second slot is never written
+    keep = np.stack([~skip, bound < 0], axis=1).ravel()
+    ours = np.concatenate([gf[0][keep], gf[1][keep]])
+    ref_gf = np.concatenate([g["stage_gradFaceX"].ravel()[keep], g["stage_gradFaceY"].ravel()[keep]])
+    assert _oracle.rel_l2(ours, ref_gf) < 1e-8
+    assert _oracle.rel_l2(s["remacro"], g["stage_ReMacro"]) < 1e-8
