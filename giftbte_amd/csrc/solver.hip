@@ -2,25 +2,31 @@
 //
 // One outer iteration of StaticBTESynthetic::solve
 // (src/StaticBTESynthetic/StaticBTESyntheticSolver.cpp:63-221) becomes:
-//   k_copy_old         copy()                       StaticBTESynthetic.cpp:2655-2690
-//   k_sweep  (x L)     _get_Re + SparseLU solve + _get_bound_ee, all systems,
-//                      one launch per upwind level   :1427-1539, Solver.cpp:84-108
-//   k_vertex_e         _set_vertex_energydensity     :1541-1616
-//   k_face_moments     _get_face_gradient + _get_non_fourier + moments
-//                      + next iteration's _get_gradient_larger
-//                                                    :1651-1937, :2485-2520, :1243-1340
-//   k_boundary         _set_bound_ee_1 + _set_macro_boundary_condition
-//                                                    :2083-2274
-//   macro loop         _marco_solution: k_vertex_T, k_rhs, PCG   :2276-2407, :2692-2734
-//   k_err              _get_magin_check_error        :2565-2611
+//   k_copy_old       copy()                                  StaticBTESynthetic.cpp:2655-2690
+//   k_source         the parts of _get_Re that do not depend on the new
+//                    upwind values, for all systems             :1427-1500
+//   k_sweep_dir      the per-system SparseLU solve: one workgroup walks every
+//                    upwind level of one direction            Solver.cpp:96-100
+//   k_bound_out      _get_bound_ee                              :1502-1539
+//   k_vertex_e       _set_vertex_energydensity                  :1541-1616
+//   k_face_moments   _get_face_gradient + _get_non_fourier + _recover_temperature
+//                    + _get_heat_flux, and the next step's _get_gradient_larger
+//                                                              :1651-1937, :2485-2520, :1243-1340
+//   k_boundary       _set_bound_ee_1 + _set_macro_boundary_condition  :2083-2274
+//   macro loop       _marco_solution: k_vertex_T, k_rhs, PCG    :2276-2407, :2692-2734
+//   k_err            _get_magin_check_error                     :2565-2611
 //
-// Layout in HBM (lanes of a wavefront run over bands; all bands of one
-// direction share geometry, upwind order and coefficients):
-//   e     [dir][cell][band]           energy density, in place
-//   grad  [dir][cell][dim][band]      LSQ gradient of the previous sweep's e
-//   eb    [dir][entry][band]          boundary values, entry = 2*bound + side
-//   eV    [dir][node][band]           vertex (IDW) values
-// A wavefront holds floor(64 / n_bands) cells of one (direction, level) list.
+// Layout in HBM.  Every per-system field is stored cell-major with one
+// contiguous row of n_dirs*n_bands values per cell (row index t = dir*B + band):
+//   e     [cell][dir][band]           energy density, updated in place
+//   grad  [cell][dim][dir][band]      LSQ gradient of the previous sweep's e
+//   eb    [entry][dir][band]          boundary values, entry = 2*bound + side
+//   eV    [node][dir][band]           inverse-distance vertex values
+//   S     [dir][position][band]       sweep sources in each direction's upwind order
+// The spatial gathers (LSQ stencils, face neighbours, vertex stencils) use the
+// same neighbours for every band and direction, so a workgroup owns one row and
+// reads whole neighbour rows with coalesced loads.  The sweep reads and writes
+// the 8*n_bands-byte (cell, dir) segments of a row.
 #include "solver.hpp"
 
 #include <algorithm>
@@ -36,6 +42,11 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
+constexpr int kSweepBlock = 512;
+constexpr int kMaxIn = 3;  // in-faces per cell of one direction (triangles, quads, tetrahedra)
+constexpr int kNT = 8;     // row entries per thread in the row kernels
+constexpr int kNTf = 4;    // the same for the register-heavy face / source kernels
+constexpr int kRowBlockMax = 512;
 
 __device__ __forceinline__ int wave_global() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); }
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
@@ -49,6 +60,33 @@ __device__ __forceinline__ double group_sum(double v, int b, int nB) {
     return v;
 }
 
+// Rows of n_dirs*n_bands values are processed by tpr threads (a power of two,
+// tpr * rpb = kBlock); row_sum reduces over the tpr threads of a row in a
+// fixed order and returns the sum to every thread of the row.
+__device__ __forceinline__ double row_sum(double v, int tpr, double *sh) {
+    int w = threadIdx.x >> 6;
+    int lim = tpr < 64 ? tpr : 64;
+    for (int off = lim >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (tpr <= 64) return v;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    int wpr = tpr >> 6, w0 = (w / wpr) * wpr;
+    double s = 0;
+    for (int k = 0; k < wpr; ++k) s += sh[w0 + k];
+    return s;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Rows are Morton-ordered, so giving every XCD one contiguous range of
+// rows keeps neighbouring rows - which every stencil re-reads - in one L2.
+constexpr int kXcd = 8;
+// The grid must be a multiple of kXcd for this to be a bijection.
+__device__ __forceinline__ int xcd_block() {
+    int nb = gridDim.x, per = nb / kXcd;
+    return (int)(blockIdx.x % kXcd) * per + (int)(blockIdx.x / kXcd);
+}
+
 struct Modes {
     const double *vg, *tau, *cap, *lr, *hr, *sx, *sy, *sz, *w;
     double kappa, cbulk;
@@ -59,152 +97,269 @@ struct SlotGeo {
     const double *nx, *ny, *nz, *area, *vol, *rx, *ry, *rz, *qx, *qy, *qz;
 };
 
+struct Rows {
+    int DB, B, tpr, rpb;  // row length, bands, threads per row, rows per block
+};
+
 // ---------------------------------------------------------------- kernels
 
 // _set_initial (StaticBTESynthetic.cpp:765-782): e = T C_b for all systems.
-__global__ void k_init_e(double *__restrict__ e, const double *__restrict__ T, const double *__restrict__ cap, int nc,
+__global__ void k_init_e(double *__restrict__ e, const double *__restrict__ T, const double *__restrict__ cap, int DB,
                          int nB, size_t total) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    int b = (int)(i % nB);
-    size_t dc = i / nB;
-    int c = (int)(dc % nc);
-    e[i] = T[c] * cap[b];
+    e[i] = T[i / DB] * cap[(i % DB) % nB];
 }
 
-// _get_gradient_larger, Order 2 (StaticBTESynthetic.cpp:1243-1340).
-__device__ __forceinline__ void lsq_gradient(const double *__restrict__ ed, int c, int b, int nB, double ec,
+// _get_gradient_larger, Order 2 (StaticBTESynthetic.cpp:1243-1340), for the
+// whole row of one cell: grad = P [(e_j - e_i) / L_x].  Each thread owns kNT
+// row entries (stride tpr), so every stencil neighbour costs kNT independent
+// loads in flight.
+__device__ __forceinline__ void row_gradient(const double *__restrict__ e, double *__restrict__ grad, int c,
                                              const int *__restrict__ gptr, const int *__restrict__ gidx,
                                              const double *__restrict__ gP, const int *__restrict__ gok, int dim,
-                                             int nnz, double invLx, double &gx, double &gy, double &gz) {
-    gx = 0; gy = 0; gz = 0;
-    if (!gok[c]) return;
+                                             int nnz, double invLx, int t0, int tpr, int DB) {
     int a = gptr[c], z = gptr[c + 1];
-    for (int k = a; k < z; ++k) {
-        double d1 = invLx * (ed[(size_t)gidx[k] * nB + b] - ec);
-        gx += gP[k] * d1;
-        gy += gP[(size_t)nnz + k] * d1;
-        if (dim == 3) gz += gP[2 * (size_t)nnz + k] * d1;
-    }
-    if (isnan(gx) || isnan(gy) || isnan(gz)) { gx = 0; gy = 0; gz = 0; }
-}
-
-__global__ void k_gradient(const double *__restrict__ e, double *__restrict__ grad, const int *__restrict__ gptr,
-                           const int *__restrict__ gidx, const double *__restrict__ gP, const int *__restrict__ gok,
-                           int dim, int nnz, double invLx, int nc, int nB, int nD, int cpw) {
-    int lane = lane_id(), sub = lane / nB, b = lane - sub * nB;
-    long item = (long)wave_global() * cpw + sub;
-    if (sub >= cpw || item >= (long)nc * nD) return;
-    int d = (int)(item / nc), c = (int)(item % nc);
-    const double *ed = e + (size_t)d * nc * nB;
-    double gx, gy, gz;
-    lsq_gradient(ed, c, b, nB, ed[(size_t)c * nB + b], gptr, gidx, gP, gok, dim, nnz, invLx, gx, gy, gz);
-    double *g = grad + (((size_t)d * nc + c) * dim) * nB + b;
-    g[0] = gx;
-    g[nB] = gy;
-    if (dim == 3) g[2 * nB] = gz;
-}
-
-// _get_bound_ee (StaticBTESynthetic.cpp:1502-1539) for all systems; used once
-// before the loop (inside the loop the sweep writes outgoing values).
-__global__ void k_bound_out(const double *__restrict__ e, const double *__restrict__ grad, double *__restrict__ eb,
-                            SlotGeo g, Modes m, const int *__restrict__ ent_list, const int *__restrict__ be_cell,
-                            const int *__restrict__ be_slot, int n_ent, int nc, int ne, int nB, int nD, int dim, int cpw) {
-    int lane = lane_id(), sub = lane / nB, b = lane - sub * nB;
-    long item = (long)wave_global() * cpw + sub;
-    if (sub >= cpw || item >= (long)n_ent * nD) return;
-    int d = (int)(item / n_ent), k = (int)(item % n_ent);
-    int en = ent_list[k], c = be_cell[en], s = be_slot[en];
-    double vb = m.vg[b];
-    double dot = vb * m.sx[d] * g.nx[s] + vb * m.sy[d] * g.ny[s] + vb * m.sz[d] * g.nz[s];
-    if (dot >= 0) {
-        const double *gr = grad + (((size_t)d * nc + c) * dim) * nB + b;
-        double gz = dim == 3 ? gr[2 * nB] : 0.0;
-        double ev = e[((size_t)d * nc + c) * nB + b] + (g.rx[s] * gr[0] + g.ry[s] * gr[nB] + g.rz[s] * gz);
-        eb[((size_t)d * ne + en) * nB + b] = ev;
-    }
-}
-
-// One upwind level of every direction.  Per (direction, cell) and band lane:
-//   e_new = [C T_old + Q hr tau - sum_out a (r.grad e_c)
-//            - sum_in,interior a (e_new_nb + r'.grad e_nb) - sum_in,boundary a eb]
-//           / (1 + sum_out a),     a = tau (A/V) (v.n)
-// which is e + delta with M delta = R of _get_Re / the SparseLU solve
-// (StaticBTESynthetic.cpp:784-840, 1427-1500; Solver.cpp:96-100) written
-// for e_new directly.  Outgoing boundary values of _get_bound_ee follow.
-__global__ void __launch_bounds__(kBlock) k_sweep(const int *__restrict__ item_c, const int *__restrict__ item_d,
-                                                  int n_items, double *__restrict__ e, const double *__restrict__ grad,
-                                                  double *__restrict__ eb, const double *__restrict__ Told,
-                                                  const double *__restrict__ Q, SlotGeo g, Modes m, int nc, int ne,
-                                                  int nB, int dim, int cpw) {
-    int lane = lane_id(), sub = lane / nB, b = lane - sub * nB;
-    int item = wave_global() * cpw + sub;
-    if (sub >= cpw || item >= n_items) return;
-    int c = item_c[item], d = item_d[item];
-    double vb = m.vg[b], tb = m.tau[b], Cb = m.cap[b], hb = m.hr[b];
-    double vx = vb * m.sx[d], vy = vb * m.sy[d], vz = vb * m.sz[d];
-    double *ed = e + (size_t)d * nc * nB;
-    const double *gd = grad + (size_t)d * nc * dim * nB;
-    double *ebd = eb + (size_t)d * ne * nB;
-    const double *gc = gd + (size_t)c * dim * nB + b;
-    double gcx = gc[0], gcy = gc[nB], gcz = dim == 3 ? gc[2 * nB] : 0.0;
-    double S = Told[c] * Cb + Q[c] * hb * tb;
-    double M = 0;
-    int nf = g.fsize[c];
-    double vol = g.vol[c];
-    for (int j = 0; j < nf; ++j) {
-        int s = c * 6 + j;
-        double dot = vx * g.nx[s] + vy * g.ny[s] + vz * g.nz[s];
-        double a = tb * g.area[s] / vol * dot;
-        if (dot >= 0) {
-            M += a;
-            S -= a * (g.rx[s] * gcx + g.ry[s] * gcy + g.rz[s] * gcz);
-        } else {
-            int nb = g.nb[s];
-            if (nb >= 0) {
-                const double *gn = gd + (size_t)nb * dim * nB + b;
-                double gnz = dim == 3 ? gn[2 * nB] : 0.0;
-                double up = ed[(size_t)nb * nB + b] + (g.qx[s] * gn[0] + g.qy[s] * gn[nB] + g.qz[s] * gnz);
-                S -= a * up;
-            } else {
-                S -= a * ebd[(size_t)g.ent[s] * nB + b];
+    bool ok = gok[c] != 0;
+    const double *ec = e + (size_t)c * DB;
+    double *gc = grad + (size_t)c * dim * DB;
+    for (int tb = t0; tb < DB; tb += kNT * tpr) {
+        double e0[kNT], gx[kNT], gy[kNT], gz[kNT];
+#pragma unroll
+        for (int j = 0; j < kNT; ++j) {
+            int t = tb + j * tpr;
+            e0[j] = t < DB ? ec[t] : 0.0;
+            gx[j] = 0; gy[j] = 0; gz[j] = 0;
+        }
+        if (ok) {
+            for (int k = a; k < z; ++k) {
+                const double *en = e + (size_t)gidx[k] * DB;
+                double p0 = gP[k], p1 = gP[(size_t)nnz + k], p2 = dim == 3 ? gP[2 * (size_t)nnz + k] : 0.0;
+#pragma unroll
+                for (int j = 0; j < kNT; ++j) {
+                    int t = tb + j * tpr;
+                    if (t < DB) {
+                        double d1 = invLx * (en[t] - e0[j]);
+                        gx[j] += p0 * d1;
+                        gy[j] += p1 * d1;
+                        gz[j] += p2 * d1;
+                    }
+                }
             }
         }
+#pragma unroll
+        for (int j = 0; j < kNT; ++j) {
+            int t = tb + j * tpr;
+            if (t >= DB) continue;
+            double x = gx[j], y = gy[j], w = dim == 3 ? gz[j] : 0.0;
+            if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
+            gc[t] = x;
+            gc[DB + t] = y;
+            if (dim == 3) gc[2 * DB + t] = w;
+        }
     }
-    M += 1.0;
-    double en = S / M;
-    ed[(size_t)c * nB + b] = en;
-    for (int j = 0; j < nf; ++j) {
-        int s = c * 6 + j;
-        int et = g.ent[s];
-        if (et < 0) continue;
-        double dot = vx * g.nx[s] + vy * g.ny[s] + vz * g.nz[s];
-        if (dot >= 0) ebd[(size_t)et * nB + b] = en + (g.rx[s] * gcx + g.ry[s] * gcy + g.rz[s] * gcz);
+}
+
+__global__ void __launch_bounds__(kRowBlockMax) k_gradient(const double *__restrict__ e, double *__restrict__ grad,
+                                                     const int *__restrict__ gptr, const int *__restrict__ gidx,
+                                                     const double *__restrict__ gP, const int *__restrict__ gok,
+                                                     int dim, int nnz, double invLx, int nc, Rows R) {
+    int c = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+    if (c >= nc) return;
+    row_gradient(e, grad, c, gptr, gidx, gP, gok, dim, nnz, invLx, threadIdx.x % R.tpr, R.tpr, R.DB);
+}
+
+// _get_bound_ee (StaticBTESynthetic.cpp:1502-1539): outgoing boundary values
+// e + r . grad e for every system, one row per boundary entry.
+__global__ void __launch_bounds__(kRowBlockMax) k_bound_out(const double *__restrict__ e, const double *__restrict__ grad,
+                                                      double *__restrict__ eb, SlotGeo g, Modes m,
+                                                      const int *__restrict__ ent_list, const int *__restrict__ be_cell,
+                                                      const int *__restrict__ be_slot, int n_ent, int dim, Rows R) {
+    int k = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+    if (k >= n_ent) return;
+    int en = ent_list[k], c = be_cell[en], s = be_slot[en];
+    double nx = g.nx[s], ny = g.ny[s], nz = g.nz[s], rx = g.rx[s], ry = g.ry[s], rz = g.rz[s];
+    const double *ec = e + (size_t)c * R.DB;
+    const double *gc = grad + (size_t)c * dim * R.DB;
+    double *eo = eb + (size_t)en * R.DB;
+    for (int t = threadIdx.x % R.tpr; t < R.DB; t += R.tpr) {
+        int d = t / R.B, b = t - d * R.B;
+        double vb = m.vg[b];
+        double dot = vb * m.sx[d] * nx + vb * m.sy[d] * ny + vb * m.sz[d] * nz;
+        if (dot >= 0) {
+            double gz = dim == 3 ? gc[2 * R.DB + t] : 0.0;
+            eo[t] = ec[t] + (rx * gc[t] + ry * gc[R.DB + t] + rz * gz);
+        }
+    }
+}
+
+// Parallel part of one transport step for every system:
+//   S = C T_old + Q hr tau - sum_out a (r.grad e_c) - sum_in,interior a (r'.grad e_nb)
+//       - sum_in,boundary a eb,      a = tau (A/V) (v.n),
+// i.e. everything of _get_Re (StaticBTESynthetic.cpp:1427-1500) that does not
+// depend on the new upwind values.  One row per cell; written to each
+// direction's upwind order for the sweep.
+__global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__ posc, double *__restrict__ S,
+                                                         const double *__restrict__ grad, const double *__restrict__ eb,
+                                                         const double *__restrict__ Told, const double *__restrict__ Q,
+                                                         SlotGeo g, Modes m, int nc, int dim, Rows R) {
+    int c = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+    if (c >= nc) return;
+    int DB = R.DB, nB = R.B, nD = DB / nB;
+    int nf = g.fsize[c];
+    double vol = g.vol[c], To = Told[c], Qc = Q[c];
+    const double *gc = grad + (size_t)c * dim * DB;
+    for (int tb = threadIdx.x % R.tpr; tb < DB; tb += kNTf * R.tpr) {
+        double s[kNTf], gcx[kNTf], gcy[kNTf], gcz[kNTf];
+#pragma unroll
+        for (int j = 0; j < kNTf; ++j) {
+            int t = tb + j * R.tpr;
+            s[j] = 0; gcx[j] = 0; gcy[j] = 0; gcz[j] = 0;
+            if (t < DB) {
+                int b = t % nB;
+                s[j] = To * m.cap[b] + Qc * m.hr[b] * m.tau[b];
+                gcx[j] = gc[t];
+                gcy[j] = gc[DB + t];
+                gcz[j] = dim == 3 ? gc[2 * DB + t] : 0.0;
+            }
+        }
+        for (int f = 0; f < nf; ++f) {
+            int k = c * 6 + f;
+            double nx = g.nx[k], ny = g.ny[k], nz = g.nz[k], av = g.area[k] / vol;
+            double rx = g.rx[k], ry = g.ry[k], rz = g.rz[k], qx = g.qx[k], qy = g.qy[k], qz = g.qz[k];
+            int nb = g.nb[k];
+            const double *gn = grad + (size_t)(nb >= 0 ? nb : 0) * dim * DB;
+            const double *ebr = eb + (size_t)(nb >= 0 ? 0 : g.ent[k]) * DB;
+#pragma unroll
+            for (int j = 0; j < kNTf; ++j) {
+                int t = tb + j * R.tpr;
+                if (t >= DB) continue;
+                int d = t / nB, b = t - d * nB;
+                double vb = m.vg[b], tb_ = m.tau[b];
+                double dot = vb * m.sx[d] * nx + vb * m.sy[d] * ny + vb * m.sz[d] * nz;
+                double a = tb_ * av * dot;
+                if (dot >= 0) {
+                    s[j] -= a * (rx * gcx[j] + ry * gcy[j] + rz * gcz[j]);
+                } else if (nb >= 0) {
+                    double gnz = dim == 3 ? gn[2 * DB + t] : 0.0;
+                    s[j] -= a * (qx * gn[t] + qy * gn[DB + t] + qz * gnz);
+                } else {
+                    s[j] -= a * ebr[t];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kNTf; ++j) {
+            int t = tb + j * R.tpr;
+            if (t >= DB) continue;
+            int d = t / nB, b = t - d * nB;
+            S[((size_t)d * nc + posc[(size_t)c * nD + d]) * nB + b] = s[j];
+        }
+    }
+}
+
+// The upwind solve of every direction (the Eigen SparseLU solve of
+// StaticBTESyntheticSolver.cpp:96-100 on the matrix of _get_coefficient,
+// StaticBTESynthetic.cpp:784-840): one workgroup walks all upwind levels of
+// one direction for a chunk of bands,
+//   e_new = (S - sum_in,interior tau v g_f e_new_nb) / (1 + tau v G_out),
+// with g_f = (A/V)(s.n) of the in-faces and G_out the sum over out-faces.
+// Levels are separated by workgroup barriers; the most recent R positions
+// stay in an LDS ring, older upwind values are read back from HBM.
+__global__ void __launch_bounds__(kSweepBlock) k_sweep_dir(
+    const int *__restrict__ item_cell, const int *__restrict__ levptr, const int *__restrict__ dlev,
+    const int *__restrict__ rec_q, const int *__restrict__ rec_c, const double *__restrict__ rec_g,
+    const double *__restrict__ rec_G, const double *__restrict__ S, double *__restrict__ e, Modes m, int nc, int nB,
+    int nD, int Bc, int nchunk, int R) {
+    extern __shared__ double ring[];
+    int d = blockIdx.x / nchunk, chunk = blockIdx.x - d * nchunk;
+    int tid = threadIdx.x, bl = tid % Bc, slot = tid / Bc, cpp = blockDim.x / Bc;
+    int b = chunk * Bc + bl;
+    bool act = slot < cpp && b < nB;
+    double tv = act ? m.tau[b] * m.vg[b] : 0.0;
+    const int *lp = levptr + dlev[d];
+    int L = dlev[d + 1] - dlev[d] - 1;
+    size_t base = (size_t)d * nc;
+    size_t DB = (size_t)nD * nB;
+    double *ed = e + (size_t)d * nB + b;  // + cell * DB
+    // operands of the first item of the next level are loaded before the
+    // barrier so that their latency overlaps it
+    double pf_s = 0, pf_G = 0, pf_g[kMaxIn];
+    int pf_q[kMaxIn], pf_c[kMaxIn], pf_cell = 0;
+    auto fetch = [&](size_t P) {
+        pf_s = S[P * nB + b];
+        pf_G = rec_G[P];
+        pf_cell = item_cell[P];
+#pragma unroll
+        for (int j = 0; j < kMaxIn; ++j) {
+            pf_q[j] = rec_q[P * kMaxIn + j];
+            pf_c[j] = rec_c[P * kMaxIn + j];
+            pf_g[j] = rec_g[P * kMaxIn + j];
+        }
+    };
+    if (act && L > 0 && lp[0] + slot < lp[1]) fetch(base + lp[0] + slot);
+    for (int l = 0; l < L; ++l) {
+        int a0 = lp[l], a1 = lp[l + 1];
+        int lo = a1 - R;
+        if (act) {
+            for (int k = a0 + slot; k < a1; k += cpp) {
+                size_t P = base + k;
+                if (k != a0 + slot) fetch(P);
+                double s = pf_s;
+                double M = 1.0 + tv * pf_G;
+#pragma unroll
+                for (int j = 0; j < kMaxIn; ++j) {
+                    int q = pf_q[j];
+                    if (q < 0) break;
+                    double ev = (q >= lo) ? ring[(q % R) * Bc + bl] : ed[(size_t)pf_c[j] * DB];
+                    s -= tv * pf_g[j] * ev;
+                }
+                double en = s / M;
+                ed[(size_t)pf_cell * DB] = en;
+                ring[(k % R) * Bc + bl] = en;
+            }
+            if (l + 1 < L && lp[l + 1] + slot < lp[l + 2]) fetch(base + lp[l + 1] + slot);
+        }
+        __syncthreads();
     }
 }
 
 // Inverse-distance vertex values (_set_vertex_energydensity,
 // StaticBTESynthetic.cpp:1541-1616): interior nodes from cells, nodes on a
-// boundary face from the boundary values of those faces.
-__global__ void k_vertex_e(const double *__restrict__ e, const double *__restrict__ eb, double *__restrict__ eV,
-                           const int *__restrict__ vptr, const int *__restrict__ vsrc, const double *__restrict__ vdist,
-                           int nn, int nc, int ne, int nB, int nD, int cpw) {
-    int lane = lane_id(), sub = lane / nB, b = lane - sub * nB;
-    long item = (long)wave_global() * cpw + sub;
-    if (sub >= cpw || item >= (long)nn * nD) return;
-    int d = (int)(item / nn), v = (int)(item % nn);
-    const double *ed = e + (size_t)d * nc * nB;
-    const double *ebd = eb + (size_t)d * ne * nB;
-    double t = 0, r = 0;
-    for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
-        int src = vsrc[k];
-        double val = src >= 0 ? ed[(size_t)src * nB + b] : ebd[(size_t)(-src - 1) * nB + b];
-        t += val / vdist[k];
-        r += 1.0 / vdist[k];
+// boundary face from the boundary values of those faces.  One row per node.
+__global__ void __launch_bounds__(kRowBlockMax) k_vertex_e(const double *__restrict__ e,
+                                                           const double *__restrict__ eb, double *__restrict__ eV,
+                                                           const int *__restrict__ vptr, const int *__restrict__ vsrc,
+                                                           const double *__restrict__ vdist, int nn, Rows R) {
+    int v = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+    if (v >= nn) return;
+    int a = vptr[v], z = vptr[v + 1];
+    int DB = R.DB;
+    for (int tb = threadIdx.x % R.tpr; tb < DB; tb += kNT * R.tpr) {
+        double sm[kNT];
+#pragma unroll
+        for (int j = 0; j < kNT; ++j) sm[j] = 0;
+        double r = 0;
+        for (int k = a; k < z; ++k) {
+            int src = vsrc[k];
+            const double *row = src >= 0 ? e + (size_t)src * DB : eb + (size_t)(-src - 1) * DB;
+            double dk = vdist[k];
+#pragma unroll
+            for (int j = 0; j < kNT; ++j) {
+                int t = tb + j * R.tpr;
+                if (t < DB) sm[j] += row[t] / dk;
+            }
+            r += 1.0 / dk;
+        }
+#pragma unroll
+        for (int j = 0; j < kNT; ++j) {
+            int t = tb + j * R.tpr;
+            if (t >= DB) continue;
+            double x = sm[j] / r;
+            if (isnan(x)) x = 0;
+            eV[(size_t)v * DB + t] = x;
+        }
     }
-    double x = t / r;
-    if (isnan(x)) x = 0;
-    eV[((size_t)d * nn + v) * nB + b] = x;
 }
 
 struct FaceK {
@@ -212,84 +367,114 @@ struct FaceK {
     const double *an, *aa, *n0x, *n0y, *n0z, *mx, *my, *m2;
 };
 
-// Per cell, looping over directions with band lanes:
+// One row per cell, over every (direction, band):
 //   face gradients (_get_face_gradient, StaticBTESynthetic.cpp:1651-1783),
 //   the non-Fourier residual (_get_non_fourier, :1883-1937),
-//   the moments (_recover_temperature, _get_total_energy, _get_heat_flux,
-//   :2485-2520), and the LSQ gradient of the new e for the next sweep.
+//   the lattice-temperature and heat-flux moments (_recover_temperature,
+//   _get_heat_flux, :2485-2520), reduced over the row,
+//   and the LSQ gradient of the new e for the next transport step.
 // kind: 0 none (specular), 1 interior face owned first by this cell,
 //       3 interior face owned first by the neighbour, 2 boundary face.
-__global__ void __launch_bounds__(kBlock) k_face_moments(
+__global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
     const double *__restrict__ e, const double *__restrict__ eb, const double *__restrict__ eV,
     double *__restrict__ grad, SlotGeo g, FaceK f, Modes m, const int *__restrict__ gptr, const int *__restrict__ gidx,
     const double *__restrict__ gP, const int *__restrict__ gok, int nnz, double invLx, double *__restrict__ T,
-    double *__restrict__ E, double *__restrict__ qx, double *__restrict__ qy, double *__restrict__ qz,
-    double *__restrict__ RM, double *__restrict__ Tb, int nc, int ne, int nn, int nB, int nD, int dim, int cpw) {
-    int lane = lane_id(), sub = lane / nB, b = lane - sub * nB;
-    int c = wave_global() * cpw + sub;
-    bool act = sub < cpw && c < nc;
-    int cc = act ? c : 0;
-    double vb = m.vg[b], tb = m.tau[b], Cb = m.cap[b], lb = m.lr[b], hb = m.hr[b];
-    double aT = 0, aTb = 0, aE = 0, ax = 0, ay = 0, az = 0, aR = 0;
+    double *__restrict__ qx, double *__restrict__ qy, double *__restrict__ qz, double *__restrict__ RM, int nc,
+    int dim, Rows R) {
+    __shared__ double sh[kRowBlockMax / 64];
+    int c = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+    bool act = c < nc;
+    int cc = act ? c : nc - 1;
+    int DB = R.DB, nB = R.B;
+    const double *ec = e + (size_t)cc * DB;
     int nf = g.fsize[cc];
-    for (int d = 0; d < nD; ++d) {
-        const double *ed = e + (size_t)d * nc * nB;
-        const double *ebd = eb + (size_t)d * ne * nB;
-        const double *eVd = eV + (size_t)d * nn * nB;
-        double ec = ed[(size_t)cc * nB + b];
-        double wd = m.w[d];
-        double vx = vb * m.sx[d], vy = vb * m.sy[d], vz = vb * m.sz[d];
-        aT += lb * ec * wd / Cb;
-        aTb += ec * wd / Cb;
-        aE += ec * wd / m.cbulk;
-        ax += vx * wd * ec;
-        ay += vy * wd * ec;
-        az += vz * wd * ec;
-        if (hb > 0) {
-            for (int j = 0; j < nf; ++j) {
-                int s = cc * 6 + j;
-                int kd = f.kind[s];
-                if (kd == 0) continue;
-                double de;
-                int oth = f.other[s];
-                if (kd == 1) de = ed[(size_t)oth * nB + b] - ec;
-                else if (kd == 3) de = ec - ed[(size_t)oth * nB + b];
-                else de = ebd[(size_t)oth * nB + b] - ec;
-                double gn = de * f.an[s] / f.aa[s];
-                double dv = eVd[(size_t)f.v1[s] * nB + b] - eVd[(size_t)f.v0[s] * nB + b];
-                double gx = dv / f.m2[s] * f.mx[s], gy = dv / f.m2[s] * f.my[s], gz = 0;
-                gx = gn * f.n0x[s] + gx;
-                gy = gn * f.n0y[s] + gy;
-                gz = gn * f.n0z[s] + gz;
-                double nx = g.nx[s], ny = g.ny[s], nz = g.nz[s], A = g.area[s];
-                aR -= wd * A * tb * (vx * gx + vy * gy + vz * gz) * (vx * nx + vy * ny + vz * nz);
-                aR += m.kappa * wd * A * lb / Cb * (nx * gx + ny * gy + nz * gz);
+    int t0 = threadIdx.x % R.tpr;
+    double aT = 0, ax = 0, ay = 0, az = 0, aR = 0;
+    for (int tb = t0; tb < DB; tb += kNTf * R.tpr) {
+        double e0[kNTf];
+#pragma unroll
+        for (int j = 0; j < kNTf; ++j) {
+            int t = tb + j * R.tpr;
+            e0[j] = 0;
+            if (t < DB) {
+                int d = t / nB, b = t - d * nB;
+                double vb = m.vg[b], wd = m.w[d];
+                double x = ec[t];
+                e0[j] = x;
+                aT += m.lr[b] * x * wd / m.cap[b];
+                ax += vb * m.sx[d] * wd * x;
+                ay += vb * m.sy[d] * wd * x;
+                az += vb * m.sz[d] * wd * x;
             }
         }
-        double gx, gy, gz;
-        lsq_gradient(ed, cc, b, nB, ec, gptr, gidx, gP, gok, dim, nnz, invLx, gx, gy, gz);
-        if (act) {
-            double *gr = grad + (((size_t)d * nc + cc) * dim) * nB + b;
-            gr[0] = gx;
-            gr[nB] = gy;
-            if (dim == 3) gr[2 * nB] = gz;
+        for (int fi = 0; fi < nf; ++fi) {
+            int s = cc * 6 + fi;
+            int kd = f.kind[s];
+            if (kd == 0) continue;
+            int oth = f.other[s];
+            const double *orow = (kd == 2 ? eb : e) + (size_t)oth * DB;
+            const double *v0 = eV + (size_t)f.v0[s] * DB, *v1 = eV + (size_t)f.v1[s] * DB;
+            double an = f.an[s], aa = f.aa[s], n0x = f.n0x[s], n0y = f.n0y[s], n0z = f.n0z[s];
+            double mx = f.mx[s], my = f.my[s], m2 = f.m2[s];
+            double nx = g.nx[s], ny = g.ny[s], nz = g.nz[s], A = g.area[s];
+#pragma unroll
+            for (int j = 0; j < kNTf; ++j) {
+                int t = tb + j * R.tpr;
+                if (t >= DB) continue;
+                int d = t / nB, b = t - d * nB;
+                if (!(m.hr[b] > 0)) continue;
+                double vb = m.vg[b], tb_ = m.tau[b], wd = m.w[d];
+                double vx = vb * m.sx[d], vy = vb * m.sy[d], vz = vb * m.sz[d];
+                double o = orow[t];
+                double de = kd == 3 ? e0[j] - o : o - e0[j];
+                double gn = de * an / aa;
+                double dv = v1[t] - v0[t];
+                double gx = dv / m2 * mx, gy = dv / m2 * my, gz = 0;
+                gx = gn * n0x + gx;
+                gy = gn * n0y + gy;
+                gz = gn * n0z + gz;
+                aR -= wd * A * tb_ * (vx * gx + vy * gy + vz * gz) * (vx * nx + vy * ny + vz * nz);
+                aR += m.kappa * wd * A * m.lr[b] / m.cap[b] * (nx * gx + ny * gy + nz * gz);
+            }
         }
     }
-    if (act) Tb[(size_t)b * nc + c] = aTb;
-    aT = group_sum(aT, b, nB);
-    aE = group_sum(aE, b, nB);
-    ax = group_sum(ax, b, nB);
-    ay = group_sum(ay, b, nB);
-    az = group_sum(az, b, nB);
-    aR = group_sum(aR, b, nB);
-    if (act && b == 0) {
+    if (act) row_gradient(e, grad, cc, gptr, gidx, gP, gok, dim, nnz, invLx, t0, R.tpr, DB);
+    aT = row_sum(aT, R.tpr, sh);
+    ax = row_sum(ax, R.tpr, sh);
+    ay = row_sum(ay, R.tpr, sh);
+    az = row_sum(az, R.tpr, sh);
+    aR = row_sum(aR, R.tpr, sh);
+    if (act && t0 == 0) {
         T[c] = aT;
-        E[c] = aE;
         qx[c] = ax;
         qy[c] = ay;
         qz[c] = az;
         RM[c] = aR;
     }
+}
+
+// Output-only moments: energy-equivalent temperature (_get_total_energy,
+// StaticBTESynthetic.cpp:2498-2507) and band temperatures
+// (_recover_temperature, :2487-2496).
+__global__ void __launch_bounds__(kRowBlockMax) k_output_moments(const double *__restrict__ e, Modes m,
+                                                                 double *__restrict__ E, double *__restrict__ Tb,
+                                                                 int nc, Rows R) {
+    __shared__ double sh[kRowBlockMax / 64];
+    int c = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+    bool act = c < nc;
+    int cc = act ? c : nc - 1;
+    int DB = R.DB, nB = R.B, nD = DB / nB;
+    int t0 = threadIdx.x % R.tpr;
+    const double *ec = e + (size_t)cc * DB;
+    double aE = 0;
+    for (int t = t0; t < DB; t += R.tpr) aE += ec[t] * m.w[t / nB] / m.cbulk;
+    for (int b = t0; b < nB; b += R.tpr) {
+        double s = 0;
+        for (int d = 0; d < nD; ++d) s += ec[d * nB + b] * m.w[d] / m.cap[b];
+        if (act) Tb[(size_t)b * nc + c] = s;
+    }
+    aE = row_sum(aE, R.tpr, sh);
+    if (act && t0 == 0) E[c] = aE;
 }
 
 // _set_bound_ee_1 (StaticBTESynthetic.cpp:2083-2232) at np=1 and
@@ -302,8 +487,8 @@ __global__ void k_boundary(double *__restrict__ eb, double *__restrict__ tmac, M
                            const int *__restrict__ be_conn, const int *__restrict__ be_spec,
                            const double *__restrict__ be_nx, const double *__restrict__ be_ny,
                            const double *__restrict__ be_nz, const double *__restrict__ be_tw,
-                           const double *__restrict__ be_dT, const double *__restrict__ be_tov, int n_ent, int ne,
-                           int nB, int nD, int cpw) {
+                           const double *__restrict__ be_dT, const double *__restrict__ be_tov, int n_ent, int nB,
+                           int nD, int cpw) {
     int lane = lane_id(), sub = lane / nB, b = lane - sub * nB;
     int k = wave_global() * cpw + sub;
     bool act = sub < cpw && k < n_ent;
@@ -311,6 +496,8 @@ __global__ void k_boundary(double *__restrict__ eb, double *__restrict__ tmac, M
     double vb = m.vg[b], Cb = m.cap[b], lb = m.lr[b], hb = m.hr[b];
     double nx = be_nx[en], ny = be_ny[en], nz = be_nz[en];
     int ty = be_type[en];
+    size_t DB = (size_t)nD * nB;
+    double *row = eb + (size_t)en * DB + b;
     double tm = 0;
     if (act) {
         double diff = 0;
@@ -319,22 +506,23 @@ __global__ void k_boundary(double *__restrict__ eb, double *__restrict__ tmac, M
             for (int q = 0; q < nD; ++q) {
                 double d1 = vb * m.sx[q] * nx + vb * m.sy[q] * ny + vb * m.sz[q] * nz;
                 if (d1 >= 0) {
-                    num += eb[((size_t)q * ne + en) * nB + b] * (d1 * m.w[q]);
+                    num += row[(size_t)q * nB] * (d1 * m.w[q]);
                     den += (d1 * m.w[q]);
                 }
             }
             diff = num / den;
         }
         for (int d = 0; d < nD; ++d) {
-            size_t at = ((size_t)d * ne + en) * nB + b;
             double dot = vb * m.sx[d] * nx + vb * m.sy[d] * ny + vb * m.sz[d] * nz;
             if (hb != 0 && dot < 0) {
-                if (ty == 1) eb[at] = Cb * be_tw[en];
-                else if (ty == 2) eb[at] = diff;
-                else if (ty == 3) eb[at] = eb[((size_t)be_spec[(size_t)en * nD + d] * ne + en) * nB + b];
-                else if (ty < 0) eb[at] = eb[((size_t)d * ne + be_conn[en]) * nB + b] + Cb * be_dT[en];
+                double v = row[(size_t)d * nB];
+                if (ty == 1) v = Cb * be_tw[en];
+                else if (ty == 2) v = diff;
+                else if (ty == 3) v = row[(size_t)be_spec[(size_t)en * nD + d] * nB];
+                else if (ty < 0) v = eb[(size_t)be_conn[en] * DB + (size_t)d * nB + b] + Cb * be_dT[en];
+                row[(size_t)d * nB] = v;
             }
-            tm += lb * eb[at] / Cb * m.w[d];
+            tm += lb * row[(size_t)d * nB] / Cb * m.w[d];
         }
     }
     tm = group_sum(tm, b, nB);
@@ -628,7 +816,7 @@ bool Solver::upload(DBuf<T> &b, const std::vector<T> &h) {
     return check(hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
 }
 
-Solver::Solver(const Problem &p, const gbte_options &o) : P(p), opt(o) {
+Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_new_, node_new_)), P(Pp), opt(o) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         err_ = "no HIP device available";
@@ -708,6 +896,19 @@ bool Solver::build() {
     for (int ib = 0; ib < nb; ++ib)
         if (p.bound_type[ib] == 4 || p.bound_type[ib] == 5) { err_ = "interface boundaries (type 4/5) are not supported"; return false; }
     cpw = 64 / nB;
+    {
+        // about kNT row entries per thread: long rows get one block of a
+        // multiple of 64 threads, short rows share a 256-thread block
+        int DB = nD * nB, need = (DB + kNT - 1) / kNT;
+        if (need > 64) {
+            row_tpr = std::min(kRowBlockMax, ((need + 63) / 64) * 64);
+            row_rpb = 1;
+        } else {
+            row_tpr = 1;
+            while (row_tpr < need) row_tpr <<= 1;
+            row_rpb = kBlock / row_tpr;
+        }
+    }
     // ---- modes
     std::vector<double> sx(p.dir_x), sy(p.dir_y), sz(p.dir_z), w(p.weight);
     std::vector<double> vg(p.vg.begin(), p.vg.begin() + nB), tau(p.tau.begin(), p.tau.begin() + nB),
@@ -754,13 +955,16 @@ bool Solver::build() {
         !upload(d_sarea, p.face_area) || !upload(d_srx, srx) || !upload(d_sry, sry) || !upload(d_srz, srz) ||
         !upload(d_sqx, sqx) || !upload(d_sqy, sqy) || !upload(d_sqz, sqz))
         return false;
-    // ---- per-direction upwind levels (band 0 decides; band signs agree
-    // except for faces parallel to a direction to rounding, whose coupling
-    // coefficient vanishes)
+    // ---- per-direction upwind order.  Band 0 decides the upwind graph; the
+    // band signs agree except for faces parallel to a direction to rounding,
+    // whose coupling coefficient vanishes.
     {
-        std::vector<std::vector<int>> lev_c, lev_d;
-        std::vector<int> level(nc), indeg(nc), q;
+        size_t NI = (size_t)nD * nc;
+        std::vector<int> item_cell(NI), posc(NI), levptr, dlev(nD + 1, 0), rec_q(NI * kMaxIn, -1), rec_c(NI * kMaxIn, -1);
+        std::vector<double> rec_g(NI * kMaxIn, 0.0), rec_G(NI, 0.0);
+        std::vector<int> level(nc), indeg(nc), q, pos(nc), cnt;
         double v0 = vg[0];
+        nlev = 0;
         for (int d = 0; d < nD; ++d) {
             auto dotf = [&](size_t s) { return v0 * sx[d] * p.face_nx[s] + v0 * sy[d] * p.face_ny[s] + v0 * sz[d] * p.face_nz[s]; };
             std::fill(indeg.begin(), indeg.end(), 0);
@@ -788,21 +992,47 @@ bool Solver::build() {
                 err_ = "the upwind graph of direction " + std::to_string(d) + " has a cycle; an acyclic (e.g. Delaunay) mesh is required";
                 return false;
             }
+            int L = 0;
+            for (int c = 0; c < nc; ++c) L = std::max(L, level[c] + 1);
+            nlev = std::max(nlev, L);
+            cnt.assign(L + 1, 0);
+            for (int c = 0; c < nc; ++c) cnt[level[c] + 1]++;
+            for (int l = 0; l < L; ++l) cnt[l + 1] += cnt[l];
+            dlev[d + 1] = dlev[d] + L + 1;
+            for (int l = 0; l <= L; ++l) levptr.push_back(cnt[l]);
             for (int c = 0; c < nc; ++c) {
-                if ((int)lev_c.size() <= level[c]) { lev_c.resize(level[c] + 1); lev_d.resize(level[c] + 1); }
-                lev_c[level[c]].push_back(c);
-                lev_d[level[c]].push_back(d);
+                pos[c] = cnt[level[c]]++;
+                posc[(size_t)c * nD + d] = pos[c];
+            }
+            size_t base = (size_t)d * nc;
+            for (int c = 0; c < nc; ++c) {
+                size_t P0 = base + pos[c];
+                item_cell[P0] = c;
+                int nin = 0;
+                double G = 0;
+                for (int j = 0; j < p.face_size[c]; ++j) {
+                    size_t s = (size_t)c * 6 + j;
+                    double sn = sx[d] * p.face_nx[s] + sy[d] * p.face_ny[s] + sz[d] * p.face_nz[s];
+                    double gf = p.face_area[s] / p.volume[c] * sn;
+                    if (dotf(s) >= 0) {
+                        G += gf;
+                    } else if (snb[s] >= 0) {
+                        if (nin >= kMaxIn) { err_ = "a cell has more in-faces than the sweep supports"; return false; }
+                        rec_q[P0 * kMaxIn + nin] = pos[snb[s]];
+                        rec_c[P0 * kMaxIn + nin] = snb[s];
+                        rec_g[P0 * kMaxIn + nin] = gf;
+                        nin++;
+                    }
+                }
+                rec_G[P0] = G;
             }
         }
-        nlev = (int)lev_c.size();
-        std::vector<int> ic, id;
-        lev_ptr.assign(1, 0);
-        for (int l = 0; l < nlev; ++l) {
-            ic.insert(ic.end(), lev_c[l].begin(), lev_c[l].end());
-            id.insert(id.end(), lev_d[l].begin(), lev_d[l].end());
-            lev_ptr.push_back((int)ic.size());
-        }
-        if (!upload(d_item_c, ic) || !upload(d_item_d, id)) return false;
+        sweep_nchunk = (nB + 31) / 32;
+        sweep_bc = (nB + sweep_nchunk - 1) / sweep_nchunk;
+        sweep_ring = 8192 / sweep_bc;
+        if (!upload(d_item_cell, item_cell) || !upload(d_posc, posc) || !upload(d_levptr, levptr) || !upload(d_dlev, dlev) ||
+            !upload(d_rec_q, rec_q) || !upload(d_rec_c, rec_c) || !upload(d_rec_g, rec_g) || !upload(d_rec_G, rec_G))
+            return false;
     }
     // ---- LSQ gradient stencil (_get_CellMatrix_larger + Get_inverse_2D,
     // StaticBTESynthetic.cpp:1000-1064, utility.cpp:401-466)
@@ -1056,7 +1286,7 @@ bool Solver::build() {
         !alloc(d_TV, nn) || !alloc(d_E, nc) || !alloc(d_qx, nc) || !alloc(d_qy, nc) || !alloc(d_qz, nc) ||
         !alloc(d_qxo, nc) || !alloc(d_qyo, nc) || !alloc(d_qzo, nc) || !alloc(d_RM, nc) || !alloc(d_Tb, (size_t)nB * nc) ||
         !alloc(d_tmac, ne) || !alloc(d_rhs, nc) || !alloc(d_r, nc) || !alloc(d_z, nc) || !alloc(d_p, nc) ||
-        !alloc(d_Ap, nc))
+        !alloc(d_Ap, nc) || !alloc(d_S, nE))
         return false;
     red_blocks = blocks_for(nc, kBlock);
     if (!alloc(d_part, (size_t)red_blocks * 4 + 8) || !alloc(d_part2, (size_t)red_blocks * 4 + 8)) return false;
@@ -1177,9 +1407,13 @@ int Solver::set_initial(const double *T, const double *qx, const double *qy, con
     if (!ok()) return GBTE_HIP_ERROR;
     const double *src[4] = {T, qx, qy, qz};
     double *dst[4] = {d_T.p, d_qx.p, d_qy.p, d_qz.p};
-    for (int k = 0; k < 4; ++k)
-        if (src[k] && !check(hipMemcpy(dst[k], src[k], nc * sizeof(double), hipMemcpyHostToDevice), "H2D"))
+    std::vector<double> tmp(nc);
+    for (int k = 0; k < 4; ++k) {
+        if (!src[k]) continue;
+        for (int c = 0; c < nc; ++c) tmp[cell_new_[c]] = src[k][c];  // deck order -> device order
+        if (!check(hipMemcpy(dst[k], tmp.data(), nc * sizeof(double), hipMemcpyHostToDevice), "H2D"))
             return GBTE_HIP_ERROR;
+    }
     return GBTE_OK;
 }
 
@@ -1203,19 +1437,25 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     DBuf<int> d_be_slot;
     if (!upload(d_be_slot, be_slot)) return GBTE_HIP_ERROR;
     size_t nE = (size_t)nD * nc * nB;
+    Rows R{nD * nB, nB, row_tpr, row_rpb};
+    int rowblk = row_tpr * row_rpb;
+    // row kernels remap blocks per XCD (xcd_block): grids are multiples of 8
+    auto xgrid = [](int n) { return ((n + kXcd - 1) / kXcd) * kXcd; };
+    int gw = xgrid(blocks_for(nc, row_rpb));
+    int bw = xgrid(blocks_for(n_ent, row_rpb));
+    int vw = xgrid(blocks_for(nn, row_rpb));
+    int bbw = blocks_for(blocks_for(n_ent, cpw), kWaves);
     // _set_initial, gradients, outgoing and incoming boundary values
     // (StaticBTESyntheticSolver.cpp:19-62)
-    hipLaunchKernelGGL(k_init_e, dim3(blocks_for((long)nE, kBlock)), dim3(kBlock), 0, s_, d_e.p, d_T.p, d_cap.p, nc, nB, nE);
-    int gw = blocks_for(blocks_for((long)nc * nD, cpw), kWaves);
-    hipLaunchKernelGGL(k_gradient, dim3(gw), dim3(kBlock), 0, s_, d_e.p, d_grad.p, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p,
-                       dim, nnz, invLx, nc, nB, nD, cpw);
-    int bw = blocks_for(blocks_for((long)n_ent * nD, cpw), kWaves);
-    hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(kBlock), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
-                       d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, nc, ne, nB, nD, dim, cpw);
-    int bbw = blocks_for(blocks_for(n_ent, cpw), kWaves);
+    hipLaunchKernelGGL(k_init_e, dim3(blocks_for((long)nE, kBlock)), dim3(kBlock), 0, s_, d_e.p, d_T.p, d_cap.p,
+                       nD * nB, nB, nE);
+    hipLaunchKernelGGL(k_gradient, dim3(gw), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p,
+                       dim, nnz, invLx, nc, R);
+    hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
+                       d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
     hipLaunchKernelGGL(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, s_, d_eb.p, d_tmac.p, m, d_ent_list.p,
                        d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p, d_be_tw.p, d_be_dT.p,
-                       d_be_tov.p, n_ent, ne, nB, nD, cpw);
+                       d_be_tov.p, n_ent, nB, nD, cpw);
     if (!check(hipGetLastError(), "setup kernels")) return GBTE_HIP_ERROR;
     errInc_ = 0;
     errTold_ = 0;
@@ -1226,28 +1466,28 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     double errT = 0, errQ = 0;
     double sweep_ms = 0, transport_ms = 0, macro_ms = 0;
     long cg_total = 0;
-    int fmw = blocks_for(blocks_for(nc, cpw), kWaves);
-    int vw = blocks_for(blocks_for((long)nn * nD, cpw), kWaves);
     for (nt = 0; nt < max_iter; ++nt) {
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
         (void)hipEventRecord(ev0_, s_);
-        for (int l = 0; l < nlev; ++l) {
-            int a = lev_ptr[l], n_items = lev_ptr[l + 1] - a;
-            int blocks = blocks_for(blocks_for(n_items, cpw), kWaves);
-            hipLaunchKernelGGL(k_sweep, dim3(blocks), dim3(kBlock), 0, s_, d_item_c.p + a, d_item_d.p + a, n_items,
-                               d_e.p, d_grad.p, d_eb.p, d_Told.p, d_Q.p, g, m, nc, ne, nB, dim, cpw);
-        }
+        hipLaunchKernelGGL(k_source, dim3(gw), dim3(rowblk), 0, s_, d_posc.p, d_S.p, d_grad.p, d_eb.p, d_Told.p, d_Q.p,
+                           g, m, nc, dim, R);
+        hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
+                           (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
+                           d_rec_q.p, d_rec_c.p, d_rec_g.p, d_rec_G.p, d_S.p, d_e.p, m, nc, nB, nD, sweep_bc,
+                           sweep_nchunk, sweep_ring);
         (void)hipEventRecord(ev1_, s_);
-        hipLaunchKernelGGL(k_vertex_e, dim3(vw), dim3(kBlock), 0, s_, d_e.p, d_eb.p, d_eV.p, d_vptr.p, d_vsrc.p,
-                           d_vdist.p, nn, nc, ne, nB, nD, cpw);
-        hipLaunchKernelGGL(k_face_moments, dim3(fmw), dim3(kBlock), 0, s_, d_e.p, d_eb.p, d_eV.p, d_grad.p, g, fk, m,
-                           d_gptr.p, d_gidx.p, d_gP.p, d_gok.p, nnz, invLx, d_T.p, d_E.p, d_qx.p, d_qy.p, d_qz.p,
-                           d_RM.p, d_Tb.p, nc, ne, nn, nB, nD, dim, cpw);
+        hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
+                           d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
+        hipLaunchKernelGGL(k_vertex_e, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p, d_eV.p, d_vptr.p, d_vsrc.p,
+                           d_vdist.p, nn, R);
+        hipLaunchKernelGGL(k_face_moments, dim3(gw), dim3(rowblk), 0, s_, d_e.p, d_eb.p, d_eV.p, d_grad.p, g, fk, m,
+                           d_gptr.p, d_gidx.p, d_gP.p, d_gok.p, nnz, invLx, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc,
+                           dim, R);
         (void)hipEventRecord(ev2_, s_);
         hipLaunchKernelGGL(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, s_, d_eb.p, d_tmac.p, m,
                            d_ent_list.p, d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p,
-                           d_be_tw.p, d_be_dT.p, d_be_tov.p, n_ent, ne, nB, nD, cpw);
+                           d_be_tw.p, d_be_dT.p, d_be_tov.p, n_ent, nB, nD, cpw);
         if (!check(hipGetLastError(), "iteration kernels")) return GBTE_HIP_ERROR;
         // _marco_solution (StaticBTESynthetic.cpp:2692-2734)
         auto tm0 = std::chrono::high_resolution_clock::now();
@@ -1301,6 +1541,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         if (errT <= tolT && errQ <= tolQ) { conv = true; ++nt; break; }
         if (errInc_ >= 10) { status = GBTE_DIVERGED; err_ = "Error increases for 10 times"; ++nt; break; }
     }
+    hipLaunchKernelGGL(k_output_moments, dim3(gw), dim3(rowblk), 0, s_, d_e.p, m, d_E.p, d_Tb.p, nc, R);
     if (!check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
     auto t1 = std::chrono::high_resolution_clock::now();
     have_state = true;
@@ -1326,11 +1567,18 @@ int Solver::get_fields(double *TL, double *TE, double *qx, double *qy, double *q
     struct {
         double *h;
         const double *d;
-        size_t n;
-    } f[6] = {{TL, d_T.p, (size_t)nc}, {TE, d_E.p, (size_t)nc}, {qx, d_qx.p, (size_t)nc},
-              {qy, d_qy.p, (size_t)nc}, {qz, d_qz.p, (size_t)nc}, {Tb, d_Tb.p, (size_t)nB * nc}};
-    for (auto &x : f)
-        if (x.h && !check(hipMemcpy(x.h, x.d, x.n * sizeof(double), hipMemcpyDeviceToHost), "D2H")) return GBTE_HIP_ERROR;
+        int rows;
+    } f[6] = {{TL, d_T.p, 1}, {TE, d_E.p, 1}, {qx, d_qx.p, 1}, {qy, d_qy.p, 1}, {qz, d_qz.p, 1}, {Tb, d_Tb.p, nB}};
+    std::vector<double> tmp;
+    for (auto &x : f) {
+        if (!x.h) continue;
+        tmp.resize((size_t)x.rows * nc);
+        if (!check(hipMemcpy(tmp.data(), x.d, tmp.size() * sizeof(double), hipMemcpyDeviceToHost), "D2H"))
+            return GBTE_HIP_ERROR;
+        // device order -> deck order
+        for (int r = 0; r < x.rows; ++r)
+            for (int c = 0; c < nc; ++c) x.h[(size_t)r * nc + c] = tmp[(size_t)r * nc + cell_new_[c]];
+    }
     return GBTE_OK;
 }
 

@@ -11,6 +11,9 @@
 
 namespace gbte {
 
+// Cells and nodes renumbered along a Morton curve (reorder.cpp).
+Problem renumber(const Problem &p, std::vector<int> &cell_new, std::vector<int> &node_new);
+
 // Owning device buffer.
 template <typename T>
 struct DBuf {
@@ -46,6 +49,7 @@ class Solver {
     int set_initial(const double *T, const double *qx, const double *qy, const double *qz);
     int solve(int max_iter, double tolT, double tolQ, gbte_stats *st);
     int get_fields(double *TL, double *TE, double *qx, double *qy, double *qz, double *Tb);
+    int max_levels() const { return nlev; }
     int keff(int axis, double dT, double *out);
 
     void set_error(const std::string &e) { err_ = e; }
@@ -65,6 +69,8 @@ class Solver {
     void launch_rhs(bool macro, const double *bval, double *rhs);
     bool inner_error(const double *a, const double *b, double *out);
 
+    std::vector<int> cell_new_, node_new_;  // deck order -> device order
+    Problem Pp;                             // renumbered problem (device order)
     const Problem &P;
     gbte_options opt{};
     std::string err_;
@@ -72,15 +78,18 @@ class Solver {
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev2_ = nullptr;
 
     int nc = 0, nb = 0, ne = 0, nn = 0, nB = 0, nD = 0, nf = 0, dim = 2, cpw = 1, nlev = 0;
+    int row_tpr = 256, row_rpb = 1;  // threads per cell row, rows per block
 
     // mode tables
     DBuf<double> d_vg, d_tau, d_cap, d_lr, d_hr, d_sx, d_sy, d_sz, d_w;
     // cell / slot geometry
     DBuf<int> d_fsize, d_snb, d_sent;
     DBuf<double> d_vol, d_Q, d_snx, d_sny, d_snz, d_sarea, d_srx, d_sry, d_srz, d_sqx, d_sqy, d_sqz;
-    // level schedule
-    DBuf<int> d_item_c, d_item_d;
-    std::vector<int> lev_ptr;
+    // per-direction upwind schedule: positions in upwind order, level
+    // pointers, in-face records (neighbour position, neighbour cell, g)
+    DBuf<int> d_item_cell, d_posc, d_levptr, d_dlev, d_rec_q, d_rec_c;
+    DBuf<double> d_rec_g, d_rec_G;
+    int sweep_bc = 1, sweep_nchunk = 1, sweep_ring = 1;
     // gradient stencil
     DBuf<int> d_gptr, d_gidx, d_gok;
     DBuf<double> d_gP;
@@ -101,7 +110,7 @@ class Solver {
     DevCSR Kmacro, Kfour;
     // state
     DBuf<double> d_e, d_grad, d_eb, d_eV, d_T, d_Told, d_T1, d_TV, d_E, d_qx, d_qy, d_qz, d_qxo, d_qyo, d_qzo,
-        d_RM, d_Tb, d_tmac, d_rhs;
+        d_RM, d_Tb, d_tmac, d_rhs, d_S;
     // CG / reduction scratch
     DBuf<double> d_r, d_z, d_p, d_Ap, d_part, d_part2;
     double *h_pin = nullptr;  // pinned host scratch
