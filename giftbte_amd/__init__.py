@@ -82,13 +82,16 @@ class BTESolver:
     """
 
     def __init__(self, problem: Problem, precision: str = "fp64", device: int = 0, verbose: bool = False,
-                 macro_rtol: float = 0.0):
+                 macro_rtol: float = 0.0, linear_solver: str = "direct"):
         self.problem = problem
         self.lib = problem.lib
         t = problem.tables
         self.nc, self.nb = t.n_cells, t.n_bands
+        if linear_solver not in ("direct", "pcg"):
+            raise ValueError("linear_solver must be 'direct' or 'pcg'")
         opts = _lib.Options(_lib.GBTE_FP64 if precision == "fp64" else _lib.GBTE_FP32_SWEEP, device, int(verbose),
-                            macro_rtol)
+                            macro_rtol,
+                            _lib.GBTE_LINSOLVE_DIRECT if linear_solver == "direct" else _lib.GBTE_LINSOLVE_PCG)
         self._h = C.c_void_p()
         rc = self.lib.gbte_create(problem.handle, C.byref(opts), C.byref(self._h))
         if rc != 0:

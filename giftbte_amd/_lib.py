@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(_HERE, "libgiftbte_hip.so")
 
 GBTE_OK, GBTE_BAD_INPUT, GBTE_DIVERGED, GBTE_NAN, GBTE_HIP_ERROR, GBTE_UNSUPPORTED = range(6)
 GBTE_FP64, GBTE_FP32_SWEEP = 0, 1
+GBTE_LINSOLVE_DIRECT, GBTE_LINSOLVE_PCG = 0, 1
 
 _pi, _pd = C.POINTER(C.c_int), C.POINTER(C.c_double)
 
@@ -71,7 +72,8 @@ class Tables(C.Structure):
 
 
 class Options(C.Structure):
-    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("macro_rtol", C.c_double)]
+    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("macro_rtol", C.c_double),
+                ("linear_solver", C.c_int)]
 
 
 class Stats(C.Structure):

@@ -706,6 +706,108 @@ __global__ void k_cg_update2(const double *__restrict__ part2, double *__restric
     if (blockIdx.x == 0 && threadIdx.x == 0) scal[2] = rr;
 }
 
+// ---- direct solve with the nested-dissection Cholesky factor (ndchol.cpp)
+// Forward: one workgroup per supernode of one elimination-tree height.  The
+// front f = [b(cols); 0(rows)] gets the children's update vectors added
+// (extend-add), then y = inv(L_kk) f_cols and u = f_rows - L_Ak y.
+// Backward: one workgroup per supernode of one depth:
+// x_cols = inv(L_kk)^T (y - L_Ak^T x_rows).  Reductions run in a fixed order.
+struct NDDev {
+    const int *perm, *col0, *ncol, *nrow, *child_ptr, *child_idx, *map, *map_off, *rows, *rows_off, *u_off;
+    const long *linv_off, *la_off;
+    const double *linv, *la;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_nd_forward(const int *__restrict__ nodes, NDDev F,
+                                                       const double *__restrict__ b, double *__restrict__ y,
+                                                       double *__restrict__ u, int max_front) {
+    extern __shared__ double sm[];
+    double *f = sm, *ys = sm + max_front;
+    int k = nodes[blockIdx.x];
+    int nc = F.ncol[k], nr = F.nrow[k], m = nc + nr, c0 = F.col0[k];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) f[i] = i < nc ? b[F.perm[c0 + i]] : 0.0;
+    __syncthreads();
+    for (int q = F.child_ptr[k]; q < F.child_ptr[k + 1]; ++q) {
+        int c = F.child_idx[q];
+        const int *mp = F.map + F.map_off[c];
+        const double *uc = u + F.u_off[c];
+        for (int i = threadIdx.x; i < F.nrow[c]; i += blockDim.x) f[mp[i]] += uc[i];
+        __syncthreads();
+    }
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const double *Li = F.linv + F.linv_off[k];
+    for (int i = w; i < nc; i += nw) {
+        const double *row = Li + (size_t)i * nc;
+        double s = 0;
+        for (int a = lane; a <= i; a += 64) s += row[a] * f[a];
+        s = wave_sum(s);
+        if (lane == 0) {
+            ys[i] = s;
+            y[c0 + i] = s;
+        }
+    }
+    __syncthreads();
+    const double *La = F.la + F.la_off[k];
+    double *uk = u + F.u_off[k];
+    for (int r = w; r < nr; r += nw) {
+        const double *row = La + (size_t)r * nc;
+        double s = 0;
+        for (int a = lane; a < nc; a += 64) s += row[a] * ys[a];
+        s = wave_sum(s);
+        if (lane == 0) uk[r] = f[nc + r] - s;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_nd_backward(const int *__restrict__ nodes, NDDev F,
+                                                        const double *__restrict__ y, double *__restrict__ xp,
+                                                        double *__restrict__ out, int max_front) {
+    extern __shared__ double sm[];
+    __shared__ double red[kWaves][64];
+    int k = nodes[blockIdx.x];
+    int nc = F.ncol[k], nr = F.nrow[k], c0 = F.col0[k];
+    double *xa = sm, *z = sm + max_front;
+    const int *rw = F.rows + F.rows_off[k];
+    for (int r = threadIdx.x; r < nr; r += blockDim.x) xa[r] = xp[rw[r]];
+    __syncthreads();
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const double *La = F.la + F.la_off[k];
+    for (int a0 = 0; a0 < nc; a0 += 64) {
+        int a = a0 + lane;
+        double s = 0;
+        if (a < nc)
+            for (int r = w; r < nr; r += nw) s += La[(size_t)r * nc + a] * xa[r];
+        red[w][lane] = s;
+        __syncthreads();
+        if (w == 0 && a < nc) {
+            double t = y[c0 + a];
+            for (int ww = 0; ww < nw; ++ww) t -= red[ww][lane];
+            z[a] = t;
+        }
+        __syncthreads();
+    }
+    const double *Li = F.linv + F.linv_off[k];
+    for (int a0 = 0; a0 < nc; a0 += 64) {
+        int a = a0 + lane;
+        double s = 0;
+        if (a < nc)
+            for (int i = a0 + w; i < nc; i += nw) s += Li[(size_t)i * nc + a] * z[i];  // zero above the diagonal
+        red[w][lane] = s;
+        __syncthreads();
+        if (w == 0 && a < nc) {
+            double t = 0;
+            for (int ww = 0; ww < nw; ++ww) t += red[ww][lane];
+            xp[c0 + a] = t;
+            out[F.perm[c0 + a]] = t;
+        }
+        __syncthreads();
+    }
+}
+
 // partial sums for the outer error check and the macro inner error
 __global__ void k_err(const double *__restrict__ T, const double *__restrict__ To, const double *__restrict__ qx,
                       const double *__restrict__ qy, const double *__restrict__ qz, const double *__restrict__ qxo,
@@ -839,10 +941,11 @@ Solver::~Solver() {
     if (s_) (void)hipStreamDestroy(s_);
 }
 
-bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m) {
+bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m, DevND &f) {
     int n = (int)rows.size();
     std::vector<int> ptr(n + 1, 0), col;
     std::vector<double> val, dinv(n, 0.0);
+    std::vector<std::vector<std::pair<int, double>>> merged(n);
     for (int i = 0; i < n; ++i) {
         std::vector<std::pair<int, double>> r = rows[i];
         std::sort(r.begin(), r.end(), [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; });
@@ -855,6 +958,7 @@ bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>>
         for (auto &e : mr) {
             col.push_back(e.first);
             val.push_back(-e.second);
+            merged[i].push_back({e.first, -e.second});
             if (e.first == i) dinv[i] = 1.0 / (-e.second);
         }
         ptr[i + 1] = (int)col.size();
@@ -883,7 +987,41 @@ bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>>
             }
         }
     m.n = n;
-    return upload(m.ptr, ptr) && upload(m.col, col) && upload(m.val, val) && upload(m.dinv, dinv);
+    if (!upload(m.ptr, ptr) || !upload(m.col, col) || !upload(m.val, val) || !upload(m.dinv, dinv)) return false;
+    if (opt.linear_solver != GBTE_LINSOLVE_DIRECT) return true;
+    NDFactor F;
+    std::string e = nd_factor(merged, P.cx, P.cy, P.cz, dim, F);
+    if (!e.empty()) { err_ = e; return false; }
+    // LDS of the solve kernels: two vectors of the largest front
+    if ((size_t)F.max_front * 2 * sizeof(double) > 60 * 1024) {
+        err_ = "nested-dissection front too large for the direct solve (use the PCG linear solver)";
+        return false;
+    }
+    int nt = (int)F.ncol.size();
+    std::vector<int> rows_off(nt + 1), u_off(nt + 1), map_off(F.map_off.begin(), F.map_off.end());
+    for (int k = 0; k <= nt; ++k) { rows_off[k] = (int)F.rows_off[k]; u_off[k] = (int)F.u_off[k]; }
+    int hmax = 0, dmax = 0;
+    for (int k = 0; k < nt; ++k) { hmax = std::max(hmax, F.height[k]); dmax = std::max(dmax, F.depth[k]); }
+    std::vector<int> fwd, bwd;
+    f.fwd_ptr.assign(1, 0);
+    for (int h = 0; h <= hmax; ++h) {
+        for (int k = 0; k < nt; ++k) if (F.height[k] == h) fwd.push_back(k);
+        f.fwd_ptr.push_back((int)fwd.size());
+    }
+    f.bwd_ptr.assign(1, 0);
+    for (int d = 0; d <= dmax; ++d) {
+        for (int k = 0; k < nt; ++k) if (F.depth[k] == d) bwd.push_back(k);
+        f.bwd_ptr.push_back((int)bwd.size());
+    }
+    f.n = n;
+    f.max_front = std::max(F.max_front, 1);
+    f.nodes = nt;
+    f.u_total = F.u_total;
+    return upload(f.perm, F.perm) && upload(f.col0, F.col0) && upload(f.ncol, F.ncol) && upload(f.nrow, F.nrow) &&
+           upload(f.child_ptr, F.child_ptr) && upload(f.child_idx, F.child_idx) && upload(f.map, F.map) &&
+           upload(f.map_off, map_off) && upload(f.rows, F.rows) && upload(f.rows_off, rows_off) &&
+           upload(f.u_off, u_off) && upload(f.linv_off, F.linv_off) && upload(f.la_off, F.la_off) &&
+           upload(f.linv, F.linv) && upload(f.la, F.la) && upload(f.fwd_nodes, fwd) && upload(f.bwd_nodes, bwd);
 }
 
 bool Solver::build() {
@@ -1277,7 +1415,7 @@ bool Solver::build() {
                 return false;
         }
         if (!upload(d_fourier_bval, fbval)) return false;
-        if (!build_matrix(rowsM, Kmacro) || !build_matrix(rowsF, Kfour)) return false;
+        if (!build_matrix(rowsM, Kmacro, NDmacro) || !build_matrix(rowsF, Kfour, NDfour)) return false;
     }
     // ---- state
     size_t nE = (size_t)nD * nc * nB;
@@ -1287,6 +1425,9 @@ bool Solver::build() {
         !alloc(d_qxo, nc) || !alloc(d_qyo, nc) || !alloc(d_qzo, nc) || !alloc(d_RM, nc) || !alloc(d_Tb, (size_t)nB * nc) ||
         !alloc(d_tmac, ne) || !alloc(d_rhs, nc) || !alloc(d_r, nc) || !alloc(d_z, nc) || !alloc(d_p, nc) ||
         !alloc(d_Ap, nc) || !alloc(d_S, nE))
+        return false;
+    if (opt.linear_solver == GBTE_LINSOLVE_DIRECT &&
+        (!alloc(d_ndy, nc) || !alloc(d_ndx, nc) || !alloc(d_ndu, (size_t)std::max<long>(1, std::max(NDmacro.u_total, NDfour.u_total)))))
         return false;
     red_blocks = blocks_for(nc, kBlock);
     if (!alloc(d_part, (size_t)red_blocks * 4 + 8) || !alloc(d_part2, (size_t)red_blocks * 4 + 8)) return false;
@@ -1332,6 +1473,33 @@ bool Solver::inner_error(const double *a, const double *b, double *out) {
     for (double v : h) s += v;
     *out = s / nc;
     return true;
+}
+
+bool Solver::linsolve(const DevCSR &K, const DevND &f, double *d_b, double *d_x, int *iters) {
+    if (opt.linear_solver == GBTE_LINSOLVE_DIRECT) {
+        if (iters) *iters = 0;
+        return nd_solve(f, d_b, d_x);
+    }
+    return cg(K, d_b, d_x, iters);
+}
+
+bool Solver::nd_solve(const DevND &f, const double *d_b, double *d_x) {
+    NDDev F{f.perm.p, f.col0.p, f.ncol.p, f.nrow.p, f.child_ptr.p, f.child_idx.p, f.map.p, f.map_off.p,
+            f.rows.p, f.rows_off.p, f.u_off.p, f.linv_off.p, f.la_off.p, f.linv.p, f.la.p};
+    size_t lds = (size_t)f.max_front * 2 * sizeof(double);
+    for (size_t h = 0; h + 1 < f.fwd_ptr.size(); ++h) {
+        int cnt = f.fwd_ptr[h + 1] - f.fwd_ptr[h];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(k_nd_forward, dim3(cnt), dim3(kBlock), lds, s_, f.fwd_nodes.p + f.fwd_ptr[h], F, d_b,
+                           d_ndy.p, d_ndu.p, f.max_front);
+    }
+    for (size_t d = 0; d + 1 < f.bwd_ptr.size(); ++d) {
+        int cnt = f.bwd_ptr[d + 1] - f.bwd_ptr[d];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(k_nd_backward, dim3(cnt), dim3(kBlock), lds, s_, f.bwd_nodes.p + f.bwd_ptr[d], F, d_ndy.p,
+                           d_ndx.p, d_x, f.max_front);
+    }
+    return check(hipGetLastError(), "nested-dissection solve");
 }
 
 bool Solver::cg(const DevCSR &K, double *d_b, double *d_x, int *iters) {
@@ -1389,7 +1557,7 @@ int Solver::fourier(gbte_stats *st) {
         hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
         launch_vertex_T(d_T.p, d_TV.p);
         launch_rhs(false, d_fourier_bval.p, d_rhs.p);
-        if (!cg(Kfour, d_rhs.p, d_T.p, nullptr)) return GBTE_HIP_ERROR;
+        if (!linsolve(Kfour, NDfour, d_rhs.p, d_T.p, nullptr)) return GBTE_HIP_ERROR;
         hipLaunchKernelGGL(k_fourier_flux, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_gptr.p, d_gidx.p, d_gP.p,
                            d_gok.p, d_nint.p, P.nbr_ptr[nc], 1.0 / P.deck.Lx, P.kappa_bulk[0], d_qx.p, d_qy.p, d_qz.p,
                            nc, dim);
@@ -1498,7 +1666,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             launch_vertex_T(d_T.p, d_TV.p);
             launch_rhs(true, d_tmac.p, d_rhs.p);
             int cgit = 0;
-            if (!cg(Kmacro, d_rhs.p, d_T.p, &cgit)) return GBTE_HIP_ERROR;
+            if (!linsolve(Kmacro, NDmacro, d_rhs.p, d_T.p, &cgit)) return GBTE_HIP_ERROR;
             cg_total += cgit;
             if (!inner_error(d_T1.p, d_T.p, &inner)) return GBTE_HIP_ERROR;
             if (inner > errInner_) inc++;

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "giftbte.h"
+#include "ndchol.hpp"
 #include "problem.hpp"
 
 namespace gbte {
@@ -38,6 +39,18 @@ struct DevCSR {
     DBuf<double> val, dinv;
 };
 
+// Nested-dissection Cholesky factor of the same matrix (ndchol.cpp) and the
+// supernodes of each elimination-tree height (forward) and depth (backward).
+struct DevND {
+    int n = 0, max_front = 0, nodes = 0;
+    DBuf<int> perm, col0, ncol, nrow, child_ptr, child_idx, map, map_off, rows, rows_off, u_off;
+    DBuf<long> linv_off, la_off;
+    DBuf<double> linv, la;
+    std::vector<int> fwd_ptr, bwd_ptr;  // level pointers into fwd_nodes / bwd_nodes
+    DBuf<int> fwd_nodes, bwd_nodes;
+    long u_total = 0;
+};
+
 class Solver {
    public:
     Solver(const Problem &p, const gbte_options &o);
@@ -61,9 +74,12 @@ class Solver {
     bool alloc(DBuf<T> &b, size_t n);
     bool check(hipError_t e, const char *what);
     bool build();
-    bool build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m);
-    // macro / Fourier linear solve of (-K) x = -rhs, warm-started from x
+    bool build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m, DevND &f);
+    // macro / Fourier linear solve of (-K) x = -rhs: the direct factor solve,
+    // or PCG warm-started from x
+    bool linsolve(const DevCSR &K, const DevND &f, double *d_rhs, double *d_x, int *iters);
     bool cg(const DevCSR &K, double *d_rhs, double *d_x, int *iters);
+    bool nd_solve(const DevND &f, const double *d_rhs, double *d_x);
     bool reduce_host(const double *d_partials, int n, std::vector<double> &out, int stride);
     void launch_vertex_T(const double *T, double *TV);
     void launch_rhs(bool macro, const double *bval, double *rhs);
@@ -108,6 +124,8 @@ class Solver {
     DBuf<double> d_mbcoef[2], d_mkA[2], d_ma2x[2], d_ma2y[2], d_mmx[2], d_mmy[2], d_mm2[2];
     DBuf<double> d_fourier_bval;
     DevCSR Kmacro, Kfour;
+    DevND NDmacro, NDfour;
+    DBuf<double> d_ndy, d_ndu, d_ndx;
     // state
     DBuf<double> d_e, d_grad, d_eb, d_eV, d_T, d_Told, d_T1, d_TV, d_E, d_qx, d_qy, d_qz, d_qxo, d_qyo, d_qzo,
         d_RM, d_Tb, d_tmac, d_rhs, d_S;

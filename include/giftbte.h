@@ -38,6 +38,13 @@ enum {
 /* Precision of the transport sweep. */
 enum { GBTE_FP64 = 0, GBTE_FP32_SWEEP = 1 };
 
+/* Linear solver of the synthetic (macro) and Fourier systems, which the
+ * reference solves with Eigen::SparseLU (StaticBTESynthetic.cpp:2692-2734,
+ * StaticFourier.cpp:934-976): an exact nested-dissection Cholesky factor
+ * (factored once at gbte_create, triangular solves on the GPU), or
+ * Jacobi-preconditioned CG to macro_rtol. */
+enum { GBTE_LINSOLVE_DIRECT = 0, GBTE_LINSOLVE_PCG = 1 };
+
 /* Read-only view of the flattened problem tables, laid out exactly as the
  * reference's StaticBTESynthetic constructor flattens BTEMesh/BTEBand/BTEAngle
  * (src/StaticBTESynthetic/StaticBTESynthetic.cpp:224-626): 6 face slots per
@@ -90,7 +97,8 @@ typedef struct gbte_options {
     int precision;        /* GBTE_FP64 or GBTE_FP32_SWEEP                       */
     int device;           /* HIP device ordinal                                 */
     int verbose;          /* print the reference's per-iteration log lines      */
-    double macro_rtol;    /* relative residual of the FP64 synthetic solve (0: default) */
+    double macro_rtol;    /* relative residual of the PCG linear solver (0: default) */
+    int linear_solver;    /* GBTE_LINSOLVE_DIRECT (default) or GBTE_LINSOLVE_PCG */
 } gbte_options;
 
 typedef struct gbte_stats {

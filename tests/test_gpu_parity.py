@@ -16,21 +16,24 @@ FIELD_TOL = 1e-9
 KEFF_TOL = 1e-8
 
 
-def _gpu_run(deck_dir, s):
+def _gpu_run(deck_dir, s, linear_solver="direct"):
     from giftbte_amd import Problem, BTESolver
     p = Problem(deck_dir)
-    sol = BTESolver(p)
+    sol = BTESolver(p, linear_solver=linear_solver)
     fp = sol.fourier()
     st = sol.solve(s["max_iter"], s["tol_temp"], s["tol_flux"])
     return p, sol, fp, st
 
 
-@pytest.mark.parametrize("name", list(decklib.DECKS))
-def test_fields_match_oracle(name, deck_root):
+CASES = [(n, "direct") for n in decklib.DECKS] + [("porous_b4", "pcg")]
+
+
+@pytest.mark.parametrize("name,linear_solver", CASES)
+def test_fields_match_oracle(name, linear_solver, deck_root):
     d = decklib.make(name, deck_root)
     s = decklib.settings(name)
-    print(f"\n[gpu parity] deck={name} settings={s}", flush=True)
-    p, sol, fp, st = _gpu_run(d, s)
+    print(f"\n[gpu parity] deck={name} linear_solver={linear_solver} settings={s}", flush=True)
+    p, sol, fp, st = _gpu_run(d, s, linear_solver)
     ref, info = _oracle.run(p.tables, s["max_iter"], s["tol_temp"], s["tol_flux"])
     assert st.fourier_passes == info["fourier_passes"]
     assert st.iterations == info["iterations"]
