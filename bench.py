@@ -70,12 +70,25 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--converge", action="store_true", help="also time a full solve to the deck tolerance")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device once (calibrates HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole problem on a single MI355X "
               "(no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     from giftbte_amd import Problem, BTESolver
+
+    if a.copy_probe > 0:
+        import torch
+        n = a.copy_probe * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {a.copy_probe} MiB read + {a.copy_probe} MiB written", file=sys.stderr, flush=True)
+        del x, y
 
     with tempfile.TemporaryDirectory() as td:
         t_gen = time.time()

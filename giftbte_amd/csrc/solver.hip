@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <numeric>
 
 namespace gbte {
 
@@ -89,6 +90,7 @@ __device__ __forceinline__ int xcd_block() {
 
 struct Modes {
     const double *vg, *tau, *cap, *lr, *hr, *sx, *sy, *sz, *w;
+    const double *lrc;  // lr / cap per band
     double kappa, cbulk;
 };
 
@@ -99,7 +101,33 @@ struct SlotGeo {
 
 struct Rows {
     int DB, B, tpr, rpb;  // row length, bands, threads per row, rows per block
+    int dstep, bstep;     // tpr = dstep * B + bstep: (dir, band) advance per stride
 };
+
+// (direction, band) of row entry t0 and its advance by tpr entries, without
+// an integer division per entry.
+struct DirBand {
+    int d, b;
+    __device__ __forceinline__ DirBand(int t0, int nB) : d(t0 / nB), b(t0 - (t0 / nB) * nB) {}
+    __device__ __forceinline__ void next(const Rows &R) {
+        d += R.dstep;
+        b += R.bstep;
+        if (b >= R.B) { b -= R.B; ++d; }
+    }
+};
+
+// Row and lane of this thread: one row per block (uniform row index, so
+// per-row data lives in scalar registers) or rpb rows of tpr lanes.
+template <bool OneRow>
+__device__ __forceinline__ void row_lane(const Rows &R, int &row, int &t0) {
+    if (OneRow) {
+        row = xcd_block();
+        t0 = threadIdx.x;
+    } else {
+        row = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+        t0 = threadIdx.x % R.tpr;
+    }
+}
 
 // ---------------------------------------------------------------- kernels
 
@@ -198,63 +226,68 @@ __global__ void __launch_bounds__(kRowBlockMax) k_bound_out(const double *__rest
 //       - sum_in,boundary a eb,      a = tau (A/V) (v.n),
 // i.e. everything of _get_Re (StaticBTESynthetic.cpp:1427-1500) that does not
 // depend on the new upwind values.  One row per cell; written to each
-// direction's upwind order for the sweep.
+// direction's upwind order for the sweep.  2D cells (at most 4 faces).
+constexpr int kMaxFace2D = 4;
+
+template <bool OneRow>
 __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__ posc, double *__restrict__ S,
                                                          const double *__restrict__ grad, const double *__restrict__ eb,
                                                          const double *__restrict__ Told, const double *__restrict__ Q,
-                                                         SlotGeo g, Modes m, int nc, int dim, Rows R) {
-    int c = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+                                                         SlotGeo g, Modes m, int nc, Rows R) {
+    int c, t0;
+    row_lane<OneRow>(R, c, t0);
     if (c >= nc) return;
     int DB = R.DB, nB = R.B, nD = DB / nB;
     int nf = g.fsize[c];
     double vol = g.vol[c], To = Told[c], Qc = Q[c];
-    const double *gc = grad + (size_t)c * dim * DB;
-    for (int tb = threadIdx.x % R.tpr; tb < DB; tb += kNTf * R.tpr) {
-        double s[kNTf], gcx[kNTf], gcy[kNTf], gcz[kNTf];
+    const double *gc = grad + (size_t)c * 2 * DB;
+    const int *pc = posc + (size_t)c * nD;
+    // faces: normal, (A/V), r from this centre, r' from the neighbour's centre
+    double fnx[kMaxFace2D], fny[kMaxFace2D], fav[kMaxFace2D], frx[kMaxFace2D], fry[kMaxFace2D], fqx[kMaxFace2D],
+        fqy[kMaxFace2D];
+    const double *fin[kMaxFace2D];  // neighbour gradient row, or boundary-value row
+    int fnb[kMaxFace2D];
 #pragma unroll
-        for (int j = 0; j < kNTf; ++j) {
-            int t = tb + j * R.tpr;
-            s[j] = 0; gcx[j] = 0; gcy[j] = 0; gcz[j] = 0;
-            if (t < DB) {
-                int b = t % nB;
-                s[j] = To * m.cap[b] + Qc * m.hr[b] * m.tau[b];
-                gcx[j] = gc[t];
-                gcy[j] = gc[DB + t];
-                gcz[j] = dim == 3 ? gc[2 * DB + t] : 0.0;
+    for (int f = 0; f < kMaxFace2D; ++f) {
+        int k = c * 6 + f;
+        bool on = f < nf;
+        fnx[f] = on ? g.nx[k] : 0.0;
+        fny[f] = on ? g.ny[k] : 0.0;
+        fav[f] = on ? g.area[k] / vol : 0.0;
+        frx[f] = on ? g.rx[k] : 0.0;
+        fry[f] = on ? g.ry[k] : 0.0;
+        fqx[f] = on ? g.qx[k] : 0.0;
+        fqy[f] = on ? g.qy[k] : 0.0;
+        int nb = on ? g.nb[k] : 0;
+        fnb[f] = nb;
+        fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)g.ent[k] * DB;
+    }
+    DirBand db(t0, nB);
+    double vb = m.vg[db.b], tau = m.tau[db.b], s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
+    for (int t = t0; t < DB; t += R.tpr) {
+        if (R.bstep != 0) {
+            vb = m.vg[db.b];
+            tau = m.tau[db.b];
+            s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
+        }
+        double sx = m.sx[db.d], sy = m.sy[db.d];
+        double gcx = gc[t], gcy = gc[DB + t];
+        double s = s0;
+#pragma unroll
+        for (int f = 0; f < kMaxFace2D; ++f) {
+            if (f >= nf) break;
+            double dot = vb * sx * fnx[f] + vb * sy * fny[f];
+            double a = tau * fav[f] * dot;
+            if (dot >= 0) {
+                s -= a * (frx[f] * gcx + fry[f] * gcy);
+            } else if (fnb[f] >= 0) {
+                s -= a * (fqx[f] * fin[f][t] + fqy[f] * fin[f][DB + t]);
+            } else {
+                s -= a * fin[f][t];
             }
         }
-        for (int f = 0; f < nf; ++f) {
-            int k = c * 6 + f;
-            double nx = g.nx[k], ny = g.ny[k], nz = g.nz[k], av = g.area[k] / vol;
-            double rx = g.rx[k], ry = g.ry[k], rz = g.rz[k], qx = g.qx[k], qy = g.qy[k], qz = g.qz[k];
-            int nb = g.nb[k];
-            const double *gn = grad + (size_t)(nb >= 0 ? nb : 0) * dim * DB;
-            const double *ebr = eb + (size_t)(nb >= 0 ? 0 : g.ent[k]) * DB;
-#pragma unroll
-            for (int j = 0; j < kNTf; ++j) {
-                int t = tb + j * R.tpr;
-                if (t >= DB) continue;
-                int d = t / nB, b = t - d * nB;
-                double vb = m.vg[b], tb_ = m.tau[b];
-                double dot = vb * m.sx[d] * nx + vb * m.sy[d] * ny + vb * m.sz[d] * nz;
-                double a = tb_ * av * dot;
-                if (dot >= 0) {
-                    s[j] -= a * (rx * gcx[j] + ry * gcy[j] + rz * gcz[j]);
-                } else if (nb >= 0) {
-                    double gnz = dim == 3 ? gn[2 * DB + t] : 0.0;
-                    s[j] -= a * (qx * gn[t] + qy * gn[DB + t] + qz * gnz);
-                } else {
-                    s[j] -= a * ebr[t];
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kNTf; ++j) {
-            int t = tb + j * R.tpr;
-            if (t >= DB) continue;
-            int d = t / nB, b = t - d * nB;
-            S[((size_t)d * nc + posc[(size_t)c * nD + d]) * nB + b] = s[j];
-        }
+        S[((size_t)db.d * nc + pc[db.d]) * nB + db.b] = s;
+        db.next(R);
     }
 }
 
@@ -326,36 +359,37 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_dir(
 
 // Inverse-distance vertex values (_set_vertex_energydensity,
 // StaticBTESynthetic.cpp:1541-1616): interior nodes from cells, nodes on a
-// boundary face from the boundary values of those faces.  One row per node.
+// boundary face from the boundary values of those faces.  One row per node;
+// vwt holds the weights (1/d) / sum(1/d).
+template <bool OneRow>
 __global__ void __launch_bounds__(kRowBlockMax) k_vertex_e(const double *__restrict__ e,
                                                            const double *__restrict__ eb, double *__restrict__ eV,
                                                            const int *__restrict__ vptr, const int *__restrict__ vsrc,
-                                                           const double *__restrict__ vdist, int nn, Rows R) {
-    int v = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+                                                           const double *__restrict__ vwt, int nn, Rows R) {
+    int v, t0;
+    row_lane<OneRow>(R, v, t0);
     if (v >= nn) return;
     int a = vptr[v], z = vptr[v + 1];
     int DB = R.DB;
-    for (int tb = threadIdx.x % R.tpr; tb < DB; tb += kNT * R.tpr) {
+    for (int tb = t0; tb < DB; tb += kNT * R.tpr) {
         double sm[kNT];
 #pragma unroll
         for (int j = 0; j < kNT; ++j) sm[j] = 0;
-        double r = 0;
         for (int k = a; k < z; ++k) {
             int src = vsrc[k];
             const double *row = src >= 0 ? e + (size_t)src * DB : eb + (size_t)(-src - 1) * DB;
-            double dk = vdist[k];
+            double wk = vwt[k];
 #pragma unroll
             for (int j = 0; j < kNT; ++j) {
                 int t = tb + j * R.tpr;
-                if (t < DB) sm[j] += row[t] / dk;
+                if (t < DB) sm[j] += row[t] * wk;
             }
-            r += 1.0 / dk;
         }
 #pragma unroll
         for (int j = 0; j < kNT; ++j) {
             int t = tb + j * R.tpr;
             if (t >= DB) continue;
-            double x = sm[j] / r;
+            double x = sm[j];
             if (isnan(x)) x = 0;
             eV[(size_t)v * DB + t] = x;
         }
@@ -364,7 +398,7 @@ __global__ void __launch_bounds__(kRowBlockMax) k_vertex_e(const double *__restr
 
 struct FaceK {
     const int *kind, *other, *v0, *v1;
-    const double *an, *aa, *n0x, *n0y, *n0z, *mx, *my, *m2;
+    const double *ga, *n0x, *n0y, *mxm, *mym;  // an/|a|^2, face normal n0, m/|m|^2
 };
 
 // One row per cell, over every (direction, band):
@@ -372,73 +406,100 @@ struct FaceK {
 //   the non-Fourier residual (_get_non_fourier, :1883-1937),
 //   the lattice-temperature and heat-flux moments (_recover_temperature,
 //   _get_heat_flux, :2485-2520), reduced over the row,
-//   and the LSQ gradient of the new e for the next transport step.
+//   and the LSQ gradient of the new e for the next transport step
+//   (_get_gradient_larger, Order 2, :1243-1340).
 // kind: 0 none (specular), 1 interior face owned first by this cell,
 //       3 interior face owned first by the neighbour, 2 boundary face.
+template <bool OneRow>
 __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
     const double *__restrict__ e, const double *__restrict__ eb, const double *__restrict__ eV,
     double *__restrict__ grad, SlotGeo g, FaceK f, Modes m, const int *__restrict__ gptr, const int *__restrict__ gidx,
     const double *__restrict__ gP, const int *__restrict__ gok, int nnz, double invLx, double *__restrict__ T,
     double *__restrict__ qx, double *__restrict__ qy, double *__restrict__ qz, double *__restrict__ RM, int nc,
-    int dim, Rows R) {
+    Rows R) {
     __shared__ double sh[kRowBlockMax / 64];
-    int c = xcd_block() * R.rpb + threadIdx.x / R.tpr;
+    int c, t0;
+    row_lane<OneRow>(R, c, t0);
     bool act = c < nc;
     int cc = act ? c : nc - 1;
     int DB = R.DB, nB = R.B;
     const double *ec = e + (size_t)cc * DB;
     int nf = g.fsize[cc];
-    int t0 = threadIdx.x % R.tpr;
-    double aT = 0, ax = 0, ay = 0, az = 0, aR = 0;
-    for (int tb = t0; tb < DB; tb += kNTf * R.tpr) {
-        double e0[kNTf];
+    int fk[kMaxFace2D];
+    const double *fo[kMaxFace2D], *f0[kMaxFace2D], *f1[kMaxFace2D];
+    double fga[kMaxFace2D], fn0x[kMaxFace2D], fn0y[kMaxFace2D], fmx[kMaxFace2D], fmy[kMaxFace2D], fnx[kMaxFace2D],
+        fny[kMaxFace2D], fA[kMaxFace2D];
 #pragma unroll
-        for (int j = 0; j < kNTf; ++j) {
-            int t = tb + j * R.tpr;
-            e0[j] = 0;
-            if (t < DB) {
-                int d = t / nB, b = t - d * nB;
-                double vb = m.vg[b], wd = m.w[d];
-                double x = ec[t];
-                e0[j] = x;
-                aT += m.lr[b] * x * wd / m.cap[b];
-                ax += vb * m.sx[d] * wd * x;
-                ay += vb * m.sy[d] * wd * x;
-                az += vb * m.sz[d] * wd * x;
-            }
-        }
-        for (int fi = 0; fi < nf; ++fi) {
-            int s = cc * 6 + fi;
-            int kd = f.kind[s];
-            if (kd == 0) continue;
-            int oth = f.other[s];
-            const double *orow = (kd == 2 ? eb : e) + (size_t)oth * DB;
-            const double *v0 = eV + (size_t)f.v0[s] * DB, *v1 = eV + (size_t)f.v1[s] * DB;
-            double an = f.an[s], aa = f.aa[s], n0x = f.n0x[s], n0y = f.n0y[s], n0z = f.n0z[s];
-            double mx = f.mx[s], my = f.my[s], m2 = f.m2[s];
-            double nx = g.nx[s], ny = g.ny[s], nz = g.nz[s], A = g.area[s];
-#pragma unroll
-            for (int j = 0; j < kNTf; ++j) {
-                int t = tb + j * R.tpr;
-                if (t >= DB) continue;
-                int d = t / nB, b = t - d * nB;
-                if (!(m.hr[b] > 0)) continue;
-                double vb = m.vg[b], tb_ = m.tau[b], wd = m.w[d];
-                double vx = vb * m.sx[d], vy = vb * m.sy[d], vz = vb * m.sz[d];
-                double o = orow[t];
-                double de = kd == 3 ? e0[j] - o : o - e0[j];
-                double gn = de * an / aa;
-                double dv = v1[t] - v0[t];
-                double gx = dv / m2 * mx, gy = dv / m2 * my, gz = 0;
-                gx = gn * n0x + gx;
-                gy = gn * n0y + gy;
-                gz = gn * n0z + gz;
-                aR -= wd * A * tb_ * (vx * gx + vy * gy + vz * gz) * (vx * nx + vy * ny + vz * nz);
-                aR += m.kappa * wd * A * m.lr[b] / m.cap[b] * (nx * gx + ny * gy + nz * gz);
-            }
-        }
+    for (int i = 0; i < kMaxFace2D; ++i) {
+        int s = cc * 6 + i;
+        int kd = i < nf ? f.kind[s] : 0;
+        fk[i] = kd;
+        bool on = kd != 0;
+        fo[i] = on ? (kd == 2 ? eb : e) + (size_t)f.other[s] * DB : ec;
+        f0[i] = on ? eV + (size_t)f.v0[s] * DB : ec;
+        f1[i] = on ? eV + (size_t)f.v1[s] * DB : ec;
+        fga[i] = on ? f.ga[s] : 0.0;
+        fn0x[i] = on ? f.n0x[s] : 0.0;
+        fn0y[i] = on ? f.n0y[s] : 0.0;
+        fmx[i] = on ? f.mxm[s] : 0.0;
+        fmy[i] = on ? f.mym[s] : 0.0;
+        fnx[i] = on ? g.nx[s] : 0.0;
+        fny[i] = on ? g.ny[s] : 0.0;
+        fA[i] = on ? g.area[s] : 0.0;
     }
-    if (act) row_gradient(e, grad, cc, gptr, gidx, gP, gok, dim, nnz, invLx, t0, R.tpr, DB);
+    int ka = gptr[cc], kz = gptr[cc + 1];
+    bool ok = gok[cc] != 0;
+    double *gcx = grad + (size_t)cc * 2 * DB, *gcy = gcx + DB;
+    DirBand db(t0, nB);
+    double vb = m.vg[db.b], tau = m.tau[db.b], lrc = m.lrc[db.b];
+    bool hr = m.hr[db.b] > 0;
+    double aT = 0, ax = 0, ay = 0, az = 0, aR = 0;
+    for (int t = t0; t < DB; t += R.tpr) {
+        if (R.bstep != 0) {
+            vb = m.vg[db.b];
+            tau = m.tau[db.b];
+            lrc = m.lrc[db.b];
+            hr = m.hr[db.b] > 0;
+        }
+        double wd = m.w[db.d];
+        double vx = vb * m.sx[db.d], vy = vb * m.sy[db.d], vz = vb * m.sz[db.d];
+        double x = ec[t];
+        aT += lrc * x * wd;
+        ax += vx * wd * x;
+        ay += vy * wd * x;
+        az += vz * wd * x;
+        if (hr) {
+            double kl = m.kappa * lrc, acc = 0;
+#pragma unroll
+            for (int i = 0; i < kMaxFace2D; ++i) {
+                if (fk[i] == 0) continue;
+                double o = fo[i][t];
+                double de = fk[i] == 3 ? x - o : o - x;
+                double gn = de * fga[i];
+                double dv = f1[i][t] - f0[i][t];
+                double gx = gn * fn0x[i] + dv * fmx[i];
+                double gy = gn * fn0y[i] + dv * fmy[i];
+                double vn = vx * fnx[i] + vy * fny[i];
+                acc += fA[i] * (kl * (fnx[i] * gx + fny[i] * gy) - tau * (vx * gx + vy * gy) * vn);
+            }
+            aR += wd * acc;
+        }
+        double gx = 0, gy = 0;
+        if (ok) {
+#pragma unroll 4
+            for (int k = ka; k < kz; ++k) {
+                double d1 = invLx * (e[(size_t)gidx[k] * DB + t] - x);
+                gx += gP[k] * d1;
+                gy += gP[(size_t)nnz + k] * d1;
+            }
+            if (isnan(gx) || isnan(gy)) { gx = 0; gy = 0; }
+        }
+        if (act) {
+            gcx[t] = gx;
+            gcy[t] = gy;
+        }
+        db.next(R);
+    }
     aT = row_sum(aT, R.tpr, sh);
     ax = row_sum(ax, R.tpr, sh);
     ay = row_sum(ay, R.tpr, sh);
@@ -1037,9 +1098,13 @@ bool Solver::build() {
     {
         // about kNT row entries per thread: long rows get one block of a
         // multiple of 64 threads, short rows share a 256-thread block
+        // when a multiple of 64 lanes is also a multiple of n_bands, every
+        // thread keeps one band for the whole row
         int DB = nD * nB, need = (DB + kNT - 1) / kNT;
         if (need > 64) {
             row_tpr = std::min(kRowBlockMax, ((need + 63) / 64) * 64);
+            int l = 64 * nB / std::gcd(64, nB);
+            if (l <= kRowBlockMax) row_tpr = std::min(kRowBlockMax / l, std::max(1, (row_tpr + l / 2) / l)) * l;
             row_rpb = 1;
         } else {
             row_tpr = 1;
@@ -1053,8 +1118,10 @@ bool Solver::build() {
         cap(p.cap.begin(), p.cap.begin() + nB), lr(p.lr.begin(), p.lr.begin() + nB), hr(p.hr.begin(), p.hr.begin() + nB);
     for (int b = 0; b < nB; ++b)
         if (!(vg[b] > 0)) { err_ = "group velocities must be positive"; return false; }
+    std::vector<double> lrc(nB);
+    for (int b = 0; b < nB; ++b) lrc[b] = lr[b] / cap[b];
     if (!upload(d_vg, vg) || !upload(d_tau, tau) || !upload(d_cap, cap) || !upload(d_lr, lr) || !upload(d_hr, hr) ||
-        !upload(d_sx, sx) || !upload(d_sy, sy) || !upload(d_sz, sz) || !upload(d_w, w))
+        !upload(d_sx, sx) || !upload(d_sy, sy) || !upload(d_sz, sz) || !upload(d_w, w) || !upload(d_lrc, lrc))
         return false;
     // ---- slots
     size_t ns = (size_t)nc * 6;
@@ -1239,14 +1306,16 @@ bool Solver::build() {
             }
         }
         std::vector<int> vptr(1, 0), vsrc, tptr(1, 0), tsrc;
-        std::vector<double> vdist, tdist;
+        std::vector<double> vwt, tdist;
         for (int v = 0; v < nn; ++v) {
-            for (auto &e : ve[v]) { vsrc.push_back(e.first); vdist.push_back(e.second); }
+            double r = 0;
+            for (auto &e : ve[v]) r += 1.0 / e.second;
+            for (auto &e : ve[v]) { vsrc.push_back(e.first); vwt.push_back(1.0 / e.second / r); }
             vptr.push_back((int)vsrc.size());
             for (auto &e : vt[v]) { tsrc.push_back(e.first); tdist.push_back(e.second); }
             tptr.push_back((int)tsrc.size());
         }
-        if (!upload(d_vptr, vptr) || !upload(d_vsrc, vsrc) || !upload(d_vdist, vdist) || !upload(d_tptr, tptr) ||
+        if (!upload(d_vptr, vptr) || !upload(d_vsrc, vsrc) || !upload(d_vwt, vwt) || !upload(d_tptr, tptr) ||
             !upload(d_tsrc, tsrc) || !upload(d_tdist, tdist))
             return false;
     }
@@ -1286,9 +1355,15 @@ bool Solver::build() {
                 my[s] = p.node_y[v1] - p.node_y[v0];
                 m2[s] = mx[s] * mx[s] + my[s] * my[s];
             }
+        std::vector<double> ga(ns), mxm(ns), mym(ns);
+        for (size_t s = 0; s < ns; ++s) {
+            ga[s] = an[s] / aa[s];
+            mxm[s] = mx[s] / m2[s];
+            mym[s] = my[s] / m2[s];
+        }
         if (!upload(d_fkind, kind) || !upload(d_fother, other) || !upload(d_fv0, fv0) || !upload(d_fv1, fv1) ||
-            !upload(d_fan, an) || !upload(d_faa, aa) || !upload(d_fn0x, n0x) || !upload(d_fn0y, n0y) || !upload(d_fn0z, n0z) ||
-            !upload(d_fmx, mx) || !upload(d_fmy, my) || !upload(d_fm2, m2))
+            !upload(d_fga, ga) || !upload(d_fn0x, n0x) || !upload(d_fn0y, n0y) || !upload(d_fmx, mxm) ||
+            !upload(d_fmy, mym))
             return false;
     }
     // ---- boundary entries
@@ -1588,11 +1663,11 @@ int Solver::set_initial(const double *T, const double *qx, const double *qy, con
 int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
-    Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, P.kappa_bulk[0], P.capacity_bulk[0]};
+    Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
+            P.capacity_bulk[0]};
     SlotGeo g{d_fsize.p, d_snb.p, d_sent.p, d_snx.p, d_sny.p, d_snz.p, d_sarea.p, d_vol.p,
               d_srx.p, d_sry.p, d_srz.p, d_sqx.p, d_sqy.p, d_sqz.p};
-    FaceK fk{d_fkind.p, d_fother.p, d_fv0.p, d_fv1.p, d_fan.p, d_faa.p, d_fn0x.p, d_fn0y.p, d_fn0z.p,
-             d_fmx.p, d_fmy.p, d_fm2.p};
+    FaceK fk{d_fkind.p, d_fother.p, d_fv0.p, d_fv1.p, d_fga.p, d_fn0x.p, d_fn0y.p, d_fmx.p, d_fmy.p};
     int nnz = P.nbr_ptr[nc];
     double invLx = 1.0 / P.deck.Lx;
     int n_ent = (int)h_ent_list.size();
@@ -1605,7 +1680,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     DBuf<int> d_be_slot;
     if (!upload(d_be_slot, be_slot)) return GBTE_HIP_ERROR;
     size_t nE = (size_t)nD * nc * nB;
-    Rows R{nD * nB, nB, row_tpr, row_rpb};
+    Rows R{nD * nB, nB, row_tpr, row_rpb, row_tpr / nB, row_tpr % nB};
+    bool one_row = row_rpb == 1;
     int rowblk = row_tpr * row_rpb;
     // row kernels remap blocks per XCD (xcd_block): grids are multiples of 8
     auto xgrid = [](int n) { return ((n + kXcd - 1) / kXcd) * kXcd; };
@@ -1638,8 +1714,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
         (void)hipEventRecord(ev0_, s_);
-        hipLaunchKernelGGL(k_source, dim3(gw), dim3(rowblk), 0, s_, d_posc.p, d_S.p, d_grad.p, d_eb.p, d_Told.p, d_Q.p,
-                           g, m, nc, dim, R);
+        hipLaunchKernelGGL(one_row ? k_source<true> : k_source<false>, dim3(gw), dim3(rowblk), 0, s_, d_posc.p, d_S.p,
+                           d_grad.p, d_eb.p, d_Told.p, d_Q.p, g, m, nc, R);
         hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
                            (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
                            d_rec_q.p, d_rec_c.p, d_rec_g.p, d_rec_G.p, d_S.p, d_e.p, m, nc, nB, nD, sweep_bc,
@@ -1647,11 +1723,11 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         (void)hipEventRecord(ev1_, s_);
         hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
-        hipLaunchKernelGGL(k_vertex_e, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p, d_eV.p, d_vptr.p, d_vsrc.p,
-                           d_vdist.p, nn, R);
-        hipLaunchKernelGGL(k_face_moments, dim3(gw), dim3(rowblk), 0, s_, d_e.p, d_eb.p, d_eV.p, d_grad.p, g, fk, m,
-                           d_gptr.p, d_gidx.p, d_gP.p, d_gok.p, nnz, invLx, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc,
-                           dim, R);
+        hipLaunchKernelGGL(one_row ? k_vertex_e<true> : k_vertex_e<false>, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p,
+                           d_eV.p, d_vptr.p, d_vsrc.p, d_vwt.p, nn, R);
+        hipLaunchKernelGGL(one_row ? k_face_moments<true> : k_face_moments<false>, dim3(gw), dim3(rowblk), 0, s_, d_e.p,
+                           d_eb.p, d_eV.p, d_grad.p, g, fk, m, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p, nnz, invLx, d_T.p,
+                           d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc, R);
         (void)hipEventRecord(ev2_, s_);
         hipLaunchKernelGGL(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, s_, d_eb.p, d_tmac.p, m,
                            d_ent_list.p, d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p,

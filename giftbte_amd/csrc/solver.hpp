@@ -97,7 +97,7 @@ class Solver {
     int row_tpr = 256, row_rpb = 1;  // threads per cell row, rows per block
 
     // mode tables
-    DBuf<double> d_vg, d_tau, d_cap, d_lr, d_hr, d_sx, d_sy, d_sz, d_w;
+    DBuf<double> d_vg, d_tau, d_cap, d_lr, d_hr, d_sx, d_sy, d_sz, d_w, d_lrc;
     // cell / slot geometry
     DBuf<int> d_fsize, d_snb, d_sent;
     DBuf<double> d_vol, d_Q, d_snx, d_sny, d_snz, d_sarea, d_srx, d_sry, d_srz, d_sqx, d_sqy, d_sqz;
@@ -111,10 +111,10 @@ class Solver {
     DBuf<double> d_gP;
     // IDW stencils (energy density: cells or boundary entries; temperature: cells)
     DBuf<int> d_vptr, d_vsrc, d_tptr, d_tsrc;
-    DBuf<double> d_vdist, d_tdist;
+    DBuf<double> d_vwt, d_tdist;
     // face-gradient / non-Fourier slot data
     DBuf<int> d_fkind, d_fother, d_fv0, d_fv1;
-    DBuf<double> d_fan, d_faa, d_fn0x, d_fn0y, d_fn0z, d_fmx, d_fmy, d_fm2;
+    DBuf<double> d_fga, d_fn0x, d_fn0y, d_fmx, d_fmy;
     // boundary entries
     std::vector<int> h_ent_list;
     DBuf<int> d_ent_list, d_be_cell, d_be_type, d_be_conn, d_be_spec;
