@@ -28,6 +28,7 @@
 // reads whole neighbour rows with coalesced loads.  The sweep reads and writes
 // the 8*n_bands-byte (cell, dir) segments of a row.
 #include "solver.hpp"
+#include "tiles.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -43,11 +44,13 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kSweepBlock = 512;
+constexpr int kSweepBlock = 1024;
 constexpr int kMaxIn = 3;  // in-faces per cell of one direction (triangles, quads, tetrahedra)
 constexpr int kNT = 8;     // row entries per thread in the row kernels
 constexpr int kNTf = 4;    // the same for the register-heavy face / source kernels
 constexpr int kRowBlockMax = 512;
+constexpr int kStencilBatch = 4;  // LSQ stencil entries loaded together
+constexpr int kNJ = 4;            // row entries per thread loaded together
 
 __device__ __forceinline__ int wave_global() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); }
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
@@ -76,6 +79,12 @@ __device__ __forceinline__ double row_sum(double v, int tpr, double *sh) {
     double s = 0;
     for (int k = 0; k < wpr; ++k) s += sh[w0 + k];
     return s;
+}
+
+// Sum over the 64 lanes of a wave (butterfly, fixed order), in every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -263,42 +272,79 @@ __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__
         fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)g.ent[k] * DB;
     }
     DirBand db(t0, nB);
-    double vb = m.vg[db.b], tau = m.tau[db.b], s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
-    for (int t = t0; t < DB; t += R.tpr) {
-        if (R.bstep != 0) {
-            vb = m.vg[db.b];
-            tau = m.tau[db.b];
-            s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
+    for (int tb = t0; tb < DB; tb += kNJ * R.tpr) {
+        // kNJ entries at once; per face, the upwind loads of all kNJ entries
+        // are issued before the arithmetic
+        int tt[kNJ], dd[kNJ], bb[kNJ];
+        double gcx[kNJ], gcy[kNJ], vx[kNJ], vy[kNJ], tau[kNJ], s[kNJ];
+#pragma unroll
+        for (int j = 0; j < kNJ; ++j) {
+            int t = tb + j * R.tpr;
+            bool in = t < DB;
+            tt[j] = in ? t : t0;
+            dd[j] = in ? db.d : 0;
+            bb[j] = in ? db.b : 0;
+            db.next(R);
+            gcx[j] = gc[tt[j]];
+            gcy[j] = gc[DB + tt[j]];
+            double vb = m.vg[bb[j]];
+            vx[j] = vb * m.sx[dd[j]];
+            vy[j] = vb * m.sy[dd[j]];
+            tau[j] = m.tau[bb[j]];
+            s[j] = To * m.cap[bb[j]] + Qc * m.hr[bb[j]] * m.tau[bb[j]];
         }
-        double sx = m.sx[db.d], sy = m.sy[db.d];
-        double gcx = gc[t], gcy = gc[DB + t];
-        double s = s0;
 #pragma unroll
         for (int f = 0; f < kMaxFace2D; ++f) {
             if (f >= nf) break;
-            double dot = vb * sx * fnx[f] + vb * sy * fny[f];
-            double a = tau * fav[f] * dot;
-            if (dot >= 0) {
-                s -= a * (frx[f] * gcx + fry[f] * gcy);
-            } else if (fnb[f] >= 0) {
-                s -= a * (fqx[f] * fin[f][t] + fqy[f] * fin[f][DB + t]);
-            } else {
-                s -= a * fin[f][t];
+            double dot[kNJ], u0[kNJ], u1[kNJ];
+#pragma unroll
+            for (int j = 0; j < kNJ; ++j) {
+                dot[j] = vx[j] * fnx[f] + vy[j] * fny[f];
+                u0[j] = 0;
+                u1[j] = 0;
+                if (dot[j] < 0) {
+                    u0[j] = fin[f][tt[j]];
+                    if (fnb[f] >= 0) u1[j] = fin[f][DB + tt[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kNJ; ++j) {
+                double a = tau[j] * fav[f] * dot[j];
+                if (dot[j] >= 0) {
+                    s[j] -= a * (frx[f] * gcx[j] + fry[f] * gcy[j]);
+                } else if (fnb[f] >= 0) {
+                    s[j] -= a * (fqx[f] * u0[j] + fqy[f] * u1[j]);
+                } else {
+                    s[j] -= a * u0[j];
+                }
             }
         }
-        S[((size_t)db.d * nc + pc[db.d]) * nB + db.b] = s;
-        db.next(R);
+#pragma unroll
+        for (int j = 0; j < kNJ; ++j)
+            if (tb + j * R.tpr < DB) S[((size_t)dd[j] * nc + pc[dd[j]]) * nB + bb[j]] = s[j];
     }
 }
 
 // The upwind solve of every direction (the Eigen SparseLU solve of
 // StaticBTESyntheticSolver.cpp:96-100 on the matrix of _get_coefficient,
 // StaticBTESynthetic.cpp:784-840): one workgroup walks all upwind levels of
-// one direction for a chunk of bands,
+// one direction for a chunk of Bc bands,
 //   e_new = (S - sum_in,interior tau v g_f e_new_nb) / (1 + tau v G_out),
 // with g_f = (A/V)(s.n) of the in-faces and G_out the sum over out-faces.
-// Levels are separated by workgroup barriers; the most recent R positions
-// stay in an LDS ring, older upwind values are read back from HBM.
+// Levels are separated by workgroup barriers.  The most recent R positions
+// stay in an LDS ring.  Each slot's items are loaded two levels ahead (source,
+// records, 1/(1 + tau v G)); upwind values that are already final at load
+// time are folded into the source then, so that after a barrier only values
+// of the two previous levels are read, from LDS.
+constexpr int kSweepPF = 2;  // items per slot and level held in registers
+
+struct SweepItem {
+    double s, inv;
+    int cell;
+    int q[kMaxIn], c[kMaxIn];  // pending upwind positions / cells (q < 0: none)
+    double a[kMaxIn];          // their coefficients tau v g_f
+};
+
 __global__ void __launch_bounds__(kSweepBlock) k_sweep_dir(
     const int *__restrict__ item_cell, const int *__restrict__ levptr, const int *__restrict__ dlev,
     const int *__restrict__ rec_q, const int *__restrict__ rec_c, const double *__restrict__ rec_g,
@@ -315,45 +361,77 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_dir(
     size_t base = (size_t)d * nc;
     size_t DB = (size_t)nD * nB;
     double *ed = e + (size_t)d * nB + b;  // + cell * DB
-    // operands of the first item of the next level are loaded before the
-    // barrier so that their latency overlaps it
-    double pf_s = 0, pf_G = 0, pf_g[kMaxIn];
-    int pf_q[kMaxIn], pf_c[kMaxIn], pf_cell = 0;
-    auto fetch = [&](size_t P) {
-        pf_s = S[P * nB + b];
-        pf_G = rec_G[P];
-        pf_cell = item_cell[P];
+    // upwind value at position q (cell c) that is final and visible:
+    // from the ring while it holds q (ring positions >= top - R, top = end of
+    // the latest level this value may be overwritten by), else from HBM
+    auto upwind = [&](int q, int c, int top) -> double {
+        return q >= top - R ? ring[(q % R) * Bc + bl] : ed[(size_t)c * DB];
+    };
+    // item k of the level ending at b1; upwind values at positions < done
+    // are final now
+    auto load = [&](SweepItem &it, int k, int b1, int done) {
+        size_t P = base + k;
+        double s = S[P * nB + b];
+        it.inv = 1.0 / (1.0 + tv * rec_G[P]);
+        it.cell = item_cell[P];
 #pragma unroll
         for (int j = 0; j < kMaxIn; ++j) {
-            pf_q[j] = rec_q[P * kMaxIn + j];
-            pf_c[j] = rec_c[P * kMaxIn + j];
-            pf_g[j] = rec_g[P * kMaxIn + j];
-        }
-    };
-    if (act && L > 0 && lp[0] + slot < lp[1]) fetch(base + lp[0] + slot);
-    for (int l = 0; l < L; ++l) {
-        int a0 = lp[l], a1 = lp[l + 1];
-        int lo = a1 - R;
-        if (act) {
-            for (int k = a0 + slot; k < a1; k += cpp) {
-                size_t P = base + k;
-                if (k != a0 + slot) fetch(P);
-                double s = pf_s;
-                double M = 1.0 + tv * pf_G;
-#pragma unroll
-                for (int j = 0; j < kMaxIn; ++j) {
-                    int q = pf_q[j];
-                    if (q < 0) break;
-                    double ev = (q >= lo) ? ring[(q % R) * Bc + bl] : ed[(size_t)pf_c[j] * DB];
-                    s -= tv * pf_g[j] * ev;
-                }
-                double en = s / M;
-                ed[(size_t)pf_cell * DB] = en;
-                ring[(k % R) * Bc + bl] = en;
+            int q = rec_q[P * kMaxIn + j], c = rec_c[P * kMaxIn + j];
+            double a = tv * rec_g[P * kMaxIn + j];
+            if (q >= 0 && q < done) {
+                s -= a * upwind(q, c, b1);
+                q = -1;
             }
-            if (l + 1 < L && lp[l + 1] + slot < lp[l + 2]) fetch(base + lp[l + 1] + slot);
+            it.q[j] = q;
+            it.c[j] = c;
+            it.a[j] = a;
+        }
+        it.s = s;
+    };
+    auto solve = [&](const SweepItem &it, int k, int a1) {
+        double s = it.s;
+#pragma unroll
+        for (int j = 0; j < kMaxIn; ++j)
+            if (it.q[j] >= 0) s -= it.a[j] * upwind(it.q[j], it.c[j], a1);
+        double en = s * it.inv;
+        ed[(size_t)it.cell * DB] = en;
+        ring[(k % R) * Bc + bl] = en;
+    };
+    // one level: solve its items, refill the buffer with level l + 2
+    auto step = [&](int l, SweepItem(&pf)[kSweepPF]) {
+        int a0 = lp[l], a1 = lp[l + 1];
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < kSweepPF; ++i) {
+                int k = a0 + slot + i * cpp;
+                if (k < a1) solve(pf[i], k, a1);
+            }
+            for (int k = a0 + slot + kSweepPF * cpp; k < a1; k += cpp) {
+                SweepItem it;
+                load(it, k, a1, a0);
+                solve(it, k, a1);
+            }
+            if (l + 2 < L) {
+                int c0 = lp[l + 2], c1 = lp[l + 3];
+#pragma unroll
+                for (int i = 0; i < kSweepPF; ++i) {
+                    int k = c0 + slot + i * cpp;
+                    if (k < c1) load(pf[i], k, c1, a0);
+                }
+            }
         }
         __syncthreads();
+    };
+    SweepItem pa[kSweepPF], pb[kSweepPF];
+    if (act)
+#pragma unroll
+        for (int i = 0; i < kSweepPF; ++i) {
+            if (L > 0 && lp[0] + slot + i * cpp < lp[1]) load(pa[i], lp[0] + slot + i * cpp, lp[1], 0);
+            if (L > 1 && lp[1] + slot + i * cpp < lp[2]) load(pb[i], lp[1] + slot + i * cpp, lp[2], 0);
+        }
+    for (int l = 0; l < L; l += 2) {
+        step(l, pa);
+        if (l + 1 < L) step(l + 1, pb);
     }
 }
 
@@ -375,14 +453,33 @@ __global__ void __launch_bounds__(kRowBlockMax) k_vertex_e(const double *__restr
         double sm[kNT];
 #pragma unroll
         for (int j = 0; j < kNT; ++j) sm[j] = 0;
-        for (int k = a; k < z; ++k) {
-            int src = vsrc[k];
-            const double *row = src >= 0 ? e + (size_t)src * DB : eb + (size_t)(-src - 1) * DB;
-            double wk = vwt[k];
+        int k = a;
+        for (; k + 1 < z; k += 2) {  // two sources: 2 kNT loads in flight
+            int s0 = vsrc[k], s1 = vsrc[k + 1];
+            const double *r0 = s0 >= 0 ? e + (size_t)s0 * DB : eb + (size_t)(-s0 - 1) * DB;
+            const double *r1 = s1 >= 0 ? e + (size_t)s1 * DB : eb + (size_t)(-s1 - 1) * DB;
+            double w0 = vwt[k], w1 = vwt[k + 1];
+            double x0[kNT], x1[kNT];
 #pragma unroll
             for (int j = 0; j < kNT; ++j) {
                 int t = tb + j * R.tpr;
-                if (t < DB) sm[j] += row[t] * wk;
+                x0[j] = t < DB ? r0[t] : 0.0;
+                x1[j] = t < DB ? r1[t] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < kNT; ++j) {
+                sm[j] += x0[j] * w0;
+                sm[j] += x1[j] * w1;
+            }
+        }
+        if (k < z) {
+            int s0 = vsrc[k];
+            const double *r0 = s0 >= 0 ? e + (size_t)s0 * DB : eb + (size_t)(-s0 - 1) * DB;
+            double w0 = vwt[k];
+#pragma unroll
+            for (int j = 0; j < kNT; ++j) {
+                int t = tb + j * R.tpr;
+                if (t < DB) sm[j] += r0[t] * w0;
             }
         }
 #pragma unroll
@@ -451,54 +548,96 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
     bool ok = gok[cc] != 0;
     double *gcx = grad + (size_t)cc * 2 * DB, *gcy = gcx + DB;
     DirBand db(t0, nB);
-    double vb = m.vg[db.b], tau = m.tau[db.b], lrc = m.lrc[db.b];
-    bool hr = m.hr[db.b] > 0;
     double aT = 0, ax = 0, ay = 0, az = 0, aR = 0;
-    for (int t = t0; t < DB; t += R.tpr) {
-        if (R.bstep != 0) {
-            vb = m.vg[db.b];
-            tau = m.tau[db.b];
-            lrc = m.lrc[db.b];
-            hr = m.hr[db.b] > 0;
-        }
-        double wd = m.w[db.d];
-        double vx = vb * m.sx[db.d], vy = vb * m.sy[db.d], vz = vb * m.sz[db.d];
-        double x = ec[t];
-        aT += lrc * x * wd;
-        ax += vx * wd * x;
-        ay += vy * wd * x;
-        az += vz * wd * x;
-        if (hr) {
-            double kl = m.kappa * lrc, acc = 0;
+    for (int tb = t0; tb < DB; tb += kNJ * R.tpr) {
+        int tt[kNJ];
+        double x[kNJ], wd[kNJ], vx[kNJ], vy[kNJ], tau[kNJ], kl[kNJ];
+        bool hr[kNJ], in[kNJ];
 #pragma unroll
-            for (int i = 0; i < kMaxFace2D; ++i) {
-                if (fk[i] == 0) continue;
-                double o = fo[i][t];
-                double de = fk[i] == 3 ? x - o : o - x;
+        for (int j = 0; j < kNJ; ++j) {
+            int t = tb + j * R.tpr;
+            in[j] = t < DB;
+            tt[j] = in[j] ? t : t0;
+            int d = in[j] ? db.d : 0, b = in[j] ? db.b : 0;
+            db.next(R);
+            x[j] = ec[tt[j]];
+            wd[j] = in[j] ? m.w[d] : 0.0;
+            double vb = m.vg[b];
+            vx[j] = vb * m.sx[d];
+            vy[j] = vb * m.sy[d];
+            double vz = vb * m.sz[d];
+            tau[j] = m.tau[b];
+            double lrc = m.lrc[b];
+            kl[j] = m.kappa * lrc;
+            hr[j] = m.hr[b] > 0;
+            aT += lrc * x[j] * wd[j];
+            ax += vx[j] * wd[j] * x[j];
+            ay += vy[j] * wd[j] * x[j];
+            az += vz * wd[j] * x[j];
+        }
+        // face gradients and the non-Fourier residual, one face at a time
+        // (3 kNJ loads in flight)
+        double acc[kNJ];
+#pragma unroll
+        for (int j = 0; j < kNJ; ++j) acc[j] = 0;
+#pragma unroll
+        for (int i = 0; i < kMaxFace2D; ++i) {
+            if (fk[i] == 0) continue;
+            double o[kNJ], w0[kNJ], w1[kNJ];
+#pragma unroll
+            for (int j = 0; j < kNJ; ++j) {
+                o[j] = fo[i][tt[j]];
+                w0[j] = f0[i][tt[j]];
+                w1[j] = f1[i][tt[j]];
+            }
+#pragma unroll
+            for (int j = 0; j < kNJ; ++j) {
+                double de = fk[i] == 3 ? x[j] - o[j] : o[j] - x[j];
                 double gn = de * fga[i];
-                double dv = f1[i][t] - f0[i][t];
+                double dv = w1[j] - w0[j];
                 double gx = gn * fn0x[i] + dv * fmx[i];
                 double gy = gn * fn0y[i] + dv * fmy[i];
-                double vn = vx * fnx[i] + vy * fny[i];
-                acc += fA[i] * (kl * (fnx[i] * gx + fny[i] * gy) - tau * (vx * gx + vy * gy) * vn);
+                double vn = vx[j] * fnx[i] + vy[j] * fny[i];
+                double r = fA[i] * (kl[j] * (fnx[i] * gx + fny[i] * gy) - tau[j] * (vx[j] * gx + vy[j] * gy) * vn);
+                acc[j] += hr[j] ? r : 0.0;
             }
-            aR += wd * acc;
         }
-        double gx = 0, gy = 0;
+#pragma unroll
+        for (int j = 0; j < kNJ; ++j) aR += wd[j] * acc[j];
+        // LSQ gradient: kStencilBatch neighbours x kNJ entries in flight
+        double gx[kNJ], gy[kNJ];
+#pragma unroll
+        for (int j = 0; j < kNJ; ++j) { gx[j] = 0; gy[j] = 0; }
         if (ok) {
-#pragma unroll 4
-            for (int k = ka; k < kz; ++k) {
-                double d1 = invLx * (e[(size_t)gidx[k] * DB + t] - x);
-                gx += gP[k] * d1;
-                gy += gP[(size_t)nnz + k] * d1;
+            for (int k = ka; k < kz; k += kStencilBatch) {
+                double nv[kStencilBatch][kNJ];
+#pragma unroll
+                for (int q = 0; q < kStencilBatch; ++q) {
+                    const double *rn = e + (size_t)gidx[k + q] * DB;
+#pragma unroll
+                    for (int j = 0; j < kNJ; ++j) nv[q][j] = rn[tt[j]];
+                }
+#pragma unroll
+                for (int q = 0; q < kStencilBatch; ++q) {
+                    double p0 = gP[k + q], p1 = gP[(size_t)nnz + k + q];
+#pragma unroll
+                    for (int j = 0; j < kNJ; ++j) {
+                        double d1 = invLx * (nv[q][j] - x[j]);
+                        gx[j] += p0 * d1;
+                        gy[j] += p1 * d1;
+                    }
+                }
             }
-            if (isnan(gx) || isnan(gy)) { gx = 0; gy = 0; }
         }
-        if (act) {
-            gcx[t] = gx;
-            gcy[t] = gy;
+#pragma unroll
+        for (int j = 0; j < kNJ; ++j) {
+            double a = gx[j], b = gy[j];
+            if (isnan(a) || isnan(b)) { a = 0; b = 0; }
+            if (act && in[j]) {
+                gcx[tt[j]] = a;
+                gcy[tt[j]] = b;
+            }
         }
-        db.next(R);
     }
     aT = row_sum(aT, R.tpr, sh);
     ax = row_sum(ax, R.tpr, sh);
@@ -511,6 +650,183 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
         qy[c] = ay;
         qz[c] = az;
         RM[c] = aR;
+    }
+}
+
+// LDS-staged variant of k_vertex_e + k_face_moments over cell tiles
+// (tiles.cpp): one workgroup per tile walks the row in 64-entry slices.  For
+// each slice the tile's neighbour rows [cells | boundary entries] are staged
+// in LDS, the inverse-distance values of the tile's face vertices are formed
+// there, and every cell of the tile then reads its stencil, face neighbours
+// and vertex values from LDS.  Lane t of a wave owns entry slice*64 + t, so
+// the mode constants of its (direction, band) sit in registers.  Each wave
+// keeps its cells' lane-indexed records and moment sums in registers over all
+// slices and reduces the moments once at the end.
+constexpr int kTileW = 64;
+constexpr int kTileBlock = 256;
+constexpr int kTileWaves = kTileBlock / 64;
+constexpr int kTileCellsPerWave = 4;
+constexpr int kTileCells = kTileWaves * kTileCellsPerWave;  // cells per tile
+constexpr int kTileRows = 128;                               // rows per tile: 64 KiB of LDS
+constexpr int kStageBatch = 16;                              // staged rows in flight per lane
+
+struct TileDev {
+    const int *cell0, *rptr, *rsrc, *vptr, *vs_ptr, *vs_row, *crec_i;
+    const double *vs_w, *crec_a, *crec_b;
+};
+
+// lane l of a wave-uniform record vector, as a scalar
+__device__ __forceinline__ int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double lane_dbl(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+__global__ void __launch_bounds__(kTileBlock) k_tile_face(const double *__restrict__ e, const double *__restrict__ eb,
+                                                          double *__restrict__ grad, Modes m, TileDev T, double invLx,
+                                                          double *__restrict__ To, double *__restrict__ qxo,
+                                                          double *__restrict__ qyo, double *__restrict__ qzo,
+                                                          double *__restrict__ RMo, int DB, int nB, int ntiles,
+                                                          int vmeta_off) {
+    extern __shared__ double lds[];
+    int tile = xcd_block();
+    if (tile >= ntiles) return;  // whole workgroup
+    int lane = threadIdx.x & 63;
+    int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int r0 = T.rptr[tile], nr = T.rptr[tile + 1] - r0;
+    int v0 = T.vptr[tile], nv = T.vptr[tile + 1] - v0;
+    int c0 = T.cell0[tile], ncell = T.cell0[tile + 1] - c0;
+    // vertex stencils of the tile in LDS: ptr[nv + 1] | rows | weights
+    int vs0 = T.vs_ptr[v0], nvs = T.vs_ptr[v0 + nv] - vs0;
+    int *vm_ptr = reinterpret_cast<int *>(lds + vmeta_off);
+    int *vm_row = vm_ptr + nv + 1;
+    double *vm_w = lds + vmeta_off + (nv + 1 + nvs + 1) / 2;
+    for (int i = threadIdx.x; i <= nv; i += blockDim.x) vm_ptr[i] = T.vs_ptr[v0 + i] - vs0;
+    for (int i = threadIdx.x; i < nvs; i += blockDim.x) {
+        vm_row[i] = T.vs_row[vs0 + i];
+        vm_w[i] = T.vs_w[vs0 + i];
+    }
+    // this wave's cells: records and moment sums
+    int ri[kTileCellsPerWave];
+    double ra[kTileCellsPerWave], rb[kTileCellsPerWave];
+    double mT[kTileCellsPerWave], mx[kTileCellsPerWave], my[kTileCellsPerWave], mz[kTileCellsPerWave],
+        mR[kTileCellsPerWave];
+#pragma unroll
+    for (int q = 0; q < kTileCellsPerWave; ++q) {
+        int i = w + kTileWaves * q;
+        size_t o = (size_t)(c0 + (i < ncell ? i : 0)) * kRec + lane;
+        ri[q] = T.crec_i[o];
+        ra[q] = T.crec_a[o];
+        rb[q] = T.crec_b[o];
+        mT[q] = 0;
+        mx[q] = 0;
+        my[q] = 0;
+        mz[q] = 0;
+        mR[q] = 0;
+    }
+    int src0 = lane < nr ? T.rsrc[r0 + lane] : 0;
+    int src1 = lane + kRec < nr ? T.rsrc[r0 + kRec + lane] : 0;
+    int nslice = (DB + kTileW - 1) / kTileW;
+    for (int slice = 0; slice < nslice; ++slice) {
+        int t = slice * kTileW + lane;
+        bool in = t < DB;
+        int tt = in ? t : 0;
+        int d = tt / nB, b = tt - d * nB;
+        double wd = in ? m.w[d] : 0.0, vb = m.vg[b];
+        double vx = vb * m.sx[d], vy = vb * m.sy[d], vz = vb * m.sz[d];
+        double tau = m.tau[b], lrc = m.lrc[b], kl = m.kappa * lrc;
+        bool hr = m.hr[b] > 0;
+        __syncthreads();  // the previous slice is done with the LDS rows
+        for (int r = w; r < nr; r += kStageBatch * kTileWaves) {
+            double x[kStageBatch];
+#pragma unroll
+            for (int j = 0; j < kStageBatch; ++j) {
+                int rr = r + j * kTileWaves;
+                x[j] = 0;
+                if (rr < nr) {
+                    int src = rr < kRec ? lane_int(src0, rr) : lane_int(src1, rr - kRec);
+                    const double *row = src >= 0 ? e + (size_t)src * DB : eb + (size_t)(-src - 1) * DB;
+                    x[j] = row[tt];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kStageBatch; ++j) {
+                int rr = r + j * kTileWaves;
+                if (rr < nr) lds[rr * kTileW + lane] = x[j];
+            }
+        }
+        __syncthreads();
+        for (int i = w; i < nv; i += kTileWaves) {
+            int a = vm_ptr[i], z = vm_ptr[i + 1];
+            double sm = 0;
+            for (int k = a; k < z; ++k) sm += lds[vm_row[k] * kTileW + lane] * vm_w[k];
+            if (isnan(sm)) sm = 0;
+            lds[(nr + i) * kTileW + lane] = sm;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kTileCellsPerWave; ++q) {
+            int i = w + kTileWaves * q;
+            if (i >= ncell) break;
+            int c = c0 + i;
+            int ci = ri[q];
+            double ca = ra[q], cb = rb[q];
+            double x = lds[lane_int(ci, 40) * kTileW + lane];
+            mT[q] += lrc * x * wd;
+            mx[q] += vx * wd * x;
+            my[q] += vy * wd * x;
+            mz[q] += vz * wd * x;
+            int nf = lane_int(ci, 43);
+            if (hr) {
+                double acc = 0;
+                for (int j = 0; j < nf; ++j) {
+                    int kd = lane_int(ci, 24 + j);
+                    if (kd == 0) continue;
+                    double o = lds[lane_int(ci, 28 + j) * kTileW + lane];
+                    double dv = lds[lane_int(ci, 36 + j) * kTileW + lane] - lds[lane_int(ci, 32 + j) * kTileW + lane];
+                    double de = kd == 3 ? x - o : o - x;
+                    double gn = de * lane_dbl(ca, 24 + j);
+                    double gx = gn * lane_dbl(ca, 28 + j) + dv * lane_dbl(ca, 36 + j);
+                    double gy = gn * lane_dbl(ca, 32 + j) + dv * lane_dbl(ca, 40 + j);
+                    double nx = lane_dbl(ca, 44 + j), ny = lane_dbl(ca, 48 + j);
+                    double vn = vx * nx + vy * ny;
+                    acc += lane_dbl(ca, 52 + j) * (kl * (nx * gx + ny * gy) - tau * (vx * gx + vy * gy) * vn);
+                }
+                mR[q] += wd * acc;
+            }
+            double gx = 0, gy = 0;
+            if (lane_int(ci, 41)) {
+                int nst = lane_int(ci, 42);
+                for (int k = 0; k < nst; ++k) {
+                    double d1 = invLx * (lds[lane_int(ci, k) * kTileW + lane] - x);
+                    gx += lane_dbl(ca, k) * d1;
+                    gy += lane_dbl(cb, k) * d1;
+                }
+                if (isnan(gx) || isnan(gy)) {
+                    gx = 0;
+                    gy = 0;
+                }
+            }
+            if (in) {
+                grad[(size_t)c * 2 * DB + t] = gx;
+                grad[((size_t)c * 2 + 1) * DB + t] = gy;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kTileCellsPerWave; ++q) {
+        int i = w + kTileWaves * q;
+        if (i >= ncell) break;
+        double a0 = wave_sum(mT[q]), a1 = wave_sum(mx[q]), a2 = wave_sum(my[q]), a3 = wave_sum(mz[q]),
+               a4 = wave_sum(mR[q]);
+        if (lane == 0) {
+            int c = c0 + i;
+            To[c] = a0;
+            qxo[c] = a1;
+            qyo[c] = a2;
+            qzo[c] = a3;
+            RMo[c] = a4;
+        }
     }
 }
 
@@ -778,11 +1094,6 @@ struct NDDev {
     const long *linv_off, *la_off;
     const double *linv, *la;
 };
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 __global__ void __launch_bounds__(kBlock) k_nd_forward(const int *__restrict__ nodes, NDDev F,
                                                        const double *__restrict__ b, double *__restrict__ y,
@@ -1173,6 +1484,7 @@ bool Solver::build() {
         for (int d = 0; d < nD; ++d) {
             auto dotf = [&](size_t s) { return v0 * sx[d] * p.face_nx[s] + v0 * sy[d] * p.face_ny[s] + v0 * sz[d] * p.face_nz[s]; };
             std::fill(indeg.begin(), indeg.end(), 0);
+            std::fill(level.begin(), level.end(), 0);
             for (int c = 0; c < nc; ++c)
                 for (int j = 0; j < p.face_size[c]; ++j) {
                     size_t s = (size_t)c * 6 + j;
@@ -1232,8 +1544,11 @@ bool Solver::build() {
                 rec_G[P0] = G;
             }
         }
-        sweep_nchunk = (nB + 31) / 32;
+        // about one workgroup per CU: split the bands of a direction into
+        // chunks; the ring takes 64 KiB of LDS
+        sweep_nchunk = std::max(1, std::min(nB, 256 / std::max(1, nD)));
         sweep_bc = (nB + sweep_nchunk - 1) / sweep_nchunk;
+        sweep_nchunk = (nB + sweep_bc - 1) / sweep_bc;
         sweep_ring = 8192 / sweep_bc;
         if (!upload(d_item_cell, item_cell) || !upload(d_posc, posc) || !upload(d_levptr, levptr) || !upload(d_dlev, dlev) ||
             !upload(d_rec_q, rec_q) || !upload(d_rec_c, rec_c) || !upload(d_rec_g, rec_g) || !upload(d_rec_G, rec_G))
@@ -1273,7 +1588,23 @@ bool Solver::build() {
             }
             gok[c] = 1;
         }
-        if (!upload(d_gptr, p.nbr_ptr) || !upload(d_gidx, p.nbr_idx) || !upload(d_gP, gP) || !upload(d_gok, gok)) return false;
+        // stencils padded to a multiple of kStencilBatch with the cell itself
+        // at weight 0 (adds exact zeros), so the device loops run in batches
+        std::vector<int> qptr(nc + 1, 0), qidx;
+        std::vector<double> q0, q1;
+        for (int c = 0; c < nc; ++c) {
+            int a = p.nbr_ptr[c], z = p.nbr_ptr[c + 1];
+            for (int k = a; k < z; ++k) { qidx.push_back(p.nbr_idx[k]); q0.push_back(gP[k]); q1.push_back(gP[(size_t)nnz + k]); }
+            while ((qidx.size() - qptr[c]) % kStencilBatch) { qidx.push_back(c); q0.push_back(0.0); q1.push_back(0.0); }
+            qptr[c + 1] = (int)qidx.size();
+        }
+        gnnz = qptr[nc];
+        h_gptr = qptr;
+        h_gidx = qidx;
+        h_gok = gok;
+        q0.insert(q0.end(), q1.begin(), q1.end());
+        h_gP = q0;
+        if (!upload(d_gptr, qptr) || !upload(d_gidx, qidx) || !upload(d_gP, q0) || !upload(d_gok, gok)) return false;
     }
     // ---- inverse-distance stencils
     {
@@ -1315,6 +1646,9 @@ bool Solver::build() {
             for (auto &e : vt[v]) { tsrc.push_back(e.first); tdist.push_back(e.second); }
             tptr.push_back((int)tsrc.size());
         }
+        h_vptr = vptr;
+        h_vsrc = vsrc;
+        h_vwt = vwt;
         if (!upload(d_vptr, vptr) || !upload(d_vsrc, vsrc) || !upload(d_vwt, vwt) || !upload(d_tptr, tptr) ||
             !upload(d_tsrc, tsrc) || !upload(d_tdist, tdist))
             return false;
@@ -1361,6 +1695,15 @@ bool Solver::build() {
             mxm[s] = mx[s] / m2[s];
             mym[s] = my[s] / m2[s];
         }
+        h_fkind = kind;
+        h_fother = other;
+        h_fga = ga;
+        h_fn0x = n0x;
+        h_fn0y = n0y;
+        h_fmxm = mxm;
+        h_fmym = mym;
+        h_fv0 = fv0;
+        h_fv1 = fv1;
         if (!upload(d_fkind, kind) || !upload(d_fother, other) || !upload(d_fv0, fv0) || !upload(d_fv1, fv1) ||
             !upload(d_fga, ga) || !upload(d_fn0x, n0x) || !upload(d_fn0y, n0y) || !upload(d_fmx, mxm) ||
             !upload(d_fmy, mym))
@@ -1491,6 +1834,50 @@ bool Solver::build() {
         }
         if (!upload(d_fourier_bval, fbval)) return false;
         if (!build_matrix(rowsM, Kmacro, NDmacro) || !build_matrix(rowsF, Kfour, NDfour)) return false;
+    }
+    // ---- LDS tiles of the face / vertex / moment kernel
+    {
+        Tiles T;
+        std::string e = build_tiles(p, h_gptr, h_gidx, h_vptr, h_vsrc, h_vwt, h_fkind, h_fother, h_fv0, h_fv1,
+                                    kTileRows, kTileCells, T);
+        if (e.find("mismatch") != std::string::npos) { err_ = "internal error: " + e; return false; }
+        use_tiles = e.empty();  // otherwise (a stencil too large for LDS) the row kernels run
+        if (use_tiles) {
+            std::vector<const std::vector<double> *> face8 = {&h_fga, &h_fn0x, &h_fn0y, &h_fmxm,
+                                                              &h_fmym, &p.face_nx, &p.face_ny, &p.face_area};
+            std::string e2 = pack_tile_records(p, h_gptr, h_gP, h_gok, h_fkind, face8, T);
+            use_tiles = e2.empty();
+        }
+        if (use_tiles) {
+            n_tiles = T.n_tiles;
+            tile_rows = T.max_rows;
+            // LDS: row slices, then the vertex stencils (ptr | rows | weights)
+            int max_meta = 0;
+            for (int t = 0; t < n_tiles; ++t) {
+                int nv = T.vptr[t + 1] - T.vptr[t], nvs = T.vs_ptr[T.vptr[t + 1]] - T.vs_ptr[T.vptr[t]];
+                max_meta = std::max(max_meta, (nv + 1 + nvs + 1) / 2 + nvs);
+            }
+            tile_lds = (size_t)(tile_rows * kTileW + max_meta) * sizeof(double);
+            if (tile_lds > 80 * 1024) use_tiles = false;
+        }
+        if (use_tiles) {
+            if (!upload(d_t_cell0, T.cell0) || !upload(d_t_rptr, T.rptr) || !upload(d_t_rsrc, T.rsrc) ||
+                !upload(d_t_vptr, T.vptr) || !upload(d_t_vs_ptr, T.vs_ptr) || !upload(d_t_vs_row, T.vs_row) ||
+                !upload(d_t_vs_w, T.vs_w) || !upload(d_t_crec_i, T.crec_i) || !upload(d_t_crec_a, T.crec_a) ||
+                !upload(d_t_crec_b, T.crec_b))
+                return false;
+        }
+        std::vector<int>().swap(h_gptr);
+        std::vector<int>().swap(h_gidx);
+        std::vector<int>().swap(h_vptr);
+        std::vector<int>().swap(h_vsrc);
+        std::vector<double>().swap(h_vwt);
+        std::vector<int>().swap(h_fkind);
+        std::vector<int>().swap(h_fother);
+        std::vector<int>().swap(h_fv0);
+        std::vector<int>().swap(h_fv1);
+        std::vector<int>().swap(h_gok);
+        for (auto *v : {&h_gP, &h_fga, &h_fn0x, &h_fn0y, &h_fmxm, &h_fmym}) std::vector<double>().swap(*v);
     }
     // ---- state
     size_t nE = (size_t)nD * nc * nB;
@@ -1634,7 +2021,7 @@ int Solver::fourier(gbte_stats *st) {
         launch_rhs(false, d_fourier_bval.p, d_rhs.p);
         if (!linsolve(Kfour, NDfour, d_rhs.p, d_T.p, nullptr)) return GBTE_HIP_ERROR;
         hipLaunchKernelGGL(k_fourier_flux, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_gptr.p, d_gidx.p, d_gP.p,
-                           d_gok.p, d_nint.p, P.nbr_ptr[nc], 1.0 / P.deck.Lx, P.kappa_bulk[0], d_qx.p, d_qy.p, d_qz.p,
+                           d_gok.p, d_nint.p, gnnz, 1.0 / P.deck.Lx, P.kappa_bulk[0], d_qx.p, d_qy.p, d_qz.p,
                            nc, dim);
         if (!inner_error(d_T1.p, d_T.p, &err)) return GBTE_HIP_ERROR;
         if (opt.verbose) printf("iterations residual: %g\n", err);
@@ -1668,7 +2055,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     SlotGeo g{d_fsize.p, d_snb.p, d_sent.p, d_snx.p, d_sny.p, d_snz.p, d_sarea.p, d_vol.p,
               d_srx.p, d_sry.p, d_srz.p, d_sqx.p, d_sqy.p, d_sqz.p};
     FaceK fk{d_fkind.p, d_fother.p, d_fv0.p, d_fv1.p, d_fga.p, d_fn0x.p, d_fn0y.p, d_fmx.p, d_fmy.p};
-    int nnz = P.nbr_ptr[nc];
+    int nnz = gnnz;
     double invLx = 1.0 / P.deck.Lx;
     int n_ent = (int)h_ent_list.size();
     std::vector<int> be_slot(ne, -1);
@@ -1723,11 +2110,19 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         (void)hipEventRecord(ev1_, s_);
         hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
-        hipLaunchKernelGGL(one_row ? k_vertex_e<true> : k_vertex_e<false>, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p,
-                           d_eV.p, d_vptr.p, d_vsrc.p, d_vwt.p, nn, R);
-        hipLaunchKernelGGL(one_row ? k_face_moments<true> : k_face_moments<false>, dim3(gw), dim3(rowblk), 0, s_, d_e.p,
-                           d_eb.p, d_eV.p, d_grad.p, g, fk, m, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p, nnz, invLx, d_T.p,
-                           d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc, R);
+        if (use_tiles) {
+            TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_vptr.p, d_t_vs_ptr.p, d_t_vs_row.p, d_t_crec_i.p,
+                       d_t_vs_w.p, d_t_crec_a.p, d_t_crec_b.p};
+            hipLaunchKernelGGL(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, s_, d_e.p, d_eb.p,
+                               d_grad.p, m, td, invLx, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nD * nB, nB, n_tiles,
+                               tile_rows * kTileW);
+        } else {
+            hipLaunchKernelGGL(one_row ? k_vertex_e<true> : k_vertex_e<false>, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p,
+                               d_eV.p, d_vptr.p, d_vsrc.p, d_vwt.p, nn, R);
+            hipLaunchKernelGGL(one_row ? k_face_moments<true> : k_face_moments<false>, dim3(gw), dim3(rowblk), 0, s_, d_e.p,
+                               d_eb.p, d_eV.p, d_grad.p, g, fk, m, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p, nnz, invLx, d_T.p,
+                               d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc, R);
+        }
         (void)hipEventRecord(ev2_, s_);
         hipLaunchKernelGGL(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, s_, d_eb.p, d_tmac.p, m,
                            d_ent_list.p, d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p,

@@ -95,6 +95,7 @@ class Solver {
 
     int nc = 0, nb = 0, ne = 0, nn = 0, nB = 0, nD = 0, nf = 0, dim = 2, cpw = 1, nlev = 0;
     int row_tpr = 256, row_rpb = 1;  // threads per cell row, rows per block
+    int gnnz = 0;                    // padded LSQ stencil entries
 
     // mode tables
     DBuf<double> d_vg, d_tau, d_cap, d_lr, d_hr, d_sx, d_sy, d_sz, d_w, d_lrc;
@@ -124,6 +125,14 @@ class Solver {
     DBuf<double> d_mbcoef[2], d_mkA[2], d_ma2x[2], d_ma2y[2], d_mmx[2], d_mmy[2], d_mm2[2];
     DBuf<double> d_fourier_bval;
     DevCSR Kmacro, Kfour;
+    // LDS tiles (tiles.cpp) and the host tables they are built from
+    bool use_tiles = false;
+    int n_tiles = 0, tile_rows = 0;
+    size_t tile_lds = 0;
+    DBuf<int> d_t_cell0, d_t_rptr, d_t_rsrc, d_t_vptr, d_t_vs_ptr, d_t_vs_row, d_t_crec_i;
+    DBuf<double> d_t_vs_w, d_t_crec_a, d_t_crec_b;
+    std::vector<int> h_gptr, h_gidx, h_vptr, h_vsrc, h_fkind, h_fother, h_fv0, h_fv1, h_gok;
+    std::vector<double> h_vwt, h_gP, h_fga, h_fn0x, h_fn0y, h_fmxm, h_fmym;
     DevND NDmacro, NDfour;
     DBuf<double> d_ndy, d_ndu, d_ndx;
     // state

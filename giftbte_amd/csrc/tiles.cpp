@@ -1,0 +1,188 @@
+// Cell tiles for the LDS-staged face / vertex / moment kernel.
+//
+// Cells are in Morton order, so a run of consecutive cells is a compact
+// patch.  Each tile is such a run plus the rows its computation touches:
+//   H  every cell of the tile and every LSQ-stencil neighbour (cells sharing a
+//      vertex, which includes the face neighbours and every cell around each
+//      vertex of the tile),
+//   B  the boundary entries of the tile's (non-specular) boundary faces and
+//      the entries that feed the inverse-distance values of the tile's
+//      boundary vertices,
+//   V  the two vertices of every face-gradient face of the tile (in the order
+//      of the face's first owner), whose values are computed in LDS from H
+//      and B rows.
+// The device stages [H | B] row slices in LDS, computes the V rows after
+// them, and reads every neighbour value from LDS.  Tiles grow greedily along
+// the Morton order until the row count would exceed max_rows.
+#include "tiles.hpp"
+
+#include <algorithm>
+#include <unordered_map>
+
+namespace gbte {
+
+std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const std::vector<int> &gidx,
+                        const std::vector<int> &vptr, const std::vector<int> &vsrc, const std::vector<double> &vwt,
+                        const std::vector<int> &fkind, const std::vector<int> &fother, const std::vector<int> &fv0,
+                        const std::vector<int> &fv1, int max_rows, int max_cells, Tiles &T) {
+    int nc = p.n_cells;
+    T = Tiles();
+    T.cell0.push_back(0);
+    T.rptr.push_back(0);
+    T.vptr.push_back(0);
+    T.vs_ptr.push_back(0);
+    T.crow.assign(nc, 0);
+    T.st_row.assign(gidx.size(), 0);
+    T.f_orow.assign((size_t)nc * 6, 0);
+    T.f_v0row.assign((size_t)nc * 6, 0);
+    T.f_v1row.assign((size_t)nc * 6, 0);
+
+    struct Sets {
+        std::vector<int> H, B, V;
+        std::unordered_map<int, int> hpos, bpos, vpos;
+        void clear() { H.clear(); B.clear(); V.clear(); hpos.clear(); bpos.clear(); vpos.clear(); }
+    } cur, trial;
+
+    auto add_cell = [&](Sets &S, int c) {
+        auto addH = [&](int x) { if (!S.hpos.count(x)) { S.hpos[x] = (int)S.H.size(); S.H.push_back(x); } };
+        auto addB = [&](int x) { if (!S.bpos.count(x)) { S.bpos[x] = (int)S.B.size(); S.B.push_back(x); } };
+        auto addV = [&](int x) { if (!S.vpos.count(x)) { S.vpos[x] = (int)S.V.size(); S.V.push_back(x); } };
+        addH(c);
+        for (int k = gptr[c]; k < gptr[c + 1]; ++k) addH(gidx[k]);
+        for (int j = 0; j < p.face_size[c]; ++j) {
+            size_t s = (size_t)c * 6 + j;
+            int kd = fkind[s];
+            if (kd == 0) continue;
+            if (kd == 2) addB(fother[s]);
+            else addH(fother[s]);
+            for (int q = 0; q < 2; ++q) {
+                int v = q == 0 ? fv0[s] : fv1[s];
+                if (S.vpos.count(v)) continue;
+                addV(v);
+                for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
+                    if (vsrc[k] >= 0) addH(vsrc[k]);
+                    else addB(-vsrc[k] - 1);
+                }
+            }
+        }
+    };
+    auto rows = [](const Sets &S) { return (int)(S.H.size() + S.B.size() + S.V.size()); };
+
+    auto close_tile = [&](int c_end) {
+        const Sets &S = cur;
+        int nh = (int)S.H.size(), nbr = (int)S.B.size();
+        for (int x : S.H) T.rsrc.push_back(x);
+        for (int x : S.B) T.rsrc.push_back(-x - 1);
+        T.rptr.push_back((int)T.rsrc.size());
+        int base_v = nh + nbr;
+        for (int v : S.V) {
+            T.vnode.push_back(v);
+            for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
+                int r = vsrc[k] >= 0 ? S.hpos.at(vsrc[k]) : nh + S.bpos.at(-vsrc[k] - 1);
+                T.vs_row.push_back(r);
+                T.vs_w.push_back(vwt[k]);
+            }
+            T.vs_ptr.push_back((int)T.vs_row.size());
+        }
+        T.vptr.push_back((int)T.vnode.size());
+        for (int c = T.cell0.back(); c < c_end; ++c) {
+            T.crow[c] = S.hpos.at(c);
+            for (int k = gptr[c]; k < gptr[c + 1]; ++k) T.st_row[k] = S.hpos.at(gidx[k]);
+            for (int j = 0; j < p.face_size[c]; ++j) {
+                size_t s = (size_t)c * 6 + j;
+                int kd = fkind[s];
+                if (kd == 0) continue;
+                T.f_orow[s] = kd == 2 ? nh + S.bpos.at(fother[s]) : S.hpos.at(fother[s]);
+                T.f_v0row[s] = base_v + S.vpos.at(fv0[s]);
+                T.f_v1row[s] = base_v + S.vpos.at(fv1[s]);
+            }
+        }
+        T.max_rows = std::max(T.max_rows, rows(S));
+        T.cell0.push_back(c_end);
+    };
+
+    int count = 0;
+    for (int c = 0; c < nc; ++c) {
+        trial = cur;
+        add_cell(trial, c);
+        if (count > 0 && (rows(trial) > max_rows || count >= max_cells)) {
+            close_tile(c);
+            cur.clear();
+            count = 0;
+            trial = cur;
+            add_cell(trial, c);
+        }
+        if (rows(trial) > max_rows) return "a cell's stencil exceeds the LDS tile";
+        cur = std::move(trial);
+        ++count;
+    }
+    if (count > 0) close_tile(nc);
+    T.n_tiles = (int)T.cell0.size() - 1;
+    // every tile-local row the device reads must lie inside the tile and
+    // name the intended cell, entry or vertex
+    for (int t = 0; t < T.n_tiles; ++t) {
+        int r0 = T.rptr[t], nr = T.rptr[t + 1] - r0, v0 = T.vptr[t], nv = T.vptr[t + 1] - v0;
+        if (nr + nv > max_rows) return "tile exceeds its row budget";
+        auto staged = [&](int r, int want) { return r >= 0 && r < nr && T.rsrc[r0 + r] == want; };
+        auto vrow = [&](int r, int node) { return r >= nr && r < nr + nv && T.vnode[v0 + r - nr] == node; };
+        for (int i = 0; i < nv; ++i) {
+            int v = T.vnode[v0 + i];
+            if (T.vs_ptr[v0 + i + 1] - T.vs_ptr[v0 + i] != vptr[v + 1] - vptr[v]) return "tile vertex stencil mismatch";
+            for (int k = 0; k < vptr[v + 1] - vptr[v]; ++k) {
+                int src = vsrc[vptr[v] + k];
+                if (!staged(T.vs_row[T.vs_ptr[v0 + i] + k], src)) return "tile vertex source mismatch";
+            }
+        }
+        for (int c = T.cell0[t]; c < T.cell0[t + 1]; ++c) {
+            if (!staged(T.crow[c], c)) return "tile cell row mismatch";
+            for (int k = gptr[c]; k < gptr[c + 1]; ++k)
+                if (!staged(T.st_row[k], gidx[k])) return "tile stencil row mismatch";
+            for (int j = 0; j < p.face_size[c]; ++j) {
+                size_t s = (size_t)c * 6 + j;
+                int kd = fkind[s];
+                if (kd == 0) continue;
+                if (!staged(T.f_orow[s], kd == 2 ? -fother[s] - 1 : fother[s])) return "tile face row mismatch";
+                if (!vrow(T.f_v0row[s], fv0[s]) || !vrow(T.f_v1row[s], fv1[s])) return "tile face vertex mismatch";
+            }
+        }
+    }
+    return "";
+}
+
+std::string pack_tile_records(const Problem &p, const std::vector<int> &gptr, const std::vector<double> &gP,
+                              const std::vector<int> &gok, const std::vector<int> &fkind,
+                              const std::vector<const std::vector<double> *> &face8, Tiles &T) {
+    int nc = p.n_cells;
+    size_t nnz = gptr[nc];
+    T.crec_i.assign((size_t)nc * kRec, 0);
+    T.crec_a.assign((size_t)nc * kRec, 0.0);
+    T.crec_b.assign((size_t)nc * kRec, 0.0);
+    for (int c = 0; c < nc; ++c) {
+        int *ri = &T.crec_i[(size_t)c * kRec];
+        double *ra = &T.crec_a[(size_t)c * kRec], *rb = &T.crec_b[(size_t)c * kRec];
+        int nst = gptr[c + 1] - gptr[c];
+        if (nst > kRecSt) return "LSQ stencil longer than a tile record";
+        int nf = p.face_size[c];
+        if (nf > 4) return "more than 4 faces per cell in a tile record";
+        for (int k = 0; k < nst; ++k) {
+            ri[k] = T.st_row[gptr[c] + k];
+            ra[k] = gP[gptr[c] + k];
+            rb[k] = gP[nnz + gptr[c] + k];
+        }
+        for (int j = 0; j < nf; ++j) {
+            size_t s = (size_t)c * 6 + j;
+            ri[24 + j] = fkind[s];
+            ri[28 + j] = T.f_orow[s];
+            ri[32 + j] = T.f_v0row[s];
+            ri[36 + j] = T.f_v1row[s];
+            for (int q = 0; q < 8; ++q) ra[24 + 4 * q + j] = (*face8[q])[s];
+        }
+        ri[40] = T.crow[c];
+        ri[41] = gok[c];
+        ri[42] = nst;
+        ri[43] = nf;
+    }
+    return "";
+}
+
+}  // namespace gbte
