@@ -1,17 +1,18 @@
 """Benchmark: steady StaticBTESynthetic outer iterations on one MI355X.
 
-Workload (the reference-runnable headline case): 2D nanoporous film, 500 nm
-square minus a centred r = 150 nm diffuse pore, seeded Delaunay mesh of about
-50k triangles, 40 Debye-like bands, 64 in-plane directions (CADOM, Nphi 16),
-left/right isothermal 301/299 K, top/bottom periodic.  Synthetic data from the
-in-repo seeded generator (giftbte_amd/decks.py).
+Workload (the 2D non-gray nanoporous film): 500 nm square minus a centred
+r = 150 nm diffuse pore, seeded Delaunay mesh of about 50k triangles, 40
+Debye-like bands, 64 in-plane directions (CADOM, Nphi 16), left/right
+isothermal 301/299 K, top/bottom periodic.  Synthetic data from the in-repo
+seeded generator (giftbte_amd/decks.py).
 
 A "step" is one outer iteration of StaticBTESynthetic::solve: every
 band x direction transport sweep, the moment reductions, the boundary update
 and the FP64 synthetic (macro) solve.  value = cell*band*direction updates/s
-over the timed steps.  The time to a converged k_eff is reported beside it.
+over the timed steps (inputs resident in HBM).  The time to a converged k_eff
+(the deck's tolerances, from the Fourier warm start) is reported beside it.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--converge]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-converge] [--no-cpu]
 """
 import argparse
 import json
@@ -24,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_porous50k.json")
 
 
 def make_deck(td, n_target, nbands, nphi, seed):
@@ -59,17 +61,41 @@ def cpu_baseline(problem, T0, target_s=15.0):
                       f"{sec:.1f} s"}
 
 
+def kernel_bytes_per_update(dim):
+    """Algorithmic HBM bytes per cell*band*dir update of each transport kernel
+    (FP64): the source kernel reads the LSQ gradient (dim values) and writes
+    the sweep source; the sweep reads the source and writes e; the face /
+    moment kernel reads e and writes the gradient.  Tables shared by all
+    band x direction systems are excluded."""
+    return {"k_source": (dim + 1) * 8, "k_sweep_dir": 2 * 8, "k_tile_face": (1 + dim) * 8}
+
+
+def load_traffic(kernel, cfg):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel calibration),
+    if it was measured on this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if tr.get("workload") != {k: cfg[k] for k in ("cells", "bands", "directions")}:
+        return None
+    k = tr.get("kernels", {}).get(kernel)
+    return None if k is None else k["hbm_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=50000)
     ap.add_argument("--bands", type=int, default=40)
     ap.add_argument("--nphi", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--converge", action="store_true", help="also time a full solve to the deck tolerance")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-converge", action="store_true", help="skip the timed solve to the deck tolerance")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -98,52 +124,59 @@ def main():
         nc, nB, nD = t.n_cells, t.n_bands, t.n_dirs
         print(f"[bench] deck {nc} cells x {nB} bands x {nD} directions ({time.time() - t_gen:.1f} s set-up)",
               file=sys.stderr, flush=True)
+        t_c = time.time()
         s = BTESolver(p)
+        print(f"[bench] solver set-up {time.time() - t_c:.2f} s", file=sys.stderr, flush=True)
         t0 = time.time()
         fp = s.fourier()
-        t_fourier = time.time() - t0
-        print(f"[bench] fourier {fp} passes {t_fourier:.2f} s", file=sys.stderr, flush=True)
+        print(f"[bench] fourier {fp} passes {time.time() - t0:.2f} s", file=sys.stderr, flush=True)
         T0 = s.fields()["T_lattice"]
         st_w = s.solve(a.warmup, 0.0, 0.0)
         print(f"[bench] warmup {st_w.iterations} it {st_w.solve_ms:.1f} ms", file=sys.stderr, flush=True)
         t1 = time.time()
-        st = s.solve(a.steps, 0.0, 0.0)
+        st = s.solve(a.steps, 0.0, 0.0)  # returns after the device has finished
         wall = time.time() - t1
         U = nc * nB * nD
-        ms_step = wall * 1e3 / a.steps
-        value = U * a.steps / wall
-        # roofline of the transport step (sweep + vertex + face/moment kernels):
-        # algorithmic bytes per update (2 + 2 dim) * 8 (read+write of e, write+read of grad)
-        bytes_upd = (2 + 2 * t.dim) * 8
-        tr_s = st.transport_ms / 1e3
-        achieved = bytes_upd * U * a.steps / tr_s / 1e9 if tr_s > 0 else 0.0
+        K = a.steps
+        cfg = {"workload": "2D porous film 500 nm, r=150 nm diffuse pore, seeded Delaunay, 40 bands, 64 dirs",
+               "cells": nc, "bands": nB, "directions": nD, "systems": nB * nD, "updates_per_step": U,
+               "parallelism": "single GPU"}
+        bpu = kernel_bytes_per_update(t.dim)
+        ktime = {"k_source": st.source_ms, "k_sweep_dir": st.sweep_ms, "k_tile_face": st.face_ms}
+        kernels = {}
+        for k, ms in ktime.items():
+            per = ms / K
+            gbs = bpu[k] * U / (per * 1e-3) / 1e9 if per > 0 else 0.0
+            kernels[k] = {"ms_per_launch": per, "bytes_per_update": bpu[k], "achieved_GBs": gbs,
+                          "frac_hbm_peak": gbs / HBM_PEAK_GBS}
+        dom = max(ktime, key=ktime.get)
         out = {
             "metric": "cell*band*dir updates/s (steady StaticBTESynthetic outer iterations, FP64)",
-            "value": value, "unit": "cell*band*dir updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-            "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64", "data": "synthetic (seeded porous Delaunay deck)",
-            "config": {"workload": "2D porous film 500 nm, r=150 nm diffuse pore, seeded Delaunay",
-                       "cells": nc, "bands": nB, "directions": nD, "systems": nB * nD,
-                       "updates_per_step": U, "parallelism": "single GPU"},
-            "breakdown_ms_per_step": {"transport": st.transport_ms / a.steps, "sweep": st.sweep_ms / a.steps,
-                                      "macro": st.macro_ms / a.steps,
-                                      "macro_passes_per_step": st.macro_passes / a.steps,
-                                      "cg_iters_per_step": st.macro_cg_iters / a.steps,
+            "value": U * K / wall, "unit": "cell*band*dir updates/s", "n_gpus": 1, "steps": K, "warmup": a.warmup,
+            "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (seeded porous Delaunay deck, seeded Debye-like bands)",
+            "config": cfg,
+            "breakdown_ms_per_step": {"source": st.source_ms / K, "sweep": st.sweep_ms / K,
+                                      "face_moments": st.face_ms / K, "transport": st.transport_ms / K,
+                                      "macro": st.macro_ms / K, "macro_passes_per_step": st.macro_passes / K,
                                       "sweep_levels": st.sweep_levels},
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "transport step (k_sweep levels + k_vertex_e + k_face_moments)",
-                         "bytes_per_update": bytes_upd},
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"],
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac_hbm_peak"],
+                         "traffic": load_traffic(dom, cfg),
+                         "bytes_per_update": bpu[dom], "updates_per_launch": U},
+            "kernels": kernels,
         }
-        if a.converge:
+        if not a.no_converge:
             s2 = BTESolver(p)
             tc0 = time.time()
             s2.fourier()
             stc = s2.solve(t.max_iter, t.tol_temp, t.tol_flux)
-            tc = time.time() - tc0
-            out["time_to_converged_s"] = tc
+            out["time_to_converged_s"] = time.time() - tc0
+            out["converged"] = bool(stc.converged)
             out["converged_iterations"] = stc.iterations
-            out["keff"] = s2.keff(0, 2.0)
+            out["keff_W_per_mK"] = s2.keff(0, 2.0)
+            print(f"[bench] converged in {stc.iterations} it, {out['time_to_converged_s']:.2f} s", file=sys.stderr,
+                  flush=True)
         if not a.no_cpu:
             out["cpu_baseline"] = cpu_baseline(p, T0)
         print(json.dumps(out))

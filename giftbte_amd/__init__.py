@@ -39,6 +39,8 @@ class SolveStats:
     macro_ms: float
     sweep_levels: int
     macro_cg_iters: int
+    source_ms: float
+    face_ms: float
 
 
 class Problem:
@@ -118,7 +120,7 @@ class BTESolver:
         self._check(rc)
         return SolveStats(st.iterations, st.macro_passes, st.fourier_passes, bool(st.converged), st.err_temp,
                           st.err_flux, st.solve_ms, st.sweep_ms, st.transport_ms, st.macro_ms,
-                          st.sweep_levels, st.macro_cg_iters)
+                          st.sweep_levels, st.macro_cg_iters, st.source_ms, st.face_ms)
 
     def fields(self) -> dict:
         nc, nb = self.nc, self.nb

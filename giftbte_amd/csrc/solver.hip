@@ -1298,18 +1298,16 @@ Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_ne
     }
     if (!check(hipSetDevice(opt.device), "hipSetDevice")) return;
     if (!check(hipStreamCreate(&s_), "hipStreamCreate")) return;
-    if (!check(hipEventCreate(&ev0_), "hipEventCreate") || !check(hipEventCreate(&ev1_), "hipEventCreate") ||
-        !check(hipEventCreate(&ev2_), "hipEventCreate"))
-        return;
+    for (hipEvent_t &ev : ev_)
+        if (!check(hipEventCreate(&ev), "hipEventCreate")) return;
     if (!check(hipHostMalloc((void **)&h_pin, 64 * sizeof(double)), "hipHostMalloc")) return;
     build();
 }
 
 Solver::~Solver() {
     if (h_pin) (void)hipHostFree(h_pin);
-    if (ev0_) (void)hipEventDestroy(ev0_);
-    if (ev1_) (void)hipEventDestroy(ev1_);
-    if (ev2_) (void)hipEventDestroy(ev2_);
+    for (hipEvent_t ev : ev_)
+        if (ev) (void)hipEventDestroy(ev);
     if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -2095,21 +2093,23 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     int nt = 0, status = GBTE_OK, macro_total = 0;
     bool conv = false;
     double errT = 0, errQ = 0;
-    double sweep_ms = 0, transport_ms = 0, macro_ms = 0;
+    double source_ms = 0, sweep_ms = 0, face_ms = 0, transport_ms = 0, macro_ms = 0;
     long cg_total = 0;
     for (nt = 0; nt < max_iter; ++nt) {
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
-        (void)hipEventRecord(ev0_, s_);
+        (void)hipEventRecord(ev_[0], s_);
         hipLaunchKernelGGL(one_row ? k_source<true> : k_source<false>, dim3(gw), dim3(rowblk), 0, s_, d_posc.p, d_S.p,
                            d_grad.p, d_eb.p, d_Told.p, d_Q.p, g, m, nc, R);
+        (void)hipEventRecord(ev_[1], s_);
         hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
                            (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
                            d_rec_q.p, d_rec_c.p, d_rec_g.p, d_rec_G.p, d_S.p, d_e.p, m, nc, nB, nD, sweep_bc,
                            sweep_nchunk, sweep_ring);
-        (void)hipEventRecord(ev1_, s_);
+        (void)hipEventRecord(ev_[2], s_);
         hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
+        (void)hipEventRecord(ev_[3], s_);
         if (use_tiles) {
             TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_vptr.p, d_t_vs_ptr.p, d_t_vs_row.p, d_t_crec_i.p,
                        d_t_vs_w.p, d_t_crec_a.p, d_t_crec_b.p};
@@ -2123,13 +2123,13 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                                d_eb.p, d_eV.p, d_grad.p, g, fk, m, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p, nnz, invLx, d_T.p,
                                d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc, R);
         }
-        (void)hipEventRecord(ev2_, s_);
+        (void)hipEventRecord(ev_[4], s_);
         hipLaunchKernelGGL(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, s_, d_eb.p, d_tmac.p, m,
                            d_ent_list.p, d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p,
                            d_be_tw.p, d_be_dT.p, d_be_tov.p, n_ent, nB, nD, cpw);
+        (void)hipEventRecord(ev_[5], s_);
         if (!check(hipGetLastError(), "iteration kernels")) return GBTE_HIP_ERROR;
         // _marco_solution (StaticBTESynthetic.cpp:2692-2734)
-        auto tm0 = std::chrono::high_resolution_clock::now();
         double inner = 1000;
         int inc = 0, passes = 0;
         while (inner > 1e-6) {
@@ -2147,18 +2147,32 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             if (inc >= 10) { err_ = "cannot congerve, try to set Order=1 or 2limit"; status = GBTE_DIVERGED; break; }
         }
         macro_total += passes;
-        macro_ms += std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - tm0).count();
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev0_, ev1_);
-        sweep_ms += ms;
-        (void)hipEventElapsedTime(&ms, ev0_, ev2_);
-        transport_ms += ms;
-        if (status != GBTE_OK) { ++nt; break; }
+        (void)hipEventRecord(ev_[6], s_);
+        auto add_times = [&]() {  // after a host sync on s_
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);
+            source_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[1], ev_[2]);
+            sweep_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[3], ev_[4]);
+            face_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[0], ev_[4]);
+            transport_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[5], ev_[6]);
+            macro_ms += ms;
+        };
+        if (status != GBTE_OK) {
+            if (!check(hipEventSynchronize(ev_[6]), "event sync")) return GBTE_HIP_ERROR;
+            add_times();
+            ++nt;
+            break;
+        }
         // _get_magin_check_error
         hipLaunchKernelGGL(k_err, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_Told.p, d_qx.p, d_qy.p, d_qz.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, d_part.p, nc, red_blocks);
         std::vector<double> h;
         if (!reduce_host(d_part.p, red_blocks, h, 4)) return GBTE_HIP_ERROR;
+        add_times();
         double sT = 0, mT = 0, sQ = 0, mQ = 0;
         for (int k = 0; k < red_blocks; ++k) {
             sT += h[k];
@@ -2197,6 +2211,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         st->macro_ms = macro_ms;
         st->sweep_levels = nlev;
         st->macro_cg_iters = (int)cg_total;
+        st->source_ms = source_ms;
+        st->face_ms = face_ms;
     }
     return status;
 }

@@ -91,7 +91,8 @@ class Solver {
     gbte_options opt{};
     std::string err_;
     hipStream_t s_ = nullptr;
-    hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev2_ = nullptr;
+    // before/after: source | sweep | bound_out | face kernels | boundary | macro loop
+    hipEvent_t ev_[7] = {};
 
     int nc = 0, nb = 0, ne = 0, nn = 0, nB = 0, nD = 0, nf = 0, dim = 2, cpw = 1, nlev = 0;
     int row_tpr = 256, row_rpb = 1;  // threads per cell row, rows per block

@@ -108,11 +108,13 @@ typedef struct gbte_stats {
     int converged;        /* 1 if tolT and tolQ were met                        */
     double err_temp, err_flux;   /* last errors (StaticBTESynthetic.cpp:2565-2611) */
     double solve_ms;      /* wall time of the outer loop                        */
-    double sweep_ms;      /* device time of the upwind sweeps (HIP events)      */
-    double transport_ms;  /* device time of sweep + vertex + face/moment kernels */
-    double macro_ms;      /* wall time of the synthetic (macro) solves          */
-    int sweep_levels;     /* upwind levels (sweep launches) per outer iteration */
-    int macro_cg_iters;   /* PCG iterations, summed over synthetic solves       */
+    double sweep_ms;      /* device time of the upwind sweep kernel (HIP events), summed */
+    double transport_ms;  /* device time from the source kernel to the end of the face/moment kernel */
+    double macro_ms;      /* device time of the synthetic (macro) loops, host waits included */
+    int sweep_levels;     /* upwind levels of the deepest direction                  */
+    int macro_cg_iters;   /* PCG iterations, summed over synthetic solves (PCG only) */
+    double source_ms;     /* device time of the source kernel, summed                */
+    double face_ms;       /* device time of the vertex/face/moment kernel(s), summed */
 } gbte_stats;
 
 /* ---- problem construction (replaces BTEMain.cpp:87-362: decks, BTEBand,
