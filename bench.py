@@ -67,7 +67,7 @@ def kernel_bytes_per_update(dim):
     the sweep source; the sweep reads the source and writes e; the face /
     moment kernel reads e and writes the gradient.  Tables shared by all
     band x direction systems are excluded."""
-    return {"k_source": (dim + 1) * 8, "k_sweep_dir": 2 * 8, "k_tile_face": (1 + dim) * 8}
+    return {"k_source": (dim + 1) * 8, "k_sweep_pc": 2 * 8, "k_tile_face": (1 + dim) * 8}
 
 
 def load_traffic(kernel, cfg):
@@ -142,7 +142,7 @@ def main():
                "cells": nc, "bands": nB, "directions": nD, "systems": nB * nD, "updates_per_step": U,
                "parallelism": "single GPU"}
         bpu = kernel_bytes_per_update(t.dim)
-        ktime = {"k_source": st.source_ms, "k_sweep_dir": st.sweep_ms, "k_tile_face": st.face_ms}
+        ktime = {"k_source": st.source_ms, "k_sweep_pc": st.sweep_ms, "k_tile_face": st.face_ms}
         kernels = {}
         for k, ms in ktime.items():
             per = ms / K

@@ -272,56 +272,31 @@ __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__
         fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)g.ent[k] * DB;
     }
     DirBand db(t0, nB);
-    for (int tb = t0; tb < DB; tb += kNJ * R.tpr) {
-        // kNJ entries at once; per face, the upwind loads of all kNJ entries
-        // are issued before the arithmetic
-        int tt[kNJ], dd[kNJ], bb[kNJ];
-        double gcx[kNJ], gcy[kNJ], vx[kNJ], vy[kNJ], tau[kNJ], s[kNJ];
-#pragma unroll
-        for (int j = 0; j < kNJ; ++j) {
-            int t = tb + j * R.tpr;
-            bool in = t < DB;
-            tt[j] = in ? t : t0;
-            dd[j] = in ? db.d : 0;
-            bb[j] = in ? db.b : 0;
-            db.next(R);
-            gcx[j] = gc[tt[j]];
-            gcy[j] = gc[DB + tt[j]];
-            double vb = m.vg[bb[j]];
-            vx[j] = vb * m.sx[dd[j]];
-            vy[j] = vb * m.sy[dd[j]];
-            tau[j] = m.tau[bb[j]];
-            s[j] = To * m.cap[bb[j]] + Qc * m.hr[bb[j]] * m.tau[bb[j]];
+    double vb = m.vg[db.b], tau = m.tau[db.b], s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
+    for (int t = t0; t < DB; t += R.tpr) {
+        if (R.bstep != 0) {
+            vb = m.vg[db.b];
+            tau = m.tau[db.b];
+            s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
         }
+        double sx = m.sx[db.d], sy = m.sy[db.d];
+        double gcx = gc[t], gcy = gc[DB + t];
+        double s = s0;
 #pragma unroll
         for (int f = 0; f < kMaxFace2D; ++f) {
             if (f >= nf) break;
-            double dot[kNJ], u0[kNJ], u1[kNJ];
-#pragma unroll
-            for (int j = 0; j < kNJ; ++j) {
-                dot[j] = vx[j] * fnx[f] + vy[j] * fny[f];
-                u0[j] = 0;
-                u1[j] = 0;
-                if (dot[j] < 0) {
-                    u0[j] = fin[f][tt[j]];
-                    if (fnb[f] >= 0) u1[j] = fin[f][DB + tt[j]];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < kNJ; ++j) {
-                double a = tau[j] * fav[f] * dot[j];
-                if (dot[j] >= 0) {
-                    s[j] -= a * (frx[f] * gcx[j] + fry[f] * gcy[j]);
-                } else if (fnb[f] >= 0) {
-                    s[j] -= a * (fqx[f] * u0[j] + fqy[f] * u1[j]);
-                } else {
-                    s[j] -= a * u0[j];
-                }
+            double dot = vb * sx * fnx[f] + vb * sy * fny[f];
+            double a = tau * fav[f] * dot;
+            if (dot >= 0) {
+                s -= a * (frx[f] * gcx + fry[f] * gcy);
+            } else if (fnb[f] >= 0) {
+                s -= a * (fqx[f] * fin[f][t] + fqy[f] * fin[f][DB + t]);
+            } else {
+                s -= a * fin[f][t];
             }
         }
-#pragma unroll
-        for (int j = 0; j < kNJ; ++j)
-            if (tb + j * R.tpr < DB) S[((size_t)dd[j] * nc + pc[dd[j]]) * nB + bb[j]] = s[j];
+        S[((size_t)db.d * nc + pc[db.d]) * nB + db.b] = s;
+        db.next(R);
     }
 }
 
@@ -432,6 +407,185 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_dir(
     for (int l = 0; l < L; l += 2) {
         step(l, pa);
         if (l + 1 < L) step(l + 1, pb);
+    }
+}
+
+// Wave-specialised variant of the upwind sweep (same equation as
+// k_sweep_dir).  The host splits every level into sub-levels of at most
+// kPcCap positions and at most kPcFar "far" upwind reads: upwind values that
+// have left the LDS ring of the last R positions by the time the sub-level
+// is solved.  Four producer waves stream each sub-level's operands from HBM
+// kPcDepth sub-levels ahead (sources, position records, far upwind values)
+// and stage them in LDS one sub-level ahead: the count of their loads per
+// sub-level is fixed, so each wait covers only loads issued sub-levels ago.
+// Twelve consumer waves read only LDS (staged operands and the ring of
+// recent values) and store e.  Far values are final when the producers read
+// them: the host guarantees kPcDepth * kPcCap <= R, so a far position lies
+// in a sub-level finished before the producers' read.
+constexpr int kPcBlock = 1024;
+constexpr int kPcProd = 256;    // producer lanes (4 waves)
+constexpr int kPcCap = 128;     // positions per sub-level (2 producer lanes each)
+constexpr int kPcFar = 16;      // far upwind reads per sub-level (16 producer lanes each)
+constexpr int kPcBc = 10;       // bands per workgroup at most
+constexpr int kPcBpl = 5;       // bands per producer lane (kPcBc / 2)
+constexpr int kPcDepth = 4;     // sub-levels in flight
+constexpr int kPcMaxSub = 2047; // sub-levels per direction (LDS tables)
+// staging buffer, in doubles: S[kPcCap][Bc] | inv[kPcCap][Bc] | g[kPcCap][3]
+// | far[kPcFar][Bc] | ints cell,q0,q1,q2 [kPcCap][4]
+constexpr int kPcStage = kPcCap * kPcBc * 2 + kPcCap * 3 + kPcFar * kPcBc + kPcCap * 2;
+
+struct PcSched {
+    const PcRec *rec;    // [dir][position]
+    const int *sl, *fp;  // per direction: sub-level position / far-entry offsets (Lsub + 1 each)
+    const int *dsl;      // per direction offset into sl / fp
+    const int *farcell;  // far upwind cells, per sub-level runs
+};
+
+struct PcProdBuf {
+    double s[kPcBpl];
+    int4 r0;             // cell,q0,q1,q2 (even lane) or g2,G (odd lane, as bits)
+    int4 r1;             // g0,g1 (both lanes)
+    int fc;              // far upwind cell
+    double fv;           // far upwind value
+};
+
+__device__ __forceinline__ double pc_dbl(int lo, int hi) { return __hiloint2double(hi, lo); }
+
+__global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *__restrict__ S,
+                                                       double *__restrict__ e, Modes m, int nc, int nB, int nD, int Bc,
+                                                       int nchunk, int R) {
+    extern __shared__ double lds[];
+    int d = blockIdx.x / nchunk, chunk = blockIdx.x - d * nchunk;
+    int b0 = chunk * Bc;
+    int sl0 = P.dsl[d], Lsub = P.dsl[d + 1] - sl0 - 1;
+    double *ring = lds;
+    double *stage = lds + (size_t)R * Bc;
+    int *sl = reinterpret_cast<int *>(stage + 2 * kPcStage);
+    int *fp = sl + (Lsub + 1);
+    for (int i = threadIdx.x; i <= Lsub; i += blockDim.x) {
+        sl[i] = P.sl[sl0 + i];
+        fp[i] = P.fp[sl0 + i];
+    }
+    __syncthreads();
+    size_t base = (size_t)d * nc;
+    size_t DB = (size_t)nD * nB;
+    int nstep = ((Lsub + kPcDepth - 1) / kPcDepth) * kPcDepth;
+    int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (wave < kPcProd / 64) {
+        // ---------------- producers
+        int pl = threadIdx.x, p = pl >> 1, h = pl & 1;
+        int fe = pl >> 4, fb = pl & 15;
+        double tv[kPcBpl];
+#pragma unroll
+        for (int i = 0; i < kPcBpl; ++i) {
+            int b = b0 + h * kPcBpl + i;
+            tv[i] = (h * kPcBpl + i < Bc && b < nB) ? m.tau[b] * m.vg[b] : 0.0;
+        }
+        PcProdBuf buf[kPcDepth];
+        // loads of sub-level lv into buf (addresses clamped to valid memory)
+        auto issue = [&](PcProdBuf &B, int lv) {
+            int l = lv < Lsub ? lv : Lsub - 1;
+            int a0 = sl[l], np = sl[l + 1] - a0, f0 = fp[l], nf = fp[l + 1] - f0;
+            int pp = p < np ? p : 0;
+            size_t Pp = base + a0 + pp;
+            const double *srow = S + Pp * nB + b0;
+#pragma unroll
+            for (int i = 0; i < kPcBpl; ++i) {
+                int jb = h * kPcBpl + i;
+                B.s[i] = srow[jb < Bc ? jb : 0];
+            }
+            const int4 *rr = reinterpret_cast<const int4 *>(P.rec + Pp);
+            B.r0 = rr[h ? 2 : 0];
+            B.r1 = rr[1];
+            B.fc = P.farcell[f0 + (fe < nf ? fe : 0)];
+        };
+        auto issue_far = [&](PcProdBuf &B) {
+            B.fv = e[(size_t)B.fc * DB + (size_t)d * nB + b0 + (fb < Bc ? fb : 0)];
+        };
+        // stage sub-level lv (its loads have arrived) into buffer lv & 1
+        auto write = [&](const PcProdBuf &B, int lv) {
+            if (lv >= Lsub) return;
+            int a0 = sl[lv], np = sl[lv + 1] - a0, nf = fp[lv + 1] - fp[lv];
+            double *st = stage + (size_t)(lv & 1) * kPcStage;
+            double *sS = st, *sI = st + kPcCap * kPcBc, *sG = sI + kPcCap * kPcBc, *sF = sG + kPcCap * 3;
+            int *sN = reinterpret_cast<int *>(sF + kPcFar * kPcBc);
+            double G = pc_dbl(B.r0.z, B.r0.w);  // odd lane
+            double Gx = __shfl_xor(G, 1, 64);
+            double Gp = h ? G : Gx;
+            if (p < np) {
+#pragma unroll
+                for (int i = 0; i < kPcBpl; ++i) {
+                    int jb = h * kPcBpl + i;
+                    if (jb < Bc) {
+                        sS[p * Bc + jb] = B.s[i];
+                        sI[p * Bc + jb] = 1.0 / (1.0 + tv[i] * Gp);
+                    }
+                }
+                if (h == 0) {
+                    sN[p * 4 + 0] = B.r0.x;
+                    sN[p * 4 + 1] = B.r0.y;
+                    sN[p * 4 + 2] = B.r0.z;
+                    sN[p * 4 + 3] = B.r0.w;
+                    sG[p * 3 + 0] = pc_dbl(B.r1.x, B.r1.y);
+                    sG[p * 3 + 1] = pc_dbl(B.r1.z, B.r1.w);
+                } else {
+                    sG[p * 3 + 2] = pc_dbl(B.r0.x, B.r0.y);
+                }
+            }
+            if (fe < nf && fb < Bc) sF[fe * Bc + fb] = B.fv;
+        };
+        // prologue: sub-levels 0 .. kPcDepth-1 in flight, far values of
+        // 0 .. kPcDepth-2, stage 0
+#pragma unroll
+        for (int q = 0; q < kPcDepth; ++q) issue(buf[q], q);
+#pragma unroll
+        for (int q = 0; q + 1 < kPcDepth; ++q) issue_far(buf[q]);
+        write(buf[0], 0);
+        __builtin_amdgcn_s_waitcnt(0);  // the loop starts from a known state of outstanding loads
+        __syncthreads();
+        // iteration j (consumers solve j): stage j+1, load j+D, far values of j+D-1
+        for (int j0 = 0; j0 < nstep; j0 += kPcDepth) {
+#pragma unroll
+            for (int q = 0; q < kPcDepth; ++q) {
+                int j = j0 + q;
+                write(buf[(q + 1) % kPcDepth], j + 1);
+                issue(buf[q % kPcDepth], j + kPcDepth);
+                issue_far(buf[(q + kPcDepth - 1) % kPcDepth]);
+                __syncthreads();
+            }
+        }
+    } else {
+        // ---------------- consumers
+        int cl = threadIdx.x - kPcProd;
+        int bl = cl % Bc, slot = cl / Bc, nslot = (kPcBlock - kPcProd) / Bc;
+        int b = b0 + bl;
+        bool act = slot < nslot && b < nB;
+        double tvb = act ? m.tau[b] * m.vg[b] : 0.0;
+        double *ed = e + (size_t)d * nB + b;
+        __syncthreads();  // prologue
+        for (int j = 0; j < nstep; ++j) {
+            if (act && j < Lsub) {
+                int a0 = sl[j], np = sl[j + 1] - a0;
+                const double *st = stage + (size_t)(j & 1) * kPcStage;
+                const double *sS = st, *sI = st + kPcCap * kPcBc, *sG = sI + kPcCap * kPcBc, *sF = sG + kPcCap * 3;
+                const int *sN = reinterpret_cast<const int *>(sF + kPcFar * kPcBc);
+                for (int p = slot; p < np; p += nslot) {
+                    int cell = sN[p * 4];
+                    double s = sS[p * Bc + bl];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        int q = sN[p * 4 + 1 + k];
+                        if (q == -1) continue;
+                        double u = q >= 0 ? ring[(q & (R - 1)) * Bc + bl] : sF[(-2 - q) * Bc + bl];
+                        s -= tvb * sG[p * 3 + k] * u;
+                    }
+                    double en = s * sI[p * Bc + bl];
+                    ed[(size_t)cell * DB] = en;
+                    ring[((a0 + p) & (R - 1)) * Bc + bl] = en;
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -1551,6 +1705,128 @@ bool Solver::build() {
         if (!upload(d_item_cell, item_cell) || !upload(d_posc, posc) || !upload(d_levptr, levptr) || !upload(d_dlev, dlev) ||
             !upload(d_rec_q, rec_q) || !upload(d_rec_c, rec_c) || !upload(d_rec_g, rec_g) || !upload(d_rec_G, rec_G))
             return false;
+        // ---- wave-specialised sweep schedule: sub-levels of at most kPcCap
+        // positions and kPcFar far upwind reads (upwind positions older than
+        // the ring of the last pc_ring positions when the sub-level runs)
+        use_pc = false;
+        pc_nchunk = std::max((nB + kPcBc - 1) / kPcBc, std::min(nB, (256 + nD - 1) / nD));
+        pc_bc = (nB + pc_nchunk - 1) / pc_nchunk;
+        pc_nchunk = (nB + pc_bc - 1) / pc_bc;
+        pc_ring = 1;
+        while ((size_t)pc_ring * 2 * pc_bc * sizeof(double) <= 80 * 1024) pc_ring *= 2;
+        if (pc_bc <= kPcBc && kPcDepth * kPcCap <= pc_ring) {
+            std::vector<PcRec> prec(NI);
+            std::vector<int> psl, pfp, pdsl(1, 0), pfar;
+            std::vector<int> freq(nc, 0), touched;
+            bool fits = true;
+            int R = pc_ring;
+            for (int d = 0; d < nD && fits; ++d) {
+                const int *lp = levptr.data() + dlev[d];
+                int L = dlev[d + 1] - dlev[d] - 1;
+                size_t base = (size_t)d * nc;
+                std::vector<int> sub(1, 0);  // sub-level starts (direction-local positions)
+                for (int l = 0; l < L; ++l) {
+                    int s0 = lp[l], far = 0;
+                    auto reset = [&]() { for (int t : touched) freq[t] = 0; touched.clear(); far = 0; };
+                    reset();
+                    for (int k = lp[l]; k < lp[l + 1]; ++k) {
+                        // tentative end k+1: ring holds positions >= k+1-R
+                        int lo_old = k - R, lo_new = k + 1 - R;
+                        int add = (lo_old >= 0) ? freq[lo_old] : 0, own = 0;
+                        for (int j = 0; j < kMaxIn; ++j) {
+                            int qq = rec_q[(base + k) * kMaxIn + j];
+                            if (qq >= 0 && qq < lo_new) own++;
+                        }
+                        if (k > s0 && (k - s0 + 1 > kPcCap || far + add + own > kPcFar)) {
+                            sub.push_back(k);
+                            s0 = k;
+                            reset();
+                            add = 0;
+                        }
+                        far += add;
+                        for (int j = 0; j < kMaxIn; ++j) {
+                            int qq = rec_q[(base + k) * kMaxIn + j];
+                            if (qq < 0) continue;
+                            if (qq < lo_new) far++;
+                            else { if (!freq[qq]) touched.push_back(qq); freq[qq]++; }
+                        }
+                    }
+                    sub.push_back(lp[l + 1]);
+                    for (int t : touched) freq[t] = 0;
+                    touched.clear();
+                }
+                int Ls = (int)sub.size() - 1;
+                if (Ls > kPcMaxSub) { fits = false; break; }
+                for (int m = 0; m < Ls; ++m) {
+                    psl.push_back(sub[m]);
+                    pfp.push_back((int)pfar.size());
+                    int a1 = sub[m + 1], nf = 0;
+                    for (int k = sub[m]; k < a1; ++k) {
+                        size_t P0 = base + k;
+                        PcRec &r = prec[P0];
+                        r.cell = item_cell[P0];
+                        r.G = rec_G[P0];
+                        for (int j = 0; j < kMaxIn; ++j) {
+                            int qq = rec_q[P0 * kMaxIn + j];
+                            r.g[j] = rec_g[P0 * kMaxIn + j];
+                            if (qq < 0) r.q[j] = -1;
+                            else if (qq >= a1 - R) r.q[j] = qq;
+                            else {
+                                r.q[j] = -2 - nf++;
+                                pfar.push_back(rec_c[P0 * kMaxIn + j]);
+                            }
+                        }
+                    }
+                    if (nf > kPcFar) { err_ = "internal error: sweep sub-level far budget"; return false; }
+                }
+                psl.push_back(sub[Ls]);
+                pfp.push_back((int)pfar.size());
+                pdsl.push_back((int)psl.size());
+            }
+            // validate what the kernel assumes: sub-levels cover the
+            // direction in order within kPcCap / kPcFar; a ring read is still
+            // in the ring and finished; a far read is finished before the
+            // producers read it, kPcDepth - 1 sub-levels ahead
+            for (int d = 0; d < nD && fits; ++d) {
+                const int *sl = psl.data() + pdsl[d];
+                const int *fp = pfp.data() + pdsl[d];
+                int Ls = pdsl[d + 1] - pdsl[d] - 1;
+                if (sl[0] != 0 || sl[Ls] != nc) { err_ = "internal error: sweep sub-levels"; return false; }
+                for (int m2 = 0; m2 < Ls; ++m2) {
+                    int a0 = sl[m2], a1 = sl[m2 + 1], nf = fp[m2 + 1] - fp[m2];
+                    if (a1 <= a0 || a1 - a0 > kPcCap || nf < 0 || nf > kPcFar) { err_ = "internal error: sweep sub-level size"; return false; }
+                    int ready = sl[std::max(0, m2 - kPcDepth + 1)];
+                    for (int k = a0; k < a1; ++k) {
+                        const PcRec &r = prec[(size_t)d * nc + k];
+                        for (int j = 0; j < kMaxIn; ++j) {
+                            int qq = r.q[j];
+                            if (qq >= 0 && !(qq < a0 && qq >= a1 - pc_ring)) { err_ = "internal error: sweep ring read"; return false; }
+                            if (qq <= -2) {
+                                int f = -2 - qq;
+                                int up = rec_q[((size_t)d * nc + k) * kMaxIn + j];
+                                if (f >= nf || pfar[fp[m2] + f] != rec_c[((size_t)d * nc + k) * kMaxIn + j] || !(up < ready)) {
+                                    err_ = "internal error: sweep far read";
+                                    return false;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            pfar.push_back(0);  // padding read by clamped loads
+            size_t lds = ((size_t)pc_ring * pc_bc + 2 * kPcStage) * sizeof(double) + 2 * (kPcMaxSub + 1) * sizeof(int);
+            int dev_lds = 0;
+            (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, opt.device);
+            if (fits && lds <= (size_t)dev_lds) {
+                pc_lds = lds;
+                use_pc = upload(d_pc_rec, prec) && upload(d_pc_sl, psl) && upload(d_pc_fp, pfp) &&
+                         upload(d_pc_dsl, pdsl) && upload(d_pc_far, pfar);
+                if (!use_pc) return false;
+                int maxsub = 0;
+                for (int d = 0; d < nD; ++d) maxsub = std::max(maxsub, pdsl[d + 1] - pdsl[d] - 1);
+                nsub = maxsub;
+            }
+        }
     }
     // ---- LSQ gradient stencil (_get_CellMatrix_larger + Get_inverse_2D,
     // StaticBTESynthetic.cpp:1000-1064, utility.cpp:401-466)
@@ -2102,10 +2378,16 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         hipLaunchKernelGGL(one_row ? k_source<true> : k_source<false>, dim3(gw), dim3(rowblk), 0, s_, d_posc.p, d_S.p,
                            d_grad.p, d_eb.p, d_Told.p, d_Q.p, g, m, nc, R);
         (void)hipEventRecord(ev_[1], s_);
-        hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
-                           (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
-                           d_rec_q.p, d_rec_c.p, d_rec_g.p, d_rec_G.p, d_S.p, d_e.p, m, nc, nB, nD, sweep_bc,
-                           sweep_nchunk, sweep_ring);
+        if (use_pc) {
+            PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
+            hipLaunchKernelGGL(k_sweep_pc, dim3(nD * pc_nchunk), dim3(kPcBlock), pc_lds, s_, ps, d_S.p, d_e.p, m, nc,
+                               nB, nD, pc_bc, pc_nchunk, pc_ring);
+        } else {
+            hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
+                               (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
+                               d_rec_q.p, d_rec_c.p, d_rec_g.p, d_rec_G.p, d_S.p, d_e.p, m, nc, nB, nD, sweep_bc,
+                               sweep_nchunk, sweep_ring);
+        }
         (void)hipEventRecord(ev_[2], s_);
         hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
@@ -2209,7 +2491,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         st->sweep_ms = sweep_ms;
         st->transport_ms = transport_ms;
         st->macro_ms = macro_ms;
-        st->sweep_levels = nlev;
+        st->sweep_levels = use_pc ? nsub : nlev;
         st->macro_cg_iters = (int)cg_total;
         st->source_ms = source_ms;
         st->face_ms = face_ms;

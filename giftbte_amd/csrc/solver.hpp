@@ -31,6 +31,14 @@ struct DBuf {
     }
 };
 
+// Upwind record of one position of the wave-specialised sweep, 48 bytes:
+// q >= 0 ring position, -1 none, -2-f far entry f of the sub-level; g the
+// in-face (A/V)(s.n) and G the sum over out-faces.
+struct PcRec {
+    int cell, q[3];
+    double g[3], G;
+};
+
 // Compressed sparse rows of the synthetic (macro) and Fourier matrices,
 // stored negated so that they are symmetric positive definite.
 struct DevCSR {
@@ -108,6 +116,12 @@ class Solver {
     DBuf<int> d_item_cell, d_posc, d_levptr, d_dlev, d_rec_q, d_rec_c;
     DBuf<double> d_rec_g, d_rec_G;
     int sweep_bc = 1, sweep_nchunk = 1, sweep_ring = 1;
+    // wave-specialised sweep (k_sweep_pc): records, sub-level tables
+    bool use_pc = false;
+    int pc_bc = 1, pc_nchunk = 1, pc_ring = 1, nsub = 0;
+    size_t pc_lds = 0;
+    DBuf<PcRec> d_pc_rec;
+    DBuf<int> d_pc_sl, d_pc_fp, d_pc_dsl, d_pc_far;
     // gradient stencil
     DBuf<int> d_gptr, d_gidx, d_gok;
     DBuf<double> d_gP;
