@@ -911,9 +911,16 @@ __global__ void __launch_bounds__(kTileBlock) k_tile_face(const double *__restri
         }
         __syncthreads();
         for (int i = w; i < nv; i += kTileWaves) {
-            int a = vm_ptr[i], z = vm_ptr[i + 1];
+            int a = vm_ptr[i], z = vm_ptr[i + 1];  // z - a is a multiple of 4
             double sm = 0;
-            for (int k = a; k < z; ++k) sm += lds[vm_row[k] * kTileW + lane] * vm_w[k];
+            for (int k = a; k < z; k += 4) {
+                double u0 = lds[vm_row[k] * kTileW + lane], u1 = lds[vm_row[k + 1] * kTileW + lane];
+                double u2 = lds[vm_row[k + 2] * kTileW + lane], u3 = lds[vm_row[k + 3] * kTileW + lane];
+                sm += u0 * vm_w[k];
+                sm += u1 * vm_w[k + 1];
+                sm += u2 * vm_w[k + 2];
+                sm += u3 * vm_w[k + 3];
+            }
             if (isnan(sm)) sm = 0;
             lds[(nr + i) * kTileW + lane] = sm;
         }
@@ -930,14 +937,24 @@ __global__ void __launch_bounds__(kTileBlock) k_tile_face(const double *__restri
             mx[q] += vx * wd * x;
             my[q] += vy * wd * x;
             mz[q] += vz * wd * x;
+            // face gradients: the 3 LDS reads of every face are issued together
             int nf = lane_int(ci, 43);
+            double fo[kMaxFace2D], f0[kMaxFace2D], f1[kMaxFace2D];
+#pragma unroll
+            for (int j = 0; j < kMaxFace2D; ++j) {
+                bool on = j < nf && lane_int(ci, 24 + j) != 0;
+                fo[j] = lds[(on ? lane_int(ci, 28 + j) : 0) * kTileW + lane];
+                f0[j] = lds[(on ? lane_int(ci, 32 + j) : 0) * kTileW + lane];
+                f1[j] = lds[(on ? lane_int(ci, 36 + j) : 0) * kTileW + lane];
+            }
             if (hr) {
                 double acc = 0;
-                for (int j = 0; j < nf; ++j) {
-                    int kd = lane_int(ci, 24 + j);
+#pragma unroll
+                for (int j = 0; j < kMaxFace2D; ++j) {
+                    int kd = j < nf ? lane_int(ci, 24 + j) : 0;
                     if (kd == 0) continue;
-                    double o = lds[lane_int(ci, 28 + j) * kTileW + lane];
-                    double dv = lds[lane_int(ci, 36 + j) * kTileW + lane] - lds[lane_int(ci, 32 + j) * kTileW + lane];
+                    double o = fo[j];
+                    double dv = f1[j] - f0[j];
                     double de = kd == 3 ? x - o : o - x;
                     double gn = de * lane_dbl(ca, 24 + j);
                     double gx = gn * lane_dbl(ca, 28 + j) + dv * lane_dbl(ca, 36 + j);
@@ -948,13 +965,23 @@ __global__ void __launch_bounds__(kTileBlock) k_tile_face(const double *__restri
                 }
                 mR[q] += wd * acc;
             }
+            // LSQ gradient, 4 stencil rows per batch (stencils are padded to
+            // multiples of 4 with the cell itself at weight 0)
             double gx = 0, gy = 0;
             if (lane_int(ci, 41)) {
                 int nst = lane_int(ci, 42);
-                for (int k = 0; k < nst; ++k) {
-                    double d1 = invLx * (lds[lane_int(ci, k) * kTileW + lane] - x);
-                    gx += lane_dbl(ca, k) * d1;
-                    gy += lane_dbl(cb, k) * d1;
+                for (int k = 0; k < nst; k += 4) {
+                    double u0 = lds[lane_int(ci, k) * kTileW + lane], u1 = lds[lane_int(ci, k + 1) * kTileW + lane];
+                    double u2 = lds[lane_int(ci, k + 2) * kTileW + lane], u3 = lds[lane_int(ci, k + 3) * kTileW + lane];
+                    double d0 = invLx * (u0 - x), d1 = invLx * (u1 - x), d2 = invLx * (u2 - x), d3 = invLx * (u3 - x);
+                    gx += lane_dbl(ca, k) * d0;
+                    gy += lane_dbl(cb, k) * d0;
+                    gx += lane_dbl(ca, k + 1) * d1;
+                    gy += lane_dbl(cb, k + 1) * d1;
+                    gx += lane_dbl(ca, k + 2) * d2;
+                    gy += lane_dbl(cb, k + 2) * d2;
+                    gx += lane_dbl(ca, k + 3) * d3;
+                    gy += lane_dbl(cb, k + 3) * d3;
                 }
                 if (isnan(gx) || isnan(gy)) {
                     gx = 0;

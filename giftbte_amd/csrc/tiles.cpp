@@ -77,10 +77,19 @@ std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const st
         int base_v = nh + nbr;
         for (int v : S.V) {
             T.vnode.push_back(v);
+            int first = 0;
             for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
                 int r = vsrc[k] >= 0 ? S.hpos.at(vsrc[k]) : nh + S.bpos.at(-vsrc[k] - 1);
+                if (k == vptr[v]) first = r;
                 T.vs_row.push_back(r);
                 T.vs_w.push_back(vwt[k]);
+            }
+            // pad to a multiple of 4 with the first source at weight 0, so
+            // the device sums in batches of 4 (x * 0 adds an exact zero
+            // unless x is not finite, which already makes the sum so)
+            while ((T.vs_row.size() - T.vs_ptr.back()) % 4) {
+                T.vs_row.push_back(first);
+                T.vs_w.push_back(0.0);
             }
             T.vs_ptr.push_back((int)T.vs_row.size());
         }
@@ -127,7 +136,12 @@ std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const st
         auto vrow = [&](int r, int node) { return r >= nr && r < nr + nv && T.vnode[v0 + r - nr] == node; };
         for (int i = 0; i < nv; ++i) {
             int v = T.vnode[v0 + i];
-            if (T.vs_ptr[v0 + i + 1] - T.vs_ptr[v0 + i] != vptr[v + 1] - vptr[v]) return "tile vertex stencil mismatch";
+            int padded = (vptr[v + 1] - vptr[v] + 3) / 4 * 4;
+            if (T.vs_ptr[v0 + i + 1] - T.vs_ptr[v0 + i] != padded) return "tile vertex stencil mismatch";
+            for (int k = vptr[v + 1] - vptr[v]; k < padded; ++k)
+                if (T.vs_w[T.vs_ptr[v0 + i] + k] != 0.0 || T.vs_row[T.vs_ptr[v0 + i] + k] < 0 ||
+                    T.vs_row[T.vs_ptr[v0 + i] + k] >= nr)
+                    return "tile vertex padding mismatch";
             for (int k = 0; k < vptr[v + 1] - vptr[v]; ++k) {
                 int src = vsrc[vptr[v] + k];
                 if (!staged(T.vs_row[T.vs_ptr[v0 + i] + k], src)) return "tile vertex source mismatch";
