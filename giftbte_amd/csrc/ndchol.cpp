@@ -12,9 +12,11 @@
 //   * factorization: multifrontal, with a dense frontal matrix per separator
 //     (supernode) and extend-add of the children's update matrices;
 //   * per supernode k the solve needs inv(L_kk) and L_{A(k),k}, where A(k)
-//     are the ancestor positions coupled to k after fill.
+//     are the ancestor positions coupled to k after fill; both are folded
+//     into F_k = [inv(L_kk); -L_{A(k),k} inv(L_kk)], so that each supernode's
+//     forward and backward step is one dense matrix-vector product.
 // The GPU solve (solver.hip, k_nd_forward / k_nd_backward) processes all
-// supernodes of one tree level per launch.
+// supernodes of one tree level per launch, in blocks of 64 rows.
 #include "ndchol.hpp"
 
 #include <algorithm>
@@ -138,18 +140,18 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
     }
     // numeric, multifrontal
     std::vector<std::vector<double>> U(nt);
-    F.rows_off.assign(nt + 1, 0); F.linv_off.assign(nt + 1, 0); F.la_off.assign(nt + 1, 0);
+    F.rows_off.assign(nt + 1, 0); F.f_off.assign(nt + 1, 0);
     F.u_off.assign(nt + 1, 0); F.map_off.assign(nt + 1, 0);
     for (int k = 0; k < nt; ++k) {
         F.rows_off[k + 1] = F.rows_off[k] + F.nrow[k];
-        F.linv_off[k + 1] = F.linv_off[k] + (long)F.ncol[k] * F.ncol[k];
-        F.la_off[k + 1] = F.la_off[k] + (long)F.nrow[k] * F.ncol[k];
+        F.f_off[k + 1] = F.f_off[k] + (long)(F.ncol[k] + F.nrow[k]) * F.ncol[k];
         F.u_off[k + 1] = F.u_off[k] + F.nrow[k];
     }
     F.u_total = F.u_off[nt];
     F.rows.resize(F.rows_off[nt]);
-    F.linv.assign(F.linv_off[nt], 0.0);
-    F.la.assign(F.la_off[nt], 0.0);
+    F.frm.assign(F.f_off[nt], 0.0);
+    F.fcm.assign(F.f_off[nt], 0.0);
+    std::vector<double> Li;
     std::vector<int> map_of_node(nt, -1);
     for (int k = 0; k < nt; ++k) {
         int nc_ = F.ncol[k], nr = F.nrow[k], m = nc_ + nr, c0 = F.col0[k], hi = c0 + nc_;
@@ -195,7 +197,7 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
             }
         }
         // inverse of L_kk (lower triangular)
-        double *Li = F.linv.data() + F.linv_off[k];
+        Li.assign((size_t)nc_ * nc_, 0.0);
         for (int col = 0; col < nc_; ++col) {
             Li[(size_t)col * nc_ + col] = 1.0 / Fm[(size_t)col * m + col];
             for (int i = col + 1; i < nc_; ++i) {
@@ -204,9 +206,21 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
                 Li[(size_t)i * nc_ + col] = -s / Fm[(size_t)i * m + i];
             }
         }
-        double *La = F.la.data() + F.la_off[k];
-        for (int r = 0; r < nr; ++r)
-            for (int a = 0; a < nc_; ++a) La[(size_t)r * nc_ + a] = Fm[(size_t)(nc_ + r) * m + a];
+        // F = [Li; -L_A Li]
+        double *Fr = F.frm.data() + F.f_off[k];
+        for (int i = 0; i < nc_; ++i)
+            for (int a = 0; a <= i; ++a) Fr[(size_t)i * nc_ + a] = Li[(size_t)i * nc_ + a];
+        for (int r = 0; r < nr; ++r) {
+            const double *la = &Fm[(size_t)(nc_ + r) * m];
+            for (int a = 0; a < nc_; ++a) {
+                double s = 0;
+                for (int j = a; j < nc_; ++j) s += la[j] * Li[(size_t)j * nc_ + a];
+                Fr[(size_t)(nc_ + r) * nc_ + a] = -s;
+            }
+        }
+        double *Fc = F.fcm.data() + F.f_off[k];
+        for (int i = 0; i < m; ++i)
+            for (int a = 0; a < nc_; ++a) Fc[(size_t)a * m + i] = Fr[(size_t)i * nc_ + a];
         U[k].assign((size_t)nr * nr, 0.0);
         for (int i = 0; i < nr; ++i)
             for (int j = 0; j <= i; ++j) U[k][(size_t)i * nr + j] = Fm[(size_t)(nc_ + i) * m + nc_ + j];
@@ -228,7 +242,7 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
 void nd_solve_host(const NDFactor &F, const std::vector<double> &b, std::vector<double> &x) {
     int nt = (int)F.ncol.size();
     std::vector<double> y(F.n), u(F.u_total), xp(F.n);
-    std::vector<double> f;
+    std::vector<double> f, v;
     for (int k = 0; k < nt; ++k) {  // postorder = a valid forward order
         int nc_ = F.ncol[k], nr = F.nrow[k], m = nc_ + nr;
         f.assign(m, 0.0);
@@ -237,32 +251,23 @@ void nd_solve_host(const NDFactor &F, const std::vector<double> &b, std::vector<
             int c = F.child_idx[q];
             for (int i = 0; i < F.nrow[c]; ++i) f[F.map[F.map_off[c] + i]] += u[F.u_off[c] + i];
         }
-        const double *Li = F.linv.data() + F.linv_off[k];
-        const double *La = F.la.data() + F.la_off[k];
-        for (int i = 0; i < nc_; ++i) {
+        const double *Fr = F.frm.data() + F.f_off[k];
+        for (int i = 0; i < m; ++i) {
             double s = 0;
-            for (int a = 0; a <= i; ++a) s += Li[(size_t)i * nc_ + a] * f[a];
-            y[F.col0[k] + i] = s;
-        }
-        for (int r = 0; r < nr; ++r) {
-            double s = f[nc_ + r];
-            for (int a = 0; a < nc_; ++a) s -= La[(size_t)r * nc_ + a] * y[F.col0[k] + a];
-            u[F.u_off[k] + r] = s;
+            for (int a = 0; a < nc_; ++a) s += Fr[(size_t)i * nc_ + a] * f[a];
+            if (i < nc_) y[F.col0[k] + i] = s;
+            else u[F.u_off[k] + i - nc_] = f[i] + s;
         }
     }
     for (int k = nt - 1; k >= 0; --k) {  // reverse postorder = a valid backward order
-        int nc_ = F.ncol[k], nr = F.nrow[k];
-        const double *Li = F.linv.data() + F.linv_off[k];
-        const double *La = F.la.data() + F.la_off[k];
-        std::vector<double> z(nc_);
-        for (int a = 0; a < nc_; ++a) {
-            double s = y[F.col0[k] + a];
-            for (int r = 0; r < nr; ++r) s -= La[(size_t)r * nc_ + a] * xp[F.rows[F.rows_off[k] + r]];
-            z[a] = s;
-        }
+        int nc_ = F.ncol[k], nr = F.nrow[k], m = nc_ + nr;
+        const double *Fr = F.frm.data() + F.f_off[k];
+        v.assign(m, 0.0);
+        for (int a = 0; a < nc_; ++a) v[a] = y[F.col0[k] + a];
+        for (int r = 0; r < nr; ++r) v[nc_ + r] = xp[F.rows[F.rows_off[k] + r]];
         for (int a = 0; a < nc_; ++a) {
             double s = 0;
-            for (int i = a; i < nc_; ++i) s += Li[(size_t)i * nc_ + a] * z[i];
+            for (int j = 0; j < m; ++j) s += Fr[(size_t)j * nc_ + a] * v[j];
             xp[F.col0[k] + a] = s;
         }
     }

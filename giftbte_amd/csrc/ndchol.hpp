@@ -14,10 +14,13 @@ struct NDFactor {
     std::vector<int> perm;  // perm[position] = row of the original matrix
     // elimination-tree nodes in postorder (children before parents)
     std::vector<int> col0, ncol, nrow, parent, height, depth;
-    std::vector<long> rows_off, linv_off, la_off, u_off, map_off;
+    std::vector<long> rows_off, f_off, u_off, map_off;
     std::vector<int> rows;        // A(k): positions (> last column of k), sorted
-    std::vector<double> linv;     // per node: inverse of L_kk, ncol x ncol, row-major (lower)
-    std::vector<double> la;       // per node: L_{A(k),k}, nrow x ncol, row-major
+    // per node, F = [inv(L_kk); -L_{A(k),k} inv(L_kk)], (ncol + nrow) x ncol,
+    // stored row-major (frm) and column-major (fcm):
+    //   forward   y = inv(L_kk) f_cols,  u = f_A - L_Ak y  =  F f_cols (+ [0; f_A])
+    //   backward  x = inv(L_kk)^T (y - L_Ak^T x_A)        =  F^T [y; x_A]
+    std::vector<double> frm, fcm;
     std::vector<int> child_ptr, child_idx;  // children of each node
     std::vector<int> map;         // per child c: local index of rows(c) in its parent's front
     long u_total = 0;

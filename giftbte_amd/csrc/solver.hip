@@ -1265,23 +1265,28 @@ __global__ void k_cg_update2(const double *__restrict__ part2, double *__restric
 }
 
 // ---- direct solve with the nested-dissection Cholesky factor (ndchol.cpp)
-// Forward: one workgroup per supernode of one elimination-tree height.  The
-// front f = [b(cols); 0(rows)] gets the children's update vectors added
-// (extend-add), then y = inv(L_kk) f_cols and u = f_rows - L_Ak y.
-// Backward: one workgroup per supernode of one depth:
-// x_cols = inv(L_kk)^T (y - L_Ak^T x_rows).  Reductions run in a fixed order.
+// Per supernode k, F_k = [inv(L_kk); -L_Ak inv(L_kk)] ((ncol + nrow) x ncol).
+// Forward, one launch per elimination-tree height: the front
+// f = [b(cols) + children; children(rows)] (extend-add of the children's
+// update vectors) and [y; u - f_A] = F_k f_cols.  Backward, one launch per
+// depth: x = F_k^T [y; x_A].  Each workgroup takes 64 output rows of one
+// supernode (lanes = rows, coalesced along the stored layout) and splits the
+// reduction over its 4 waves; partial sums are added in a fixed order.
 struct NDDev {
     const int *perm, *col0, *ncol, *nrow, *child_ptr, *child_idx, *map, *map_off, *rows, *rows_off, *u_off;
-    const long *linv_off, *la_off;
-    const double *linv, *la;
+    const long *f_off;
+    const double *frm, *fcm;  // F row-major / column-major
 };
 
-__global__ void __launch_bounds__(kBlock) k_nd_forward(const int *__restrict__ nodes, NDDev F,
+constexpr int kNdRows = 64;
+
+__global__ void __launch_bounds__(kBlock) k_nd_forward(const int2 *__restrict__ items, NDDev F,
                                                        const double *__restrict__ b, double *__restrict__ y,
-                                                       double *__restrict__ u, int max_front) {
-    extern __shared__ double sm[];
-    double *f = sm, *ys = sm + max_front;
-    int k = nodes[blockIdx.x];
+                                                       double *__restrict__ u) {
+    extern __shared__ double f[];
+    __shared__ double red[kWaves][kNdRows];
+    int2 it = items[blockIdx.x];
+    int k = it.x, r0 = it.y;
     int nc = F.ncol[k], nr = F.nrow[k], m = nc + nr, c0 = F.col0[k];
     for (int i = threadIdx.x; i < m; i += blockDim.x) f[i] = i < nc ? b[F.perm[c0 + i]] : 0.0;
     __syncthreads();
@@ -1292,72 +1297,66 @@ __global__ void __launch_bounds__(kBlock) k_nd_forward(const int *__restrict__ n
         for (int i = threadIdx.x; i < F.nrow[c]; i += blockDim.x) f[mp[i]] += uc[i];
         __syncthreads();
     }
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    const double *Li = F.linv + F.linv_off[k];
-    for (int i = w; i < nc; i += nw) {
-        const double *row = Li + (size_t)i * nc;
-        double s = 0;
-        for (int a = lane; a <= i; a += 64) s += row[a] * f[a];
-        s = wave_sum(s);
-        if (lane == 0) {
-            ys[i] = s;
-            y[c0 + i] = s;
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int i = r0 + lane;
+    bool act = i < m;
+    const double *Fc = F.fcm + F.f_off[k];  // element (i, a) at a * m + i
+    int per = (nc + kWaves - 1) / kWaves, a0 = w * per, a1 = min(nc, a0 + per);
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (act) {
+        int a = a0;
+        for (; a + 3 < a1; a += 4) {
+            s0 += Fc[(size_t)a * m + i] * f[a];
+            s1 += Fc[(size_t)(a + 1) * m + i] * f[a + 1];
+            s2 += Fc[(size_t)(a + 2) * m + i] * f[a + 2];
+            s3 += Fc[(size_t)(a + 3) * m + i] * f[a + 3];
         }
+        for (; a < a1; ++a) s0 += Fc[(size_t)a * m + i] * f[a];
     }
+    red[w][lane] = (s0 + s1) + (s2 + s3);
     __syncthreads();
-    const double *La = F.la + F.la_off[k];
-    double *uk = u + F.u_off[k];
-    for (int r = w; r < nr; r += nw) {
-        const double *row = La + (size_t)r * nc;
-        double s = 0;
-        for (int a = lane; a < nc; a += 64) s += row[a] * ys[a];
-        s = wave_sum(s);
-        if (lane == 0) uk[r] = f[nc + r] - s;
+    if (w == 0 && act) {
+        double t = 0;
+        for (int ww = 0; ww < kWaves; ++ww) t += red[ww][lane];
+        if (i < nc) y[c0 + i] = t;
+        else u[F.u_off[k] + i - nc] = f[i] + t;
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_nd_backward(const int *__restrict__ nodes, NDDev F,
+__global__ void __launch_bounds__(kBlock) k_nd_backward(const int2 *__restrict__ items, NDDev F,
                                                         const double *__restrict__ y, double *__restrict__ xp,
-                                                        double *__restrict__ out, int max_front) {
-    extern __shared__ double sm[];
-    __shared__ double red[kWaves][64];
-    int k = nodes[blockIdx.x];
-    int nc = F.ncol[k], nr = F.nrow[k], c0 = F.col0[k];
-    double *xa = sm, *z = sm + max_front;
+                                                        double *__restrict__ out) {
+    extern __shared__ double v[];
+    __shared__ double red[kWaves][kNdRows];
+    int2 it = items[blockIdx.x];
+    int k = it.x, r0 = it.y;
+    int nc = F.ncol[k], nr = F.nrow[k], m = nc + nr, c0 = F.col0[k];
     const int *rw = F.rows + F.rows_off[k];
-    for (int r = threadIdx.x; r < nr; r += blockDim.x) xa[r] = xp[rw[r]];
+    for (int j = threadIdx.x; j < m; j += blockDim.x) v[j] = j < nc ? y[c0 + j] : xp[rw[j - nc]];
     __syncthreads();
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    const double *La = F.la + F.la_off[k];
-    for (int a0 = 0; a0 < nc; a0 += 64) {
-        int a = a0 + lane;
-        double s = 0;
-        if (a < nc)
-            for (int r = w; r < nr; r += nw) s += La[(size_t)r * nc + a] * xa[r];
-        red[w][lane] = s;
-        __syncthreads();
-        if (w == 0 && a < nc) {
-            double t = y[c0 + a];
-            for (int ww = 0; ww < nw; ++ww) t -= red[ww][lane];
-            z[a] = t;
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int a = r0 + lane;
+    bool act = a < nc;
+    const double *Fr = F.frm + F.f_off[k];  // element (j, a) at j * nc + a
+    int per = (m + kWaves - 1) / kWaves, j0 = w * per, j1 = min(m, j0 + per);
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (act) {
+        int j = j0;
+        for (; j + 3 < j1; j += 4) {
+            s0 += Fr[(size_t)j * nc + a] * v[j];
+            s1 += Fr[(size_t)(j + 1) * nc + a] * v[j + 1];
+            s2 += Fr[(size_t)(j + 2) * nc + a] * v[j + 2];
+            s3 += Fr[(size_t)(j + 3) * nc + a] * v[j + 3];
         }
-        __syncthreads();
+        for (; j < j1; ++j) s0 += Fr[(size_t)j * nc + a] * v[j];
     }
-    const double *Li = F.linv + F.linv_off[k];
-    for (int a0 = 0; a0 < nc; a0 += 64) {
-        int a = a0 + lane;
-        double s = 0;
-        if (a < nc)
-            for (int i = a0 + w; i < nc; i += nw) s += Li[(size_t)i * nc + a] * z[i];  // zero above the diagonal
-        red[w][lane] = s;
-        __syncthreads();
-        if (w == 0 && a < nc) {
-            double t = 0;
-            for (int ww = 0; ww < nw; ++ww) t += red[ww][lane];
-            xp[c0 + a] = t;
-            out[F.perm[c0 + a]] = t;
-        }
-        __syncthreads();
+    red[w][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (w == 0 && act) {
+        double t = 0;
+        for (int ww = 0; ww < kWaves; ++ww) t += red[ww][lane];
+        xp[c0 + a] = t;
+        out[F.perm[c0 + a]] = t;
     }
 }
 
@@ -1543,8 +1542,8 @@ bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>>
     NDFactor F;
     std::string e = nd_factor(merged, P.cx, P.cy, P.cz, dim, F);
     if (!e.empty()) { err_ = e; return false; }
-    // LDS of the solve kernels: two vectors of the largest front
-    if ((size_t)F.max_front * 2 * sizeof(double) > 60 * 1024) {
+    // LDS of the solve kernels: one vector of the largest front
+    if ((size_t)F.max_front * sizeof(double) > 60 * 1024) {
         err_ = "nested-dissection front too large for the direct solve (use the PCG linear solver)";
         return false;
     }
@@ -1553,15 +1552,20 @@ bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>>
     for (int k = 0; k <= nt; ++k) { rows_off[k] = (int)F.rows_off[k]; u_off[k] = (int)F.u_off[k]; }
     int hmax = 0, dmax = 0;
     for (int k = 0; k < nt; ++k) { hmax = std::max(hmax, F.height[k]); dmax = std::max(dmax, F.depth[k]); }
-    std::vector<int> fwd, bwd;
+    // work items (supernode, first output row) per level, 64 rows each
+    std::vector<int2> fwd, bwd;
     f.fwd_ptr.assign(1, 0);
     for (int h = 0; h <= hmax; ++h) {
-        for (int k = 0; k < nt; ++k) if (F.height[k] == h) fwd.push_back(k);
+        for (int k = 0; k < nt; ++k)
+            if (F.height[k] == h)
+                for (int r = 0; r < F.ncol[k] + F.nrow[k]; r += kNdRows) fwd.push_back(make_int2(k, r));
         f.fwd_ptr.push_back((int)fwd.size());
     }
     f.bwd_ptr.assign(1, 0);
     for (int d = 0; d <= dmax; ++d) {
-        for (int k = 0; k < nt; ++k) if (F.depth[k] == d) bwd.push_back(k);
+        for (int k = 0; k < nt; ++k)
+            if (F.depth[k] == d)
+                for (int r = 0; r < F.ncol[k]; r += kNdRows) bwd.push_back(make_int2(k, r));
         f.bwd_ptr.push_back((int)bwd.size());
     }
     f.n = n;
@@ -1571,8 +1575,8 @@ bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>>
     return upload(f.perm, F.perm) && upload(f.col0, F.col0) && upload(f.ncol, F.ncol) && upload(f.nrow, F.nrow) &&
            upload(f.child_ptr, F.child_ptr) && upload(f.child_idx, F.child_idx) && upload(f.map, F.map) &&
            upload(f.map_off, map_off) && upload(f.rows, F.rows) && upload(f.rows_off, rows_off) &&
-           upload(f.u_off, u_off) && upload(f.linv_off, F.linv_off) && upload(f.la_off, F.la_off) &&
-           upload(f.linv, F.linv) && upload(f.la, F.la) && upload(f.fwd_nodes, fwd) && upload(f.bwd_nodes, bwd);
+           upload(f.u_off, u_off) && upload(f.f_off, F.f_off) && upload(f.frm, F.frm) && upload(f.fcm, F.fcm) &&
+           upload(f.fwd_items, fwd) && upload(f.bwd_items, bwd);
 }
 
 bool Solver::build() {
@@ -2248,19 +2252,19 @@ bool Solver::linsolve(const DevCSR &K, const DevND &f, double *d_b, double *d_x,
 
 bool Solver::nd_solve(const DevND &f, const double *d_b, double *d_x) {
     NDDev F{f.perm.p, f.col0.p, f.ncol.p, f.nrow.p, f.child_ptr.p, f.child_idx.p, f.map.p, f.map_off.p,
-            f.rows.p, f.rows_off.p, f.u_off.p, f.linv_off.p, f.la_off.p, f.linv.p, f.la.p};
-    size_t lds = (size_t)f.max_front * 2 * sizeof(double);
+            f.rows.p, f.rows_off.p, f.u_off.p, f.f_off.p, f.frm.p, f.fcm.p};
+    size_t lds = (size_t)f.max_front * sizeof(double);
     for (size_t h = 0; h + 1 < f.fwd_ptr.size(); ++h) {
         int cnt = f.fwd_ptr[h + 1] - f.fwd_ptr[h];
         if (cnt == 0) continue;
-        hipLaunchKernelGGL(k_nd_forward, dim3(cnt), dim3(kBlock), lds, s_, f.fwd_nodes.p + f.fwd_ptr[h], F, d_b,
-                           d_ndy.p, d_ndu.p, f.max_front);
+        hipLaunchKernelGGL(k_nd_forward, dim3(cnt), dim3(kBlock), lds, s_, f.fwd_items.p + f.fwd_ptr[h], F, d_b,
+                           d_ndy.p, d_ndu.p);
     }
     for (size_t d = 0; d + 1 < f.bwd_ptr.size(); ++d) {
         int cnt = f.bwd_ptr[d + 1] - f.bwd_ptr[d];
         if (cnt == 0) continue;
-        hipLaunchKernelGGL(k_nd_backward, dim3(cnt), dim3(kBlock), lds, s_, f.bwd_nodes.p + f.bwd_ptr[d], F, d_ndy.p,
-                           d_ndx.p, d_x, f.max_front);
+        hipLaunchKernelGGL(k_nd_backward, dim3(cnt), dim3(kBlock), lds, s_, f.bwd_items.p + f.bwd_ptr[d], F, d_ndy.p,
+                           d_ndx.p, d_x);
     }
     return check(hipGetLastError(), "nested-dissection solve");
 }

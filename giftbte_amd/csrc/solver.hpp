@@ -48,14 +48,15 @@ struct DevCSR {
 };
 
 // Nested-dissection Cholesky factor of the same matrix (ndchol.cpp) and the
-// supernodes of each elimination-tree height (forward) and depth (backward).
+// 64-row work items of each elimination-tree height (forward) and depth
+// (backward).
 struct DevND {
     int n = 0, max_front = 0, nodes = 0;
     DBuf<int> perm, col0, ncol, nrow, child_ptr, child_idx, map, map_off, rows, rows_off, u_off;
-    DBuf<long> linv_off, la_off;
-    DBuf<double> linv, la;
-    std::vector<int> fwd_ptr, bwd_ptr;  // level pointers into fwd_nodes / bwd_nodes
-    DBuf<int> fwd_nodes, bwd_nodes;
+    DBuf<long> f_off;
+    DBuf<double> frm, fcm;
+    std::vector<int> fwd_ptr, bwd_ptr;  // level pointers into fwd_items / bwd_items
+    DBuf<int2> fwd_items, bwd_items;    // (supernode, first row) per workgroup
     long u_total = 0;
 };
 
