@@ -84,7 +84,7 @@ class BTESolver:
     """
 
     def __init__(self, problem: Problem, precision: str = "fp64", device: int = 0, verbose: bool = False,
-                 macro_rtol: float = 0.0, linear_solver: str = "direct"):
+                 macro_rtol: float = 0.0, linear_solver: str = "direct", fallback_kernels: bool = False):
         self.problem = problem
         self.lib = problem.lib
         t = problem.tables
@@ -93,7 +93,8 @@ class BTESolver:
             raise ValueError("linear_solver must be 'direct' or 'pcg'")
         opts = _lib.Options(_lib.GBTE_FP64 if precision == "fp64" else _lib.GBTE_FP32_SWEEP, device, int(verbose),
                             macro_rtol,
-                            _lib.GBTE_LINSOLVE_DIRECT if linear_solver == "direct" else _lib.GBTE_LINSOLVE_PCG)
+                            _lib.GBTE_LINSOLVE_DIRECT if linear_solver == "direct" else _lib.GBTE_LINSOLVE_PCG,
+                            int(bool(fallback_kernels)))
         self._h = C.c_void_p()
         rc = self.lib.gbte_create(problem.handle, C.byref(opts), C.byref(self._h))
         if rc != 0:

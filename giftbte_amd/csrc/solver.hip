@@ -1745,7 +1745,7 @@ bool Solver::build() {
         pc_nchunk = (nB + pc_bc - 1) / pc_bc;
         pc_ring = 1;
         while ((size_t)pc_ring * 2 * pc_bc * sizeof(double) <= 80 * 1024) pc_ring *= 2;
-        if (pc_bc <= kPcBc && kPcDepth * kPcCap <= pc_ring) {
+        if (!opt.fallback_kernels && pc_bc <= kPcBc && kPcDepth * kPcCap <= pc_ring) {
             std::vector<PcRec> prec(NI);
             std::vector<int> psl, pfp, pdsl(1, 0), pfar;
             std::vector<int> freq(nc, 0), touched;
@@ -2143,8 +2143,9 @@ bool Solver::build() {
     // ---- LDS tiles of the face / vertex / moment kernel
     {
         Tiles T;
-        std::string e = build_tiles(p, h_gptr, h_gidx, h_vptr, h_vsrc, h_vwt, h_fkind, h_fother, h_fv0, h_fv1,
-                                    kTileRows, kTileCells, T);
+        std::string e = opt.fallback_kernels ? std::string("fallback kernels requested")
+                                             : build_tiles(p, h_gptr, h_gidx, h_vptr, h_vsrc, h_vwt, h_fkind, h_fother,
+                                                           h_fv0, h_fv1, kTileRows, kTileCells, T);
         if (e.find("mismatch") != std::string::npos) { err_ = "internal error: " + e; return false; }
         use_tiles = e.empty();  // otherwise (a stencil too large for LDS) the row kernels run
         if (use_tiles) {

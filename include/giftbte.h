@@ -99,6 +99,9 @@ typedef struct gbte_options {
     int verbose;          /* print the reference's per-iteration log lines      */
     double macro_rtol;    /* relative residual of the PCG linear solver (0: default) */
     int linear_solver;    /* GBTE_LINSOLVE_DIRECT (default) or GBTE_LINSOLVE_PCG */
+    int fallback_kernels; /* 1: use the kernels meant for decks the fast ones do not
+                             fit (level-synchronous sweep, untiled face kernels);
+                             same results, for testing those paths */
 } gbte_options;
 
 typedef struct gbte_stats {

@@ -16,24 +16,30 @@ FIELD_TOL = 1e-9
 KEFF_TOL = 1e-8
 
 
-def _gpu_run(deck_dir, s, linear_solver="direct"):
+def _gpu_run(deck_dir, s, variant="direct"):
     from giftbte_amd import Problem, BTESolver
     p = Problem(deck_dir)
-    sol = BTESolver(p, linear_solver=linear_solver)
+    if variant == "fallback":
+        sol = BTESolver(p, fallback_kernels=True)
+    else:
+        sol = BTESolver(p, linear_solver=variant)
     fp = sol.fourier()
     st = sol.solve(s["max_iter"], s["tol_temp"], s["tol_flux"])
     return p, sol, fp, st
 
 
-CASES = [(n, "direct") for n in decklib.DECKS] + [("porous_b4", "pcg")]
+# default kernels on every deck; PCG linear solver and the fallback kernels
+# (level-synchronous sweep, untiled face kernels) on two decks
+CASES = ([(n, "direct") for n in decklib.DECKS] + [("porous_b4", "pcg")] +
+         [("porous_b4", "fallback"), ("square_b20_diffuse", "fallback")])
 
 
-@pytest.mark.parametrize("name,linear_solver", CASES)
-def test_fields_match_oracle(name, linear_solver, deck_root):
+@pytest.mark.parametrize("name,variant", CASES)
+def test_fields_match_oracle(name, variant, deck_root):
     d = decklib.make(name, deck_root)
     s = decklib.settings(name)
-    print(f"\n[gpu parity] deck={name} linear_solver={linear_solver} settings={s}", flush=True)
-    p, sol, fp, st = _gpu_run(d, s, linear_solver)
+    print(f"\n[gpu parity] deck={name} variant={variant} settings={s}", flush=True)
+    p, sol, fp, st = _gpu_run(d, s, variant)
     ref, info = _oracle.run(p.tables, s["max_iter"], s["tol_temp"], s["tol_flux"])
     assert st.fourier_passes == info["fourier_passes"]
     assert st.iterations == info["iterations"]
