@@ -819,9 +819,9 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
 constexpr int kTileW = 64;
 constexpr int kTileBlock = 256;
 constexpr int kTileWaves = kTileBlock / 64;
-constexpr int kTileCellsPerWave = 4;
+constexpr int kTileCellsPerWave = 2;
 constexpr int kTileCells = kTileWaves * kTileCellsPerWave;  // cells per tile
-constexpr int kTileRows = 128;                               // rows per tile: 64 KiB of LDS
+constexpr int kTileRows = 64;                                // rows per tile: 32 KiB of LDS
 constexpr int kStageBatch = 16;                              // staged rows in flight per lane
 
 struct TileDev {
@@ -836,7 +836,7 @@ __device__ __forceinline__ double lane_dbl(double v, int l) {
                             __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
-__global__ void __launch_bounds__(kTileBlock) k_tile_face(const double *__restrict__ e, const double *__restrict__ eb,
+__global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__restrict__ e, const double *__restrict__ eb,
                                                           double *__restrict__ grad, Modes m, TileDev T, double invLx,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
