@@ -273,6 +273,7 @@ __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__
     }
     DirBand db(t0, nB);
     double vb = m.vg[db.b], tau = m.tau[db.b], s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
+#pragma unroll 2  // two entries' loads in flight
     for (int t = t0; t < DB; t += R.tpr) {
         if (R.bstep != 0) {
             vb = m.vg[db.b];
@@ -421,7 +422,10 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_dir(
 // Twelve consumer waves read only LDS (staged operands and the ring of
 // recent values) and store e.  Far values are final when the producers read
 // them: the host guarantees kPcDepth * kPcCap <= R, so a far position lies
-// in a sub-level finished before the producers' read.
+// in a sub-level finished before the producers' read.  The band chunks of one
+// direction run the same schedule at about the same pace and write the same
+// cells' e rows, so they are placed on one XCD, where those partial-line
+// writes merge in its L2.
 constexpr int kPcBlock = 1024;
 constexpr int kPcProd = 256;    // producer lanes (4 waves)
 constexpr int kPcCap = 128;     // positions per sub-level (2 producer lanes each)
@@ -455,7 +459,8 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
                                                        double *__restrict__ e, Modes m, int nc, int nB, int nD, int Bc,
                                                        int nchunk, int R) {
     extern __shared__ double lds[];
-    int d = blockIdx.x / nchunk, chunk = blockIdx.x - d * nchunk;
+    int bid = gridDim.x % kXcd == 0 ? xcd_block() : (int)blockIdx.x;
+    int d = bid / nchunk, chunk = bid - d * nchunk;
     int b0 = chunk * Bc;
     int sl0 = P.dsl[d], Lsub = P.dsl[d + 1] - sl0 - 1;
     double *ring = lds;
