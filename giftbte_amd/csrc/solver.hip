@@ -813,48 +813,54 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
 }
 
 // LDS-staged variant of k_vertex_e + k_face_moments over cell tiles
-// (tiles.cpp): one workgroup per tile walks the row in 64-entry slices.  For
+// (tiles.cpp).  One workgroup per tile walks the row in 64-entry slices.  For
 // each slice the tile's neighbour rows [cells | boundary entries] are staged
-// in LDS, the inverse-distance values of the tile's face vertices are formed
-// there, and every cell of the tile then reads its stencil, face neighbours
-// and vertex values from LDS.  Lane t of a wave owns entry slice*64 + t, so
-// the mode constants of its (direction, band) sit in registers.  Each wave
-// keeps its cells' lane-indexed records and moment sums in registers over all
-// slices and reduces the moments once at the end.
+// in LDS and the inverse-distance values of the tile's face vertices are
+// formed there.  The two linear maps of the slice then run on the matrix
+// cores (v_mfma_f64_16x16x4f64):
+//   LSQ gradient  G[cell + 8 comp][t] = sum_r C[cell + 8 comp][r] E[r][t]
+//                 over the tile's cell rows, one 16-entry block per wave;
+//   moments       Y[j][y] += sum_t K_j(t) E[yrow[y]][t], accumulated in the
+//                 matrix registers over all slices, with K_j(t) the
+//                 per-(direction, band) factors of T, q and of the residual's
+//                 face terms (a table built with the modes).
+// After the last slice each cell's T, q and non-Fourier residual are short
+// combinations of Y entries (tiles.hpp).
 constexpr int kTileW = 64;
+constexpr int kTileStride = kTileW + 1;  // LDS row stride: the moment product reads 16 rows of one column
 constexpr int kTileBlock = 256;
 constexpr int kTileWaves = kTileBlock / 64;
-constexpr int kTileCellsPerWave = 2;
-constexpr int kTileCells = kTileWaves * kTileCellsPerWave;  // cells per tile
-constexpr int kTileRows = 64;                                // rows per tile: 32 KiB of LDS
-constexpr int kStageBatch = 16;                              // staged rows in flight per lane
+constexpr int kTileCells = kTileCellsMax;  // cells per tile
+constexpr int kTileRows = kTileRowsMax;    // rows per tile: 33 KiB of LDS
+constexpr int kStageBatch = 16;            // staged rows in flight per lane
+constexpr int kTileJ = 8;                  // moment factors K_0..K_7
+
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 struct TileDev {
-    const int *cell0, *rptr, *rsrc, *vptr, *vs_ptr, *vs_row, *crec_i;
-    const double *vs_w, *crec_a, *crec_b;
+    const int *cell0, *rptr, *rsrc, *nh, *vptr, *vs_ptr, *vs_row, *ny, *yrow, *fin_i;
+    const double *vs_w, *cmf, *kwf, *fin_d;
 };
 
-// lane l of a wave-uniform record vector, as a scalar
 __device__ __forceinline__ int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-__device__ __forceinline__ double lane_dbl(double v, int l) {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
-                            __builtin_amdgcn_readlane(__double2loint(v), l));
-}
 
 __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__restrict__ e, const double *__restrict__ eb,
-                                                          double *__restrict__ grad, Modes m, TileDev T, double invLx,
+                                                          double *__restrict__ grad, TileDev T,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
-                                                          double *__restrict__ RMo, int DB, int nB, int ntiles,
-                                                          int vmeta_off) {
+                                                          double *__restrict__ RMo, int DB, int ntiles, int vmeta_off) {
     extern __shared__ double lds[];
     int tile = xcd_block();
     if (tile >= ntiles) return;  // whole workgroup
-    int lane = threadIdx.x & 63;
+    int lane = threadIdx.x & 63, l16 = lane & 15, lq = lane >> 4;
     int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int r0 = T.rptr[tile], nr = T.rptr[tile + 1] - r0;
     int v0 = T.vptr[tile], nv = T.vptr[tile + 1] - v0;
     int c0 = T.cell0[tile], ncell = T.cell0[tile + 1] - c0;
+    int nks = (T.nh[tile] + 3) >> 2;  // LSQ k-steps: rows [0, 4 nks)
+    int ny = T.ny[tile];
+    int wpb = ny <= 16 ? 4 : ny <= 32 ? 2 : 1;  // waves per 16-row block of Y
+    int yb = w / wpb, nys = kTileKs / wpb, ys0 = (w - yb * wpb) * nys;
     // vertex stencils of the tile in LDS: ptr[nv + 1] | rows | weights
     int vs0 = T.vs_ptr[v0], nvs = T.vs_ptr[v0 + nv] - vs0;
     int *vm_ptr = reinterpret_cast<int *>(lds + vmeta_off);
@@ -865,36 +871,28 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
         vm_row[i] = T.vs_row[vs0 + i];
         vm_w[i] = T.vs_w[vs0 + i];
     }
-    // this wave's cells: records and moment sums
-    int ri[kTileCellsPerWave];
-    double ra[kTileCellsPerWave], rb[kTileCellsPerWave];
-    double mT[kTileCellsPerWave], mx[kTileCellsPerWave], my[kTileCellsPerWave], mz[kTileCellsPerWave],
-        mR[kTileCellsPerWave];
+    // rows [nr + nv, 4 nks) enter the LSQ product with zero weight and are
+    // written by nothing else: clear them once
+    for (int r = nr + nv + w; r < 4 * nks; r += kTileWaves) lds[r * kTileStride + lane] = 0.0;
+    // LSQ coefficients (A operands), the same for every slice
+    double cm[kTileKs];
+    const double *cmt = T.cmf + (size_t)tile * kTileKs * 64 + lane;
 #pragma unroll
-    for (int q = 0; q < kTileCellsPerWave; ++q) {
-        int i = w + kTileWaves * q;
-        size_t o = (size_t)(c0 + (i < ncell ? i : 0)) * kRec + lane;
-        ri[q] = T.crec_i[o];
-        ra[q] = T.crec_a[o];
-        rb[q] = T.crec_b[o];
-        mT[q] = 0;
-        mx[q] = 0;
-        my[q] = 0;
-        mz[q] = 0;
-        mR[q] = 0;
-    }
+    for (int k = 0; k < kTileKs; ++k) cm[k] = k < nks ? cmt[k * 64] : 0.0;
+    // B operands: Y reads 16 rows of one column, G 4 rows of 16 columns
+    const double *yrd = lds + T.yrow[(size_t)tile * kYRows + yb * 16 + l16] * kTileStride + lq;
+    const double *grd = lds + lq * kTileStride + 16 * w + l16;
+    v4d accY = {0.0, 0.0, 0.0, 0.0};
     int src0 = lane < nr ? T.rsrc[r0 + lane] : 0;
-    int src1 = lane + kRec < nr ? T.rsrc[r0 + kRec + lane] : 0;
     int nslice = (DB + kTileW - 1) / kTileW;
     for (int slice = 0; slice < nslice; ++slice) {
         int t = slice * kTileW + lane;
-        bool in = t < DB;
-        int tt = in ? t : 0;
-        int d = tt / nB, b = tt - d * nB;
-        double wd = in ? m.w[d] : 0.0, vb = m.vg[b];
-        double vx = vb * m.sx[d], vy = vb * m.sy[d], vz = vb * m.sz[d];
-        double tau = m.tau[b], lrc = m.lrc[b], kl = m.kappa * lrc;
-        bool hr = m.hr[b] > 0;
+        int tt = t < DB ? t : 0;
+        // this wave's K fragments of the slice
+        double kw[kTileKs];
+        const double *kws = T.kwf + ((size_t)slice * kTileKs + ys0) * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < kTileKs; ++s) kw[s] = s < nys ? kws[s * 64] : 0.0;
         __syncthreads();  // the previous slice is done with the LDS rows
         for (int r = w; r < nr; r += kStageBatch * kTileWaves) {
             double x[kStageBatch];
@@ -903,7 +901,7 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
                 int rr = r + j * kTileWaves;
                 x[j] = 0;
                 if (rr < nr) {
-                    int src = rr < kRec ? lane_int(src0, rr) : lane_int(src1, rr - kRec);
+                    int src = lane_int(src0, rr);
                     const double *row = src >= 0 ? e + (size_t)src * DB : eb + (size_t)(-src - 1) * DB;
                     x[j] = row[tt];
                 }
@@ -911,7 +909,7 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
 #pragma unroll
             for (int j = 0; j < kStageBatch; ++j) {
                 int rr = r + j * kTileWaves;
-                if (rr < nr) lds[rr * kTileW + lane] = x[j];
+                if (rr < nr) lds[rr * kTileStride + lane] = x[j];
             }
         }
         __syncthreads();
@@ -919,100 +917,77 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
             int a = vm_ptr[i], z = vm_ptr[i + 1];  // z - a is a multiple of 4
             double sm = 0;
             for (int k = a; k < z; k += 4) {
-                double u0 = lds[vm_row[k] * kTileW + lane], u1 = lds[vm_row[k + 1] * kTileW + lane];
-                double u2 = lds[vm_row[k + 2] * kTileW + lane], u3 = lds[vm_row[k + 3] * kTileW + lane];
+                double u0 = lds[vm_row[k] * kTileStride + lane], u1 = lds[vm_row[k + 1] * kTileStride + lane];
+                double u2 = lds[vm_row[k + 2] * kTileStride + lane], u3 = lds[vm_row[k + 3] * kTileStride + lane];
                 sm += u0 * vm_w[k];
                 sm += u1 * vm_w[k + 1];
                 sm += u2 * vm_w[k + 2];
                 sm += u3 * vm_w[k + 3];
             }
             if (isnan(sm)) sm = 0;
-            lds[(nr + i) * kTileW + lane] = sm;
+            lds[(nr + i) * kTileStride + lane] = sm;
         }
         __syncthreads();
+        // moments: k-steps [ys0, ys0 + nys) of row block yb
 #pragma unroll
-        for (int q = 0; q < kTileCellsPerWave; ++q) {
-            int i = w + kTileWaves * q;
-            if (i >= ncell) break;
-            int c = c0 + i;
-            int ci = ri[q];
-            double ca = ra[q], cb = rb[q];
-            double x = lds[lane_int(ci, 40) * kTileW + lane];
-            mT[q] += lrc * x * wd;
-            mx[q] += vx * wd * x;
-            my[q] += vy * wd * x;
-            mz[q] += vz * wd * x;
-            // face gradients: the 3 LDS reads of every face are issued together
-            int nf = lane_int(ci, 43);
-            double fo[kMaxFace2D], f0[kMaxFace2D], f1[kMaxFace2D];
+        for (int s = 0; s < kTileKs; ++s)
+            if (s < nys) accY = __builtin_amdgcn_mfma_f64_16x16x4f64(kw[s], yrd[4 * (ys0 + s)], accY, 0, 0, 0);
+        // LSQ gradient of entries [16 w, 16 w + 16) of the slice
+        v4d g = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int j = 0; j < kMaxFace2D; ++j) {
-                bool on = j < nf && lane_int(ci, 24 + j) != 0;
-                fo[j] = lds[(on ? lane_int(ci, 28 + j) : 0) * kTileW + lane];
-                f0[j] = lds[(on ? lane_int(ci, 32 + j) : 0) * kTileW + lane];
-                f1[j] = lds[(on ? lane_int(ci, 36 + j) : 0) * kTileW + lane];
-            }
-            if (hr) {
-                double acc = 0;
+        for (int k = 0; k < kTileKs; ++k)
+            if (k < nks) g = __builtin_amdgcn_mfma_f64_16x16x4f64(cm[k], grd[4 * k * kTileStride], g, 0, 0, 0);
+        // lane holds rows lq + 4 i: (cell lq, x), (cell lq + 4, x), (cell lq, y), (cell lq + 4, y)
+        int tg = slice * kTileW + 16 * w + l16;
+        if (tg < DB) {
 #pragma unroll
-                for (int j = 0; j < kMaxFace2D; ++j) {
-                    int kd = j < nf ? lane_int(ci, 24 + j) : 0;
-                    if (kd == 0) continue;
-                    double o = fo[j];
-                    double dv = f1[j] - f0[j];
-                    double de = kd == 3 ? x - o : o - x;
-                    double gn = de * lane_dbl(ca, 24 + j);
-                    double gx = gn * lane_dbl(ca, 28 + j) + dv * lane_dbl(ca, 36 + j);
-                    double gy = gn * lane_dbl(ca, 32 + j) + dv * lane_dbl(ca, 40 + j);
-                    double nx = lane_dbl(ca, 44 + j), ny = lane_dbl(ca, 48 + j);
-                    double vn = vx * nx + vy * ny;
-                    acc += lane_dbl(ca, 52 + j) * (kl * (nx * gx + ny * gy) - tau * (vx * gx + vy * gy) * vn);
-                }
-                mR[q] += wd * acc;
-            }
-            // LSQ gradient, 4 stencil rows per batch (stencils are padded to
-            // multiples of 4 with the cell itself at weight 0)
-            double gx = 0, gy = 0;
-            if (lane_int(ci, 41)) {
-                int nst = lane_int(ci, 42);
-                for (int k = 0; k < nst; k += 4) {
-                    double u0 = lds[lane_int(ci, k) * kTileW + lane], u1 = lds[lane_int(ci, k + 1) * kTileW + lane];
-                    double u2 = lds[lane_int(ci, k + 2) * kTileW + lane], u3 = lds[lane_int(ci, k + 3) * kTileW + lane];
-                    double d0 = invLx * (u0 - x), d1 = invLx * (u1 - x), d2 = invLx * (u2 - x), d3 = invLx * (u3 - x);
-                    gx += lane_dbl(ca, k) * d0;
-                    gy += lane_dbl(cb, k) * d0;
-                    gx += lane_dbl(ca, k + 1) * d1;
-                    gy += lane_dbl(cb, k + 1) * d1;
-                    gx += lane_dbl(ca, k + 2) * d2;
-                    gy += lane_dbl(cb, k + 2) * d2;
-                    gx += lane_dbl(ca, k + 3) * d3;
-                    gy += lane_dbl(cb, k + 3) * d3;
-                }
+            for (int h = 0; h < 2; ++h) {
+                int cl = lq + 4 * h;
+                double gx = g[h], gy = g[2 + h];
                 if (isnan(gx) || isnan(gy)) {
                     gx = 0;
                     gy = 0;
                 }
-            }
-            if (in) {
-                grad[(size_t)c * 2 * DB + t] = gx;
-                grad[((size_t)c * 2 + 1) * DB + t] = gy;
+                if (cl < ncell) {
+                    grad[(size_t)(c0 + cl) * 2 * DB + tg] = gx;
+                    grad[((size_t)(c0 + cl) * 2 + 1) * DB + tg] = gy;
+                }
             }
         }
     }
+    // Y partials (lane holds Y[lq + 4 i][yb * 16 + l16]) -> LDS, summed over
+    // the waves of a row block
+    __syncthreads();
+    double *Yp = lds, *Y = lds + kTileWaves * 16 * kTileJ;
 #pragma unroll
-    for (int q = 0; q < kTileCellsPerWave; ++q) {
-        int i = w + kTileWaves * q;
-        if (i >= ncell) break;
-        double a0 = wave_sum(mT[q]), a1 = wave_sum(mx[q]), a2 = wave_sum(my[q]), a3 = wave_sum(mz[q]),
-               a4 = wave_sum(mR[q]);
-        if (lane == 0) {
-            int c = c0 + i;
-            To[c] = a0;
-            qxo[c] = a1;
-            qyo[c] = a2;
-            qzo[c] = a3;
-            RMo[c] = a4;
+    for (int i = 0; i < 2; ++i) Yp[(w * 16 + l16) * kTileJ + lq + 4 * i] = accY[i];
+    __syncthreads();
+    int nyr = (kTileWaves / wpb) * 16;
+    for (int idx = threadIdx.x; idx < nyr * kTileJ; idx += kTileBlock) {
+        int y = idx / kTileJ, j = idx - y * kTileJ, blk = y >> 4;
+        double sum = 0;
+        for (int u = 0; u < wpb; ++u) sum += Yp[((blk * wpb + u) * 16 + (y & 15)) * kTileJ + j];
+        Y[idx] = sum;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncell) {
+        int c = c0 + threadIdx.x;
+        const int *fi = T.fin_i + (size_t)c * kFinI;
+        const double *fd = T.fin_d + (size_t)c * kFinD;
+        const double *yo = Y + fi[0] * kTileJ;
+        To[c] = yo[4];
+        qxo[c] = yo[5];
+        qyo[c] = yo[6];
+        qzo[c] = yo[7];
+        double r = 0;
+        int nf = fi[1];
+        for (int f = 0; f < nf; ++f) {
+            const double *ya = Y + fi[2 + 3 * f] * kTileJ, *ya0 = Y + fi[3 + 3 * f] * kTileJ,
+                         *ya1 = Y + fi[4 + 3 * f] * kTileJ;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) r += fd[8 * f + j] * (ya[j] - yo[j]) + fd[8 * f + 4 + j] * (ya1[j] - ya0[j]);
         }
+        RMo[c] = r;
     }
 }
 
@@ -2156,7 +2131,7 @@ bool Solver::build() {
         if (use_tiles) {
             std::vector<const std::vector<double> *> face8 = {&h_fga, &h_fn0x, &h_fn0y, &h_fmxm,
                                                               &h_fmym, &p.face_nx, &p.face_ny, &p.face_area};
-            std::string e2 = pack_tile_records(p, h_gptr, h_gP, h_gok, h_fkind, face8, T);
+            std::string e2 = pack_tile_mfma(p, h_gptr, h_gP, h_gok, h_fkind, face8, 1.0 / P.deck.Lx, T);
             use_tiles = e2.empty();
         }
         if (use_tiles) {
@@ -2168,15 +2143,46 @@ bool Solver::build() {
                 int nv = T.vptr[t + 1] - T.vptr[t], nvs = T.vs_ptr[T.vptr[t + 1]] - T.vs_ptr[T.vptr[t]];
                 max_meta = std::max(max_meta, (nv + 1 + nvs + 1) / 2 + nvs);
             }
-            tile_lds = (size_t)(tile_rows * kTileW + max_meta) * sizeof(double);
+            tile_lds = (size_t)(kTileRows * kTileStride + max_meta) * sizeof(double);
             if (tile_lds > 80 * 1024) use_tiles = false;
         }
         if (use_tiles) {
+            // K_j(t) of the moment product, as A fragments per slice and k-step:
+            // lane l holds K_{l % 16}(64 slice + 4 k + l / 16), zero for j >= 8
+            // and past the row
+            int DB = nD * nB, nslice = (DB + kTileW - 1) / kTileW;
+            std::vector<double> kwf((size_t)nslice * kTileKs * 64, 0.0);
+            double kap = P.kappa_bulk[0];
+            for (int sl = 0; sl < nslice; ++sl)
+                for (int k = 0; k < kTileKs; ++k)
+                    for (int l = 0; l < 64; ++l) {
+                        int j = l % 16, t = sl * kTileW + 4 * k + l / 16;
+                        if (j >= kTileJ || t >= DB) continue;
+                        int d = t / nB, b = t - d * nB;
+                        double wd = w[d], vb = vg[b], h = hr[b] > 0 ? 1.0 : 0.0, tv2 = tau[b] * vb * vb;
+                        double K[kTileJ] = {h * wd * kap * lrc[b],        h * wd * tv2 * sx[d] * sx[d],
+                                            h * wd * tv2 * sx[d] * sy[d], h * wd * tv2 * sy[d] * sy[d],
+                                            wd * lrc[b],                  wd * vb * sx[d],
+                                            wd * vb * sy[d],              wd * vb * sz[d]};
+                        kwf[((size_t)sl * kTileKs + k) * 64 + l] = K[j];
+                    }
             if (!upload(d_t_cell0, T.cell0) || !upload(d_t_rptr, T.rptr) || !upload(d_t_rsrc, T.rsrc) ||
-                !upload(d_t_vptr, T.vptr) || !upload(d_t_vs_ptr, T.vs_ptr) || !upload(d_t_vs_row, T.vs_row) ||
-                !upload(d_t_vs_w, T.vs_w) || !upload(d_t_crec_i, T.crec_i) || !upload(d_t_crec_a, T.crec_a) ||
-                !upload(d_t_crec_b, T.crec_b))
+                !upload(d_t_nh, T.nh) || !upload(d_t_vptr, T.vptr) || !upload(d_t_vs_ptr, T.vs_ptr) ||
+                !upload(d_t_vs_row, T.vs_row) || !upload(d_t_vs_w, T.vs_w) || !upload(d_t_ny, T.ny) ||
+                !upload(d_t_yrow, T.yrow) || !upload(d_t_fin_i, T.fin_i) || !upload(d_t_cmf, T.cmf) ||
+                !upload(d_t_kwf, kwf) || !upload(d_t_fin_d, T.fin_d))
                 return false;
+            if (opt.verbose) {
+                int ymax = 0;
+                long ysum = 0, hsum = 0;
+                for (int t = 0; t < n_tiles; ++t) {
+                    ymax = std::max(ymax, T.ny[t]);
+                    ysum += T.ny[t];
+                    hsum += T.nh[t];
+                }
+                printf("[gbte] %d tiles: mean %.1f cell rows, mean %.1f / max %d moment rows\n", n_tiles,
+                       (double)hsum / n_tiles, (double)ysum / n_tiles, ymax);
+            }
         }
         std::vector<int>().swap(h_gptr);
         std::vector<int>().swap(h_gidx);
@@ -2430,11 +2436,11 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
         (void)hipEventRecord(ev_[3], s_);
         if (use_tiles) {
-            TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_vptr.p, d_t_vs_ptr.p, d_t_vs_row.p, d_t_crec_i.p,
-                       d_t_vs_w.p, d_t_crec_a.p, d_t_crec_b.p};
+            TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p, d_t_vs_row.p,
+                       d_t_ny.p,    d_t_yrow.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p,   d_t_kwf.p,     d_t_fin_d.p};
             hipLaunchKernelGGL(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, s_, d_e.p, d_eb.p,
-                               d_grad.p, m, td, invLx, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nD * nB, nB, n_tiles,
-                               tile_rows * kTileW);
+                               d_grad.p, td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nD * nB, n_tiles,
+                               kTileRows * kTileStride);
         } else {
             hipLaunchKernelGGL(one_row ? k_vertex_e<true> : k_vertex_e<false>, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p,
                                d_eV.p, d_vptr.p, d_vsrc.p, d_vwt.p, nn, R);

@@ -74,6 +74,7 @@ std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const st
         for (int x : S.H) T.rsrc.push_back(x);
         for (int x : S.B) T.rsrc.push_back(-x - 1);
         T.rptr.push_back((int)T.rsrc.size());
+        T.nh.push_back(nh);
         int base_v = nh + nbr;
         for (int v : S.V) {
             T.vnode.push_back(v);
@@ -163,38 +164,90 @@ std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const st
     return "";
 }
 
-std::string pack_tile_records(const Problem &p, const std::vector<int> &gptr, const std::vector<double> &gP,
-                              const std::vector<int> &gok, const std::vector<int> &fkind,
-                              const std::vector<const std::vector<double> *> &face8, Tiles &T) {
+std::string pack_tile_mfma(const Problem &p, const std::vector<int> &gptr, const std::vector<double> &gP,
+                           const std::vector<int> &gok, const std::vector<int> &fkind,
+                           const std::vector<const std::vector<double> *> &face8, double invL, Tiles &T) {
     int nc = p.n_cells;
     size_t nnz = gptr[nc];
-    T.crec_i.assign((size_t)nc * kRec, 0);
-    T.crec_a.assign((size_t)nc * kRec, 0.0);
-    T.crec_b.assign((size_t)nc * kRec, 0.0);
-    for (int c = 0; c < nc; ++c) {
-        int *ri = &T.crec_i[(size_t)c * kRec];
-        double *ra = &T.crec_a[(size_t)c * kRec], *rb = &T.crec_b[(size_t)c * kRec];
-        int nst = gptr[c + 1] - gptr[c];
-        if (nst > kRecSt) return "LSQ stencil longer than a tile record";
-        int nf = p.face_size[c];
-        if (nf > 4) return "more than 4 faces per cell in a tile record";
-        for (int k = 0; k < nst; ++k) {
-            ri[k] = T.st_row[gptr[c] + k];
-            ra[k] = gP[gptr[c] + k];
-            rb[k] = gP[nnz + gptr[c] + k];
+    T.cmf.assign((size_t)T.n_tiles * kTileKs * 64, 0.0);
+    T.ny.assign(T.n_tiles, 0);
+    T.yrow.assign((size_t)T.n_tiles * kYRows, 0);
+    T.fin_i.assign((size_t)nc * kFinI, 0);
+    T.fin_d.assign((size_t)nc * kFinD, 0.0);
+    const std::vector<double> &ga = *face8[0], &n0x = *face8[1], &n0y = *face8[2], &mxm = *face8[3],
+                              &mym = *face8[4], &fnx = *face8[5], &fny = *face8[6], &area = *face8[7];
+    for (int t = 0; t < T.n_tiles; ++t) {
+        int c0 = T.cell0[t], ncell = T.cell0[t + 1] - c0, nh = T.nh[t];
+        if (ncell > kTileCellsMax || nh > kTileRowsMax) return "tile exceeds the matrix-core shape";
+        // C[m][r], m = cell + 8 * component
+        std::vector<double> C((size_t)16 * kTileRowsMax, 0.0);
+        for (int i = 0; i < ncell; ++i) {
+            int c = c0 + i;
+            if (!gok[c]) continue;
+            int own = T.crow[c];
+            for (int q = 0; q < 2; ++q) {
+                double *Cm = &C[(size_t)(i + 8 * q) * kTileRowsMax];
+                double sum = 0;
+                for (int k = gptr[c]; k < gptr[c + 1]; ++k) {
+                    double wgt = invL * gP[q * nnz + k];
+                    Cm[T.st_row[k]] += wgt;
+                    sum += wgt;
+                }
+                Cm[own] -= sum;
+            }
         }
-        for (int j = 0; j < nf; ++j) {
-            size_t s = (size_t)c * 6 + j;
-            ri[24 + j] = fkind[s];
-            ri[28 + j] = T.f_orow[s];
-            ri[32 + j] = T.f_v0row[s];
-            ri[36 + j] = T.f_v1row[s];
-            for (int q = 0; q < 8; ++q) ra[24 + 4 * q + j] = (*face8[q])[s];
+        double *F = &T.cmf[(size_t)t * kTileKs * 64];
+        for (int k = 0; k < kTileKs; ++k)
+            for (int l = 0; l < 64; ++l) {
+                int r = 4 * k + l / 16;
+                F[k * 64 + l] = r < nh ? C[(size_t)(l % 16) * kTileRowsMax + r] : 0.0;
+            }
+        // moment rows: own cells first, then face neighbours / entries and face vertices
+        std::vector<int> ylist;
+        std::unordered_map<int, int> ypos;
+        auto addY = [&](int r) {
+            auto it = ypos.find(r);
+            if (it != ypos.end()) return it->second;
+            ypos[r] = (int)ylist.size();
+            ylist.push_back(r);
+            return (int)ylist.size() - 1;
+        };
+        for (int i = 0; i < ncell; ++i) addY(T.crow[c0 + i]);
+        for (int i = 0; i < ncell; ++i) {
+            int c = c0 + i;
+            int nf = p.face_size[c];
+            if (nf > 4) return "more than 4 faces per cell in a tile";
+            int *fi = &T.fin_i[(size_t)c * kFinI];
+            double *fd = &T.fin_d[(size_t)c * kFinD];
+            int yo = ypos.at(T.crow[c]);
+            fi[0] = yo;
+            fi[1] = nf;
+            for (int j = 0; j < nf; ++j) {
+                size_t s = (size_t)c * 6 + j;
+                int kd = fkind[s];
+                fi[2 + 3 * j] = fi[3 + 3 * j] = fi[4 + 3 * j] = yo;
+                if (kd == 0) continue;
+                fi[2 + 3 * j] = addY(T.f_orow[s]);
+                fi[3 + 3 * j] = addY(T.f_v0row[s]);
+                fi[4 + 3 * j] = addY(T.f_v1row[s]);
+                // face term area (kl n.g - tau (v.g)(v.n)), g = de ga n0 + dv m,
+                // de = sg (other - own), dv = v1 - v0, v = vg s; its
+                // direction part expands over sx^2, sx sy, sy^2
+                double sg = kd == 3 ? -1.0 : 1.0, A = area[s], nx = fnx[s], ny = fny[s];
+                double ax = n0x[s], ay = n0y[s], bx = mxm[s], by = mym[s], Ag = sg * A * ga[s];
+                fd[8 * j + 0] = Ag * (nx * ax + ny * ay);
+                fd[8 * j + 1] = -Ag * ax * nx;
+                fd[8 * j + 2] = -Ag * (ax * ny + ay * nx);
+                fd[8 * j + 3] = -Ag * ay * ny;
+                fd[8 * j + 4] = A * (nx * bx + ny * by);
+                fd[8 * j + 5] = -A * bx * nx;
+                fd[8 * j + 6] = -A * (bx * ny + by * nx);
+                fd[8 * j + 7] = -A * by * ny;
+            }
         }
-        ri[40] = T.crow[c];
-        ri[41] = gok[c];
-        ri[42] = nst;
-        ri[43] = nf;
+        if ((int)ylist.size() > kYRows) return "tile has more moment rows than the matrix-core shape";
+        T.ny[t] = (int)ylist.size();
+        for (size_t y = 0; y < ylist.size(); ++y) T.yrow[(size_t)t * kYRows + y] = ylist[y];
     }
     return "";
 }

@@ -12,25 +12,34 @@ struct Tiles {
     int n_tiles = 0, max_rows = 0;
     std::vector<int> cell0;            // tile t owns cells [cell0[t], cell0[t+1])
     std::vector<int> rptr, rsrc;       // staged rows of tile t: rsrc >= 0 cell row of e, < 0 -(entry+1) row of eb
+    std::vector<int> nh;               // per tile: its H rows (cells) come first among the staged rows
     std::vector<int> vptr, vnode;      // vertex rows of tile t (placed after its staged rows)
     std::vector<int> vs_ptr, vs_row;   // per vertex row: sources (tile-local rows) ...
     std::vector<double> vs_w;          // ... and inverse-distance weights
     std::vector<int> crow;             // per cell: its tile-local row
     std::vector<int> st_row;           // per LSQ stencil entry: tile-local row
     std::vector<int> f_orow, f_v0row, f_v1row;  // per (cell, face) slot: tile-local rows
-    // lane-indexed records, one 64-lane vector per cell (pack_tile_records)
-    std::vector<int> crec_i;
-    std::vector<double> crec_a, crec_b;
+    // matrix-core tables (pack_tile_mfma)
+    std::vector<double> cmf;           // per tile: LSQ coefficients as A fragments [kTileKs][64]
+    std::vector<int> ny, yrow;         // per tile: moment rows, their tile-local rows [kYRows]
+    std::vector<int> fin_i;            // per cell [kFinI]: own Y index, face count, per face Y(other, v0, v1)
+    std::vector<double> fin_d;         // per cell [kFinD]: per face the factors of Y_0..3 (face value, vertex)
 };
 
-// Lane layout of a cell record (kRec = 64 lanes):
-//   ints   [0, kRecSt) stencil rows, [24,28) face kind, [28,32) other row,
-//          [32,36) vertex-0 row, [36,40) vertex-1 row, 40 own row, 41 gradient
-//          flag, 42 stencil count, 43 face count
-//   dbl a  [0, kRecSt) stencil weight x, [24..56) per face j: ga, n0x, n0y,
-//          m_x/|m|^2, m_y/|m|^2, nx, ny, area at 24 + 4*q + j
-//   dbl b  [0, kRecSt) stencil weight y
-constexpr int kRec = 64, kRecSt = 24;
+// The LSQ gradient of a tile's 8 cells is one product per slice, G = C E:
+// C[m][r], m = cell + 8 * component, r < nh a tile row, holds the stencil
+// weights / Lx and minus their sum on the cell's own row.  cmf stores C in
+// the A-operand order of v_mfma_f64_16x16x4f64: k-step k, lane l holds
+// C[l % 16][4 k + l / 16].
+//
+// The moments are Y[j][y] = sum over the slice entries t of K_j(t) E[yrow[y]][t]
+// for the rows y the moments read (own cells, face neighbours and boundary
+// entries, face vertices): T, q_x, q_y, q_z are Y_4..7 of the cell's own row
+// and the non-Fourier residual is sum_faces sum_{j<4} cde_j (Y_j[other] -
+// Y_j[own]) + cdv_j (Y_j[v1] - Y_j[v0]), with K_0..3 the per-(direction,
+// band) factors kl, tau vg^2 (sx^2, sx sy, sy^2) of its face terms.
+constexpr int kTileCellsMax = 8, kTileRowsMax = 64, kTileKs = kTileRowsMax / 4, kYRows = 64;
+constexpr int kFinI = 16, kFinD = 32;
 
 // gptr/gidx: LSQ stencil; vptr/vsrc/vwt: inverse-distance vertex stencil
 // (vsrc >= 0 cell, < 0 -(entry+1)); fkind/fother/fv0/fv1 per (cell, face)
@@ -41,10 +50,11 @@ std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const st
                         const std::vector<int> &fkind, const std::vector<int> &fother, const std::vector<int> &fv0,
                         const std::vector<int> &fv1, int max_rows, int max_cells, Tiles &T);
 
-// Fill the lane-indexed records from the tile tables and the per-slot face
-// data (ga, n0x, n0y, mxm, mym, nx, ny, area) and LSQ weights gP (x then y).
-std::string pack_tile_records(const Problem &p, const std::vector<int> &gptr, const std::vector<double> &gP,
-                              const std::vector<int> &gok, const std::vector<int> &fkind,
-                              const std::vector<const std::vector<double> *> &face8, Tiles &T);
+// Fill the matrix-core tables from the tile tables, the LSQ weights gP (x
+// then y, per stencil entry), the gradient flags and the per-slot face data
+// face8 = (ga, n0x, n0y, mxm, mym, nx, ny, area); invL scales the gradient.
+std::string pack_tile_mfma(const Problem &p, const std::vector<int> &gptr, const std::vector<double> &gP,
+                           const std::vector<int> &gok, const std::vector<int> &fkind,
+                           const std::vector<const std::vector<double> *> &face8, double invL, Tiles &T);
 
 }  // namespace gbte
