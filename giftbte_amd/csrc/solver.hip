@@ -238,19 +238,28 @@ __global__ void __launch_bounds__(kRowBlockMax) k_bound_out(const double *__rest
 // direction's upwind order for the sweep.  2D cells (at most 4 faces).
 constexpr int kMaxFace2D = 4;
 
+// Per-cell face record of the source kernel, one contiguous block so that a
+// few scalar loads fetch it: doubles [q * 4 + f] for q = nx, ny, A/V, r_x,
+// r_y, r'_x, r'_y (kSrcD per cell), ints [f] = neighbour cell, or -(entry+1)
+// for a boundary face, or the cell itself past its last face (kSrcI).
+constexpr int kSrcD = 32, kSrcI = 4;
+
 template <bool OneRow>
 __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__ posc, double *__restrict__ S,
                                                          const double *__restrict__ grad, const double *__restrict__ eb,
                                                          const double *__restrict__ Told, const double *__restrict__ Q,
-                                                         SlotGeo g, Modes m, int nc, Rows R) {
+                                                         const int *__restrict__ fsize, const double *__restrict__ srec,
+                                                         const int *__restrict__ srei, Modes m, int nc, Rows R) {
     int c, t0;
     row_lane<OneRow>(R, c, t0);
     if (c >= nc) return;
     int DB = R.DB, nB = R.B, nD = DB / nB;
-    int nf = g.fsize[c];
-    double vol = g.vol[c], To = Told[c], Qc = Q[c];
+    int nf = fsize[c];
+    double To = Told[c], Qc = Q[c];
     const double *gc = grad + (size_t)c * 2 * DB;
     const int *pc = posc + (size_t)c * nD;
+    const double *rd = srec + (size_t)c * kSrcD;
+    const int *ri = srei + (size_t)c * kSrcI;
     // faces: normal, (A/V), r from this centre, r' from the neighbour's centre
     double fnx[kMaxFace2D], fny[kMaxFace2D], fav[kMaxFace2D], frx[kMaxFace2D], fry[kMaxFace2D], fqx[kMaxFace2D],
         fqy[kMaxFace2D];
@@ -258,18 +267,16 @@ __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__
     int fnb[kMaxFace2D];
 #pragma unroll
     for (int f = 0; f < kMaxFace2D; ++f) {
-        int k = c * 6 + f;
-        bool on = f < nf;
-        fnx[f] = on ? g.nx[k] : 0.0;
-        fny[f] = on ? g.ny[k] : 0.0;
-        fav[f] = on ? g.area[k] / vol : 0.0;
-        frx[f] = on ? g.rx[k] : 0.0;
-        fry[f] = on ? g.ry[k] : 0.0;
-        fqx[f] = on ? g.qx[k] : 0.0;
-        fqy[f] = on ? g.qy[k] : 0.0;
-        int nb = on ? g.nb[k] : 0;
+        fnx[f] = rd[f];
+        fny[f] = rd[4 + f];
+        fav[f] = rd[8 + f];
+        frx[f] = rd[12 + f];
+        fry[f] = rd[16 + f];
+        fqx[f] = rd[20 + f];
+        fqy[f] = rd[24 + f];
+        int nb = ri[f];
         fnb[f] = nb;
-        fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)g.ent[k] * DB;
+        fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)(-nb - 1) * DB;
     }
     DirBand db(t0, nB);
     double vb = m.vg[db.b], tau = m.tau[db.b], s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
@@ -1629,6 +1636,28 @@ bool Solver::build() {
                 return false;
             }
         }
+    {
+        std::vector<double> srec((size_t)nc * kSrcD, 0.0);
+        std::vector<int> srei((size_t)nc * kSrcI, 0);
+        for (int c = 0; c < nc; ++c) {
+            if (p.face_size[c] > kMaxFace2D) { err_ = "more than 4 faces per 2D cell"; return false; }
+            for (int f = 0; f < kSrcI; ++f) {
+                double *r = &srec[(size_t)c * kSrcD];
+                srei[(size_t)c * kSrcI + f] = c;
+                if (f >= p.face_size[c]) continue;
+                size_t s = (size_t)c * 6 + f;
+                r[f] = p.face_nx[s];
+                r[4 + f] = p.face_ny[s];
+                r[8 + f] = p.face_area[s] / p.volume[c];
+                r[12 + f] = srx[s];
+                r[16 + f] = sry[s];
+                r[20 + f] = sqx[s];
+                r[24 + f] = sqy[s];
+                srei[(size_t)c * kSrcI + f] = snb[s] >= 0 ? snb[s] : -sent[s] - 1;
+            }
+        }
+        if (!upload(d_srec, srec) || !upload(d_srei, srei)) return false;
+    }
     if (!upload(d_fsize, p.face_size) || !upload(d_snb, snb) || !upload(d_sent, sent) || !upload(d_vol, p.volume) ||
         !upload(d_Q, p.heat_source) || !upload(d_snx, p.face_nx) || !upload(d_sny, p.face_ny) || !upload(d_snz, p.face_nz) ||
         !upload(d_sarea, p.face_area) || !upload(d_srx, srx) || !upload(d_sry, sry) || !upload(d_srz, srz) ||
@@ -2419,7 +2448,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
         (void)hipEventRecord(ev_[0], s_);
         hipLaunchKernelGGL(one_row ? k_source<true> : k_source<false>, dim3(gw), dim3(rowblk), 0, s_, d_posc.p, d_S.p,
-                           d_grad.p, d_eb.p, d_Told.p, d_Q.p, g, m, nc, R);
+                           d_grad.p, d_eb.p, d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc, R);
         (void)hipEventRecord(ev_[1], s_);
         if (use_pc) {
             PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};

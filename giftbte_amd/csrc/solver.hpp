@@ -130,7 +130,8 @@ class Solver {
     DBuf<int> d_vptr, d_vsrc, d_tptr, d_tsrc;
     DBuf<double> d_vwt, d_tdist;
     // face-gradient / non-Fourier slot data
-    DBuf<int> d_fkind, d_fother, d_fv0, d_fv1;
+    DBuf<int> d_fkind, d_fother, d_fv0, d_fv1, d_srei;
+    DBuf<double> d_srec;  // per-cell face records of the source kernel
     DBuf<double> d_fga, d_fn0x, d_fn0y, d_fmx, d_fmy;
     // boundary entries
     std::vector<int> h_ent_list;
