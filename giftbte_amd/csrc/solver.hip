@@ -19,7 +19,7 @@
 // Layout in HBM.  Every per-system field is stored cell-major with one
 // contiguous row of n_dirs*n_bands values per cell (row index t = dir*B + band):
 //   e     [cell][dir][band]           energy density, updated in place
-//   grad  [cell][dim][dir][band]      LSQ gradient of the previous sweep's e
+//   grad  [cell][dir][band][dim]      LSQ gradient of the previous sweep's e
 //   eb    [entry][dir][band]          boundary values, entry = 2*bound + side
 //   eV    [node][dir][band]           inverse-distance vertex values
 //   S     [dir][position][band]       sweep sources in each direction's upwind order
@@ -190,9 +190,9 @@ __device__ __forceinline__ void row_gradient(const double *__restrict__ e, doubl
             if (t >= DB) continue;
             double x = gx[j], y = gy[j], w = dim == 3 ? gz[j] : 0.0;
             if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
-            gc[t] = x;
-            gc[DB + t] = y;
-            if (dim == 3) gc[2 * DB + t] = w;
+            gc[(size_t)t * dim] = x;
+            gc[(size_t)t * dim + 1] = y;
+            if (dim == 3) gc[(size_t)t * dim + 2] = w;
         }
     }
 }
@@ -224,8 +224,8 @@ __global__ void __launch_bounds__(kRowBlockMax) k_bound_out(const double *__rest
         double vb = m.vg[b];
         double dot = vb * m.sx[d] * nx + vb * m.sy[d] * ny + vb * m.sz[d] * nz;
         if (dot >= 0) {
-            double gz = dim == 3 ? gc[2 * R.DB + t] : 0.0;
-            eo[t] = ec[t] + (rx * gc[t] + ry * gc[R.DB + t] + rz * gz);
+            double gz = dim == 3 ? gc[(size_t)t * dim + 2] : 0.0;
+            eo[t] = ec[t] + (rx * gc[(size_t)t * dim] + ry * gc[(size_t)t * dim + 1] + rz * gz);
         }
     }
 }
@@ -288,7 +288,8 @@ __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__
             s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
         }
         double sx = m.sx[db.d], sy = m.sy[db.d];
-        double gcx = gc[t], gcy = gc[DB + t];
+        double2 gv = *reinterpret_cast<const double2 *>(gc + 2 * t);
+        double gcx = gv.x, gcy = gv.y;
         double s = s0;
 #pragma unroll
         for (int f = 0; f < kMaxFace2D; ++f) {
@@ -298,7 +299,8 @@ __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__
             if (dot >= 0) {
                 s -= a * (frx[f] * gcx + fry[f] * gcy);
             } else if (fnb[f] >= 0) {
-                s -= a * (fqx[f] * fin[f][t] + fqy[f] * fin[f][DB + t]);
+                double2 nv = *reinterpret_cast<const double2 *>(fin[f] + 2 * t);
+                s -= a * (fqx[f] * nv.x + fqy[f] * nv.y);
             } else {
                 s -= a * fin[f][t];
             }
@@ -712,7 +714,7 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
     }
     int ka = gptr[cc], kz = gptr[cc + 1];
     bool ok = gok[cc] != 0;
-    double *gcx = grad + (size_t)cc * 2 * DB, *gcy = gcx + DB;
+    double *gcx = grad + (size_t)cc * 2 * DB, *gcy = gcx + 1;  // [t][x, y]
     DirBand db(t0, nB);
     double aT = 0, ax = 0, ay = 0, az = 0, aR = 0;
     for (int tb = t0; tb < DB; tb += kNJ * R.tpr) {
@@ -800,8 +802,8 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
             double a = gx[j], b = gy[j];
             if (isnan(a) || isnan(b)) { a = 0; b = 0; }
             if (act && in[j]) {
-                gcx[tt[j]] = a;
-                gcy[tt[j]] = b;
+                gcx[2 * (size_t)tt[j]] = a;
+                gcy[2 * (size_t)tt[j]] = b;
             }
         }
     }
@@ -955,10 +957,7 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
                     gx = 0;
                     gy = 0;
                 }
-                if (cl < ncell) {
-                    grad[(size_t)(c0 + cl) * 2 * DB + tg] = gx;
-                    grad[((size_t)(c0 + cl) * 2 + 1) * DB + tg] = gy;
-                }
+                if (cl < ncell) *reinterpret_cast<double2 *>(grad + (size_t)(c0 + cl) * 2 * DB + 2 * tg) = {gx, gy};
             }
         }
     }
