@@ -443,9 +443,10 @@ constexpr int kPcBc = 10;       // bands per workgroup at most
 constexpr int kPcBpl = 5;       // bands per producer lane (kPcBc / 2)
 constexpr int kPcDepth = 4;     // sub-levels in flight
 constexpr int kPcMaxSub = 2047; // sub-levels per direction (LDS tables)
-// staging buffer, in doubles: S[kPcCap][Bc] | inv[kPcCap][Bc] | g[kPcCap][3]
-// | far[kPcFar][Bc] | ints cell,q0,q1,q2 [kPcCap][4]
-constexpr int kPcStage = kPcCap * kPcBc * 2 + kPcCap * 3 + kPcFar * kPcBc + kPcCap * 2;
+// staging buffer, in doubles (16-byte aligned sections, read as 16-byte
+// vectors): (S, inv)[kPcCap][Bc] | g0,g1,g2,-[kPcCap] | far[kPcFar][Bc] |
+// ints cell,q0,q1,q2 [kPcCap]
+constexpr int kPcStage = kPcCap * kPcBc * 2 + kPcCap * 4 + kPcFar * kPcBc + kPcCap * 2;
 
 struct PcSched {
     const PcRec *rec;    // [dir][position]
@@ -521,7 +522,7 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
             if (lv >= Lsub) return;
             int a0 = sl[lv], np = sl[lv + 1] - a0, nf = fp[lv + 1] - fp[lv];
             double *st = stage + (size_t)(lv & 1) * kPcStage;
-            double *sS = st, *sI = st + kPcCap * kPcBc, *sG = sI + kPcCap * kPcBc, *sF = sG + kPcCap * 3;
+            double *sSI = st, *sG = st + kPcCap * kPcBc * 2, *sF = sG + kPcCap * 4;
             int *sN = reinterpret_cast<int *>(sF + kPcFar * kPcBc);
             double G = pc_dbl(B.r0.z, B.r0.w);  // odd lane
             double Gx = __shfl_xor(G, 1, 64);
@@ -531,8 +532,8 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
                 for (int i = 0; i < kPcBpl; ++i) {
                     int jb = h * kPcBpl + i;
                     if (jb < Bc) {
-                        sS[p * Bc + jb] = B.s[i];
-                        sI[p * Bc + jb] = 1.0 / (1.0 + tv[i] * Gp);
+                        sSI[2 * (p * Bc + jb)] = B.s[i];
+                        sSI[2 * (p * Bc + jb) + 1] = 1.0 / (1.0 + tv[i] * Gp);
                     }
                 }
                 if (h == 0) {
@@ -540,10 +541,10 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
                     sN[p * 4 + 1] = B.r0.y;
                     sN[p * 4 + 2] = B.r0.z;
                     sN[p * 4 + 3] = B.r0.w;
-                    sG[p * 3 + 0] = pc_dbl(B.r1.x, B.r1.y);
-                    sG[p * 3 + 1] = pc_dbl(B.r1.z, B.r1.w);
+                    sG[p * 4 + 0] = pc_dbl(B.r1.x, B.r1.y);
+                    sG[p * 4 + 1] = pc_dbl(B.r1.z, B.r1.w);
                 } else {
-                    sG[p * 3 + 2] = pc_dbl(B.r0.x, B.r0.y);
+                    sG[p * 4 + 2] = pc_dbl(B.r0.x, B.r0.y);
                 }
             }
             if (fe < nf && fb < Bc) sF[fe * Bc + fb] = B.fv;
@@ -581,19 +582,22 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
             if (act && j < Lsub) {
                 int a0 = sl[j], np = sl[j + 1] - a0;
                 const double *st = stage + (size_t)(j & 1) * kPcStage;
-                const double *sS = st, *sI = st + kPcCap * kPcBc, *sG = sI + kPcCap * kPcBc, *sF = sG + kPcCap * 3;
+                const double *sSI = st, *sG = st + kPcCap * kPcBc * 2, *sF = sG + kPcCap * 4;
                 const int *sN = reinterpret_cast<const int *>(sF + kPcFar * kPcBc);
                 for (int p = slot; p < np; p += nslot) {
-                    int cell = sN[p * 4];
-                    double s = sS[p * Bc + bl];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        int q = sN[p * 4 + 1 + k];
-                        if (q == -1) continue;
+                    int4 nq = *reinterpret_cast<const int4 *>(sN + p * 4);
+                    double2 si = *reinterpret_cast<const double2 *>(sSI + 2 * (p * Bc + bl));
+                    int cell = nq.x;
+                    double s = si.x;
+                    auto up = [&](int q, double gq) {
+                        if (q == -1) return;
                         double u = q >= 0 ? ring[(q & (R - 1)) * Bc + bl] : sF[(-2 - q) * Bc + bl];
-                        s -= tvb * sG[p * 3 + k] * u;
-                    }
-                    double en = s * sI[p * Bc + bl];
+                        s -= tvb * gq * u;
+                    };
+                    up(nq.y, sG[p * 4]);
+                    up(nq.z, sG[p * 4 + 1]);
+                    up(nq.w, sG[p * 4 + 2]);
+                    double en = s * si.y;
                     ed[(size_t)cell * DB] = en;
                     ring[((a0 + p) & (R - 1)) * Bc + bl] = en;
                 }
