@@ -1475,6 +1475,9 @@ Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_ne
 }
 
 Solver::~Solver() {
+    if (macro_exec_) (void)hipGraphExecDestroy(macro_exec_);
+    if (macro_graph_) (void)hipGraphDestroy(macro_graph_);
+    if (h_part) (void)hipHostFree(h_part);
     if (h_pin) (void)hipHostFree(h_pin);
     for (hipEvent_t ev : ev_)
         if (ev) (void)hipEventDestroy(ev);
@@ -2241,6 +2244,7 @@ bool Solver::build() {
         (!alloc(d_ndy, nc) || !alloc(d_ndx, nc) || !alloc(d_ndu, (size_t)std::max<long>(1, std::max(NDmacro.u_total, NDfour.u_total)))))
         return false;
     red_blocks = blocks_for(nc, kBlock);
+    if (!check(hipHostMalloc((void **)&h_part, (size_t)red_blocks * sizeof(double)), "hipHostMalloc")) return false;
     if (!alloc(d_part, (size_t)red_blocks * 4 + 8) || !alloc(d_part2, (size_t)red_blocks * 4 + 8)) return false;
     if (!check(hipMemsetAsync(d_eb.p, 0, d_eb.n * sizeof(double), s_), "hipMemsetAsync") ||
         !check(hipMemsetAsync(d_qz.p, 0, nc * sizeof(double), s_), "hipMemsetAsync"))
@@ -2283,6 +2287,33 @@ bool Solver::inner_error(const double *a, const double *b, double *out) {
     double s = 0;
     for (double v : h) s += v;
     *out = s / nc;
+    return true;
+}
+
+// One pass of the synthetic loop with the direct solver, T1 = T, T = K^-1
+// rhs(T1), as one HIP graph (about two dozen small kernels that would
+// otherwise each wait for the host to enqueue them after the previous
+// pass's sync); returns the mean squared change like inner_error.
+bool Solver::macro_pass_direct(double *out) {
+    if (!macro_exec_) {
+        if (!check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
+        hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
+        launch_vertex_T(d_T.p, d_TV.p);
+        launch_rhs(true, d_tmac.p, d_rhs.p);
+        bool ok = nd_solve(NDmacro, d_rhs.p, d_T.p);
+        hipLaunchKernelGGL(k_sqdiff, dim3(red_blocks), dim3(kBlock), 0, s_, d_T1.p, d_T.p, d_part.p, nc);
+        ok = check(hipMemcpyAsync(h_part, d_part.p, (size_t)red_blocks * sizeof(double), hipMemcpyDeviceToHost, s_),
+                   "D2H") && ok;
+        hipGraph_t g = nullptr;
+        if (!check(hipStreamEndCapture(s_, &g), "hipStreamEndCapture") || !ok) return false;
+        macro_graph_ = g;
+        if (!check(hipGraphInstantiate(&macro_exec_, g, nullptr, nullptr, 0), "hipGraphInstantiate")) return false;
+    }
+    if (!check(hipGraphLaunch(macro_exec_, s_), "hipGraphLaunch") || !check(hipStreamSynchronize(s_), "sync"))
+        return false;
+    double sum = 0;
+    for (int i = 0; i < red_blocks; ++i) sum += h_part[i];
+    *out = sum / nc;
     return true;
 }
 
@@ -2490,13 +2521,17 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         double inner = 1000;
         int inc = 0, passes = 0;
         while (inner > 1e-6) {
-            hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
-            launch_vertex_T(d_T.p, d_TV.p);
-            launch_rhs(true, d_tmac.p, d_rhs.p);
-            int cgit = 0;
-            if (!linsolve(Kmacro, NDmacro, d_rhs.p, d_T.p, &cgit)) return GBTE_HIP_ERROR;
-            cg_total += cgit;
-            if (!inner_error(d_T1.p, d_T.p, &inner)) return GBTE_HIP_ERROR;
+            if (opt.linear_solver == GBTE_LINSOLVE_DIRECT) {
+                if (!macro_pass_direct(&inner)) return GBTE_HIP_ERROR;
+            } else {
+                hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
+                launch_vertex_T(d_T.p, d_TV.p);
+                launch_rhs(true, d_tmac.p, d_rhs.p);
+                int cgit = 0;
+                if (!linsolve(Kmacro, NDmacro, d_rhs.p, d_T.p, &cgit)) return GBTE_HIP_ERROR;
+                cg_total += cgit;
+                if (!inner_error(d_T1.p, d_T.p, &inner)) return GBTE_HIP_ERROR;
+            }
             if (inner > errInner_) inc++;
             errInner_ = inner;
             passes++;
