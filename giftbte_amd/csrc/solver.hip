@@ -243,6 +243,7 @@ constexpr int kMaxFace2D = 4;
 // r_y, r'_x, r'_y (kSrcD per cell), ints [f] = neighbour cell, or -(entry+1)
 // for a boundary face, or the cell itself past its last face (kSrcI).
 constexpr int kSrcD = 32, kSrcI = 4;
+constexpr int kSrcBatch = 8;  // entries per lane whose loads are issued before their stores
 
 template <bool OneRow>
 __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__ posc, double *__restrict__ S,
@@ -280,33 +281,46 @@ __global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__
     }
     DirBand db(t0, nB);
     double vb = m.vg[db.b], tau = m.tau[db.b], s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
-#pragma unroll 2  // two entries' loads in flight
-    for (int t = t0; t < DB; t += R.tpr) {
-        if (R.bstep != 0) {
-            vb = m.vg[db.b];
-            tau = m.tau[db.b];
-            s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
-        }
-        double sx = m.sx[db.d], sy = m.sy[db.d];
-        double2 gv = *reinterpret_cast<const double2 *>(gc + 2 * t);
-        double gcx = gv.x, gcy = gv.y;
-        double s = s0;
+    // entries in batches of kSrcBatch: every load of a batch is issued before
+    // its stores (the stores are counted by the same wait counter)
+    for (int tb = t0; tb < DB; tb += kSrcBatch * R.tpr) {
+        double so[kSrcBatch];
+        size_t sa[kSrcBatch];
 #pragma unroll
-        for (int f = 0; f < kMaxFace2D; ++f) {
-            if (f >= nf) break;
-            double dot = vb * sx * fnx[f] + vb * sy * fny[f];
-            double a = tau * fav[f] * dot;
-            if (dot >= 0) {
-                s -= a * (frx[f] * gcx + fry[f] * gcy);
-            } else if (fnb[f] >= 0) {
-                double2 nv = *reinterpret_cast<const double2 *>(fin[f] + 2 * t);
-                s -= a * (fqx[f] * nv.x + fqy[f] * nv.y);
-            } else {
-                s -= a * fin[f][t];
+        for (int j = 0; j < kSrcBatch; ++j) {
+            int t = tb + j * R.tpr;
+            bool in = t < DB;
+            int tt = in ? t : t0, d = in ? db.d : 0, bb = in ? db.b : 0;
+            if (R.bstep != 0) {
+                vb = m.vg[bb];
+                tau = m.tau[bb];
+                s0 = To * m.cap[bb] + Qc * m.hr[bb] * m.tau[bb];
             }
+            double sx = m.sx[d], sy = m.sy[d];
+            double2 gv = *reinterpret_cast<const double2 *>(gc + 2 * tt);
+            double gcx = gv.x, gcy = gv.y;
+            double s = s0;
+#pragma unroll
+            for (int f = 0; f < kMaxFace2D; ++f) {
+                if (f >= nf) break;
+                double dot = vb * sx * fnx[f] + vb * sy * fny[f];
+                double a = tau * fav[f] * dot;
+                if (dot >= 0) {
+                    s -= a * (frx[f] * gcx + fry[f] * gcy);
+                } else if (fnb[f] >= 0) {
+                    double2 nv = *reinterpret_cast<const double2 *>(fin[f] + 2 * tt);
+                    s -= a * (fqx[f] * nv.x + fqy[f] * nv.y);
+                } else {
+                    s -= a * fin[f][tt];
+                }
+            }
+            so[j] = s;
+            sa[j] = in ? ((size_t)d * nc + pc[d]) * nB + bb : ~(size_t)0;
+            db.next(R);
         }
-        S[((size_t)db.d * nc + pc[db.d]) * nB + db.b] = s;
-        db.next(R);
+#pragma unroll
+        for (int j = 0; j < kSrcBatch; ++j)
+            if (sa[j] != ~(size_t)0) S[sa[j]] = so[j];
     }
 }
 
