@@ -245,14 +245,16 @@ constexpr int kMaxFace2D = 4;
 constexpr int kSrcD = 32, kSrcI = 4;
 constexpr int kSrcBatch = 8;  // entries per lane whose loads are issued before their stores
 
-template <bool OneRow>
-__global__ void __launch_bounds__(kRowBlockMax) k_source(const int *__restrict__ posc, double *__restrict__ S,
+// One wave per cell row (kSrcRows rows per workgroup); the row index is
+// wave-uniform, so the cell's face record sits in scalar registers.
+constexpr int kSrcRows = 4;
+
+__global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict__ posc, double *__restrict__ S,
                                                          const double *__restrict__ grad, const double *__restrict__ eb,
                                                          const double *__restrict__ Told, const double *__restrict__ Q,
                                                          const int *__restrict__ fsize, const double *__restrict__ srec,
                                                          const int *__restrict__ srei, Modes m, int nc, Rows R) {
-    int c, t0;
-    row_lane<OneRow>(R, c, t0);
+    int c = __builtin_amdgcn_readfirstlane(xcd_block() * kSrcRows + (int)(threadIdx.x >> 6)), t0 = threadIdx.x & 63;
     if (c >= nc) return;
     int DB = R.DB, nB = R.B, nD = DB / nB;
     int nf = fsize[c];
@@ -2462,6 +2464,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!upload(d_be_slot, be_slot)) return GBTE_HIP_ERROR;
     size_t nE = (size_t)nD * nc * nB;
     Rows R{nD * nB, nB, row_tpr, row_rpb, row_tpr / nB, row_tpr % nB};
+    Rows Rsrc{nD * nB, nB, 64, kSrcRows, 64 / nB, 64 % nB};  // k_source: one wave per row
     bool one_row = row_rpb == 1;
     int rowblk = row_tpr * row_rpb;
     // row kernels remap blocks per XCD (xcd_block): grids are multiples of 8
@@ -2495,8 +2498,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
         (void)hipEventRecord(ev_[0], s_);
-        hipLaunchKernelGGL(one_row ? k_source<true> : k_source<false>, dim3(gw), dim3(rowblk), 0, s_, d_posc.p, d_S.p,
-                           d_grad.p, d_eb.p, d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc, R);
+        hipLaunchKernelGGL(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), 0, s_, d_posc.p, d_S.p,
+                           d_grad.p, d_eb.p, d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc, Rsrc);
         (void)hipEventRecord(ev_[1], s_);
         if (use_pc) {
             PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
