@@ -528,6 +528,15 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
             const int4 *rr = reinterpret_cast<const int4 *>(P.rec + Pp);
             B.r0 = rr[h ? 2 : 0];
             B.r1 = rr[1];
+            (void)f0;
+            (void)nf;
+        };
+        // far upwind cells of sub-level lv, loaded one sub-level before the
+        // rest of lv (so that the dependent far-value load waits for a load
+        // issued two sub-levels earlier)
+        auto issue_fc = [&](PcProdBuf &B, int lv) {
+            int l = lv < Lsub ? lv : Lsub - 1;
+            int f0 = fp[l], nf = fp[l + 1] - f0;
             B.fc = P.farcell[f0 + (fe < nf ? fe : 0)];
         };
         auto issue_far = [&](PcProdBuf &B) {
@@ -568,18 +577,24 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
         // prologue: sub-levels 0 .. kPcDepth-1 in flight, far values of
         // 0 .. kPcDepth-2, stage 0
 #pragma unroll
-        for (int q = 0; q < kPcDepth; ++q) issue(buf[q], q);
+        for (int q = 0; q < kPcDepth; ++q) {
+            issue_fc(buf[q], q);
+            issue(buf[q], q);
+        }
 #pragma unroll
         for (int q = 0; q + 1 < kPcDepth; ++q) issue_far(buf[q]);
         write(buf[0], 0);
+        issue_fc(buf[0], kPcDepth);
         __builtin_amdgcn_s_waitcnt(0);  // the loop starts from a known state of outstanding loads
         __syncthreads();
-        // iteration j (consumers solve j): stage j+1, load j+D, far values of j+D-1
+        // iteration j (consumers solve j): stage j+1, far cells of j+D+1 into
+        // the buffer just staged, load j+D, far values of j+D-1
         for (int j0 = 0; j0 < nstep; j0 += kPcDepth) {
 #pragma unroll
             for (int q = 0; q < kPcDepth; ++q) {
                 int j = j0 + q;
                 write(buf[(q + 1) % kPcDepth], j + 1);
+                issue_fc(buf[(q + 1) % kPcDepth], j + 1 + kPcDepth);
                 issue(buf[q % kPcDepth], j + kPcDepth);
                 issue_far(buf[(q + kPcDepth - 1) % kPcDepth]);
                 __syncthreads();
