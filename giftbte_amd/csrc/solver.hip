@@ -460,9 +460,9 @@ constexpr int kPcBpl = 5;       // bands per producer lane (kPcBc / 2)
 constexpr int kPcDepth = 4;     // sub-levels in flight
 constexpr int kPcMaxSub = 2047; // sub-levels per direction (LDS tables)
 // staging buffer, in doubles (16-byte aligned sections, read as 16-byte
-// vectors): (S, inv)[kPcCap][Bc] | g0,g1,g2,-[kPcCap] | far[kPcFar][Bc] |
-// ints cell,q0,q1,q2 [kPcCap]
-constexpr int kPcStage = kPcCap * kPcBc * 2 + kPcCap * 4 + kPcFar * kPcBc + kPcCap * 2;
+// vectors): S[kPcCap][Bc] | g0,g1,g2,G [kPcCap] | far[kPcFar][Bc] | ints
+// cell,q0,q1,q2 [kPcCap]
+constexpr int kPcStage = kPcCap * kPcBc + kPcCap * 4 + kPcFar * kPcBc + kPcCap * 2;
 
 struct PcSched {
     const PcRec *rec;    // [dir][position]
@@ -506,12 +506,6 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
         // ---------------- producers
         int pl = threadIdx.x, p = pl >> 1, h = pl & 1;
         int fe = pl >> 4, fb = pl & 15;
-        double tv[kPcBpl];
-#pragma unroll
-        for (int i = 0; i < kPcBpl; ++i) {
-            int b = b0 + h * kPcBpl + i;
-            tv[i] = (h * kPcBpl + i < Bc && b < nB) ? m.tau[b] * m.vg[b] : 0.0;
-        }
         PcProdBuf buf[kPcDepth];
         // loads of sub-level lv into buf (addresses clamped to valid memory)
         auto issue = [&](PcProdBuf &B, int lv) {
@@ -547,19 +541,13 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
             if (lv >= Lsub) return;
             int a0 = sl[lv], np = sl[lv + 1] - a0, nf = fp[lv + 1] - fp[lv];
             double *st = stage + (size_t)(lv & 1) * kPcStage;
-            double *sSI = st, *sG = st + kPcCap * kPcBc * 2, *sF = sG + kPcCap * 4;
+            double *sS = st, *sG = st + kPcCap * kPcBc, *sF = sG + kPcCap * 4;
             int *sN = reinterpret_cast<int *>(sF + kPcFar * kPcBc);
-            double G = pc_dbl(B.r0.z, B.r0.w);  // odd lane
-            double Gx = __shfl_xor(G, 1, 64);
-            double Gp = h ? G : Gx;
             if (p < np) {
 #pragma unroll
                 for (int i = 0; i < kPcBpl; ++i) {
                     int jb = h * kPcBpl + i;
-                    if (jb < Bc) {
-                        sSI[2 * (p * Bc + jb)] = B.s[i];
-                        sSI[2 * (p * Bc + jb) + 1] = 1.0 / (1.0 + tv[i] * Gp);
-                    }
+                    if (jb < Bc) sS[p * Bc + jb] = B.s[i];
                 }
                 if (h == 0) {
                     sN[p * 4 + 0] = B.r0.x;
@@ -570,6 +558,7 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
                     sG[p * 4 + 1] = pc_dbl(B.r1.z, B.r1.w);
                 } else {
                     sG[p * 4 + 2] = pc_dbl(B.r0.x, B.r0.y);
+                    sG[p * 4 + 3] = pc_dbl(B.r0.z, B.r0.w);
                 }
             }
             if (fe < nf && fb < Bc) sF[fe * Bc + fb] = B.fv;
@@ -613,13 +602,14 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
             if (act && j < Lsub) {
                 int a0 = sl[j], np = sl[j + 1] - a0;
                 const double *st = stage + (size_t)(j & 1) * kPcStage;
-                const double *sSI = st, *sG = st + kPcCap * kPcBc * 2, *sF = sG + kPcCap * 4;
+                const double *sS = st, *sG = st + kPcCap * kPcBc, *sF = sG + kPcCap * 4;
                 const int *sN = reinterpret_cast<const int *>(sF + kPcFar * kPcBc);
                 for (int p = slot; p < np; p += nslot) {
                     int4 nq = *reinterpret_cast<const int4 *>(sN + p * 4);
-                    double2 si = *reinterpret_cast<const double2 *>(sSI + 2 * (p * Bc + bl));
+                    double2 g23 = *reinterpret_cast<const double2 *>(sG + p * 4 + 2);
+                    double den = 1.0 + tvb * g23.y;
                     int cell = nq.x;
-                    double s = si.x;
+                    double s = sS[p * Bc + bl];
                     auto up = [&](int q, double gq) {
                         if (q == -1) return;
                         double u = q >= 0 ? ring[(q & (R - 1)) * Bc + bl] : sF[(-2 - q) * Bc + bl];
@@ -627,8 +617,8 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
                     };
                     up(nq.y, sG[p * 4]);
                     up(nq.z, sG[p * 4 + 1]);
-                    up(nq.w, sG[p * 4 + 2]);
-                    double en = s * si.y;
+                    up(nq.w, g23.x);
+                    double en = s / den;
                     ed[(size_t)cell * DB] = en;
                     ring[((a0 + p) & (R - 1)) * Bc + bl] = en;
                 }
