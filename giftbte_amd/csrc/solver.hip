@@ -246,7 +246,9 @@ constexpr int kSrcD = 32, kSrcI = 4;
 constexpr int kSrcBatch = 8;  // entries per lane whose loads are issued before their stores
 
 // One wave per cell row (kSrcRows rows per workgroup); the row index is
-// wave-uniform, so the cell's face record sits in scalar registers.
+// wave-uniform, so the cell's face record sits in scalar registers.  The
+// per-band and per-direction mode constants and the rows' upwind positions
+// are copied to LDS first (dynamic LDS: 4 nB + 2 nD doubles, kSrcRows nD ints).
 constexpr int kSrcRows = 4;
 
 __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict__ posc, double *__restrict__ S,
@@ -254,13 +256,33 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
                                                          const double *__restrict__ Told, const double *__restrict__ Q,
                                                          const int *__restrict__ fsize, const double *__restrict__ srec,
                                                          const int *__restrict__ srei, Modes m, int nc, Rows R) {
-    int c = __builtin_amdgcn_readfirstlane(xcd_block() * kSrcRows + (int)(threadIdx.x >> 6)), t0 = threadIdx.x & 63;
-    if (c >= nc) return;
+    extern __shared__ double lds[];
+    int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int c0 = xcd_block() * kSrcRows, t0 = threadIdx.x & 63;
     int DB = R.DB, nB = R.B, nD = DB / nB;
+    double *mvg = lds, *mtau = mvg + nB, *mcap = mtau + nB, *mhr = mcap + nB, *msx = mhr + nB, *msy = msx + nD;
+    int *mpos = reinterpret_cast<int *>(msy + nD);
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+        mvg[i] = m.vg[i];
+        mtau[i] = m.tau[i];
+        mcap[i] = m.cap[i];
+        mhr[i] = m.hr[i];
+    }
+    for (int i = threadIdx.x; i < nD; i += blockDim.x) {
+        msx[i] = m.sx[i];
+        msy[i] = m.sy[i];
+    }
+    for (int i = threadIdx.x; i < kSrcRows * nD; i += blockDim.x) {
+        int r = c0 + i / nD;
+        mpos[i] = r < nc ? posc[(size_t)r * nD + i % nD] : 0;
+    }
+    __syncthreads();
+    int c = c0 + w;
+    if (c >= nc) return;
     int nf = fsize[c];
     double To = Told[c], Qc = Q[c];
     const double *gc = grad + (size_t)c * 2 * DB;
-    const int *pc = posc + (size_t)c * nD;
+    const int *pc = mpos + w * nD;
     const double *rd = srec + (size_t)c * kSrcD;
     const int *ri = srei + (size_t)c * kSrcI;
     // faces: normal, (A/V), r from this centre, r' from the neighbour's centre
@@ -282,7 +304,7 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
         fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)(-nb - 1) * DB;
     }
     DirBand db(t0, nB);
-    double vb = m.vg[db.b], tau = m.tau[db.b], s0 = To * m.cap[db.b] + Qc * m.hr[db.b] * m.tau[db.b];
+    double vb = mvg[db.b], tau = mtau[db.b], s0 = To * mcap[db.b] + Qc * mhr[db.b] * mtau[db.b];
     // entries in batches of kSrcBatch: every load of a batch is issued before
     // its stores (the stores are counted by the same wait counter)
     for (int tb = t0; tb < DB; tb += kSrcBatch * R.tpr) {
@@ -294,11 +316,11 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
             bool in = t < DB;
             int tt = in ? t : t0, d = in ? db.d : 0, bb = in ? db.b : 0;
             if (R.bstep != 0) {
-                vb = m.vg[bb];
-                tau = m.tau[bb];
-                s0 = To * m.cap[bb] + Qc * m.hr[bb] * m.tau[bb];
+                vb = mvg[bb];
+                tau = mtau[bb];
+                s0 = To * mcap[bb] + Qc * mhr[bb] * mtau[bb];
             }
-            double sx = m.sx[d], sy = m.sy[d];
+            double sx = msx[d], sy = msy[d];
             double2 gv = *reinterpret_cast<const double2 *>(gc + 2 * tt);
             double gcx = gv.x, gcy = gv.y;
             double s = s0;
@@ -2503,7 +2525,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
         (void)hipEventRecord(ev_[0], s_);
-        hipLaunchKernelGGL(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), 0, s_, d_posc.p, d_S.p,
+        hipLaunchKernelGGL(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows),
+                           (size_t)(4 * nB + 2 * nD) * sizeof(double) + (size_t)kSrcRows * nD * sizeof(int), s_, d_posc.p, d_S.p,
                            d_grad.p, d_eb.p, d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc, Rsrc);
         (void)hipEventRecord(ev_[1], s_);
         if (use_pc) {
