@@ -1518,9 +1518,12 @@ Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_ne
 }
 
 Solver::~Solver() {
-    if (macro_exec_) (void)hipGraphExecDestroy(macro_exec_);
-    if (macro_graph_) (void)hipGraphDestroy(macro_graph_);
+    for (int q = 0; q < 2; ++q) {
+        if (macro_exec_[q]) (void)hipGraphExecDestroy(macro_exec_[q]);
+        if (macro_graph_[q]) (void)hipGraphDestroy(macro_graph_[q]);
+    }
     if (h_part) (void)hipHostFree(h_part);
+    if (h_err) (void)hipHostFree(h_err);
     if (h_pin) (void)hipHostFree(h_pin);
     for (hipEvent_t ev : ev_)
         if (ev) (void)hipEventDestroy(ev);
@@ -2287,7 +2290,10 @@ bool Solver::build() {
         (!alloc(d_ndy, nc) || !alloc(d_ndx, nc) || !alloc(d_ndu, (size_t)std::max<long>(1, std::max(NDmacro.u_total, NDfour.u_total)))))
         return false;
     red_blocks = blocks_for(nc, kBlock);
-    if (!check(hipHostMalloc((void **)&h_part, (size_t)red_blocks * sizeof(double)), "hipHostMalloc")) return false;
+    if (!check(hipHostMalloc((void **)&h_part, (size_t)2 * red_blocks * sizeof(double)), "hipHostMalloc") ||
+        !check(hipHostMalloc((void **)&h_err, (size_t)4 * red_blocks * sizeof(double)), "hipHostMalloc") ||
+        !alloc(d_mpart, (size_t)2 * red_blocks))
+        return false;
     if (!alloc(d_part, (size_t)red_blocks * 4 + 8) || !alloc(d_part2, (size_t)red_blocks * 4 + 8)) return false;
     if (!check(hipMemsetAsync(d_eb.p, 0, d_eb.n * sizeof(double), s_), "hipMemsetAsync") ||
         !check(hipMemsetAsync(d_qz.p, 0, nc * sizeof(double), s_), "hipMemsetAsync"))
@@ -2334,30 +2340,36 @@ bool Solver::inner_error(const double *a, const double *b, double *out) {
 }
 
 // One pass of the synthetic loop with the direct solver, T1 = T, T = K^-1
-// rhs(T1), as one HIP graph (about two dozen small kernels that would
-// otherwise each wait for the host to enqueue them after the previous
-// pass's sync); returns the mean squared change like inner_error.
-bool Solver::macro_pass_direct(double *out) {
-    if (!macro_exec_) {
+// rhs(T1), as one HIP graph (about two dozen small kernels): enqueued without
+// a sync; graph q leaves the pass's squared-change partials in h_part + q
+// red_blocks once the stream has reached it (macro_inner reads them).
+bool Solver::macro_launch(int q) {
+    if (!macro_exec_[q]) {
         if (!check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
         hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
         launch_vertex_T(d_T.p, d_TV.p);
         launch_rhs(true, d_tmac.p, d_rhs.p);
         bool ok = nd_solve(NDmacro, d_rhs.p, d_T.p);
-        hipLaunchKernelGGL(k_sqdiff, dim3(red_blocks), dim3(kBlock), 0, s_, d_T1.p, d_T.p, d_part.p, nc);
-        ok = check(hipMemcpyAsync(h_part, d_part.p, (size_t)red_blocks * sizeof(double), hipMemcpyDeviceToHost, s_),
-                   "D2H") && ok;
+        double *dp = d_mpart.p + (size_t)q * red_blocks;
+        hipLaunchKernelGGL(k_sqdiff, dim3(red_blocks), dim3(kBlock), 0, s_, d_T1.p, d_T.p, dp, nc);
+        ok = check(hipMemcpyAsync(h_part + (size_t)q * red_blocks, dp, (size_t)red_blocks * sizeof(double),
+                                  hipMemcpyDeviceToHost, s_),
+                   "D2H") &&
+             ok;
         hipGraph_t g = nullptr;
         if (!check(hipStreamEndCapture(s_, &g), "hipStreamEndCapture") || !ok) return false;
-        macro_graph_ = g;
-        if (!check(hipGraphInstantiate(&macro_exec_, g, nullptr, nullptr, 0), "hipGraphInstantiate")) return false;
+        macro_graph_[q] = g;
+        if (!check(hipGraphInstantiate(&macro_exec_[q], g, nullptr, nullptr, 0), "hipGraphInstantiate")) return false;
     }
-    if (!check(hipGraphLaunch(macro_exec_, s_), "hipGraphLaunch") || !check(hipStreamSynchronize(s_), "sync"))
-        return false;
+    return check(hipGraphLaunch(macro_exec_[q], s_), "hipGraphLaunch");
+}
+
+// mean squared change of the pass run by graph q (after a sync), summed like
+// inner_error
+double Solver::macro_inner(int q) const {
     double sum = 0;
-    for (int i = 0; i < red_blocks; ++i) sum += h_part[i];
-    *out = sum / nc;
-    return true;
+    for (int i = 0; i < red_blocks; ++i) sum += h_part[(size_t)q * red_blocks + i];
+    return sum / nc;
 }
 
 bool Solver::linsolve(const DevCSR &K, const DevND &f, double *d_b, double *d_x, int *iters) {
@@ -2565,26 +2577,56 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         // _marco_solution (StaticBTESynthetic.cpp:2692-2734)
         double inner = 1000;
         int inc = 0, passes = 0;
-        while (inner > 1e-6) {
-            if (opt.linear_solver == GBTE_LINSOLVE_DIRECT) {
-                if (!macro_pass_direct(&inner)) return GBTE_HIP_ERROR;
+        bool have_err = false;  // h_err already holds this iteration's outer error partials
+        auto account = [&](double in) {  // the loop's bookkeeping after one pass; false: stop
+            inner = in;
+            if (inner > errInner_) inc++;
+            errInner_ = inner;
+            passes++;
+            if (opt.verbose) printf("internal residual: %g\n", inner);
+            if (inc >= 10) { err_ = "cannot congerve, try to set Order=1 or 2limit"; status = GBTE_DIVERGED; return false; }
+            return inner > 1e-6;
+        };
+        if (opt.linear_solver == GBTE_LINSOLVE_DIRECT) {
+            // two passes and the outer error of their result are enqueued
+            // behind one sync (most iterations take exactly two passes); a
+            // loop that stops after the first pass undoes the second (T = T1)
+            if (!macro_launch(0) || !macro_launch(1)) return GBTE_HIP_ERROR;
+            (void)hipEventRecord(ev_[6], s_);
+            hipLaunchKernelGGL(k_err, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_Told.p, d_qx.p, d_qy.p, d_qz.p,
+                               d_qxo.p, d_qyo.p, d_qzo.p, d_part.p, nc, red_blocks);
+            if (!check(hipMemcpyAsync(h_err, d_part.p, (size_t)4 * red_blocks * sizeof(double), hipMemcpyDeviceToHost,
+                                      s_),
+                       "D2H") ||
+                !check(hipStreamSynchronize(s_), "sync"))
+                return GBTE_HIP_ERROR;
+            if (!account(macro_inner(0))) {
+                hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T1.p, d_T.p, nc);
+            } else if (!account(macro_inner(1))) {
+                have_err = status == GBTE_OK;
             } else {
+                for (bool more = true; more;) {
+                    if (!macro_launch(passes & 1)) return GBTE_HIP_ERROR;
+                    (void)hipEventRecord(ev_[6], s_);
+                    if (!check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
+                    more = account(macro_inner(passes & 1));
+                }
+            }
+        } else {
+            while (inner > 1e-6) {
                 hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
                 launch_vertex_T(d_T.p, d_TV.p);
                 launch_rhs(true, d_tmac.p, d_rhs.p);
                 int cgit = 0;
                 if (!linsolve(Kmacro, NDmacro, d_rhs.p, d_T.p, &cgit)) return GBTE_HIP_ERROR;
                 cg_total += cgit;
-                if (!inner_error(d_T1.p, d_T.p, &inner)) return GBTE_HIP_ERROR;
+                double in = 0;
+                if (!inner_error(d_T1.p, d_T.p, &in)) return GBTE_HIP_ERROR;
+                if (!account(in)) break;
             }
-            if (inner > errInner_) inc++;
-            errInner_ = inner;
-            passes++;
-            if (opt.verbose) printf("internal residual: %g\n", inner);
-            if (inc >= 10) { err_ = "cannot congerve, try to set Order=1 or 2limit"; status = GBTE_DIVERGED; break; }
+            (void)hipEventRecord(ev_[6], s_);
         }
         macro_total += passes;
-        (void)hipEventRecord(ev_[6], s_);
         auto add_times = [&]() {  // after a host sync on s_
             float ms = 0;
             (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);
@@ -2605,10 +2647,14 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             break;
         }
         // _get_magin_check_error
-        hipLaunchKernelGGL(k_err, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_Told.p, d_qx.p, d_qy.p, d_qz.p,
-                           d_qxo.p, d_qyo.p, d_qzo.p, d_part.p, nc, red_blocks);
         std::vector<double> h;
-        if (!reduce_host(d_part.p, red_blocks, h, 4)) return GBTE_HIP_ERROR;
+        if (have_err) {
+            h.assign(h_err, h_err + (size_t)4 * red_blocks);
+        } else {
+            hipLaunchKernelGGL(k_err, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_Told.p, d_qx.p, d_qy.p, d_qz.p,
+                               d_qxo.p, d_qyo.p, d_qzo.p, d_part.p, nc, red_blocks);
+            if (!reduce_host(d_part.p, red_blocks, h, 4)) return GBTE_HIP_ERROR;
+        }
         add_times();
         double sT = 0, mT = 0, sQ = 0, mQ = 0;
         for (int k = 0; k < red_blocks; ++k) {

@@ -93,7 +93,8 @@ class Solver {
     void launch_vertex_T(const double *T, double *TV);
     void launch_rhs(bool macro, const double *bval, double *rhs);
     bool inner_error(const double *a, const double *b, double *out);
-    bool macro_pass_direct(double *out);
+    bool macro_launch(int q);
+    double macro_inner(int q) const;
 
     std::vector<int> cell_new_, node_new_;  // deck order -> device order
     Problem Pp;                             // renumbered problem (device order)
@@ -159,9 +160,11 @@ class Solver {
     // CG / reduction scratch
     DBuf<double> d_r, d_z, d_p, d_Ap, d_part, d_part2;
     double *h_pin = nullptr;  // pinned host scratch
-    double *h_part = nullptr;  // pinned copy of the reduction partials
-    hipGraph_t macro_graph_ = nullptr;
-    hipGraphExec_t macro_exec_ = nullptr;
+    double *h_part = nullptr;  // pinned copies of the synthetic passes' change partials (2 slots)
+    double *h_err = nullptr;   // pinned copy of the outer-error partials
+    DBuf<double> d_mpart;      // the passes' change partials (2 slots)
+    hipGraph_t macro_graph_[2] = {nullptr, nullptr};
+    hipGraphExec_t macro_exec_[2] = {nullptr, nullptr};
     int red_blocks = 0;
     bool have_state = false;
     double errInner_ = 0, errTold_ = 0, errQold_ = 0;
