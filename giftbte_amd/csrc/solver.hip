@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
         }
 #pragma unroll
         for (int j = 0; j < kSrcBatch; ++j)
-            if (sa[j] != ~(size_t)0) S[sa[j]] = so[j];
+            if (sa[j] != ~(size_t)0) __builtin_nontemporal_store(so[j], S + sa[j]);  // read by the next launch
     }
 }
 
@@ -892,6 +892,7 @@ constexpr int kStageBatch = 16;            // staged rows in flight per lane
 constexpr int kTileJ = 8;                  // moment factors K_0..K_7
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 
 struct TileDev {
     const int *cell0, *rptr, *rsrc, *nh, *vptr, *vs_ptr, *vs_row, *ny, *yrow, *fin_i;
@@ -1004,7 +1005,11 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
                     gx = 0;
                     gy = 0;
                 }
-                if (cl < ncell) *reinterpret_cast<double2 *>(grad + (size_t)(c0 + cl) * 2 * DB + 2 * tg) = {gx, gy};
+                // streaming stores: the gradients are read back by the next
+                // launch, from HBM, and should not evict staged rows from L2
+                if (cl < ncell)
+                    __builtin_nontemporal_store(v2d{gx, gy},
+                                                reinterpret_cast<v2d *>(grad + (size_t)(c0 + cl) * 2 * DB + 2 * tg));
             }
         }
     }
