@@ -101,6 +101,12 @@ struct Modes {
     const double *vg, *tau, *cap, *lr, *hr, *sx, *sy, *sz, *w;
     const double *lrc;  // lr / cap per band
     double kappa, cbulk;
+    // Energy densities are held as deviations e - C_b eref from the
+    // equilibrium at eref (the deck's Tref): the transport equations are
+    // linear and every face sum of (A/V)(s.n) vanishes, so only the sources,
+    // the thermalising walls and the moments see the shift.  off* are the
+    // moments of C_b eref, added back to T, E, T_band, q and T_mac.
+    double eref, offT, offE, offW, offQx, offQy, offQz, offMac;
 };
 
 struct SlotGeo {
@@ -140,12 +146,13 @@ __device__ __forceinline__ void row_lane(const Rows &R, int &row, int &t0) {
 
 // ---------------------------------------------------------------- kernels
 
-// _set_initial (StaticBTESynthetic.cpp:765-782): e = T C_b for all systems.
+// _set_initial (StaticBTESynthetic.cpp:765-782): e = T C_b for all systems
+// (held as (T - eref) C_b).
 __global__ void k_init_e(double *__restrict__ e, const double *__restrict__ T, const double *__restrict__ cap, int DB,
-                         int nB, size_t total) {
+                         int nB, size_t total, double eref) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    e[i] = T[i / DB] * cap[(i % DB) % nB];
+    e[i] = (T[i / DB] - eref) * cap[(i % DB) % nB];
 }
 
 // _get_gradient_larger, Order 2 (StaticBTESynthetic.cpp:1243-1340), for the
@@ -304,6 +311,7 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
         fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)(-nb - 1) * DB;
     }
     DirBand db(t0, nB);
+    To -= m.eref;  // the C T_old term of a deviation
     double vb = mvg[db.b], tau = mtau[db.b], s0 = To * mcap[db.b] + Qc * mhr[db.b] * mtau[db.b];
     // entries in batches of kSrcBatch: every load of a batch is issued before
     // its stores (the stores are counted by the same wait counter)
@@ -860,10 +868,10 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
     az = row_sum(az, R.tpr, sh);
     aR = row_sum(aR, R.tpr, sh);
     if (act && t0 == 0) {
-        T[c] = aT;
-        qx[c] = ax;
-        qy[c] = ay;
-        qz[c] = az;
+        T[c] = aT + m.offT;
+        qx[c] = ax + m.offQx;
+        qy[c] = ay + m.offQy;
+        qz[c] = az + m.offQz;
         RM[c] = aR;
     }
 }
@@ -905,7 +913,8 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
                                                           double *__restrict__ grad, TileDev T,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
-                                                          double *__restrict__ RMo, int DB, int ntiles, int vmeta_off) {
+                                                          double *__restrict__ RMo, double4 off, int DB, int ntiles,
+                                                          int vmeta_off) {
     extern __shared__ double lds[];
     int tile = xcd_block();
     if (tile >= ntiles) return;  // whole workgroup
@@ -1033,10 +1042,10 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
         const int *fi = T.fin_i + (size_t)c * kFinI;
         const double *fd = T.fin_d + (size_t)c * kFinD;
         const double *yo = Y + fi[0] * kTileJ;
-        To[c] = yo[4];
-        qxo[c] = yo[5];
-        qyo[c] = yo[6];
-        qzo[c] = yo[7];
+        To[c] = yo[4] + off.x;
+        qxo[c] = yo[5] + off.y;
+        qyo[c] = yo[6] + off.z;
+        qzo[c] = yo[7] + off.w;
         double r = 0;
         int nf = fi[1];
         for (int f = 0; f < nf; ++f) {
@@ -1067,10 +1076,10 @@ __global__ void __launch_bounds__(kRowBlockMax) k_output_moments(const double *_
     for (int b = t0; b < nB; b += R.tpr) {
         double s = 0;
         for (int d = 0; d < nD; ++d) s += ec[d * nB + b] * m.w[d] / m.cap[b];
-        if (act) Tb[(size_t)b * nc + c] = s;
+        if (act) Tb[(size_t)b * nc + c] = s + m.offW;
     }
     aE = row_sum(aE, R.tpr, sh);
-    if (act && t0 == 0) E[c] = aE;
+    if (act && t0 == 0) E[c] = aE + m.offE;
 }
 
 // _set_bound_ee_1 (StaticBTESynthetic.cpp:2083-2232) at np=1 and
@@ -1124,7 +1133,7 @@ __global__ void k_boundary(double *__restrict__ eb, double *__restrict__ tmac, M
     tm = group_sum(tm, b, nB);
     if (act && b == 0) {
         double ov = be_tov[en];
-        tmac[en] = isnan(ov) ? tm : ov;
+        tmac[en] = isnan(ov) ? tm + m.offMac : ov;
     }
 }
 
@@ -1658,6 +1667,29 @@ bool Solver::build() {
         if (!(vg[b] > 0)) { err_ = "group velocities must be positive"; return false; }
     std::vector<double> lrc(nB);
     for (int b = 0; b < nB; ++b) lrc[b] = lr[b] / cap[b];
+    // moments of the equilibrium C_b eref that the deviations leave out
+    {
+        eref_ = p.deck.tref;
+        double sT = 0, sE = 0, sW = 0, sQx = 0, sQy = 0, sQz = 0, sM = 0;
+        for (int d = 0; d < nD; ++d) {
+            sW += w[d];
+            for (int b = 0; b < nB; ++b) {
+                sT += w[d] * lrc[b] * cap[b];
+                sE += cap[b] * w[d] / P.capacity_bulk[0];
+                sQx += w[d] * vg[b] * sx[d] * cap[b];
+                sQy += w[d] * vg[b] * sy[d] * cap[b];
+                sQz += w[d] * vg[b] * sz[d] * cap[b];
+                sM += lr[b] * cap[b] / cap[b] * w[d];
+            }
+        }
+        offT_ = eref_ * sT;
+        offE_ = eref_ * sE;
+        offW_ = eref_ * sW;
+        offQ_[0] = eref_ * sQx;
+        offQ_[1] = eref_ * sQy;
+        offQ_[2] = eref_ * sQz;
+        offMac_ = eref_ * sM;
+    }
     if (!upload(d_vg, vg) || !upload(d_tau, tau) || !upload(d_cap, cap) || !upload(d_lr, lr) || !upload(d_hr, hr) ||
         !upload(d_sx, sx) || !upload(d_sy, sy) || !upload(d_sz, sz) || !upload(d_w, w) || !upload(d_lrc, lrc))
         return false;
@@ -2094,7 +2126,7 @@ bool Solver::build() {
                 int ty = p.bound_type[ib];
                 btype[en] = ty;
                 bnx[en] = p.face_nx[s]; bny[en] = p.face_ny[s]; bnz[en] = p.face_nz[s];
-                if (ty == 1) btw[en] = p.bound_temp[ib];
+                if (ty == 1) btw[en] = p.bound_temp[ib] - p.deck.tref;
                 if (ty < 0) {
                     int cb = p.bound_connect[ib];
                     bconn[en] = cb * 2 + ic;
@@ -2491,7 +2523,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
     Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
-            P.capacity_bulk[0]};
+            P.capacity_bulk[0], eref_, offT_, offE_, offW_, offQ_[0], offQ_[1], offQ_[2], offMac_};
     SlotGeo g{d_fsize.p, d_snb.p, d_sent.p, d_snx.p, d_sny.p, d_snz.p, d_sarea.p, d_vol.p,
               d_srx.p, d_sry.p, d_srz.p, d_sqx.p, d_sqy.p, d_sqz.p};
     FaceK fk{d_fkind.p, d_fother.p, d_fv0.p, d_fv1.p, d_fga.p, d_fn0x.p, d_fn0y.p, d_fmx.p, d_fmy.p};
@@ -2520,7 +2552,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     // _set_initial, gradients, outgoing and incoming boundary values
     // (StaticBTESyntheticSolver.cpp:19-62)
     hipLaunchKernelGGL(k_init_e, dim3(blocks_for((long)nE, kBlock)), dim3(kBlock), 0, s_, d_e.p, d_T.p, d_cap.p,
-                       nD * nB, nB, nE);
+                       nD * nB, nB, nE, m.eref);
     hipLaunchKernelGGL(k_gradient, dim3(gw), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p,
                        dim, nnz, invLx, nc, R);
     hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
@@ -2564,7 +2596,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p, d_t_vs_row.p,
                        d_t_ny.p,    d_t_yrow.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p,   d_t_kwf.p,     d_t_fin_d.p};
             hipLaunchKernelGGL(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, s_, d_e.p, d_eb.p,
-                               d_grad.p, td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nD * nB, n_tiles,
+                               d_grad.p, td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p,
+                               make_double4(m.offT, m.offQx, m.offQy, m.offQz), nD * nB, n_tiles,
                                kTileRows * kTileStride);
         } else {
             hipLaunchKernelGGL(one_row ? k_vertex_e<true> : k_vertex_e<false>, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p,
