@@ -26,11 +26,11 @@ int gbte_create(const gbte_problem *p, const gbte_options *opts, gbte_handle **o
     *out = nullptr;
     gbte_options o{};
     if (opts) o = *opts;
-    if (o.precision != GBTE_FP64) {
+    if (o.precision != GBTE_FP64 && o.precision != GBTE_FP32_SWEEP) {
         auto *h = new gbte_handle();
-        h->err = "only the FP64 sweep is available in this build";
+        h->err = "unknown precision option";
         *out = h;
-        return GBTE_UNSUPPORTED;
+        return GBTE_BAD_INPUT;
     }
     auto *h = new gbte_handle();
     h->s.reset(new gbte::Solver(p->p, o));

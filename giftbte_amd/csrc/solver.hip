@@ -109,6 +109,22 @@ struct Modes {
     double eref, offT, offE, offW, offQx, offQy, offQz, offMac;
 };
 
+// Storage of the per-system fields (e, its gradient, boundary values, sweep
+// sources): ST = double, or float for GBTE_FP32_SWEEP; arithmetic is FP64.
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+__device__ __forceinline__ double2 ld2(const float *p) {
+    float2 v = *reinterpret_cast<const float2 *>(p);
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st2_nt(double *p, double a, double b) {
+    __builtin_nontemporal_store(v2d{a, b}, reinterpret_cast<v2d *>(p));
+}
+__device__ __forceinline__ void st2_nt(float *p, double a, double b) {
+    __builtin_nontemporal_store(v2f{(float)a, (float)b}, reinterpret_cast<v2f *>(p));
+}
+
 struct SlotGeo {
     const int *fsize, *nb, *ent;
     const double *nx, *ny, *nz, *area, *vol, *rx, *ry, *rz, *qx, *qy, *qz;
@@ -148,7 +164,8 @@ __device__ __forceinline__ void row_lane(const Rows &R, int &row, int &t0) {
 
 // _set_initial (StaticBTESynthetic.cpp:765-782): e = T C_b for all systems
 // (held as (T - eref) C_b).
-__global__ void k_init_e(double *__restrict__ e, const double *__restrict__ T, const double *__restrict__ cap, int DB,
+template <typename ST>
+__global__ void k_init_e(ST *__restrict__ e, const double *__restrict__ T, const double *__restrict__ cap, int DB,
                          int nB, size_t total, double eref) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
@@ -159,14 +176,15 @@ __global__ void k_init_e(double *__restrict__ e, const double *__restrict__ T, c
 // whole row of one cell: grad = P [(e_j - e_i) / L_x].  Each thread owns kNT
 // row entries (stride tpr), so every stencil neighbour costs kNT independent
 // loads in flight.
-__device__ __forceinline__ void row_gradient(const double *__restrict__ e, double *__restrict__ grad, int c,
+template <typename ST>
+__device__ __forceinline__ void row_gradient(const ST *__restrict__ e, ST *__restrict__ grad, int c,
                                              const int *__restrict__ gptr, const int *__restrict__ gidx,
                                              const double *__restrict__ gP, const int *__restrict__ gok, int dim,
                                              int nnz, double invLx, int t0, int tpr, int DB) {
     int a = gptr[c], z = gptr[c + 1];
     bool ok = gok[c] != 0;
-    const double *ec = e + (size_t)c * DB;
-    double *gc = grad + (size_t)c * dim * DB;
+    const ST *ec = e + (size_t)c * DB;
+    ST *gc = grad + (size_t)c * dim * DB;
     for (int tb = t0; tb < DB; tb += kNT * tpr) {
         double e0[kNT], gx[kNT], gy[kNT], gz[kNT];
 #pragma unroll
@@ -177,7 +195,7 @@ __device__ __forceinline__ void row_gradient(const double *__restrict__ e, doubl
         }
         if (ok) {
             for (int k = a; k < z; ++k) {
-                const double *en = e + (size_t)gidx[k] * DB;
+                const ST *en = e + (size_t)gidx[k] * DB;
                 double p0 = gP[k], p1 = gP[(size_t)nnz + k], p2 = dim == 3 ? gP[2 * (size_t)nnz + k] : 0.0;
 #pragma unroll
                 for (int j = 0; j < kNT; ++j) {
@@ -204,7 +222,8 @@ __device__ __forceinline__ void row_gradient(const double *__restrict__ e, doubl
     }
 }
 
-__global__ void __launch_bounds__(kRowBlockMax) k_gradient(const double *__restrict__ e, double *__restrict__ grad,
+template <typename ST>
+__global__ void __launch_bounds__(kRowBlockMax) k_gradient(const ST *__restrict__ e, ST *__restrict__ grad,
                                                      const int *__restrict__ gptr, const int *__restrict__ gidx,
                                                      const double *__restrict__ gP, const int *__restrict__ gok,
                                                      int dim, int nnz, double invLx, int nc, Rows R) {
@@ -215,17 +234,18 @@ __global__ void __launch_bounds__(kRowBlockMax) k_gradient(const double *__restr
 
 // _get_bound_ee (StaticBTESynthetic.cpp:1502-1539): outgoing boundary values
 // e + r . grad e for every system, one row per boundary entry.
-__global__ void __launch_bounds__(kRowBlockMax) k_bound_out(const double *__restrict__ e, const double *__restrict__ grad,
-                                                      double *__restrict__ eb, SlotGeo g, Modes m,
+template <typename ST>
+__global__ void __launch_bounds__(kRowBlockMax) k_bound_out(const ST *__restrict__ e, const ST *__restrict__ grad,
+                                                      ST *__restrict__ eb, SlotGeo g, Modes m,
                                                       const int *__restrict__ ent_list, const int *__restrict__ be_cell,
                                                       const int *__restrict__ be_slot, int n_ent, int dim, Rows R) {
     int k = xcd_block() * R.rpb + threadIdx.x / R.tpr;
     if (k >= n_ent) return;
     int en = ent_list[k], c = be_cell[en], s = be_slot[en];
     double nx = g.nx[s], ny = g.ny[s], nz = g.nz[s], rx = g.rx[s], ry = g.ry[s], rz = g.rz[s];
-    const double *ec = e + (size_t)c * R.DB;
-    const double *gc = grad + (size_t)c * dim * R.DB;
-    double *eo = eb + (size_t)en * R.DB;
+    const ST *ec = e + (size_t)c * R.DB;
+    const ST *gc = grad + (size_t)c * dim * R.DB;
+    ST *eo = eb + (size_t)en * R.DB;
     for (int t = threadIdx.x % R.tpr; t < R.DB; t += R.tpr) {
         int d = t / R.B, b = t - d * R.B;
         double vb = m.vg[b];
@@ -258,8 +278,9 @@ constexpr int kSrcBatch = 8;  // entries per lane whose loads are issued before 
 // are copied to LDS first (dynamic LDS: 4 nB + 2 nD doubles, kSrcRows nD ints).
 constexpr int kSrcRows = 4;
 
-__global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict__ posc, double *__restrict__ S,
-                                                         const double *__restrict__ grad, const double *__restrict__ eb,
+template <typename ST>
+__global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict__ posc, ST *__restrict__ S,
+                                                         const ST *__restrict__ grad, const ST *__restrict__ eb,
                                                          const double *__restrict__ Told, const double *__restrict__ Q,
                                                          const int *__restrict__ fsize, const double *__restrict__ srec,
                                                          const int *__restrict__ srei, Modes m, int nc, Rows R) {
@@ -288,14 +309,14 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
     if (c >= nc) return;
     int nf = fsize[c];
     double To = Told[c], Qc = Q[c];
-    const double *gc = grad + (size_t)c * 2 * DB;
+    const ST *gc = grad + (size_t)c * 2 * DB;
     const int *pc = mpos + w * nD;
     const double *rd = srec + (size_t)c * kSrcD;
     const int *ri = srei + (size_t)c * kSrcI;
     // faces: normal, (A/V), r from this centre, r' from the neighbour's centre
     double fnx[kMaxFace2D], fny[kMaxFace2D], fav[kMaxFace2D], frx[kMaxFace2D], fry[kMaxFace2D], fqx[kMaxFace2D],
         fqy[kMaxFace2D];
-    const double *fin[kMaxFace2D];  // neighbour gradient row, or boundary-value row
+    const ST *fin[kMaxFace2D];  // neighbour gradient row, or boundary-value row
     int fnb[kMaxFace2D];
 #pragma unroll
     for (int f = 0; f < kMaxFace2D; ++f) {
@@ -329,7 +350,7 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
                 s0 = To * mcap[bb] + Qc * mhr[bb] * mtau[bb];
             }
             double sx = msx[d], sy = msy[d];
-            double2 gv = *reinterpret_cast<const double2 *>(gc + 2 * tt);
+            double2 gv = ld2(gc + 2 * tt);
             double gcx = gv.x, gcy = gv.y;
             double s = s0;
 #pragma unroll
@@ -340,7 +361,7 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
                 if (dot >= 0) {
                     s -= a * (frx[f] * gcx + fry[f] * gcy);
                 } else if (fnb[f] >= 0) {
-                    double2 nv = *reinterpret_cast<const double2 *>(fin[f] + 2 * tt);
+                    double2 nv = ld2(fin[f] + 2 * tt);
                     s -= a * (fqx[f] * nv.x + fqy[f] * nv.y);
                 } else {
                     s -= a * fin[f][tt];
@@ -352,7 +373,7 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
         }
 #pragma unroll
         for (int j = 0; j < kSrcBatch; ++j)
-            if (sa[j] != ~(size_t)0) __builtin_nontemporal_store(so[j], S + sa[j]);  // read by the next launch
+            if (sa[j] != ~(size_t)0) __builtin_nontemporal_store((ST)so[j], S + sa[j]);  // read by the next launch
     }
 }
 
@@ -501,18 +522,20 @@ struct PcSched {
     const int *farcell;  // far upwind cells, per sub-level runs
 };
 
+template <typename ST>
 struct PcProdBuf {
-    double s[kPcBpl];
+    ST s[kPcBpl];        // loads in the storage type, converted when staged
     int4 r0;             // cell,q0,q1,q2 (even lane) or g2,G (odd lane, as bits)
     int4 r1;             // g0,g1 (both lanes)
     int fc;              // far upwind cell
-    double fv;           // far upwind value
+    ST fv;               // far upwind value
 };
 
 __device__ __forceinline__ double pc_dbl(int lo, int hi) { return __hiloint2double(hi, lo); }
 
-__global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *__restrict__ S,
-                                                       double *__restrict__ e, Modes m, int nc, int nB, int nD, int Bc,
+template <typename ST>
+__global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__restrict__ S,
+                                                       ST *__restrict__ e, Modes m, int nc, int nB, int nD, int Bc,
                                                        int nchunk, int R) {
     extern __shared__ double lds[];
     int bid = gridDim.x % kXcd == 0 ? xcd_block() : (int)blockIdx.x;
@@ -536,14 +559,14 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
         // ---------------- producers
         int pl = threadIdx.x, p = pl >> 1, h = pl & 1;
         int fe = pl >> 4, fb = pl & 15;
-        PcProdBuf buf[kPcDepth];
+        PcProdBuf<ST> buf[kPcDepth];
         // loads of sub-level lv into buf (addresses clamped to valid memory)
-        auto issue = [&](PcProdBuf &B, int lv) {
+        auto issue = [&](PcProdBuf<ST> &B, int lv) {
             int l = lv < Lsub ? lv : Lsub - 1;
             int a0 = sl[l], np = sl[l + 1] - a0, f0 = fp[l], nf = fp[l + 1] - f0;
             int pp = p < np ? p : 0;
             size_t Pp = base + a0 + pp;
-            const double *srow = S + Pp * nB + b0;
+            const ST *srow = S + Pp * nB + b0;
 #pragma unroll
             for (int i = 0; i < kPcBpl; ++i) {
                 int jb = h * kPcBpl + i;
@@ -558,16 +581,16 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
         // far upwind cells of sub-level lv, loaded one sub-level before the
         // rest of lv (so that the dependent far-value load waits for a load
         // issued two sub-levels earlier)
-        auto issue_fc = [&](PcProdBuf &B, int lv) {
+        auto issue_fc = [&](PcProdBuf<ST> &B, int lv) {
             int l = lv < Lsub ? lv : Lsub - 1;
             int f0 = fp[l], nf = fp[l + 1] - f0;
             B.fc = P.farcell[f0 + (fe < nf ? fe : 0)];
         };
-        auto issue_far = [&](PcProdBuf &B) {
+        auto issue_far = [&](PcProdBuf<ST> &B) {
             B.fv = e[(size_t)B.fc * DB + (size_t)d * nB + b0 + (fb < Bc ? fb : 0)];
         };
         // stage sub-level lv (its loads have arrived) into buffer lv & 1
-        auto write = [&](const PcProdBuf &B, int lv) {
+        auto write = [&](const PcProdBuf<ST> &B, int lv) {
             if (lv >= Lsub) return;
             int a0 = sl[lv], np = sl[lv + 1] - a0, nf = fp[lv + 1] - fp[lv];
             double *st = stage + (size_t)(lv & 1) * kPcStage;
@@ -626,7 +649,7 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const double *
         int b = b0 + bl;
         bool act = slot < nslot && b < nB;
         double tvb = act ? m.tau[b] * m.vg[b] : 0.0;
-        double *ed = e + (size_t)d * nB + b;
+        ST *ed = e + (size_t)d * nB + b;
         __syncthreads();  // prologue
         for (int j = 0; j < nstep; ++j) {
             if (act && j < Lsub) {
@@ -900,7 +923,6 @@ constexpr int kStageBatch = 16;            // staged rows in flight per lane
 constexpr int kTileJ = 8;                  // moment factors K_0..K_7
 
 typedef double v4d __attribute__((ext_vector_type(4)));
-typedef double v2d __attribute__((ext_vector_type(2)));
 
 struct TileDev {
     const int *cell0, *rptr, *rsrc, *nh, *vptr, *vs_ptr, *vs_row, *ny, *yrow, *fin_i;
@@ -909,8 +931,9 @@ struct TileDev {
 
 __device__ __forceinline__ int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-__global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__restrict__ e, const double *__restrict__ eb,
-                                                          double *__restrict__ grad, TileDev T,
+template <typename ST>
+__global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restrict__ e, const ST *__restrict__ eb,
+                                                          ST *__restrict__ grad, TileDev T,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
                                                           double *__restrict__ RMo, double4 off, int DB, int ntiles,
@@ -961,21 +984,21 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
         for (int s = 0; s < kTileKs; ++s) kw[s] = s < nys ? kws[s * 64] : 0.0;
         __syncthreads();  // the previous slice is done with the LDS rows
         for (int r = w; r < nr; r += kStageBatch * kTileWaves) {
-            double x[kStageBatch];
+            ST x[kStageBatch];  // converted when written to LDS, so the loads stay batched
 #pragma unroll
             for (int j = 0; j < kStageBatch; ++j) {
                 int rr = r + j * kTileWaves;
                 x[j] = 0;
                 if (rr < nr) {
                     int src = lane_int(src0, rr);
-                    const double *row = src >= 0 ? e + (size_t)src * DB : eb + (size_t)(-src - 1) * DB;
+                    const ST *row = src >= 0 ? e + (size_t)src * DB : eb + (size_t)(-src - 1) * DB;
                     x[j] = row[tt];
                 }
             }
 #pragma unroll
             for (int j = 0; j < kStageBatch; ++j) {
                 int rr = r + j * kTileWaves;
-                if (rr < nr) lds[rr * kTileStride + lane] = x[j];
+                if (rr < nr) lds[rr * kTileStride + lane] = (double)x[j];
             }
         }
         __syncthreads();
@@ -1016,9 +1039,7 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
                 }
                 // streaming stores: the gradients are read back by the next
                 // launch, from HBM, and should not evict staged rows from L2
-                if (cl < ncell)
-                    __builtin_nontemporal_store(v2d{gx, gy},
-                                                reinterpret_cast<v2d *>(grad + (size_t)(c0 + cl) * 2 * DB + 2 * tg));
+                if (cl < ncell) st2_nt(grad + (size_t)(c0 + cl) * 2 * DB + 2 * tg, gx, gy);
             }
         }
     }
@@ -1061,7 +1082,8 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const double *__res
 // Output-only moments: energy-equivalent temperature (_get_total_energy,
 // StaticBTESynthetic.cpp:2498-2507) and band temperatures
 // (_recover_temperature, :2487-2496).
-__global__ void __launch_bounds__(kRowBlockMax) k_output_moments(const double *__restrict__ e, Modes m,
+template <typename ST>
+__global__ void __launch_bounds__(kRowBlockMax) k_output_moments(const ST *__restrict__ e, Modes m,
                                                                  double *__restrict__ E, double *__restrict__ Tb,
                                                                  int nc, Rows R) {
     __shared__ double sh[kRowBlockMax / 64];
@@ -1070,7 +1092,7 @@ __global__ void __launch_bounds__(kRowBlockMax) k_output_moments(const double *_
     int cc = act ? c : nc - 1;
     int DB = R.DB, nB = R.B, nD = DB / nB;
     int t0 = threadIdx.x % R.tpr;
-    const double *ec = e + (size_t)cc * DB;
+    const ST *ec = e + (size_t)cc * DB;
     double aE = 0;
     for (int t = t0; t < DB; t += R.tpr) aE += ec[t] * m.w[t / nB] / m.cbulk;
     for (int b = t0; b < nB; b += R.tpr) {
@@ -1087,7 +1109,8 @@ __global__ void __launch_bounds__(kRowBlockMax) k_output_moments(const double *_
 // boundary entry; directions are visited in increasing order, as the
 // reference's per-system Allgather makes later directions see earlier
 // updates (this matters only for specular partners that are incoming).
-__global__ void k_boundary(double *__restrict__ eb, double *__restrict__ tmac, Modes m,
+template <typename ST>
+__global__ void k_boundary(ST *__restrict__ eb, double *__restrict__ tmac, Modes m,
                            const int *__restrict__ ent_list, const int *__restrict__ be_type,
                            const int *__restrict__ be_conn, const int *__restrict__ be_spec,
                            const double *__restrict__ be_nx, const double *__restrict__ be_ny,
@@ -1102,7 +1125,7 @@ __global__ void k_boundary(double *__restrict__ eb, double *__restrict__ tmac, M
     double nx = be_nx[en], ny = be_ny[en], nz = be_nz[en];
     int ty = be_type[en];
     size_t DB = (size_t)nD * nB;
-    double *row = eb + (size_t)en * DB + b;
+    ST *row = eb + (size_t)en * DB + b;
     double tm = 0;
     if (act) {
         double diff = 0;
@@ -1518,6 +1541,7 @@ bool Solver::upload(DBuf<T> &b, const std::vector<T> &h) {
 }
 
 Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_new_, node_new_)), P(Pp), opt(o) {
+    fp32_ = opt.precision == GBTE_FP32_SWEEP;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         err_ = "no HIP device available";
@@ -2316,12 +2340,21 @@ bool Solver::build() {
     }
     // ---- state
     size_t nE = (size_t)nD * nc * nB;
-    if (!alloc(d_e, nE) || !alloc(d_grad, nE * dim) || !alloc(d_eb, (size_t)nD * ne * nB) ||
+    // FP32 sweeps: e, its gradient, the boundary values and S in float
+    // (buffers sized in doubles)
+    if (fp32_ && (opt.fallback_kernels || !use_pc || !use_tiles)) {
+        err_ = "FP32 sweeps need the tiled face kernel and the wave-specialised sweep (unsupported for this mesh or "
+               "with fallback kernels)";
+        return false;
+    }
+    auto half = [&](size_t n) { return fp32_ ? (n + 1) / 2 : n; };
+    size_t nEB = (size_t)nD * ne * nB;
+    if (!alloc(d_e, half(nE)) || !alloc(d_grad, half(nE * dim)) || !alloc(d_eb, half(nEB)) ||
         !alloc(d_eV, (size_t)nD * nn * nB) || !alloc(d_T, nc) || !alloc(d_Told, nc) || !alloc(d_T1, nc) ||
         !alloc(d_TV, nn) || !alloc(d_E, nc) || !alloc(d_qx, nc) || !alloc(d_qy, nc) || !alloc(d_qz, nc) ||
         !alloc(d_qxo, nc) || !alloc(d_qyo, nc) || !alloc(d_qzo, nc) || !alloc(d_RM, nc) || !alloc(d_Tb, (size_t)nB * nc) ||
         !alloc(d_tmac, ne) || !alloc(d_rhs, nc) || !alloc(d_r, nc) || !alloc(d_z, nc) || !alloc(d_p, nc) ||
-        !alloc(d_Ap, nc) || !alloc(d_S, nE))
+        !alloc(d_Ap, nc) || !alloc(d_S, half(nE)))
         return false;
     if (opt.linear_solver == GBTE_LINSOLVE_DIRECT &&
         (!alloc(d_ndy, nc) || !alloc(d_ndx, nc) || !alloc(d_ndu, (size_t)std::max<long>(1, std::max(NDmacro.u_total, NDfour.u_total)))))
@@ -2519,6 +2552,23 @@ int Solver::set_initial(const double *T, const double *qx, const double *qy, con
     return GBTE_OK;
 }
 
+// The per-system fields in their storage type, and a launch of a kernel
+// templated on it (FP64, or FP32 with GBTE_FP32_SWEEP).
+template <typename ST>
+static ST *rp(DBuf<double> &b) {
+    return reinterpret_cast<ST *>(b.p);
+}
+#define GBTE_LAUNCH_ST(kernel, grid, block, lds, ...)                             \
+    do {                                                                         \
+        if (fp32_) {                                                             \
+            using ST = float;                                                    \
+            hipLaunchKernelGGL(kernel<float>, grid, block, lds, s_, __VA_ARGS__); \
+        } else {                                                                 \
+            using ST = double;                                                   \
+            hipLaunchKernelGGL(kernel<double>, grid, block, lds, s_, __VA_ARGS__); \
+        }                                                                        \
+    } while (0)
+
 int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
@@ -2551,13 +2601,13 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     int bbw = blocks_for(blocks_for(n_ent, cpw), kWaves);
     // _set_initial, gradients, outgoing and incoming boundary values
     // (StaticBTESyntheticSolver.cpp:19-62)
-    hipLaunchKernelGGL(k_init_e, dim3(blocks_for((long)nE, kBlock)), dim3(kBlock), 0, s_, d_e.p, d_T.p, d_cap.p,
-                       nD * nB, nB, nE, m.eref);
-    hipLaunchKernelGGL(k_gradient, dim3(gw), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p,
-                       dim, nnz, invLx, nc, R);
-    hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
+    GBTE_LAUNCH_ST(k_init_e, dim3(blocks_for((long)nE, kBlock)), dim3(kBlock), 0, rp<ST>(d_e), d_T.p, d_cap.p,
+                   nD * nB, nB, nE, m.eref);
+    GBTE_LAUNCH_ST(k_gradient, dim3(gw), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), d_gptr.p, d_gidx.p, d_gP.p,
+                   d_gok.p, dim, nnz, invLx, nc, R);
+    GBTE_LAUNCH_ST(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), rp<ST>(d_eb), g, m,
                        d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
-    hipLaunchKernelGGL(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, s_, d_eb.p, d_tmac.p, m, d_ent_list.p,
+    GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m, d_ent_list.p,
                        d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p, d_be_tw.p, d_be_dT.p,
                        d_be_tov.p, n_ent, nB, nD, cpw);
     if (!check(hipGetLastError(), "setup kernels")) return GBTE_HIP_ERROR;
@@ -2574,13 +2624,14 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
         (void)hipEventRecord(ev_[0], s_);
-        hipLaunchKernelGGL(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows),
-                           (size_t)(4 * nB + 2 * nD) * sizeof(double) + (size_t)kSrcRows * nD * sizeof(int), s_, d_posc.p, d_S.p,
-                           d_grad.p, d_eb.p, d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc, Rsrc);
+        GBTE_LAUNCH_ST(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows),
+                       (size_t)(4 * nB + 2 * nD) * sizeof(double) + (size_t)kSrcRows * nD * sizeof(int), d_posc.p,
+                       rp<ST>(d_S), rp<ST>(d_grad), rp<ST>(d_eb), d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc,
+                       Rsrc);
         (void)hipEventRecord(ev_[1], s_);
         if (use_pc) {
             PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
-            hipLaunchKernelGGL(k_sweep_pc, dim3(nD * pc_nchunk), dim3(kPcBlock), pc_lds, s_, ps, d_S.p, d_e.p, m, nc,
+            GBTE_LAUNCH_ST(k_sweep_pc, dim3(nD * pc_nchunk), dim3(kPcBlock), pc_lds, ps, rp<ST>(d_S), rp<ST>(d_e), m, nc,
                                nB, nD, pc_bc, pc_nchunk, pc_ring);
         } else {
             hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
@@ -2589,14 +2640,15 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                                sweep_nchunk, sweep_ring);
         }
         (void)hipEventRecord(ev_[2], s_);
-        hipLaunchKernelGGL(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, s_, d_e.p, d_grad.p, d_eb.p, g, m,
+        GBTE_LAUNCH_ST(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), rp<ST>(d_eb), g,
+                       m,
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
         (void)hipEventRecord(ev_[3], s_);
         if (use_tiles) {
             TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p, d_t_vs_row.p,
                        d_t_ny.p,    d_t_yrow.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p,   d_t_kwf.p,     d_t_fin_d.p};
-            hipLaunchKernelGGL(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, s_, d_e.p, d_eb.p,
-                               d_grad.p, td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p,
+            GBTE_LAUNCH_ST(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, rp<ST>(d_e), rp<ST>(d_eb),
+                           rp<ST>(d_grad), td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p,
                                make_double4(m.offT, m.offQx, m.offQy, m.offQz), nD * nB, n_tiles,
                                kTileRows * kTileStride);
         } else {
@@ -2607,7 +2659,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                                d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc, R);
         }
         (void)hipEventRecord(ev_[4], s_);
-        hipLaunchKernelGGL(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, s_, d_eb.p, d_tmac.p, m,
+        GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m,
                            d_ent_list.p, d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p,
                            d_be_tw.p, d_be_dT.p, d_be_tov.p, n_ent, nB, nD, cpw);
         (void)hipEventRecord(ev_[5], s_);
@@ -2715,7 +2767,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         if (errT <= tolT && errQ <= tolQ) { conv = true; ++nt; break; }
         if (errInc_ >= 10) { status = GBTE_DIVERGED; err_ = "Error increases for 10 times"; ++nt; break; }
     }
-    hipLaunchKernelGGL(k_output_moments, dim3(gw), dim3(rowblk), 0, s_, d_e.p, m, d_E.p, d_Tb.p, nc, R);
+    GBTE_LAUNCH_ST(k_output_moments, dim3(gw), dim3(rowblk), 0, rp<ST>(d_e), m, d_E.p, d_Tb.p, nc, R);
     if (!check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
     auto t1 = std::chrono::high_resolution_clock::now();
     have_state = true;

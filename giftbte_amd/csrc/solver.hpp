@@ -160,6 +160,7 @@ class Solver {
     // CG / reduction scratch
     DBuf<double> d_r, d_z, d_p, d_Ap, d_part, d_part2;
     double *h_pin = nullptr;  // pinned host scratch
+    bool fp32_ = false;   // GBTE_FP32_SWEEP: per-system fields stored in float
     double eref_ = 300.0;  // energy densities are held as e - C_b eref_
     double offT_ = 0, offE_ = 0, offW_ = 0, offQ_[3] = {0, 0, 0}, offMac_ = 0;  // moments of C_b eref_
     double *h_part = nullptr;  // pinned copies of the synthetic passes' change partials (2 slots)
