@@ -9,8 +9,11 @@ seeded generator (giftbte_amd/decks.py).
 A "step" is one outer iteration of StaticBTESynthetic::solve: every
 band x direction transport sweep, the moment reductions, the boundary update
 and the FP64 synthetic (macro) solve.  value = cell*band*direction updates/s
-over the timed steps (inputs resident in HBM).  The time to a converged k_eff
-(the deck's tolerances, from the Fourier warm start) is reported beside it.
+of the plain FP64 solver over the timed steps (inputs resident in HBM); the
+same figures for FP32 storage of the per-system fields (double arithmetic,
+recovery, synthetic solve and reductions) are under "mixed".  The time to a
+converged k_eff (the deck's tolerances, from the Fourier warm start) is
+reported for both, with the mixed k_eff's error against FP64.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-converge] [--no-cpu]
 """
@@ -61,13 +64,13 @@ def cpu_baseline(problem, T0, target_s=15.0):
                       f"{sec:.1f} s"}
 
 
-def kernel_bytes_per_update(dim):
+def kernel_bytes_per_update(dim, word=8):
     """Algorithmic HBM bytes per cell*band*dir update of each transport kernel
-    (FP64): the source kernel reads the LSQ gradient (dim values) and writes
-    the sweep source; the sweep reads the source and writes e; the face /
-    moment kernel reads e and writes the gradient.  Tables shared by all
-    band x direction systems are excluded."""
-    return {"k_source": (dim + 1) * 8, "k_sweep_pc": 2 * 8, "k_tile_face": (1 + dim) * 8}
+    (word = 8 for FP64 storage, 4 for FP32): the source kernel reads the LSQ
+    gradient (dim values) and writes the sweep source; the sweep reads the
+    source and writes e; the face / moment kernel reads e and writes the
+    gradient.  Tables shared by all band x direction systems are excluded."""
+    return {"k_source": (dim + 1) * word, "k_sweep_pc": 2 * word, "k_tile_face": (1 + dim) * word}
 
 
 def load_traffic(kernel, cfg):
@@ -124,59 +127,76 @@ def main():
         nc, nB, nD = t.n_cells, t.n_bands, t.n_dirs
         print(f"[bench] deck {nc} cells x {nB} bands x {nD} directions ({time.time() - t_gen:.1f} s set-up)",
               file=sys.stderr, flush=True)
-        t_c = time.time()
-        s = BTESolver(p)
-        print(f"[bench] solver set-up {time.time() - t_c:.2f} s", file=sys.stderr, flush=True)
-        t0 = time.time()
-        fp = s.fourier()
-        print(f"[bench] fourier {fp} passes {time.time() - t0:.2f} s", file=sys.stderr, flush=True)
-        T0 = s.fields()["T_lattice"]
-        st_w = s.solve(a.warmup, 0.0, 0.0)
-        print(f"[bench] warmup {st_w.iterations} it {st_w.solve_ms:.1f} ms", file=sys.stderr, flush=True)
-        t1 = time.time()
-        st = s.solve(a.steps, 0.0, 0.0)  # returns after the device has finished
-        wall = time.time() - t1
         U = nc * nB * nD
         K = a.steps
         cfg = {"workload": "2D porous film 500 nm, r=150 nm diffuse pore, seeded Delaunay, 40 bands, 64 dirs",
                "cells": nc, "bands": nB, "directions": nD, "systems": nB * nD, "updates_per_step": U,
                "parallelism": "single GPU"}
-        bpu = kernel_bytes_per_update(t.dim)
-        ktime = {"k_source": st.source_ms, "k_sweep_pc": st.sweep_ms, "k_tile_face": st.face_ms}
-        kernels = {}
-        for k, ms in ktime.items():
-            per = ms / K
-            gbs = bpu[k] * U / (per * 1e-3) / 1e9 if per > 0 else 0.0
-            kernels[k] = {"ms_per_launch": per, "bytes_per_update": bpu[k], "achieved_GBs": gbs,
-                          "frac_hbm_peak": gbs / HBM_PEAK_GBS}
-        dom = max(ktime, key=ktime.get)
+        runs = {}
+        T0 = None
+        for prec in ("fp64", "fp32"):
+            t_c = time.time()
+            s = BTESolver(p, precision=prec)
+            fp = s.fourier()
+            if T0 is None:
+                T0 = s.fields()["T_lattice"]
+            st_w = s.solve(a.warmup, 0.0, 0.0)
+            print(f"[bench] {prec}: set-up + fourier ({fp} passes) {time.time() - t_c:.2f} s, warmup "
+                  f"{st_w.iterations} it {st_w.solve_ms:.1f} ms", file=sys.stderr, flush=True)
+            t1 = time.time()
+            st = s.solve(a.steps, 0.0, 0.0)  # returns after the device has finished
+            wall = time.time() - t1
+            s.close()
+            bpu = kernel_bytes_per_update(t.dim, 8 if prec == "fp64" else 4)
+            ktime = {"k_source": st.source_ms, "k_sweep_pc": st.sweep_ms, "k_tile_face": st.face_ms}
+            kernels = {}
+            for k, ms in ktime.items():
+                per = ms / K
+                gbs = bpu[k] * U / (per * 1e-3) / 1e9 if per > 0 else 0.0
+                kernels[k] = {"ms_per_launch": per, "bytes_per_update": bpu[k], "achieved_GBs": gbs,
+                              "frac_hbm_peak": gbs / HBM_PEAK_GBS}
+            dom = max(ktime, key=ktime.get)
+            r = {"value": U * K / wall, "ms_per_step": wall * 1e3 / K,
+                 "breakdown_ms_per_step": {"source": st.source_ms / K, "sweep": st.sweep_ms / K,
+                                           "face_moments": st.face_ms / K, "transport": st.transport_ms / K,
+                                           "macro": st.macro_ms / K, "macro_passes_per_step": st.macro_passes / K,
+                                           "sweep_levels": st.sweep_levels},
+                 "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"],
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac_hbm_peak"],
+                              "traffic": load_traffic(dom, cfg) if prec == "fp64" else None,
+                              "bytes_per_update": bpu[dom], "updates_per_launch": U},
+                 "kernels": kernels}
+            if not a.no_converge:
+                s2 = BTESolver(p, precision=prec)
+                tc0 = time.time()
+                s2.fourier()
+                stc = s2.solve(t.max_iter, t.tol_temp, t.tol_flux)
+                r["time_to_converged_s"] = time.time() - tc0
+                r["converged"] = bool(stc.converged)
+                r["converged_iterations"] = stc.iterations
+                r["keff_W_per_mK"] = s2.keff(0, 2.0)
+                s2.close()
+                print(f"[bench] {prec}: converged in {stc.iterations} it, {r['time_to_converged_s']:.3f} s",
+                      file=sys.stderr, flush=True)
+            runs[prec] = r
+        f64, f32 = runs["fp64"], runs["fp32"]
         out = {
-            "metric": "cell*band*dir updates/s (steady StaticBTESynthetic outer iterations, FP64)",
-            "value": U * K / wall, "unit": "cell*band*dir updates/s", "n_gpus": 1, "steps": K, "warmup": a.warmup,
-            "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "metric": "time-to-converged k_eff (1e-6 rel) & cell\u00b7band\u00b7dir updates/s, mixed vs FP64",
+            "value": f64["value"], "unit": "cell*band*dir updates/s", "n_gpus": 1, "steps": K, "warmup": a.warmup,
+            "ms_per_step": f64["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic (seeded porous Delaunay deck, seeded Debye-like bands)",
             "config": cfg,
-            "breakdown_ms_per_step": {"source": st.source_ms / K, "sweep": st.sweep_ms / K,
-                                      "face_moments": st.face_ms / K, "transport": st.transport_ms / K,
-                                      "macro": st.macro_ms / K, "macro_passes_per_step": st.macro_passes / K,
-                                      "sweep_levels": st.sweep_levels},
-            "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"],
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac_hbm_peak"],
-                         "traffic": load_traffic(dom, cfg),
-                         "bytes_per_update": bpu[dom], "updates_per_launch": U},
-            "kernels": kernels,
         }
-        if not a.no_converge:
-            s2 = BTESolver(p)
-            tc0 = time.time()
-            s2.fourier()
-            stc = s2.solve(t.max_iter, t.tol_temp, t.tol_flux)
-            out["time_to_converged_s"] = time.time() - tc0
-            out["converged"] = bool(stc.converged)
-            out["converged_iterations"] = stc.iterations
-            out["keff_W_per_mK"] = s2.keff(0, 2.0)
-            print(f"[bench] converged in {stc.iterations} it, {out['time_to_converged_s']:.2f} s", file=sys.stderr,
-                  flush=True)
+        out.update({k: v for k, v in f64.items() if k not in ("value", "ms_per_step")})
+        mixed = {"dtype": "f32 storage of e, gradients and sources; f64 arithmetic, recovery, synthetic solve "
+                          "and reductions",
+                 "value": f32["value"], "ms_per_step": f32["ms_per_step"],
+                 "speedup_updates_vs_fp64": f32["value"] / f64["value"]}
+        mixed.update({k: v for k, v in f32.items() if k not in ("value", "ms_per_step")})
+        if "keff_W_per_mK" in f64:
+            mixed["keff_rel_err_vs_fp64"] = abs(f32["keff_W_per_mK"] - f64["keff_W_per_mK"]) / abs(f64["keff_W_per_mK"])
+            mixed["speedup_time_to_converged_vs_fp64"] = f64["time_to_converged_s"] / f32["time_to_converged_s"]
+        out["mixed"] = mixed
         if not a.no_cpu:
             out["cpu_baseline"] = cpu_baseline(p, T0)
         print(json.dumps(out))

@@ -91,6 +91,8 @@ class BTESolver:
         self.nc, self.nb = t.n_cells, t.n_bands
         if linear_solver not in ("direct", "pcg"):
             raise ValueError("linear_solver must be 'direct' or 'pcg'")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32' (FP32 storage of the per-system fields)")
         opts = _lib.Options(_lib.GBTE_FP64 if precision == "fp64" else _lib.GBTE_FP32_SWEEP, device, int(verbose),
                             macro_rtol,
                             _lib.GBTE_LINSOLVE_DIRECT if linear_solver == "direct" else _lib.GBTE_LINSOLVE_PCG,

@@ -32,6 +32,13 @@ int gbte_create(const gbte_problem *p, const gbte_options *opts, gbte_handle **o
         *out = h;
         return GBTE_BAD_INPUT;
     }
+    if (o.precision == GBTE_FP32_SWEEP && o.fallback_kernels) {
+        auto *h = new gbte_handle();
+        h->err = "FP32 storage needs the tiled face kernel and the wave-specialised sweep, which fallback_kernels "
+                 "turns off";
+        *out = h;
+        return GBTE_BAD_INPUT;
+    }
     auto *h = new gbte_handle();
     h->s.reset(new gbte::Solver(p->p, o));
     *out = h;

@@ -35,7 +35,16 @@ enum {
     GBTE_UNSUPPORTED = 5  /* a reference feature outside this build             */
 };
 
-/* Precision of the transport sweep. */
+/* Storage of the per-(band, direction) fields: energy densities, boundary
+ * values, LSQ gradients and sweep sources.  GBTE_FP32_SWEEP stores them in
+ * float (half the HBM traffic of the transport kernels) and keeps every
+ * arithmetic operation, the temperature / flux recovery, the synthetic
+ * solve and the residual and k_eff reductions in double.  The energy
+ * densities are held as deviations from the equilibrium at the deck's Tref,
+ * so float keeps about 7 significant digits of the deviation itself.
+ * GBTE_FP32_SWEEP needs the tiled face kernel and the wave-specialised sweep
+ * (gbte_create returns GBTE_BAD_INPUT with fallback_kernels or on a deck
+ * those kernels do not fit).  The reference computes in double throughout. */
 enum { GBTE_FP64 = 0, GBTE_FP32_SWEEP = 1 };
 
 /* Linear solver of the synthetic (macro) and Fourier systems, which the

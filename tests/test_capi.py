@@ -81,3 +81,18 @@ def test_solver_without_gpu_fails_loudly(tmp_path):
     assert rc == _lib.GBTE_HIP_ERROR and "HIP device" in msg
     with pytest.raises(GBTEError):
         BTESolver(p)
+
+
+def test_option_errors_are_bad_input(tmp_path):
+    """Option checks come before any device work, so they hold without a GPU."""
+    lib = _lib.load()
+    p = Problem(decks.write_deck(str(tmp_path / "o"), decks.gray_square_spec(3, nphi=2)))
+    for opts, word in ((_lib.Options(7, 0, 0, 0.0), "precision"),
+                       (_lib.Options(_lib.GBTE_FP32_SWEEP, 0, 0, 0.0, 0, 1), "fallback_kernels")):
+        h = C.c_void_p()
+        rc = lib.gbte_create(p.handle, C.byref(opts), C.byref(h))
+        msg = lib.gbte_last_error(h).decode()
+        lib.gbte_destroy(h)
+        assert rc == _lib.GBTE_BAD_INPUT and word in msg
+    with pytest.raises(ValueError):
+        BTESolver(p, precision="fp16")

@@ -1,0 +1,90 @@
+"""FP32 storage of the per-system fields (GBTE_FP32_SWEEP) vs the FP64 oracle.
+
+The energy densities, boundary values, LSQ gradients and sweep sources are
+stored in float; every operation, the recovery of temperatures and fluxes,
+the synthetic solve and the reductions stay in double.  Bar: converged
+k_eff within 1e-6 relative of the FP64 result, temperature and flux fields
+within 1e-6 relative L2 of the oracle, and the same outer-iteration,
+synthetic-pass and Fourier-pass counts as the oracle (a run that stops on
+the deck tolerance may end one iteration either side).  Measured errors
+are printed per deck.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import _oracle
+import decklib
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-6
+KEFF_TOL = 1e-6
+
+
+def _keff_ref(p, ref, dT):
+    vol = p.array("volume")
+    ys = p.array("node_y")
+    return float((ref["qx"] * vol).sum() / ((ys.max() - ys.min()) * dT))
+
+
+@pytest.mark.parametrize("name", list(decklib.DECKS))
+def test_fp32_fields_match_oracle(name, deck_root):
+    from giftbte_amd import Problem, BTESolver
+    d = decklib.make(name, deck_root)
+    s = decklib.settings(name)
+    print(f"\n[gpu mixed] deck={name} settings={s}", flush=True)
+    p = Problem(d)
+    sol = BTESolver(p, precision="fp32")
+    fp = sol.fourier()
+    st = sol.solve(s["max_iter"], s["tol_temp"], s["tol_flux"])
+    ref, info = _oracle.run(p.tables, s["max_iter"], s["tol_temp"], s["tol_flux"])
+    assert fp == info["fourier_passes"]
+    if s["tol_temp"] > 0:
+        assert abs(st.iterations - info["iterations"]) <= 1
+    else:
+        assert st.iterations == info["iterations"]
+        assert st.macro_passes == info["macro_passes"]
+    f = sol.fields()
+    errs = {k: _oracle.rel_l2(f[k], ref[k]) for k in ("T_lattice", "T_energy", "T_band")}
+    q = np.concatenate([f["qx"], f["qy"], f["qz"]])
+    qr = np.concatenate([ref["qx"], ref["qy"], ref["qz"]])
+    errs["q"] = _oracle.rel_l2(q, qr)
+    bt = p.array("bound_type")
+    if (bt == 1).sum() and np.ptp(p.array("bound_temp")[bt == 1]) > 0:
+        k_ref = _keff_ref(p, ref, 2.0)
+        errs["keff"] = abs(sol.keff(0, 2.0) - k_ref) / abs(k_ref)
+    print("[gpu mixed] " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for k, v in errs.items():
+        assert v < (KEFF_TOL if k == "keff" else FIELD_TOL), k
+
+
+def test_fp32_converged_keff_porous50k(deck_root):
+    """The 2D porous film at full size (about 50k triangles, 40 bands, 64
+    directions), FP32 storage vs FP64 on the GPU (itself pinned to the oracle
+    above), both run from the Fourier warm start to the deck tolerance."""
+    from giftbte_amd import Problem, BTESolver, decks
+    d = os.path.join(str(deck_root), "porous50k")
+    if not os.path.exists(os.path.join(d, "input", "CONTROL")):
+        decks.write_deck(d, decks.porous_spec(n_target=50000, nbands=40, nphi=16, pore_radius=0.3, seed=1,
+                                              scale=5e-7))
+    p = Problem(d)
+    t = p.tables
+    out = {}
+    for prec in ("fp64", "fp32"):
+        print(f"\n[gpu mixed] porous50k {prec}", flush=True)
+        sol = BTESolver(p, precision=prec)
+        sol.fourier()
+        st = sol.solve(t.max_iter, t.tol_temp, t.tol_flux)
+        assert st.converged
+        out[prec] = (st.iterations, sol.keff(0, 2.0), sol.fields())
+        sol.close()
+    (i64, k64, f64), (i32, k32, f32) = out["fp64"], out["fp32"]
+    rel = abs(k32 - k64) / abs(k64)
+    eT = _oracle.rel_l2(f32["T_lattice"], f64["T_lattice"])
+    print(f"[gpu mixed] iterations {i64}/{i32} keff {k64:.9g}/{k32:.9g} rel {rel:.2e} T {eT:.2e}", flush=True)
+    assert abs(i32 - i64) <= 1
+    assert rel < KEFF_TOL
+    assert eT < FIELD_TOL
+
