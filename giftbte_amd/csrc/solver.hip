@@ -902,16 +902,18 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
 // LDS-staged variant of k_vertex_e + k_face_moments over cell tiles
 // (tiles.cpp).  One workgroup per tile walks the row in 64-entry slices.  For
 // each slice the tile's neighbour rows [cells | boundary entries] are staged
-// in LDS and the inverse-distance values of the tile's face vertices are
-// formed there.  The two linear maps of the slice then run on the matrix
-// cores (v_mfma_f64_16x16x4f64):
+// in LDS and two linear maps of the slice run on the matrix cores:
 //   LSQ gradient  G[cell + 8 comp][t] = sum_r C[cell + 8 comp][r] E[r][t]
-//                 over the tile's cell rows, one 16-entry block per wave;
-//   moments       Y[j][y] += sum_t K_j(t) E[yrow[y]][t], accumulated in the
-//                 matrix registers over all slices, with K_j(t) the
-//                 per-(direction, band) factors of T, q and of the residual's
-//                 face terms (a table built with the modes).
-// After the last slice each cell's T, q and non-Fourier residual are short
+//                 over the tile's cell rows, one 16-entry block per wave
+//                 (v_mfma_f64_16x16x4f64);
+//   moments       Y[j][r] += sum_t K_j(t) E[r][t] over the staged rows,
+//                 accumulated in the matrix registers over all slices, with
+//                 K_j(t) the per-(direction, band) factors of T, q and of the
+//                 residual's face terms (v_mfma_f64_4x4x4f64: four j per
+//                 product and no idle rows).
+// The face vertices' inverse-distance values enter only through their
+// moments, so after the last slice a vertex's Y is the weighted sum of its
+// sources' Y, and each cell's T, q and non-Fourier residual are short
 // combinations of Y entries (tiles.hpp).
 constexpr int kTileW = 64;
 constexpr int kTileStride = kTileW + 1;  // LDS row stride: the moment product reads 16 rows of one column
@@ -920,12 +922,14 @@ constexpr int kTileWaves = kTileBlock / 64;
 constexpr int kTileCells = kTileCellsMax;  // cells per tile
 constexpr int kTileRows = kTileRowsMax;    // rows per tile: 33 KiB of LDS
 constexpr int kStageBatch = 16;            // staged rows in flight per lane
-constexpr int kTileJ = 8;                  // moment factors K_0..K_7
+constexpr int kTileJ = 8;                  // moment factors K_0..K_7, two groups of four
+static_assert(kTileRows == 4 * 16, "four 16-row blocks of moments");
+static_assert(2 * kYRows * kTileJ * 2 <= kTileRows * kTileStride, "moment partials fit in the row buffer");
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 struct TileDev {
-    const int *cell0, *rptr, *rsrc, *nh, *vptr, *vs_ptr, *vs_row, *ny, *yrow, *fin_i;
+    const int *cell0, *rptr, *rsrc, *nh, *vptr, *vs_ptr, *vs_row, *fin_i;
     const double *vs_w, *cmf, *kwf, *fin_d;
 };
 
@@ -936,8 +940,7 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restric
                                                           ST *__restrict__ grad, TileDev T,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
-                                                          double *__restrict__ RMo, double4 off, int DB, int ntiles,
-                                                          int vmeta_off) {
+                                                          double *__restrict__ RMo, double4 off, int DB, int ntiles) {
     extern __shared__ double lds[];
     int tile = xcd_block();
     if (tile >= ntiles) return;  // whole workgroup
@@ -947,31 +950,27 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restric
     int v0 = T.vptr[tile], nv = T.vptr[tile + 1] - v0;
     int c0 = T.cell0[tile], ncell = T.cell0[tile + 1] - c0;
     int nks = (T.nh[tile] + 3) >> 2;  // LSQ k-steps: rows [0, 4 nks)
-    int ny = T.ny[tile];
-    int wpb = ny <= 16 ? 4 : ny <= 32 ? 2 : 1;  // waves per 16-row block of Y
-    int yb = w / wpb, nys = kTileKs / wpb, ys0 = (w - yb * wpb) * nys;
-    // vertex stencils of the tile in LDS: ptr[nv + 1] | rows | weights
-    int vs0 = T.vs_ptr[v0], nvs = T.vs_ptr[v0 + nv] - vs0;
-    int *vm_ptr = reinterpret_cast<int *>(lds + vmeta_off);
-    int *vm_row = vm_ptr + nv + 1;
-    double *vm_w = lds + vmeta_off + (nv + 1 + nvs + 1) / 2;
-    for (int i = threadIdx.x; i <= nv; i += blockDim.x) vm_ptr[i] = T.vs_ptr[v0 + i] - vs0;
-    for (int i = threadIdx.x; i < nvs; i += blockDim.x) {
-        vm_row[i] = T.vs_row[vs0 + i];
-        vm_w[i] = T.vs_w[vs0 + i];
-    }
-    // rows [nr + nv, 4 nks) enter the LSQ product with zero weight and are
-    // written by nothing else: clear them once
-    for (int r = nr + nv + w; r < 4 * nks; r += kTileWaves) lds[r * kTileStride + lane] = 0.0;
+    // moment work: (16-row block, j group) units, wave w takes group w & 1 of
+    // blocks w >> 1 and (w >> 1) + 2; a single block is split over the
+    // k-steps instead (halves ks0 = 0, 8)
+    int nyb = (nr + 15) >> 4;
+    int yg = w & 1;
+    int yb = nyb == 1 ? 0 : w >> 1;
+    int ks0 = nyb == 1 ? 8 * (w >> 1) : 0, nys = nyb == 1 ? 8 : kTileKs;
+    bool y0 = yb < nyb, y1 = nyb > 2 && yb + 2 < nyb;
+    // rows [nr, kTileRows) are read by the products with zero weight (LSQ) or
+    // into unused columns (moments) and are written by nothing else: clear
+    // them once
+    for (int r = nr + w; r < kTileRows; r += kTileWaves) lds[r * kTileStride + lane] = 0.0;
     // LSQ coefficients (A operands), the same for every slice
     double cm[kTileKs];
     const double *cmt = T.cmf + (size_t)tile * kTileKs * 64 + lane;
 #pragma unroll
     for (int k = 0; k < kTileKs; ++k) cm[k] = k < nks ? cmt[k * 64] : 0.0;
     // B operands: Y reads 16 rows of one column, G 4 rows of 16 columns
-    const double *yrd = lds + T.yrow[(size_t)tile * kYRows + yb * 16 + l16] * kTileStride + lq;
+    const double *yrd = lds + (16 * yb + l16) * kTileStride + 4 * ks0 + lq;
     const double *grd = lds + lq * kTileStride + 16 * w + l16;
-    v4d accY = {0.0, 0.0, 0.0, 0.0};
+    double acc0 = 0.0, acc1 = 0.0;
     int src0 = lane < nr ? T.rsrc[r0 + lane] : 0;
     int nslice = (DB + kTileW - 1) / kTileW;
     for (int slice = 0; slice < nslice; ++slice) {
@@ -979,7 +978,7 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restric
         int tt = t < DB ? t : 0;
         // this wave's K fragments of the slice
         double kw[kTileKs];
-        const double *kws = T.kwf + ((size_t)slice * kTileKs + ys0) * 64 + lane;
+        const double *kws = T.kwf + (((size_t)slice * 2 + yg) * kTileKs + ks0) * 64 + lane;
 #pragma unroll
         for (int s = 0; s < kTileKs; ++s) kw[s] = s < nys ? kws[s * 64] : 0.0;
         __syncthreads();  // the previous slice is done with the LDS rows
@@ -1002,25 +1001,13 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restric
             }
         }
         __syncthreads();
-        for (int i = w; i < nv; i += kTileWaves) {
-            int a = vm_ptr[i], z = vm_ptr[i + 1];  // z - a is a multiple of 4
-            double sm = 0;
-            for (int k = a; k < z; k += 4) {
-                double u0 = lds[vm_row[k] * kTileStride + lane], u1 = lds[vm_row[k + 1] * kTileStride + lane];
-                double u2 = lds[vm_row[k + 2] * kTileStride + lane], u3 = lds[vm_row[k + 3] * kTileStride + lane];
-                sm += u0 * vm_w[k];
-                sm += u1 * vm_w[k + 1];
-                sm += u2 * vm_w[k + 2];
-                sm += u3 * vm_w[k + 3];
-            }
-            if (isnan(sm)) sm = 0;
-            lds[(nr + i) * kTileStride + lane] = sm;
-        }
-        __syncthreads();
-        // moments: k-steps [ys0, ys0 + nys) of row block yb
+        // moments: k-steps [ks0, ks0 + nys) of group yg, blocks yb and yb + 2
 #pragma unroll
         for (int s = 0; s < kTileKs; ++s)
-            if (s < nys) accY = __builtin_amdgcn_mfma_f64_16x16x4f64(kw[s], yrd[4 * (ys0 + s)], accY, 0, 0, 0);
+            if (s < nys) {
+                if (y0) acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], yrd[4 * s], acc0, 0, 0, 0);
+                if (y1) acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], yrd[32 * kTileStride + 4 * s], acc1, 0, 0, 0);
+            }
         // LSQ gradient of entries [16 w, 16 w + 16) of the slice
         v4d g = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1043,19 +1030,27 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restric
             }
         }
     }
-    // Y partials (lane holds Y[lq + 4 i][yb * 16 + l16]) -> LDS, summed over
-    // the waves of a row block
+    // moment partials (lane holds Y[4 yg + lq][16 block + l16]) -> LDS
+    // Yp[part][row][j], part 1 only for a single block split over k-steps
     __syncthreads();
-    double *Yp = lds, *Y = lds + kTileWaves * 16 * kTileJ;
-#pragma unroll
-    for (int i = 0; i < 2; ++i) Yp[(w * 16 + l16) * kTileJ + lq + 4 * i] = accY[i];
+    double *Yp = lds, *Y = lds + 2 * kYRows * kTileJ;
+    int part = nyb == 1 ? w >> 1 : 0;
+    if (y0) Yp[(part * kYRows + 16 * yb + l16) * kTileJ + 4 * yg + lq] = acc0;
+    if (y1) Yp[(part * kYRows + 16 * (yb + 2) + l16) * kTileJ + 4 * yg + lq] = acc1;
     __syncthreads();
-    int nyr = (kTileWaves / wpb) * 16;
-    for (int idx = threadIdx.x; idx < nyr * kTileJ; idx += kTileBlock) {
-        int y = idx / kTileJ, j = idx - y * kTileJ, blk = y >> 4;
-        double sum = 0;
-        for (int u = 0; u < wpb; ++u) sum += Yp[((blk * wpb + u) * 16 + (y & 15)) * kTileJ + j];
-        Y[idx] = sum;
+    for (int idx = threadIdx.x; idx < 16 * nyb * kTileJ; idx += kTileBlock)
+        Y[idx] = nyb == 1 ? Yp[idx] + Yp[kYRows * kTileJ + idx] : Yp[idx];
+    __syncthreads();
+    // face vertices: Y_j = sum_k w_k Y_j[source k] for the residual's j < 4
+    // (_set_vertex_energydensity, StaticBTESynthetic.cpp:1541-1616; its NaN -> 0
+    // rule applies to the combined value)
+    for (int idx = threadIdx.x; idx < 4 * nv; idx += kTileBlock) {
+        int i = idx >> 2, j = idx & 3;
+        int a = T.vs_ptr[v0 + i], z = T.vs_ptr[v0 + i + 1];
+        double sm = 0;
+        for (int k = a; k < z; ++k) sm += T.vs_w[k] * Y[T.vs_row[k] * kTileJ + j];
+        if (isnan(sm)) sm = 0;
+        Y[(kYRows + i) * kTileJ + j] = sm;
     }
     __syncthreads();
     if ((int)threadIdx.x < ncell) {
@@ -2280,50 +2275,47 @@ bool Solver::build() {
             n_tiles = T.n_tiles;
             tile_rows = T.max_rows;
             // LDS: row slices, then the vertex stencils (ptr | rows | weights)
-            int max_meta = 0;
-            for (int t = 0; t < n_tiles; ++t) {
-                int nv = T.vptr[t + 1] - T.vptr[t], nvs = T.vs_ptr[T.vptr[t + 1]] - T.vs_ptr[T.vptr[t]];
-                max_meta = std::max(max_meta, (nv + 1 + nvs + 1) / 2 + nvs);
-            }
-            tile_lds = (size_t)(kTileRows * kTileStride + max_meta) * sizeof(double);
-            if (tile_lds > 80 * 1024) use_tiles = false;
+            tile_lds = (size_t)kTileRows * kTileStride * sizeof(double);
         }
         if (use_tiles) {
-            // K_j(t) of the moment product, as A fragments per slice and k-step:
-            // lane l holds K_{l % 16}(64 slice + 4 k + l / 16), zero for j >= 8
-            // and past the row
+            // K_j(t) of the moment product, as A fragments of
+            // v_mfma_f64_4x4x4f64 per slice, j group g and k-step: lane l
+            // holds K_{4 g + l % 4}(64 slice + 4 k + l / 16) (the same for the
+            // four blocks l / 4 % 4), zero past the row
             int DB = nD * nB, nslice = (DB + kTileW - 1) / kTileW;
-            std::vector<double> kwf((size_t)nslice * kTileKs * 64, 0.0);
+            std::vector<double> kwf((size_t)nslice * 2 * kTileKs * 64, 0.0);
             double kap = P.kappa_bulk[0];
             for (int sl = 0; sl < nslice; ++sl)
+              for (int gq = 0; gq < 2; ++gq)
                 for (int k = 0; k < kTileKs; ++k)
                     for (int l = 0; l < 64; ++l) {
-                        int j = l % 16, t = sl * kTileW + 4 * k + l / 16;
-                        if (j >= kTileJ || t >= DB) continue;
+                        int j = 4 * gq + l % 4, t = sl * kTileW + 4 * k + l / 16;
+                        if (t >= DB) continue;
                         int d = t / nB, b = t - d * nB;
                         double wd = w[d], vb = vg[b], h = hr[b] > 0 ? 1.0 : 0.0, tv2 = tau[b] * vb * vb;
                         double K[kTileJ] = {h * wd * kap * lrc[b],        h * wd * tv2 * sx[d] * sx[d],
                                             h * wd * tv2 * sx[d] * sy[d], h * wd * tv2 * sy[d] * sy[d],
                                             wd * lrc[b],                  wd * vb * sx[d],
                                             wd * vb * sy[d],              wd * vb * sz[d]};
-                        kwf[((size_t)sl * kTileKs + k) * 64 + l] = K[j];
+                        kwf[(((size_t)sl * 2 + gq) * kTileKs + k) * 64 + l] = K[j];
                     }
             if (!upload(d_t_cell0, T.cell0) || !upload(d_t_rptr, T.rptr) || !upload(d_t_rsrc, T.rsrc) ||
                 !upload(d_t_nh, T.nh) || !upload(d_t_vptr, T.vptr) || !upload(d_t_vs_ptr, T.vs_ptr) ||
-                !upload(d_t_vs_row, T.vs_row) || !upload(d_t_vs_w, T.vs_w) || !upload(d_t_ny, T.ny) ||
-                !upload(d_t_yrow, T.yrow) || !upload(d_t_fin_i, T.fin_i) || !upload(d_t_cmf, T.cmf) ||
+                !upload(d_t_vs_row, T.vs_row) || !upload(d_t_vs_w, T.vs_w) ||
+                !upload(d_t_fin_i, T.fin_i) || !upload(d_t_cmf, T.cmf) ||
                 !upload(d_t_kwf, kwf) || !upload(d_t_fin_d, T.fin_d))
                 return false;
             if (opt.verbose) {
-                int ymax = 0;
-                long ysum = 0, hsum = 0;
+                int rmax = 0;
+                long rsum = 0, hsum = 0;
                 for (int t = 0; t < n_tiles; ++t) {
-                    ymax = std::max(ymax, T.ny[t]);
-                    ysum += T.ny[t];
+                    int nr = T.rptr[t + 1] - T.rptr[t];
+                    rmax = std::max(rmax, nr);
+                    rsum += nr;
                     hsum += T.nh[t];
                 }
-                printf("[gbte] %d tiles: mean %.1f cell rows, mean %.1f / max %d moment rows\n", n_tiles,
-                       (double)hsum / n_tiles, (double)ysum / n_tiles, ymax);
+                printf("[gbte] %d tiles: mean %.1f cell rows, mean %.1f / max %d staged rows\n", n_tiles,
+                       (double)hsum / n_tiles, (double)rsum / n_tiles, rmax);
             }
         }
         std::vector<int>().swap(h_gptr);
@@ -2645,12 +2637,11 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
         (void)hipEventRecord(ev_[3], s_);
         if (use_tiles) {
-            TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p, d_t_vs_row.p,
-                       d_t_ny.p,    d_t_yrow.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p,   d_t_kwf.p,     d_t_fin_d.p};
+            TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p,
+                       d_t_vs_row.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p, d_t_kwf.p, d_t_fin_d.p};
             GBTE_LAUNCH_ST(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, rp<ST>(d_e), rp<ST>(d_eb),
                            rp<ST>(d_grad), td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p,
-                               make_double4(m.offT, m.offQx, m.offQy, m.offQz), nD * nB, n_tiles,
-                               kTileRows * kTileStride);
+                               make_double4(m.offT, m.offQx, m.offQy, m.offQz), nD * nB, n_tiles);
         } else {
             hipLaunchKernelGGL(one_row ? k_vertex_e<true> : k_vertex_e<false>, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p,
                                d_eV.p, d_vptr.p, d_vsrc.p, d_vwt.p, nn, R);

@@ -148,7 +148,7 @@ class Solver {
     bool use_tiles = false;
     int n_tiles = 0, tile_rows = 0;
     size_t tile_lds = 0;
-    DBuf<int> d_t_cell0, d_t_rptr, d_t_rsrc, d_t_nh, d_t_vptr, d_t_vs_ptr, d_t_vs_row, d_t_ny, d_t_yrow, d_t_fin_i;
+    DBuf<int> d_t_cell0, d_t_rptr, d_t_rsrc, d_t_nh, d_t_vptr, d_t_vs_ptr, d_t_vs_row, d_t_fin_i;
     DBuf<double> d_t_vs_w, d_t_cmf, d_t_kwf, d_t_fin_d;
     std::vector<int> h_gptr, h_gidx, h_vptr, h_vsrc, h_fkind, h_fother, h_fv0, h_fv1, h_gok;
     std::vector<double> h_vwt, h_gP, h_fga, h_fn0x, h_fn0y, h_fmxm, h_fmym;

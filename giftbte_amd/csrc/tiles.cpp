@@ -170,8 +170,6 @@ std::string pack_tile_mfma(const Problem &p, const std::vector<int> &gptr, const
     int nc = p.n_cells;
     size_t nnz = gptr[nc];
     T.cmf.assign((size_t)T.n_tiles * kTileKs * 64, 0.0);
-    T.ny.assign(T.n_tiles, 0);
-    T.yrow.assign((size_t)T.n_tiles * kYRows, 0);
     T.fin_i.assign((size_t)nc * kFinI, 0);
     T.fin_d.assign((size_t)nc * kFinD, 0.0);
     const std::vector<double> &ga = *face8[0], &n0x = *face8[1], &n0y = *face8[2], &mxm = *face8[3],
@@ -202,24 +200,16 @@ std::string pack_tile_mfma(const Problem &p, const std::vector<int> &gptr, const
                 int r = 4 * k + l / 16;
                 F[k * 64 + l] = r < nh ? C[(size_t)(l % 16) * kTileRowsMax + r] : 0.0;
             }
-        // moment rows: own cells first, then face neighbours / entries and face vertices
-        std::vector<int> ylist;
-        std::unordered_map<int, int> ypos;
-        auto addY = [&](int r) {
-            auto it = ypos.find(r);
-            if (it != ypos.end()) return it->second;
-            ypos[r] = (int)ylist.size();
-            ylist.push_back(r);
-            return (int)ylist.size() - 1;
-        };
-        for (int i = 0; i < ncell; ++i) addY(T.crow[c0 + i]);
+        // moment rows are the staged rows; a face vertex's moments are the
+        // inverse-distance combination of its sources' (kYRows + its index)
+        int nr = T.rptr[t + 1] - T.rptr[t];
         for (int i = 0; i < ncell; ++i) {
             int c = c0 + i;
             int nf = p.face_size[c];
             if (nf > 4) return "more than 4 faces per cell in a tile";
             int *fi = &T.fin_i[(size_t)c * kFinI];
             double *fd = &T.fin_d[(size_t)c * kFinD];
-            int yo = ypos.at(T.crow[c]);
+            int yo = T.crow[c];
             fi[0] = yo;
             fi[1] = nf;
             for (int j = 0; j < nf; ++j) {
@@ -227,9 +217,12 @@ std::string pack_tile_mfma(const Problem &p, const std::vector<int> &gptr, const
                 int kd = fkind[s];
                 fi[2 + 3 * j] = fi[3 + 3 * j] = fi[4 + 3 * j] = yo;
                 if (kd == 0) continue;
-                fi[2 + 3 * j] = addY(T.f_orow[s]);
-                fi[3 + 3 * j] = addY(T.f_v0row[s]);
-                fi[4 + 3 * j] = addY(T.f_v1row[s]);
+                int v0 = T.f_v0row[s] - nr, v1 = T.f_v1row[s] - nr;
+                if (T.f_orow[s] < 0 || T.f_orow[s] >= nr || v0 < 0 || v1 < 0 || v0 >= kYRows || v1 >= kYRows)
+                    return "tile moment row mismatch";
+                fi[2 + 3 * j] = T.f_orow[s];
+                fi[3 + 3 * j] = kYRows + v0;
+                fi[4 + 3 * j] = kYRows + v1;
                 // face term area (kl n.g - tau (v.g)(v.n)), g = de ga n0 + dv m,
                 // de = sg (other - own), dv = v1 - v0, v = vg s; its
                 // direction part expands over sx^2, sx sy, sy^2
@@ -245,9 +238,7 @@ std::string pack_tile_mfma(const Problem &p, const std::vector<int> &gptr, const
                 fd[8 * j + 7] = -A * by * ny;
             }
         }
-        if ((int)ylist.size() > kYRows) return "tile has more moment rows than the matrix-core shape";
-        T.ny[t] = (int)ylist.size();
-        for (size_t y = 0; y < ylist.size(); ++y) T.yrow[(size_t)t * kYRows + y] = ylist[y];
+        if (nr > kYRows || T.vptr[t + 1] - T.vptr[t] > kYRows) return "tile has more moment rows than the matrix-core shape";
     }
     return "";
 }

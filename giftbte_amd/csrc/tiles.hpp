@@ -21,7 +21,6 @@ struct Tiles {
     std::vector<int> f_orow, f_v0row, f_v1row;  // per (cell, face) slot: tile-local rows
     // matrix-core tables (pack_tile_mfma)
     std::vector<double> cmf;           // per tile: LSQ coefficients as A fragments [kTileKs][64]
-    std::vector<int> ny, yrow;         // per tile: moment rows, their tile-local rows [kYRows]
     std::vector<int> fin_i;            // per cell [kFinI]: own Y index, face count, per face Y(other, v0, v1)
     std::vector<double> fin_d;         // per cell [kFinD]: per face the factors of Y_0..3 (face value, vertex)
 };
@@ -32,12 +31,13 @@ struct Tiles {
 // the A-operand order of v_mfma_f64_16x16x4f64: k-step k, lane l holds
 // C[l % 16][4 k + l / 16].
 //
-// The moments are Y[j][y] = sum over the slice entries t of K_j(t) E[yrow[y]][t]
-// for the rows y the moments read (own cells, face neighbours and boundary
-// entries, face vertices): T, q_x, q_y, q_z are Y_4..7 of the cell's own row
-// and the non-Fourier residual is sum_faces sum_{j<4} cde_j (Y_j[other] -
-// Y_j[own]) + cdv_j (Y_j[v1] - Y_j[v0]), with K_0..3 the per-(direction,
-// band) factors kl, tau vg^2 (sx^2, sx sy, sy^2) of its face terms.
+// The moments are Y[j][r] = sum over the row entries t of K_j(t) E[r][t] for
+// the staged rows r, and for a face vertex the inverse-distance combination
+// of its sources' Y (Y is linear in the rows; Y index kYRows + the vertex's
+// tile index).  T, q_x, q_y, q_z are Y_4..7 of the cell's own row and the
+// non-Fourier residual is sum_faces sum_{j<4} cde_j (Y_j[other] - Y_j[own]) +
+// cdv_j (Y_j[v1] - Y_j[v0]), with K_0..3 the per-(direction, band) factors
+// kl, tau vg^2 (sx^2, sx sy, sy^2) of its face terms.
 constexpr int kTileCellsMax = 8, kTileRowsMax = 64, kTileKs = kTileRowsMax / 4, kYRows = 64;
 constexpr int kFinI = 16, kFinD = 32;
 
