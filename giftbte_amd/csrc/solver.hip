@@ -924,7 +924,12 @@ constexpr int kTileRows = kTileRowsMax;    // rows per tile: 33 KiB of LDS
 constexpr int kStageBatch = 16;            // staged rows in flight per lane
 constexpr int kTileJ = 8;                  // moment factors K_0..K_7, two groups of four
 static_assert(kTileRows == 4 * 16, "four 16-row blocks of moments");
-static_assert(2 * kYRows * kTileJ * 2 <= kTileRows * kTileStride, "moment partials fit in the row buffer");
+static_assert(2 * kYRows * kTileJ * 2 * sizeof(double) <= kTileRows * kTileStride * sizeof(float),
+              "moment partials fit in the row buffer");
+// workgroups per CU: FP32 storage stages float rows (16 KiB of LDS) and
+// holds the LSQ coefficients in float, which leaves room for a fifth
+template <typename ST>
+constexpr int tile_wg_per_cu() { return sizeof(ST) == 4 ? 5 : 4; }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -936,12 +941,14 @@ struct TileDev {
 __device__ __forceinline__ int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
 template <typename ST>
-__global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restrict__ e, const ST *__restrict__ eb,
+__global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(const ST *__restrict__ e, const ST *__restrict__ eb,
                                                           ST *__restrict__ grad, TileDev T,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
                                                           double *__restrict__ RMo, double4 off, int DB, int ntiles) {
     extern __shared__ double lds[];
+    ST *rows = reinterpret_cast<ST *>(lds);  // staged rows in the storage type: no rounding
+    constexpr bool kKEarly = sizeof(ST) == 8;
     int tile = xcd_block();
     if (tile >= ntiles) return;  // whole workgroup
     int lane = threadIdx.x & 63, l16 = lane & 15, lq = lane >> 4;
@@ -961,15 +968,16 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restric
     // rows [nr, kTileRows) are read by the products with zero weight (LSQ) or
     // into unused columns (moments) and are written by nothing else: clear
     // them once
-    for (int r = nr + w; r < kTileRows; r += kTileWaves) lds[r * kTileStride + lane] = 0.0;
-    // LSQ coefficients (A operands), the same for every slice
-    double cm[kTileKs];
+    for (int r = nr + w; r < kTileRows; r += kTileWaves) rows[r * kTileStride + lane] = 0;
+    // LSQ coefficients (A operands), the same for every slice; in the storage
+    // type, as the gradients they make are stored in it
+    ST cm[kTileKs];
     const double *cmt = T.cmf + (size_t)tile * kTileKs * 64 + lane;
 #pragma unroll
-    for (int k = 0; k < kTileKs; ++k) cm[k] = k < nks ? cmt[k * 64] : 0.0;
+    for (int k = 0; k < kTileKs; ++k) cm[k] = k < nks ? (ST)cmt[k * 64] : (ST)0;
     // B operands: Y reads 16 rows of one column, G 4 rows of 16 columns
-    const double *yrd = lds + (16 * yb + l16) * kTileStride + 4 * ks0 + lq;
-    const double *grd = lds + lq * kTileStride + 16 * w + l16;
+    const ST *yrd = rows + (16 * yb + l16) * kTileStride + 4 * ks0 + lq;
+    const ST *grd = rows + lq * kTileStride + 16 * w + l16;
     double acc0 = 0.0, acc1 = 0.0;
     int src0 = lane < nr ? T.rsrc[r0 + lane] : 0;
     int nslice = (DB + kTileW - 1) / kTileW;
@@ -978,12 +986,15 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restric
         int tt = t < DB ? t : 0;
         // this wave's K fragments of the slice
         double kw[kTileKs];
-        const double *kws = T.kwf + (((size_t)slice * 2 + yg) * kTileKs + ks0) * 64 + lane;
+        auto load_k = [&]() {
+            const double *kws = T.kwf + (((size_t)slice * 2 + yg) * kTileKs + ks0) * 64 + lane;
 #pragma unroll
-        for (int s = 0; s < kTileKs; ++s) kw[s] = s < nys ? kws[s * 64] : 0.0;
+            for (int s = 0; s < kTileKs; ++s) kw[s] = s < nys ? kws[s * 64] : 0.0;
+        };
+        if constexpr (kKEarly) load_k();  // in flight over the staging
         __syncthreads();  // the previous slice is done with the LDS rows
         for (int r = w; r < nr; r += kStageBatch * kTileWaves) {
-            ST x[kStageBatch];  // converted when written to LDS, so the loads stay batched
+            ST x[kStageBatch];
 #pragma unroll
             for (int j = 0; j < kStageBatch; ++j) {
                 int rr = r + j * kTileWaves;
@@ -997,22 +1008,26 @@ __global__ void __launch_bounds__(kTileBlock, 4) k_tile_face(const ST *__restric
 #pragma unroll
             for (int j = 0; j < kStageBatch; ++j) {
                 int rr = r + j * kTileWaves;
-                if (rr < nr) lds[rr * kTileStride + lane] = (double)x[j];
+                if (rr < nr) rows[rr * kTileStride + lane] = x[j];
             }
         }
+        // float staging needs the registers: its K loads wait for the rows
+        if constexpr (!kKEarly) load_k();
         __syncthreads();
         // moments: k-steps [ks0, ks0 + nys) of group yg, blocks yb and yb + 2
 #pragma unroll
         for (int s = 0; s < kTileKs; ++s)
             if (s < nys) {
-                if (y0) acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], yrd[4 * s], acc0, 0, 0, 0);
-                if (y1) acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], yrd[32 * kTileStride + 4 * s], acc1, 0, 0, 0);
+                if (y0) acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], (double)yrd[4 * s], acc0, 0, 0, 0);
+                if (y1)
+                    acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], (double)yrd[32 * kTileStride + 4 * s], acc1, 0, 0, 0);
             }
         // LSQ gradient of entries [16 w, 16 w + 16) of the slice
         v4d g = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < kTileKs; ++k)
-            if (k < nks) g = __builtin_amdgcn_mfma_f64_16x16x4f64(cm[k], grd[4 * k * kTileStride], g, 0, 0, 0);
+            if (k < nks)
+                g = __builtin_amdgcn_mfma_f64_16x16x4f64((double)cm[k], (double)grd[4 * k * kTileStride], g, 0, 0, 0);
         // lane holds rows lq + 4 i: (cell lq, x), (cell lq + 4, x), (cell lq, y), (cell lq + 4, y)
         int tg = slice * kTileW + 16 * w + l16;
         if (tg < DB) {
@@ -2275,7 +2290,7 @@ bool Solver::build() {
             n_tiles = T.n_tiles;
             tile_rows = T.max_rows;
             // LDS: row slices, then the vertex stencils (ptr | rows | weights)
-            tile_lds = (size_t)kTileRows * kTileStride * sizeof(double);
+            tile_lds = (size_t)kTileRows * kTileStride * (fp32_ ? sizeof(float) : sizeof(double));
         }
         if (use_tiles) {
             // K_j(t) of the moment product, as A fragments of
