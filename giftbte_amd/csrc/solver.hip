@@ -945,7 +945,8 @@ __global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(
                                                           ST *__restrict__ grad, TileDev T,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
-                                                          double *__restrict__ RMo, double4 off, int DB, int ntiles) {
+                                                          double *__restrict__ RMo, double4 off, int DB, int ntiles,
+                                                          int moments) {
     extern __shared__ double lds[];
     ST *rows = reinterpret_cast<ST *>(lds);  // staged rows in the storage type: no rounding
     constexpr bool kKEarly = sizeof(ST) == 8;
@@ -1017,7 +1018,7 @@ __global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(
         // moments: k-steps [ks0, ks0 + nys) of group yg, blocks yb and yb + 2
 #pragma unroll
         for (int s = 0; s < kTileKs; ++s)
-            if (s < nys) {
+            if (moments && s < nys) {
                 if (y0) acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], (double)yrd[4 * s], acc0, 0, 0, 0);
                 if (y1)
                     acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], (double)yrd[32 * kTileStride + 4 * s], acc1, 0, 0, 0);
@@ -1045,6 +1046,7 @@ __global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(
             }
         }
     }
+    if (!moments) return;  // the set-up gradient (whole workgroup)
     // moment partials (lane holds Y[4 yg + lq][16 block + l16]) -> LDS
     // Yp[part][row][j], part 1 only for a single block split over k-steps
     __syncthreads();
@@ -1091,27 +1093,28 @@ __global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(
 
 // Output-only moments: energy-equivalent temperature (_get_total_energy,
 // StaticBTESynthetic.cpp:2498-2507) and band temperatures
-// (_recover_temperature, :2487-2496).
+// (_recover_temperature, :2487-2496).  One wave per cell row, lanes over the
+// bands: each lane sums its band over the directions in order (one coalesced
+// load of the row's band values per direction) and E is the sum of the band
+// sums.
+constexpr int kOutRows = 4;
+
 template <typename ST>
-__global__ void __launch_bounds__(kRowBlockMax) k_output_moments(const ST *__restrict__ e, Modes m,
+__global__ void __launch_bounds__(64 * kOutRows) k_output_moments(const ST *__restrict__ e, Modes m,
                                                                  double *__restrict__ E, double *__restrict__ Tb,
-                                                                 int nc, Rows R) {
-    __shared__ double sh[kRowBlockMax / 64];
-    int c = xcd_block() * R.rpb + threadIdx.x / R.tpr;
-    bool act = c < nc;
-    int cc = act ? c : nc - 1;
-    int DB = R.DB, nB = R.B, nD = DB / nB;
-    int t0 = threadIdx.x % R.tpr;
-    const ST *ec = e + (size_t)cc * DB;
+                                                                 int nc, int nB, int nD) {
+    int c = xcd_block() * kOutRows + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (c >= nc) return;  // whole wave
+    const ST *ec = e + (size_t)c * nD * nB;
     double aE = 0;
-    for (int t = t0; t < DB; t += R.tpr) aE += ec[t] * m.w[t / nB] / m.cbulk;
-    for (int b = t0; b < nB; b += R.tpr) {
+    for (int b = lane; b < nB; b += 64) {
         double s = 0;
-        for (int d = 0; d < nD; ++d) s += ec[d * nB + b] * m.w[d] / m.cap[b];
-        if (act) Tb[(size_t)b * nc + c] = s + m.offW;
+        for (int d = 0; d < nD; ++d) s += ec[d * nB + b] * m.w[d];
+        aE += s;
+        Tb[(size_t)b * nc + c] = s / m.cap[b] + m.offW;
     }
-    aE = row_sum(aE, R.tpr, sh);
-    if (act && t0 == 0) E[c] = aE + m.offE;
+    for (int o = 32; o > 0; o >>= 1) aE += __shfl_xor(aE, o);
+    if (lane == 0) E[c] = aE / m.cbulk + m.offE;
 }
 
 // _set_bound_ee_1 (StaticBTESynthetic.cpp:2083-2232) at np=1 and
@@ -2610,8 +2613,18 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     // (StaticBTESyntheticSolver.cpp:19-62)
     GBTE_LAUNCH_ST(k_init_e, dim3(blocks_for((long)nE, kBlock)), dim3(kBlock), 0, rp<ST>(d_e), d_T.p, d_cap.p,
                    nD * nB, nB, nE, m.eref);
-    GBTE_LAUNCH_ST(k_gradient, dim3(gw), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), d_gptr.p, d_gidx.p, d_gP.p,
-                   d_gok.p, dim, nnz, invLx, nc, R);
+    TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p,
+               d_t_vs_row.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p, d_t_kwf.p, d_t_fin_d.p};
+    if (use_tiles) {
+        // _get_gradient_larger of the initial e: the tile kernel's LSQ product
+        // alone (the tiles hold every cell's full stencil)
+        GBTE_LAUNCH_ST(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, rp<ST>(d_e), rp<ST>(d_eb),
+                       rp<ST>(d_grad), td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, make_double4(0, 0, 0, 0), nD * nB,
+                       n_tiles, 0);
+    } else {
+        GBTE_LAUNCH_ST(k_gradient, dim3(gw), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), d_gptr.p, d_gidx.p,
+                       d_gP.p, d_gok.p, dim, nnz, invLx, nc, R);
+    }
     GBTE_LAUNCH_ST(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), rp<ST>(d_eb), g, m,
                        d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
     GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m, d_ent_list.p,
@@ -2652,11 +2665,9 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
         (void)hipEventRecord(ev_[3], s_);
         if (use_tiles) {
-            TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p,
-                       d_t_vs_row.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p, d_t_kwf.p, d_t_fin_d.p};
             GBTE_LAUNCH_ST(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, rp<ST>(d_e), rp<ST>(d_eb),
                            rp<ST>(d_grad), td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p,
-                               make_double4(m.offT, m.offQx, m.offQy, m.offQz), nD * nB, n_tiles);
+                               make_double4(m.offT, m.offQx, m.offQy, m.offQz), nD * nB, n_tiles, 1);
         } else {
             hipLaunchKernelGGL(one_row ? k_vertex_e<true> : k_vertex_e<false>, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p,
                                d_eV.p, d_vptr.p, d_vsrc.p, d_vwt.p, nn, R);
@@ -2773,7 +2784,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         if (errT <= tolT && errQ <= tolQ) { conv = true; ++nt; break; }
         if (errInc_ >= 10) { status = GBTE_DIVERGED; err_ = "Error increases for 10 times"; ++nt; break; }
     }
-    GBTE_LAUNCH_ST(k_output_moments, dim3(gw), dim3(rowblk), 0, rp<ST>(d_e), m, d_E.p, d_Tb.p, nc, R);
+    GBTE_LAUNCH_ST(k_output_moments, dim3(xgrid(blocks_for(nc, kOutRows))), dim3(64 * kOutRows), 0, rp<ST>(d_e), m,
+                   d_E.p, d_Tb.p, nc, nB, nD);
     if (!check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
     auto t1 = std::chrono::high_resolution_clock::now();
     have_state = true;
