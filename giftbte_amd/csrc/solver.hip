@@ -920,16 +920,12 @@ constexpr int kTileStride = kTileW + 1;  // LDS row stride: the moment product r
 constexpr int kTileBlock = 256;
 constexpr int kTileWaves = kTileBlock / 64;
 constexpr int kTileCells = kTileCellsMax;  // cells per tile
-constexpr int kTileRows = kTileRowsMax;    // rows per tile: 33 KiB of LDS
+constexpr int kTileRows = 60;              // staged rows per tile: 30.5 KiB of LDS in FP64, five workgroups per CU
 constexpr int kStageBatch = 16;            // staged rows in flight per lane
 constexpr int kTileJ = 8;                  // moment factors K_0..K_7, two groups of four
-static_assert(kTileRows == 4 * 16, "four 16-row blocks of moments");
-static_assert(2 * kYRows * kTileJ * 2 * sizeof(double) <= kTileRows * kTileStride * sizeof(float),
-              "moment partials fit in the row buffer");
-// workgroups per CU: FP32 storage stages float rows (16 KiB of LDS) and
-// holds the LSQ coefficients in float, which leaves room for a fifth
-template <typename ST>
-constexpr int tile_wg_per_cu() { return sizeof(ST) == 4 ? 5 : 4; }
+static_assert(kTileRows <= kYRows && kTileRows <= kTileRowsMax && kTileRows > 48, "four 16-row blocks of moments");
+constexpr int kTileMomentLds = 2 * kYRows * kTileJ * 2;  // doubles of the moment partials and totals
+static_assert(kTileMomentLds <= kTileRows * kTileStride, "moment partials fit in the FP64 row buffer");
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -941,7 +937,7 @@ struct TileDev {
 __device__ __forceinline__ int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
 template <typename ST>
-__global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(const ST *__restrict__ e, const ST *__restrict__ eb,
+__global__ void __launch_bounds__(kTileBlock, 5) k_tile_face(const ST *__restrict__ e, const ST *__restrict__ eb,
                                                           ST *__restrict__ grad, TileDev T,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
@@ -949,7 +945,7 @@ __global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(
                                                           int moments) {
     extern __shared__ double lds[];
     ST *rows = reinterpret_cast<ST *>(lds);  // staged rows in the storage type: no rounding
-    constexpr bool kKEarly = sizeof(ST) == 8;
+    constexpr bool kKEarly = false;
     int tile = xcd_block();
     if (tile >= ntiles) return;  // whole workgroup
     int lane = threadIdx.x & 63, l16 = lane & 15, lq = lane >> 4;
@@ -977,8 +973,10 @@ __global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(
 #pragma unroll
     for (int k = 0; k < kTileKs; ++k) cm[k] = k < nks ? (ST)cmt[k * 64] : (ST)0;
     // B operands: Y reads 16 rows of one column, G 4 rows of 16 columns
-    const ST *yrd = rows + (16 * yb + l16) * kTileStride + 4 * ks0 + lq;
+    // (rows past kTileRows feed only columns of rows past nr: clamp them)
+    const ST *yrd = rows + min(16 * yb + l16, kTileRows - 1) * kTileStride + 4 * ks0 + lq;
     const ST *grd = rows + lq * kTileStride + 16 * w + l16;
+    const ST *yrd1 = rows + min(16 * yb + 32 + l16, kTileRows - 1) * kTileStride + 4 * ks0 + lq;
     double acc0 = 0.0, acc1 = 0.0;
     int src0 = lane < nr ? T.rsrc[r0 + lane] : 0;
     int nslice = (DB + kTileW - 1) / kTileW;
@@ -1021,7 +1019,7 @@ __global__ void __launch_bounds__(kTileBlock, tile_wg_per_cu<ST>()) k_tile_face(
             if (moments && s < nys) {
                 if (y0) acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], (double)yrd[4 * s], acc0, 0, 0, 0);
                 if (y1)
-                    acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], (double)yrd[32 * kTileStride + 4 * s], acc1, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(kw[s], (double)yrd1[4 * s], acc1, 0, 0, 0);
             }
         // LSQ gradient of entries [16 w, 16 w + 16) of the slice
         v4d g = {0.0, 0.0, 0.0, 0.0};
@@ -2293,7 +2291,8 @@ bool Solver::build() {
             n_tiles = T.n_tiles;
             tile_rows = T.max_rows;
             // LDS: row slices, then the vertex stencils (ptr | rows | weights)
-            tile_lds = (size_t)kTileRows * kTileStride * (fp32_ ? sizeof(float) : sizeof(double));
+            tile_lds = std::max((size_t)kTileRows * kTileStride * (fp32_ ? sizeof(float) : sizeof(double)),
+                                (size_t)kTileMomentLds * sizeof(double));
         }
         if (use_tiles) {
             // K_j(t) of the moment product, as A fragments of

@@ -9,11 +9,12 @@
 //      the entries that feed the inverse-distance values of the tile's
 //      boundary vertices,
 //   V  the two vertices of every face-gradient face of the tile (in the order
-//      of the face's first owner), whose values are computed in LDS from H
-//      and B rows.
-// The device stages [H | B] row slices in LDS, computes the V rows after
-// them, and reads every neighbour value from LDS.  Tiles grow greedily along
-// the Morton order until the row count would exceed max_rows.
+//      of the face's first owner), inverse-distance combinations of H and B
+//      rows.
+// The device stages [H | B] row slices in LDS and reads every neighbour value
+// from there; a vertex enters only through its moments, which are the same
+// combination of its sources' moments.  Tiles grow greedily along the Morton
+// order until the staged rows would exceed max_rows or the vertices kYRows.
 #include "tiles.hpp"
 
 #include <algorithm>
@@ -66,7 +67,8 @@ std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const st
             }
         }
     };
-    auto rows = [](const Sets &S) { return (int)(S.H.size() + S.B.size() + S.V.size()); };
+    auto rows = [](const Sets &S) { return (int)(S.H.size() + S.B.size()); };
+    auto fits = [&](const Sets &S) { return rows(S) <= max_rows && (int)S.V.size() <= kYRows; };
 
     auto close_tile = [&](int c_end) {
         const Sets &S = cur;
@@ -115,14 +117,14 @@ std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const st
     for (int c = 0; c < nc; ++c) {
         trial = cur;
         add_cell(trial, c);
-        if (count > 0 && (rows(trial) > max_rows || count >= max_cells)) {
+        if (count > 0 && (!fits(trial) || count >= max_cells)) {
             close_tile(c);
             cur.clear();
             count = 0;
             trial = cur;
             add_cell(trial, c);
         }
-        if (rows(trial) > max_rows) return "a cell's stencil exceeds the LDS tile";
+        if (!fits(trial)) return "a cell's stencil exceeds the LDS tile";
         cur = std::move(trial);
         ++count;
     }
@@ -132,7 +134,7 @@ std::string build_tiles(const Problem &p, const std::vector<int> &gptr, const st
     // name the intended cell, entry or vertex
     for (int t = 0; t < T.n_tiles; ++t) {
         int r0 = T.rptr[t], nr = T.rptr[t + 1] - r0, v0 = T.vptr[t], nv = T.vptr[t + 1] - v0;
-        if (nr + nv > max_rows) return "tile exceeds its row budget";
+        if (nr > max_rows || nv > kYRows) return "tile exceeds its row budget";
         auto staged = [&](int r, int want) { return r >= 0 && r < nr && T.rsrc[r0 + r] == want; };
         auto vrow = [&](int r, int node) { return r >= nr && r < nr + nv && T.vnode[v0 + r - nr] == node; };
         for (int i = 0; i < nv; ++i) {
