@@ -73,10 +73,10 @@ def kernel_bytes_per_update(dim, word=8):
     return {"k_source": (dim + 1) * word, "k_sweep_pc": 2 * word, "k_tile_face": (1 + dim) * word}
 
 
-def load_traffic(kernel, cfg):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel calibration),
-    if it was measured on this workload."""
+def load_traffic(kernel, cfg, prec):
+    """HBM bytes per launch of `kernel` at storage precision `prec` from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE scaled by a
+    copy-kernel calibration), if it was measured on this workload."""
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
@@ -84,7 +84,7 @@ def load_traffic(kernel, cfg):
         return None
     if tr.get("workload") != {k: cfg[k] for k in ("cells", "bands", "directions")}:
         return None
-    k = tr.get("kernels", {}).get(kernel)
+    k = tr.get("kernels", {}).get(prec, {}).get(kernel)
     return None if k is None else k["hbm_bytes_per_launch"]
 
 
@@ -163,7 +163,7 @@ def main():
                                            "sweep_levels": st.sweep_levels},
                  "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"],
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac_hbm_peak"],
-                              "traffic": load_traffic(dom, cfg) if prec == "fp64" else None,
+                              "traffic": load_traffic(dom, cfg, prec),
                               "bytes_per_update": bpu[dom], "updates_per_launch": U},
                  "kernels": kernels}
             if not a.no_converge:
