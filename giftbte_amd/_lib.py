@@ -12,6 +12,9 @@ LIB_PATH = os.path.join(_HERE, "libgiftbte_hip.so")
 GBTE_OK, GBTE_BAD_INPUT, GBTE_DIVERGED, GBTE_NAN, GBTE_HIP_ERROR, GBTE_UNSUPPORTED = range(6)
 GBTE_FP64, GBTE_FP32_SWEEP = 0, 1
 GBTE_LINSOLVE_DIRECT, GBTE_LINSOLVE_PCG = 0, 1
+GBTE_ORDER_1, GBTE_ORDER_2, GBTE_ORDER_2LIMIT = 1, 2, 3
+GBTE_SCHEME_SYNTHETIC, GBTE_SCHEME_SEQUENTIAL = 0, 1
+GBTE_MATRIX_LU, GBTE_MATRIX_BICGSTAB = 0, 1
 
 _pi, _pd = C.POINTER(C.c_int), C.POINTER(C.c_double)
 
@@ -34,6 +37,7 @@ class Tables(C.Structure):
         ("vg", _pd), ("tau", _pd), ("cap", _pd), ("lr", _pd), ("hr", _pd),
         ("dir_x", _pd), ("dir_y", _pd), ("dir_z", _pd), ("weight", _pd), ("initial_temp", _pd),
         ("max_iter", C.c_int), ("tol_temp", C.c_double), ("tol_flux", C.c_double), ("tref", C.c_double),
+        ("order", C.c_int), ("scheme", C.c_int), ("matrix_solver", C.c_int),
     ]
 
     def shape_of(self, name: str):

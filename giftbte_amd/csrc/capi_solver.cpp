@@ -32,6 +32,13 @@ int gbte_create(const gbte_problem *p, const gbte_options *opts, gbte_handle **o
         *out = h;
         return GBTE_BAD_INPUT;
     }
+    if (p->p.deck.order != "2") {
+        auto *h = new gbte_handle();
+        h->err = "Order " + p->p.deck.order +
+                 " (solve_firstorder / the 2limit limiter) is not supported: this build runs Order 2";
+        *out = h;
+        return GBTE_UNSUPPORTED;
+    }
     if (o.precision == GBTE_FP32_SWEEP && o.fallback_kernels) {
         auto *h = new gbte_handle();
         h->err = "FP32 storage needs the tiled face kernel and the wave-specialised sweep, which fallback_kernels "

@@ -651,6 +651,11 @@ std::string load_problem(const std::string &dir, Problem &P) {
     std::string err = parse_deck(dir, d);
     if (!err.empty()) return err;
     if (d.state != "Steady") return "State=" + d.state + " is not supported: this build is the steady solver";
+    // the dispatch of SolutionAll::_BTE_Solver (Solution.cpp:287-406), which
+    // exits on any other value
+    if (d.order != "1" && d.order != "2" && d.order != "2limit") return "no this Order: " + d.order;
+    if (d.method != "Synthetic" && d.method != "Sequential") return "no this IterativeScheme: " + d.method;
+    if (d.matrix != "LU" && d.matrix != "BICGSTAB") return "no this MatrixSolver: " + d.matrix;
     if (d.dim != 2) return "GeometryDimension " + std::to_string(d.dim) + " decks are not supported by this deck reader (2D only)";
     if (!d.multiscale.empty()) return "Multiscale Fourier/BTE region files are not supported";
     Bands B;
@@ -843,6 +848,9 @@ gbte_tables Problem::tables() const {
     t.dir_x = dir_x.data(); t.dir_y = dir_y.data(); t.dir_z = dir_z.data(); t.weight = weight.data();
     t.initial_temp = initial_temp.data();
     t.max_iter = deck.max_iter; t.tol_temp = deck.tol_temp; t.tol_flux = deck.tol_flux; t.tref = deck.tref;
+    t.order = deck.order == "1" ? GBTE_ORDER_1 : deck.order == "2limit" ? GBTE_ORDER_2LIMIT : GBTE_ORDER_2;
+    t.scheme = deck.method == "Sequential" ? GBTE_SCHEME_SEQUENTIAL : GBTE_SCHEME_SYNTHETIC;
+    t.matrix_solver = deck.matrix == "BICGSTAB" ? GBTE_MATRIX_BICGSTAB : GBTE_MATRIX_LU;
     return t;
 }
 

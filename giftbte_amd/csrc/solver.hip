@@ -2581,6 +2581,7 @@ static ST *rp(DBuf<double> &b) {
 int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
+    bool sequential = P.deck.method == "Sequential";
     Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
             P.capacity_bulk[0], eref_, offT_, offE_, offW_, offQ_[0], offQ_[1], offQ_[2], offMac_};
     SlotGeo g{d_fsize.p, d_snb.p, d_sent.p, d_snx.p, d_sny.p, d_snz.p, d_sarea.p, d_vol.p,
@@ -2693,7 +2694,11 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             if (inc >= 10) { err_ = "cannot congerve, try to set Order=1 or 2limit"; status = GBTE_DIVERGED; return false; }
             return inner > 1e-6;
         };
-        if (opt.linear_solver == GBTE_LINSOLVE_DIRECT) {
+        if (sequential) {
+            // solve_DOM (StaticBTESyntheticSolver.cpp:968-1178): no synthetic
+            // correction, T stays the recovered lattice temperature
+            (void)hipEventRecord(ev_[6], s_);
+        } else if (opt.linear_solver == GBTE_LINSOLVE_DIRECT) {
             // two passes and the outer error of their result are enqueued
             // behind one sync (most iterations take exactly two passes); a
             // loop that stops after the first pass undoes the second (T = T1)

@@ -1,7 +1,7 @@
 /*
  * giftbte.h - C ABI of libgiftbte_hip.so, the MI355X solver for GiftBTE's
- * steady StaticBTESynthetic path (Order 2, IterativeScheme Synthetic,
- * MatrixSolver LU) and its StaticFourier warm start.
+ * steady StaticBTESynthetic path (Order 2, IterativeScheme Synthetic or
+ * Sequential, MatrixSolver LU or BICGSTAB) and its StaticFourier warm start.
  *
  * The reference exposes this path as C++ classes with no FFI:
  *   StaticFourier(BTEMesh*, BTEBoundaryCondition*, BTEBand*, int, int) + _solve()
@@ -34,6 +34,19 @@ enum {
     GBTE_HIP_ERROR = 4,   /* HIP runtime failure, or no GPU                     */
     GBTE_UNSUPPORTED = 5  /* a reference feature outside this build             */
 };
+
+/* The deck's Order, IterativeScheme and MatrixSolver (CONTROL), dispatched
+ * by SolutionAll::_BTE_Solver (src/Solution/Solution.cpp:287-406) to
+ * solve / solve_Iterative (StaticBTESyntheticSolver.cpp:12-486, Order 2 and
+ * 2limit, Synthetic), solve_DOM / solve_DOM_Iterative (:968-1392, Sequential)
+ * and the first-order variants (:487-966, :1394-1802).  gbte_solve runs
+ * Order 2 with either scheme; Order 1 and 2limit give GBTE_UNSUPPORTED.
+ * BICGSTAB differs from LU only in the per-system linear solve (Eigen BiCGSTAB
+ * with an incomplete-LU preconditioner at its default tolerance, machine
+ * epsilon), so both run the exact solves here. */
+enum { GBTE_ORDER_1 = 1, GBTE_ORDER_2 = 2, GBTE_ORDER_2LIMIT = 3 };
+enum { GBTE_SCHEME_SYNTHETIC = 0, GBTE_SCHEME_SEQUENTIAL = 1 };
+enum { GBTE_MATRIX_LU = 0, GBTE_MATRIX_BICGSTAB = 1 };
 
 /* Storage of the per-(band, direction) fields: energy densities, boundary
  * values, LSQ gradients and sweep sources.  GBTE_FP32_SWEEP stores them in
@@ -97,6 +110,7 @@ typedef struct gbte_tables {
     /* CONTROL values from the deck */
     int max_iter;
     double tol_temp, tol_flux, tref;
+    int order, scheme, matrix_solver;      /* GBTE_ORDER_*, GBTE_SCHEME_*, GBTE_MATRIX_* */
 } gbte_tables;
 
 typedef struct gbte_problem gbte_problem;
@@ -153,7 +167,9 @@ int gbte_fourier_solve(gbte_handle *h, gbte_stats *stats);
  * NULL to keep the current field. */
 int gbte_set_initial(gbte_handle *h, const double *T, const double *qx, const double *qy, const double *qz);
 /* StaticBTESynthetic::solve(0, max_iter, 0, tolT, tolQ)
- * (StaticBTESyntheticSolver.cpp:12-247). */
+ * (StaticBTESyntheticSolver.cpp:12-247), or solve_DOM for the Sequential
+ * scheme (:968-1178: the same transport iteration without the synthetic
+ * correction, T the recovered lattice temperature). */
 int gbte_solve(gbte_handle *h, int max_iter, double tol_temp, double tol_flux, gbte_stats *stats);
 /* Output fields (caller-owned, n_cells doubles each; T_band n_bands*n_cells;
  * NULL skips a field): macro (lattice) temperature, energy-equivalent

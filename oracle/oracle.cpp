@@ -829,6 +829,11 @@ struct Oracle {
     int solve(const double *T0, const double *q0x, const double *q0y, const double *q0z, int maxIter, double tolT,
               double tolQ, int *iters, int *status) {
         if (dim != 2) { g_err = "the oracle covers 2D meshes"; return -1; }
+        if (t.order != GBTE_ORDER_2) { g_err = "the oracle covers Order 2"; return -1; }
+        // Sequential: StaticBTESynthetic::solve_DOM (StaticBTESyntheticSolver.cpp:968-1178),
+        // the same per-system steps without the vertex / face-gradient /
+        // non-Fourier terms and without the synthetic correction
+        bool seq = t.scheme == GBTE_SCHEME_SEQUENTIAL;
         for (int ib = 0; ib < nb; ++ib)
             if (t.bound_type[ib] == 4 || t.bound_type[ib] == 5) { g_err = "interface boundaries are not in the oracle"; return -1; }
         size_t ne = (size_t)nB * nD * nc;
@@ -874,16 +879,20 @@ struct Oracle {
                     gradient(b, d);
                     if (!sweep(b, d, order, indeg, Re, diag)) return -1;
                     bound_ee(b, d);
-                    vertex_e(b, d);
-                    if (!face_gradient(b, d)) return -1;
-                    non_fourier(b, d);
+                    if (!seq) {
+                        vertex_e(b, d);
+                        if (!face_gradient(b, d)) return -1;
+                        non_fourier(b, d);
+                    }
                     moments(b, d);
                 }
             if (!set_bound_ee_1()) return -1;
-            macro_bc();
-            int p = macro_solution();
-            if (p < 0) { *status = GBTE_DIVERGED; ++nt; break; }
-            macroPasses += p;
+            if (!seq) {
+                macro_bc();
+                int p = macro_solution();
+                if (p < 0) { *status = GBTE_DIVERGED; ++nt; break; }
+                macroPasses += p;
+            }
             bool conv = check_error(tolT, tolQ);
             if (std::isnan(errT) || std::isnan(errQ)) { *status = GBTE_NAN; ++nt; break; }
             if (conv) { ++nt; break; }

@@ -1,6 +1,7 @@
 """Named test decks (seeded) shared by the CPU and GPU tests."""
 from __future__ import annotations
 
+import dataclasses
 import os
 
 from giftbte_amd import decks
@@ -25,6 +26,13 @@ DECKS = {
         mesh=decks.square_split_mesh(10), bands=decks.debye_bands(20, 7),
         bcs=[(1, 2, 0.0), (2, 1, 299.0), (3, 2, 0.0), (4, 1, 301.0)], scale=3e-7, nphi=4),
                            dict(max_iter=5, tol_temp=0.0, tol_flux=0.0)),
+    # IterativeScheme Sequential (StaticBTESynthetic::solve_DOM: no synthetic
+    # correction): a fixed run and one stopped by the tolerances
+    "porous_b4_seq": (lambda: dataclasses.replace(decks.porous_spec(n_target=2000, nbands=4, nphi=4, seed=3),
+                                                  method="Sequential"),
+                      dict(max_iter=25, tol_temp=0.0, tol_flux=0.0)),
+    "gray12_seq": (lambda: dataclasses.replace(decks.gray_square_spec(12), method="Sequential"),
+                   dict(max_iter=400, tol_temp=1e-6, tol_flux=1e-4)),
 }
 
 

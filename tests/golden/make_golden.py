@@ -2,7 +2,7 @@
 
 Run from the repo root after `make -f oracle/ref.mk` (needs /root/reference):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [deck names]
 
 For every deck below, the seeded deck is written by giftbte_amd.decks, the
 reference's steady path is run at np=1 by ref_dump (which links the
@@ -43,8 +43,13 @@ STAGES = ["stage_system", "stage_ebound_init", "stage_gradX", "stage_gradY", "st
 
 
 def golden_decks():
+    import dataclasses
     import decklib
     from giftbte_amd import decks
+
+    def sequential(factory):
+        return lambda: dataclasses.replace(factory(), method="Sequential")
+
     return {
         # name: (deck factory, mesh format, max_iter, tolT, tolQ, stage system)
         "gray40": (decklib.DECKS["gray40"][0], "COMSOL", 500, 1e-8, 1e-4, (0, 3)),
@@ -52,6 +57,10 @@ def golden_decks():
         "msh_gray6": (lambda: decks.gray_square_spec(6), "MSH", 5, 0.0, 0.0, (0, 1)),
         "quad_cross8": (lambda: decks.crossplane_spec(os.path.join(HERE, "Si_15.dat"), n=8), "COMSOL", 3, 0.0, 0.0,
                         (4, 17)),
+        # IterativeScheme Sequential (StaticBTESynthetic::solve_DOM): a fixed
+        # run and one stopped by the deck tolerances
+        "porous_b4_seq": (sequential(decklib.DECKS["porous_b4"][0]), "COMSOL", 25, 0.0, 0.0, (1, 2)),
+        "gray12_seq": (sequential(lambda: decks.gray_square_spec(12)), "COMSOL", 400, 1e-6, 1e-4, (0, 3)),
     }
 
 
@@ -80,8 +89,10 @@ def run_one(name, factory, fmt, max_iter, tolT, tolQ, stage):
 def main():
     if not os.path.exists(REF_DUMP):
         sys.exit("oracle/_ref/ref_dump is missing: make -f oracle/ref.mk")
+    only = sys.argv[1:]  # optional deck names: write only those fixtures
     for name, args in golden_decks().items():
-        run_one(name, *args)
+        if not only or name in only:
+            run_one(name, *args)
 
 
 if __name__ == "__main__":

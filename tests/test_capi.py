@@ -96,3 +96,36 @@ def test_option_errors_are_bad_input(tmp_path):
         assert rc == _lib.GBTE_BAD_INPUT and word in msg
     with pytest.raises(ValueError):
         BTESolver(p, precision="fp16")
+
+
+def test_scheme_in_tables_and_validated(tmp_path):
+    """Order / IterativeScheme / MatrixSolver reach the tables; values the
+    reference's dispatch exits on are bad input (Solution.cpp:287-406)."""
+    import dataclasses
+    base = decks.gray_square_spec(3, nphi=2)
+    p = Problem(decks.write_deck(str(tmp_path / "syn"), base))
+    t = p.tables
+    assert (t.order, t.scheme, t.matrix_solver) == (_lib.GBTE_ORDER_2, _lib.GBTE_SCHEME_SYNTHETIC, _lib.GBTE_MATRIX_LU)
+    p = Problem(decks.write_deck(str(tmp_path / "seq"), dataclasses.replace(base, method="Sequential",
+                                                                            matrix="BICGSTAB", order="2limit")))
+    t = p.tables
+    assert (t.order, t.scheme, t.matrix_solver) == (_lib.GBTE_ORDER_2LIMIT, _lib.GBTE_SCHEME_SEQUENTIAL,
+                                                    _lib.GBTE_MATRIX_BICGSTAB)
+    for field, bad in (("method", "Jacobi"), ("order", "3"), ("matrix", "QR")):
+        with pytest.raises(GBTEError) as e:
+            Problem(decks.write_deck(str(tmp_path / f"bad_{field}"), dataclasses.replace(base, **{field: bad})))
+        assert e.value.code == _lib.GBTE_BAD_INPUT
+
+
+def test_order_1_is_unsupported(tmp_path):
+    """Order 1 and 2limit are refused before any device work."""
+    import dataclasses
+    lib = _lib.load()
+    for order in ("1", "2limit"):
+        p = Problem(decks.write_deck(str(tmp_path / f"o{order}"),
+                                     dataclasses.replace(decks.gray_square_spec(3, nphi=2), order=order)))
+        h = C.c_void_p()
+        rc = lib.gbte_create(p.handle, C.byref(_lib.Options(0, 0, 0, 0.0)), C.byref(h))
+        msg = lib.gbte_last_error(h).decode()
+        lib.gbte_destroy(h)
+        assert rc == _lib.GBTE_UNSUPPORTED and "Order" in msg
