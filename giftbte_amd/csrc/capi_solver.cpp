@@ -32,10 +32,9 @@ int gbte_create(const gbte_problem *p, const gbte_options *opts, gbte_handle **o
         *out = h;
         return GBTE_BAD_INPUT;
     }
-    if (p->p.deck.order != "2") {
+    if (p->p.deck.order == "2limit") {
         auto *h = new gbte_handle();
-        h->err = "Order " + p->p.deck.order +
-                 " (solve_firstorder / the 2limit limiter) is not supported: this build runs Order 2";
+        h->err = "Order 2limit (the gradient limiter) is not supported: this build runs Order 1 and 2";
         *out = h;
         return GBTE_UNSUPPORTED;
     }

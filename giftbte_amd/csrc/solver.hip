@@ -759,7 +759,7 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
     double *__restrict__ grad, SlotGeo g, FaceK f, Modes m, const int *__restrict__ gptr, const int *__restrict__ gidx,
     const double *__restrict__ gP, const int *__restrict__ gok, int nnz, double invLx, double *__restrict__ T,
     double *__restrict__ qx, double *__restrict__ qy, double *__restrict__ qz, double *__restrict__ RM, int nc,
-    Rows R) {
+    Rows R, int want_grad) {
     __shared__ double sh[kRowBlockMax / 64];
     int c, t0;
     row_lane<OneRow>(R, c, t0);
@@ -879,7 +879,7 @@ __global__ void __launch_bounds__(kRowBlockMax) k_face_moments(
         for (int j = 0; j < kNJ; ++j) {
             double a = gx[j], b = gy[j];
             if (isnan(a) || isnan(b)) { a = 0; b = 0; }
-            if (act && in[j]) {
+            if (act && in[j] && want_grad) {  // Order 1 keeps the gradients zero
                 gcx[2 * (size_t)tt[j]] = a;
                 gcy[2 * (size_t)tt[j]] = b;
             }
@@ -929,6 +929,9 @@ static_assert(kTileMomentLds <= kTileRows * kTileStride, "moment partials fit in
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// k_tile_face work: the moments (T, q, residual) and / or the LSQ gradient
+constexpr int kTileMoments = 1, kTileGradient = 2;
+
 struct TileDev {
     const int *cell0, *rptr, *rsrc, *nh, *vptr, *vs_ptr, *vs_row, *fin_i;
     const double *vs_w, *cmf, *kwf, *fin_d;
@@ -942,9 +945,10 @@ __global__ void __launch_bounds__(kTileBlock, 5) k_tile_face(const ST *__restric
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
                                                           double *__restrict__ RMo, double4 off, int DB, int ntiles,
-                                                          int moments) {
+                                                          int mode) {
     extern __shared__ double lds[];
     ST *rows = reinterpret_cast<ST *>(lds);  // staged rows in the storage type: no rounding
+    bool moments = mode & kTileMoments, gradient = mode & kTileGradient;
     constexpr bool kKEarly = false;
     int tile = xcd_block();
     if (tile >= ntiles) return;  // whole workgroup
@@ -1025,11 +1029,11 @@ __global__ void __launch_bounds__(kTileBlock, 5) k_tile_face(const ST *__restric
         v4d g = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < kTileKs; ++k)
-            if (k < nks)
+            if (gradient && k < nks)
                 g = __builtin_amdgcn_mfma_f64_16x16x4f64((double)cm[k], (double)grd[4 * k * kTileStride], g, 0, 0, 0);
         // lane holds rows lq + 4 i: (cell lq, x), (cell lq + 4, x), (cell lq, y), (cell lq + 4, y)
         int tg = slice * kTileW + 16 * w + l16;
-        if (tg < DB) {
+        if (gradient && tg < DB) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 int cl = lq + 4 * h;
@@ -2581,7 +2585,7 @@ static ST *rp(DBuf<double> &b) {
 int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
-    bool sequential = P.deck.method == "Sequential";
+    bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1";
     Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
             P.capacity_bulk[0], eref_, offT_, offE_, offW_, offQ_[0], offQ_[1], offQ_[2], offMac_};
     SlotGeo g{d_fsize.p, d_snb.p, d_sent.p, d_snx.p, d_sny.p, d_snz.p, d_sarea.p, d_vol.p,
@@ -2615,12 +2619,20 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                    nD * nB, nB, nE, m.eref);
     TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p,
                d_t_vs_row.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p, d_t_kwf.p, d_t_fin_d.p};
-    if (use_tiles) {
+    if (first_order) {
+        // Order 1 (solve_firstorder, StaticBTESyntheticSolver.cpp:487-966):
+        // the gradients are zero throughout (:537-542), so the outgoing
+        // boundary values are the cell values (_get_bound_ee_firstorder,
+        // StaticBTESynthetic.cpp:2883-2906) and the source has no gradient
+        // terms (_get_Re_firstorder, :2841-2881)
+        if (!check(hipMemsetAsync(d_grad.p, 0, d_grad.n * sizeof(double), s_), "zero gradients"))
+            return GBTE_HIP_ERROR;
+    } else if (use_tiles) {
         // _get_gradient_larger of the initial e: the tile kernel's LSQ product
         // alone (the tiles hold every cell's full stencil)
         GBTE_LAUNCH_ST(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, rp<ST>(d_e), rp<ST>(d_eb),
                        rp<ST>(d_grad), td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, make_double4(0, 0, 0, 0), nD * nB,
-                       n_tiles, 0);
+                       n_tiles, kTileGradient);
     } else {
         GBTE_LAUNCH_ST(k_gradient, dim3(gw), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), d_gptr.p, d_gidx.p,
                        d_gP.p, d_gok.p, dim, nnz, invLx, nc, R);
@@ -2667,13 +2679,14 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         if (use_tiles) {
             GBTE_LAUNCH_ST(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, rp<ST>(d_e), rp<ST>(d_eb),
                            rp<ST>(d_grad), td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p,
-                               make_double4(m.offT, m.offQx, m.offQy, m.offQz), nD * nB, n_tiles, 1);
+                               make_double4(m.offT, m.offQx, m.offQy, m.offQz), nD * nB, n_tiles,
+                               first_order ? kTileMoments : kTileMoments | kTileGradient);
         } else {
             hipLaunchKernelGGL(one_row ? k_vertex_e<true> : k_vertex_e<false>, dim3(vw), dim3(rowblk), 0, s_, d_e.p, d_eb.p,
                                d_eV.p, d_vptr.p, d_vsrc.p, d_vwt.p, nn, R);
             hipLaunchKernelGGL(one_row ? k_face_moments<true> : k_face_moments<false>, dim3(gw), dim3(rowblk), 0, s_, d_e.p,
                                d_eb.p, d_eV.p, d_grad.p, g, fk, m, d_gptr.p, d_gidx.p, d_gP.p, d_gok.p, nnz, invLx, d_T.p,
-                               d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc, R);
+                               d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc, R, first_order ? 0 : 1);
         }
         (void)hipEventRecord(ev_[4], s_);
         GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m,

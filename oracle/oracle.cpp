@@ -187,6 +187,7 @@ struct Oracle {
     Banded Kmacro;
     int macroPasses = 0;
 
+    bool first = false;  // Order 1
     explicit Oracle(const gbte_tables &tt) : t(tt) {
         nc = t.n_cells; nb = t.n_bounds; nf = t.n_faces; nn = t.n_nodes; nB = t.n_bands; nD = t.n_dirs;
         nm = t.n_matters; dim = t.dim; Lx = t.Lx;
@@ -391,6 +392,9 @@ struct Oracle {
     // _get_Re (StaticBTESynthetic.cpp:1427-1500) followed by the exact solve
     // of the upwind matrix of _get_coefficient (:784-840) and e += delta
     // (StaticBTESyntheticSolver.cpp:96-100).
+    // first: Order 1, _get_Re_firstorder (StaticBTESynthetic.cpp:2841-2881):
+    // Re holds only C T_old, the heat source and the incoming boundary values,
+    // and the solve gives e itself (StaticBTESyntheticSolver.cpp:575-579)
     bool sweep(int b, int d, vector<int> &order, vector<int> &indeg, vector<double> &Re, vector<double> &diag) {
         double *ed = E_(b, d);
         for (int i = 0; i < nc; ++i) Re[i] = 0;
@@ -401,7 +405,9 @@ struct Oracle {
                 size_t s = slot(ie, j);
                 double dp = dotvn(m, b, d, s);
                 double tmp = tau(m, b) * t.face_area[s] / t.volume[ie] * dp;
-                if (dp >= 0) {
+                if (first) {
+                    if (dp >= 0) dg += tmp;
+                } else if (dp >= 0) {
                     double ax = t.face_cx[s] - t.cx[ie], ay = t.face_cy[s] - t.cy[ie], az = t.face_cz[s] - t.cz[ie];
                     double ev = (ed[ie] + (ax * gradX[ie] + ay * gradY[ie] + az * gradZ[ie]) * 1.0);
                     Re[ie] -= tmp * ev;
@@ -417,7 +423,7 @@ struct Oracle {
             diag[ie] = dg;
             Re[ie] += Told[ie] * cap(m, b);
             Re[ie] += t.heat_source[ie] * hr(m, b) * tau(m, b);
-            Re[ie] -= ed[ie];
+            if (!first) Re[ie] -= ed[ie];
         }
         for (int ib = 0; ib < nb; ++ib)
             for (int ic = 0; ic < 2; ++ic) {
@@ -472,7 +478,7 @@ struct Oracle {
             g_err = "upwind graph of a direction has a cycle; the exact sweep needs an acyclic mesh";
             return false;
         }
-        for (int i = 0; i < nc; ++i) ed[i] = delta[i] + ed[i];
+        for (int i = 0; i < nc; ++i) ed[i] = first ? delta[i] : delta[i] + ed[i];
         return true;
     }
 
@@ -829,7 +835,11 @@ struct Oracle {
     int solve(const double *T0, const double *q0x, const double *q0y, const double *q0z, int maxIter, double tolT,
               double tolQ, int *iters, int *status) {
         if (dim != 2) { g_err = "the oracle covers 2D meshes"; return -1; }
-        if (t.order != GBTE_ORDER_2) { g_err = "the oracle covers Order 2"; return -1; }
+        if (t.order == GBTE_ORDER_2LIMIT) { g_err = "the oracle covers Order 1 and 2"; return -1; }
+        // Order 1 (solve_firstorder, StaticBTESyntheticSolver.cpp:487-966):
+        // the gradients stay zero, so the boundary values are the cell values
+        // (_get_bound_ee_firstorder, StaticBTESynthetic.cpp:2883-2906)
+        first = t.order == GBTE_ORDER_1;
         // Sequential: StaticBTESynthetic::solve_DOM (StaticBTESyntheticSolver.cpp:968-1178),
         // the same per-system steps without the vertex / face-gradient /
         // non-Fourier terms and without the synthetic correction
@@ -859,7 +869,7 @@ struct Oracle {
         cell_matrix();
         for (int d = 0; d < nD; d++)
             for (int b = 0; b < nB; ++b) {
-                gradient(b, d);
+                if (!first) gradient(b, d);
                 bound_ee(b, d);
             }
         if (!set_bound_ee_1()) return -1;
@@ -876,7 +886,7 @@ struct Oracle {
             std::fill(eboundLocal.begin(), eboundLocal.end(), 0.0);
             for (int d = 0; d < nD; d++)
                 for (int b = 0; b < nB; ++b) {
-                    gradient(b, d);
+                    if (!first) gradient(b, d);
                     if (!sweep(b, d, order, indeg, Re, diag)) return -1;
                     bound_ee(b, d);
                     if (!seq) {

@@ -2,7 +2,7 @@
 //
 // Links against the reference GiftBTE objects built by oracle/ref.mk and
 // replays the steady flow of src/BTEMain.cpp:15-421 at np=1 for the
-// Order=2 / IterativeScheme=Synthetic or Sequential / MatrixSolver=LU branches
+// Order=1 or 2 / IterativeScheme=Synthetic or Sequential / MatrixSolver=LU branches
 // (src/Solution/Solution.cpp:262-414), then writes every field the GPU
 // solver is compared against as .npy files at full FP64 precision.  The
 // reference's own .dat outputs carry 6 significant digits only
@@ -335,8 +335,9 @@ int main(int argc, char **argv) {
     if (argc >= 8) { sb = atoi(argv[6]); sd = atoi(argv[7]); }
 
     Deck deck = parse_deck();
-    if (deck.order != "2" || (deck.method != "Synthetic" && deck.method != "Sequential") || deck.matrix != "LU") {
-        fprintf(stderr, "ref_dump covers Order=2 Synthetic / Sequential LU only\n");
+    if ((deck.order != "2" && deck.order != "1") || (deck.method != "Synthetic" && deck.method != "Sequential") ||
+        deck.matrix != "LU") {
+        fprintf(stderr, "ref_dump covers Order=1/2 Synthetic / Sequential LU only\n");
         return 3;
     }
     std::ifstream bandFile("input/PHONON");
@@ -380,10 +381,16 @@ int main(int argc, char **argv) {
     solver.errorOld = 0;
     solver.errorOld_heat = 0;
     dump_tables(solver, "");
-    if (deck.method == "Sequential")
-        solver.solve_DOM(0, maxIter, 0, tolT, tolQ);  // StaticBTESyntheticSolver.cpp:968
-    else
+    // the dispatch of Solution.cpp:287-340
+    bool seq = deck.method == "Sequential";
+    if (deck.order == "1") {
+        if (seq) solver.solve_DOM_firstorder(0, maxIter, 0, tolT, tolQ);  // StaticBTESyntheticSolver.cpp:1394
+        else solver.solve_firstorder(0, maxIter, 0, tolT, tolQ);         // :487
+    } else if (seq) {
+        solver.solve_DOM(0, maxIter, 0, tolT, tolQ);  // :968
+    } else {
         solver.solve(0, maxIter, 0, tolT, tolQ);
+    }
     size_t nc = solver.numCell, nB = solver.numBand, nD = solver.numDirection, nb = solver.numBound;
     dump_d("T_lattice", solver.temperature, {nc});
     dump_d("T_energy", solver.totalEnergy, {nc});

@@ -33,6 +33,11 @@ DECKS = {
                       dict(max_iter=25, tol_temp=0.0, tol_flux=0.0)),
     "gray12_seq": (lambda: dataclasses.replace(decks.gray_square_spec(12), method="Sequential"),
                    dict(max_iter=400, tol_temp=1e-6, tol_flux=1e-4)),
+    # Order 1 (solve_firstorder / solve_DOM_firstorder: no gradient terms)
+    "porous_b4_o1": (lambda: dataclasses.replace(decks.porous_spec(n_target=2000, nbands=4, nphi=4, seed=3), order="1"),
+                     dict(max_iter=25, tol_temp=0.0, tol_flux=0.0)),
+    "gray12_o1_seq": (lambda: dataclasses.replace(decks.gray_square_spec(12), method="Sequential", order="1"),
+                      dict(max_iter=400, tol_temp=1e-6, tol_flux=1e-4)),
 }
 
 

@@ -50,6 +50,9 @@ def golden_decks():
     def sequential(factory):
         return lambda: dataclasses.replace(factory(), method="Sequential")
 
+    def first_order(factory):
+        return lambda: dataclasses.replace(factory(), order="1")
+
     return {
         # name: (deck factory, mesh format, max_iter, tolT, tolQ, stage system)
         "gray40": (decklib.DECKS["gray40"][0], "COMSOL", 500, 1e-8, 1e-4, (0, 3)),
@@ -61,6 +64,10 @@ def golden_decks():
         # run and one stopped by the deck tolerances
         "porous_b4_seq": (sequential(decklib.DECKS["porous_b4"][0]), "COMSOL", 25, 0.0, 0.0, (1, 2)),
         "gray12_seq": (sequential(lambda: decks.gray_square_spec(12)), "COMSOL", 400, 1e-6, 1e-4, (0, 3)),
+        # Order 1 (solve_firstorder / solve_DOM_firstorder)
+        "porous_b4_o1": (first_order(decklib.DECKS["porous_b4"][0]), "COMSOL", 25, 0.0, 0.0, (1, 2)),
+        "gray12_o1_seq": (first_order(sequential(lambda: decks.gray_square_spec(12))), "COMSOL", 400, 1e-6, 1e-4,
+                          (0, 3)),
     }
 
 

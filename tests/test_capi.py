@@ -117,11 +117,11 @@ def test_scheme_in_tables_and_validated(tmp_path):
         assert e.value.code == _lib.GBTE_BAD_INPUT
 
 
-def test_order_1_is_unsupported(tmp_path):
-    """Order 1 and 2limit are refused before any device work."""
+def test_order_2limit_is_unsupported(tmp_path):
+    """The 2limit limiter is refused before any device work."""
     import dataclasses
     lib = _lib.load()
-    for order in ("1", "2limit"):
+    for order in ("2limit",):
         p = Problem(decks.write_deck(str(tmp_path / f"o{order}"),
                                      dataclasses.replace(decks.gray_square_spec(3, nphi=2), order=order)))
         h = C.c_void_p()
