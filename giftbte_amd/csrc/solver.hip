@@ -377,6 +377,85 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
     }
 }
 
+// Order 2limit: the limiter of _get_gradient_larger (StaticBTESynthetic.cpp:
+// 1372-1422) applied to the stored LSQ gradients.  The range is the cell's
+// value, its face neighbours' and the incoming non-specular boundary values
+// of side 0 of its boundary faces; the faces are then visited in order, and
+// each face whose extrapolated value leaves the range sets the factor anew.
+// The stored gradient becomes the limited one (g * limit; the reference keeps
+// the factor apart and applies it to r.g).  One wave per cell row.
+constexpr int kLimNone = -2147483647 - 1;  // face without a contribution to the range
+
+template <typename ST>
+__global__ void __launch_bounds__(64 * kSrcRows) k_limit(const ST *__restrict__ e, ST *__restrict__ grad,
+                                                        const ST *__restrict__ eb, const int *__restrict__ fsize,
+                                                        const double *__restrict__ srec, const int *__restrict__ slim,
+                                                        Modes m, int nc, int DB, int nB) {
+    int c = __builtin_amdgcn_readfirstlane(xcd_block() * kSrcRows + (int)(threadIdx.x >> 6));
+    int lane = threadIdx.x & 63;
+    if (c >= nc) return;  // whole wave
+    int nf = fsize[c];
+    const double *rd = srec + (size_t)c * kSrcD;
+    const int *li = slim + (size_t)c * kSrcI;
+    double fnx[kMaxFace2D], fny[kMaxFace2D], fax[kMaxFace2D], fay[kMaxFace2D];
+    int fk[kMaxFace2D];
+#pragma unroll
+    for (int f = 0; f < kMaxFace2D; ++f) {
+        fnx[f] = rd[f];
+        fny[f] = rd[4 + f];
+        fax[f] = rd[12 + f];
+        fay[f] = rd[16 + f];
+        fk[f] = li[f];
+    }
+    const ST *ec = e + (size_t)c * DB;
+    ST *gc = grad + (size_t)c * 2 * DB;
+    for (int t = lane; t < DB; t += 64) {
+        int d = t / nB, b = t - d * nB;
+        double vb = m.vg[b], sx = m.sx[d], sy = m.sy[d];
+        double e0 = ec[t], mx = e0, mn = e0;
+#pragma unroll
+        for (int f = 0; f < kMaxFace2D; ++f) {
+            if (f >= nf) break;
+            int k = fk[f];
+            double v;
+            if (k >= 0) v = e[(size_t)k * DB + t];
+            else if (k != kLimNone && vb * sx * fnx[f] + vb * sy * fny[f] < 0) v = eb[(size_t)(-k - 1) * DB + t];
+            else continue;
+            if (v > mx) mx = v;
+            if (v < mn) mn = v;
+        }
+        double2 g = ld2(gc + 2 * t);
+        double L = 1.0;
+#pragma unroll
+        for (int f = 0; f < kMaxFace2D; ++f) {
+            if (f >= nf) break;
+            double p = g.x * fax[f] + g.y * fay[f];
+            if (p * L + e0 < mn) {
+                double y = (mn - e0) / p;
+                L = (y * y + 2 * y) / (y * y + y + 2);
+            }
+            if (p * L + e0 > mx) {
+                double y = (mx - e0) / p;
+                L = (y * y + 2 * y) / (y * y + y + 2);
+            }
+        }
+        if (L != 1.0) {
+            gc[2 * t] = (ST)(g.x * L);
+            gc[2 * t + 1] = (ST)(g.y * L);
+        }
+    }
+}
+
+// Boundary values before their first write: the reference allocates them
+// uninitialised (StaticBTESynthetic.cpp:310) and only the 2limit limiter of
+// the set-up gradients reads such entries; they start as an absolute zero,
+// which is -C_b Tref as a deviation.
+template <typename ST>
+__global__ void k_eb_zero(ST *__restrict__ eb, const double *__restrict__ cap, double eref, size_t n, int nB) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) eb[i] = (ST)(-cap[i % nB] * eref);
+}
+
 // The upwind solve of every direction (the Eigen SparseLU solve of
 // StaticBTESyntheticSolver.cpp:96-100 on the matrix of _get_coefficient,
 // StaticBTESynthetic.cpp:784-840): one workgroup walks all upwind levels of
@@ -1766,7 +1845,7 @@ bool Solver::build() {
         }
     {
         std::vector<double> srec((size_t)nc * kSrcD, 0.0);
-        std::vector<int> srei((size_t)nc * kSrcI, 0);
+        std::vector<int> srei((size_t)nc * kSrcI, 0), slim((size_t)nc * kSrcI, kLimNone);
         for (int c = 0; c < nc; ++c) {
             if (p.face_size[c] > kMaxFace2D) { err_ = "more than 4 faces per 2D cell"; return false; }
             for (int f = 0; f < kSrcI; ++f) {
@@ -1782,9 +1861,14 @@ bool Solver::build() {
                 r[20 + f] = sqx[s];
                 r[24 + f] = sqy[s];
                 srei[(size_t)c * kSrcI + f] = snb[s] >= 0 ? snb[s] : -sent[s] - 1;
+                // the limiter's range: the neighbour, or side 0 of a
+                // non-specular boundary (StaticBTESynthetic.cpp:1380-1399)
+                int fb = p.face_bound[s];
+                if (fb == -1) slim[(size_t)c * kSrcI + f] = p.face_neighbor[s];
+                else if (p.bound_type[fb] != 3) slim[(size_t)c * kSrcI + f] = -(2 * fb) - 1;
             }
         }
-        if (!upload(d_srec, srec) || !upload(d_srei, srei)) return false;
+        if (!upload(d_srec, srec) || !upload(d_srei, srei) || !upload(d_slim, slim)) return false;
     }
     if (!upload(d_fsize, p.face_size) || !upload(d_snb, snb) || !upload(d_sent, sent) || !upload(d_vol, p.volume) ||
         !upload(d_Q, p.heat_source) || !upload(d_snx, p.face_nx) || !upload(d_sny, p.face_ny) || !upload(d_snz, p.face_nz) ||
@@ -2381,6 +2465,16 @@ bool Solver::build() {
     if (!check(hipMemsetAsync(d_eb.p, 0, d_eb.n * sizeof(double), s_), "hipMemsetAsync") ||
         !check(hipMemsetAsync(d_qz.p, 0, nc * sizeof(double), s_), "hipMemsetAsync"))
         return false;
+    if (P.deck.order == "2limit") {
+        size_t n = nEB;
+        dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+        if (fp32_)
+            hipLaunchKernelGGL(k_eb_zero<float>, grid, dim3(kBlock), 0, s_, reinterpret_cast<float *>(d_eb.p),
+                               d_cap.p, eref_, n, nB);
+        else
+            hipLaunchKernelGGL(k_eb_zero<double>, grid, dim3(kBlock), 0, s_, d_eb.p, d_cap.p, eref_, n, nB);
+        if (!check(hipGetLastError(), "boundary value set-up")) return false;
+    }
     // initial temperature and flux (Solution.cpp:78-90)
     if (!check(hipMemcpy(d_T.p, p.initial_temp.data(), nc * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
         !check(hipMemsetAsync(d_qx.p, 0, nc * sizeof(double), s_), "hipMemsetAsync") ||
@@ -2585,7 +2679,8 @@ static ST *rp(DBuf<double> &b) {
 int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
-    bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1";
+    bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1",
+         limiter = P.deck.order == "2limit";
     Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
             P.capacity_bulk[0], eref_, offT_, offE_, offW_, offQ_[0], offQ_[1], offQ_[2], offMac_};
     SlotGeo g{d_fsize.p, d_snb.p, d_sent.p, d_snx.p, d_sny.p, d_snz.p, d_sarea.p, d_vol.p,
@@ -2619,6 +2714,18 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                    nD * nB, nB, nE, m.eref);
     TileDev td{d_t_cell0.p, d_t_rptr.p, d_t_rsrc.p, d_t_nh.p, d_t_vptr.p, d_t_vs_ptr.p,
                d_t_vs_row.p, d_t_fin_i.p, d_t_vs_w.p, d_t_cmf.p, d_t_kwf.p, d_t_fin_d.p};
+    // _get_gradient_larger of e: the tile kernel's LSQ product alone (the
+    // tiles hold every cell's full stencil), or the row kernel
+    auto launch_gradient = [&]() {
+        if (use_tiles) {
+            GBTE_LAUNCH_ST(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, rp<ST>(d_e), rp<ST>(d_eb),
+                           rp<ST>(d_grad), td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, make_double4(0, 0, 0, 0),
+                           nD * nB, n_tiles, kTileGradient);
+        } else {
+            GBTE_LAUNCH_ST(k_gradient, dim3(gw), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), d_gptr.p, d_gidx.p,
+                           d_gP.p, d_gok.p, dim, nnz, invLx, nc, R);
+        }
+    };
     if (first_order) {
         // Order 1 (solve_firstorder, StaticBTESyntheticSolver.cpp:487-966):
         // the gradients are zero throughout (:537-542), so the outgoing
@@ -2627,16 +2734,12 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         // terms (_get_Re_firstorder, :2841-2881)
         if (!check(hipMemsetAsync(d_grad.p, 0, d_grad.n * sizeof(double), s_), "zero gradients"))
             return GBTE_HIP_ERROR;
-    } else if (use_tiles) {
-        // _get_gradient_larger of the initial e: the tile kernel's LSQ product
-        // alone (the tiles hold every cell's full stencil)
-        GBTE_LAUNCH_ST(k_tile_face, dim3(xgrid(n_tiles)), dim3(kTileBlock), tile_lds, rp<ST>(d_e), rp<ST>(d_eb),
-                       rp<ST>(d_grad), td, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, make_double4(0, 0, 0, 0), nD * nB,
-                       n_tiles, kTileGradient);
     } else {
-        GBTE_LAUNCH_ST(k_gradient, dim3(gw), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), d_gptr.p, d_gidx.p,
-                       d_gP.p, d_gok.p, dim, nnz, invLx, nc, R);
+        launch_gradient();
     }
+    if (limiter)
+        GBTE_LAUNCH_ST(k_limit, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), 0, rp<ST>(d_e),
+                       rp<ST>(d_grad), rp<ST>(d_eb), d_fsize.p, d_srec.p, d_slim.p, m, nc, nD * nB, nB);
     GBTE_LAUNCH_ST(k_bound_out, dim3(std::max(bw, 1)), dim3(rowblk), 0, rp<ST>(d_e), rp<ST>(d_grad), rp<ST>(d_eb), g, m,
                        d_ent_list.p, d_be_cell.p, d_be_slot.p, n_ent, dim, R);
     GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m, d_ent_list.p,
@@ -2655,6 +2758,14 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     for (nt = 0; nt < max_iter; ++nt) {
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
+        // this iteration's gradients were formed by the previous iteration's
+        // face kernel (the first iteration's by a fresh pass: the set-up ones
+        // were limited in place); the limiter needs the boundary values
+        // written since
+        if (limiter && nt == 0) launch_gradient();
+        if (limiter)
+            GBTE_LAUNCH_ST(k_limit, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), 0, rp<ST>(d_e),
+                           rp<ST>(d_grad), rp<ST>(d_eb), d_fsize.p, d_srec.p, d_slim.p, m, nc, nD * nB, nB);
         (void)hipEventRecord(ev_[0], s_);
         GBTE_LAUNCH_ST(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows),
                        (size_t)(4 * nB + 2 * nD) * sizeof(double) + (size_t)kSrcRows * nD * sizeof(int), d_posc.p,

@@ -132,7 +132,7 @@ class Solver {
     DBuf<int> d_vptr, d_vsrc, d_tptr, d_tsrc;
     DBuf<double> d_vwt, d_tdist;
     // face-gradient / non-Fourier slot data
-    DBuf<int> d_fkind, d_fother, d_fv0, d_fv1, d_srei;
+    DBuf<int> d_fkind, d_fother, d_fv0, d_fv1, d_srei, d_slim;
     DBuf<double> d_srec;  // per-cell face records of the source kernel
     DBuf<double> d_fga, d_fn0x, d_fn0y, d_fmx, d_fmy;
     // boundary entries

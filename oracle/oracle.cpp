@@ -187,10 +187,13 @@ struct Oracle {
     Banded Kmacro;
     int macroPasses = 0;
 
-    bool first = false;  // Order 1
+    bool first = false;    // Order 1
+    bool limiter = false;  // Order 2limit
+    vector<double> lim;    // limiter factor per cell of the current system
     explicit Oracle(const gbte_tables &tt) : t(tt) {
         nc = t.n_cells; nb = t.n_bounds; nf = t.n_faces; nn = t.n_nodes; nB = t.n_bands; nD = t.n_dirs;
         nm = t.n_matters; dim = t.dim; Lx = t.Lx;
+        lim.assign(nc, 1.0);
     }
     size_t slot(int i, int j) const { return (size_t)i * 6 + j; }
     double vgx(int m, int b, int d) const { return t.vg[(size_t)m * nB + b] * t.dir_x[d]; }
@@ -387,6 +390,39 @@ struct Oracle {
             if (std::isnan(gx) || std::isnan(gy) || std::isnan(gz)) continue;
             gradX[i] = gx; gradY[i] = gy; gradZ[i] = gz;
         }
+        for (int i = 0; i < nc; ++i) lim[i] = 1.0;
+        if (!limiter) return;
+        // Order 2limit (StaticBTESynthetic.cpp:1372-1422): the range of the
+        // cell, its face neighbours and the incoming non-specular boundary
+        // values (side 0), then the faces in order; each face that
+        // overshoots the range sets the factor anew
+        for (int i = 0; i < nc; ++i) {
+            int m = t.matter[i];
+            double mx = ed[i], mn = ed[i];
+            for (int j = 0; j < t.face_size[i]; ++j) {
+                size_t s = slot(i, j);
+                int fb = t.face_bound[s];
+                double v;
+                if (fb == -1) v = ed[t.face_neighbor[s]];
+                else if (t.bound_type[fb] != 3 && dotvn(m, b, d, s) < 0) v = EB(b, d, fb, 0);
+                else continue;
+                if (v > mx) mx = v;
+                if (v < mn) mn = v;
+            }
+            for (int j = 0; j < t.face_size[i]; ++j) {
+                size_t s = slot(i, j);
+                double ax = t.face_cx[s] - t.cx[i], ay = t.face_cy[s] - t.cy[i], az = t.face_cz[s] - t.cz[i];
+                double p = gradX[i] * ax + gradY[i] * ay + gradZ[i] * az;
+                if (p * lim[i] + ed[i] < mn) {
+                    double y = (mn - ed[i]) / p;
+                    lim[i] = (y * y + 2 * y) / (y * y + y + 2);
+                }
+                if (p * lim[i] + ed[i] > mx) {
+                    double y = (mx - ed[i]) / p;
+                    lim[i] = (y * y + 2 * y) / (y * y + y + 2);
+                }
+            }
+        }
     }
 
     // _get_Re (StaticBTESynthetic.cpp:1427-1500) followed by the exact solve
@@ -409,13 +445,13 @@ struct Oracle {
                     if (dp >= 0) dg += tmp;
                 } else if (dp >= 0) {
                     double ax = t.face_cx[s] - t.cx[ie], ay = t.face_cy[s] - t.cy[ie], az = t.face_cz[s] - t.cz[ie];
-                    double ev = (ed[ie] + (ax * gradX[ie] + ay * gradY[ie] + az * gradZ[ie]) * 1.0);
+                    double ev = (ed[ie] + (ax * gradX[ie] + ay * gradY[ie] + az * gradZ[ie]) * lim[ie]);
                     Re[ie] -= tmp * ev;
                     dg += tmp;
                 } else if (t.face_bound[s] == -1) {
                     int nbr = t.face_neighbor[s];
                     double ax = t.face_cx[s] - t.cx[nbr], ay = t.face_cy[s] - t.cy[nbr], az = t.face_cz[s] - t.cz[nbr];
-                    double ev = (ed[nbr] + (ax * gradX[nbr] + ay * gradY[nbr] + az * gradZ[nbr]) * 1.0);
+                    double ev = (ed[nbr] + (ax * gradX[nbr] + ay * gradY[nbr] + az * gradZ[nbr]) * lim[nbr]);
                     Re[ie] -= tmp * ev;
                 }
             }
@@ -492,7 +528,7 @@ struct Oracle {
                 size_t s = slot(ie, jf);
                 if (dotvn(t.matter[ie], b, d, s) >= 0) {
                     double ax = t.face_cx[s] - t.cx[ie], ay = t.face_cy[s] - t.cy[ie], az = t.face_cz[s] - t.cz[ie];
-                    double ev = (ed[ie] + (ax * gradX[ie] + ay * gradY[ie] + az * gradZ[ie]) * 1.0);
+                    double ev = (ed[ie] + (ax * gradX[ie] + ay * gradY[ie] + az * gradZ[ie]) * lim[ie]);
                     eboundLocal[((size_t)b * nb + ib) * 2 + ic] = ev;
                     EB(b, d, ib, ic) = ev;
                 }
@@ -835,7 +871,7 @@ struct Oracle {
     int solve(const double *T0, const double *q0x, const double *q0y, const double *q0z, int maxIter, double tolT,
               double tolQ, int *iters, int *status) {
         if (dim != 2) { g_err = "the oracle covers 2D meshes"; return -1; }
-        if (t.order == GBTE_ORDER_2LIMIT) { g_err = "the oracle covers Order 1 and 2"; return -1; }
+        limiter = t.order == GBTE_ORDER_2LIMIT;
         // Order 1 (solve_firstorder, StaticBTESyntheticSolver.cpp:487-966):
         // the gradients stay zero, so the boundary values are the cell values
         // (_get_bound_ee_firstorder, StaticBTESynthetic.cpp:2883-2906)
@@ -854,7 +890,7 @@ struct Oracle {
         qx.assign(q0x, q0x + nc); qy.assign(q0y, q0y + nc); qz.assign(q0z, q0z + nc);
         qxo.assign(nc, 0.0); qyo.assign(nc, 0.0); qzo.assign(nc, 0.0);
         Tband.assign((size_t)nB * nc, 0.0); E.assign(nc, 0.0); ReMacro.assign(nc, 0.0);
-        gradX.assign(nc, 0.0); gradY.assign(nc, 0.0); gradZ.assign(nc, 0.0);
+        gradX.assign(nc, 0.0); gradY.assign(nc, 0.0); gradZ.assign(nc, 0.0); lim.assign(nc, 1.0);
         eV.assign((size_t)nn * nm, 0.0); TV.assign((size_t)nn * nm, 0.0);
         gfx.assign((size_t)nf * 2, 0.0); gfy.assign((size_t)nf * 2, 0.0); gfz.assign((size_t)nf * 2, 0.0);
         tmac.assign(nb, vector<double>(2, 0.0));

@@ -2,7 +2,7 @@
 //
 // Links against the reference GiftBTE objects built by oracle/ref.mk and
 // replays the steady flow of src/BTEMain.cpp:15-421 at np=1 for the
-// Order=1 or 2 / IterativeScheme=Synthetic or Sequential / MatrixSolver=LU branches
+// Order=1, 2 or 2limit / IterativeScheme=Synthetic or Sequential / MatrixSolver=LU branches
 // (src/Solution/Solution.cpp:262-414), then writes every field the GPU
 // solver is compared against as .npy files at full FP64 precision.  The
 // reference's own .dat outputs carry 6 significant digits only
@@ -16,6 +16,7 @@
 //   tolT/tolQ stop thresholds (0 0 runs exactly max_iter iterations)
 //   stage_*   optional: also dump every per-system stage of iteration 0 for
 //             system (band, direction)
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -335,9 +336,9 @@ int main(int argc, char **argv) {
     if (argc >= 8) { sb = atoi(argv[6]); sd = atoi(argv[7]); }
 
     Deck deck = parse_deck();
-    if ((deck.order != "2" && deck.order != "1") || (deck.method != "Synthetic" && deck.method != "Sequential") ||
-        deck.matrix != "LU") {
-        fprintf(stderr, "ref_dump covers Order=1/2 Synthetic / Sequential LU only\n");
+    if ((deck.order != "2" && deck.order != "1" && deck.order != "2limit") ||
+        (deck.method != "Synthetic" && deck.method != "Sequential") || deck.matrix != "LU") {
+        fprintf(stderr, "ref_dump covers Order=1/2/2limit Synthetic / Sequential LU only\n");
         return 3;
     }
     std::ifstream bandFile("input/PHONON");
@@ -377,6 +378,12 @@ int main(int argc, char **argv) {
 
     StaticBTESynthetic solver(&dm->BTEMeshes, bcs, bands, angles, 1, 0);
     init_solver_state(solver, *dm, sa);
+    // ebound is allocated without initialisation (StaticBTESynthetic.cpp:310).
+    // Only the 2limit limiter reads entries before the first _set_bound_ee_1
+    // writes them (the incoming side-0 values of the set-up gradients,
+    // :1389-1399), so that run depends on what the allocation held: zero it,
+    // which is what fresh pages give, so the fixture is defined.
+    std::fill(solver.ebound, solver.ebound + (size_t)solver.numBand * solver.numDirection * solver.numBound * 2, 0.0);
     solver.error_old = 0;
     solver.errorOld = 0;
     solver.errorOld_heat = 0;
@@ -386,10 +393,10 @@ int main(int argc, char **argv) {
     if (deck.order == "1") {
         if (seq) solver.solve_DOM_firstorder(0, maxIter, 0, tolT, tolQ);  // StaticBTESyntheticSolver.cpp:1394
         else solver.solve_firstorder(0, maxIter, 0, tolT, tolQ);         // :487
-    } else if (seq) {
-        solver.solve_DOM(0, maxIter, 0, tolT, tolQ);  // :968
     } else {
-        solver.solve(0, maxIter, 0, tolT, tolQ);
+        int lim = deck.order == "2limit" ? 1 : 0;  // Use_Limiter (Solution.cpp:363-400)
+        if (seq) solver.solve_DOM(0, maxIter, lim, tolT, tolQ);  // :968
+        else solver.solve(0, maxIter, lim, tolT, tolQ);
     }
     size_t nc = solver.numCell, nB = solver.numBand, nD = solver.numDirection, nb = solver.numBound;
     dump_d("T_lattice", solver.temperature, {nc});

@@ -53,6 +53,9 @@ def golden_decks():
     def first_order(factory):
         return lambda: dataclasses.replace(factory(), order="1")
 
+    def limited(factory):
+        return lambda: dataclasses.replace(factory(), order="2limit")
+
     return {
         # name: (deck factory, mesh format, max_iter, tolT, tolQ, stage system)
         "gray40": (decklib.DECKS["gray40"][0], "COMSOL", 500, 1e-8, 1e-4, (0, 3)),
@@ -68,6 +71,10 @@ def golden_decks():
         "porous_b4_o1": (first_order(decklib.DECKS["porous_b4"][0]), "COMSOL", 25, 0.0, 0.0, (1, 2)),
         "gray12_o1_seq": (first_order(sequential(lambda: decks.gray_square_spec(12))), "COMSOL", 400, 1e-6, 1e-4,
                           (0, 3)),
+        # Order 2limit (the gradient limiter of _get_gradient_larger)
+        "porous_b4_2limit": (limited(decklib.DECKS["porous_b4"][0]), "COMSOL", 25, 0.0, 0.0, (1, 2)),
+        "square_b20_2limit_seq": (limited(sequential(decklib.DECKS["square_b20_diffuse"][0])), "COMSOL", 5, 0.0, 0.0,
+                                  (3, 5)),
     }
 
 

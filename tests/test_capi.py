@@ -116,16 +116,3 @@ def test_scheme_in_tables_and_validated(tmp_path):
             Problem(decks.write_deck(str(tmp_path / f"bad_{field}"), dataclasses.replace(base, **{field: bad})))
         assert e.value.code == _lib.GBTE_BAD_INPUT
 
-
-def test_order_2limit_is_unsupported(tmp_path):
-    """The 2limit limiter is refused before any device work."""
-    import dataclasses
-    lib = _lib.load()
-    for order in ("2limit",):
-        p = Problem(decks.write_deck(str(tmp_path / f"o{order}"),
-                                     dataclasses.replace(decks.gray_square_spec(3, nphi=2), order=order)))
-        h = C.c_void_p()
-        rc = lib.gbte_create(p.handle, C.byref(_lib.Options(0, 0, 0, 0.0)), C.byref(h))
-        msg = lib.gbte_last_error(h).decode()
-        lib.gbte_destroy(h)
-        assert rc == _lib.GBTE_UNSUPPORTED and "Order" in msg

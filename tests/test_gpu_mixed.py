@@ -32,10 +32,17 @@ def _keff_ref(p, ref, dT):
 @pytest.mark.parametrize("name", list(decklib.DECKS))
 def test_fp32_fields_match_oracle(name, deck_root):
     from giftbte_amd import Problem, BTESolver
+    from giftbte_amd import GBTEError, _lib
     d = decklib.make(name, deck_root)
     s = decklib.settings(name)
     print(f"\n[gpu mixed] deck={name} settings={s}", flush=True)
     p = Problem(d)
+    if p.tables.order == _lib.GBTE_ORDER_2LIMIT:
+        # the limiter's switch is discontinuous: FP32 storage is refused
+        with pytest.raises(GBTEError) as e:
+            BTESolver(p, precision="fp32")
+        assert e.value.code == _lib.GBTE_UNSUPPORTED
+        return
     sol = BTESolver(p, precision="fp32")
     fp = sol.fourier()
     st = sol.solve(s["max_iter"], s["tol_temp"], s["tol_flux"])

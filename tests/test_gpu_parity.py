@@ -31,7 +31,8 @@ def _gpu_run(deck_dir, s, variant="direct"):
 # default kernels on every deck; PCG linear solver and the fallback kernels
 # (level-synchronous sweep, untiled face kernels) on two decks
 CASES = ([(n, "direct") for n in decklib.DECKS] + [("porous_b4", "pcg")] +
-         [("porous_b4", "fallback"), ("square_b20_diffuse", "fallback"), ("porous_b4_o1", "fallback")])
+         [("porous_b4", "fallback"), ("square_b20_diffuse", "fallback"), ("porous_b4_o1", "fallback"),
+          ("porous_b4_2limit", "fallback")])
 
 
 @pytest.mark.parametrize("name,variant", CASES)
