@@ -1,7 +1,8 @@
 /*
  * giftbte.h - C ABI of libgiftbte_hip.so, the MI355X solver for GiftBTE's
- * steady StaticBTESynthetic path (Order 2, IterativeScheme Synthetic or
- * Sequential, MatrixSolver LU or BICGSTAB) and its StaticFourier warm start.
+ * steady StaticBTESynthetic path (Order 2, 2limit or 1, IterativeScheme
+ * Synthetic or Sequential, MatrixSolver LU or BICGSTAB) and its StaticFourier
+ * warm start.
  *
  * The reference exposes this path as C++ classes with no FFI:
  *   StaticFourier(BTEMesh*, BTEBoundaryCondition*, BTEBand*, int, int) + _solve()
@@ -40,7 +41,8 @@ enum {
  * solve / solve_Iterative (StaticBTESyntheticSolver.cpp:12-486, Order 2 and
  * 2limit, Synthetic), solve_DOM / solve_DOM_Iterative (:968-1392, Sequential)
  * and the first-order variants (:487-966, :1394-1802).  gbte_solve runs
- * Order 2 with either scheme; Order 1 and 2limit give GBTE_UNSUPPORTED.
+ * all of them (2limit: the limiter of StaticBTESynthetic.cpp:1372-1422; not
+ * with GBTE_FP32_SWEEP, whose float rounding moves the limiter's switch).
  * BICGSTAB differs from LU only in the per-system linear solve (Eigen BiCGSTAB
  * with an incomplete-LU preconditioner at its default tolerance, machine
  * epsilon), so both run the exact solves here. */
