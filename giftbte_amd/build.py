@@ -46,7 +46,8 @@ def build_lib() -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    srcs = ["problem.cpp", "reorder.cpp", "ndchol.cpp", "tiles.cpp", "capi_problem.cpp", "capi_solver.cpp", "solver.hip"]
+    srcs = ["problem.cpp", "bands.cpp", "reorder.cpp", "ndchol.cpp", "tiles.cpp", "capi_problem.cpp", "capi_solver.cpp",
+            "solver.hip"]
     for s in srcs:
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, s + ".o")

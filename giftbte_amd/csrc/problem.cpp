@@ -8,6 +8,8 @@
 // directly.
 #include "problem.hpp"
 
+#include "bands.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -92,7 +94,8 @@ struct Bands {
     std::vector<int> geo_matter;
 };
 
-// src/BTEBand/BTEBand.cpp:14-80 (DATABASE and inline rows) and :791-844.
+// src/BTEBand/BTEBand.cpp:14-80 (DATABASE and inline rows), :81-740
+// (ShengBTE / ALAMODE binning, bands.cpp) and :791-844.
 std::string parse_bands(const std::string &dir, int dim_mat, Bands &B) {
     std::ifstream f(join(dir, "input/PHONON"));
     std::string s;
@@ -116,8 +119,20 @@ std::string parse_bands(const std::string &dir, int dim_mat, Bands &B) {
                     size_t k = (size_t)m * B.nband + b;
                     db >> B.v[k] >> B.tau[k] >> B.C[k] >> B.lr[k] >> B.hr[k];
                 }
-            } else if (s.find("ShengBTE") != std::string::npos || s.find("ALAMODE") != std::string::npos) {
-                return "ShengBTE/ALAMODE band binning is not supported; use a DATABASE band file";
+            } else if ((s.find("ShengBTE") != std::string::npos && s.find("ShengBTE") > 0) ||
+                       (s.find("ALAMODE") != std::string::npos && s.find("ALAMODE") > 0)) {
+                // spectral binning of ShengBTE / ALAMODE mode data (bands.cpp)
+                bool sheng = s.find("ShengBTE") != std::string::npos && s.find("ShengBTE") > 0;
+                std::string path;
+                g >> path;
+                BandColumns bc;
+                std::string err = sheng ? bin_shengbte(join(dir, path), B.nband, bc)
+                                        : bin_alamode(join(dir, path), s.length(), B.nband, bc);
+                if (!err.empty()) return err;
+                for (int b = 0; b < B.nband; ++b) {
+                    size_t k = (size_t)m * B.nband + b;
+                    B.v[k] = bc.v[b]; B.tau[k] = bc.tau[b]; B.C[k] = bc.C[b]; B.lr[k] = bc.lr[b]; B.hr[k] = bc.hr[b];
+                }
             } else {
                 for (int b = 0; b < B.nband; ++b) {
                     std::getline(g, s);

@@ -62,6 +62,10 @@ class DeckSpec:
     tref: float = 300.0
     uniform_heat: float = 0.0
     extra_geometry: dict = field(default_factory=dict)
+    # (kind, writer): band data binned by the reader instead of the DATABASE
+    # rows; kind "ShengBTE" or "ALAMODE", writer(input_dir) writes the files
+    # and returns the path named in PHONON (relative to the deck root)
+    band_source: tuple | None = None
 
 
 # ----------------------------------------------------------------- meshes
@@ -305,9 +309,13 @@ def write_deck(root: str, spec: DeckSpec, mesh_format: str = "COMSOL") -> str:
                 f"input/mesh.{'msh' if mesh_format == 'MSH' else 'mphtxt'}\n")
         if spec.uniform_heat:
             f.write(f"UniformHeat=\n{_f(spec.uniform_heat)}\n")
+    material = "DATABASE\ninput/bands.dat"
+    if spec.band_source is not None:
+        kind, writer = spec.band_source
+        material = f"{kind}\n{writer(inp)}"
     with open(os.path.join(inp, "PHONON"), "w") as f:
         f.write(f"MaterialNumber=\n1\nBandNumber=\n{len(bd.v)}\n\n"
-                f"MaterialFile 1= DATABASE\ninput/bands.dat\n\n\n"
+                f"MaterialFile 1= {material}\n\n\n"
                 f"GeometryMaterialType=\n1 1\n\n"
                 f"MaterialDimension=\n{spec.material_dim}\nAngle_method=\n{spec.angle_method}\n"
                 f"Ntheta=\n{spec.ntheta}\nNphi=\n{spec.nphi}\n")
@@ -337,6 +345,80 @@ def porous_spec(n_target=50000, nbands=40, nphi=16, pore_radius=0.3, seed=1, sca
                 bcs=bcs, scale=scale, material_dim=2, angle_method=1, nphi=nphi, tol_temp=1e-8)
     base.update(kw)
     return DeckSpec(**base)
+
+
+def _fortran_row(vals) -> str:
+    """One data row as ShengBTE writes them: right-aligned E-format fields,
+    no trailing blanks (the reader's token scan needs that)."""
+    return "".join(("%20.10E" % v) if np.isfinite(v) else "%20s" % "NaN" for v in vals)
+
+
+def shengbte_writer(seed: int, nirr: int = 6, nbranch: int = 6, nfull: int = 16):
+    """Seeded ShengBTE-format mode data (CONTROL, BTE.omega,
+    BTE.qpoints_full, BTE.w_final, BTE.v) for the band binning of
+    src/BTEBand/BTEBand.cpp:81-628: nirr irreducible q-points with nbranch
+    branches, expanded to nfull q-points.  The first q-point is Gamma (zero
+    acoustic frequencies and NaN scattering rates, as ShengBTE writes them)."""
+    def write(inp):
+        rng = np.random.default_rng(seed)
+        d = os.path.join(inp, "shengbte")
+        os.makedirs(d, exist_ok=True)
+        with open(os.path.join(d, "CONTROL"), "w") as f:
+            f.write("&allocations\n        nelements=1,\n        natoms=2,\n        ngrid(:)=4 2 2\n&end\n"
+                    "&crystal\n        lfactor=0.0529177210818,\n"
+                    "        lattvec(:,1)= -5.372296320673052 0.000000000000000 5.372296320673052\n"
+                    "        lattvec(:,2)=  0.000000000000000 5.372296320673052 5.372296320673052\n"
+                    "        lattvec(:,3)= -5.372296320673052 5.372296320673052 0.000000000000000\n"
+                    "        elements=\"Ge\"\n&end\n&parameters\n        T=300\n        scalebroad=1\n&end\n"
+                    "&flags\n        convergence = .true.\n        isotopes=.true.\n&end\n")
+        om = np.sort(rng.uniform(2.0, 55.0, (nirr, nbranch)), axis=1)
+        om[0, :3] = 0.0
+        rate = rng.uniform(1e-3, 2.0, (nbranch, nirr))
+        vel = rng.uniform(-5.0, 5.0, (nbranch, nirr, 3))
+        with open(os.path.join(d, "BTE.omega"), "w") as f:
+            for q in range(nirr):
+                f.write(_fortran_row(om[q]) + "\n")
+        irr = np.concatenate([np.arange(1, nirr + 1), rng.integers(1, nirr + 1, nfull - nirr)])
+        with open(os.path.join(d, "BTE.qpoints_full"), "w") as f:
+            for i, q in enumerate(irr):
+                f.write("%9d %9d" % (i + 1, q) + _fortran_row(rng.uniform(0, 0.5, 3)) + "\n")
+        with open(os.path.join(d, "BTE.w_final"), "w") as f:
+            for b in range(nbranch):
+                for q in range(nirr):
+                    f.write(_fortran_row([om[q, b], np.nan if om[q, b] == 0 else rate[b, q]]) + "\n")
+        with open(os.path.join(d, "BTE.v"), "w") as f:
+            for b in range(nbranch):
+                for q in range(nirr):
+                    f.write(_fortran_row(vel[b, q] * (0.0 if om[q, b] == 0 else 1.0)) + "\n")
+        return "input/shengbte"
+    return write
+
+
+def alamode_writer(seed: int, nk: int = 12, nmode: int = 6):
+    """Seeded ALAMODE lifetime file (the analyzer's "Phonon lifetime" table:
+    ik, is, frequency, lifetime [ps], |v|, MFP, multiplicity and the 9
+    conductivity tensor components) for src/BTEBand/BTEBand.cpp:629-740.
+    The first k-point has zero lifetimes and velocities (Gamma)."""
+    def write(inp):
+        rng = np.random.default_rng(seed)
+        path = os.path.join(inp, "tau_alamode.dat")
+        with open(path, "w") as f:
+            f.write("# Result analyzer ver. 1.0.5\n# Input file : synthetic.result\n"
+                    "# Phonon lifetime at temperature 300 K.\n"
+                    f"# kpoint range 1 {nk}\n# mode   range 1 {nmode}\n"
+                    "#  ik,  is, Frequency [cm^{-1}], Lifetime [ps], |Velocity| [m/s], MFP [nm], Multiplicity, "
+                    "Thermal conductivity par mode (xx, xy, ...) [W/mK]\n")
+            for k in range(nk):
+                for m in range(nmode):
+                    freq = rng.uniform(5.0, 500.0)
+                    tau = 0.0 if k == 0 else rng.uniform(1.0, 5000.0)
+                    v = 0.0 if k == 0 else rng.uniform(100.0, 9000.0)
+                    mult = int(rng.integers(1, 9))
+                    kap = rng.uniform(0, 0.3, 9) if k else np.zeros(9)
+                    f.write("%5d%5d" % (k + 1, m + 1) + "".join("%15.6g" % x for x in (freq, tau, v, tau * v * 1e-3))
+                            + "%5d" % mult + "".join("%15.6g" % x for x in kap) + "\n")
+        return "input/tau_alamode.dat"
+    return write
 
 
 def read_band_table(path: str) -> Bands:

@@ -36,6 +36,7 @@ TABLES = ["elementFaceSize", "elementFaceNeighbor", "elementFaceBound", "element
           "boundaryFace", "boundaryType", "boundaryThermal", "boundaryConnect", "faceCell", "faceFace", "faceBound",
           "kappaBulk", "capacityBulk", "directionX", "directionY", "directionZ", "modeWeight", "relaxationTime",
           "heatCapacity", "latticeRatio", "heatRatio", "sizes", "scale", "cellMatrix"]
+OPTIONAL = ["groupVelocity"]  # written into fixtures made after it was added
 FIELDS = ["fourier_T", "fourier_qx", "fourier_qy", "T_lattice", "T_energy", "qx", "qy", "T_band", "final_errors"]
 STAGES = ["stage_system", "stage_ebound_init", "stage_gradX", "stage_gradY", "stage_e_new", "stage_ebound_sys",
           "stage_e_vertex", "stage_gradFaceX", "stage_gradFaceY", "stage_ReMacro", "stage_T_sys", "stage_E_sys",
@@ -56,6 +57,10 @@ def golden_decks():
     def limited(factory):
         return lambda: dataclasses.replace(factory(), order="2limit")
 
+    def banded(kind, writer):
+        return lambda: dataclasses.replace(decks.gray_square_spec(6), bands=decks.debye_bands(4, 1),
+                                           band_source=(kind, writer))
+
     return {
         # name: (deck factory, mesh format, max_iter, tolT, tolQ, stage system)
         "gray40": (decklib.DECKS["gray40"][0], "COMSOL", 500, 1e-8, 1e-4, (0, 3)),
@@ -75,6 +80,9 @@ def golden_decks():
         "porous_b4_2limit": (limited(decklib.DECKS["porous_b4"][0]), "COMSOL", 25, 0.0, 0.0, (1, 2)),
         "square_b20_2limit_seq": (limited(sequential(decklib.DECKS["square_b20_diffuse"][0])), "COMSOL", 5, 0.0, 0.0,
                                   (3, 5)),
+        # band data binned from ShengBTE / ALAMODE mode files (seeded, small)
+        "sheng_gray6": (banded("ShengBTE", decks.shengbte_writer(3)), "COMSOL", 5, 0.0, 0.0, (2, 3)),
+        "alamode_gray6": (banded("ALAMODE", decks.alamode_writer(4)), "COMSOL", 5, 0.0, 0.0, (1, 5)),
     }
 
 
@@ -87,7 +95,7 @@ def run_one(name, factory, fmt, max_iter, tolT, tolQ, stage):
         log = subprocess.run([REF_DUMP, deck, out, str(max_iter), repr(tolT), repr(tolQ), str(stage[0]),
                               str(stage[1])], cwd=deck, capture_output=True, text=True, check=True).stdout
         data = {}
-        for k in TABLES + FIELDS + STAGES:
+        for k in TABLES + FIELDS + STAGES + OPTIONAL:
             data[k] = np.load(os.path.join(out, k + ".npy"))
         data["count_iterations"] = np.int64(len(re.findall(r"Iteration #", log)))
         data["count_macro_passes"] = np.int64(len(re.findall(r"internal residual", log)))

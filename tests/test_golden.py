@@ -70,6 +70,18 @@ def test_tables_bitwise(loaded, name):
         assert np.array_equal(a, b), ours
 
 
+@pytest.mark.parametrize("name", [n for n in NAMES if "sheng" in n or "alamode" in n])
+def test_binned_bands_match_reference(loaded, name):
+    """ShengBTE / ALAMODE binning (BTEBand.cpp:81-740): the band columns are
+    bit-identical (checked with the other tables); the group velocity, which
+    the dump recovers from the stored v*s products, to rounding."""
+    p, g = loaded[name]
+    assert np.allclose(p.array("vg"), g["groupVelocity"].reshape(p.array("vg").shape), rtol=1e-14, atol=0)
+    for ours, ref in (("tau", "relaxationTime"), ("cap", "heatCapacity"), ("lr", "latticeRatio"),
+                      ("hr", "heatRatio")):
+        assert np.array_equal(p.array(ours), g[ref].reshape(p.array(ours).shape)), ours
+
+
 @pytest.mark.parametrize("name", NAMES)
 def test_oracle_run_matches_reference(loaded, name):
     p, g = loaded[name]
