@@ -59,6 +59,14 @@ class Problem:
     def handle(self):
         return self._h
 
+    def write_fields(self, out_dir: str, T_lattice, T_energy, qx, qy, qz, T_band):
+        """The reference's output files from given fields (deck order), no
+        device needed."""
+        arrs = [np.ascontiguousarray(x, np.float64).ravel() for x in (T_lattice, T_energy, qx, qy, qz, T_band)]
+        rc = self.lib.gbte_write_fields(self.handle, *[_lib.dptr(x) for x in arrs], out_dir.encode())
+        if rc != 0:
+            raise GBTEError(rc, "cannot write the output files")
+
     def array(self, name: str) -> np.ndarray:
         return self.tables.array(name)
 
@@ -138,6 +146,11 @@ class BTESolver:
         v = C.c_double()
         self._check(self.lib.gbte_keff(self._h, axis, delta_t, C.byref(v)))
         return v.value
+
+    def write_outputs(self, out_dir: str = "."):
+        """The reference's output files (TempLattice.dat, Tempband.dat,
+        Temperature.dat, HeatFlux.dat, Result.vtk) in out_dir."""
+        self._check(self.lib.gbte_write_outputs(self._h, self.problem.handle, out_dir.encode()))
 
     def close(self):
         if getattr(self, "_h", None):

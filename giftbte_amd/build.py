@@ -46,7 +46,7 @@ def build_lib() -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    srcs = ["problem.cpp", "bands.cpp", "reorder.cpp", "ndchol.cpp", "tiles.cpp", "capi_problem.cpp", "capi_solver.cpp",
+    srcs = ["problem.cpp", "bands.cpp", "outputs.cpp", "reorder.cpp", "ndchol.cpp", "tiles.cpp", "capi_problem.cpp", "capi_solver.cpp",
             "solver.hip"]
     for s in srcs:
         src = os.path.join(CSRC, s)

@@ -2,6 +2,7 @@
 #include <cstdio>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "solver.hpp"
 
@@ -82,6 +83,18 @@ int gbte_get_fields(gbte_handle *h, double *T_lattice, double *T_energy, double 
                     double *T_band) {
     if (!h || !h->s || !h->s->ok()) return GBTE_HIP_ERROR;
     return finish(h, h->s->get_fields(T_lattice, T_energy, qx, qy, qz, T_band));
+}
+
+int gbte_write_outputs(gbte_handle *h, const gbte_problem *p, const char *dir) {
+    if (!h || !h->s || !h->s->ok()) return GBTE_HIP_ERROR;
+    if (!p) return GBTE_BAD_INPUT;
+    size_t nc = p->p.n_cells, nB = p->p.n_bands;
+    std::vector<double> T(nc), E(nc), qx(nc), qy(nc), qz(nc), Tb(nB * nc);
+    int rc = finish(h, h->s->get_fields(T.data(), E.data(), qx.data(), qy.data(), qz.data(), Tb.data()));
+    if (rc != GBTE_OK) return rc;
+    rc = gbte_write_fields(p, T.data(), E.data(), qx.data(), qy.data(), qz.data(), Tb.data(), dir);
+    if (rc != GBTE_OK) h->err = "cannot write the output files";
+    return rc;
 }
 
 int gbte_keff(gbte_handle *h, int axis, double delta_T, double *keff) {

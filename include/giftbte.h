@@ -179,6 +179,16 @@ int gbte_solve(gbte_handle *h, int max_iter, double tol_temp, double tol_flux, g
  * temperatures.  StaticBTESynthetic.cpp:2613-2653. */
 int gbte_get_fields(gbte_handle *h, double *T_lattice, double *T_energy, double *qx, double *qy, double *qz,
                     double *T_band);
+/* The reference's output files in directory dir (NULL: the working
+ * directory): TempLattice.dat, Tempband.dat, Temperature.dat, HeatFlux.dat
+ * and Result.vtk, as StaticBTESynthetic::_print_out
+ * (StaticBTESynthetic.cpp:2613-2653) and SolutionAll::_print_out
+ * (Solution.cpp:128-233) leave them after a steady run, from the handle's
+ * fields.  gbte_write_fields writes the same files from caller fields in deck
+ * order (n_cells each, T_band n_bands * n_cells) and needs no device. */
+int gbte_write_outputs(gbte_handle *h, const gbte_problem *p, const char *dir);
+int gbte_write_fields(const gbte_problem *p, const double *T_lattice, const double *T_energy, const double *qx,
+                      const double *qy, const double *qz, const double *T_band, const char *dir);
 /* Effective conductivity sum_i q_axis,i V_i / (A_perp dT), with A_perp the
  * domain extent normal to the axis (L_y for axis 0 in 2D).  No reference
  * function; the MATLAB check bin/examples/cross-plane/verify_kappa_crossplane.m:8

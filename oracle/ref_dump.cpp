@@ -409,6 +409,16 @@ int main(int argc, char **argv) {
     dump_d("T_old", solver.temperatureOld, {nc});
     double errs[2] = {solver.error, solver.error_heat};
     dump_d("final_errors", errs, {2});
+    // the hand-back and the final outputs of main (Solution.cpp:408-413,
+    // BTEMain.cpp:393): Temperature.dat, HeatFlux.dat and Result.vtk; the
+    // solver wrote TempLattice.dat and Tempband.dat itself
+    for (size_t i = 0; i < dm->volumeElementsBTE.size(); ++i) {
+        sa.solution.Temperature[dm->elementProjection[i]] = solver.totalEnergy[i];
+        sa.solution.heatFluxX[dm->elementProjection[i]] = solver.heatFluxXGlobal[i];
+        sa.solution.heatFluxY[dm->elementProjection[i]] = solver.heatFluxYGlobal[i];
+        sa.solution.heatFluxZ[dm->elementProjection[i]] = solver.heatFluxZGlobal[i];
+    }
+    sa._print_out(dm);
     MPI_Finalize();
     return 0;
 }

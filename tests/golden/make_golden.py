@@ -36,7 +36,10 @@ TABLES = ["elementFaceSize", "elementFaceNeighbor", "elementFaceBound", "element
           "boundaryFace", "boundaryType", "boundaryThermal", "boundaryConnect", "faceCell", "faceFace", "faceBound",
           "kappaBulk", "capacityBulk", "directionX", "directionY", "directionZ", "modeWeight", "relaxationTime",
           "heatCapacity", "latticeRatio", "heatRatio", "sizes", "scale", "cellMatrix"]
-OPTIONAL = ["groupVelocity"]  # written into fixtures made after it was added
+OPTIONAL = ["groupVelocity", "qz"]  # written into fixtures made after they were added
+# the reference's output files after the run (text), for the decks listed
+OUTPUT_FILES = ["TempLattice.dat", "Tempband.dat", "Temperature.dat", "HeatFlux.dat", "Result.vtk"]
+OUTPUT_DECKS = ["msh_gray6", "quad_cross8", "sheng_gray6"]
 FIELDS = ["fourier_T", "fourier_qx", "fourier_qy", "T_lattice", "T_energy", "qx", "qy", "T_band", "final_errors"]
 STAGES = ["stage_system", "stage_ebound_init", "stage_gradX", "stage_gradY", "stage_e_new", "stage_ebound_sys",
           "stage_e_vertex", "stage_gradFaceX", "stage_gradFaceY", "stage_ReMacro", "stage_T_sys", "stage_E_sys",
@@ -97,6 +100,10 @@ def run_one(name, factory, fmt, max_iter, tolT, tolQ, stage):
         data = {}
         for k in TABLES + FIELDS + STAGES + OPTIONAL:
             data[k] = np.load(os.path.join(out, k + ".npy"))
+        if name in OUTPUT_DECKS:
+            for fn in OUTPUT_FILES:
+                with open(os.path.join(deck, fn)) as f:
+                    data["out_" + fn] = np.array(f.read())
         data["count_iterations"] = np.int64(len(re.findall(r"Iteration #", log)))
         data["count_macro_passes"] = np.int64(len(re.findall(r"internal residual", log)))
         data["count_fourier_passes"] = np.int64(len(re.findall(r"iterations residual", log)))

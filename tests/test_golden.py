@@ -82,6 +82,18 @@ def test_binned_bands_match_reference(loaded, name):
         assert np.array_equal(p.array(ours), g[ref].reshape(p.array(ours).shape)), ours
 
 
+@pytest.mark.parametrize("name", make_golden.OUTPUT_DECKS)
+def test_output_files_match_reference(loaded, name, tmp_path):
+    """The output writer (gbte_write_fields) fed the reference's own final
+    fields writes the reference's files (StaticBTESynthetic.cpp:2613-2653,
+    Solution.cpp:128-233) character for character."""
+    p, g = loaded[name]
+    p.write_fields(str(tmp_path), g["T_lattice"], g["T_energy"], g["qx"], g["qy"], g["qz"], g["T_band"])
+    for fn in make_golden.OUTPUT_FILES:
+        with open(tmp_path / fn) as f:
+            assert f.read() == str(g["out_" + fn]), fn
+
+
 @pytest.mark.parametrize("name", NAMES)
 def test_oracle_run_matches_reference(loaded, name):
     p, g = loaded[name]
