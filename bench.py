@@ -15,11 +15,20 @@ recovery, synthetic solve and reductions) are under "mixed".  The time to a
 converged k_eff (the deck's tolerances, from the Fourier warm start) is
 reported for both, with the mixed k_eff's error against FP64.
 
+The CPU baseline is the reference itself (oracle/_ref/BTE_CPU, compiled from
+its sources) run with MPI ranks on this host on a bounded sample of the
+workload, or, where that binary is absent, the oracle's single-core transport
+loop (also reported beside it as cpu_port).
+
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-converge] [--no-cpu]
 """
 import argparse
+import dataclasses
 import json
 import os
+import re
+import shutil
+import subprocess
 import sys
 import tempfile
 import time
@@ -62,6 +71,48 @@ def cpu_baseline(problem, T0, target_s=15.0):
                       f"IDW, face gradients, non-Fourier residual, moments) over {n} of "
                       f"{t.n_bands * t.n_dirs} band x direction systems of one outer iteration, same mesh; "
                       f"{sec:.1f} s"}
+
+
+def reference_baseline(td, n_target, nbands, seed, nphi=4, iters=2, timeout=900):
+    """The compiled reference (oracle/_ref/BTE_CPU, built from the reference
+    sources by oracle/ref.mk) run with MPI ranks on this host's cores, on a
+    bounded sample of the workload: the same mesh and bands, 4 * nphi of the
+    directions.  Rate = updates of one outer iteration / the reference's own
+    per-iteration "inner loop" timer (StaticBTESyntheticSolver.cpp:77-190),
+    which leaves out its per-iteration file output.  The rank count is the
+    largest divisor of the direction count within 16 cores (its per-system
+    collectives need the divisor)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "BTE_CPU")
+    mpi = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpi)):
+        return None
+    from giftbte_amd import decks
+    nd = 4 * nphi
+    cores = min(16, len(os.sched_getaffinity(0)))
+    ranks = max(r for r in range(1, nd + 1) if nd % r == 0 and r <= cores)
+    spec = decks.porous_spec(n_target=n_target, nbands=nbands, nphi=nphi, pore_radius=0.3, seed=seed, scale=5e-7)
+    spec = dataclasses.replace(spec, max_iter=iters, tol_temp=0.0, tol_flux=0.0)
+    deck = decks.write_deck(os.path.join(td, "refdeck"), spec)
+    t0 = time.time()
+    try:
+        r = subprocess.run(["timeout", str(timeout), mpi, "-np", str(ranks), exe], cwd=deck, capture_output=True,
+                           text=True)
+    except OSError:
+        return None
+    wall = time.time() - t0
+    ms = [float(x) for x in re.findall(r"Time taken by inner loop: ([0-9.eE+-]+) milliseconds", r.stdout)]
+    if r.returncode != 0 or not ms:
+        print(f"[bench] reference baseline failed (rc {r.returncode})", file=sys.stderr, flush=True)
+        return None
+    from giftbte_amd import Problem
+    t = Problem(deck).tables
+    upd = float(t.n_cells) * t.n_bands * t.n_dirs
+    sec = sum(ms) / len(ms) / 1e3
+    return {"value": upd / sec, "unit": "cell*band*dir updates/s", "cores": ranks, "kind": "reference",
+            "sample": f"the reference's BTE_CPU (compiled from its sources), mpiexec -np {ranks}, on the same "
+                      f"{t.n_cells}-cell mesh and {t.n_bands} bands with {t.n_dirs} of the directions "
+                      f"({t.n_bands * t.n_dirs} systems); mean of its per-iteration inner-loop timer over {len(ms)} "
+                      f"outer iterations, {sec * 1e3:.0f} ms; {wall:.0f} s wall with set-up"}
 
 
 def kernel_bytes_per_update(dim, word=8):
@@ -198,7 +249,11 @@ def main():
             mixed["speedup_time_to_converged_vs_fp64"] = f64["time_to_converged_s"] / f32["time_to_converged_s"]
         out["mixed"] = mixed
         if not a.no_cpu:
-            out["cpu_baseline"] = cpu_baseline(p, T0)
+            port = cpu_baseline(p, T0)
+            ref = reference_baseline(td, a.cells, a.bands, a.seed)
+            out["cpu_baseline"] = ref if ref is not None else port
+            if ref is not None:
+                out["cpu_port"] = port
         print(json.dumps(out))
 
 
