@@ -164,6 +164,33 @@ class BTESolver:
             pass
 
 
+def solve_batch(solvers, max_iter, tol_temp, tol_flux):
+    """BTESolver.solve on several independent solvers at once (gbte_solve_batch:
+    one host thread per solver, the device runs them side by side).
+    max_iter / tol_temp / tol_flux: one value or one per solver.  Returns the
+    SolveStats list; raises GBTEError for the first failing solver."""
+    n = len(solvers)
+    if n == 0:
+        return []
+    lib = solvers[0].lib
+
+    def per(x, typ):
+        v = list(x) if hasattr(x, "__len__") else [x] * n
+        return (typ * n)(*v)
+
+    hs = (C.c_void_p * n)(*[s._h for s in solvers])
+    st = (_lib.Stats * n)()
+    status = (C.c_int * n)()
+    rc = lib.gbte_solve_batch(hs, n, per(max_iter, C.c_int), per(tol_temp, C.c_double), per(tol_flux, C.c_double),
+                              st, status)
+    if rc != 0:
+        bad = next(i for i in range(n) if status[i] != 0)
+        raise GBTEError(status[bad], solvers[bad].lib.gbte_last_error(solvers[bad]._h).decode())
+    return [SolveStats(x.iterations, x.macro_passes, x.fourier_passes, bool(x.converged), x.err_temp, x.err_flux,
+                       x.solve_ms, x.sweep_ms, x.transport_ms, x.macro_ms, x.sweep_levels, x.macro_cg_iters,
+                       x.source_ms, x.face_ms) for x in st]
+
+
 def run_deck(deck_dir: str, max_iter: int | None = None, tol_temp: float | None = None,
              tol_flux: float | None = None, precision: str = "fp64", verbose: bool = False):
     """The reference's steady flow on one deck: Fourier warm start, then the

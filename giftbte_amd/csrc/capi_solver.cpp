@@ -2,6 +2,7 @@
 #include <cstdio>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "solver.hpp"
@@ -77,6 +78,28 @@ int gbte_solve(gbte_handle *h, int max_iter, double tol_temp, double tol_flux, g
         return GBTE_BAD_INPUT;
     }
     return finish(h, h->s->solve(max_iter, tol_temp, tol_flux, stats));
+}
+
+int gbte_solve_batch(gbte_handle *const *hs, int n, const int *max_iter, const double *tol_temp,
+                     const double *tol_flux, gbte_stats *stats, int *status) {
+    if (!hs || n <= 0 || !max_iter || !tol_temp || !tol_flux) return GBTE_BAD_INPUT;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j)
+            if (hs[i] == hs[j]) return GBTE_BAD_INPUT;  // one solve per handle at a time
+    std::vector<int> rc(n, GBTE_OK);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int i = 0; i < n; ++i)
+        th.emplace_back([&, i]() {
+            rc[i] = gbte_solve(hs[i], max_iter[i], tol_temp[i], tol_flux[i], stats ? &stats[i] : nullptr);
+        });
+    for (auto &t : th) t.join();
+    int first = GBTE_OK;
+    for (int i = 0; i < n; ++i) {
+        if (status) status[i] = rc[i];
+        if (first == GBTE_OK && rc[i] != GBTE_OK) first = rc[i];
+    }
+    return first;
 }
 
 int gbte_get_fields(gbte_handle *h, double *T_lattice, double *T_energy, double *qx, double *qy, double *qz,

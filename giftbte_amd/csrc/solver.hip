@@ -2616,6 +2616,7 @@ bool Solver::cg(const DevCSR &K, double *d_b, double *d_x, int *iters) {
 
 int Solver::fourier(gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
+    if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;  // the calling thread's device
     // StaticFourier::_solve (StaticFourier.cpp:934-976) from T = initial
     double err = 1000;
     int passes = 0;
@@ -2647,6 +2648,7 @@ int Solver::fourier(gbte_stats *st) {
 
 int Solver::set_initial(const double *T, const double *qx, const double *qy, const double *qz) {
     if (!ok()) return GBTE_HIP_ERROR;
+    if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;  // the calling thread's device
     const double *src[4] = {T, qx, qy, qz};
     double *dst[4] = {d_T.p, d_qx.p, d_qy.p, d_qz.p};
     std::vector<double> tmp(nc);
@@ -2678,6 +2680,7 @@ static ST *rp(DBuf<double> &b) {
 
 int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
+    if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;  // the calling thread's device
     auto t0 = std::chrono::high_resolution_clock::now();
     bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1",
          limiter = P.deck.order == "2limit";
@@ -2938,6 +2941,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
 
 int Solver::get_fields(double *TL, double *TE, double *qx, double *qy, double *qz, double *Tb) {
     if (!ok()) return GBTE_HIP_ERROR;
+    if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;  // the calling thread's device
     struct {
         double *h;
         const double *d;
@@ -2958,6 +2962,7 @@ int Solver::get_fields(double *TL, double *TE, double *qx, double *qy, double *q
 
 int Solver::keff(int axis, double dT, double *out) {
     if (!ok()) return GBTE_HIP_ERROR;
+    if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;  // the calling thread's device
     if (axis < 0 || axis >= dim || dT == 0) { err_ = "keff: bad axis or zero temperature difference"; return GBTE_BAD_INPUT; }
     std::vector<double> q(nc);
     const double *src = axis == 0 ? d_qx.p : (axis == 1 ? d_qy.p : d_qz.p);

@@ -173,6 +173,14 @@ int gbte_set_initial(gbte_handle *h, const double *T, const double *qx, const do
  * scheme (:968-1178: the same transport iteration without the synthetic
  * correction, T the recovered lattice temperature). */
 int gbte_solve(gbte_handle *h, int max_iter, double tol_temp, double tol_flux, gbte_stats *stats);
+/* Many independent problems at once: gbte_solve on each of the n distinct
+ * handles, concurrently (one host thread per handle; each handle keeps its
+ * own stream, graphs and buffers, so the device runs the solves side by
+ * side).  Per-handle arguments, stats and status (any may be NULL except the
+ * arguments); returns the first failing status or GBTE_OK.  No reference
+ * counterpart: the reference solves one deck per process. */
+int gbte_solve_batch(gbte_handle *const *hs, int n, const int *max_iter, const double *tol_temp,
+                     const double *tol_flux, gbte_stats *stats, int *status);
 /* Output fields (caller-owned, n_cells doubles each; T_band n_bands*n_cells;
  * NULL skips a field): macro (lattice) temperature, energy-equivalent
  * temperature (what Solution.cpp:408-413 copies back), heat flux, band

@@ -116,3 +116,14 @@ def test_scheme_in_tables_and_validated(tmp_path):
             Problem(decks.write_deck(str(tmp_path / f"bad_{field}"), dataclasses.replace(base, **{field: bad})))
         assert e.value.code == _lib.GBTE_BAD_INPUT
 
+
+
+def test_batch_argument_errors():
+    """gbte_solve_batch refuses empty batches and repeated handles before any
+    device work."""
+    lib = _lib.load()
+    one = (C.c_int * 2)(1, 1)
+    tol = (C.c_double * 2)(0.0, 0.0)
+    hs = (C.c_void_p * 2)(None, None)
+    assert lib.gbte_solve_batch(hs, 0, one, tol, tol, None, None) == _lib.GBTE_BAD_INPUT
+    assert lib.gbte_solve_batch(hs, 2, one, tol, tol, None, None) == _lib.GBTE_BAD_INPUT
