@@ -87,6 +87,17 @@ int gbte_solve_batch(gbte_handle *const *hs, int n, const int *max_iter, const d
         for (int j = 0; j < i; ++j)
             if (hs[i] == hs[j]) return GBTE_BAD_INPUT;  // one solve per handle at a time
     std::vector<int> rc(n, GBTE_OK);
+    // graphs are recorded here, one handle after another: a capture must not
+    // overlap another thread's allocations and synchronous copies
+    for (int i = 0; i < n; ++i) {
+        if (!hs[i] || !hs[i]->s || !hs[i]->s->ok()) return GBTE_HIP_ERROR;
+        int r = hs[i]->s->prepare();
+        if (r != GBTE_OK) {
+            hs[i]->err = hs[i]->s->error();
+            if (status) status[i] = r;
+            return r;
+        }
+    }
     std::vector<std::thread> th;
     th.reserve(n);
     for (int i = 0; i < n; ++i)

@@ -1642,7 +1642,12 @@ Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_ne
         return;
     }
     if (!check(hipSetDevice(opt.device), "hipSetDevice")) return;
-    if (!check(hipStreamCreate(&s_), "hipStreamCreate")) return;
+    // non-blocking: no implicit ordering with the legacy null stream, so a
+    // synchronous copy in another thread (another handle's set-up) cannot
+    // break this stream's graph capture; every public call ends with a sync
+    // of this stream, so the synchronous host copies at call entry and exit
+    // never overlap its work.  Default priority.
+    if (!check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return;
     for (hipEvent_t &ev : ev_)
         if (!check(hipEventCreate(&ev), "hipEventCreate")) return;
     if (!check(hipHostMalloc((void **)&h_pin, 64 * sizeof(double)), "hipHostMalloc")) return;
@@ -2521,6 +2526,11 @@ bool Solver::inner_error(const double *a, const double *b, double *out) {
 // a sync; graph q leaves the pass's squared-change partials in h_part + q
 // red_blocks once the stream has reached it (macro_inner reads them).
 bool Solver::macro_launch(int q) {
+    return macro_capture(q) && check(hipGraphLaunch(macro_exec_[q], s_), "hipGraphLaunch");
+}
+
+// Record synthetic pass q as a graph (once per handle).  Nothing runs.
+bool Solver::macro_capture(int q) {
     if (!macro_exec_[q]) {
         if (!check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
         hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
@@ -2538,7 +2548,15 @@ bool Solver::macro_launch(int q) {
         macro_graph_[q] = g;
         if (!check(hipGraphInstantiate(&macro_exec_[q], g, nullptr, nullptr, 0), "hipGraphInstantiate")) return false;
     }
-    return check(hipGraphLaunch(macro_exec_[q], s_), "hipGraphLaunch");
+    return true;
+}
+
+int Solver::prepare() {
+    if (!ok()) return GBTE_HIP_ERROR;
+    if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;
+    if (opt.linear_solver == GBTE_LINSOLVE_DIRECT && P.deck.method != "Sequential")
+        if (!macro_capture(0) || !macro_capture(1)) return GBTE_HIP_ERROR;
+    return GBTE_OK;
 }
 
 // mean squared change of the pass run by graph q (after a sync), summed like

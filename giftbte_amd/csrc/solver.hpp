@@ -73,6 +73,10 @@ class Solver {
     int get_fields(double *TL, double *TE, double *qx, double *qy, double *qz, double *Tb);
     int max_levels() const { return nlev; }
     int keff(int axis, double dT, double *out);
+    // record the synthetic-pass graphs now (the first solve does it
+    // otherwise); batched solves do this before their threads start, as a
+    // capture must not overlap another thread's allocations and copies
+    int prepare();
 
     void set_error(const std::string &e) { err_ = e; }
 
@@ -94,6 +98,7 @@ class Solver {
     void launch_rhs(bool macro, const double *bval, double *rhs);
     bool inner_error(const double *a, const double *b, double *out);
     bool macro_launch(int q);
+    bool macro_capture(int q);
     double macro_inner(int q) const;
 
     std::vector<int> cell_new_, node_new_;  // deck order -> device order
