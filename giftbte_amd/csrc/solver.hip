@@ -1007,6 +1007,7 @@ constexpr int kTileMomentLds = 2 * kYRows * kTileJ * 2;  // doubles of the momen
 static_assert(kTileMomentLds <= kTileRows * kTileStride, "moment partials fit in the FP64 row buffer");
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 // k_tile_face work: the moments (T, q, residual) and / or the LSQ gradient
 constexpr int kTileMoments = 1, kTileGradient = 2;
@@ -1018,8 +1019,11 @@ struct TileDev {
 
 __device__ __forceinline__ int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// workgroups per CU: five in FP64 (30.5 KiB of LDS, 96 VGPRs); FP32 storage
+// stages float rows (16 KiB) and runs the LSQ product in float, which fits
+// seven (72 VGPRs)
 template <typename ST>
-__global__ void __launch_bounds__(kTileBlock, 5) k_tile_face(const ST *__restrict__ e, const ST *__restrict__ eb,
+__global__ void __launch_bounds__(kTileBlock, sizeof(ST) == 4 ? 7 : 5) k_tile_face(const ST *__restrict__ e, const ST *__restrict__ eb,
                                                           ST *__restrict__ grad, TileDev T,
                                                           double *__restrict__ To, double *__restrict__ qxo,
                                                           double *__restrict__ qyo, double *__restrict__ qzo,
@@ -1051,8 +1055,18 @@ __global__ void __launch_bounds__(kTileBlock, 5) k_tile_face(const ST *__restric
     for (int r = nr + w; r < kTileRows; r += kTileWaves) rows[r * kTileStride + lane] = 0;
     // LSQ coefficients (A operands), the same for every slice; in the storage
     // type, as the gradients they make are stored in it
+    // FP32 storage runs the product on v_mfma_f32_16x16x4f32, whose lane l
+    // holds D rows 4 (l / 16) + i (element i): the A rows are taken in the
+    // order m -> C row 2 (m / 4) + (m % 4) / 2 + 8 (m % 2), so that a lane
+    // holds (x, y) of cells 2 (l / 16) and 2 (l / 16) + 1
+    constexpr bool kF32Lsq = sizeof(ST) == 4;
+    int arow = lane;
+    if constexpr (kF32Lsq) {
+        int m = lane & 15;
+        arow = (lane & ~15) | (2 * (m >> 2) + ((m & 3) >> 1) + 8 * (m & 1));
+    }
     ST cm[kTileKs];
-    const double *cmt = T.cmf + (size_t)tile * kTileKs * 64 + lane;
+    const double *cmt = T.cmf + (size_t)tile * kTileKs * 64 + arow;
 #pragma unroll
     for (int k = 0; k < kTileKs; ++k) cm[k] = k < nks ? (ST)cmt[k * 64] : (ST)0;
     // B operands: Y reads 16 rows of one column, G 4 rows of 16 columns
@@ -1106,16 +1120,29 @@ __global__ void __launch_bounds__(kTileBlock, 5) k_tile_face(const ST *__restric
             }
         // LSQ gradient of entries [16 w, 16 w + 16) of the slice
         v4d g = {0.0, 0.0, 0.0, 0.0};
+        if constexpr (kF32Lsq) {
+            v4f gf = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int k = 0; k < kTileKs; ++k)
-            if (gradient && k < nks)
-                g = __builtin_amdgcn_mfma_f64_16x16x4f64((double)cm[k], (double)grd[4 * k * kTileStride], g, 0, 0, 0);
-        // lane holds rows lq + 4 i: (cell lq, x), (cell lq + 4, x), (cell lq, y), (cell lq + 4, y)
+            for (int k = 0; k < kTileKs; ++k)
+                if (gradient && k < nks)
+                    gf = __builtin_amdgcn_mfma_f32_16x16x4f32((float)cm[k], (float)grd[4 * k * kTileStride], gf, 0, 0,
+                                                              0);
+            // lane holds (cell 2 lq, x), (2 lq, y), (2 lq + 1, x), (2 lq + 1, y)
+            g = v4d{gf[0], gf[2], gf[1], gf[3]};
+        } else {
+#pragma unroll
+            for (int k = 0; k < kTileKs; ++k)
+                if (gradient && k < nks)
+                    g = __builtin_amdgcn_mfma_f64_16x16x4f64((double)cm[k], (double)grd[4 * k * kTileStride], g, 0, 0,
+                                                             0);
+        }
+        // g[h], g[2 + h] are (x, y) of the lane's cell h: FP64 lq + 4 h (its
+        // rows lq + 4 i), FP32 2 lq + h
         int tg = slice * kTileW + 16 * w + l16;
         if (gradient && tg < DB) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                int cl = lq + 4 * h;
+                int cl = kF32Lsq ? 2 * lq + h : lq + 4 * h;
                 double gx = g[h], gy = g[2 + h];
                 if (isnan(gx) || isnan(gy)) {
                     gx = 0;
