@@ -10,8 +10,8 @@ A "step" is one outer iteration of StaticBTESynthetic::solve: every
 band x direction transport sweep, the moment reductions, the boundary update
 and the FP64 synthetic (macro) solve.  value = cell*band*direction updates/s
 of the plain FP64 solver over the timed steps (inputs resident in HBM); the
-same figures for FP32 storage of the per-system fields (double arithmetic,
-recovery, synthetic solve and reductions) are under "mixed".  The time to a
+same figures for FP32 storage of the per-system fields (float gradients;
+double sweeps, recovery, synthetic solve and reductions) are under "mixed".  The time to a
 converged k_eff (the deck's tolerances, from the Fourier warm start) is
 reported for both, with the mixed k_eff's error against FP64.
 
@@ -295,8 +295,8 @@ def main():
             "config": cfg,
         }
         out.update({k: v for k, v in f64.items() if k not in ("value", "ms_per_step")})
-        mixed = {"dtype": "f32 storage of e, gradients and sources; f64 arithmetic, recovery, synthetic solve "
-                          "and reductions",
+        mixed = {"dtype": "f32 storage of e, gradients and sources, f32 gradient arithmetic; f64 sweeps, "
+                          "recovery, synthetic solve and reductions",
                  "value": f32["value"], "ms_per_step": f32["ms_per_step"],
                  "speedup_updates_vs_fp64": f32["value"] / f64["value"]}
         mixed.update({k: v for k, v in f32.items() if k not in ("value", "ms_per_step")})

@@ -52,9 +52,10 @@ enum { GBTE_MATRIX_LU = 0, GBTE_MATRIX_BICGSTAB = 1 };
 
 /* Storage of the per-(band, direction) fields: energy densities, boundary
  * values, LSQ gradients and sweep sources.  GBTE_FP32_SWEEP stores them in
- * float (half the HBM traffic of the transport kernels) and keeps every
- * arithmetic operation, the temperature / flux recovery, the synthetic
- * solve and the residual and k_eff reductions in double.  The energy
+ * float (half the HBM traffic of the transport kernels).  The LSQ gradient
+ * product runs in float on the matrix cores; every other operation, the
+ * temperature / flux recovery, the synthetic solve and the residual and
+ * k_eff reductions stay in double.  The energy
  * densities are held as deviations from the equilibrium at the deck's Tref,
  * so float keeps about 7 significant digits of the deviation itself.
  * GBTE_FP32_SWEEP needs the tiled face kernel and the wave-specialised sweep

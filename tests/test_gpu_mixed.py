@@ -1,8 +1,9 @@
 """FP32 storage of the per-system fields (GBTE_FP32_SWEEP) vs the FP64 oracle.
 
 The energy densities, boundary values, LSQ gradients and sweep sources are
-stored in float; every operation, the recovery of temperatures and fluxes,
-the synthetic solve and the reductions stay in double.  Bar: converged
+stored in float and the LSQ gradients computed in float; every other
+operation, the recovery of temperatures and fluxes, the synthetic solve and
+the reductions stay in double.  Bar: converged
 k_eff within 1e-6 relative of the FP64 result, temperature and flux fields
 within 1e-6 relative L2 of the oracle, and the same outer-iteration,
 synthetic-pass and Fourier-pass counts as the oracle (a run that stops on
