@@ -63,6 +63,9 @@ class Problem:
         """The reference's output files from given fields (deck order), no
         device needed."""
         arrs = [np.ascontiguousarray(x, np.float64).ravel() for x in (T_lattice, T_energy, qx, qy, qz, T_band)]
+        nc, nb = self.tables.n_cells, self.tables.n_bands
+        if any(a.size != nc for a in arrs[:5]) or arrs[5].size != nb * nc:
+            raise ValueError(f"write_fields needs {nc} values per field and {nb * nc} band temperatures")
         rc = self.lib.gbte_write_fields(self.handle, *[_lib.dptr(x) for x in arrs], out_dir.encode())
         if rc != 0:
             raise GBTEError(rc, "cannot write the output files")

@@ -1,5 +1,6 @@
 // C ABI: solver entry points (declared in include/giftbte.h).
 #include <cstdio>
+#include <exception>
 #include <memory>
 #include <string>
 #include <thread>
@@ -102,7 +103,12 @@ int gbte_solve_batch(gbte_handle *const *hs, int n, const int *max_iter, const d
     th.reserve(n);
     for (int i = 0; i < n; ++i)
         th.emplace_back([&, i]() {
-            rc[i] = gbte_solve(hs[i], max_iter[i], tol_temp[i], tol_flux[i], stats ? &stats[i] : nullptr);
+            try {
+                rc[i] = gbte_solve(hs[i], max_iter[i], tol_temp[i], tol_flux[i], stats ? &stats[i] : nullptr);
+            } catch (const std::exception &e) {  // a thread must not end the process
+                hs[i]->err = std::string("solve failed: ") + e.what();
+                rc[i] = GBTE_HIP_ERROR;
+            }
         });
     for (auto &t : th) t.join();
     int first = GBTE_OK;

@@ -94,6 +94,16 @@ def test_output_files_match_reference(loaded, name, tmp_path):
             assert f.read() == str(g["out_" + fn]), fn
 
 
+def test_output_writer_rejects_short_fields(loaded, tmp_path):
+    """Arrays shorter than the mesh are refused before any pointer reaches C."""
+    name = make_golden.OUTPUT_DECKS[0]
+    p, g = loaded[name]
+    with pytest.raises(ValueError):
+        p.write_fields(str(tmp_path), g["T_lattice"][:-1], g["T_energy"], g["qx"], g["qy"], g["qz"], g["T_band"])
+    with pytest.raises(ValueError):
+        p.write_fields(str(tmp_path), g["T_lattice"], g["T_energy"], g["qx"], g["qy"], g["qz"], g["T_band"][:-1])
+
+
 @pytest.mark.parametrize("name", NAMES)
 def test_oracle_run_matches_reference(loaded, name):
     p, g = loaded[name]
