@@ -270,7 +270,8 @@ constexpr int kMaxFace2D = 4;
 // r_y, r'_x, r'_y (kSrcD per cell), ints [f] = neighbour cell, or -(entry+1)
 // for a boundary face, or the cell itself past its last face (kSrcI).
 constexpr int kSrcD = 32, kSrcI = 4;
-constexpr int kSrcBatch = 8;  // entries per lane whose loads are issued before their stores
+constexpr int kSrcBatch = 2;  // entries per lane whose loads are issued before their stores
+constexpr int kSrcWaves = 4;  // waves per SIMD the register budget allows
 
 // One wave per cell row (kSrcRows rows per workgroup); the row index is
 // wave-uniform, so the cell's face record sits in scalar registers.  The
@@ -279,7 +280,7 @@ constexpr int kSrcBatch = 8;  // entries per lane whose loads are issued before 
 constexpr int kSrcRows = 4;
 
 template <typename ST>
-__global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict__ posc, ST *__restrict__ S,
+__global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source(const int *__restrict__ posc, ST *__restrict__ S,
                                                          const ST *__restrict__ grad, const ST *__restrict__ eb,
                                                          const double *__restrict__ Told, const double *__restrict__ Q,
                                                          const int *__restrict__ fsize, const double *__restrict__ srec,
@@ -317,7 +318,7 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
     double fnx[kMaxFace2D], fny[kMaxFace2D], fav[kMaxFace2D], frx[kMaxFace2D], fry[kMaxFace2D], fqx[kMaxFace2D],
         fqy[kMaxFace2D];
     const ST *fin[kMaxFace2D];  // neighbour gradient row, or boundary-value row
-    int fnb[kMaxFace2D];
+    bool fint[kMaxFace2D];      // interior face
 #pragma unroll
     for (int f = 0; f < kMaxFace2D; ++f) {
         fnx[f] = rd[f];
@@ -328,9 +329,10 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
         fqx[f] = rd[20 + f];
         fqy[f] = rd[24 + f];
         int nb = ri[f];
-        fnb[f] = nb;
+        fint[f] = nb >= 0;
         fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)(-nb - 1) * DB;
     }
+    (void)nf;  // faces past the last have zero records (their term is exactly 0)
     DirBand db(t0, nB);
     To -= m.eref;  // the C T_old term of a deviation
     double vb = mvg[db.b], tau = mtau[db.b], s0 = To * mcap[db.b] + Qc * mhr[db.b] * mtau[db.b];
@@ -350,22 +352,31 @@ __global__ void __launch_bounds__(64 * kSrcRows) k_source(const int *__restrict_
                 s0 = To * mcap[bb] + Qc * mhr[bb] * mtau[bb];
             }
             double sx = msx[d], sy = msy[d];
+            // straight-line: every face issues one load before any is used
+            // (an incoming face's upwind row: the neighbour's gradient pair,
+            // or the aligned pair of boundary values holding entry tt; an
+            // outgoing face re-reads the cell's own pair), then the term is
+            // selected
             double2 gv = ld2(gc + 2 * tt);
+            double dot[kMaxFace2D];
+            double2 nv[kMaxFace2D];
+#pragma unroll
+            for (int f = 0; f < kMaxFace2D; ++f) {
+                dot[f] = vb * sx * fnx[f] + vb * sy * fny[f];
+                size_t off = dot[f] >= 0 ? (size_t)(gc - fin[f]) + 2 * tt : (fint[f] ? 2 * tt : (tt & ~1));
+                // an opaque offset: the compiler must not turn the
+                // outgoing lanes' re-read into a branch around the load
+                asm volatile("" : "+v"(off));
+                nv[f] = ld2(fin[f] + off);
+            }
             double gcx = gv.x, gcy = gv.y;
             double s = s0;
 #pragma unroll
             for (int f = 0; f < kMaxFace2D; ++f) {
-                if (f >= nf) break;
-                double dot = vb * sx * fnx[f] + vb * sy * fny[f];
-                double a = tau * fav[f] * dot;
-                if (dot >= 0) {
-                    s -= a * (frx[f] * gcx + fry[f] * gcy);
-                } else if (fnb[f] >= 0) {
-                    double2 nv = ld2(fin[f] + 2 * tt);
-                    s -= a * (fqx[f] * nv.x + fqy[f] * nv.y);
-                } else {
-                    s -= a * fin[f][tt];
-                }
+                double a = tau * fav[f] * dot[f];
+                double uo = frx[f] * gcx + fry[f] * gcy;
+                double ui = fint[f] ? fqx[f] * nv[f].x + fqy[f] * nv[f].y : ((tt & 1) ? nv[f].y : nv[f].x);
+                s -= a * (dot[f] >= 0 ? uo : ui);
             }
             so[j] = s;
             sa[j] = in ? ((size_t)d * nc + pc[d]) * nB + bb : ~(size_t)0;
@@ -576,8 +587,8 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_dir(
 // sub-level is fixed, so each wait covers only loads issued sub-levels ago.
 // Twelve consumer waves read only LDS (staged operands and the ring of
 // recent values) and store e.  Far values are final when the producers read
-// them: the host guarantees kPcDepth * kPcCap <= R, so a far position lies
-// in a sub-level finished before the producers' read.  The band chunks of one
+// them: the host guarantees (kPcDepth + 1) * kPcCap <= R, so a far position
+// lies in a sub-level finished before the producers' read.  The band chunks of one
 // direction run the same schedule at about the same pace and write the same
 // cells' e rows, so they are placed on one XCD, where those partial-line
 // writes merge in its L2.
@@ -657,9 +668,9 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
             (void)f0;
             (void)nf;
         };
-        // far upwind cells of sub-level lv, loaded one sub-level before the
-        // rest of lv (so that the dependent far-value load waits for a load
-        // issued two sub-levels earlier)
+        // far upwind cells of sub-level lv, loaded kPcDepth - 1 sub-levels
+        // before the far values that they address (so that the dependent
+        // far-value load waits only for loads that old)
         auto issue_fc = [&](PcProdBuf<ST> &B, int lv) {
             int l = lv < Lsub ? lv : Lsub - 1;
             int f0 = fp[l], nf = fp[l + 1] - f0;
@@ -695,29 +706,33 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
             }
             if (fe < nf && fb < Bc) sF[fe * Bc + fb] = B.fv;
         };
-        // prologue: sub-levels 0 .. kPcDepth-1 in flight, far values of
-        // 0 .. kPcDepth-2, stage 0
+        // prologue: sub-levels 0 .. kPcDepth-1 in flight with their far
+        // values, stage 0, far cells of kPcDepth .. 2 kPcDepth-2
 #pragma unroll
         for (int q = 0; q < kPcDepth; ++q) {
             issue_fc(buf[q], q);
             issue(buf[q], q);
         }
 #pragma unroll
-        for (int q = 0; q + 1 < kPcDepth; ++q) issue_far(buf[q]);
+        for (int q = 0; q < kPcDepth; ++q) issue_far(buf[q]);
         write(buf[0], 0);
-        issue_fc(buf[0], kPcDepth);
+#pragma unroll
+        for (int q = 0; q + 1 < kPcDepth; ++q) issue_fc(buf[q], kPcDepth + q);
         __builtin_amdgcn_s_waitcnt(0);  // the loop starts from a known state of outstanding loads
         __syncthreads();
-        // iteration j (consumers solve j): stage j+1, far cells of j+D+1 into
-        // the buffer just staged, load j+D, far values of j+D-1
+        // iteration j (consumers solve j): stage j+1, far cells of j+2D-1
+        // (into the buffer whose far cell was used last iteration), load j+D
+        // and its far values.  Every load is consumed kPcDepth - 1 sub-levels
+        // after it is issued; the far values of j+D lie in sub-levels up to
+        // j-1 (the host checks this), which are finished.
         for (int j0 = 0; j0 < nstep; j0 += kPcDepth) {
 #pragma unroll
             for (int q = 0; q < kPcDepth; ++q) {
                 int j = j0 + q;
                 write(buf[(q + 1) % kPcDepth], j + 1);
-                issue_fc(buf[(q + 1) % kPcDepth], j + 1 + kPcDepth);
-                issue(buf[q % kPcDepth], j + kPcDepth);
-                issue_far(buf[(q + kPcDepth - 1) % kPcDepth]);
+                issue_fc(buf[(q + kPcDepth - 1) % kPcDepth], j + 2 * kPcDepth - 1);
+                issue(buf[q], j + kPcDepth);
+                issue_far(buf[q]);
                 __syncthreads();
             }
         }
@@ -1998,7 +2013,7 @@ bool Solver::build() {
         pc_nchunk = (nB + pc_bc - 1) / pc_bc;
         pc_ring = 1;
         while ((size_t)pc_ring * 2 * pc_bc * sizeof(double) <= 80 * 1024) pc_ring *= 2;
-        if (!opt.fallback_kernels && pc_bc <= kPcBc && kPcDepth * kPcCap <= pc_ring) {
+        if (!opt.fallback_kernels && pc_bc <= kPcBc && (kPcDepth + 1) * kPcCap <= pc_ring) {
             std::vector<PcRec> prec(NI);
             std::vector<int> psl, pfp, pdsl(1, 0), pfar;
             std::vector<int> freq(nc, 0), touched;
@@ -2070,7 +2085,7 @@ bool Solver::build() {
             // validate what the kernel assumes: sub-levels cover the
             // direction in order within kPcCap / kPcFar; a ring read is still
             // in the ring and finished; a far read is finished before the
-            // producers read it, kPcDepth - 1 sub-levels ahead
+            // producers read it, kPcDepth sub-levels ahead
             for (int d = 0; d < nD && fits; ++d) {
                 const int *sl = psl.data() + pdsl[d];
                 const int *fp = pfp.data() + pdsl[d];
@@ -2079,7 +2094,7 @@ bool Solver::build() {
                 for (int m2 = 0; m2 < Ls; ++m2) {
                     int a0 = sl[m2], a1 = sl[m2 + 1], nf = fp[m2 + 1] - fp[m2];
                     if (a1 <= a0 || a1 - a0 > kPcCap || nf < 0 || nf > kPcFar) { err_ = "internal error: sweep sub-level size"; return false; }
-                    int ready = sl[std::max(0, m2 - kPcDepth + 1)];
+                    int ready = sl[std::max(0, m2 - kPcDepth)];
                     for (int k = a0; k < a1; ++k) {
                         const PcRec &r = prec[(size_t)d * nc + k];
                         for (int j = 0; j < kMaxIn; ++j) {
@@ -2478,7 +2493,7 @@ bool Solver::build() {
     }
     auto half = [&](size_t n) { return fp32_ ? (n + 1) / 2 : n; };
     size_t nEB = (size_t)nD * ne * nB;
-    if (!alloc(d_e, half(nE)) || !alloc(d_grad, half(nE * dim)) || !alloc(d_eb, half(nEB)) ||
+    if (!alloc(d_e, half(nE)) || !alloc(d_grad, half(nE * dim)) || !alloc(d_eb, half(nEB + 2)) ||
         !alloc(d_eV, (size_t)nD * nn * nB) || !alloc(d_T, nc) || !alloc(d_Told, nc) || !alloc(d_T1, nc) ||
         !alloc(d_TV, nn) || !alloc(d_E, nc) || !alloc(d_qx, nc) || !alloc(d_qy, nc) || !alloc(d_qz, nc) ||
         !alloc(d_qxo, nc) || !alloc(d_qyo, nc) || !alloc(d_qzo, nc) || !alloc(d_RM, nc) || !alloc(d_Tb, (size_t)nB * nc) ||
