@@ -388,6 +388,114 @@ __global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source(const int *
     }
 }
 
+// k_source for FP32 storage with an even band count and a row length that is
+// a multiple of 4: each lane takes two neighbouring bands b, b + 1 (b even)
+// of one direction, so that one 16-byte load brings both entries' gradient
+// pairs (or the aligned quad of boundary values holding both) and one 8-byte
+// store writes both sources.  With one entry per lane the float kernel issues
+// as many memory instructions as the FP64 one for half the bytes.  A face
+// loads its upwind row unless both entries are outgoing; each entry's
+// arithmetic is k_source's.  R.tpr = 128 (64 lanes x 2 entries).
+__global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source_f32x2(
+    const int *__restrict__ posc, float *__restrict__ S, const float *__restrict__ grad,
+    const float *__restrict__ eb, const double *__restrict__ Told, const double *__restrict__ Q,
+    const int *__restrict__ fsize, const double *__restrict__ srec, const int *__restrict__ srei, Modes m, int nc,
+    Rows R) {
+    extern __shared__ double lds[];
+    int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int c0 = xcd_block() * kSrcRows, t0 = 2 * (int)(threadIdx.x & 63);
+    int DB = R.DB, nB = R.B, nD = DB / nB;
+    double *mvg = lds, *mtau = mvg + nB, *mcap = mtau + nB, *mhr = mcap + nB, *msx = mhr + nB, *msy = msx + nD;
+    int *mpos = reinterpret_cast<int *>(msy + nD);
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+        mvg[i] = m.vg[i];
+        mtau[i] = m.tau[i];
+        mcap[i] = m.cap[i];
+        mhr[i] = m.hr[i];
+    }
+    for (int i = threadIdx.x; i < nD; i += blockDim.x) {
+        msx[i] = m.sx[i];
+        msy[i] = m.sy[i];
+    }
+    for (int i = threadIdx.x; i < kSrcRows * nD; i += blockDim.x) {
+        int r = c0 + i / nD;
+        mpos[i] = r < nc ? posc[(size_t)r * nD + i % nD] : 0;
+    }
+    __syncthreads();
+    int c = c0 + w;
+    if (c >= nc) return;
+    double To = Told[c], Qc = Q[c];
+    const float *gc = grad + (size_t)c * 2 * DB;
+    const int *pc = mpos + w * nD;
+    const double *rd = srec + (size_t)c * kSrcD;
+    const int *ri = srei + (size_t)c * kSrcI;
+    double fnx[kMaxFace2D], fny[kMaxFace2D], fav[kMaxFace2D], frx[kMaxFace2D], fry[kMaxFace2D], fqx[kMaxFace2D],
+        fqy[kMaxFace2D];
+    const float *fin[kMaxFace2D];
+    bool fint[kMaxFace2D];
+#pragma unroll
+    for (int f = 0; f < kMaxFace2D; ++f) {
+        fnx[f] = rd[f];
+        fny[f] = rd[4 + f];
+        fav[f] = rd[8 + f];
+        frx[f] = rd[12 + f];
+        fry[f] = rd[16 + f];
+        fqx[f] = rd[20 + f];
+        fqy[f] = rd[24 + f];
+        int nb = ri[f];
+        fint[f] = nb >= 0;
+        fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)(-nb - 1) * DB;
+    }
+    (void)fsize;  // faces past the last have zero records (their term is exactly 0)
+    DirBand db(t0, nB);
+    To -= m.eref;
+    for (int tb = t0; tb < DB; tb += kSrcBatch * R.tpr) {
+        double so0[kSrcBatch], so1[kSrcBatch];
+        size_t sa[kSrcBatch];
+#pragma unroll
+        for (int j = 0; j < kSrcBatch; ++j) {
+            int t = tb + j * R.tpr;
+            bool in = t < DB;
+            int tt = in ? t : t0, d = in ? db.d : 0, bb = in ? db.b : 0;
+            double vb0 = mvg[bb], vb1 = mvg[bb + 1], tau0 = mtau[bb], tau1 = mtau[bb + 1];
+            double s0 = To * mcap[bb] + Qc * mhr[bb] * mtau[bb];
+            double s1 = To * mcap[bb + 1] + Qc * mhr[bb + 1] * mtau[bb + 1];
+            double sx = msx[d], sy = msy[d];
+            float4 gv = *reinterpret_cast<const float4 *>(gc + 2 * tt);
+            double dot0[kMaxFace2D], dot1[kMaxFace2D];
+            float4 nv[kMaxFace2D];
+#pragma unroll
+            for (int f = 0; f < kMaxFace2D; ++f) {
+                dot0[f] = vb0 * sx * fnx[f] + vb0 * sy * fny[f];
+                dot1[f] = vb1 * sx * fnx[f] + vb1 * sy * fny[f];
+                bool out = dot0[f] >= 0 && dot1[f] >= 0;
+                size_t off = out ? (size_t)(gc - fin[f]) + 2 * tt : (fint[f] ? 2 * tt : (tt & ~3));
+                asm volatile("" : "+v"(off));  // keep it a plain load
+                nv[f] = *reinterpret_cast<const float4 *>(fin[f] + off);
+            }
+            double g0x = gv.x, g0y = gv.y, g1x = gv.z, g1y = gv.w;
+            bool hi = (tt & 2) != 0;
+#pragma unroll
+            for (int f = 0; f < kMaxFace2D; ++f) {
+                double a0 = tau0 * fav[f] * dot0[f], a1 = tau1 * fav[f] * dot1[f];
+                double uo0 = frx[f] * g0x + fry[f] * g0y, uo1 = frx[f] * g1x + fry[f] * g1y;
+                double nx0 = nv[f].x, ny0 = nv[f].y, nx1 = nv[f].z, ny1 = nv[f].w;
+                double ui0 = fint[f] ? fqx[f] * nx0 + fqy[f] * ny0 : (hi ? nx1 : nx0);
+                double ui1 = fint[f] ? fqx[f] * nx1 + fqy[f] * ny1 : (hi ? ny1 : ny0);
+                s0 -= a0 * (dot0[f] >= 0 ? uo0 : ui0);
+                s1 -= a1 * (dot1[f] >= 0 ? uo1 : ui1);
+            }
+            so0[j] = s0;
+            so1[j] = s1;
+            sa[j] = in ? ((size_t)d * nc + pc[d]) * nB + bb : ~(size_t)0;
+            db.next(R);
+        }
+#pragma unroll
+        for (int j = 0; j < kSrcBatch; ++j)
+            if (sa[j] != ~(size_t)0) st2_nt(S + sa[j], so0[j], so1[j]);
+    }
+}
+
 // Order 2limit: the limiter of _get_gradient_larger (StaticBTESynthetic.cpp:
 // 1372-1422) applied to the stored LSQ gradients.  The range is the cell's
 // value, its face neighbours' and the incoming non-specular boundary values
@@ -2763,6 +2871,11 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     size_t nE = (size_t)nD * nc * nB;
     Rows R{nD * nB, nB, row_tpr, row_rpb, row_tpr / nB, row_tpr % nB};
     Rows Rsrc{nD * nB, nB, 64, kSrcRows, 64 / nB, 64 % nB};  // k_source: one wave per row
+    // FP32 storage: two bands per lane where the bands pair up and the rows
+    // hold whole 16-byte quads
+    bool src_pairs = fp32_ && nB % 2 == 0 && (nD * nB) % 4 == 0;
+    Rows Rsrc2{nD * nB, nB, 128, kSrcRows, 128 / nB, 128 % nB};
+    size_t src_lds = (size_t)(4 * nB + 2 * nD) * sizeof(double) + (size_t)kSrcRows * nD * sizeof(int);
     bool one_row = row_rpb == 1;
     int rowblk = row_tpr * row_rpb;
     // row kernels remap blocks per XCD (xcd_block): grids are multiples of 8
@@ -2830,10 +2943,14 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             GBTE_LAUNCH_ST(k_limit, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), 0, rp<ST>(d_e),
                            rp<ST>(d_grad), rp<ST>(d_eb), d_fsize.p, d_srec.p, d_slim.p, m, nc, nD * nB, nB);
         (void)hipEventRecord(ev_[0], s_);
-        GBTE_LAUNCH_ST(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows),
-                       (size_t)(4 * nB + 2 * nD) * sizeof(double) + (size_t)kSrcRows * nD * sizeof(int), d_posc.p,
-                       rp<ST>(d_S), rp<ST>(d_grad), rp<ST>(d_eb), d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc,
-                       Rsrc);
+        if (src_pairs)
+            hipLaunchKernelGGL(k_source_f32x2, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), src_lds, s_,
+                               d_posc.p, rp<float>(d_S), rp<float>(d_grad), rp<float>(d_eb), d_Told.p, d_Q.p,
+                               d_fsize.p, d_srec.p, d_srei.p, m, nc, Rsrc2);
+        else
+            GBTE_LAUNCH_ST(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), src_lds, d_posc.p,
+                           rp<ST>(d_S), rp<ST>(d_grad), rp<ST>(d_eb), d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p,
+                           m, nc, Rsrc);
         (void)hipEventRecord(ev_[1], s_);
         if (use_pc) {
             PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
