@@ -1619,6 +1619,20 @@ __global__ void __launch_bounds__(kBlock) k_nd_forward(const int2 *__restrict__ 
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     if (act) {
         int a = a0;
+        // 16 columns' loads in flight before their products; each
+        // accumulator still takes its columns in order (a, a + 4, ...)
+        for (; a + 15 < a1; a += 16) {
+            double x[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) x[q] = Fc[(size_t)(a + q) * m + i];
+#pragma unroll
+            for (int q = 0; q < 16; q += 4) {
+                s0 += x[q] * f[a + q];
+                s1 += x[q + 1] * f[a + q + 1];
+                s2 += x[q + 2] * f[a + q + 2];
+                s3 += x[q + 3] * f[a + q + 3];
+            }
+        }
         for (; a + 3 < a1; a += 4) {
             s0 += Fc[(size_t)a * m + i] * f[a];
             s1 += Fc[(size_t)(a + 1) * m + i] * f[a + 1];
@@ -1656,6 +1670,18 @@ __global__ void __launch_bounds__(kBlock) k_nd_backward(const int2 *__restrict__
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     if (act) {
         int j = j0;
+        for (; j + 15 < j1; j += 16) {  // as in k_nd_forward
+            double x[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) x[q] = Fr[(size_t)(j + q) * nc + a];
+#pragma unroll
+            for (int q = 0; q < 16; q += 4) {
+                s0 += x[q] * v[j + q];
+                s1 += x[q + 1] * v[j + q + 1];
+                s2 += x[q + 2] * v[j + q + 2];
+                s3 += x[q + 3] * v[j + q + 3];
+            }
+        }
         for (; j + 3 < j1; j += 4) {
             s0 += Fr[(size_t)j * nc + a] * v[j];
             s1 += Fr[(size_t)(j + 1) * nc + a] * v[j + 1];
