@@ -279,6 +279,67 @@ constexpr int kSrcWaves = 4;  // waves per SIMD the register budget allows
 // are copied to LDS first (dynamic LDS: 4 nB + 2 nD doubles, kSrcRows nD ints).
 constexpr int kSrcRows = 4;
 
+// What the source kernels share: the mode constants and the rows' upwind
+// positions in LDS, and one cell's face record (normal, A/V, r from this
+// centre, r' from the neighbour's centre, the neighbour's gradient row or the
+// boundary-value row).  Faces past the last have zero records, so their term
+// is exactly 0.
+struct SrcLds {
+    double *vg, *tau, *cap, *hr, *sx, *sy;
+    int *pos;
+};
+
+__device__ __forceinline__ SrcLds src_stage(double *lds, const int *__restrict__ posc, const Modes &m, int c0, int nc,
+                                            int nB, int nD) {
+    SrcLds L;
+    L.vg = lds;
+    L.tau = L.vg + nB;
+    L.cap = L.tau + nB;
+    L.hr = L.cap + nB;
+    L.sx = L.hr + nB;
+    L.sy = L.sx + nD;
+    L.pos = reinterpret_cast<int *>(L.sy + nD);
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+        L.vg[i] = m.vg[i];
+        L.tau[i] = m.tau[i];
+        L.cap[i] = m.cap[i];
+        L.hr[i] = m.hr[i];
+    }
+    for (int i = threadIdx.x; i < nD; i += blockDim.x) {
+        L.sx[i] = m.sx[i];
+        L.sy[i] = m.sy[i];
+    }
+    for (int i = threadIdx.x; i < kSrcRows * nD; i += blockDim.x) {
+        int r = c0 + i / nD;
+        L.pos[i] = r < nc ? posc[(size_t)r * nD + i % nD] : 0;
+    }
+    __syncthreads();
+    return L;
+}
+
+template <typename ST>
+struct SrcFaces {
+    double nx[kMaxFace2D], ny[kMaxFace2D], av[kMaxFace2D], rx[kMaxFace2D], ry[kMaxFace2D], qx[kMaxFace2D],
+        qy[kMaxFace2D];
+    const ST *in[kMaxFace2D];  // neighbour gradient row, or boundary-value row
+    bool interior[kMaxFace2D];
+    __device__ __forceinline__ SrcFaces(const double *rd, const int *ri, const ST *grad, const ST *eb, int DB) {
+#pragma unroll
+        for (int f = 0; f < kMaxFace2D; ++f) {
+            nx[f] = rd[f];
+            ny[f] = rd[4 + f];
+            av[f] = rd[8 + f];
+            rx[f] = rd[12 + f];
+            ry[f] = rd[16 + f];
+            qx[f] = rd[20 + f];
+            qy[f] = rd[24 + f];
+            int nb = ri[f];
+            interior[f] = nb >= 0;
+            in[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)(-nb - 1) * DB;
+        }
+    }
+};
+
 template <typename ST>
 __global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source(const int *__restrict__ posc, ST *__restrict__ S,
                                                          const ST *__restrict__ grad, const ST *__restrict__ eb,
@@ -289,50 +350,18 @@ __global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source(const int *
     int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int c0 = xcd_block() * kSrcRows, t0 = threadIdx.x & 63;
     int DB = R.DB, nB = R.B, nD = DB / nB;
-    double *mvg = lds, *mtau = mvg + nB, *mcap = mtau + nB, *mhr = mcap + nB, *msx = mhr + nB, *msy = msx + nD;
-    int *mpos = reinterpret_cast<int *>(msy + nD);
-    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
-        mvg[i] = m.vg[i];
-        mtau[i] = m.tau[i];
-        mcap[i] = m.cap[i];
-        mhr[i] = m.hr[i];
-    }
-    for (int i = threadIdx.x; i < nD; i += blockDim.x) {
-        msx[i] = m.sx[i];
-        msy[i] = m.sy[i];
-    }
-    for (int i = threadIdx.x; i < kSrcRows * nD; i += blockDim.x) {
-        int r = c0 + i / nD;
-        mpos[i] = r < nc ? posc[(size_t)r * nD + i % nD] : 0;
-    }
-    __syncthreads();
+    SrcLds L = src_stage(lds, posc, m, c0, nc, nB, nD);
+    const double *mvg = L.vg, *mtau = L.tau, *mcap = L.cap, *mhr = L.hr, *msx = L.sx, *msy = L.sy;
     int c = c0 + w;
     if (c >= nc) return;
-    int nf = fsize[c];
+    (void)fsize;
     double To = Told[c], Qc = Q[c];
     const ST *gc = grad + (size_t)c * 2 * DB;
-    const int *pc = mpos + w * nD;
-    const double *rd = srec + (size_t)c * kSrcD;
-    const int *ri = srei + (size_t)c * kSrcI;
-    // faces: normal, (A/V), r from this centre, r' from the neighbour's centre
-    double fnx[kMaxFace2D], fny[kMaxFace2D], fav[kMaxFace2D], frx[kMaxFace2D], fry[kMaxFace2D], fqx[kMaxFace2D],
-        fqy[kMaxFace2D];
-    const ST *fin[kMaxFace2D];  // neighbour gradient row, or boundary-value row
-    bool fint[kMaxFace2D];      // interior face
-#pragma unroll
-    for (int f = 0; f < kMaxFace2D; ++f) {
-        fnx[f] = rd[f];
-        fny[f] = rd[4 + f];
-        fav[f] = rd[8 + f];
-        frx[f] = rd[12 + f];
-        fry[f] = rd[16 + f];
-        fqx[f] = rd[20 + f];
-        fqy[f] = rd[24 + f];
-        int nb = ri[f];
-        fint[f] = nb >= 0;
-        fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)(-nb - 1) * DB;
-    }
-    (void)nf;  // faces past the last have zero records (their term is exactly 0)
+    const int *pc = L.pos + w * nD;
+    SrcFaces<ST> F(srec + (size_t)c * kSrcD, srei + (size_t)c * kSrcI, grad, eb, DB);
+    const double *fnx = F.nx, *fny = F.ny, *fav = F.av, *frx = F.rx, *fry = F.ry, *fqx = F.qx, *fqy = F.qy;
+    const ST *const *fin = F.in;
+    const bool *fint = F.interior;
     DirBand db(t0, nB);
     To -= m.eref;  // the C T_old term of a deviation
     double vb = mvg[db.b], tau = mtau[db.b], s0 = To * mcap[db.b] + Qc * mhr[db.b] * mtau[db.b];
@@ -405,48 +434,18 @@ __global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source_f32x2(
     int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int c0 = xcd_block() * kSrcRows, t0 = 2 * (int)(threadIdx.x & 63);
     int DB = R.DB, nB = R.B, nD = DB / nB;
-    double *mvg = lds, *mtau = mvg + nB, *mcap = mtau + nB, *mhr = mcap + nB, *msx = mhr + nB, *msy = msx + nD;
-    int *mpos = reinterpret_cast<int *>(msy + nD);
-    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
-        mvg[i] = m.vg[i];
-        mtau[i] = m.tau[i];
-        mcap[i] = m.cap[i];
-        mhr[i] = m.hr[i];
-    }
-    for (int i = threadIdx.x; i < nD; i += blockDim.x) {
-        msx[i] = m.sx[i];
-        msy[i] = m.sy[i];
-    }
-    for (int i = threadIdx.x; i < kSrcRows * nD; i += blockDim.x) {
-        int r = c0 + i / nD;
-        mpos[i] = r < nc ? posc[(size_t)r * nD + i % nD] : 0;
-    }
-    __syncthreads();
+    SrcLds L = src_stage(lds, posc, m, c0, nc, nB, nD);
+    const double *mvg = L.vg, *mtau = L.tau, *mcap = L.cap, *mhr = L.hr, *msx = L.sx, *msy = L.sy;
     int c = c0 + w;
     if (c >= nc) return;
+    (void)fsize;
     double To = Told[c], Qc = Q[c];
     const float *gc = grad + (size_t)c * 2 * DB;
-    const int *pc = mpos + w * nD;
-    const double *rd = srec + (size_t)c * kSrcD;
-    const int *ri = srei + (size_t)c * kSrcI;
-    double fnx[kMaxFace2D], fny[kMaxFace2D], fav[kMaxFace2D], frx[kMaxFace2D], fry[kMaxFace2D], fqx[kMaxFace2D],
-        fqy[kMaxFace2D];
-    const float *fin[kMaxFace2D];
-    bool fint[kMaxFace2D];
-#pragma unroll
-    for (int f = 0; f < kMaxFace2D; ++f) {
-        fnx[f] = rd[f];
-        fny[f] = rd[4 + f];
-        fav[f] = rd[8 + f];
-        frx[f] = rd[12 + f];
-        fry[f] = rd[16 + f];
-        fqx[f] = rd[20 + f];
-        fqy[f] = rd[24 + f];
-        int nb = ri[f];
-        fint[f] = nb >= 0;
-        fin[f] = nb >= 0 ? grad + (size_t)nb * 2 * DB : eb + (size_t)(-nb - 1) * DB;
-    }
-    (void)fsize;  // faces past the last have zero records (their term is exactly 0)
+    const int *pc = L.pos + w * nD;
+    SrcFaces<float> F(srec + (size_t)c * kSrcD, srei + (size_t)c * kSrcI, grad, eb, DB);
+    const double *fnx = F.nx, *fny = F.ny, *fav = F.av, *frx = F.rx, *fry = F.ry, *fqx = F.qx, *fqy = F.qy;
+    const float *const *fin = F.in;
+    const bool *fint = F.interior;
     DirBand db(t0, nB);
     To -= m.eref;
     for (int tb = t0; tb < DB; tb += kSrcBatch * R.tpr) {
