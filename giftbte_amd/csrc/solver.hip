@@ -289,8 +289,8 @@ struct SrcLds {
     int *pos;
 };
 
-__device__ __forceinline__ SrcLds src_stage(double *lds, const int *__restrict__ posc, const Modes &m, int c0, int nc,
-                                            int nB, int nD) {
+__device__ __forceinline__ SrcLds src_stage(double *lds, const int *__restrict__ posc, const Modes &m, int c0,
+                                            int nrows, const int *__restrict__ cells, int nB, int nD) {
     SrcLds L;
     L.vg = lds;
     L.tau = L.vg + nB;
@@ -311,7 +311,8 @@ __device__ __forceinline__ SrcLds src_stage(double *lds, const int *__restrict__
     }
     for (int i = threadIdx.x; i < kSrcRows * nD; i += blockDim.x) {
         int r = c0 + i / nD;
-        L.pos[i] = r < nc ? posc[(size_t)r * nD + i % nD] : 0;
+        int c = r < nrows ? (cells ? cells[r] : r) : -1;
+        L.pos[i] = c >= 0 ? posc[(size_t)c * nD + i % nD] : 0;
     }
     __syncthreads();
     return L;
@@ -345,17 +346,20 @@ __global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source(const int *
                                                          const ST *__restrict__ grad, const ST *__restrict__ eb,
                                                          const double *__restrict__ Told, const double *__restrict__ Q,
                                                          const int *__restrict__ fsize, const double *__restrict__ srec,
-                                                         const int *__restrict__ srei, Modes m, int nc, Rows R) {
+                                                         const int *__restrict__ srei, Modes m, int nc, Rows R,
+                                                         const int *__restrict__ cells, int nrows, int with_told) {
     extern __shared__ double lds[];
     int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int c0 = xcd_block() * kSrcRows, t0 = threadIdx.x & 63;
     int DB = R.DB, nB = R.B, nD = DB / nB;
-    SrcLds L = src_stage(lds, posc, m, c0, nc, nB, nD);
+    SrcLds L = src_stage(lds, posc, m, c0, nrows, cells, nB, nD);
     const double *mvg = L.vg, *mtau = L.tau, *mcap = L.cap, *mhr = L.hr, *msx = L.sx, *msy = L.sy;
-    int c = c0 + w;
-    if (c >= nc) return;
+    int row = c0 + w;
+    if (row >= nrows) return;
+    int c = cells ? cells[row] : row;
     (void)fsize;
-    double To = Told[c], Qc = Q[c];
+    // the C T_old term of a deviation, or none (added by the sweep instead)
+    double To = with_told ? Told[c] - m.eref : 0.0, Qc = Q[c];
     const ST *gc = grad + (size_t)c * 2 * DB;
     const int *pc = L.pos + w * nD;
     SrcFaces<ST> F(srec + (size_t)c * kSrcD, srei + (size_t)c * kSrcI, grad, eb, DB);
@@ -363,7 +367,6 @@ __global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source(const int *
     const ST *const *fin = F.in;
     const bool *fint = F.interior;
     DirBand db(t0, nB);
-    To -= m.eref;  // the C T_old term of a deviation
     double vb = mvg[db.b], tau = mtau[db.b], s0 = To * mcap[db.b] + Qc * mhr[db.b] * mtau[db.b];
     // entries in batches of kSrcBatch: every load of a batch is issued before
     // its stores (the stores are counted by the same wait counter)
@@ -429,17 +432,19 @@ __global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source_f32x2(
     const int *__restrict__ posc, float *__restrict__ S, const float *__restrict__ grad,
     const float *__restrict__ eb, const double *__restrict__ Told, const double *__restrict__ Q,
     const int *__restrict__ fsize, const double *__restrict__ srec, const int *__restrict__ srei, Modes m, int nc,
-    Rows R) {
+    Rows R, const int *__restrict__ cells, int nrows, int with_told) {
     extern __shared__ double lds[];
     int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int c0 = xcd_block() * kSrcRows, t0 = 2 * (int)(threadIdx.x & 63);
     int DB = R.DB, nB = R.B, nD = DB / nB;
-    SrcLds L = src_stage(lds, posc, m, c0, nc, nB, nD);
+    SrcLds L = src_stage(lds, posc, m, c0, nrows, cells, nB, nD);
     const double *mvg = L.vg, *mtau = L.tau, *mcap = L.cap, *mhr = L.hr, *msx = L.sx, *msy = L.sy;
-    int c = c0 + w;
-    if (c >= nc) return;
+    int row = c0 + w;
+    if (row >= nrows) return;
+    int c = cells ? cells[row] : row;
     (void)fsize;
-    double To = Told[c], Qc = Q[c];
+    // the C T_old term of a deviation, or none (added by the sweep instead)
+    double To = with_told ? Told[c] - m.eref : 0.0, Qc = Q[c];
     const float *gc = grad + (size_t)c * 2 * DB;
     const int *pc = L.pos + w * nD;
     SrcFaces<float> F(srec + (size_t)c * kSrcD, srei + (size_t)c * kSrcI, grad, eb, DB);
@@ -447,7 +452,6 @@ __global__ void __launch_bounds__(64 * kSrcRows, kSrcWaves) k_source_f32x2(
     const float *const *fin = F.in;
     const bool *fint = F.interior;
     DirBand db(t0, nB);
-    To -= m.eref;
     for (int tb = t0; tb < DB; tb += kSrcBatch * R.tpr) {
         double so0[kSrcBatch], so1[kSrcBatch];
         size_t sa[kSrcBatch];
@@ -709,8 +713,9 @@ constexpr int kPcDepth = 4;     // sub-levels in flight
 constexpr int kPcMaxSub = 2047; // sub-levels per direction (LDS tables)
 // staging buffer, in doubles (16-byte aligned sections, read as 16-byte
 // vectors): S[kPcCap][Bc] | g0,g1,g2,G [kPcCap] | far[kPcFar][Bc] | ints
-// cell,q0,q1,q2 [kPcCap]
-constexpr int kPcStage = kPcCap * kPcBc + kPcCap * 4 + kPcFar * kPcBc + kPcCap * 2;
+// cell,q0,q1,q2 [kPcCap] | T_old - Tref [kPcCap] (when the sources leave out
+// their C T_old term)
+constexpr int kPcStage = kPcCap * kPcBc + kPcCap * 4 + kPcFar * kPcBc + kPcCap * 2 + kPcCap;
 
 struct PcSched {
     const PcRec *rec;    // [dir][position]
@@ -726,14 +731,17 @@ struct PcProdBuf {
     int4 r1;             // g0,g1 (both lanes)
     int fc;              // far upwind cell
     ST fv;               // far upwind value
+    double tp;           // T_old - Tref of the position's cell (with tpos)
 };
 
 __device__ __forceinline__ double pc_dbl(int lo, int hi) { return __hiloint2double(hi, lo); }
 
+// tpos: [dir][upwind position] T_old - Tref, or null when S holds the whole
+// source; with it, the source is S + C_b tpos.
 template <typename ST>
 __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__restrict__ S,
                                                        ST *__restrict__ e, Modes m, int nc, int nB, int nD, int Bc,
-                                                       int nchunk, int R) {
+                                                       int nchunk, int R, const double *__restrict__ tpos) {
     extern __shared__ double lds[];
     int bid = gridDim.x % kXcd == 0 ? xcd_block() : (int)blockIdx.x;
     int d = bid / nchunk, chunk = bid - d * nchunk;
@@ -772,6 +780,7 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
             const int4 *rr = reinterpret_cast<const int4 *>(P.rec + Pp);
             B.r0 = rr[h ? 2 : 0];
             B.r1 = rr[1];
+            if (tpos) B.tp = tpos[Pp];
             (void)f0;
             (void)nf;
         };
@@ -793,7 +802,9 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
             double *st = stage + (size_t)(lv & 1) * kPcStage;
             double *sS = st, *sG = st + kPcCap * kPcBc, *sF = sG + kPcCap * 4;
             int *sN = reinterpret_cast<int *>(sF + kPcFar * kPcBc);
+            double *sT = sF + kPcFar * kPcBc + kPcCap * 2;
             if (p < np) {
+                if (tpos && h == 0) sT[p] = B.tp;
 #pragma unroll
                 for (int i = 0; i < kPcBpl; ++i) {
                     int jb = h * kPcBpl + i;
@@ -850,6 +861,7 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
         int b = b0 + bl;
         bool act = slot < nslot && b < nB;
         double tvb = act ? m.tau[b] * m.vg[b] : 0.0;
+        double cb = act ? m.cap[b] : 0.0;
         ST *ed = e + (size_t)d * nB + b;
         __syncthreads();  // prologue
         for (int j = 0; j < nstep; ++j) {
@@ -858,12 +870,14 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
                 const double *st = stage + (size_t)(j & 1) * kPcStage;
                 const double *sS = st, *sG = st + kPcCap * kPcBc, *sF = sG + kPcCap * 4;
                 const int *sN = reinterpret_cast<const int *>(sF + kPcFar * kPcBc);
+                const double *sT = sF + kPcFar * kPcBc + kPcCap * 2;
                 for (int p = slot; p < np; p += nslot) {
                     int4 nq = *reinterpret_cast<const int4 *>(sN + p * 4);
                     double2 g23 = *reinterpret_cast<const double2 *>(sG + p * 4 + 2);
                     double den = 1.0 + tvb * g23.y;
                     int cell = nq.x;
                     double s = sS[p * Bc + bl];
+                    if (tpos) s += cb * sT[p];
                     auto up = [&](int q, double gq) {
                         if (q == -1) return;
                         double u = q >= 0 ? ring[(q & (R - 1)) * Bc + bl] : sF[(-2 - q) * Bc + bl];
@@ -1403,6 +1417,14 @@ __global__ void k_boundary(ST *__restrict__ eb, double *__restrict__ tmac, Modes
     }
 }
 
+// T_old - Tref in each direction's upwind order, for a sweep whose sources
+// leave out their C T_old term (item_cell: [dir][position] -> cell).
+__global__ void k_tpos(const int *__restrict__ item_cell, const double *__restrict__ Told, double *__restrict__ tpos,
+                       double eref, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tpos[i] = Told[item_cell[i]] - eref;
+}
+
 // copy() (StaticBTESynthetic.cpp:2655-2690): old fields.
 __global__ void k_copy_old(const double *__restrict__ T, const double *__restrict__ qx, const double *__restrict__ qy,
                            const double *__restrict__ qz, double *__restrict__ To, double *__restrict__ qxo,
@@ -1823,8 +1845,13 @@ Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_ne
     // of this stream, so the synchronous host copies at call entry and exit
     // never overlap its work.  Default priority.
     if (!check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return;
+    // the side stream of the interior sources (default priority as well)
+    if (!check(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return;
     for (hipEvent_t &ev : ev_)
         if (!check(hipEventCreate(&ev), "hipEventCreate")) return;
+    for (auto &set : ev2_)
+        for (hipEvent_t &ev : set)
+            if (!check(hipEventCreate(&ev), "hipEventCreate")) return;
     if (!check(hipHostMalloc((void **)&h_pin, 64 * sizeof(double)), "hipHostMalloc")) return;
     build();
 }
@@ -1837,8 +1864,13 @@ Solver::~Solver() {
     if (h_part) (void)hipHostFree(h_part);
     if (h_err) (void)hipHostFree(h_err);
     if (h_pin) (void)hipHostFree(h_pin);
+    if (s2_) (void)hipStreamSynchronize(s2_);
     for (hipEvent_t ev : ev_)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto &set : ev2_)
+        for (hipEvent_t ev : set)
+            if (ev) (void)hipEventDestroy(ev);
+    if (s2_) (void)hipStreamDestroy(s2_);
     if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -2049,6 +2081,17 @@ bool Solver::build() {
             }
         }
         if (!upload(d_srec, srec) || !upload(d_srei, srei) || !upload(d_slim, slim)) return false;
+        // cells without / with a boundary face: the first sources read only
+        // gradients, so they can be formed before the boundary update
+        std::vector<int> ci, cbd;
+        for (int c = 0; c < nc; ++c) {
+            bool bnd = false;
+            for (int f = 0; f < p.face_size[c]; ++f) bnd = bnd || srei[(size_t)c * kSrcI + f] < 0;
+            (bnd ? cbd : ci).push_back(c);
+        }
+        n_src_int = (int)ci.size();
+        n_src_bnd = (int)cbd.size();
+        if ((!ci.empty() && !upload(d_src_int, ci)) || (!cbd.empty() && !upload(d_src_bnd, cbd))) return false;
     }
     if (!upload(d_fsize, p.face_size) || !upload(d_snb, snb) || !upload(d_sent, sent) || !upload(d_vol, p.volume) ||
         !upload(d_Q, p.heat_source) || !upload(d_snx, p.face_nx) || !upload(d_sny, p.face_ny) || !upload(d_snz, p.face_nz) ||
@@ -2633,6 +2676,7 @@ bool Solver::build() {
         !alloc(d_tmac, ne) || !alloc(d_rhs, nc) || !alloc(d_r, nc) || !alloc(d_z, nc) || !alloc(d_p, nc) ||
         !alloc(d_Ap, nc) || !alloc(d_S, half(nE)))
         return false;
+    if (use_pc && !alloc(d_tpos, (size_t)nD * nc)) return false;
     if (opt.linear_solver == GBTE_LINSOLVE_DIRECT &&
         (!alloc(d_ndy, nc) || !alloc(d_ndx, nc) || !alloc(d_ndu, (size_t)std::max<long>(1, std::max(NDmacro.u_total, NDfour.u_total)))))
         return false;
@@ -2874,6 +2918,8 @@ static ST *rp(DBuf<double> &b) {
 int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;  // the calling thread's device
+    // an earlier call that failed mid-iteration may have left side-stream work
+    if (!check(hipStreamSynchronize(s2_), "sync")) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
     bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1",
          limiter = P.deck.order == "2limit";
@@ -2956,7 +3002,33 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     double errT = 0, errQ = 0;
     double source_ms = 0, sweep_ms = 0, face_ms = 0, transport_ms = 0, macro_ms = 0;
     long cg_total = 0;
+    // Sources in two parts (not with 2limit, whose limiter rewrites the
+    // gradients after the boundary update, nor with the level-synchronous
+    // sweep): the cells without a boundary face read only gradients, so
+    // their sources (without the C T_old term) are formed on the side stream
+    // right after the face kernel, beside the boundary update, the synthetic
+    // passes and the host's error check; the cells with one follow on this
+    // stream, and the sweep adds C_b (T_old - Tref).
+    bool overlap = use_pc && !limiter;
+    auto launch_src = [&](hipStream_t st, const int *cells, int nrows, int with_told) {
+        if (nrows <= 0) return;
+        dim3 grid(xgrid(blocks_for(nrows, kSrcRows))), blk(64 * kSrcRows);
+        if (src_pairs)
+            hipLaunchKernelGGL(k_source_f32x2, grid, blk, src_lds, st, d_posc.p, rp<float>(d_S), rp<float>(d_grad),
+                               rp<float>(d_eb), d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc, Rsrc2, cells,
+                               nrows, with_told);
+        else if (fp32_)
+            hipLaunchKernelGGL(k_source<float>, grid, blk, src_lds, st, d_posc.p, rp<float>(d_S), rp<float>(d_grad),
+                               rp<float>(d_eb), d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc, Rsrc, cells, nrows,
+                               with_told);
+        else
+            hipLaunchKernelGGL(k_source<double>, grid, blk, src_lds, st, d_posc.p, d_S.p, d_grad.p, d_eb.p, d_Told.p,
+                               d_Q.p, d_fsize.p, d_srec.p, d_srei.p, m, nc, Rsrc, cells, nrows, with_told);
+    };
+    bool side_pending = false;  // this iteration's interior sources are on the side stream
     for (nt = 0; nt < max_iter; ++nt) {
+        bool side_src = side_pending;
+        side_pending = false;
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
                            d_qxo.p, d_qyo.p, d_qzo.p, nc);
         // this iteration's gradients were formed by the previous iteration's
@@ -2968,19 +3040,22 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             GBTE_LAUNCH_ST(k_limit, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), 0, rp<ST>(d_e),
                            rp<ST>(d_grad), rp<ST>(d_eb), d_fsize.p, d_srec.p, d_slim.p, m, nc, nD * nB, nB);
         (void)hipEventRecord(ev_[0], s_);
-        if (src_pairs)
-            hipLaunchKernelGGL(k_source_f32x2, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), src_lds, s_,
-                               d_posc.p, rp<float>(d_S), rp<float>(d_grad), rp<float>(d_eb), d_Told.p, d_Q.p,
-                               d_fsize.p, d_srec.p, d_srei.p, m, nc, Rsrc2);
-        else
-            GBTE_LAUNCH_ST(k_source, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), src_lds, d_posc.p,
-                           rp<ST>(d_S), rp<ST>(d_grad), rp<ST>(d_eb), d_Told.p, d_Q.p, d_fsize.p, d_srec.p, d_srei.p,
-                           m, nc, Rsrc);
+        if (!overlap) {
+            launch_src(s_, nullptr, nc, 1);
+        } else {
+            if (!side_src) launch_src(s_, d_src_int.p, n_src_int, 0);
+            launch_src(s_, d_src_bnd.p, n_src_bnd, 0);
+            size_t npos = (size_t)nD * nc;
+            hipLaunchKernelGGL(k_tpos, dim3((unsigned)((npos + kBlock - 1) / kBlock)), dim3(kBlock), 0, s_,
+                               d_item_cell.p, d_Told.p, d_tpos.p, m.eref, npos);
+            if (side_src && !check(hipStreamWaitEvent(s_, ev2_[nt & 1][2], 0), "hipStreamWaitEvent"))
+                return GBTE_HIP_ERROR;
+        }
         (void)hipEventRecord(ev_[1], s_);
         if (use_pc) {
             PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
             GBTE_LAUNCH_ST(k_sweep_pc, dim3(nD * pc_nchunk), dim3(kPcBlock), pc_lds, ps, rp<ST>(d_S), rp<ST>(d_e), m, nc,
-                               nB, nD, pc_bc, pc_nchunk, pc_ring);
+                           nB, nD, pc_bc, pc_nchunk, pc_ring, overlap ? d_tpos.p : nullptr);
         } else {
             hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
                                (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
@@ -3005,6 +3080,15 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
                                d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc, R, first_order ? 0 : 1);
         }
         (void)hipEventRecord(ev_[4], s_);
+        if (overlap && nt + 1 < max_iter) {
+            hipEvent_t *e2 = ev2_[(nt + 1) & 1];
+            (void)hipEventRecord(e2[0], s_);
+            if (!check(hipStreamWaitEvent(s2_, e2[0], 0), "hipStreamWaitEvent")) return GBTE_HIP_ERROR;
+            (void)hipEventRecord(e2[1], s2_);
+            launch_src(s2_, d_src_int.p, n_src_int, 0);
+            (void)hipEventRecord(e2[2], s2_);
+            side_pending = true;
+        }
         GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m,
                            d_ent_list.p, d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p,
                            d_be_tw.p, d_be_dT.p, d_be_tov.p, n_ent, nB, nD, cpw);
@@ -3071,6 +3155,10 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);
             source_ms += ms;
+            if (side_src) {  // completed: this stream waited for it and has been synced
+                (void)hipEventElapsedTime(&ms, ev2_[nt & 1][1], ev2_[nt & 1][2]);
+                source_ms += ms;
+            }
             (void)hipEventElapsedTime(&ms, ev_[1], ev_[2]);
             sweep_ms += ms;
             (void)hipEventElapsedTime(&ms, ev_[3], ev_[4]);
@@ -3119,7 +3207,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     }
     GBTE_LAUNCH_ST(k_output_moments, dim3(xgrid(blocks_for(nc, kOutRows))), dim3(64 * kOutRows), 0, rp<ST>(d_e), m,
                    d_E.p, d_Tb.p, nc, nB, nD);
-    if (!check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
+    // the side stream may hold the sources of an iteration that did not run
+    if (!check(hipStreamSynchronize(s2_), "sync") || !check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
     auto t1 = std::chrono::high_resolution_clock::now();
     have_state = true;
     if (st) {

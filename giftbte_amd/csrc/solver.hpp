@@ -109,6 +109,10 @@ class Solver {
     hipStream_t s_ = nullptr;
     // before/after: source | sweep | bound_out | face kernels | boundary | macro loop
     hipEvent_t ev_[7] = {};
+    // side stream: the next iteration's interior sources run there, after the
+    // face kernel, beside the boundary update and the synthetic passes
+    hipStream_t s2_ = nullptr;
+    hipEvent_t ev2_[2][3] = {};  // per launch parity: gradients ready | side sources start | end
 
     int nc = 0, nb = 0, ne = 0, nn = 0, nB = 0, nD = 0, nf = 0, dim = 2, cpw = 1, nlev = 0;
     int row_tpr = 256, row_rpb = 1;  // threads per cell row, rows per block
@@ -122,6 +126,9 @@ class Solver {
     // per-direction upwind schedule: positions in upwind order, level
     // pointers, in-face records (neighbour position, neighbour cell, g)
     DBuf<int> d_item_cell, d_posc, d_levptr, d_dlev, d_rec_q, d_rec_c;
+    DBuf<int> d_src_int, d_src_bnd;  // cells without / with a boundary face
+    int n_src_int = 0, n_src_bnd = 0;
+    DBuf<double> d_tpos;             // [dir][position] T_old - Tref
     DBuf<double> d_rec_g, d_rec_G;
     int sweep_bc = 1, sweep_nchunk = 1, sweep_ring = 1;
     // wave-specialised sweep (k_sweep_pc): records, sub-level tables
