@@ -92,8 +92,10 @@ int gbte_solve_batch(gbte_handle *const *hs, int n, const int *max_iter, const d
     // overlap another thread's allocations and synchronous copies
     for (int i = 0; i < n; ++i) {
         if (!hs[i] || !hs[i]->s || !hs[i]->s->ok()) return GBTE_HIP_ERROR;
+        hs[i]->s->set_single_stream(true);
         int r = hs[i]->s->prepare();
         if (r != GBTE_OK) {
+            for (int j = 0; j <= i; ++j) hs[j]->s->set_single_stream(false);
             hs[i]->err = hs[i]->s->error();
             if (status) status[i] = r;
             return r;
@@ -111,6 +113,7 @@ int gbte_solve_batch(gbte_handle *const *hs, int n, const int *max_iter, const d
             }
         });
     for (auto &t : th) t.join();
+    for (int i = 0; i < n; ++i) hs[i]->s->set_single_stream(false);
     int first = GBTE_OK;
     for (int i = 0; i < n; ++i) {
         if (status) status[i] = rc[i];

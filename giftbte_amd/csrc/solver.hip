@@ -1845,13 +1845,8 @@ Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_ne
     // of this stream, so the synchronous host copies at call entry and exit
     // never overlap its work.  Default priority.
     if (!check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return;
-    // the side stream of the interior sources (default priority as well)
-    if (!check(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return;
     for (hipEvent_t &ev : ev_)
         if (!check(hipEventCreate(&ev), "hipEventCreate")) return;
-    for (auto &set : ev2_)
-        for (hipEvent_t &ev : set)
-            if (!check(hipEventCreate(&ev), "hipEventCreate")) return;
     if (!check(hipHostMalloc((void **)&h_pin, 64 * sizeof(double)), "hipHostMalloc")) return;
     build();
 }
@@ -2919,7 +2914,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;  // the calling thread's device
     // an earlier call that failed mid-iteration may have left side-stream work
-    if (!check(hipStreamSynchronize(s2_), "sync")) return GBTE_HIP_ERROR;
+    if (s2_ && !check(hipStreamSynchronize(s2_), "sync")) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
     bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1",
          limiter = P.deck.order == "2limit";
@@ -3009,7 +3004,20 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     // right after the face kernel, beside the boundary update, the synthetic
     // passes and the host's error check; the cells with one follow on this
     // stream, and the sweep adds C_b (T_old - Tref).
-    bool overlap = use_pc && !limiter;
+    // split: the sources leave out the C T_old term, which the sweep adds,
+    // and the interior part runs on the side stream.  Not in batched solves:
+    // their many small instances already run side by side, and the extra
+    // launches cost more than the overlap gains there.
+    bool split = use_pc && !limiter && !single_stream_, overlap = split;
+    if (overlap && !s2_) {
+        // created at first use, so that handles that never use it do not
+        // take a share of the hardware queues (default priority as well)
+        if (!check(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking), "hipStreamCreateWithFlags"))
+            return GBTE_HIP_ERROR;
+        for (auto &set : ev2_)
+            for (hipEvent_t &ev : set)
+                if (!check(hipEventCreate(&ev), "hipEventCreate")) return GBTE_HIP_ERROR;
+    }
     auto launch_src = [&](hipStream_t st, const int *cells, int nrows, int with_told) {
         if (nrows <= 0) return;
         dim3 grid(xgrid(blocks_for(nrows, kSrcRows))), blk(64 * kSrcRows);
@@ -3040,7 +3048,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             GBTE_LAUNCH_ST(k_limit, dim3(xgrid(blocks_for(nc, kSrcRows))), dim3(64 * kSrcRows), 0, rp<ST>(d_e),
                            rp<ST>(d_grad), rp<ST>(d_eb), d_fsize.p, d_srec.p, d_slim.p, m, nc, nD * nB, nB);
         (void)hipEventRecord(ev_[0], s_);
-        if (!overlap) {
+        if (!split) {
             launch_src(s_, nullptr, nc, 1);
         } else {
             if (!side_src) launch_src(s_, d_src_int.p, n_src_int, 0);
@@ -3055,7 +3063,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         if (use_pc) {
             PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
             GBTE_LAUNCH_ST(k_sweep_pc, dim3(nD * pc_nchunk), dim3(kPcBlock), pc_lds, ps, rp<ST>(d_S), rp<ST>(d_e), m, nc,
-                           nB, nD, pc_bc, pc_nchunk, pc_ring, overlap ? d_tpos.p : nullptr);
+                           nB, nD, pc_bc, pc_nchunk, pc_ring, split ? d_tpos.p : nullptr);
         } else {
             hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
                                (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
@@ -3208,7 +3216,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     GBTE_LAUNCH_ST(k_output_moments, dim3(xgrid(blocks_for(nc, kOutRows))), dim3(64 * kOutRows), 0, rp<ST>(d_e), m,
                    d_E.p, d_Tb.p, nc, nB, nD);
     // the side stream may hold the sources of an iteration that did not run
-    if (!check(hipStreamSynchronize(s2_), "sync") || !check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
+    if ((s2_ && !check(hipStreamSynchronize(s2_), "sync")) || !check(hipStreamSynchronize(s_), "sync"))
+        return GBTE_HIP_ERROR;
     auto t1 = std::chrono::high_resolution_clock::now();
     have_state = true;
     if (st) {

@@ -77,6 +77,9 @@ class Solver {
     // otherwise); batched solves do this before their threads start, as a
     // capture must not overlap another thread's allocations and copies
     int prepare();
+    // one stream per solve and the sources in one pass (batched solves, which
+    // keep the GPU busy by running side by side)
+    void set_single_stream(bool on) { single_stream_ = on; }
 
     void set_error(const std::string &e) { err_ = e; }
 
@@ -113,6 +116,7 @@ class Solver {
     // face kernel, beside the boundary update and the synthetic passes
     hipStream_t s2_ = nullptr;
     hipEvent_t ev2_[2][3] = {};  // per launch parity: gradients ready | side sources start | end
+    bool single_stream_ = false;
 
     int nc = 0, nb = 0, ne = 0, nn = 0, nB = 0, nD = 0, nf = 0, dim = 2, cpw = 1, nlev = 0;
     int row_tpr = 256, row_rpb = 1;  // threads per cell row, rows per block

@@ -2,8 +2,11 @@
 size solved side by side match the same instances solved one at a time.
 
 Each handle keeps its own stream, graphs and buffers and the kernels are
-deterministic, so the fields are expected to be identical; the bar is 1e-13
-relative (max abs difference over max abs value) and the counts must match.
+deterministic.  A batched solve forms each source in one pass; a single solve
+forms the C T_old term in the sweep instead, a rounding-level difference in
+summation order.  The bar is 1e-10 relative (max abs difference over max abs
+value), ten times tighter than the oracle parity bar, and the counts must
+match.
 """
 import time
 
@@ -48,11 +51,14 @@ def test_batch_matches_one_at_a_time(tmp_path):
     sts = solve_batch(sols, [r[0] for r in runs], [r[1] for r in runs], [r[2] for r in runs])
     t_batch = time.time() - t1
     print(f"[gpu batch] one at a time {t_alone:.2f} s (with set-up), batch solve {t_batch:.2f} s", flush=True)
+    worst = 0.0
     for i, (s, st) in enumerate(zip(sols, sts)):
         st0, f0 = alone[i]
         assert (st.iterations, st.macro_passes) == (st0.iterations, st0.macro_passes), i
         f = s.fields()
         for k, v in f.items():
             scale = max(np.abs(f0[k]).max(), 1e-300)
-            assert np.abs(v - f0[k]).max() <= 1e-13 * scale, (i, k)
+            worst = max(worst, np.abs(v - f0[k]).max() / scale)
+            assert np.abs(v - f0[k]).max() <= 1e-10 * scale, (i, k)
         s.close()
+    print(f"[gpu batch] largest relative difference to one at a time: {worst:.1e}", flush=True)
