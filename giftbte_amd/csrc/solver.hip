@@ -3051,8 +3051,10 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         if (!split) {
             launch_src(s_, nullptr, nc, 1);
         } else {
-            if (!side_src) launch_src(s_, d_src_int.p, n_src_int, 0);
-            launch_src(s_, d_src_bnd.p, n_src_bnd, 0);
+            if (!side_src) {
+                launch_src(s_, d_src_int.p, n_src_int, 0);
+                launch_src(s_, d_src_bnd.p, n_src_bnd, 0);
+            }
             size_t npos = (size_t)nD * nc;
             hipLaunchKernelGGL(k_tpos, dim3((unsigned)((npos + kBlock - 1) / kBlock)), dim3(kBlock), 0, s_,
                                d_item_cell.p, d_Told.p, d_tpos.p, m.eref, npos);
@@ -3094,12 +3096,19 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             if (!check(hipStreamWaitEvent(s2_, e2[0], 0), "hipStreamWaitEvent")) return GBTE_HIP_ERROR;
             (void)hipEventRecord(e2[1], s2_);
             launch_src(s2_, d_src_int.p, n_src_int, 0);
-            (void)hipEventRecord(e2[2], s2_);
             side_pending = true;
         }
         GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m,
                            d_ent_list.p, d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p,
                            d_be_tw.p, d_be_dT.p, d_be_tov.p, n_ent, nB, nD, cpw);
+        if (side_pending) {
+            // the cells with a boundary face read the values just updated
+            hipEvent_t *e2 = ev2_[(nt + 1) & 1];
+            (void)hipEventRecord(e2[3], s_);
+            if (!check(hipStreamWaitEvent(s2_, e2[3], 0), "hipStreamWaitEvent")) return GBTE_HIP_ERROR;
+            launch_src(s2_, d_src_bnd.p, n_src_bnd, 0);
+            (void)hipEventRecord(e2[2], s2_);
+        }
         (void)hipEventRecord(ev_[5], s_);
         if (!check(hipGetLastError(), "iteration kernels")) return GBTE_HIP_ERROR;
         // _marco_solution (StaticBTESynthetic.cpp:2692-2734)

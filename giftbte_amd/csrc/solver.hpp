@@ -115,7 +115,7 @@ class Solver {
     // side stream: the next iteration's interior sources run there, after the
     // face kernel, beside the boundary update and the synthetic passes
     hipStream_t s2_ = nullptr;
-    hipEvent_t ev2_[2][3] = {};  // per launch parity: gradients ready | side sources start | end
+    hipEvent_t ev2_[2][4] = {};  // per launch parity: gradients ready | side sources start | end | boundary values ready
     bool single_stream_ = false;
 
     int nc = 0, nb = 0, ne = 0, nn = 0, nB = 0, nD = 0, nf = 0, dim = 2, cpw = 1, nlev = 0;
