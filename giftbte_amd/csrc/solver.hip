@@ -3002,12 +3002,10 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     // sweep): the cells without a boundary face read only gradients, so
     // their sources (without the C T_old term) are formed on the side stream
     // right after the face kernel, beside the boundary update, the synthetic
-    // passes and the host's error check; the cells with one follow on this
-    // stream, and the sweep adds C_b (T_old - Tref).
-    // split: the sources leave out the C T_old term, which the sweep adds,
-    // and the interior part runs on the side stream.  Not in batched solves:
-    // their many small instances already run side by side, and the extra
-    // launches cost more than the overlap gains there.
+    // passes and the host's error check; the cells with one follow there once
+    // the boundary update has run, and the sweep adds C_b (T_old - Tref).
+    // Not in batched solves: their many small instances already run side by
+    // side, and the extra launches cost more than the overlap gains there.
     bool split = use_pc && !limiter && !single_stream_, overlap = split;
     if (overlap && !s2_) {
         // created at first use, so that handles that never use it do not
