@@ -187,7 +187,9 @@ def solve_batch(solvers, max_iter, tol_temp, tol_flux):
     rc = lib.gbte_solve_batch(hs, n, per(max_iter, C.c_int), per(tol_temp, C.c_double), per(tol_flux, C.c_double),
                               st, status)
     if rc != 0:
-        bad = next(i for i in range(n) if status[i] != 0)
+        bad = next((i for i in range(n) if status[i] != 0), None)
+        if bad is None:
+            raise GBTEError(rc, "solve_batch failed")
         raise GBTEError(status[bad], solvers[bad].lib.gbte_last_error(solvers[bad]._h).decode())
     return [SolveStats(x.iterations, x.macro_passes, x.fourier_passes, bool(x.converged), x.err_temp, x.err_flux,
                        x.solve_ms, x.sweep_ms, x.transport_ms, x.macro_ms, x.sweep_levels, x.macro_cg_iters,

@@ -88,16 +88,28 @@ int gbte_solve_batch(gbte_handle *const *hs, int n, const int *max_iter, const d
         for (int j = 0; j < i; ++j)
             if (hs[i] == hs[j]) return GBTE_BAD_INPUT;  // one solve per handle at a time
     std::vector<int> rc(n, GBTE_OK);
+    // every handle is checked before any is changed; status[] is always filled
+    int bad = GBTE_OK;
+    for (int i = 0; i < n; ++i) {
+        rc[i] = (!hs[i] || !hs[i]->s || !hs[i]->s->ok()) ? GBTE_HIP_ERROR : GBTE_OK;
+        if (rc[i] != GBTE_OK && hs[i]) hs[i]->err = "solve_batch: handle was not created successfully";
+        if (bad == GBTE_OK) bad = rc[i];
+    }
+    if (bad != GBTE_OK) {
+        if (status)
+            for (int i = 0; i < n; ++i) status[i] = rc[i];
+        return bad;
+    }
     // graphs are recorded here, one handle after another: a capture must not
     // overlap another thread's allocations and synchronous copies
     for (int i = 0; i < n; ++i) {
-        if (!hs[i] || !hs[i]->s || !hs[i]->s->ok()) return GBTE_HIP_ERROR;
         hs[i]->s->set_single_stream(true);
         int r = hs[i]->s->prepare();
         if (r != GBTE_OK) {
             for (int j = 0; j <= i; ++j) hs[j]->s->set_single_stream(false);
             hs[i]->err = hs[i]->s->error();
-            if (status) status[i] = r;
+            if (status)
+                for (int j = 0; j < n; ++j) status[j] = j == i ? r : GBTE_OK;
             return r;
         }
     }
@@ -131,6 +143,13 @@ int gbte_get_fields(gbte_handle *h, double *T_lattice, double *T_energy, double 
 int gbte_write_outputs(gbte_handle *h, const gbte_problem *p, const char *dir) {
     if (!h || !h->s || !h->s->ok()) return GBTE_HIP_ERROR;
     if (!p) return GBTE_BAD_INPUT;
+    if (p->p.n_cells != h->s->n_cells() || p->p.n_bands != h->s->n_bands()) {
+        // the field buffers below are sized from p and filled from the handle
+        h->err = "write_outputs: the problem is not the one the handle was created from (" +
+                 std::to_string(p->p.n_cells) + " cells x " + std::to_string(p->p.n_bands) + " bands, handle " +
+                 std::to_string(h->s->n_cells()) + " x " + std::to_string(h->s->n_bands()) + ")";
+        return GBTE_BAD_INPUT;
+    }
     size_t nc = p->p.n_cells, nB = p->p.n_bands;
     std::vector<double> T(nc), E(nc), qx(nc), qy(nc), qz(nc), Tb(nB * nc);
     int rc = finish(h, h->s->get_fields(T.data(), E.data(), qx.data(), qy.data(), qz.data(), Tb.data()));

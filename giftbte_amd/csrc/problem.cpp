@@ -345,6 +345,70 @@ std::string read_msh_2d(std::istream &in, double Lx, double Ly, double Lz, RawMe
     return "";
 }
 
+// Gmsh MSH 4.1, 3D.  The reference's 3D readers were removed
+// (src/BTEMesh/ReadinMesh.cpp:404-406, 531-533); this one reads the same
+// subset as its 2D reader above: one node section, boundary triangles
+// (element type 2, entity tag = BC index) and tetrahedra (type 4, entity
+// tag - 1 = region).  Points and lines are skipped.
+std::string read_msh_3d(std::istream &in, double Lx, double Ly, double Lz, RawMesh &m) {
+    std::string s;
+    std::getline(in, s);
+    size_t p = s.find("MeshFormat");
+    if (p == std::string::npos || p == 0) return "Error: mesh file format error";
+    if (!getline_find(in, "Nodes", s)) return "Error: mesh file format error";
+    long nblock, nn;
+    in >> nblock >> nn;
+    std::getline(in, s);
+    if (!in || nn <= 0) return "Error: mesh file format error";
+    m.x.assign(nn, 0); m.y.assign(nn, 0); m.z.assign(nn, 0);
+    for (long b = 0; b < nblock; ++b) {
+        long dim, tag, para, nin;
+        in >> dim >> tag >> para >> nin;
+        std::vector<long> tags(nin);
+        for (long j = 0; j < nin; ++j) in >> tags[j];
+        for (long j = 0; j < nin; ++j) {
+            double x, y, z;
+            in >> x >> y >> z;
+            if (tags[j] < 1 || tags[j] > nn) return "Error: mesh node tag out of range";
+            m.x[tags[j] - 1] = x * Lx;
+            m.y[tags[j] - 1] = y * Ly;
+            m.z[tags[j] - 1] = z * Lz;
+        }
+    }
+    if (!in) return "Error: mesh file format error";
+    if (!getline_find(in, "Elements", s)) return "Error: mesh file format error";
+    long ncell;
+    in >> nblock >> ncell;
+    std::getline(in, s);
+    for (long b = 0; b < nblock; ++b) {
+        long dim, tag, type, nin;
+        in >> dim >> tag >> type >> nin;
+        std::getline(in, s);
+        for (long j = 0; j < nin; ++j) {
+            long t;
+            if (type == 2) {
+                std::vector<int> a(3);
+                in >> t >> a[0] >> a[1] >> a[2];
+                for (auto &v : a) v -= 1;
+                m.bound_index.push_back((int)tag);
+                m.bounds.push_back(a);
+            } else if (type == 4) {
+                std::vector<int> a(4);
+                in >> t >> a[0] >> a[1] >> a[2] >> a[3];
+                for (auto &v : a) v -= 1;
+                m.cells.push_back(a);
+                m.cell_index.push_back((int)tag - 1);
+            } else if (type == 1 || type == 15) {
+                std::getline(in, s);
+            } else {
+                return "Error: 3D meshes of tetrahedra only (Gmsh element type " + std::to_string(type) + ")";
+            }
+        }
+    }
+    if (!in && !in.eof()) return "Error: mesh file format error";
+    return "";
+}
+
 inline bool contains(const std::vector<int> &v, int a) { return std::find(v.begin(), v.end(), a) != v.end(); }
 
 struct CellG {
@@ -445,6 +509,95 @@ std::string build_2d(const RawMesh &rm, const Bands &B, std::vector<V3> &nodes, 
             for (size_t j = 0; j < cells[k].faces.size(); ++j) {
                 const auto &fv = cells[k].faces[j].vert;
                 if ((be[1] == fv[0] || be[1] == fv[1]) && (be[0] == fv[0] || be[0] == fv[1])) {
+                    bounds[i].cell.push_back(k);
+                    bounds[i].face.push_back((int)j);
+                }
+            }
+    }
+    return "";
+}
+
+// Tetrahedral geometry, the counterpart of build_2d for the removed 3D
+// branch of BTEMesh::setMeshParams (src/BTEMesh/BTEMesh.cpp:841-843):
+//   volume   |det(v0 - v3, v1 - v3, v2 - v3)| / 6, the closed form of the
+//            reference's get_tet_volume (src/utility/utility.cpp:71-86);
+//   centre   the vertex mean; face j the triangle opposite vertex j,
+//            (v[j+1], v[j+2], v[j+3]) mod 4, area by Heron's formula
+//            (get_tri_area), centre the vertex mean, unit normal the
+//            normalised cross product turned away from the cell centre;
+//   neighbour the other cell holding the face's three vertices, and each
+//            boundary triangle the (cell, face) holding its vertices.
+std::string build_3d(const RawMesh &rm, const Bands &B, std::vector<V3> &nodes, std::vector<CellG> &cells,
+                     std::vector<BoundG> &bounds, std::vector<std::vector<int>> &node_cells) {
+    size_t nn = rm.x.size();
+    nodes.resize(nn);
+    for (size_t i = 0; i < nn; ++i) nodes[i] = {rm.x[i], rm.y[i], rm.z[i]};
+    cells.resize(rm.cells.size());
+    for (size_t i = 0; i < cells.size(); ++i) {
+        cells[i].vert = rm.cells[i];
+        cells[i].index = rm.cell_index[i];
+        if (cells[i].index < 0 || cells[i].index >= (int)B.geo_matter.size())
+            return "Error: some region does not have material, check PHONON and GEOMETRY";
+        cells[i].matter = B.geo_matter[cells[i].index];
+        if (cells[i].vert.size() != 4) return "Error: 3D cells must be tetrahedra";
+        for (int v : cells[i].vert)
+            if (v < 0 || v >= (int)nn) return "Error: mesh element references a missing node";
+    }
+    for (auto &c : cells) {
+        const auto &v = c.vert;
+        const V3 &p0 = nodes[v[0]], &p1 = nodes[v[1]], &p2 = nodes[v[2]], &p3 = nodes[v[3]];
+        double a[3][3] = {{p0.x - p3.x, p1.x - p3.x, p2.x - p3.x},
+                          {p0.y - p3.y, p1.y - p3.y, p2.y - p3.y},
+                          {p0.z - p3.z, p1.z - p3.z, p2.z - p3.z}};
+        double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                     a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+        c.volume = std::fabs(det) / 6;
+        if (!(c.volume > 0)) return "Error: degenerate tetrahedron";
+        c.center.x = (((p0.x + p1.x) + p2.x) + p3.x) / 4;
+        c.center.y = (((p0.y + p1.y) + p2.y) + p3.y) / 4;
+        c.center.z = (((p0.z + p1.z) + p2.z) + p3.z) / 4;
+        c.faces.resize(4);
+        for (int j = 0; j < 4; ++j) {
+            auto &f = c.faces[j];
+            f.vert = {v[(j + 1) % 4], v[(j + 2) % 4], v[(j + 3) % 4]};
+            const V3 &a0 = nodes[f.vert[0]], &a1 = nodes[f.vert[1]], &a2 = nodes[f.vert[2]];
+            f.area = tri_area(a0, a1, a2);
+            f.c.x = ((a0.x + a1.x) + a2.x) / 3;
+            f.c.y = ((a0.y + a1.y) + a2.y) / 3;
+            f.c.z = ((a0.z + a1.z) + a2.z) / 3;
+            double ux = a1.x - a0.x, uy = a1.y - a0.y, uz = a1.z - a0.z;
+            double wx = a2.x - a0.x, wy = a2.y - a0.y, wz = a2.z - a0.z;
+            double nx = uy * wz - uz * wy, ny = uz * wx - ux * wz, nz = ux * wy - uy * wx;
+            double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+            f.n = {nx / len, ny / len, nz / len};
+            double bx = c.center.x - f.c.x, by = c.center.y - f.c.y, bz = c.center.z - f.c.z;
+            if (bx * f.n.x + by * f.n.y + bz * f.n.z > 0) f.n = {-f.n.x, -f.n.y, -f.n.z};
+        }
+    }
+    node_cells.assign(nn, {});
+    for (size_t i = 0; i < cells.size(); ++i)
+        for (int v : cells[i].vert) node_cells[v].push_back((int)i);
+    auto holds = [&](int k, const std::vector<int> &fv) {
+        const auto &cv = cells[k].vert;
+        return contains(cv, fv[0]) && contains(cv, fv[1]) && contains(cv, fv[2]);
+    };
+    for (size_t i = 0; i < cells.size(); ++i)
+        for (auto &f : cells[i].faces) {
+            f.neighbor = -1;
+            for (int k : node_cells[f.vert[0]])
+                if (k != (int)i && holds(k, f.vert)) f.neighbor = k;
+            f.bound = -1;
+        }
+    bounds.resize(rm.bounds.size());
+    for (size_t i = 0; i < rm.bounds.size(); ++i) {
+        bounds[i].index = rm.bound_index[i];
+        const auto &be = rm.bounds[i];
+        for (int v : be)
+            if (v < 0 || v >= (int)nn) return "Error: boundary element references a missing node";
+        for (int k : node_cells[be[0]])
+            for (size_t j = 0; j < cells[k].faces.size(); ++j) {
+                const auto &fv = cells[k].faces[j].vert;
+                if (contains(fv, be[0]) && contains(fv, be[1]) && contains(fv, be[2])) {
                     bounds[i].cell.push_back(k);
                     bounds[i].face.push_back((int)j);
                 }
@@ -671,7 +824,9 @@ std::string load_problem(const std::string &dir, Problem &P) {
     if (d.order != "1" && d.order != "2" && d.order != "2limit") return "no this Order: " + d.order;
     if (d.method != "Synthetic" && d.method != "Sequential") return "no this IterativeScheme: " + d.method;
     if (d.matrix != "LU" && d.matrix != "BICGSTAB") return "no this MatrixSolver: " + d.matrix;
-    if (d.dim != 2) return "GeometryDimension " + std::to_string(d.dim) + " decks are not supported by this deck reader (2D only)";
+    if (d.dim != 2 && d.dim != 3)
+        return "GeometryDimension " + std::to_string(d.dim) + " decks are not supported (2D and 3D only)";
+    if (d.dim == 3 && d.mesh_type != "MSH") return "3D decks are read from Gmsh 4.1 meshes (MeshType MSH)";
     if (!d.multiscale.empty()) return "Multiscale Fourier/BTE region files are not supported";
     Bands B;
     err = parse_bands(dir, d.material_dim, B);
@@ -684,6 +839,7 @@ std::string load_problem(const std::string &dir, Problem &P) {
         std::ifstream mf(join(dir, d.mesh_file));
         if (!mf.is_open()) return "Error: Mesh file not open";
         if (d.mesh_type == "COMSOL") err = read_comsol_2d(mf, d.Lx, d.Ly, rm);
+        else if (d.mesh_type == "MSH" && d.dim == 3) err = read_msh_3d(mf, d.Lx, d.Ly, d.Lz, rm);
         else if (d.mesh_type == "MSH") err = read_msh_2d(mf, d.Lx, d.Ly, d.Lz, rm);
         else err = "Error: does not support this mesh type";
         if (!err.empty()) return err;
@@ -692,7 +848,7 @@ std::string load_problem(const std::string &dir, Problem &P) {
     std::vector<CellG> cells;
     std::vector<BoundG> bounds;
     std::vector<std::vector<int>> node_cells;
-    err = build_2d(rm, B, nodes, cells, bounds, node_cells);
+    err = d.dim == 3 ? build_3d(rm, B, nodes, cells, bounds, node_cells) : build_2d(rm, B, nodes, cells, bounds, node_cells);
     if (!err.empty()) return err;
     err = apply_bcs(bcs, nodes, cells, bounds, B.nband);
     if (!err.empty()) return err;
@@ -722,7 +878,7 @@ std::string load_problem(const std::string &dir, Problem &P) {
 
     // ---- flatten (StaticBTESynthetic.cpp:224-626)
     int nc = (int)cells.size(), nb = (int)bounds.size(), nn = (int)nodes.size();
-    P.dim = 2;
+    P.dim = d.dim;
     P.n_cells = nc;
     P.n_nodes = nn;
     P.n_bounds = nb;
@@ -768,13 +924,17 @@ std::string load_problem(const std::string &dir, Problem &P) {
     // same-matter vertex neighbours (StaticBTESynthetic.cpp:453-465)
     P.nbr_ptr.assign(nc + 1, 0);
     P.nbr_idx.clear();
-    for (int i = 0; i < nc; ++i) {
-        std::vector<int> lst;
-        for (int v : cells[i].vert)
-            for (int k : node_cells[v])
-                if (!contains(lst, k) && cells[k].matter == cells[i].matter && k != i) lst.push_back(k);
-        P.nbr_idx.insert(P.nbr_idx.end(), lst.begin(), lst.end());
-        P.nbr_ptr[i + 1] = (int)P.nbr_idx.size();
+    {
+        std::vector<int> stamp(nc, -1);  // stamp[k] == i: k is already in cell i's list
+        for (int i = 0; i < nc; ++i) {
+            for (int v : cells[i].vert)
+                for (int k : node_cells[v])
+                    if (stamp[k] != i && cells[k].matter == cells[i].matter && k != i) {
+                        stamp[k] = i;
+                        P.nbr_idx.push_back(k);
+                    }
+            P.nbr_ptr[i + 1] = (int)P.nbr_idx.size();
+        }
     }
     P.node_x.resize(nn); P.node_y.resize(nn); P.node_z.resize(nn);
     for (int i = 0; i < nn; ++i) { P.node_x[i] = nodes[i].x; P.node_y[i] = nodes[i].y; P.node_z[i] = nodes[i].z; }

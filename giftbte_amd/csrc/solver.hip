@@ -2499,11 +2499,9 @@ bool Solver::build() {
         std::vector<int> bcell(be_cell);
         if (!upload(d_ent_list, h_ent_list) || !upload(d_be_cell, bcell) || !upload(d_be_type, btype) ||
             !upload(d_be_conn, bconn) || !upload(d_be_spec, spec) || !upload(d_be_nx, bnx) || !upload(d_be_ny, bny) ||
-            !upload(d_be_nz, bnz) || !upload(d_be_tw, btw) || !upload(d_be_dT, bdT) || !upload(d_be_tov, btov))
+            !upload(d_be_nz, bnz) || !upload(d_be_tw, btw) || !upload(d_be_dT, bdT) || !upload(d_be_tov, btov) ||
+            !upload(d_be_slot, be_slot))  // face slot per entry, for k_bound_out
             return false;
-        // slot index per entry for k_bound_out
-        DBuf<int> tmp;
-        (void)tmp;
     }
     // ---- macro (set 0) and Fourier (set 1) right-hand sides and matrices
     {
@@ -2699,7 +2697,14 @@ bool Solver::build() {
         !check(hipMemsetAsync(d_qx.p, 0, nc * sizeof(double), s_), "hipMemsetAsync") ||
         !check(hipMemsetAsync(d_qy.p, 0, nc * sizeof(double), s_), "hipMemsetAsync"))
         return false;
-    // entry -> slot table for k_bound_out lives in d_be_slot (reuse d_fother layout is not possible)
+    // interior face count per cell (the Fourier flux's LSQ switch)
+    {
+        std::vector<int> nint(nc, 0);
+        for (int c = 0; c < nc; ++c)
+            for (int j = 0; j < p.face_size[c]; ++j)
+                if (p.face_bound[(size_t)c * 6 + j] == -1) nint[c]++;
+        if (!upload(d_nint, nint)) return false;
+    }
     return check(hipStreamSynchronize(s_), "build sync");
 }
 
@@ -2852,12 +2857,6 @@ int Solver::fourier(gbte_stats *st) {
     // StaticFourier::_solve (StaticFourier.cpp:934-976) from T = initial
     double err = 1000;
     int passes = 0;
-    std::vector<int> nint(nc, 0);
-    for (int c = 0; c < nc; ++c)
-        for (int j = 0; j < P.face_size[c]; ++j)
-            if (P.face_bound[(size_t)c * 6 + j] == -1) nint[c]++;
-    DBuf<int> d_nint;
-    if (!upload(d_nint, nint)) return GBTE_HIP_ERROR;
     if (!check(hipMemcpy(d_T.p, P.initial_temp.data(), nc * sizeof(double), hipMemcpyHostToDevice), "H2D"))
         return GBTE_HIP_ERROR;
     while (err > 1e-5) {
@@ -2926,14 +2925,6 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     int nnz = gnnz;
     double invLx = 1.0 / P.deck.Lx;
     int n_ent = (int)h_ent_list.size();
-    std::vector<int> be_slot(ne, -1);
-    for (int ib = 0; ib < nb; ++ib)
-        for (int ic = 0; ic < 2; ++ic) {
-            int c = P.bound_cell[ib * 2 + ic];
-            if (c >= 0) be_slot[ib * 2 + ic] = c * 6 + P.bound_face[ib * 2 + ic];
-        }
-    DBuf<int> d_be_slot;
-    if (!upload(d_be_slot, be_slot)) return GBTE_HIP_ERROR;
     size_t nE = (size_t)nD * nc * nB;
     Rows R{nD * nB, nB, row_tpr, row_rpb, row_tpr / nB, row_tpr % nB};
     Rows Rsrc{nD * nB, nB, 64, kSrcRows, 64 / nB, 64 % nB};  // k_source: one wave per row
@@ -3094,6 +3085,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             if (!check(hipStreamWaitEvent(s2_, e2[0], 0), "hipStreamWaitEvent")) return GBTE_HIP_ERROR;
             (void)hipEventRecord(e2[1], s2_);
             launch_src(s2_, d_src_int.p, n_src_int, 0);
+            (void)hipEventRecord(e2[4], s2_);
             side_pending = true;
         }
         GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m,
@@ -3104,6 +3096,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             hipEvent_t *e2 = ev2_[(nt + 1) & 1];
             (void)hipEventRecord(e2[3], s_);
             if (!check(hipStreamWaitEvent(s2_, e2[3], 0), "hipStreamWaitEvent")) return GBTE_HIP_ERROR;
+            (void)hipEventRecord(e2[5], s2_);
             launch_src(s2_, d_src_bnd.p, n_src_bnd, 0);
             (void)hipEventRecord(e2[2], s2_);
         }
@@ -3168,10 +3161,16 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         macro_total += passes;
         auto add_times = [&]() {  // after a host sync on s_
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);
-            source_ms += ms;
-            if (side_src) {  // completed: this stream waited for it and has been synced
-                (void)hipEventElapsedTime(&ms, ev2_[nt & 1][1], ev2_[nt & 1][2]);
+            if (side_src) {
+                // the two side-stream source launches, each between its own
+                // events (completed: this stream waited for them and has been
+                // synced); the main stream's interval holds k_tpos and the wait
+                (void)hipEventElapsedTime(&ms, ev2_[nt & 1][1], ev2_[nt & 1][4]);
+                source_ms += ms;
+                (void)hipEventElapsedTime(&ms, ev2_[nt & 1][5], ev2_[nt & 1][2]);
+                source_ms += ms;
+            } else {
+                (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);
                 source_ms += ms;
             }
             (void)hipEventElapsedTime(&ms, ev_[1], ev_[2]);

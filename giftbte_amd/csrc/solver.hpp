@@ -72,6 +72,8 @@ class Solver {
     int solve(int max_iter, double tolT, double tolQ, gbte_stats *st);
     int get_fields(double *TL, double *TE, double *qx, double *qy, double *qz, double *Tb);
     int max_levels() const { return nlev; }
+    int n_cells() const { return nc; }
+    int n_bands() const { return nB; }
     int keff(int axis, double dT, double *out);
     // record the synthetic-pass graphs now (the first solve does it
     // otherwise); batched solves do this before their threads start, as a
@@ -115,7 +117,9 @@ class Solver {
     // side stream: the next iteration's interior sources run there, after the
     // face kernel, beside the boundary update and the synthetic passes
     hipStream_t s2_ = nullptr;
-    hipEvent_t ev2_[2][4] = {};  // per launch parity: gradients ready | side sources start | end | boundary values ready
+    // per launch parity: gradients ready | interior sources start | boundary-cell sources end | boundary values
+    // ready | interior sources end | boundary-cell sources start
+    hipEvent_t ev2_[2][6] = {};
     bool single_stream_ = false;
 
     int nc = 0, nb = 0, ne = 0, nn = 0, nB = 0, nD = 0, nf = 0, dim = 2, cpw = 1, nlev = 0;
@@ -153,7 +157,7 @@ class Solver {
     DBuf<double> d_fga, d_fn0x, d_fn0y, d_fmx, d_fmy;
     // boundary entries
     std::vector<int> h_ent_list;
-    DBuf<int> d_ent_list, d_be_cell, d_be_type, d_be_conn, d_be_spec;
+    DBuf<int> d_ent_list, d_be_cell, d_be_type, d_be_conn, d_be_spec, d_be_slot, d_nint;
     DBuf<double> d_be_nx, d_be_ny, d_be_nz, d_be_tw, d_be_dT, d_be_tov;
     // macro / Fourier right-hand side slot data (two sets)
     DBuf<int> d_mbent[2], d_mnonorth[2], d_mv0[2], d_mv1[2];

@@ -35,6 +35,18 @@ class Mesh2D:
 
 
 @dataclass
+class Mesh3D:
+    nodes: np.ndarray          # (N, 3) float64, unit-cube coordinates
+    tets: np.ndarray           # (M, 4) int, positively oriented
+    faces: np.ndarray          # (F, 3) int, boundary triangles
+    face_entity: np.ndarray    # (F,) int, 0-based entity id (BC index = entity + 1)
+
+
+# boundary entity ids of the tetrahedral cube generator (BC index = entity + 1)
+X0, X1, Y0, Y1, Z0, Z1, VOID = 0, 1, 2, 3, 4, 5, 6
+
+
+@dataclass
 class Bands:
     v: np.ndarray
     tau: np.ndarray
@@ -45,7 +57,7 @@ class Bands:
 
 @dataclass
 class DeckSpec:
-    mesh: Mesh2D
+    mesh: "Mesh2D | Mesh3D"
     bands: Bands
     bcs: list                      # [(index, type, value), ...]
     scale: float = 1e-7
@@ -185,6 +197,78 @@ def porous_delaunay_mesh(n_target: int, pore_radius: float, seed: int) -> Mesh2D
     return Mesh2D(P, tri, bedges, ent)
 
 
+def cube_void_tet_mesh(m: int, void: float = 0.4, seed: int = 1, jitter: float = 0.3) -> Mesh3D:
+    """Seeded Delaunay tetrahedra of the unit cube minus a centred cubic void.
+
+    The points are the nodes of an (m+1)^3 lattice outside the void (the void
+    faces sit on the lattice planes nearest to (1 -+ void) / 2), each moved by
+    a seeded jitter of up to ``jitter`` lattice spacings within the planes it
+    lies on: interior nodes in 3D, face nodes within their face, edge nodes
+    along their edge, corners not at all.  All points are tetrahedralised
+    together (scipy's Qhull) and tetrahedra whose centroid lies in the void
+    are dropped.  A Delaunay tetrahedralisation has an acyclic upwind graph
+    for every direction, and dropping cells keeps it so.  Boundary triangles
+    are classified by the cube face they lie on; the rest face the void.
+    """
+    from scipy.spatial import Delaunay
+
+    rng = np.random.default_rng(seed)
+    h = 1.0 / m
+    lo = int(round((1.0 - void) / 2 * m))
+    hi = m - lo
+    g = np.arange(m + 1)
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    ijk = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+    inside = np.all((ijk > lo) & (ijk < hi), axis=1) if hi > lo else np.zeros(len(ijk), bool)
+    ijk = ijk[~inside]
+    # a coordinate may move when it is not on an outer or a void plane
+    on_outer = (ijk == 0) | (ijk == m)
+    in_void_box = np.all((ijk >= lo) & (ijk <= hi), axis=1) if hi > lo else np.zeros(len(ijk), bool)
+    on_void = in_void_box[:, None] & ((ijk == lo) | (ijk == hi))
+    free = ~(on_outer | on_void)
+    P = ijk * h + free * rng.uniform(-jitter, jitter, ijk.shape) * h
+    tet = Delaunay(P).simplices.astype(np.int64)
+    cen = P[tet].mean(axis=1)
+    a, b = lo * h, hi * h
+    if hi > lo:
+        tet = tet[~np.all((cen > a) & (cen < b), axis=1)]
+    used = np.unique(tet)
+    remap = -np.ones(len(P), np.int64)
+    remap[used] = np.arange(len(used))
+    P = P[used]
+    tet = remap[tet]
+    # positive orientation
+    d = np.einsum("ij,ij->i", np.cross(P[tet[:, 1]] - P[tet[:, 0]], P[tet[:, 2]] - P[tet[:, 0]]),
+                  P[tet[:, 3]] - P[tet[:, 0]])
+    neg = d < 0
+    tet[neg] = tet[neg][:, [0, 2, 1, 3]]
+    # Qhull can close a planar patch of the hull (four cocircular points on a
+    # cube face) with a flat tetrahedron; dropping it leaves that patch's
+    # triangles on the face, which the classification below checks
+    tet = tet[np.abs(d) > 1e-9 * h ** 3]
+    used = np.unique(tet)
+    remap = -np.ones(len(P), np.int64)
+    remap[used] = np.arange(len(used))
+    P = P[used]
+    tet = remap[tet]
+    # boundary triangles = faces of exactly one tetrahedron
+    f = np.concatenate([tet[:, [1, 2, 3]], tet[:, [0, 3, 2]], tet[:, [0, 1, 3]], tet[:, [0, 2, 1]]], 0)
+    fs = np.sort(f, 1)
+    _, inv, cnt = np.unique(fs, axis=0, return_inverse=True, return_counts=True)
+    bf = f[cnt[inv.ravel()] == 1]
+    X = P[bf]  # (F, 3 vertices, 3)
+    tol = 1e-12
+    ent = np.full(len(bf), -1, np.int64)
+    for e, (ax, val) in enumerate([(0, 0.0), (0, 1.0), (1, 0.0), (1, 1.0), (2, 0.0), (2, 1.0)]):
+        ent[np.all(np.abs(X[:, :, ax] - val) < tol, axis=1)] = e
+    if hi > lo:
+        in_box = np.all((X > a - tol) & (X < b + tol), axis=(1, 2))
+        ent[(ent < 0) & in_box] = VOID
+    if (ent < 0).any():
+        raise RuntimeError("cube_void_tet_mesh: a boundary triangle lies on neither the cube nor the void")
+    return Mesh3D(P, tet, bf, ent)
+
+
 # ------------------------------------------------------------------ bands
 
 def gray_band(v=6400.0, tau=15.6e-12, C=1.66e6) -> Bands:
@@ -281,11 +365,42 @@ def write_msh(path: str, m: Mesh2D) -> None:
         f.write("\n".join(lines) + "\n")
 
 
+def write_msh3d(path: str, m: Mesh3D) -> None:
+    """Gmsh 4.1 text mesh, 3D, in the same subset as the 2D writer: one node
+    block, one triangle block per boundary entity (entity tag = BC index),
+    one tetrahedron block (entity tag 1 -> region 0)."""
+    n = len(m.nodes)
+    out = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$Nodes", f"1 {n} 1 {n}", f"3 1 0 {n}"]
+    out.append("\n".join(str(i + 1) for i in range(n)))
+    out.append("\n".join(f"{_f(x)} {_f(y)} {_f(z)}" for x, y, z in m.nodes.tolist()))
+    out.append("$EndNodes")
+    ents = sorted(set(m.face_entity.tolist()))
+    nel = len(m.faces) + len(m.tets)
+    out += ["$Elements", f"{len(ents) + 1} {nel} 1 {nel}"]
+    tag = 1
+    for e in ents:
+        sel = m.faces[m.face_entity == e] + 1
+        out.append(f"2 {e + 1} 2 {len(sel)}")
+        out.append("\n".join(f"{tag + i} {a} {b} {c}" for i, (a, b, c) in enumerate(sel.tolist())))
+        tag += len(sel)
+    t = m.tets + 1
+    out.append(f"3 1 4 {len(t)}")
+    out.append("\n".join(f"{tag + i} {a} {b} {c} {d}" for i, (a, b, c, d) in enumerate(t.tolist())))
+    out.append("$EndElements")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
 def write_deck(root: str, spec: DeckSpec, mesh_format: str = "COMSOL") -> str:
-    """Write a complete deck under ``root``; returns ``root``."""
+    """Write a complete deck under ``root``; returns ``root``.  A Mesh3D is
+    written as a Gmsh 4.1 3D mesh (GeometryDimension 3)."""
     inp = os.path.join(root, "input")
     os.makedirs(inp, exist_ok=True)
-    if mesh_format == "MSH":
+    is3d = isinstance(spec.mesh, Mesh3D)
+    if is3d:
+        mesh_format = "MSH"
+        write_msh3d(os.path.join(inp, "mesh.msh"), spec.mesh)
+    elif mesh_format == "MSH":
         write_msh(os.path.join(inp, "mesh.msh"), spec.mesh)
     else:
         write_mphtxt(os.path.join(inp, "mesh.mphtxt"), spec.mesh)
@@ -304,7 +419,8 @@ def write_deck(root: str, spec: DeckSpec, mesh_format: str = "COMSOL") -> str:
                 f"ResidualTemp=\n{_f(spec.tol_temp)}\nResidualFlux=\n{_f(spec.tol_flux)}\n"
                 f"Tref=\n{_f(spec.tref)}\n")
     with open(os.path.join(inp, "GEOMETRY"), "w") as f:
-        f.write(f"GeometryDimension=\n2\nScaleX=\n{_f(spec.scale)}\nScaleY=\n{_f(spec.scale)}\n"
+        f.write(f"GeometryDimension=\n{3 if is3d else 2}\nScaleX=\n{_f(spec.scale)}\nScaleY=\n{_f(spec.scale)}\n"
+                + (f"ScaleZ=\n{_f(spec.scale)}\n" if is3d else "") +
                 f"BCFile=\ninput/bc.dat\nMeshType=\n{mesh_format}\nMeshFile=\n"
                 f"input/mesh.{'msh' if mesh_format == 'MSH' else 'mphtxt'}\n")
         if spec.uniform_heat:
@@ -343,6 +459,22 @@ def porous_spec(n_target=50000, nbands=40, nphi=16, pore_radius=0.3, seed=1, sca
            (LEFT + 1, 1, 301.0), (PORE + 1, 2, 0.0)]
     base = dict(mesh=porous_delaunay_mesh(n_target, pore_radius, seed), bands=debye_bands(nbands, seed),
                 bcs=bcs, scale=scale, material_dim=2, angle_method=1, nphi=nphi, tol_temp=1e-8)
+    base.update(kw)
+    return DeckSpec(**base)
+
+
+def cube_void_spec(m=38, nbands=40, ntheta=2, nphi=6, void=0.4, seed=1, scale=2e-7, **kw) -> DeckSpec:
+    """3D film: unit cube (scaled; 200 nm by default) minus a centred cubic
+    void (0.4 of the edge), seeded Delaunay tetrahedra from an (m+1)^3
+    jittered lattice; x = 0 at 301 K and x = 1 at 299 K (isothermal), the
+    other four faces and the void diffuse; Debye-like bands; Gauss-Legendre
+    3D directions (8 * ntheta * nphi)."""
+    bcs = [(X0 + 1, 1, 301.0), (X1 + 1, 1, 299.0), (Y0 + 1, 2, 0.0), (Y1 + 1, 2, 0.0), (Z0 + 1, 2, 0.0),
+           (Z1 + 1, 2, 0.0)]
+    if void > 0:
+        bcs.append((VOID + 1, 2, 0.0))
+    base = dict(mesh=cube_void_tet_mesh(m, void, seed), bands=debye_bands(nbands, seed), bcs=bcs, scale=scale,
+                material_dim=3, angle_method=2, ntheta=ntheta, nphi=nphi, tol_temp=1e-8)
     base.update(kw)
     return DeckSpec(**base)
 

@@ -175,6 +175,7 @@ struct Oracle {
     double Lx;
     vector<vector<vector<double>>> cellMat;  // [cell][dim][nbr]
     vector<int> cmSet;                       // CellMatrix assigned
+    vector<vector<vector<double>>> faceMat;  // 3D: [face][3][face vertices]
     // BTE state
     vector<double> e;                        // [band][dir][cell]
     vector<double> ebound;                   // [band][dir][bound][2]
@@ -243,6 +244,45 @@ struct Oracle {
             }
         out.assign((size_t)nn * nm, 0.0);
         for (size_t k = 0; k < out.size(); ++k) out[k] = tmp[k] / R[k];
+    }
+
+    // _get_face_matrix (StaticBTESynthetic.cpp:1617-1649), 3D: per unique
+    // face, from its first side, (J^T J)^{-1} J^T of the rows [n; (x_{m+1} -
+    // x_m) / L_x] (Get_inverse_3D), so that FaceMatrix . [0; dv / L_x] is the
+    // tangential gradient of a vertex field.
+    void face_matrix() {
+        faceMat.assign(nf, {});
+        if (dim != 3) return;
+        for (int f = 0; f < nf; ++f) {
+            int i = t.fface_cell[f * 2], j = t.fface_face[f * 2];
+            size_t s = slot(i, j);
+            int nv = t.face_nvert[s];
+            vector<vector<double>> J(nv, vector<double>(3));
+            J[0][0] = t.face_nx[s]; J[0][1] = t.face_ny[s]; J[0][2] = t.face_nz[s];
+            for (int m = 0; m < nv - 1; ++m) {
+                int a = t.face_vert[s * 4 + m], b = t.face_vert[s * 4 + m + 1];
+                J[m + 1][0] = (1.0 / Lx * (t.node_x[b] - t.node_x[a]));
+                J[m + 1][1] = (1.0 / Lx * (t.node_y[b] - t.node_y[a]));
+                J[m + 1][2] = (1.0 / Lx * (t.node_z[b] - t.node_z[a]));
+            }
+            lsq_3d(J, faceMat[f]);
+        }
+    }
+    // FaceMatrix[f] . d1 with d1 = [0, (v[m+1] - v[m]) / L_x], the vertices of
+    // slot s in their order, vertex values vf[vertex * stride + off]
+    void tangential(int f, size_t s, const vector<double> &vf, int stride, int off, double g[3]) const {
+        int nv = t.face_nvert[s];
+        vector<double> d1(nv, 0.0);
+        for (int m = 0; m < nv - 1; ++m) {
+            int a = t.face_vert[s * 4 + m], b = t.face_vert[s * 4 + m + 1];
+            d1[m + 1] = 1.0 / Lx * (vf[(size_t)b * stride + off] - vf[(size_t)a * stride + off]);
+        }
+        g[0] = 0; g[1] = 0; g[2] = 0;
+        for (int m = 0; m < nv; ++m) {
+            g[0] += faceMat[f][0][m] * d1[m];
+            g[1] += faceMat[f][1][m] * d1[m];
+            g[2] += faceMat[f][2][m] * d1[m];
+        }
     }
 
     // ------------------------------------------------ Fourier warm start
@@ -332,8 +372,25 @@ struct Oracle {
                         gx = (Tv[v1] - Tv[v0]) / (mx * mx + my * my) * mx;
                         gy = (Tv[v1] - Tv[v0]) / (mx * mx + my * my) * my;
                     } else {
-                        g_err = "3D Fourier right-hand side is not in the oracle";
-                        return -1;
+                        // Get_gradient_3D on the cell's own face (StaticFourier.cpp:571-593)
+                        int nv = t.face_nvert[s];
+                        vector<vector<double>> J(nv, vector<double>(3)), Pf;
+                        vector<double> d1(nv, 0.0);
+                        J[0][0] = t.face_nx[s]; J[0][1] = t.face_ny[s]; J[0][2] = t.face_nz[s];
+                        for (int m = 0; m < nv - 1; ++m) {
+                            int a = t.face_vert[s * 4 + m], c = t.face_vert[s * 4 + m + 1];
+                            J[m + 1][0] = (1.0 / Lx * (t.node_x[c] - t.node_x[a]));
+                            J[m + 1][1] = (1.0 / Lx * (t.node_y[c] - t.node_y[a]));
+                            J[m + 1][2] = (1.0 / Lx * (t.node_z[c] - t.node_z[a]));
+                            d1[m + 1] = 1.0 / Lx * (Tv[c] - Tv[a]);
+                        }
+                        lsq_3d(J, Pf);
+                        gx = 0; gy = 0; gz = 0;
+                        for (int m = 0; m < nv; ++m) {
+                            gx += Pf[0][m] * d1[m];
+                            gy += Pf[1][m] * d1[m];
+                            gz += Pf[2][m] * d1[m];
+                        }
                     }
                     if (fb == -1) {
                         int nbr = t.face_neighbor[s];
@@ -344,7 +401,7 @@ struct Oracle {
                 }
             banded_solve(K, Re, Tn);
             TF = Tn;
-            // _get_heat_flux (StaticFourier.cpp:615-710), 2D
+            // _get_heat_flux (StaticFourier.cpp:615-710), 2D and 3D
             for (int i = 0; i < nc; ++i) {
                 int nump = 0;
                 for (int j = 0; j < t.face_size[i]; ++j)
@@ -577,10 +634,10 @@ struct Oracle {
         }
     }
 
-    // _get_face_gradient, 2D (StaticBTESynthetic.cpp:1651-1783)
+    // _get_face_gradient (StaticBTESynthetic.cpp:1651-1783); in 3D the
+    // tangential part is FaceMatrix . d1 (:1680-1699, :1751-1770)
     bool face_gradient(int b, int d) {
         const double *ed = E_(b, d);
-        if (dim != 2) { g_err = "3D face gradients are not in the oracle"; return false; }
         for (int f = 0; f < nf; ++f) {
             int fb = t.fface_bound[f];
             if (fb == -1) {
@@ -588,10 +645,17 @@ struct Oracle {
                 size_t s = slot(i, j);
                 double ax = t.cx[i2] - t.cx[i], ay = t.cy[i2] - t.cy[i], az = t.cz[i2] - t.cz[i];
                 double gn = (ed[i2] - ed[i]) * (ax * t.face_nx[s] + ay * t.face_ny[s] + az * t.face_nz[s]) / (ax * ax + ay * ay + az * az);
-                int v0 = t.face_vert[s * 4], v1 = t.face_vert[s * 4 + 1];
-                double mx = t.node_x[v1] - t.node_x[v0], my = t.node_y[v1] - t.node_y[v0];
-                double dv = eV[(size_t)v1 * nm + t.matter[i]] - eV[(size_t)v0 * nm + t.matter[i]];
-                double gx = dv / (mx * mx + my * my) * mx, gy = dv / (mx * mx + my * my) * my, gz = 0;
+                double gx, gy, gz;
+                if (dim == 2) {
+                    int v0 = t.face_vert[s * 4], v1 = t.face_vert[s * 4 + 1];
+                    double mx = t.node_x[v1] - t.node_x[v0], my = t.node_y[v1] - t.node_y[v0];
+                    double dv = eV[(size_t)v1 * nm + t.matter[i]] - eV[(size_t)v0 * nm + t.matter[i]];
+                    gx = dv / (mx * mx + my * my) * mx; gy = dv / (mx * mx + my * my) * my; gz = 0;
+                } else {
+                    double g3[3];
+                    tangential(f, s, eV, nm, t.matter[i], g3);
+                    gx = g3[0]; gy = g3[1]; gz = g3[2];
+                }
                 gx = gn * t.face_nx[s] + gx;
                 gy = gn * t.face_ny[s] + gy;
                 gz = gn * t.face_nz[s] + gz;
@@ -608,12 +672,19 @@ struct Oracle {
                     double ax = t.face_cx[s] - t.cx[ie], ay = t.face_cy[s] - t.cy[ie], az = t.face_cz[s] - t.cz[ie];
                     double ef = EB(b, d, fb, ibc);
                     double gn = (ef - ed[ie]) * (ax * t.face_nx[s] + ay * t.face_ny[s] + az * t.face_nz[s]) / (ax * ax + ay * ay + az * az);
-                    int v0 = t.face_vert[s * 4], v1 = t.face_vert[s * 4 + 1];
-                    double mx = t.node_x[v1] - t.node_x[v0], my = t.node_y[v1] - t.node_y[v0];
                     // the reference indexes the vertex field without the
-                    // matter offset here (StaticBTESynthetic.cpp:1741-1742)
-                    double dv = eV[v1] - eV[v0];
-                    double gx = dv / (mx * mx + my * my) * mx, gy = dv / (mx * mx + my * my) * my, gz = 0;
+                    // matter offset here (StaticBTESynthetic.cpp:1741-1742, 1751-1758)
+                    double gx, gy, gz;
+                    if (dim == 2) {
+                        int v0 = t.face_vert[s * 4], v1 = t.face_vert[s * 4 + 1];
+                        double mx = t.node_x[v1] - t.node_x[v0], my = t.node_y[v1] - t.node_y[v0];
+                        double dv = eV[v1] - eV[v0];
+                        gx = dv / (mx * mx + my * my) * mx; gy = dv / (mx * mx + my * my) * my; gz = 0;
+                    } else {
+                        double g3[3];
+                        tangential(f, s, eV, 1, 0, g3);
+                        gx = g3[0]; gy = g3[1]; gz = g3[2];
+                    }
                     gx = gn * t.face_nx[s] + gx;
                     gy = gn * t.face_ny[s] + gy;
                     gz = gn * t.face_nz[s] + gz;
@@ -780,7 +851,8 @@ struct Oracle {
         return true;
     }
 
-    // _get_Re_macro (StaticBTESynthetic.cpp:2307-2407), 2D
+    // _get_Re_macro (StaticBTESynthetic.cpp:2307-2407); the 3D tangential
+    // gradient comes from the face's first side (:2351-2381)
     bool macro_rhs(vector<double> &Re) {
         for (int ie = 0; ie < nc; ++ie) {
             double r = ReMacro[ie];
@@ -805,12 +877,19 @@ struct Oracle {
                 }
                 double dn = ax * t.face_nx[s] + ay * t.face_ny[s] + az * t.face_nz[s];
                 double a2x = t.face_nx[s] - ax / dn, a2y = t.face_ny[s] - ay / dn, a2z = t.face_nz[s] - az / dn;
-                if (dim != 2) { g_err = "3D macro right-hand side is not in the oracle"; return false; }
-                int v0 = t.face_vert[s * 4], v1 = t.face_vert[s * 4 + 1];
-                double gx = t.node_x[v1] - t.node_x[v0], gy = t.node_y[v1] - t.node_y[v0];
-                double dv = TV[(size_t)v1 * nm + t.matter[ie]] - TV[(size_t)v0 * nm + t.matter[ie]];
-                double tx = dv / (gx * gx + gy * gy) * gx, ty = dv / (gx * gx + gy * gy) * gy;
-                r -= kap * t.face_area[s] * (a2x * tx + a2y * ty + a2z * az);
+                if (dim == 2) {
+                    int v0 = t.face_vert[s * 4], v1 = t.face_vert[s * 4 + 1];
+                    double gx = t.node_x[v1] - t.node_x[v0], gy = t.node_y[v1] - t.node_y[v0];
+                    double dv = TV[(size_t)v1 * nm + t.matter[ie]] - TV[(size_t)v0 * nm + t.matter[ie]];
+                    double tx = dv / (gx * gx + gy * gy) * gx, ty = dv / (gx * gx + gy * gy) * gy;
+                    r -= kap * t.face_area[s] * (a2x * tx + a2y * ty + a2z * az);
+                } else {
+                    int f = t.face_index[s];
+                    int i1 = t.fface_cell[f * 2], j1 = t.fface_face[f * 2];
+                    double g3[3];
+                    tangential(f, slot(i1, j1), TV, nm, t.matter[i1], g3);
+                    r -= kap * t.face_area[s] * (a2x * g3[0] + a2y * g3[1] + a2z * g3[2]);
+                }
             }
             Re[ie] = r;
         }
@@ -870,7 +949,6 @@ struct Oracle {
     // StaticBTESynthetic::solve (StaticBTESyntheticSolver.cpp:12-247)
     int solve(const double *T0, const double *q0x, const double *q0y, const double *q0z, int maxIter, double tolT,
               double tolQ, int *iters, int *status) {
-        if (dim != 2) { g_err = "the oracle covers 2D meshes"; return -1; }
         limiter = t.order == GBTE_ORDER_2LIMIT;
         // Order 1 (solve_firstorder, StaticBTESyntheticSolver.cpp:487-966):
         // the gradients stay zero, so the boundary values are the cell values
@@ -903,6 +981,7 @@ struct Oracle {
             }
         if (!macro_matrix()) return -1;
         cell_matrix();
+        face_matrix();
         for (int d = 0; d < nD; d++)
             for (int b = 0; b < nB; ++b) {
                 if (!first) gradient(b, d);
@@ -1017,6 +1096,7 @@ int oracle_stages(const gbte_tables *t, int band, int dir, double *grad, double 
         for (int d = 0; d < nD; ++d)
             for (int i = 0; i < nc; ++i) o.E_(b, d)[i] = TF[i] * o.cap(t->matter[i], b);
     o.cell_matrix();
+    o.face_matrix();
     for (int d = 0; d < nD; d++)
         for (int b = 0; b < nB; ++b) { o.gradient(b, d); o.bound_ee(b, d); }
     if (!o.set_bound_ee_1()) return -1;
@@ -1063,6 +1143,7 @@ double oracle_transport_sample(const gbte_tables *t, const double *T0, int n_sys
         for (int d = 0; d < nD; ++d)
             for (int i = 0; i < nc; ++i) o.E_(b, d)[i] = o.T[i] * o.cap(t->matter[i], b);
     o.cell_matrix();
+    o.face_matrix();
     // boundary values at equilibrium with the cells (timing only)
     for (int b = 0; b < nB; ++b)
         for (int d = 0; d < nD; ++d)

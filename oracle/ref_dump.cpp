@@ -2,13 +2,13 @@
 //
 // Links against the reference GiftBTE objects built by oracle/ref.mk and
 // replays the steady flow of src/BTEMain.cpp:15-421 at np=1 for the
-// Order=1, 2 or 2limit / IterativeScheme=Synthetic or Sequential / MatrixSolver=LU branches
+// Order=1, 2 or 2limit / IterativeScheme=Synthetic or Sequential / MatrixSolver=LU or BICGSTAB branches
 // (src/Solution/Solution.cpp:262-414), then writes every field the GPU
 // solver is compared against as .npy files at full FP64 precision.  The
 // reference's own .dat outputs carry 6 significant digits only
 // (src/StaticBTESynthetic/StaticBTESynthetic.cpp:2613-2653).
 //
-// usage: ref_dump <deck_dir> <out_dir> <max_iter> <tolT> <tolQ> [stage_band stage_dir]
+// usage: ref_dump <deck_dir> <out_dir> <max_iter> <tolT> <tolQ> [stage_band stage_dir] [--tables DIR]
 //   deck_dir  a COPY of a deck (the reference writes its .dat files into the
 //             working directory on every iteration)
 //   out_dir   where the .npy files go
@@ -16,6 +16,8 @@
 //   tolT/tolQ stop thresholds (0 0 runs exactly max_iter iterations)
 //   stage_*   optional: also dump every per-system stage of iteration 0 for
 //             system (band, direction)
+//   --tables  build the mesh from flattened tables in DIR instead of the
+//             deck's mesh file (3D decks, whose reference reader was removed)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -315,6 +317,176 @@ void stage_dump(StaticBTESynthetic &s, int sb, int sd) {
     dump_i("stage_system", sys, {2});
 }
 
+// ---- 3D: the mesh from flattened tables.  The reference's 3D readers and
+// 3D geometry builder were removed (src/BTEMesh/ReadinMesh.cpp:404-406,
+// 531-533, src/BTEMesh/BTEMesh.cpp:841-843), but its solvers keep their 3D
+// branches; this fills the public members of a BTEMesh
+// (includes/BTEMesh/BTEMesh.h:21-37) from raw little-endian arrays written
+// by the test suite (one <name>.bin per table of include/giftbte.h) and runs
+// the reference's StaticFourier and StaticBTESynthetic on it.
+template <typename T>
+std::vector<T> read_bin(const std::string &dir, const std::string &name, size_t n) {
+    std::vector<T> v(n);
+    std::string path = dir + "/" + name + ".bin";
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) { perror(path.c_str()); exit(2); }
+    if (n && fread(v.data(), sizeof(T), n, f) != n) { fprintf(stderr, "short read: %s\n", path.c_str()); exit(2); }
+    fclose(f);
+    return v;
+}
+
+void mesh_from_tables(const std::string &dir, BTEMesh &m) {
+    std::vector<int> sz = read_bin<int>(dir, "sizes", 5);
+    std::vector<double> sc = read_bin<double>(dir, "scale", 3);
+    size_t nc = sz[0], nn = sz[1], nb = sz[2], nf = sz[3];
+    m.Dimension = sz[4];
+    m.L_x = sc[0]; m.L_y = sc[1]; m.L_z = sc[2];
+    auto fsize = read_bin<int>(dir, "face_size", nc), fnb = read_bin<int>(dir, "face_neighbor", nc * 6),
+         fbd = read_bin<int>(dir, "face_bound", nc * 6), fidx = read_bin<int>(dir, "face_index", nc * 6),
+         fnv = read_bin<int>(dir, "face_nvert", nc * 6), fv = read_bin<int>(dir, "face_vert", nc * 24),
+         cnv = read_bin<int>(dir, "cell_nvert", nc), cv = read_bin<int>(dir, "cell_vert", nc * 8),
+         mat = read_bin<int>(dir, "matter", nc);
+    auto fa = read_bin<double>(dir, "face_area", nc * 6), fnx = read_bin<double>(dir, "face_nx", nc * 6),
+         fny = read_bin<double>(dir, "face_ny", nc * 6), fnz = read_bin<double>(dir, "face_nz", nc * 6),
+         fcx = read_bin<double>(dir, "face_cx", nc * 6), fcy = read_bin<double>(dir, "face_cy", nc * 6),
+         fcz = read_bin<double>(dir, "face_cz", nc * 6), vol = read_bin<double>(dir, "volume", nc),
+         cx = read_bin<double>(dir, "cx", nc), cy = read_bin<double>(dir, "cy", nc), cz = read_bin<double>(dir, "cz", nc),
+         heat = read_bin<double>(dir, "heat_source", nc), t0 = read_bin<double>(dir, "initial_temp", nc);
+    auto nx = read_bin<double>(dir, "node_x", nn), ny = read_bin<double>(dir, "node_y", nn),
+         nz = read_bin<double>(dir, "node_z", nn);
+    auto bc = read_bin<int>(dir, "bound_cell", nb * 2), bf = read_bin<int>(dir, "bound_face", nb * 2),
+         bt = read_bin<int>(dir, "bound_type", nb), bconn = read_bin<int>(dir, "bound_connect", nb);
+    auto btemp = read_bin<double>(dir, "bound_temp", nb);
+    auto ffc = read_bin<int>(dir, "fface_cell", nf * 2), fff = read_bin<int>(dir, "fface_face", nf * 2),
+         ffb = read_bin<int>(dir, "fface_bound", nf);
+    m.Nodes.resize(nn);
+    for (size_t i = 0; i < nn; ++i) { m.Nodes[i].x = nx[i]; m.Nodes[i].y = ny[i]; m.Nodes[i].z = nz[i]; }
+    m.Elements.resize(nc);
+    for (size_t i = 0; i < nc; ++i) {
+        Cell &c = m.Elements[i];
+        c.volume = vol[i];
+        c.index = 0;
+        c.matter = mat[i];
+        c.heat_source = heat[i];
+        c.initial_temperature = t0[i];
+        c.center = Point(cx[i], cy[i], cz[i]);
+        c.vertexes.assign(cv.begin() + i * 8, cv.begin() + i * 8 + cnv[i]);
+        c.faces.resize(fsize[i]);
+        for (int j = 0; j < fsize[i]; ++j) {
+            size_t k = i * 6 + j;
+            Face &f = c.faces[j];
+            f.area = fa[k];
+            f.neighbor = fnb[k];
+            f.bound = fbd[k];
+            f.index = fidx[k];
+            f.norm = vec(fnx[k], fny[k], fnz[k]);
+            f.center = Point(fcx[k], fcy[k], fcz[k]);
+            f.vertexes.assign(fv.begin() + k * 4, fv.begin() + k * 4 + fnv[k]);
+        }
+        for (int v : c.vertexes) m.Nodes[v].cells.push_back((int)i);
+    }
+    m.Boundaries.resize(nb);
+    for (size_t i = 0; i < nb; ++i) {
+        Boundary &b = m.Boundaries[i];
+        b.index = (int)i;
+        b.connection = bconn[i];
+        b.type = bt[i];
+        b.Temperature = btemp[i];
+        for (int k = 0; k < 2; ++k)
+            if (bc[i * 2 + k] >= 0) { b.cellindex.push_back(bc[i * 2 + k]); b.faceindex.push_back(bf[i * 2 + k]); }
+    }
+    m.Element_Faces.resize(nf);
+    for (size_t i = 0; i < nf; ++i) {
+        faces &f = m.Element_Faces[i];
+        f.Boundindex = ffb[i];
+        for (int k = 0; k < 2; ++k)
+            if (ffc[i * 2 + k] >= 0) { f.cellindex.push_back(ffc[i * 2 + k]); f.faceindex.push_back(fff[i * 2 + k]); }
+    }
+}
+
+// The steady flow of main on a mesh built from tables: _set_initial
+// (Solution.cpp:78-124), the Fourier warm start (:236-260) and the BTE solve
+// with its hand-over (:262-414), at np=1.
+int run_from_tables(const Deck &deck, BTEBand *bands, BTEBoundaryCondition *bcs, const std::string &tables,
+                    int maxIter, double tolT, double tolQ, int sb, int sd) {
+    BTEMesh mesh;
+    mesh_from_tables(tables, mesh);
+    BTEAngle *angles = new BTEAngle(deck.ntheta, deck.nphi, deck.dimMat, deck.angleMethod);
+    size_t nc = mesh.Elements.size(), nbd = mesh.Boundaries.size();
+    std::vector<double> T(nc), qx(nc, 0.0), qy(nc, 0.0), qz(nc, 0.0);
+    {
+        StaticFourier fs(&mesh, bcs, bands, 1, 0);
+        for (size_t i = 0; i < nc; ++i) {
+            fs.temperature[i] = mesh.Elements[i].initial_temperature;
+            fs.heatFluxXGlobal[i] = 0; fs.heatFluxYGlobal[i] = 0; fs.heatFluxZGlobal[i] = 0;
+        }
+        for (size_t ib = 0; ib < nbd; ++ib) {
+            const Boundary &b = mesh.Boundaries[ib];
+            fs.boundaryTemp[ib] = 0;
+            fs.boundaryFlux[ib] = 0;
+            if (b.type == 1) fs.boundaryTemp[ib] = b.Temperature;
+        }
+        for (size_t ib = 0; ib < nbd; ++ib) {
+            const Boundary &b = mesh.Boundaries[ib];
+            if (b.type < 0 && b.Temperature != 0) {
+                fs.boundaryTemp[ib] = b.Temperature / 2;
+                fs.boundaryTemp[b.connection] = -b.Temperature / 2;
+            }
+        }
+        fs._solve();
+        for (size_t i = 0; i < nc; ++i) {
+            T[i] = fs.temperature[i];
+            qx[i] = fs.heatFluxXGlobal[i]; qy[i] = fs.heatFluxYGlobal[i]; qz[i] = fs.heatFluxZGlobal[i];
+        }
+    }
+    if (mesh.Dimension != 3) std::fill(qz.begin(), qz.end(), 0.0);
+    dump_d("fourier_T", T);
+    dump_d("fourier_qx", qx);
+    dump_d("fourier_qy", qy);
+    dump_d("fourier_qz", qz);
+    auto hand_over = [&](StaticBTESynthetic &s) {
+        for (size_t i = 0; i < nc; ++i) {
+            s.temperature[i] = T[i];
+            s.heatFluxXGlobal[i] = qx[i]; s.heatFluxYGlobal[i] = qy[i]; s.heatFluxZGlobal[i] = qz[i];
+        }
+    };
+    if (sb >= 0) {
+        StaticBTESynthetic st(&mesh, bcs, bands, angles, 1, 0);
+        hand_over(st);
+        st.error_old = 0;
+        stage_dump(st, sb, sd);
+    }
+    StaticBTESynthetic solver(&mesh, bcs, bands, angles, 1, 0);
+    hand_over(solver);
+    std::fill(solver.ebound, solver.ebound + (size_t)solver.numBand * solver.numDirection * solver.numBound * 2, 0.0);
+    solver.error_old = 0;
+    solver.errorOld = 0;
+    solver.errorOld_heat = 0;
+    dump_tables(solver, "");
+    bool seq = deck.method == "Sequential";
+    if (deck.order == "1") {
+        if (seq) solver.solve_DOM_firstorder(0, maxIter, 0, tolT, tolQ);
+        else solver.solve_firstorder(0, maxIter, 0, tolT, tolQ);
+    } else {
+        int lim = deck.order == "2limit" ? 1 : 0;
+        if (seq) solver.solve_DOM(0, maxIter, lim, tolT, tolQ);
+        else solver.solve(0, maxIter, lim, tolT, tolQ);
+    }
+    size_t nB = solver.numBand, nD = solver.numDirection, nb = solver.numBound;
+    dump_d("T_lattice", solver.temperature, {nc});
+    dump_d("T_energy", solver.totalEnergy, {nc});
+    dump_d("qx", solver.heatFluxXGlobal, {nc});
+    dump_d("qy", solver.heatFluxYGlobal, {nc});
+    dump_d("qz", solver.heatFluxZGlobal, {nc});
+    dump_d("T_band", solver.temperatureband, {nB, nc});
+    dump_d("ebound", solver.ebound, {nB, nD, nb, 2});
+    dump_d("T_old", solver.temperatureOld, {nc});
+    double errs[2] = {solver.error, solver.error_heat};
+    dump_d("final_errors", errs, {2});
+    MPI_Finalize();
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -333,12 +505,23 @@ int main(int argc, char **argv) {
     int maxIter = atoi(argv[3]);
     double tolT = atof(argv[4]), tolQ = atof(argv[5]);
     int sb = -1, sd = -1;
-    if (argc >= 8) { sb = atoi(argv[6]); sd = atoi(argv[7]); }
+    std::string tables;  // --tables DIR: the mesh from flattened tables (3D decks)
+    std::vector<std::string> pos;
+    for (int k = 6; k < argc; ++k) {
+        if (std::string(argv[k]) == "--tables" && k + 1 < argc) {
+            tables = argv[++k];
+            if (tables[0] != '/') tables = std::string(cwd0) + "/" + tables;
+        } else {
+            pos.push_back(argv[k]);
+        }
+    }
+    if (pos.size() >= 2) { sb = atoi(pos[0].c_str()); sd = atoi(pos[1].c_str()); }
 
     Deck deck = parse_deck();
     if ((deck.order != "2" && deck.order != "1" && deck.order != "2limit") ||
-        (deck.method != "Synthetic" && deck.method != "Sequential") || deck.matrix != "LU") {
-        fprintf(stderr, "ref_dump covers Order=1/2/2limit Synthetic / Sequential LU only\n");
+        (deck.method != "Synthetic" && deck.method != "Sequential") ||
+        (deck.matrix != "LU" && deck.matrix != "BICGSTAB")) {
+        fprintf(stderr, "ref_dump covers Order=1/2/2limit Synthetic / Sequential LU / BICGSTAB only\n");
         return 3;
     }
     std::ifstream bandFile("input/PHONON");
@@ -347,6 +530,7 @@ int main(int argc, char **argv) {
     std::ifstream bcFile(deck.bcFile);
     BTEBoundaryCondition *bcs = new BTEBoundaryCondition(bcFile, bands);
     bcFile.close();
+    if (!tables.empty()) return run_from_tables(deck, bands, bcs, tables, maxIter, tolT, tolQ, sb, sd);
     std::ifstream heatfile(deck.heatFile), inittemp(""), geofile(deck.meshFile);
     DistributeMesh *dm = new DistributeMesh(deck.dimGeo, geofile, deck.Lx, deck.Ly, deck.Lz, deck.meshType, bands, bcs,
                                             heatfile, deck.uniformHeat, deck.heatType, inittemp, deck.tref);
@@ -389,13 +573,17 @@ int main(int argc, char **argv) {
     solver.errorOld_heat = 0;
     dump_tables(solver, "");
     // the dispatch of Solution.cpp:287-340
-    bool seq = deck.method == "Sequential";
+    bool seq = deck.method == "Sequential", bicg = deck.matrix == "BICGSTAB";
     if (deck.order == "1") {
-        if (seq) solver.solve_DOM_firstorder(0, maxIter, 0, tolT, tolQ);  // StaticBTESyntheticSolver.cpp:1394
+        if (seq && bicg) solver.solve_DOM_firstorder_Iterative(0, maxIter, 0, tolT, tolQ);  // :1598
+        else if (seq) solver.solve_DOM_firstorder(0, maxIter, 0, tolT, tolQ);  // StaticBTESyntheticSolver.cpp:1394
+        else if (bicg) solver.solve_firstorder_Iterative(0, maxIter, 0, tolT, tolQ);  // :727
         else solver.solve_firstorder(0, maxIter, 0, tolT, tolQ);         // :487
     } else {
         int lim = deck.order == "2limit" ? 1 : 0;  // Use_Limiter (Solution.cpp:363-400)
-        if (seq) solver.solve_DOM(0, maxIter, lim, tolT, tolQ);  // :968
+        if (seq && bicg) solver.solve_DOM_Iterative(0, maxIter, lim, tolT, tolQ);  // :1180
+        else if (seq) solver.solve_DOM(0, maxIter, lim, tolT, tolQ);  // :968
+        else if (bicg) solver.solve_Iterative(0, maxIter, lim, tolT, tolQ);  // :249
         else solver.solve(0, maxIter, lim, tolT, tolQ);
     }
     size_t nc = solver.numCell, nB = solver.numBand, nD = solver.numDirection, nb = solver.numBound;

@@ -45,6 +45,14 @@ DECKS = {
     "square_b20_2limit_seq": (lambda: dataclasses.replace(DECKS["square_b20_diffuse"][0](), order="2limit",
                                                           method="Sequential"),
                               dict(max_iter=5, tol_temp=0.0, tol_flux=0.0)),
+    # MatrixSolver BICGSTAB (solve_Iterative / solve_DOM_Iterative: Eigen
+    # BiCGSTAB per system instead of the LU factor)
+    "porous_b4_bicg": (lambda: dataclasses.replace(decks.porous_spec(n_target=2000, nbands=4, nphi=4, seed=3),
+                                                   matrix="BICGSTAB"),
+                       dict(max_iter=25, tol_temp=0.0, tol_flux=0.0)),
+    "gray12_bicg_seq": (lambda: dataclasses.replace(decks.gray_square_spec(12), method="Sequential",
+                                                    matrix="BICGSTAB"),
+                        dict(max_iter=400, tol_temp=1e-6, tol_flux=1e-4)),
 }
 
 

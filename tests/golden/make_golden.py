@@ -44,6 +44,7 @@ FIELDS = ["fourier_T", "fourier_qx", "fourier_qy", "T_lattice", "T_energy", "qx"
 STAGES = ["stage_system", "stage_ebound_init", "stage_gradX", "stage_gradY", "stage_e_new", "stage_ebound_sys",
           "stage_e_vertex", "stage_gradFaceX", "stage_gradFaceY", "stage_ReMacro", "stage_T_sys", "stage_E_sys",
           "stage_qx_sys", "stage_qy_sys"]
+STAGES3D = ["stage_gradZ", "stage_gradFaceZ", "stage_qz_sys", "fourier_qz"]  # 3D fixtures only
 
 
 def golden_decks():
@@ -83,22 +84,63 @@ def golden_decks():
         "porous_b4_2limit": (limited(decklib.DECKS["porous_b4"][0]), "COMSOL", 25, 0.0, 0.0, (1, 2)),
         "square_b20_2limit_seq": (limited(sequential(decklib.DECKS["square_b20_diffuse"][0])), "COMSOL", 5, 0.0, 0.0,
                                   (3, 5)),
+        # a uniform heat source with cold walls all round (the Q hr tau source
+        # term and the -QV right-hand side of the synthetic solve)
+        "gray_heat": (decklib.DECKS["gray_heat"][0], "COMSOL", 6, 0.0, 0.0, (0, 5)),
+        # 20 bands with diffuse walls, and the 40-band x 64-direction mode shape
+        "square_b20_diffuse": (decklib.DECKS["square_b20_diffuse"][0], "COMSOL", 5, 0.0, 0.0, (7, 9)),
+        "porous_b40": (decklib.DECKS["porous_b40"][0], "COMSOL", 4, 0.0, 0.0, (23, 41)),
+        # MatrixSolver BICGSTAB (solve_Iterative, solve_DOM_Iterative)
+        "porous_b4_bicg": (decklib.DECKS["porous_b4_bicg"][0], "COMSOL", 25, 0.0, 0.0, (1, 2)),
+        "gray12_bicg_seq": (decklib.DECKS["gray12_bicg_seq"][0], "COMSOL", 400, 1e-6, 1e-4, (0, 3)),
+        # 3D: tetrahedra of a cube with a cubic void, Gauss 3D directions; the
+        # reference's solvers run on the tables (ref_dump --tables)
+        "cube3d_b4": (lambda: decks.cube_void_spec(m=5, nbands=4, ntheta=1, nphi=2, seed=2), "MSH", 6, 0.0, 0.0,
+                      (2, 5)),
+        "cube3d_gray_seq": (lambda: dataclasses.replace(
+            decks.cube_void_spec(m=4, nbands=1, ntheta=1, nphi=2, void=0.0, seed=4), bands=decks.gray_band(),
+            method="Sequential"), "MSH", 300, 1e-6, 1e-4, (0, 7)),
         # band data binned from ShengBTE / ALAMODE mode files (seeded, small)
         "sheng_gray6": (banded("ShengBTE", decks.shengbte_writer(3)), "COMSOL", 5, 0.0, 0.0, (2, 3)),
         "alamode_gray6": (banded("ALAMODE", decks.alamode_writer(4)), "COMSOL", 5, 0.0, 0.0, (1, 5)),
     }
 
 
+# tables a 3D run hands to ref_dump (--tables): the reference's 3D mesh
+# reader and geometry were removed, so its solvers run on ours
+TABLE_BINS = ["face_size", "face_neighbor", "face_bound", "face_index", "face_nvert", "face_vert", "cell_nvert",
+              "cell_vert", "matter", "face_area", "face_nx", "face_ny", "face_nz", "face_cx", "face_cy", "face_cz",
+              "volume", "cx", "cy", "cz", "heat_source", "initial_temp", "node_x", "node_y", "node_z", "bound_cell",
+              "bound_face", "bound_type", "bound_connect", "bound_temp", "fface_cell", "fface_face", "fface_bound"]
+
+
+def write_tables(problem, out_dir):
+    """The problem's flattened tables as raw little-endian arrays."""
+    os.makedirs(out_dir, exist_ok=True)
+    t = problem.tables
+    for name in TABLE_BINS:
+        a = problem.array(name)
+        a = a.astype("<i4" if a.dtype.kind == "i" else "<f8")
+        a.tofile(os.path.join(out_dir, name + ".bin"))
+    np.array([t.n_cells, t.n_nodes, t.n_bounds, t.n_faces, t.dim], "<i4").tofile(os.path.join(out_dir, "sizes.bin"))
+    np.array([t.Lx, t.Ly, t.Lz], "<f8").tofile(os.path.join(out_dir, "scale.bin"))
+
+
 def run_one(name, factory, fmt, max_iter, tolT, tolQ, stage):
-    from giftbte_amd import decks
+    from giftbte_amd import decks, Problem
     td = tempfile.mkdtemp()
     try:
-        deck = decks.write_deck(os.path.join(td, name), factory(), mesh_format=fmt)
+        spec = factory()
+        deck = decks.write_deck(os.path.join(td, name), spec, mesh_format=fmt)
         out = os.path.join(td, "out")
+        extra = []
+        if isinstance(spec.mesh, decks.Mesh3D):
+            write_tables(Problem(deck), os.path.join(td, "tables"))
+            extra = ["--tables", os.path.join(td, "tables")]
         log = subprocess.run([REF_DUMP, deck, out, str(max_iter), repr(tolT), repr(tolQ), str(stage[0]),
-                              str(stage[1])], cwd=deck, capture_output=True, text=True, check=True).stdout
+                              str(stage[1])] + extra, cwd=deck, capture_output=True, text=True, check=True).stdout
         data = {}
-        for k in TABLES + FIELDS + STAGES + OPTIONAL:
+        for k in TABLES + FIELDS + STAGES + OPTIONAL + (STAGES3D if extra else []):
             data[k] = np.load(os.path.join(out, k + ".npy"))
         if name in OUTPUT_DECKS:
             for fn in OUTPUT_FILES:

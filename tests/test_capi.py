@@ -127,3 +127,20 @@ def test_batch_argument_errors():
     hs = (C.c_void_p * 2)(None, None)
     assert lib.gbte_solve_batch(hs, 0, one, tol, tol, None, None) == _lib.GBTE_BAD_INPUT
     assert lib.gbte_solve_batch(hs, 2, one, tol, tol, None, None) == _lib.GBTE_BAD_INPUT
+
+
+def test_solve_batch_checks_every_handle_first():
+    """A null handle in a batch: nothing runs and the status entry is filled
+    (repeated handles are refused as a bad argument before that)."""
+    lib = _lib.load()
+    hs = (C.c_void_p * 1)(None)
+    mi = (C.c_int * 1)(1)
+    tt = (C.c_double * 1)(0.0)
+    st = (_lib.Stats * 1)()
+    status = (C.c_int * 1)(-7)
+    rc = lib.gbte_solve_batch(hs, 1, mi, tt, tt, st, status)
+    assert rc == _lib.GBTE_HIP_ERROR
+    assert list(status) == [_lib.GBTE_HIP_ERROR]
+    hs2 = (C.c_void_p * 2)(None, None)
+    assert lib.gbte_solve_batch(hs2, 2, (C.c_int * 2)(1, 1), (C.c_double * 2)(0, 0), (C.c_double * 2)(0, 0),
+                                (_lib.Stats * 2)(), (C.c_int * 2)()) == _lib.GBTE_BAD_INPUT

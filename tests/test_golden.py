@@ -112,12 +112,13 @@ def test_oracle_run_matches_reference(loaded, name):
     assert info["iterations"] == int(g["count_iterations"])
     assert info["macro_passes"] == int(g["count_macro_passes"])
     assert info["fourier_passes"] == int(g["count_fourier_passes"])
-    assert _oracle.rel_l2(f["T_lattice"], g["T_lattice"]) < 1e-12
-    assert _oracle.rel_l2(f["T_energy"], g["T_energy"]) < 1e-12
-    assert _oracle.rel_l2(f["T_band"], g["T_band"]) < 1e-12
+    # measured on the fixtures: T, E, T_band <= 4e-14 and q <= 1.1e-11
+    assert _oracle.rel_l2(f["T_lattice"], g["T_lattice"]) < 2e-13
+    assert _oracle.rel_l2(f["T_energy"], g["T_energy"]) < 2e-13
+    assert _oracle.rel_l2(f["T_band"], g["T_band"]) < 2e-13
     q = np.concatenate([f["qx"], f["qy"]])
     qr = np.concatenate([g["qx"], g["qy"]])
-    assert _oracle.rel_l2(q, qr) < 1e-9
+    assert _oracle.rel_l2(q, qr) < 1e-10
     assert np.allclose([info["err_temp"], info["err_flux"]], g["final_errors"], rtol=1e-6, atol=0)
 
 
@@ -128,7 +129,8 @@ def test_oracle_fourier_matches_reference(loaded, name):
     assert passes == int(g["count_fourier_passes"])
     assert _oracle.rel_l2(f["T"], g["fourier_T"]) < 1e-13
     q = np.concatenate([f["qx"], f["qy"]])
-    assert _oracle.rel_l2(q, np.concatenate([g["fourier_qx"], g["fourier_qy"]])) < 1e-8
+    # measured: <= 1.3e-11
+    assert _oracle.rel_l2(q, np.concatenate([g["fourier_qx"], g["fourier_qy"]])) < 1e-10
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -145,8 +147,9 @@ def test_oracle_stages_match_reference(loaded, name):
     valid = np.broadcast_to((g["boundaryCell"] >= 0)[None, None], g["stage_ebound_init"].shape)
     assert _oracle.rel_l2(s["ebound_init"][valid.ravel()], g["stage_ebound_init"][valid]) < 1e-13
     grad = s["grad"].reshape(3, nc)
-    gs = np.linalg.norm(np.concatenate([g["stage_gradX"], g["stage_gradY"]]))
-    assert np.linalg.norm(np.concatenate([grad[0] - g["stage_gradX"], grad[1] - g["stage_gradY"]])) <= 1e-9 * gs + 1e-300
+    comps = ["X", "Y", "Z"][:t.dim]
+    gs = np.linalg.norm(np.concatenate([g["stage_grad" + c] for c in comps]))
+    assert np.linalg.norm(np.concatenate([grad[k] - g["stage_grad" + c] for k, c in enumerate(comps)])) <= 1e-9 * gs + 1e-300
     assert _oracle.rel_l2(s["e_new"], g["stage_e_new"]) < 1e-13
     vs = (g["boundaryCell"] >= 0).ravel()
     assert _oracle.rel_l2(s["ebound_sys"][vs], g["stage_ebound_sys"].ravel()[vs]) < 1e-13
@@ -159,7 +162,7 @@ def test_oracle_stages_match_reference(loaded, name):
     skip[bound >= 0] = btype[bound[bound >= 0]] == 3
     # boundary faces have one side only; the second slot is never written
     keep = np.stack([~skip, bound < 0], axis=1).ravel()
-    ours = np.concatenate([gf[0][keep], gf[1][keep]])
-    ref_gf = np.concatenate([g["stage_gradFaceX"].ravel()[keep], g["stage_gradFaceY"].ravel()[keep]])
+    ours = np.concatenate([gf[k][keep] for k in range(t.dim)])
+    ref_gf = np.concatenate([g["stage_gradFace" + c].ravel()[keep] for c in comps])
     assert _oracle.rel_l2(ours, ref_gf) < 1e-8
     assert _oracle.rel_l2(s["remacro"], g["stage_ReMacro"]) < 1e-8

@@ -46,11 +46,13 @@ def test_fields_match_oracle(name, variant, deck_root):
     assert st.iterations == info["iterations"]
     assert st.macro_passes == info["macro_passes"]
     f = sol.fields()
-    for k in ("T_lattice", "T_energy", "T_band"):
-        assert _oracle.rel_l2(f[k], ref[k]) < FIELD_TOL, k
+    errs = {k: _oracle.rel_l2(f[k], ref[k]) for k in ("T_lattice", "T_energy", "T_band")}
     q = np.concatenate([f["qx"], f["qy"], f["qz"]])
     qr = np.concatenate([ref["qx"], ref["qy"], ref["qz"]])
-    assert _oracle.rel_l2(q, qr) < FIELD_TOL
+    errs["q"] = _oracle.rel_l2(q, qr)
+    print("  rel errors vs oracle: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for k, v in errs.items():
+        assert v < FIELD_TOL, k
     # k_eff along x with the deck's 2 K drop where it has one
     t = p.tables
     bt = p.array("bound_type")
@@ -60,6 +62,7 @@ def test_fields_match_oracle(name, variant, deck_root):
         vol = p.array("volume")
         ys = p.array("node_y")
         k_ref = float((ref["qx"] * vol).sum() / ((ys.max() - ys.min()) * dT))
+        print(f"  k_eff gpu {k_gpu:.12g} oracle {k_ref:.12g} rel {abs(k_gpu - k_ref) / abs(k_ref):.2e}", flush=True)
         assert abs(k_gpu - k_ref) <= KEFF_TOL * abs(k_ref)
     del t
 
