@@ -22,11 +22,42 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <thread>
 
 namespace gbte {
 namespace {
 
 constexpr int kLeaf = 96;
+constexpr int kPanel = 32;  // pivot columns per trailing update of a front
+
+// body(j) for j in [lo, hi) on the host's cores when the work is large
+// (columns are independent); the split balances sum (len - j).
+template <typename Body>
+void parallel_cols(int lo, int hi, int len, Body body) {
+    long work = 0;
+    for (int j = lo; j < hi; ++j) work += len - j;
+    unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    int nth = (int)std::min<long>(hw, std::max<long>(1, work / 200000));
+    if (nth <= 1 || hi - lo < 2 * nth) {
+        for (int j = lo; j < hi; ++j) body(j);
+        return;
+    }
+    std::vector<int> cut(nth + 1, hi);
+    cut[0] = lo;
+    long acc = 0, per = (work + nth - 1) / nth;
+    int t = 1;
+    for (int j = lo; j < hi && t < nth; ++j) {
+        acc += len - j;
+        if (acc >= per * t) cut[t++] = j + 1;
+    }
+    std::vector<std::thread> th;
+    for (int q = 0; q < nth; ++q)
+        if (cut[q] < cut[q + 1])
+            th.emplace_back([&, q]() {
+                for (int j = cut[q]; j < cut[q + 1]; ++j) body(j);
+            });
+    for (auto &x : th) x.join();
+}
 
 struct Dissect {
     const std::vector<std::vector<std::pair<int, double>>> &A;
@@ -161,11 +192,12 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
             if (pp < hi) return pp - c0;
             return nc_ + (int)(std::lower_bound(rows[k].begin(), rows[k].end(), pp) - rows[k].begin());
         };
-        std::vector<double> Fm((size_t)m * m, 0.0);
+        // the front, lower triangle column-major: element (i, j), i >= j, at C[j * m + i]
+        std::vector<double> C((size_t)m * m, 0.0);
         for (int a = 0; a < nc_; ++a)
             for (auto &e : A[F.perm[c0 + a]]) {
                 int pp = pos[e.first];
-                if (pp >= c0 + a) Fm[(size_t)loc(pp) * m + a] += e.second;
+                if (pp >= c0 + a) C[(size_t)a * m + loc(pp)] += e.second;
             }
         for (int q = F.child_ptr[k]; q < F.child_ptr[k + 1]; ++q) {
             int c = F.child_idx[q];
@@ -178,52 +210,80 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
             for (int i = 0; i < nrc; ++i)
                 for (int j = 0; j <= i; ++j) {
                     int li = mp[i], lj = mp[j];
-                    if (li >= lj) Fm[(size_t)li * m + lj] += Uc[(size_t)i * nrc + j];
-                    else Fm[(size_t)lj * m + li] += Uc[(size_t)i * nrc + j];
+                    if (li >= lj) C[(size_t)lj * m + li] += Uc[(size_t)i * nrc + j];
+                    else C[(size_t)li * m + lj] += Uc[(size_t)i * nrc + j];
                 }
             std::vector<double>().swap(U[c]);
         }
-        // partial dense Cholesky of the first nc_ columns
-        for (int a = 0; a < nc_; ++a) {
-            double d = Fm[(size_t)a * m + a];
-            if (!(d > 0) || !std::isfinite(d)) return "synthetic/Fourier matrix is not positive definite";
-            double l = std::sqrt(d);
-            Fm[(size_t)a * m + a] = l;
-            for (int i = a + 1; i < m; ++i) Fm[(size_t)i * m + a] /= l;
-            for (int j = a + 1; j < m; ++j) {
-                double lj = Fm[(size_t)j * m + a];
-                if (lj == 0) continue;
-                for (int i = j; i < m; ++i) Fm[(size_t)i * m + j] -= Fm[(size_t)i * m + a] * lj;
+        // partial dense Cholesky of the first nc_ columns, right-looking in
+        // panels of kPanel columns; every entry is updated by the pivot
+        // columns in increasing order, as in the unblocked form
+        for (int a0 = 0; a0 < nc_; a0 += kPanel) {
+            int a1 = std::min(nc_, a0 + kPanel);
+            for (int a = a0; a < a1; ++a) {
+                double *ca = &C[(size_t)a * m];
+                double d = ca[a];
+                if (!(d > 0) || !std::isfinite(d)) return "synthetic/Fourier matrix is not positive definite";
+                double l = std::sqrt(d);
+                ca[a] = l;
+                for (int i = a + 1; i < m; ++i) ca[i] /= l;
+                for (int j = a + 1; j < a1; ++j) {
+                    double lj = ca[j];
+                    if (lj == 0) continue;
+                    double *cj = &C[(size_t)j * m];
+                    for (int i = j; i < m; ++i) cj[i] -= ca[i] * lj;
+                }
             }
+            parallel_cols(a1, m, m, [&](int j) {
+                double *cj = &C[(size_t)j * m];
+                for (int a = a0; a < a1; ++a) {
+                    const double *ca = &C[(size_t)a * m];
+                    double lj = ca[j];
+                    if (lj == 0) continue;
+                    for (int i = j; i < m; ++i) cj[i] -= ca[i] * lj;
+                }
+            });
         }
-        // inverse of L_kk (lower triangular)
+        // inverse of L_kk (lower triangular), column-major: Li[a * nc_ + i];
+        // column a by forward substitution, the terms of each entry in
+        // increasing order
         Li.assign((size_t)nc_ * nc_, 0.0);
-        for (int col = 0; col < nc_; ++col) {
-            Li[(size_t)col * nc_ + col] = 1.0 / Fm[(size_t)col * m + col];
-            for (int i = col + 1; i < nc_; ++i) {
-                double s = 0;
-                for (int j = col; j < i; ++j) s += Fm[(size_t)i * m + j] * Li[(size_t)j * nc_ + col];
-                Li[(size_t)i * nc_ + col] = -s / Fm[(size_t)i * m + i];
+        parallel_cols(0, nc_, nc_, [&](int col) {
+            double *x = &Li[(size_t)col * nc_];
+            x[col] = 1.0;
+            for (int j = col; j < nc_; ++j) {
+                const double *cj = &C[(size_t)j * m];
+                if (j == col) {
+                    x[j] = 1.0 / cj[j];
+                } else {
+                    x[j] = x[j] / cj[j];
+                }
+                double xj = x[j];
+                for (int i = j + 1; i < nc_; ++i) x[i] -= cj[i] * xj;
             }
-        }
+        });
         // F = [Li; -L_A Li]
         double *Fr = F.frm.data() + F.f_off[k];
         for (int i = 0; i < nc_; ++i)
-            for (int a = 0; a <= i; ++a) Fr[(size_t)i * nc_ + a] = Li[(size_t)i * nc_ + a];
-        for (int r = 0; r < nr; ++r) {
-            const double *la = &Fm[(size_t)(nc_ + r) * m];
-            for (int a = 0; a < nc_; ++a) {
-                double s = 0;
-                for (int j = a; j < nc_; ++j) s += la[j] * Li[(size_t)j * nc_ + a];
-                Fr[(size_t)(nc_ + r) * nc_ + a] = -s;
+            for (int a = 0; a <= i; ++a) Fr[(size_t)i * nc_ + a] = Li[(size_t)a * nc_ + i];
+        parallel_cols(0, nc_, nc_, [&](int a) {
+            std::vector<double> acc(nr, 0.0);
+            const double *la = &Li[(size_t)a * nc_];
+            for (int j = a; j < nc_; ++j) {
+                double lv = la[j];
+                const double *cj = &C[(size_t)j * m + nc_];
+                for (int r = 0; r < nr; ++r) acc[r] += cj[r] * lv;
             }
-        }
+            for (int r = 0; r < nr; ++r) Fr[(size_t)(nc_ + r) * nc_ + a] = -acc[r];
+        });
         double *Fc = F.fcm.data() + F.f_off[k];
         for (int i = 0; i < m; ++i)
             for (int a = 0; a < nc_; ++a) Fc[(size_t)a * m + i] = Fr[(size_t)i * nc_ + a];
         U[k].assign((size_t)nr * nr, 0.0);
-        for (int i = 0; i < nr; ++i)
-            for (int j = 0; j <= i; ++j) U[k][(size_t)i * nr + j] = Fm[(size_t)(nc_ + i) * m + nc_ + j];
+        for (int j = 0; j < nr; ++j) {
+            const double *cj = &C[(size_t)(nc_ + j) * m + nc_];
+            for (int i = j; i < nr; ++i) U[k][(size_t)i * nr + j] = cj[i];
+        }
     }
     // map offsets per node (as child of its parent)
     for (int k = 0; k < nt; ++k) F.map_off[k] = map_of_node[k] < 0 ? 0 : map_of_node[k];

@@ -27,6 +27,7 @@
 // same neighbours for every band and direction, so a workgroup owns one row and
 // reads whole neighbour rows with coalesced loads.  The sweep reads and writes
 // the 8*n_bands-byte (cell, dir) segments of a row.
+#include "devcommon.hpp"
 #include "solver.hpp"
 #include "tiles.hpp"
 
@@ -37,77 +38,11 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <thread>
 
 namespace gbte {
 
 namespace {
-
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
-constexpr int kSweepBlock = 1024;
-constexpr int kMaxIn = 3;  // in-faces per cell of one direction (triangles, quads, tetrahedra)
-constexpr int kNT = 8;     // row entries per thread in the row kernels
-constexpr int kNTf = 4;    // the same for the register-heavy face / source kernels
-constexpr int kRowBlockMax = 512;
-constexpr int kStencilBatch = 4;  // LSQ stencil entries loaded together
-constexpr int kNJ = 4;            // row entries per thread loaded together
-
-__device__ __forceinline__ int wave_global() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); }
-__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
-
-// Sum over the nB lanes of one cell group; valid in the group's first lane.
-__device__ __forceinline__ double group_sum(double v, int b, int nB) {
-    for (int off = 32; off >= 1; off >>= 1) {
-        double o = __shfl_down(v, off, 64);
-        if (off < nB && b + off < nB) v += o;
-    }
-    return v;
-}
-
-// Rows of n_dirs*n_bands values are processed by tpr threads (a power of two,
-// tpr * rpb = kBlock); row_sum reduces over the tpr threads of a row in a
-// fixed order and returns the sum to every thread of the row.
-__device__ __forceinline__ double row_sum(double v, int tpr, double *sh) {
-    int w = threadIdx.x >> 6;
-    int lim = tpr < 64 ? tpr : 64;
-    for (int off = lim >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (tpr <= 64) return v;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
-    __syncthreads();
-    int wpr = tpr >> 6, w0 = (w / wpr) * wpr;
-    double s = 0;
-    for (int k = 0; k < wpr; ++k) s += sh[w0 + k];
-    return s;
-}
-
-// Sum over the 64 lanes of a wave (butterfly, fixed order), in every lane.
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Rows are Morton-ordered, so giving every XCD one contiguous range of
-// rows keeps neighbouring rows - which every stencil re-reads - in one L2.
-constexpr int kXcd = 8;
-// The grid must be a multiple of kXcd for this to be a bijection.
-__device__ __forceinline__ int xcd_block() {
-    int nb = gridDim.x, per = nb / kXcd;
-    return (int)(blockIdx.x % kXcd) * per + (int)(blockIdx.x / kXcd);
-}
-
-struct Modes {
-    const double *vg, *tau, *cap, *lr, *hr, *sx, *sy, *sz, *w;
-    const double *lrc;  // lr / cap per band
-    double kappa, cbulk;
-    // Energy densities are held as deviations e - C_b eref from the
-    // equilibrium at eref (the deck's Tref): the transport equations are
-    // linear and every face sum of (A/V)(s.n) vanishes, so only the sources,
-    // the thermalising walls and the moments see the shift.  off* are the
-    // moments of C_b eref, added back to T, E, T_band, q and T_mac.
-    double eref, offT, offE, offW, offQx, offQy, offQz, offMac;
-};
 
 // Storage of the per-system fields (e, its gradient, boundary values, sweep
 // sources): ST = double, or float for GBTE_FP32_SWEEP; arithmetic is FP64.
@@ -710,7 +645,6 @@ constexpr int kPcFar = 16;      // far upwind reads per sub-level (16 producer l
 constexpr int kPcBc = 10;       // bands per workgroup at most
 constexpr int kPcBpl = 5;       // bands per producer lane (kPcBc / 2)
 constexpr int kPcDepth = 4;     // sub-levels in flight
-constexpr int kPcMaxSub = 2047; // sub-levels per direction (LDS tables)
 // staging buffer, in doubles (16-byte aligned sections, read as 16-byte
 // vectors): S[kPcCap][Bc] | g0,g1,g2,G [kPcCap] | far[kPcFar][Bc] | ints
 // cell,q0,q1,q2 [kPcCap] | T_old - Tref [kPcCap] (when the sources leave out
@@ -1802,7 +1736,6 @@ __global__ void k_fourier_flux(const double *__restrict__ T, const int *__restri
     if (dim == 3) qz[c] = -kappa * gz;
 }
 
-inline int blocks_for(long n, int per_block) { return (int)((n + per_block - 1) / per_block); }
 
 }  // namespace
 
@@ -1816,20 +1749,6 @@ bool Solver::check(hipError_t e, const char *what) {
     return false;
 }
 
-template <typename T>
-bool Solver::alloc(DBuf<T> &b, size_t n) {
-    b.release();
-    b.n = n;
-    if (n == 0) return true;
-    return check(hipMalloc((void **)&b.p, n * sizeof(T)), "hipMalloc");
-}
-
-template <typename T>
-bool Solver::upload(DBuf<T> &b, const std::vector<T> &h) {
-    if (!alloc(b, h.size())) return false;
-    if (h.empty()) return true;
-    return check(hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
-}
 
 Solver::Solver(const Problem &p, const gbte_options &o) : Pp(renumber(p, cell_new_, node_new_)), P(Pp), opt(o) {
     fp32_ = opt.precision == GBTE_FP32_SWEEP;
@@ -1869,7 +1788,24 @@ Solver::~Solver() {
     if (s_) (void)hipStreamDestroy(s_);
 }
 
-bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m, DevND &f) {
+// The rows of -K with duplicate entries merged (periodic or repeated faces),
+// in column order: the matrix the factorization and the PCG take.
+static std::vector<std::vector<std::pair<int, double>>> negated_rows(
+    const std::vector<std::vector<std::pair<int, double>>> &rows) {
+    std::vector<std::vector<std::pair<int, double>>> merged(rows.size());
+    for (size_t i = 0; i < rows.size(); ++i) {
+        std::vector<std::pair<int, double>> r = rows[i];
+        std::sort(r.begin(), r.end(), [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; });
+        for (auto &e : r) {
+            if (!merged[i].empty() && merged[i].back().first == e.first) merged[i].back().second += -e.second;
+            else merged[i].push_back({e.first, -e.second});
+        }
+    }
+    return merged;
+}
+
+bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m, DevND &f,
+                          NDFactor *pre, const std::string *pre_err) {
     int n = (int)rows.size();
     std::vector<int> ptr(n + 1, 0), col;
     std::vector<double> val, dinv(n, 0.0);
@@ -1917,9 +1853,10 @@ bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>>
     m.n = n;
     if (!upload(m.ptr, ptr) || !upload(m.col, col) || !upload(m.val, val) || !upload(m.dinv, dinv)) return false;
     if (opt.linear_solver != GBTE_LINSOLVE_DIRECT) return true;
-    NDFactor F;
-    std::string e = nd_factor(merged, P.cx, P.cy, P.cz, dim, F);
+    NDFactor F0;
+    std::string e = pre ? *pre_err : nd_factor(merged, P.cx, P.cy, P.cz, dim, F0);
     if (!e.empty()) { err_ = e; return false; }
+    NDFactor &F = pre ? *pre : F0;
     // LDS of the solve kernels: one vector of the largest front
     if ((size_t)F.max_front * sizeof(double) > 60 * 1024) {
         err_ = "nested-dissection front too large for the direct solve (use the PCG linear solver)";
@@ -1962,7 +1899,11 @@ bool Solver::build() {
     nc = p.n_cells; nb = p.n_bounds; ne = 2 * nb; nn = p.n_nodes; nB = p.n_bands; nD = p.n_dirs; nf = p.n_faces;
     dim = p.dim;
     if (p.n_matters != 1) { err_ = "only single-material problems are supported"; return false; }
-    if (dim != 2) { err_ = "only 2D meshes are supported by this solver build"; return false; }
+    if (dim != 2 && dim != 3) { err_ = "2D and 3D meshes only"; return false; }
+    if (dim == 3 && p.deck.order == "2limit") {
+        err_ = "Order 2limit is not supported on 3D meshes (the limiter runs on the 2D kernels only)";
+        return false;
+    }
     if (nB < 1 || nB > 64) { err_ = "the number of bands must be between 1 and 64"; return false; }
     for (int ib = 0; ib < nb; ++ib)
         if (p.bound_type[ib] == 4 || p.bound_type[ib] == 5) { err_ = "interface boundaries (type 4/5) are not supported"; return false; }
@@ -2050,7 +1991,7 @@ bool Solver::build() {
                 return false;
             }
         }
-    {
+    if (dim == 2) {
         std::vector<double> srec((size_t)nc * kSrcD, 0.0);
         std::vector<int> srei((size_t)nc * kSrcI, 0), slim((size_t)nc * kSrcI, kLimNone);
         for (int c = 0; c < nc; ++c) {
@@ -2226,7 +2167,6 @@ bool Solver::build() {
                     touched.clear();
                 }
                 int Ls = (int)sub.size() - 1;
-                if (Ls > kPcMaxSub) { fits = false; break; }
                 for (int m = 0; m < Ls; ++m) {
                     psl.push_back(sub[m]);
                     pfp.push_back((int)pfar.size());
@@ -2284,7 +2224,10 @@ bool Solver::build() {
                 }
             }
             pfar.push_back(0);  // padding read by clamped loads
-            size_t lds = ((size_t)pc_ring * pc_bc + 2 * kPcStage) * sizeof(double) + 2 * (kPcMaxSub + 1) * sizeof(int);
+            int maxsub0 = 0;
+            for (int d = 0; d < nD; ++d) maxsub0 = std::max(maxsub0, pdsl[d + 1] - pdsl[d] - 1);
+            // the kernel copies one direction's sub-level tables to LDS
+            size_t lds = ((size_t)pc_ring * pc_bc + 2 * kPcStage) * sizeof(double) + 2 * (size_t)(maxsub0 + 1) * sizeof(int);
             int dev_lds = 0;
             (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, opt.device);
             if (fits && lds <= (size_t)dev_lds) {
@@ -2299,8 +2242,8 @@ bool Solver::build() {
         }
     }
     // ---- LSQ gradient stencil (_get_CellMatrix_larger + Get_inverse_2D,
-    // StaticBTESynthetic.cpp:1000-1064, utility.cpp:401-466)
-    {
+    // StaticBTESynthetic.cpp:1000-1064, utility.cpp:401-466); 3D: build3d
+    if (dim == 2) {
         int nnz = p.nbr_ptr[nc];
         std::vector<double> gP((size_t)nnz * dim, 0.0);
         std::vector<int> gok(nc, 0);
@@ -2397,8 +2340,8 @@ bool Solver::build() {
             !upload(d_tsrc, tsrc) || !upload(d_tdist, tdist))
             return false;
     }
-    // ---- face-gradient slot data
-    {
+    // ---- face-gradient slot data (2D; build3d holds the 3D face terms)
+    if (dim == 2) {
         std::vector<int> kind(ns, 0), other(ns, 0), fv0(ns, 0), fv1(ns, 0);
         std::vector<double> an(ns, 1), aa(ns, 1), n0x(ns, 0), n0y(ns, 0), n0z(ns, 0), mx(ns, 0), my(ns, 0), m2(ns, 1);
         for (int c = 0; c < nc; ++c)
@@ -2575,10 +2518,21 @@ bool Solver::build() {
                 return false;
         }
         if (!upload(d_fourier_bval, fbval)) return false;
-        if (!build_matrix(rowsM, Kmacro, NDmacro) || !build_matrix(rowsF, Kfour, NDfour)) return false;
+        // the two factorizations are independent host work: the Fourier one
+        // runs on a second host thread
+        NDFactor Ff;
+        std::string ef;
+        std::thread th;
+        bool direct = opt.linear_solver == GBTE_LINSOLVE_DIRECT;
+        if (direct) th = std::thread([&]() { ef = nd_factor(negated_rows(rowsF), P.cx, P.cy, P.cz, dim, Ff); });
+        bool okM = build_matrix(rowsM, Kmacro, NDmacro, nullptr, nullptr);
+        if (direct) th.join();
+        if (!okM || !build_matrix(rowsF, Kfour, NDfour, direct ? &Ff : nullptr, direct ? &ef : nullptr)) return false;
     }
-    // ---- LDS tiles of the face / vertex / moment kernel
-    {
+    // ---- 3D tables (solver3d.hip)
+    if (dim == 3 && !build3d(be_slot)) return false;
+    // ---- LDS tiles of the face / vertex / moment kernel (2D)
+    if (dim == 2) {
         Tiles T;
         std::string e = opt.fallback_kernels ? std::string("fallback kernels requested")
                                              : build_tiles(p, h_gptr, h_gidx, h_vptr, h_vsrc, h_vwt, h_fkind, h_fother,
@@ -2655,7 +2609,7 @@ bool Solver::build() {
     size_t nE = (size_t)nD * nc * nB;
     // FP32 sweeps: e, its gradient, the boundary values and S in float
     // (buffers sized in doubles)
-    if (fp32_ && (opt.fallback_kernels || !use_pc || !use_tiles)) {
+    if (fp32_ && (opt.fallback_kernels || !use_pc || (dim == 2 && !use_tiles))) {
         err_ = "FP32 sweeps need the tiled face kernel and the wave-specialised sweep (unsupported for this mesh or "
                "with fallback kernels)";
         return false;
@@ -2663,13 +2617,13 @@ bool Solver::build() {
     auto half = [&](size_t n) { return fp32_ ? (n + 1) / 2 : n; };
     size_t nEB = (size_t)nD * ne * nB;
     if (!alloc(d_e, half(nE)) || !alloc(d_grad, half(nE * dim)) || !alloc(d_eb, half(nEB + 2)) ||
-        !alloc(d_eV, (size_t)nD * nn * nB) || !alloc(d_T, nc) || !alloc(d_Told, nc) || !alloc(d_T1, nc) ||
+        !alloc(d_eV, dim == 2 ? (size_t)nD * nn * nB : 0) || !alloc(d_T, nc) || !alloc(d_Told, nc) || !alloc(d_T1, nc) ||
         !alloc(d_TV, nn) || !alloc(d_E, nc) || !alloc(d_qx, nc) || !alloc(d_qy, nc) || !alloc(d_qz, nc) ||
         !alloc(d_qxo, nc) || !alloc(d_qyo, nc) || !alloc(d_qzo, nc) || !alloc(d_RM, nc) || !alloc(d_Tb, (size_t)nB * nc) ||
         !alloc(d_tmac, ne) || !alloc(d_rhs, nc) || !alloc(d_r, nc) || !alloc(d_z, nc) || !alloc(d_p, nc) ||
         !alloc(d_Ap, nc) || !alloc(d_S, half(nE)))
         return false;
-    if (use_pc && !alloc(d_tpos, (size_t)nD * nc)) return false;
+    if (use_pc && dim == 2 && !alloc(d_tpos, (size_t)nD * nc)) return false;
     if (opt.linear_solver == GBTE_LINSOLVE_DIRECT &&
         (!alloc(d_ndy, nc) || !alloc(d_ndx, nc) || !alloc(d_ndu, (size_t)std::max<long>(1, std::max(NDmacro.u_total, NDfour.u_total)))))
         return false;
@@ -2723,6 +2677,10 @@ void Solver::launch_vertex_T(const double *T, double *TV) {
 }
 
 void Solver::launch_rhs(bool macro, const double *bval, double *rhs) {
+    if (dim == 3) {
+        launch_rhs3(macro, bval, rhs);
+        return;
+    }
     int set = macro ? 0 : 1;
     RhsK k{d_mbent[set].p, d_mnonorth[set].p, d_mv0[set].p, d_mv1[set].p, d_mbcoef[set].p, d_mkA[set].p,
            d_ma2x[set].p, d_ma2y[set].p, d_mmx[set].p, d_mmy[set].p, d_mm2[set].p};
@@ -2892,6 +2850,10 @@ int Solver::set_initial(const double *T, const double *qx, const double *qy, con
     return GBTE_OK;
 }
 
+// grid sizes of the kernels that remap blocks per XCD (xcd_block) must be
+// multiples of kXcd
+static int xgrid_of(int n) { return ((n + kXcd - 1) / kXcd) * kXcd; }
+
 // The per-system fields in their storage type, and a launch of a kernel
 // templated on it (FP64, or FP32 with GBTE_FP32_SWEEP).
 template <typename ST>
@@ -2912,6 +2874,7 @@ static ST *rp(DBuf<double> &b) {
 int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     if (!ok()) return GBTE_HIP_ERROR;
     if (!check(hipSetDevice(opt.device), "hipSetDevice")) return GBTE_HIP_ERROR;  // the calling thread's device
+    if (dim == 3) return solve3d(max_iter, tolT, tolQ, st);
     // an earlier call that failed mid-iteration may have left side-stream work
     if (s2_ && !check(hipStreamSynchronize(s2_), "sync")) return GBTE_HIP_ERROR;
     auto t0 = std::chrono::high_resolution_clock::now();
@@ -3224,6 +3187,180 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     // the side stream may hold the sources of an iteration that did not run
     if ((s2_ && !check(hipStreamSynchronize(s2_), "sync")) || !check(hipStreamSynchronize(s_), "sync"))
         return GBTE_HIP_ERROR;
+    auto t1 = std::chrono::high_resolution_clock::now();
+    have_state = true;
+    if (st) {
+        st->iterations = nt;
+        st->macro_passes = macro_total;
+        st->fourier_passes = fourier_passes_;
+        st->converged = conv ? 1 : 0;
+        st->err_temp = errT;
+        st->err_flux = errQ;
+        st->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->sweep_ms = sweep_ms;
+        st->transport_ms = transport_ms;
+        st->macro_ms = macro_ms;
+        st->sweep_levels = use_pc ? nsub : nlev;
+        st->macro_cg_iters = (int)cg_total;
+        st->source_ms = source_ms;
+        st->face_ms = face_ms;
+    }
+    return status;
+}
+
+// StaticBTESynthetic::solve on tetrahedra (StaticBTESyntheticSolver.cpp:12-247,
+// solve_DOM :968-1178 for the Sequential scheme, the first-order variants
+// :487-966, :1394-1597 with zero gradients): the 3D kernels of solver3d.hip,
+// the sweep, boundary update, synthetic passes and error check of the 2D loop.
+int Solver::solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st) {
+    auto t0 = std::chrono::high_resolution_clock::now();
+    bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1";
+    Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
+            P.capacity_bulk[0], eref_, offT_, offE_, offW_, offQ_[0], offQ_[1], offQ_[2], offMac_};
+    size_t nE = (size_t)nD * nc * nB;
+    int n_ent = (int)h_ent_list.size();
+    int bbw = blocks_for(blocks_for(n_ent, cpw), kWaves);
+    auto launch_boundary = [&]() {
+        GBTE_LAUNCH_ST(k_boundary, dim3(std::max(bbw, 1)), dim3(kBlock), 0, rp<ST>(d_eb), d_tmac.p, m, d_ent_list.p,
+                       d_be_type.p, d_be_conn.p, d_be_spec.p, d_be_nx.p, d_be_ny.p, d_be_nz.p, d_be_tw.p, d_be_dT.p,
+                       d_be_tov.p, n_ent, nB, nD, cpw);
+    };
+    auto launch_gradient = [&]() {
+        if (first_order) {
+            // Order 1: the gradients stay zero (StaticBTESyntheticSolver.cpp:537-542)
+            (void)check(hipMemsetAsync(d_grad.p, 0, d_grad.n * sizeof(double), s_), "zero gradients");
+        } else {
+            launch_grad3();
+        }
+    };
+    // _set_initial, gradients, outgoing and incoming boundary values
+    // (StaticBTESyntheticSolver.cpp:19-62)
+    GBTE_LAUNCH_ST(k_init_e, dim3(blocks_for((long)nE, kBlock)), dim3(kBlock), 0, rp<ST>(d_e), d_T.p, d_cap.p,
+                   nD * nB, nB, nE, m.eref);
+    launch_gradient();
+    launch_bound_out3(m);
+    launch_boundary();
+    if (!check(hipGetLastError(), "setup kernels")) return GBTE_HIP_ERROR;
+    errInc_ = 0;
+    errTold_ = 0;
+    errQold_ = 0;
+    errInner_ = 0;
+    int nt = 0, status = GBTE_OK, macro_total = 0;
+    bool conv = false;
+    double errT = 0, errQ = 0;
+    double source_ms = 0, sweep_ms = 0, face_ms = 0, transport_ms = 0, macro_ms = 0;
+    long cg_total = 0;
+    for (nt = 0; nt < max_iter; ++nt) {
+        hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
+                           d_qxo.p, d_qyo.p, d_qzo.p, nc);
+        (void)hipEventRecord(ev_[0], s_);
+        launch_source3(m);
+        (void)hipEventRecord(ev_[1], s_);
+        if (use_pc) {
+            PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
+            GBTE_LAUNCH_ST(k_sweep_pc, dim3(nD * pc_nchunk), dim3(kPcBlock), pc_lds, ps, rp<ST>(d_S), rp<ST>(d_e), m,
+                           nc, nB, nD, pc_bc, pc_nchunk, pc_ring, nullptr);
+        } else {
+            hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
+                               (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
+                               d_rec_q.p, d_rec_c.p, d_rec_g.p, d_rec_G.p, d_S.p, d_e.p, m, nc, nB, nD, sweep_bc,
+                               sweep_nchunk, sweep_ring);
+        }
+        (void)hipEventRecord(ev_[2], s_);
+        launch_bound_out3(m);
+        (void)hipEventRecord(ev_[3], s_);
+        if (!first_order) launch_grad3();
+        launch_moments3(m);
+        (void)hipEventRecord(ev_[4], s_);
+        launch_boundary();
+        (void)hipEventRecord(ev_[5], s_);
+        if (!check(hipGetLastError(), "iteration kernels")) return GBTE_HIP_ERROR;
+        // _marco_solution (StaticBTESynthetic.cpp:2692-2734)
+        double inner = 1000;
+        int inc = 0, passes = 0;
+        auto account = [&](double in) {
+            inner = in;
+            if (inner > errInner_) inc++;
+            errInner_ = inner;
+            passes++;
+            if (opt.verbose) printf("internal residual: %g\n", inner);
+            if (inc >= 10) { err_ = "cannot congerve, try to set Order=1 or 2limit"; status = GBTE_DIVERGED; return false; }
+            return inner > 1e-6;
+        };
+        if (sequential) {
+            (void)hipEventRecord(ev_[6], s_);
+        } else if (opt.linear_solver == GBTE_LINSOLVE_DIRECT) {
+            for (bool more = true; more;) {
+                if (!macro_launch(passes & 1)) return GBTE_HIP_ERROR;
+                (void)hipEventRecord(ev_[6], s_);
+                if (!check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
+                more = account(macro_inner((passes) & 1));
+            }
+        } else {
+            while (inner > 1e-6) {
+                hipLaunchKernelGGL(k_copy, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_T1.p, nc);
+                launch_vertex_T(d_T.p, d_TV.p);
+                launch_rhs(true, d_tmac.p, d_rhs.p);
+                int cgit = 0;
+                if (!linsolve(Kmacro, NDmacro, d_rhs.p, d_T.p, &cgit)) return GBTE_HIP_ERROR;
+                cg_total += cgit;
+                double in = 0;
+                if (!inner_error(d_T1.p, d_T.p, &in)) return GBTE_HIP_ERROR;
+                if (!account(in)) break;
+            }
+            (void)hipEventRecord(ev_[6], s_);
+        }
+        macro_total += passes;
+        auto add_times = [&]() {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);
+            source_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[1], ev_[2]);
+            sweep_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[3], ev_[4]);
+            face_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[0], ev_[4]);
+            transport_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[5], ev_[6]);
+            macro_ms += ms;
+        };
+        if (status != GBTE_OK) {
+            if (!check(hipEventSynchronize(ev_[6]), "event sync")) return GBTE_HIP_ERROR;
+            add_times();
+            ++nt;
+            break;
+        }
+        // _get_magin_check_error
+        std::vector<double> h;
+        hipLaunchKernelGGL(k_err, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_Told.p, d_qx.p, d_qy.p, d_qz.p,
+                           d_qxo.p, d_qyo.p, d_qzo.p, d_part.p, nc, red_blocks);
+        if (!reduce_host(d_part.p, red_blocks, h, 4)) return GBTE_HIP_ERROR;
+        add_times();
+        double sT = 0, mT = 0, sQ = 0, mQ = 0;
+        for (int k = 0; k < red_blocks; ++k) {
+            sT += h[k];
+            mT = std::max(mT, h[red_blocks + k]);
+            sQ += h[2 * red_blocks + k];
+            mQ = std::max(mQ, h[3 * red_blocks + k]);
+        }
+        errT = sT / nc / mT;
+        errQ = sQ / nc / mQ;
+        if (opt.verbose)
+            printf("Iteration #%d\t Margin per band per cell: Temperature:%g Heat Flux:%g\n", nt, errT, errQ);
+        if ((errT > errTold_ && errQ > errQold_) || std::isnan(errT) || std::isnan(errQ) || std::isinf(errT) ||
+            std::isinf(errQ))
+            errInc_++;
+        else
+            errInc_ = 0;
+        errTold_ = errT;
+        errQold_ = errQ;
+        if (std::isnan(errT) || std::isnan(errQ)) { status = GBTE_NAN; err_ = "NaN in the iteration errors"; ++nt; break; }
+        if (errT <= tolT && errQ <= tolQ) { conv = true; ++nt; break; }
+        if (errInc_ >= 10) { status = GBTE_DIVERGED; err_ = "Error increases for 10 times"; ++nt; break; }
+    }
+    GBTE_LAUNCH_ST(k_output_moments, dim3(xgrid_of(blocks_for(nc, kOutRows))), dim3(64 * kOutRows), 0, rp<ST>(d_e), m,
+                   d_E.p, d_Tb.p, nc, nB, nD);
+    if (!check(hipStreamSynchronize(s_), "sync")) return GBTE_HIP_ERROR;
     auto t1 = std::chrono::high_resolution_clock::now();
     have_state = true;
     if (st) {

@@ -15,6 +15,19 @@ namespace gbte {
 // Cells and nodes renumbered along a Morton curve (reorder.cpp).
 Problem renumber(const Problem &p, std::vector<int> &cell_new, std::vector<int> &node_new);
 
+// Per-mode constants handed to the kernels (device pointers and scalars).
+struct Modes {
+    const double *vg, *tau, *cap, *lr, *hr, *sx, *sy, *sz, *w;
+    const double *lrc;  // lr / cap per band
+    double kappa, cbulk;
+    // Energy densities are held as deviations e - C_b eref from the
+    // equilibrium at eref (the deck's Tref): the transport equations are
+    // linear and every face sum of (A/V)(s.n) vanishes, so only the sources,
+    // the thermalising walls and the moments see the shift.  off* are the
+    // moments of C_b eref, added back to T, E, T_band, q and T_mac.
+    double eref, offT, offE, offW, offQx, offQy, offQz, offMac;
+};
+
 // Owning device buffer.
 template <typename T>
 struct DBuf {
@@ -92,7 +105,19 @@ class Solver {
     bool alloc(DBuf<T> &b, size_t n);
     bool check(hipError_t e, const char *what);
     bool build();
-    bool build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m, DevND &f);
+    // pre: the nested-dissection factor of the same rows, computed elsewhere
+    // (with its error message in pre_err), or null to factor here
+    bool build_matrix(const std::vector<std::vector<std::pair<int, double>>> &rows, DevCSR &m, DevND &f,
+                      NDFactor *pre, const std::string *pre_err);
+    // tetrahedral meshes (solver3d.hip): device tables, the solve loop and
+    // the launches of the 3D kernels
+    bool build3d(const std::vector<int> &be_slot);
+    int solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st);
+    void launch_rhs3(bool macro, const double *bval, double *rhs);
+    void launch_grad3();
+    void launch_source3(const Modes &m);
+    void launch_bound_out3(const Modes &m);
+    void launch_moments3(const Modes &m);
     // macro / Fourier linear solve of (-K) x = -rhs: the direct factor solve,
     // or PCG warm-started from x
     bool linsolve(const DevCSR &K, const DevND &f, double *d_rhs, double *d_x, int *iters);
@@ -173,6 +198,13 @@ class Solver {
     std::vector<int> h_gptr, h_gidx, h_vptr, h_vsrc, h_fkind, h_fother, h_fv0, h_fv1, h_gok;
     std::vector<double> h_vwt, h_gP, h_fga, h_fn0x, h_fn0y, h_fmxm, h_fmym;
     DevND NDmacro, NDfour;
+    // 3D: source records, LSQ gradient tiles, moments, residual and
+    // right-hand-side coefficients (solver3d.hip)
+    DBuf<double> d_s3d, d_Y, d_YV, d_f3d, d_r3d[2];
+    DBuf<int> d_s3i, d_g3_tile, d_g3_rptr, d_g3_rows, d_g3_sptr, d_g3_self, d_g3_ok, d_f3i, d_r3i[2];
+    DBuf<char> d_g3_ent;
+    int g3_ntiles = 0, g3_rows = 0, g3_row_cap = 256, g3_cell_cap = 32;
+    size_t g3_lds = 0;
     DBuf<double> d_ndy, d_ndu, d_ndx;
     // state
     DBuf<double> d_e, d_grad, d_eb, d_eV, d_T, d_Told, d_T1, d_TV, d_E, d_qx, d_qy, d_qz, d_qxo, d_qyo, d_qzo,
@@ -194,5 +226,20 @@ class Solver {
     int errInc_ = 0;
     int fourier_passes_ = 0;
 };
+
+template <typename T>
+bool Solver::alloc(DBuf<T> &b, size_t n) {
+    b.release();
+    b.n = n;
+    if (n == 0) return true;
+    return check(hipMalloc((void **)&b.p, n * sizeof(T)), "hipMalloc");
+}
+
+template <typename T>
+bool Solver::upload(DBuf<T> &b, const std::vector<T> &h) {
+    if (!alloc(b, h.size())) return false;
+    if (h.empty()) return true;
+    return check(hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
 
 }  // namespace gbte

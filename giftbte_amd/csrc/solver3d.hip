@@ -1,0 +1,793 @@
+// MI355X (gfx950) kernels and host set-up for tetrahedral (3D) meshes.
+//
+// One outer iteration of StaticBTESynthetic::solve
+// (src/StaticBTESynthetic/StaticBTESyntheticSolver.cpp:63-221) on tetrahedra:
+//   k_source3     the part of _get_Re that does not depend on the new upwind
+//                 values, for all systems              StaticBTESynthetic.cpp:1427-1500
+//   k_sweep_pc    the per-system SparseLU solve as an upwind sweep (solver.hip)
+//   k_bound_out3  _get_bound_ee                                      :1502-1539
+//   k_grad3       the next step's _get_gradient_larger (3D branch)   :1329-1371
+//   k_mom3        per-row moments of e and of the boundary values: T, q and
+//                 the seven direction factors of the non-Fourier face terms
+//   k_vert3       _set_vertex_energydensity on those moments         :1541-1616
+//   k_fin3        _recover_temperature, _get_heat_flux and
+//                 _get_non_fourier with _get_face_gradient's 3D branch
+//                 (FaceMatrix, _get_face_matrix :1617-1649)        :1651-1937, :2485-2520
+//   k_boundary    _set_bound_ee_1 + _set_macro_boundary_condition (solver.hip)
+//   synthetic     _marco_solution with the 3D _get_Re_macro (k_rhs3) :2307-2407, :2692-2734
+//
+// Layout: e [cell][dir][band] as in 2D; the LSQ gradient is stored per cell
+// as three contiguous component rows, grad [cell][x|y|z][dir][band], so that
+// each component of a row is one coalesced load.
+//
+// The LSQ gradient of a tetrahedron reads its whole vertex-neighbour stencil
+// (about 80 cells on a Delaunay mesh), so every row is re-read by about 80
+// gradients.  k_grad3 stages the rows a tile of Morton-consecutive cells
+// needs, one 64-entry slice at a time, in LDS and forms the gradients of the
+// tile's cells from there: HBM / L2 see each staged row once per tile, and the
+// stencil arithmetic runs out of LDS.
+//
+// The non-Fourier residual is linear in the face values, so the per-system
+// sums over faces factor through per-row moments Y_j = sum_sys K_j(sys) e:
+//   Y_0            = sum [hr>0] w kappa lr/C e
+//   Y_1..6 (ab)    = sum [hr>0] w tau v^2 s_a s_b e   (xx, xy, xz, yy, yz, zz)
+//   Y_7            = sum w lr/C e                     (lattice temperature)
+//   Y_8..10 (a)    = sum w v s_a e                    (heat flux)
+// with the vertex moments the inverse-distance combinations of the row
+// moments.  Each face then contributes fixed geometric multiples of
+// differences of Y (host-computed, 21 per face slot), which is the
+// reference's sum over systems reordered.
+#include "devcommon.hpp"
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+
+namespace gbte {
+
+namespace {
+
+constexpr int kY = 12;            // row moments (11 used)
+constexpr int kYV = 8;            // vertex moments (the seven residual factors)
+constexpr int kS3D = 40;          // source record doubles per cell: 4 faces x (n, A/V, r, r')
+constexpr int kS3I = 4;           // source record ints per cell
+constexpr int kSrc3Rows = 4;      // k_source3: cell rows (waves) per workgroup
+constexpr int kG3Block = 512;     // k_grad3: 8 waves
+constexpr int kG3Waves = kG3Block / 64;
+constexpr int kG3W = 64;          // entries per slice
+constexpr int kG3Batch = 8;       // staged rows in flight per wave
+constexpr int kFin = 21;          // k_fin3 coefficients per face slot
+constexpr int kMomRows = 4;       // k_mom3: rows (waves) per workgroup
+
+// One LSQ stencil entry of k_grad3: tile-local row and the weights / L_x.
+template <typename AT>
+struct G3Ent;
+template <>
+struct alignas(32) G3Ent<double> {
+    int loc, pad;
+    double px, py, pz;
+};
+template <>
+struct alignas(16) G3Ent<float> {
+    int loc;
+    float px, py, pz;
+};
+
+struct G3Dev {
+    const int *tile_cell0;  // [n_tiles + 1]: tile t owns cells [cell0[t], cell0[t+1])
+    const int *rptr, *rows;  // staged rows of tile t: rows[rptr[t] .. rptr[t+1])
+    const int *sptr;        // per cell: stencil entries [sptr[c], sptr[c+1])
+    const int *self;        // per cell: tile-local row of the cell
+    const int *ok;          // per cell: the reference computes a gradient (stencil > 3)
+};
+
+// ----------------------------------------------------------------- kernels
+
+// _get_gradient_larger, 3D (StaticBTESynthetic.cpp:1329-1371), for the cells
+// of one tile: grad = sum_j (P_j / L_x) (e_j - e_i), zero for a stencil of 3
+// or fewer cells or a NaN.  The tile's staged rows (its cells and their
+// stencils) are copied to LDS one 64-entry slice at a time; each wave then
+// takes one cell of the tile at a time, its lanes one entry each, with the
+// stencil weights in scalar registers.  Arithmetic in the storage type.
+template <typename ST>
+__global__ void __launch_bounds__(kG3Block) k_grad3(const ST *__restrict__ e, ST *__restrict__ grad, G3Dev g,
+                                                    const G3Ent<ST> *__restrict__ ent, int n_tiles, int DB) {
+    extern __shared__ double lds_raw[];
+    ST *rowbuf = reinterpret_cast<ST *>(lds_raw);
+    int tile = xcd_block();
+    if (tile >= n_tiles) return;
+    int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    int c0 = g.tile_cell0[tile], c1 = g.tile_cell0[tile + 1];
+    int r0 = g.rptr[tile], nr = g.rptr[tile + 1] - r0;
+    const int *rows = g.rows + r0;
+    for (int s0 = 0; s0 < DB; s0 += kG3W) {
+        int t = s0 + lane;
+        bool in = t < DB;
+        int tt = in ? t : s0;  // a valid address for lanes past the row end
+        __syncthreads();  // the previous slice's reads are done
+        for (int rb = w * kG3Batch; rb < nr; rb += kG3Waves * kG3Batch) {
+            ST v[kG3Batch];
+#pragma unroll
+            for (int k = 0; k < kG3Batch; ++k) {
+                int r = rb + k < nr ? rb + k : rb;
+                v[k] = e[(size_t)rows[r] * DB + tt];
+            }
+#pragma unroll
+            for (int k = 0; k < kG3Batch; ++k)
+                if (rb + k < nr) rowbuf[(rb + k) * kG3W + lane] = v[k];
+        }
+        __syncthreads();
+        for (int c = c0 + w; c < c1; c += kG3Waves) {
+            int a = g.sptr[c], z = g.sptr[c + 1];
+            ST self = rowbuf[g.self[c] * kG3W + lane];
+            ST gx = 0, gy = 0, gz = 0;
+            if (g.ok[c]) {
+                for (int k = a; k < z; ++k) {
+                    G3Ent<ST> q = ent[k];
+                    ST d = rowbuf[q.loc * kG3W + lane] - self;
+                    gx += q.px * d;
+                    gy += q.py * d;
+                    gz += q.pz * d;
+                }
+                if (isnan(gx) || isnan(gy) || isnan(gz)) { gx = 0; gy = 0; gz = 0; }
+            }
+            if (in) {
+                ST *gc = grad + (size_t)c * 3 * DB + t;
+                __builtin_nontemporal_store(gx, gc);
+                __builtin_nontemporal_store(gy, gc + DB);
+                __builtin_nontemporal_store(gz, gc + 2 * DB);
+            }
+        }
+    }
+}
+
+// The source of one transport step for every system (_get_Re,
+// StaticBTESynthetic.cpp:1427-1500, less the new upwind values):
+//   S = C (T_old - Tref) + Q hr tau - sum_out a (r . g_c) - sum_in,interior a (r' . g_nb)
+//       - sum_in,boundary a eb,      a = tau (A/V) (v . n),
+// written to each direction's upwind order for the sweep.  One wave per cell
+// row; the cell's face record (normal, A/V, r from its centre, r' from the
+// neighbour's centre) sits in scalar registers, and every face issues its
+// loads (the neighbour's three gradient components or the boundary value)
+// before any term is formed.
+template <typename ST>
+__global__ void __launch_bounds__(64 * kSrc3Rows) k_source3(const int *__restrict__ posc, ST *__restrict__ S,
+                                                            const ST *__restrict__ grad, const ST *__restrict__ eb,
+                                                            const double *__restrict__ Told,
+                                                            const double *__restrict__ Q,
+                                                            const double *__restrict__ rec,
+                                                            const int *__restrict__ reci, Modes m, int nc, int nB,
+                                                            int nD) {
+    extern __shared__ double lds[];
+    double *mvg = lds, *mtau = mvg + nB, *mcap = mtau + nB, *mhr = mcap + nB, *msx = mhr + nB, *msy = msx + nD,
+           *msz = msy + nD;
+    int *pos = reinterpret_cast<int *>(msz + nD);
+    int row0 = xcd_block() * kSrc3Rows;
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+        mvg[i] = m.vg[i];
+        mtau[i] = m.tau[i];
+        mcap[i] = m.cap[i];
+        mhr[i] = m.hr[i];
+    }
+    for (int i = threadIdx.x; i < nD; i += blockDim.x) {
+        msx[i] = m.sx[i];
+        msy[i] = m.sy[i];
+        msz[i] = m.sz[i];
+    }
+    for (int i = threadIdx.x; i < kSrc3Rows * nD; i += blockDim.x) {
+        int r = row0 + i / nD;
+        pos[i] = r < nc ? posc[(size_t)r * nD + i % nD] : 0;
+    }
+    __syncthreads();
+    int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int c = row0 + w, lane = threadIdx.x & 63;
+    if (c >= nc) return;
+    int DB = nD * nB;
+    const double *rd = rec + (size_t)c * kS3D;
+    double fnx[4], fny[4], fnz[4], fav[4], frx[4], fry[4], frz[4], fqx[4], fqy[4], fqz[4];
+    const ST *fin[4];
+    bool fint[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const double *r = rd + f * 10;
+        fnx[f] = r[0]; fny[f] = r[1]; fnz[f] = r[2]; fav[f] = r[3];
+        frx[f] = r[4]; fry[f] = r[5]; frz[f] = r[6];
+        fqx[f] = r[7]; fqy[f] = r[8]; fqz[f] = r[9];
+        int nb = reci[(size_t)c * kS3I + f];
+        fint[f] = nb >= 0;
+        fin[f] = nb >= 0 ? grad + (size_t)nb * 3 * DB : eb + (size_t)(-nb - 1) * DB;
+    }
+    double To = Told[c] - m.eref, Qc = Q[c];
+    const ST *gc = grad + (size_t)c * 3 * DB;
+    const int *pc = pos + w * nD;
+    for (int t = lane; t < DB; t += 64) {
+        int d = t / nB, b = t - d * nB;
+        double vb = mvg[b], tau = mtau[b], sx = msx[d], sy = msy[d], sz = msz[d];
+        double s = To * mcap[b] + Qc * mhr[b] * mtau[b];
+        double gx = gc[t], gy = gc[t + DB], gz = gc[t + 2 * DB];
+        double ux[4], uy[4], uz[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            ux[f] = fin[f][t];
+            uy[f] = fint[f] ? (double)fin[f][t + DB] : 0.0;
+            uz[f] = fint[f] ? (double)fin[f][t + 2 * DB] : 0.0;
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double dot = vb * sx * fnx[f] + vb * sy * fny[f] + vb * sz * fnz[f];
+            double a = tau * fav[f] * dot;
+            double uo = frx[f] * gx + fry[f] * gy + frz[f] * gz;
+            double ui = fint[f] ? fqx[f] * ux[f] + fqy[f] * uy[f] + fqz[f] * uz[f] : ux[f];
+            s -= a * (dot >= 0 ? uo : ui);
+        }
+        __builtin_nontemporal_store((ST)s, S + ((size_t)d * nc + pc[d]) * nB + b);
+    }
+}
+
+// _get_bound_ee (StaticBTESynthetic.cpp:1502-1539): outgoing boundary values
+// e + r . grad e, one wave per boundary entry.
+template <typename ST>
+__global__ void __launch_bounds__(256) k_bound_out3(const ST *__restrict__ e, const ST *__restrict__ grad,
+                                                    ST *__restrict__ eb, Modes m, const int *__restrict__ ent_list,
+                                                    const int *__restrict__ be_cell, const int *__restrict__ be_slot,
+                                                    const double *__restrict__ snx, const double *__restrict__ sny,
+                                                    const double *__restrict__ snz, const double *__restrict__ srx,
+                                                    const double *__restrict__ sry, const double *__restrict__ srz,
+                                                    int n_ent, int nB, int nD) {
+    int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    if (k >= n_ent) return;
+    int en = ent_list[k], c = be_cell[en], s = be_slot[en];
+    double nx = snx[s], ny = sny[s], nz = snz[s], rx = srx[s], ry = sry[s], rz = srz[s];
+    int DB = nD * nB;
+    const ST *ec = e + (size_t)c * DB, *gc = grad + (size_t)c * 3 * DB;
+    ST *eo = eb + (size_t)en * DB;
+    for (int t = lane; t < DB; t += 64) {
+        int d = t / nB, b = t - d * nB;
+        double vb = m.vg[b];
+        double dot = vb * m.sx[d] * nx + vb * m.sy[d] * ny + vb * m.sz[d] * nz;
+        if (dot >= 0) eo[t] = (ST)(ec[t] + (rx * gc[t] + ry * gc[t + DB] + rz * gc[t + 2 * DB]));
+    }
+}
+
+// Row moments Y_0..10 (see the top of this file) of the cell rows of e and
+// of the boundary-entry rows of eb.  One wave per row with the lanes over the
+// bands: each lane sums its band over the directions with the direction
+// factors (w, w s_a s_b, w s_a), then multiplies by the band factors and the
+// wave adds the bands in a fixed order.
+template <typename ST>
+__global__ void __launch_bounds__(64 * kMomRows) k_mom3(const ST *__restrict__ e, const ST *__restrict__ eb,
+                                                        const int *__restrict__ ent_list, Modes m,
+                                                        double *__restrict__ Y, int nc, int n_ent, int nB, int nD) {
+    int r = xcd_block() * kMomRows + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r >= nc + n_ent) return;  // whole wave
+    const ST *row = r < nc ? e + (size_t)r * nD * nB : eb + (size_t)ent_list[r - nc] * nD * nB;
+    double y[11];
+#pragma unroll
+    for (int j = 0; j < 11; ++j) y[j] = 0;
+    for (int b = lane; b < ((nB + 63) & ~63); b += 64) {
+        bool act = b < nB;
+        double sw = 0, sxx = 0, sxy = 0, sxz = 0, syy = 0, syz = 0, szz = 0, qx = 0, qy = 0, qz = 0;
+        if (act)
+            for (int d = 0; d < nD; ++d) {
+                double v = row[d * nB + b], wd = m.w[d], sx = m.sx[d], sy = m.sy[d], sz = m.sz[d];
+                double wv = wd * v;
+                sw += wv;
+                sxx += wv * (sx * sx);
+                sxy += wv * (sx * sy);
+                sxz += wv * (sx * sz);
+                syy += wv * (sy * sy);
+                syz += wv * (sy * sz);
+                szz += wv * (sz * sz);
+                qx += wv * sx;
+                qy += wv * sy;
+                qz += wv * sz;
+            }
+        double vb = act ? m.vg[b] : 0.0, lrc = act ? m.lrc[b] : 0.0;
+        double h = act && m.hr[b] > 0 ? 1.0 : 0.0;
+        double kk = h * m.kappa * lrc, tv = h * (act ? m.tau[b] : 0.0) * vb * vb;
+        y[0] += kk * sw;
+        y[1] += tv * sxx;
+        y[2] += tv * sxy;
+        y[3] += tv * sxz;
+        y[4] += tv * syy;
+        y[5] += tv * syz;
+        y[6] += tv * szz;
+        y[7] += lrc * sw;
+        y[8] += vb * qx;
+        y[9] += vb * qy;
+        y[10] += vb * qz;
+    }
+#pragma unroll
+    for (int j = 0; j < 11; ++j) y[j] = wave_sum(y[j]);
+    // boundary entry en's moments are row nc + en
+    size_t yr = r < nc ? (size_t)r : (size_t)nc + ent_list[r - nc];
+    if (lane < 11) {
+        double v = y[0];
+#pragma unroll
+        for (int j = 1; j < 11; ++j) v = lane == j ? y[j] : v;
+        Y[yr * kY + lane] = v;
+    }
+}
+
+// Vertex moments: the inverse-distance combination of the sources' row
+// moments (cells, or the boundary entries of a boundary node), as
+// _set_vertex_energydensity combines e (StaticBTESynthetic.cpp:1541-1616);
+// a node without sources gets 0 (the reference's NaN -> 0).
+__global__ void k_vert3(const double *__restrict__ Y, const int *__restrict__ vptr, const int *__restrict__ vsrc,
+                        const double *__restrict__ vwt, double *__restrict__ YV, int nn, int nc) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    double a[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
+        int s = vsrc[k];
+        const double *y = Y + (size_t)(s >= 0 ? s : nc + (-s - 1)) * kY;
+        double wt = vwt[k];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) a[j] += wt * y[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 7; ++j) YV[(size_t)v * kYV + j] = a[j];
+}
+
+// Per cell: T = Y_7, q = Y_8..10 (plus the moments of C_b Tref) and the
+// non-Fourier residual, sum over the face slots of
+//   sum_j ce_j (Y_j[other] - Y_j[cell]) + cv1_j (YV_j[u1] - YV_j[u0]) + cv2_j (YV_j[u2] - YV_j[u1]).
+__global__ void k_fin3(const double *__restrict__ Y, const double *__restrict__ YV, const int *__restrict__ fi,
+                       const double *__restrict__ fd, Modes m, double *__restrict__ T, double *__restrict__ qx,
+                       double *__restrict__ qy, double *__restrict__ qz, double *__restrict__ RM, int nc) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double *yc = Y + (size_t)c * kY;
+    T[c] = yc[7] + m.offT;
+    qx[c] = yc[8] + m.offQx;
+    qy[c] = yc[9] + m.offQy;
+    qz[c] = yc[10] + m.offQz;
+    double r = 0;
+    for (int f = 0; f < 4; ++f) {
+        const int *q = fi + ((size_t)c * 4 + f) * 4;
+        if (q[0] < 0) continue;  // no term (specular wall)
+        const double *yo = Y + (size_t)q[0] * kY;
+        const double *v0 = YV + (size_t)q[1] * kYV, *v1 = YV + (size_t)q[2] * kYV, *v2 = YV + (size_t)q[3] * kYV;
+        const double *k = fd + ((size_t)c * 4 + f) * kFin;
+#pragma unroll
+        for (int j = 0; j < 7; ++j)
+            r += k[j] * (yo[j] - yc[j]) + k[7 + j] * (v1[j] - v0[j]) + k[14 + j] * (v2[j] - v1[j]);
+    }
+    RM[c] = r;
+}
+
+// Right-hand side of the synthetic solve (_get_Re_macro, 3D branch,
+// StaticBTESynthetic.cpp:2307-2407) or of the Fourier solve (_get_Re,
+// StaticFourier.cpp:520-613), negated for the SPD form: the boundary
+// temperatures' terms and the non-orthogonal terms kappa A a2 . g_t, with the
+// tangential gradient g_t of the vertex temperatures a fixed combination of
+// the face's two vertex differences (w1, w2 per slot).
+__global__ void k_rhs3(const double *__restrict__ RM, const double *__restrict__ Q, const double *__restrict__ vol,
+                       const double *__restrict__ bval, const double *__restrict__ TV, const int *__restrict__ ri,
+                       const double *__restrict__ rd, double *__restrict__ rhs, int nc) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double r = RM ? RM[c] : 0.0;
+    r -= Q[c] * vol[c];
+    for (int f = 0; f < 4; ++f) {
+        const int *q = ri + ((size_t)c * 4 + f) * 4;
+        const double *k = rd + ((size_t)c * 4 + f) * 3;
+        if (q[0] >= 0) r += -k[0] * bval[q[0]];
+        if (q[1] >= 0) r -= k[1] * (TV[q[2]] - TV[q[1]]) + k[2] * (TV[q[3]] - TV[q[2]]);
+    }
+    rhs[c] = -r;
+}
+
+// (J^T J)^{-1} J^T of a J with three columns (Get_inverse_3D,
+// src/utility/utility.cpp:488-548), row-major 3 x m.
+void lsq3(const std::vector<double> &J, int m, std::vector<double> &P) {
+    double A[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            A[i][j] = 0;
+            for (int k = 0; k < m; ++k) A[i][j] += J[k * 3 + i] * J[k * 3 + j];
+        }
+    double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) + A[0][1] * (A[1][2] * A[2][0] - A[1][0] * A[2][2]) +
+                 A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    double I[3][3];
+    I[0][0] = 1.0 / det * (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
+    I[1][0] = 1.0 / det * (A[0][2] * A[2][1] - A[0][1] * A[2][2]);
+    I[2][0] = 1.0 / det * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+    I[0][1] = 1.0 / det * (A[1][2] * A[2][0] - A[1][0] * A[2][2]);
+    I[1][1] = 1.0 / det * (A[0][0] * A[2][2] - A[0][2] * A[2][0]);
+    I[2][1] = 1.0 / det * (A[1][0] * A[0][2] - A[0][0] * A[1][2]);
+    I[0][2] = 1.0 / det * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    I[1][2] = 1.0 / det * (A[0][1] * A[2][0] - A[0][0] * A[2][1]);
+    I[2][2] = 1.0 / det * (A[0][0] * A[1][1] - A[1][0] * A[0][1]);
+    P.assign((size_t)3 * m, 0.0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < m; ++j)
+            for (int k = 0; k < 3; ++k) P[(size_t)i * m + j] += I[i][k] * J[j * 3 + k];
+}
+
+// the face's vertex-gradient matrix: J rows [n; (x_{k+1} - x_k) / L_x] of the
+// slot's vertices in their order (_get_face_matrix, StaticBTESynthetic.cpp:
+// 1617-1649, and Get_gradient_3D's J in StaticFourier.cpp:571-593)
+void face_lsq(const Problem &p, size_t s, double invLx, std::vector<double> &P) {
+    int nv = p.face_nvert[s];
+    std::vector<double> J((size_t)nv * 3);
+    J[0] = p.face_nx[s]; J[1] = p.face_ny[s]; J[2] = p.face_nz[s];
+    for (int k = 0; k + 1 < nv; ++k) {
+        int a = p.face_vert[s * 4 + k], b = p.face_vert[s * 4 + k + 1];
+        J[(k + 1) * 3 + 0] = (invLx * (p.node_x[b] - p.node_x[a]));
+        J[(k + 1) * 3 + 1] = (invLx * (p.node_y[b] - p.node_y[a]));
+        J[(k + 1) * 3 + 2] = (invLx * (p.node_z[b] - p.node_z[a]));
+    }
+    lsq3(J, nv, P);
+}
+
+}  // namespace
+
+// ======================================================================
+// host side
+// ======================================================================
+
+// Device tables of the tetrahedral path (called from build() for dim 3).
+// be_slot: boundary entry -> face slot.
+bool Solver::build3d(const std::vector<int> &be_slot) {
+    const Problem &p = P;
+    size_t ns = (size_t)nc * 6;
+    double invLx = 1.0 / p.deck.Lx;
+    for (int c = 0; c < nc; ++c)
+        if (p.face_size[c] != 4) { err_ = "3D meshes of tetrahedra only"; return false; }
+    // ---- source records
+    {
+        std::vector<double> rec((size_t)nc * kS3D, 0.0);
+        std::vector<int> reci((size_t)nc * kS3I, 0);
+        for (int c = 0; c < nc; ++c)
+            for (int f = 0; f < 4; ++f) {
+                size_t s = (size_t)c * 6 + f;
+                double *r = &rec[(size_t)c * kS3D + f * 10];
+                r[0] = p.face_nx[s]; r[1] = p.face_ny[s]; r[2] = p.face_nz[s];
+                r[3] = p.face_area[s] / p.volume[c];
+                r[4] = p.face_cx[s] - p.cx[c]; r[5] = p.face_cy[s] - p.cy[c]; r[6] = p.face_cz[s] - p.cz[c];
+                int fb = p.face_bound[s];
+                if (fb == -1) {
+                    int n = p.face_neighbor[s];
+                    r[7] = p.face_cx[s] - p.cx[n]; r[8] = p.face_cy[s] - p.cy[n]; r[9] = p.face_cz[s] - p.cz[n];
+                    reci[(size_t)c * kS3I + f] = n;
+                } else {
+                    int ibc = -1;
+                    for (int k = 0; k < 2; ++k)
+                        if (p.bound_cell[fb * 2 + k] == c && p.bound_face[fb * 2 + k] == f) ibc = k;
+                    if (ibc < 0) { err_ = "boundary face without a boundary entry"; return false; }
+                    reci[(size_t)c * kS3I + f] = -(fb * 2 + ibc) - 1;
+                }
+            }
+        if (!upload(d_s3d, rec) || !upload(d_s3i, reci)) return false;
+    }
+    // ---- LSQ weights (_get_CellMatrix_larger + Get_inverse_3D): padded row
+    // tables for the Fourier flux kernel, and the k_grad3 tiles
+    std::vector<double> P3;  // [comp][nbr entry], weights (not yet / L_x)
+    std::vector<int> gok(nc, 0);
+    {
+        int nnz = p.nbr_ptr[nc];
+        P3.assign((size_t)3 * nnz, 0.0);
+        std::vector<double> J, Pc;
+        for (int c = 0; c < nc; ++c) {
+            int a = p.nbr_ptr[c], m = p.nbr_ptr[c + 1] - a;
+            if (m <= 3) continue;
+            J.assign((size_t)m * 3, 0.0);
+            for (int k = 0; k < m; ++k) {
+                int n = p.nbr_idx[a + k];
+                J[k * 3 + 0] = (invLx * (p.cx[n] - p.cx[c]));
+                J[k * 3 + 1] = (invLx * (p.cy[n] - p.cy[c]));
+                J[k * 3 + 2] = (invLx * (p.cz[n] - p.cz[c]));
+            }
+            lsq3(J, m, Pc);
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < m; ++k) P3[(size_t)i * nnz + a + k] = Pc[(size_t)i * m + k];
+            gok[c] = 1;
+        }
+        std::vector<int> qptr(nc + 1, 0), qidx;
+        std::vector<double> q[3];
+        for (int c = 0; c < nc; ++c) {
+            for (int k = p.nbr_ptr[c]; k < p.nbr_ptr[c + 1]; ++k) {
+                qidx.push_back(p.nbr_idx[k]);
+                for (int i = 0; i < 3; ++i) q[i].push_back(P3[(size_t)i * nnz + k]);
+            }
+            while ((qidx.size() - qptr[c]) % kStencilBatch) {
+                qidx.push_back(c);
+                for (int i = 0; i < 3; ++i) q[i].push_back(0.0);
+            }
+            qptr[c + 1] = (int)qidx.size();
+        }
+        gnnz = qptr[nc];
+        std::vector<double> qa(q[0]);
+        qa.insert(qa.end(), q[1].begin(), q[1].end());
+        qa.insert(qa.end(), q[2].begin(), q[2].end());
+        if (!upload(d_gptr, qptr) || !upload(d_gidx, qidx) || !upload(d_gP, qa) || !upload(d_gok, gok)) return false;
+    }
+    // ---- k_grad3 tiles: Morton-consecutive cells while the union of their
+    // stencils stays within the row budget
+    {
+        int nnz = p.nbr_ptr[nc];
+        int cap = g3_row_cap;
+        std::vector<int> tile0(1, 0), rptr(1, 0), rows, sptr(nc + 1, 0), self(nc, 0);
+        std::vector<int> loc_of(nc, -1), tile_rows;
+        std::vector<int> e_loc;
+        std::vector<int> e_cell;  // entry -> stencil position (for the weights)
+        auto row_count = [&](int c, const std::vector<int> &cur) {
+            int add = loc_of[c] < 0 ? 1 : 0;
+            for (int k = p.nbr_ptr[c]; k < p.nbr_ptr[c + 1]; ++k) add += loc_of[p.nbr_idx[k]] < 0 ? 1 : 0;
+            return (int)cur.size() + add;
+        };
+        int c = 0;
+        while (c < nc) {
+            tile_rows.clear();
+            int cstart = c;
+            while (c < nc && c - cstart < g3_cell_cap) {
+                if (c > cstart && row_count(c, tile_rows) > cap) break;
+                if (row_count(c, tile_rows) > cap) { err_ = "a 3D LSQ stencil is larger than the tile row budget"; return false; }
+                auto add_row = [&](int r) {
+                    if (loc_of[r] < 0) { loc_of[r] = (int)tile_rows.size(); tile_rows.push_back(r); }
+                };
+                add_row(c);
+                for (int k = p.nbr_ptr[c]; k < p.nbr_ptr[c + 1]; ++k) add_row(p.nbr_idx[k]);
+                ++c;
+            }
+            // per cell of the tile: its row and its stencil entries
+            for (int cc = cstart; cc < c; ++cc) {
+                self[cc] = loc_of[cc];
+                for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) {
+                    e_loc.push_back(loc_of[p.nbr_idx[k]]);
+                    e_cell.push_back(k);
+                }
+                sptr[cc + 1] = (int)e_loc.size();
+            }
+            for (int r : tile_rows) loc_of[r] = -1;
+            rows.insert(rows.end(), tile_rows.begin(), tile_rows.end());
+            rptr.push_back((int)rows.size());
+            tile0.push_back(c);
+        }
+        g3_ntiles = (int)tile0.size() - 1;
+        g3_rows = 0;
+        for (int t = 0; t < g3_ntiles; ++t) g3_rows = std::max(g3_rows, rptr[t + 1] - rptr[t]);
+        // validate what the kernel assumes: every local row names the cell it
+        // stands for and lies inside its tile
+        for (int t = 0; t < g3_ntiles; ++t)
+            for (int cc = tile0[t]; cc < tile0[t + 1]; ++cc) {
+                int nr = rptr[t + 1] - rptr[t];
+                if (self[cc] < 0 || self[cc] >= nr || rows[rptr[t] + self[cc]] != cc) { err_ = "internal error: 3D tile self row"; return false; }
+                for (int k = sptr[cc]; k < sptr[cc + 1]; ++k) {
+                    int l = e_loc[k];
+                    if (l < 0 || l >= nr || rows[rptr[t] + l] != p.nbr_idx[e_cell[k]]) { err_ = "internal error: 3D tile stencil row"; return false; }
+                }
+            }
+        size_t ne3 = e_loc.size();
+        if (fp32_) {
+            std::vector<G3Ent<float>> ent(ne3);
+            for (size_t k = 0; k < ne3; ++k) {
+                int j = e_cell[k];
+                ent[k].loc = e_loc[k];
+                ent[k].px = (float)(P3[j] * invLx);
+                ent[k].py = (float)(P3[(size_t)nnz + j] * invLx);
+                ent[k].pz = (float)(P3[2 * (size_t)nnz + j] * invLx);
+            }
+            DBuf<char> &dst = d_g3_ent;
+            if (!alloc(dst, ne3 * sizeof(G3Ent<float>)) ||
+                !check(hipMemcpy(dst.p, ent.data(), ne3 * sizeof(G3Ent<float>), hipMemcpyHostToDevice), "H2D"))
+                return false;
+        } else {
+            std::vector<G3Ent<double>> ent(ne3);
+            for (size_t k = 0; k < ne3; ++k) {
+                int j = e_cell[k];
+                ent[k].loc = e_loc[k];
+                ent[k].pad = 0;
+                ent[k].px = P3[j] * invLx;
+                ent[k].py = P3[(size_t)nnz + j] * invLx;
+                ent[k].pz = P3[2 * (size_t)nnz + j] * invLx;
+            }
+            DBuf<char> &dst = d_g3_ent;
+            if (!alloc(dst, ne3 * sizeof(G3Ent<double>)) ||
+                !check(hipMemcpy(dst.p, ent.data(), ne3 * sizeof(G3Ent<double>), hipMemcpyHostToDevice), "H2D"))
+                return false;
+        }
+        g3_lds = (size_t)g3_rows * kG3W * (fp32_ ? sizeof(float) : sizeof(double));
+        int dev_lds = 0;
+        (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, opt.device);
+        if (g3_lds > (size_t)dev_lds) { err_ = "3D gradient tiles exceed the LDS of a workgroup"; return false; }
+        if (!upload(d_g3_tile, tile0) || !upload(d_g3_rptr, rptr) || !upload(d_g3_rows, rows) ||
+            !upload(d_g3_sptr, sptr) || !upload(d_g3_self, self) || !upload(d_g3_ok, gok))
+            return false;
+    }
+    // ---- moments and the residual's face coefficients
+    std::vector<std::vector<double>> FM(nf);  // per unique face, from its first side: 3 x nv
+    for (int f = 0; f < nf; ++f) {
+        int i0 = p.fface_cell[(size_t)f * 2], j0 = p.fface_face[(size_t)f * 2];
+        face_lsq(p, (size_t)i0 * 6 + j0, invLx, FM[f]);
+    }
+    {
+        std::vector<int> fi(ns / 6 * 4 * 4, -1);
+        std::vector<double> fd(ns / 6 * 4 * kFin, 0.0);
+        const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {0, 1, 2, 1, 2, 2};
+        for (int c = 0; c < nc; ++c)
+            for (int f = 0; f < 4; ++f) {
+                size_t s = (size_t)c * 6 + f;
+                int *q = &fi[((size_t)c * 4 + f) * 4];
+                double *k = &fd[((size_t)c * 4 + f) * kFin];
+                int fb = p.face_bound[s], fidx = p.face_index[s];
+                size_t s0;        // the slot whose normal and vertex order define the face gradient
+                double sign, ax, ay, az, At;
+                if (fb == -1) {
+                    int i0 = p.fface_cell[(size_t)fidx * 2], j0 = p.fface_face[(size_t)fidx * 2];
+                    int i1 = p.fface_cell[(size_t)fidx * 2 + 1];
+                    s0 = (size_t)i0 * 6 + j0;
+                    sign = c == i0 ? 1.0 : -1.0;
+                    q[0] = c == i0 ? i1 : i0;
+                    ax = p.cx[i1] - p.cx[i0]; ay = p.cy[i1] - p.cy[i0]; az = p.cz[i1] - p.cz[i0];
+                    At = p.face_area[s0];  // the reference's second side takes the first side's area here
+                } else if (p.bound_type[fb] != 3) {
+                    int ibc = 0;
+                    for (int kk = 0; kk < 2; ++kk)
+                        if (p.bound_cell[fb * 2 + kk] == c) ibc = kk;
+                    s0 = s;
+                    sign = 1.0;
+                    q[0] = nc + fb * 2 + ibc;
+                    ax = p.face_cx[s] - p.cx[c]; ay = p.face_cy[s] - p.cy[c]; az = p.face_cz[s] - p.cz[c];
+                    At = p.face_area[s];
+                } else {
+                    continue;  // specular: no non-Fourier term (StaticBTESynthetic.cpp:1911)
+                }
+                const std::vector<double> &F = FM[fidx];
+                int nv = p.face_nvert[s0];
+                if (nv != 3 || (int)F.size() != 9) { err_ = "3D faces must be triangles"; return false; }
+                for (int kk = 0; kk < 3; ++kk) q[1 + kk] = p.face_vert[s0 * 4 + kk];
+                double n0[3] = {p.face_nx[s0], p.face_ny[s0], p.face_nz[s0]};
+                double n[3] = {p.face_nx[s], p.face_ny[s], p.face_nz[s]};
+                double Ak = p.face_area[s];
+                double ga = (ax * n0[0] + ay * n0[1] + az * n0[2]) / (ax * ax + ay * ay + az * az);
+                // F columns: F[i * 3 + m] is row i (x, y, z), column m (m = 1, 2 the vertex differences)
+                double nn0 = n[0] * n0[0] + n[1] * n0[1] + n[2] * n0[2];
+                k[0] = Ak * sign * ga * nn0;
+                for (int j = 0; j < 6; ++j) {
+                    int a = pa[j], b = pb[j];
+                    double cp = a == b ? n[a] * n0[a] : n[b] * n0[a] + n[a] * n0[b];
+                    k[1 + j] = -At * sign * ga * cp;
+                }
+                for (int mcol = 1; mcol <= 2; ++mcol) {
+                    double fx = F[0 * 3 + mcol], fy = F[1 * 3 + mcol], fz = F[2 * 3 + mcol];
+                    double fv[3] = {fx, fy, fz};
+                    double *kv = k + 7 * mcol;
+                    kv[0] = Ak * (n[0] * fx + n[1] * fy + n[2] * fz) * invLx;
+                    for (int j = 0; j < 6; ++j) {
+                        int a = pa[j], b = pb[j];
+                        double cp = a == b ? n[a] * fv[a] : n[b] * fv[a] + n[a] * fv[b];
+                        kv[1 + j] = -At * cp * invLx;
+                    }
+                }
+            }
+        if (!upload(d_f3i, fi) || !upload(d_f3d, fd)) return false;
+        if (!alloc(d_Y, (size_t)(nc + ne) * kY) || !alloc(d_YV, (size_t)nn * kYV)) return false;
+        if (!check(hipMemsetAsync(d_Y.p, 0, d_Y.n * sizeof(double), s_), "hipMemsetAsync")) return false;
+    }
+    // ---- synthetic (set 0) and Fourier (set 1) right-hand sides
+    {
+        double kap = p.kappa_bulk[0];
+        for (int set = 0; set < 2; ++set) {
+            std::vector<int> ri((size_t)nc * 16, -1);
+            std::vector<double> rd((size_t)nc * 12, 0.0);
+            std::vector<double> Pf;
+            for (int c = 0; c < nc; ++c)
+                for (int f = 0; f < 4; ++f) {
+                    size_t s = (size_t)c * 6 + f;
+                    int *q = &ri[((size_t)c * 4 + f) * 4];
+                    double *k = &rd[((size_t)c * 4 + f) * 3];
+                    int fb = p.face_bound[s];
+                    double ax, ay, az, kA;
+                    if (fb == -1) {
+                        int n = p.face_neighbor[s];
+                        ax = p.cx[n] - p.cx[c]; ay = p.cy[n] - p.cy[c]; az = p.cz[n] - p.cz[c];
+                        kA = (set == 0) ? kap * p.face_area[s] : 2 * (kap * kap) / (kap + kap) * p.face_area[s];
+                    } else {
+                        int ty = p.bound_type[fb];
+                        bool dir = set == 0 ? ty != 3
+                                            : (ty == 1 || (ty < 0 && (p.bound_temp[fb] != 0 ||
+                                                                       p.bound_temp[p.bound_connect[fb]] != 0)));
+                        if (!dir) continue;
+                        int ibc = 0;
+                        for (int kk = 0; kk < 2; ++kk)
+                            if (p.bound_cell[fb * 2 + kk] == c) ibc = kk;
+                        ax = p.face_cx[s] - p.cx[c]; ay = p.face_cy[s] - p.cy[c]; az = p.face_cz[s] - p.cz[c];
+                        double tmp = kap * p.face_area[s];
+                        tmp = tmp / (p.face_nx[s] * ax + p.face_ny[s] * ay + p.face_nz[s] * az);
+                        k[0] = tmp;
+                        q[0] = set == 0 ? fb * 2 + ibc : fb * 2;
+                        kA = kap * p.face_area[s];
+                    }
+                    double dn = ax * p.face_nx[s] + ay * p.face_ny[s] + az * p.face_nz[s];
+                    double a2[3] = {p.face_nx[s] - ax / dn, p.face_ny[s] - ay / dn, p.face_nz[s] - az / dn};
+                    // tangential gradient: the synthetic solve takes the face's
+                    // first side (FaceMatrix), the Fourier solve the cell's own
+                    // face (Get_gradient_3D)
+                    size_t sv;
+                    if (set == 0) {
+                        int fidx = p.face_index[s];
+                        sv = (size_t)p.fface_cell[(size_t)fidx * 2] * 6 + p.fface_face[(size_t)fidx * 2];
+                        Pf = FM[fidx];
+                    } else {
+                        sv = s;
+                        face_lsq(p, s, invLx, Pf);
+                    }
+                    for (int kk = 0; kk < 3; ++kk) q[1 + kk] = p.face_vert[sv * 4 + kk];
+                    for (int mcol = 1; mcol <= 2; ++mcol)
+                        k[mcol] = kA * (a2[0] * Pf[0 * 3 + mcol] + a2[1] * Pf[1 * 3 + mcol] + a2[2] * Pf[2 * 3 + mcol]) * invLx;
+                }
+            if (!upload(d_r3i[set], ri) || !upload(d_r3d[set], rd)) return false;
+        }
+    }
+    (void)be_slot;
+    return true;
+}
+
+void Solver::launch_rhs3(bool macro, const double *bval, double *rhs) {
+    int set = macro ? 0 : 1;
+    hipLaunchKernelGGL(k_rhs3, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, s_, macro ? d_RM.p : nullptr, d_Q.p,
+                       d_vol.p, bval, d_TV.p, d_r3i[set].p, d_r3d[set].p, rhs, nc);
+}
+
+// The 3D transport kernels of one outer iteration (and of the set-up), on
+// the solver's stream.
+void Solver::launch_grad3() {
+    G3Dev g{d_g3_tile.p, d_g3_rptr.p, d_g3_rows.p, d_g3_sptr.p, d_g3_self.p, d_g3_ok.p};
+    int grid = ((g3_ntiles + kXcd - 1) / kXcd) * kXcd;
+    int DB = nD * nB;
+    if (fp32_)
+        hipLaunchKernelGGL(k_grad3<float>, dim3(grid), dim3(kG3Block), g3_lds, s_, reinterpret_cast<float *>(d_e.p),
+                           reinterpret_cast<float *>(d_grad.p), g,
+                           reinterpret_cast<const G3Ent<float> *>(d_g3_ent.p), g3_ntiles, DB);
+    else
+        hipLaunchKernelGGL(k_grad3<double>, dim3(grid), dim3(kG3Block), g3_lds, s_, d_e.p, d_grad.p, g,
+                           reinterpret_cast<const G3Ent<double> *>(d_g3_ent.p), g3_ntiles, DB);
+}
+
+void Solver::launch_source3(const Modes &m) {
+    int grid = ((blocks_for(nc, kSrc3Rows) + kXcd - 1) / kXcd) * kXcd;
+    size_t lds = (size_t)(4 * nB + 3 * nD) * sizeof(double) + (size_t)kSrc3Rows * nD * sizeof(int);
+    if (fp32_)
+        hipLaunchKernelGGL(k_source3<float>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
+                           reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
+                           reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD);
+    else
+        hipLaunchKernelGGL(k_source3<double>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p, d_S.p, d_grad.p,
+                           d_eb.p, d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD);
+}
+
+void Solver::launch_bound_out3(const Modes &m) {
+    int n_ent = (int)h_ent_list.size();
+    if (n_ent == 0) return;
+    dim3 grid(blocks_for((long)n_ent * 64, 256));
+    if (fp32_)
+        hipLaunchKernelGGL(k_bound_out3<float>, grid, dim3(256), 0, s_, reinterpret_cast<const float *>(d_e.p),
+                           reinterpret_cast<const float *>(d_grad.p), reinterpret_cast<float *>(d_eb.p), m,
+                           d_ent_list.p, d_be_cell.p, d_be_slot.p, d_snx.p, d_sny.p, d_snz.p, d_srx.p, d_sry.p,
+                           d_srz.p, n_ent, nB, nD);
+    else
+        hipLaunchKernelGGL(k_bound_out3<double>, grid, dim3(256), 0, s_, d_e.p, d_grad.p, d_eb.p, m, d_ent_list.p,
+                           d_be_cell.p, d_be_slot.p, d_snx.p, d_sny.p, d_snz.p, d_srx.p, d_sry.p, d_srz.p, n_ent, nB,
+                           nD);
+}
+
+void Solver::launch_moments3(const Modes &m) {
+    int n_ent = (int)h_ent_list.size();
+    int grid = ((blocks_for(nc + n_ent, kMomRows) + kXcd - 1) / kXcd) * kXcd;
+    if (fp32_)
+        hipLaunchKernelGGL(k_mom3<float>, dim3(grid), dim3(64 * kMomRows), 0, s_, reinterpret_cast<const float *>(d_e.p),
+                           reinterpret_cast<const float *>(d_eb.p), d_ent_list.p, m, d_Y.p, nc, n_ent, nB, nD);
+    else
+        hipLaunchKernelGGL(k_mom3<double>, dim3(grid), dim3(64 * kMomRows), 0, s_, d_e.p, d_eb.p, d_ent_list.p, m,
+                           d_Y.p, nc, n_ent, nB, nD);
+    hipLaunchKernelGGL(k_vert3, dim3(blocks_for(nn, kBlock)), dim3(kBlock), 0, s_, d_Y.p, d_vptr.p, d_vsrc.p,
+                       d_vwt.p, d_YV.p, nn, nc);
+    hipLaunchKernelGGL(k_fin3, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, s_, d_Y.p, d_YV.p, d_f3i.p, d_f3d.p,
+                       m, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc);
+}
+
+}  // namespace gbte

@@ -214,8 +214,8 @@ def cube_void_tet_mesh(m: int, void: float = 0.4, seed: int = 1, jitter: float =
 
     rng = np.random.default_rng(seed)
     h = 1.0 / m
-    lo = int(round((1.0 - void) / 2 * m))
-    hi = m - lo
+    lo = int(round((1.0 - void) / 2 * m)) if void > 0 else m
+    hi = m - lo if void > 0 else m
     g = np.arange(m + 1)
     I, J, K = np.meshgrid(g, g, g, indexing="ij")
     ijk = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)

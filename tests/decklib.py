@@ -53,6 +53,14 @@ DECKS = {
     "gray12_bicg_seq": (lambda: dataclasses.replace(decks.gray_square_spec(12), method="Sequential",
                                                     matrix="BICGSTAB"),
                         dict(max_iter=400, tol_temp=1e-6, tol_flux=1e-4)),
+    # 3D: tetrahedra of a cube with a cubic void (diffuse walls and void,
+    # isothermal x = 0 / x = 1), 8 bands, Gauss 3D directions (32)
+    "cube3d_b8": (lambda: decks.cube_void_spec(m=6, nbands=8, ntheta=2, nphi=2, seed=6),
+                  dict(max_iter=8, tol_temp=0.0, tol_flux=0.0)),
+    # 3D gray solid cube (no void), Sequential scheme, stopped by the tolerances
+    "cube3d_gray_seq": (lambda: dataclasses.replace(
+        decks.cube_void_spec(m=5, nbands=1, ntheta=1, nphi=3, void=0.0, seed=8), bands=decks.gray_band(),
+        method="Sequential"), dict(max_iter=300, tol_temp=1e-6, tol_flux=1e-4)),
 }
 
 
