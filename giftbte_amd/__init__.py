@@ -150,6 +150,11 @@ class BTESolver:
         self._check(self.lib.gbte_keff(self._h, axis, delta_t, C.byref(v)))
         return v.value
 
+    def tuning_log(self) -> str:
+        """The kernel variants timed at set-up, their milliseconds per launch
+        and the choices made (empty where nothing was tuned)."""
+        return self.lib.gbte_tuning_log(self._h).decode()
+
     def write_outputs(self, out_dir: str = "."):
         """The reference's output files (TempLattice.dat, Tempband.dat,
         Temperature.dat, HeatFlux.dat, Result.vtk) in out_dir."""

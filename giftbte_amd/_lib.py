@@ -118,6 +118,8 @@ def load(path: str | None = None) -> C.CDLL:
         lib.gbte_keff.argtypes = [vp, C.c_int, C.c_double, _pd]
         lib.gbte_last_error.argtypes = [vp]
         lib.gbte_last_error.restype = C.c_char_p
+        lib.gbte_tuning_log.argtypes = [vp]
+        lib.gbte_tuning_log.restype = C.c_char_p
         lib.gbte_destroy.argtypes = [vp]
         lib.gbte_destroy.restype = None
         lib.gbte_write_outputs.argtypes = [vp, vp, C.c_char_p]
@@ -131,7 +133,7 @@ def load(path: str | None = None) -> C.CDLL:
 EXPORTED = [
     "gbte_problem_load", "gbte_problem_tables", "gbte_problem_free", "gbte_create", "gbte_fourier_solve",
     "gbte_set_initial", "gbte_solve", "gbte_get_fields", "gbte_keff", "gbte_last_error", "gbte_problem_error",
-    "gbte_destroy", "gbte_write_outputs", "gbte_write_fields", "gbte_solve_batch",
+    "gbte_destroy", "gbte_write_outputs", "gbte_write_fields", "gbte_solve_batch", "gbte_tuning_log",
 ]
 
 

@@ -165,6 +165,7 @@ int gbte_keff(gbte_handle *h, int axis, double delta_T, double *keff) {
 }
 
 const char *gbte_last_error(const gbte_handle *h) { return h ? h->err.c_str() : "null handle"; }
+const char *gbte_tuning_log(const gbte_handle *h) { return h && h->s ? h->s->tuning_log().c_str() : ""; }
 
 void gbte_destroy(gbte_handle *h) { delete h; }
 

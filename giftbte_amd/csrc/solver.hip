@@ -641,7 +641,9 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_dir(
 constexpr int kPcBlock = 1024;
 constexpr int kPcProd = 256;    // producer lanes (4 waves)
 constexpr int kPcCap = 128;     // positions per sub-level (2 producer lanes each)
-constexpr int kPcFar = 16;      // far upwind reads per sub-level (16 producer lanes each)
+constexpr int kPcFar = 16;      // far upwind reads per sub-level on 2D meshes (16 producer lanes each)
+constexpr int kPcFar3 = 64;     // the same on tetrahedra (four per producer lane), whose upwind
+                                // neighbours more often lie behind the ring
 constexpr int kPcBc = 10;       // bands per workgroup at most
 constexpr int kPcBpl = 5;       // bands per producer lane (kPcBc / 2)
 constexpr int kPcDepth = 4;     // sub-levels in flight
@@ -649,30 +651,44 @@ constexpr int kPcDepth = 4;     // sub-levels in flight
 // vectors): S[kPcCap][Bc] | g0,g1,g2,G [kPcCap] | far[kPcFar][Bc] | ints
 // cell,q0,q1,q2 [kPcCap] | T_old - Tref [kPcCap] (when the sources leave out
 // their C T_old term)
-constexpr int kPcStage = kPcCap * kPcBc + kPcCap * 4 + kPcFar * kPcBc + kPcCap * 2 + kPcCap;
+static_assert(kPcCap <= 256 && kPcFar3 <= 255, "sub-level sizes pack into 8 + 8 bits");
+constexpr int pc_stage(int far) { return kPcCap * kPcBc + kPcCap * 4 + far * kPcBc + kPcCap * 2 + kPcCap; }
 
-struct PcSched {
-    const PcRec *rec;    // [dir][position]
-    const int *sl, *fp;  // per direction: sub-level position / far-entry offsets (Lsub + 1 each)
-    const int *dsl;      // per direction offset into sl / fp
-    const int *farcell;  // far upwind cells, per sub-level runs
-};
-
-template <typename ST>
+template <typename ST, int FPL>
 struct PcProdBuf {
     ST s[kPcBpl];        // loads in the storage type, converted when staged
     int4 r0;             // cell,q0,q1,q2 (even lane) or g2,G (odd lane, as bits)
     int4 r1;             // g0,g1 (both lanes)
-    int fc;              // far upwind cell
-    ST fv;               // far upwind value
+    int fc[FPL];         // far upwind cells (far entries fe + 16 i)
+    ST fv[FPL];          // far upwind values
     double tp;           // T_old - Tref of the position's cell (with tpos)
 };
 
 __device__ __forceinline__ double pc_dbl(int lo, int hi) { return __hiloint2double(hi, lo); }
 
+// Position and far-entry offsets of one sub-level, advanced one sub-level at
+// a time from the packed sizes (and held at the last sub-level past the end).
+struct PcCursor {
+    const unsigned short *sz;
+    int l = 0, a0 = 0, f0, np, nf;
+    __device__ PcCursor(const unsigned short *s, int fbase) : sz(s), f0(fbase) { unpack(); }
+    __device__ void unpack() {
+        int v = __builtin_amdgcn_readfirstlane(sz[l]);
+        np = (v & 255) + 1;
+        nf = v >> 8;
+    }
+    __device__ void next(int Lsub) {
+        if (l + 1 >= Lsub) return;
+        a0 += np;
+        f0 += nf;
+        ++l;
+        unpack();
+    }
+};
+
 // tpos: [dir][upwind position] T_old - Tref, or null when S holds the whole
 // source; with it, the source is S + C_b tpos.
-template <typename ST>
+template <typename ST, int FAR>
 __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__restrict__ S,
                                                        ST *__restrict__ e, Modes m, int nc, int nB, int nD, int Bc,
                                                        int nchunk, int R, const double *__restrict__ tpos) {
@@ -683,12 +699,10 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
     int sl0 = P.dsl[d], Lsub = P.dsl[d + 1] - sl0 - 1;
     double *ring = lds;
     double *stage = lds + (size_t)R * Bc;
-    int *sl = reinterpret_cast<int *>(stage + 2 * kPcStage);
-    int *fp = sl + (Lsub + 1);
-    for (int i = threadIdx.x; i <= Lsub; i += blockDim.x) {
-        sl[i] = P.sl[sl0 + i];
-        fp[i] = P.fp[sl0 + i];
-    }
+    constexpr int kStage = pc_stage(FAR), FPL = FAR / 16;
+    unsigned short *sz = reinterpret_cast<unsigned short *>(stage + 2 * kStage);
+    for (int i = threadIdx.x; i <= Lsub; i += blockDim.x) sz[i] = P.size[sl0 + i];
+    int fbase = P.f0[d];
     __syncthreads();
     size_t base = (size_t)d * nc;
     size_t DB = (size_t)nD * nB;
@@ -698,11 +712,15 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
         // ---------------- producers
         int pl = threadIdx.x, p = pl >> 1, h = pl & 1;
         int fe = pl >> 4, fb = pl & 15;
-        PcProdBuf<ST> buf[kPcDepth];
-        // loads of sub-level lv into buf (addresses clamped to valid memory)
-        auto issue = [&](PcProdBuf<ST> &B, int lv) {
-            int l = lv < Lsub ? lv : Lsub - 1;
-            int a0 = sl[l], np = sl[l + 1] - a0, f0 = fp[l], nf = fp[l + 1] - f0;
+        PcProdBuf<ST, FPL> buf[kPcDepth];
+        // each of the three producer streams (loads, far cells, staging)
+        // walks the sub-levels in order with a cursor of its own
+        PcCursor cI(sz, fbase), cF(sz, fbase), cW(sz, fbase);
+        // loads of the next sub-level (the last one again past the end:
+        // addresses stay valid) into B
+        auto issue = [&](PcProdBuf<ST, FPL> &B) {
+            int a0 = cI.a0, np = cI.np;
+            cI.next(Lsub);
             int pp = p < np ? p : 0;
             size_t Pp = base + a0 + pp;
             const ST *srow = S + Pp * nB + b0;
@@ -715,28 +733,32 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
             B.r0 = rr[h ? 2 : 0];
             B.r1 = rr[1];
             if (tpos) B.tp = tpos[Pp];
-            (void)f0;
-            (void)nf;
         };
         // far upwind cells of sub-level lv, loaded kPcDepth - 1 sub-levels
         // before the far values that they address (so that the dependent
         // far-value load waits only for loads that old)
-        auto issue_fc = [&](PcProdBuf<ST> &B, int lv) {
-            int l = lv < Lsub ? lv : Lsub - 1;
-            int f0 = fp[l], nf = fp[l + 1] - f0;
-            B.fc = P.farcell[f0 + (fe < nf ? fe : 0)];
+        auto issue_fc = [&](PcProdBuf<ST, FPL> &B) {
+            int f0 = cF.f0, nf = cF.nf;
+            cF.next(Lsub);
+#pragma unroll
+            for (int i = 0; i < FPL; ++i) {
+                int fi = fe + 16 * i;
+                B.fc[i] = P.farcell[f0 + (fi < nf ? fi : 0)];
+            }
         };
-        auto issue_far = [&](PcProdBuf<ST> &B) {
-            B.fv = e[(size_t)B.fc * DB + (size_t)d * nB + b0 + (fb < Bc ? fb : 0)];
+        auto issue_far = [&](PcProdBuf<ST, FPL> &B) {
+#pragma unroll
+            for (int i = 0; i < FPL; ++i) B.fv[i] = e[(size_t)B.fc[i] * DB + (size_t)d * nB + b0 + (fb < Bc ? fb : 0)];
         };
         // stage sub-level lv (its loads have arrived) into buffer lv & 1
-        auto write = [&](const PcProdBuf<ST> &B, int lv) {
+        auto write = [&](const PcProdBuf<ST, FPL> &B, int lv) {
             if (lv >= Lsub) return;
-            int a0 = sl[lv], np = sl[lv + 1] - a0, nf = fp[lv + 1] - fp[lv];
-            double *st = stage + (size_t)(lv & 1) * kPcStage;
+            int np = cW.np, nf = cW.nf;
+            cW.next(Lsub);
+            double *st = stage + (size_t)(lv & 1) * kStage;
             double *sS = st, *sG = st + kPcCap * kPcBc, *sF = sG + kPcCap * 4;
-            int *sN = reinterpret_cast<int *>(sF + kPcFar * kPcBc);
-            double *sT = sF + kPcFar * kPcBc + kPcCap * 2;
+            int *sN = reinterpret_cast<int *>(sF + FAR * kPcBc);
+            double *sT = sF + FAR * kPcBc + kPcCap * 2;
             if (p < np) {
                 if (tpos && h == 0) sT[p] = B.tp;
 #pragma unroll
@@ -756,20 +778,22 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
                     sG[p * 4 + 3] = pc_dbl(B.r0.z, B.r0.w);
                 }
             }
-            if (fe < nf && fb < Bc) sF[fe * Bc + fb] = B.fv;
+#pragma unroll
+            for (int i = 0; i < FPL; ++i)
+                if (fe + 16 * i < nf && fb < Bc) sF[(fe + 16 * i) * Bc + fb] = B.fv[i];
         };
         // prologue: sub-levels 0 .. kPcDepth-1 in flight with their far
         // values, stage 0, far cells of kPcDepth .. 2 kPcDepth-2
 #pragma unroll
         for (int q = 0; q < kPcDepth; ++q) {
-            issue_fc(buf[q], q);
-            issue(buf[q], q);
+            issue_fc(buf[q]);
+            issue(buf[q]);
         }
 #pragma unroll
         for (int q = 0; q < kPcDepth; ++q) issue_far(buf[q]);
         write(buf[0], 0);
 #pragma unroll
-        for (int q = 0; q + 1 < kPcDepth; ++q) issue_fc(buf[q], kPcDepth + q);
+        for (int q = 0; q + 1 < kPcDepth; ++q) issue_fc(buf[q]);  // sub-levels kPcDepth + q
         __builtin_amdgcn_s_waitcnt(0);  // the loop starts from a known state of outstanding loads
         __syncthreads();
         // iteration j (consumers solve j): stage j+1, far cells of j+2D-1
@@ -782,8 +806,8 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
             for (int q = 0; q < kPcDepth; ++q) {
                 int j = j0 + q;
                 write(buf[(q + 1) % kPcDepth], j + 1);
-                issue_fc(buf[(q + kPcDepth - 1) % kPcDepth], j + 2 * kPcDepth - 1);
-                issue(buf[q], j + kPcDepth);
+                issue_fc(buf[(q + kPcDepth - 1) % kPcDepth]);  // sub-level j + 2 kPcDepth - 1
+                issue(buf[q]);                                 // sub-level j + kPcDepth
                 issue_far(buf[q]);
                 __syncthreads();
             }
@@ -797,14 +821,16 @@ __global__ void __launch_bounds__(kPcBlock) k_sweep_pc(PcSched P, const ST *__re
         double tvb = act ? m.tau[b] * m.vg[b] : 0.0;
         double cb = act ? m.cap[b] : 0.0;
         ST *ed = e + (size_t)d * nB + b;
+        PcCursor cC(sz, fbase);
         __syncthreads();  // prologue
         for (int j = 0; j < nstep; ++j) {
+            int a0 = cC.a0, np = cC.np;
+            cC.next(Lsub);
             if (act && j < Lsub) {
-                int a0 = sl[j], np = sl[j + 1] - a0;
-                const double *st = stage + (size_t)(j & 1) * kPcStage;
+                const double *st = stage + (size_t)(j & 1) * kStage;
                 const double *sS = st, *sG = st + kPcCap * kPcBc, *sF = sG + kPcCap * 4;
-                const int *sN = reinterpret_cast<const int *>(sF + kPcFar * kPcBc);
-                const double *sT = sF + kPcFar * kPcBc + kPcCap * 2;
+                const int *sN = reinterpret_cast<const int *>(sF + FAR * kPcBc);
+                const double *sT = sF + FAR * kPcBc + kPcCap * 2;
                 for (int p = slot; p < np; p += nslot) {
                     int4 nq = *reinterpret_cast<const int4 *>(sN + p * 4);
                     double2 g23 = *reinterpret_cast<const double2 *>(sG + p * 4 + 2);
@@ -2120,6 +2146,7 @@ bool Solver::build() {
         // positions and kPcFar far upwind reads (upwind positions older than
         // the ring of the last pc_ring positions when the sub-level runs)
         use_pc = false;
+        pc_far = dim == 3 ? kPcFar3 : kPcFar;
         pc_nchunk = std::max((nB + kPcBc - 1) / kPcBc, std::min(nB, (256 + nD - 1) / nD));
         pc_bc = (nB + pc_nchunk - 1) / pc_nchunk;
         pc_nchunk = (nB + pc_bc - 1) / pc_bc;
@@ -2148,7 +2175,7 @@ bool Solver::build() {
                             int qq = rec_q[(base + k) * kMaxIn + j];
                             if (qq >= 0 && qq < lo_new) own++;
                         }
-                        if (k > s0 && (k - s0 + 1 > kPcCap || far + add + own > kPcFar)) {
+                        if (k > s0 && (k - s0 + 1 > kPcCap || far + add + own > pc_far)) {
                             sub.push_back(k);
                             s0 = k;
                             reset();
@@ -2187,7 +2214,7 @@ bool Solver::build() {
                             }
                         }
                     }
-                    if (nf > kPcFar) { err_ = "internal error: sweep sub-level far budget"; return false; }
+                    if (nf > pc_far) { err_ = "internal error: sweep sub-level far budget"; return false; }
                 }
                 psl.push_back(sub[Ls]);
                 pfp.push_back((int)pfar.size());
@@ -2204,7 +2231,7 @@ bool Solver::build() {
                 if (sl[0] != 0 || sl[Ls] != nc) { err_ = "internal error: sweep sub-levels"; return false; }
                 for (int m2 = 0; m2 < Ls; ++m2) {
                     int a0 = sl[m2], a1 = sl[m2 + 1], nf = fp[m2 + 1] - fp[m2];
-                    if (a1 <= a0 || a1 - a0 > kPcCap || nf < 0 || nf > kPcFar) { err_ = "internal error: sweep sub-level size"; return false; }
+                    if (a1 <= a0 || a1 - a0 > kPcCap || nf < 0 || nf > pc_far) { err_ = "internal error: sweep sub-level size"; return false; }
                     int ready = sl[std::max(0, m2 - kPcDepth)];
                     for (int k = a0; k < a1; ++k) {
                         const PcRec &r = prec[(size_t)d * nc + k];
@@ -2227,12 +2254,23 @@ bool Solver::build() {
             int maxsub0 = 0;
             for (int d = 0; d < nD; ++d) maxsub0 = std::max(maxsub0, pdsl[d + 1] - pdsl[d] - 1);
             // the kernel copies one direction's sub-level tables to LDS
-            size_t lds = ((size_t)pc_ring * pc_bc + 2 * kPcStage) * sizeof(double) + 2 * (size_t)(maxsub0 + 1) * sizeof(int);
+            size_t lds = ((size_t)pc_ring * pc_bc + 2 * pc_stage(pc_far)) * sizeof(double) +
+                     (size_t)(maxsub0 + 1) * sizeof(unsigned short);
             int dev_lds = 0;
             (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, opt.device);
+            if (opt.verbose)
+                printf("wave-specialised sweep: ring %d, schedule %s, %d sub-levels at most, %zu of %d bytes of LDS\n",
+                       pc_ring, fits ? "built" : "rejected", maxsub0, lds, dev_lds);
             if (fits && lds <= (size_t)dev_lds) {
                 pc_lds = lds;
-                use_pc = upload(d_pc_rec, prec) && upload(d_pc_sl, psl) && upload(d_pc_fp, pfp) &&
+                std::vector<unsigned short> psize(psl.size(), 0);
+                std::vector<int> pf0(nD);
+                for (int d = 0; d < nD; ++d) {
+                    pf0[d] = pfp[pdsl[d]];
+                    for (int k = pdsl[d]; k + 1 < pdsl[d + 1]; ++k)
+                        psize[k] = (unsigned short)((psl[k + 1] - psl[k] - 1) | (pfp[k + 1] - pfp[k]) << 8);
+                }
+                use_pc = upload(d_pc_rec, prec) && upload(d_pc_size, psize) && upload(d_pc_f0, pf0) &&
                          upload(d_pc_dsl, pdsl) && upload(d_pc_far, pfar);
                 if (!use_pc) return false;
                 int maxsub = 0;
@@ -2659,6 +2697,7 @@ bool Solver::build() {
                 if (p.face_bound[(size_t)c * 6 + j] == -1) nint[c]++;
         if (!upload(d_nint, nint)) return false;
     }
+    if (dim == 3 && !tune3d()) return false;
     return check(hipStreamSynchronize(s_), "build sync");
 }
 
@@ -2854,6 +2893,38 @@ int Solver::set_initial(const double *T, const double *qx, const double *qy, con
 // multiples of kXcd
 static int xgrid_of(int n) { return ((n + kXcd - 1) / kXcd) * kXcd; }
 
+Modes Solver::modes() const {
+    return Modes{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
+                 P.capacity_bulk[0], eref_, offT_, offE_, offW_, offQ_[0], offQ_[1], offQ_[2], offMac_};
+}
+
+// One whole-source sweep of every direction (tetrahedral meshes): the
+// wave-specialised kernel or the level-synchronous one.
+void Solver::launch_sweep3(const Modes &m) {
+    if (use_pc) {
+        PcSched ps{d_pc_rec.p, d_pc_size.p, d_pc_f0.p, d_pc_dsl.p, d_pc_far.p};
+        launch_sweep_pc(ps, m, nullptr);
+    } else {
+        hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
+                           (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
+                           d_rec_q.p, d_rec_c.p, d_rec_g.p, d_rec_G.p, d_S.p, d_e.p, m, nc, nB, nD, sweep_bc,
+                           sweep_nchunk, sweep_ring);
+    }
+}
+
+// The wave-specialised sweep at the handle's storage type and far budget.
+void Solver::launch_sweep_pc(const PcSched &ps, const Modes &m, const double *tpos) {
+    dim3 g(nD * pc_nchunk), b(kPcBlock);
+#define GBTE_PC(ST_, F_)                                                                                          \
+    hipLaunchKernelGGL((k_sweep_pc<ST_, F_>), g, b, pc_lds, s_, ps, reinterpret_cast<const ST_ *>(d_S.p),            \
+                       reinterpret_cast<ST_ *>(d_e.p), m, nc, nB, nD, pc_bc, pc_nchunk, pc_ring, tpos)
+    if (fp32_ && pc_far == kPcFar3) GBTE_PC(float, kPcFar3);
+    else if (fp32_) GBTE_PC(float, kPcFar);
+    else if (pc_far == kPcFar3) GBTE_PC(double, kPcFar3);
+    else GBTE_PC(double, kPcFar);
+#undef GBTE_PC
+}
+
 // The per-system fields in their storage type, and a launch of a kernel
 // templated on it (FP64, or FP32 with GBTE_FP32_SWEEP).
 template <typename ST>
@@ -2880,8 +2951,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     auto t0 = std::chrono::high_resolution_clock::now();
     bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1",
          limiter = P.deck.order == "2limit";
-    Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
-            P.capacity_bulk[0], eref_, offT_, offE_, offW_, offQ_[0], offQ_[1], offQ_[2], offMac_};
+    Modes m = modes();
     SlotGeo g{d_fsize.p, d_snb.p, d_sent.p, d_snx.p, d_sny.p, d_snz.p, d_sarea.p, d_vol.p,
               d_srx.p, d_sry.p, d_srz.p, d_sqx.p, d_sqy.p, d_sqz.p};
     FaceK fk{d_fkind.p, d_fother.p, d_fv0.p, d_fv1.p, d_fga.p, d_fn0x.p, d_fn0y.p, d_fmx.p, d_fmy.p};
@@ -3015,9 +3085,8 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         }
         (void)hipEventRecord(ev_[1], s_);
         if (use_pc) {
-            PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
-            GBTE_LAUNCH_ST(k_sweep_pc, dim3(nD * pc_nchunk), dim3(kPcBlock), pc_lds, ps, rp<ST>(d_S), rp<ST>(d_e), m, nc,
-                           nB, nD, pc_bc, pc_nchunk, pc_ring, split ? d_tpos.p : nullptr);
+            PcSched ps{d_pc_rec.p, d_pc_size.p, d_pc_f0.p, d_pc_dsl.p, d_pc_far.p};
+            launch_sweep_pc(ps, m, split ? d_tpos.p : nullptr);
         } else {
             hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
                                (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
@@ -3215,8 +3284,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
 int Solver::solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     auto t0 = std::chrono::high_resolution_clock::now();
     bool sequential = P.deck.method == "Sequential", first_order = P.deck.order == "1";
-    Modes m{d_vg.p, d_tau.p, d_cap.p, d_lr.p, d_hr.p, d_sx.p, d_sy.p, d_sz.p, d_w.p, d_lrc.p, P.kappa_bulk[0],
-            P.capacity_bulk[0], eref_, offT_, offE_, offW_, offQ_[0], offQ_[1], offQ_[2], offMac_};
+    Modes m = modes();
     size_t nE = (size_t)nD * nc * nB;
     int n_ent = (int)h_ent_list.size();
     int bbw = blocks_for(blocks_for(n_ent, cpw), kWaves);
@@ -3256,16 +3324,7 @@ int Solver::solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         (void)hipEventRecord(ev_[0], s_);
         launch_source3(m);
         (void)hipEventRecord(ev_[1], s_);
-        if (use_pc) {
-            PcSched ps{d_pc_rec.p, d_pc_sl.p, d_pc_fp.p, d_pc_dsl.p, d_pc_far.p};
-            GBTE_LAUNCH_ST(k_sweep_pc, dim3(nD * pc_nchunk), dim3(kPcBlock), pc_lds, ps, rp<ST>(d_S), rp<ST>(d_e), m,
-                           nc, nB, nD, pc_bc, pc_nchunk, pc_ring, nullptr);
-        } else {
-            hipLaunchKernelGGL(k_sweep_dir, dim3(nD * sweep_nchunk), dim3(kSweepBlock),
-                               (size_t)sweep_ring * sweep_bc * sizeof(double), s_, d_item_cell.p, d_levptr.p, d_dlev.p,
-                               d_rec_q.p, d_rec_c.p, d_rec_g.p, d_rec_G.p, d_S.p, d_e.p, m, nc, nB, nD, sweep_bc,
-                               sweep_nchunk, sweep_ring);
-        }
+        launch_sweep3(m);
         (void)hipEventRecord(ev_[2], s_);
         launch_bound_out3(m);
         (void)hipEventRecord(ev_[3], s_);

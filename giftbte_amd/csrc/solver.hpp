@@ -52,6 +52,15 @@ struct PcRec {
     double g[3], G;
 };
 
+// The wave-specialised sweep's schedule, as device pointers.
+struct PcSched {
+    const PcRec *rec;            // [dir][position]
+    const unsigned short *size;  // per sub-level: (positions - 1) | far reads << 8, Lsub + 1 per direction
+    const int *f0;               // per direction: offset of its first far entry in farcell
+    const int *dsl;              // per direction offset into size
+    const int *farcell;          // far upwind cells, per sub-level runs
+};
+
 // Compressed sparse rows of the synthetic (macro) and Fourier matrices,
 // stored negated so that they are symmetric positive definite.
 struct DevCSR {
@@ -79,6 +88,7 @@ class Solver {
     ~Solver();
     bool ok() const { return err_.empty(); }
     const std::string &error() const { return err_; }
+    const std::string &tuning_log() const { return tune_log_; }
 
     int fourier(gbte_stats *st);
     int set_initial(const double *T, const double *qx, const double *qy, const double *qz);
@@ -118,6 +128,10 @@ class Solver {
     void launch_source3(const Modes &m);
     void launch_bound_out3(const Modes &m);
     void launch_moments3(const Modes &m);
+    bool tune3d();
+    void launch_sweep_pc(const PcSched &ps, const Modes &m, const double *tpos);
+    void launch_sweep3(const Modes &m);
+    Modes modes() const;
     // macro / Fourier linear solve of (-K) x = -rhs: the direct factor solve,
     // or PCG warm-started from x
     bool linsolve(const DevCSR &K, const DevND &f, double *d_rhs, double *d_x, int *iters);
@@ -166,10 +180,11 @@ class Solver {
     int sweep_bc = 1, sweep_nchunk = 1, sweep_ring = 1;
     // wave-specialised sweep (k_sweep_pc): records, sub-level tables
     bool use_pc = false;
-    int pc_bc = 1, pc_nchunk = 1, pc_ring = 1, nsub = 0;
+    int pc_bc = 1, pc_nchunk = 1, pc_ring = 1, nsub = 0, pc_far = 16;
     size_t pc_lds = 0;
     DBuf<PcRec> d_pc_rec;
-    DBuf<int> d_pc_sl, d_pc_fp, d_pc_dsl, d_pc_far;
+    DBuf<int> d_pc_f0, d_pc_dsl, d_pc_far;
+    DBuf<unsigned short> d_pc_size;
     // gradient stencil
     DBuf<int> d_gptr, d_gidx, d_gok;
     DBuf<double> d_gP;
@@ -202,8 +217,14 @@ class Solver {
     // right-hand-side coefficients (solver3d.hip)
     DBuf<double> d_s3d, d_Y, d_YV, d_f3d, d_r3d[2];
     DBuf<int> d_s3i, d_g3_tile, d_g3_rptr, d_g3_rows, d_g3_sptr, d_g3_self, d_g3_ok, d_f3i, d_r3i[2];
-    DBuf<char> d_g3_ent;
-    int g3_ntiles = 0, g3_rows = 0, g3_row_cap = 256, g3_cell_cap = 32;
+    DBuf<char> d_g3_ent, d_g3g_ent;
+    DBuf<int> d_g3g_ptr, d_g3g_idx;
+    int g3_mode = 0;        // 0: LDS tiles, 1: direct gather (chosen by tune3d)
+    std::string tune_log_;  // the autotuner's candidates, timings and choices
+    // tile budgets (rows staged per slice, stencil-table entries, cells) and
+    // the largest tile built; the table sits after the rows in LDS
+    int g3_ntiles = 0, g3_rows = 0, g3_ents = 0, g3_row_cap = 0, g3_ent_cap = 0, g3_cell_cap = 64,
+        g3_group = 10;  // 64-entry slices per workgroup
     size_t g3_lds = 0;
     DBuf<double> d_ndy, d_ndu, d_ndx;
     // state

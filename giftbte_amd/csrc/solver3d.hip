@@ -57,7 +57,9 @@ constexpr int kSrc3Rows = 4;      // k_source3: cell rows (waves) per workgroup
 constexpr int kG3Block = 512;     // k_grad3: 8 waves
 constexpr int kG3Waves = kG3Block / 64;
 constexpr int kG3W = 64;          // entries per slice
-constexpr int kG3Batch = 8;       // staged rows in flight per wave
+constexpr int kG3Ent = 8;         // stencil entries per batch (stencils padded to a multiple)
+constexpr int kG3CellsMax = 64;   // cells per tile at most
+constexpr int kG3RowsPerWave = 40;  // staged rows per wave at most (row budget <= 8 x 40)
 constexpr int kFin = 21;          // k_fin3 coefficients per face slot
 constexpr int kMomRows = 4;       // k_mom3: rows (waves) per workgroup
 
@@ -87,59 +89,157 @@ struct G3Dev {
 
 // _get_gradient_larger, 3D (StaticBTESynthetic.cpp:1329-1371), for the cells
 // of one tile: grad = sum_j (P_j / L_x) (e_j - e_i), zero for a stencil of 3
-// or fewer cells or a NaN.  The tile's staged rows (its cells and their
-// stencils) are copied to LDS one 64-entry slice at a time; each wave then
-// takes one cell of the tile at a time, its lanes one entry each, with the
-// stencil weights in scalar registers.  Arithmetic in the storage type.
+// or fewer cells (zero weights) or a NaN.  The tile's stencil table (row,
+// weights) is copied to LDS once; the staged rows (its cells and their
+// stencils) one 64-entry slice at a time.  Each wave then takes one cell of
+// the tile at a time, its lanes one entry each: every stencil read (the
+// table entry, a broadcast, and the row value) is an LDS read.  Arithmetic in
+// the storage type.
 template <typename ST>
 __global__ void __launch_bounds__(kG3Block) k_grad3(const ST *__restrict__ e, ST *__restrict__ grad, G3Dev g,
-                                                    const G3Ent<ST> *__restrict__ ent, int n_tiles, int DB) {
+                                                    const G3Ent<ST> *__restrict__ ent, int n_tiles, int DB,
+                                                    int row_cap, int group) {
     extern __shared__ double lds_raw[];
     ST *rowbuf = reinterpret_cast<ST *>(lds_raw);
-    int tile = xcd_block();
-    if (tile >= n_tiles) return;
+    int *cinfo = reinterpret_cast<int *>(rowbuf + (size_t)row_cap * kG3W);  // per tile cell: first entry, end, row
+    G3Ent<ST> *tab = reinterpret_cast<G3Ent<ST> *>(cinfo + 3 * kG3CellsMax);
+    // workgroup = (slice group, tile), tiles fastest: the workgroups an XCD
+    // runs together take Morton-consecutive tiles at the same slices, so the
+    // halo rows that neighbouring tiles share are read from its L2
+    int id = xcd_block(), sg = id / n_tiles, tile = id - sg * n_tiles;
+    int s_begin = sg * group * kG3W, s_end = min(DB, s_begin + group * kG3W);
+    if (s_begin >= DB) return;
     int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     int c0 = g.tile_cell0[tile], c1 = g.tile_cell0[tile + 1];
     int r0 = g.rptr[tile], nr = g.rptr[tile + 1] - r0;
+    int e0 = g.sptr[c0], ne = g.sptr[c1] - e0;
     const int *rows = g.rows + r0;
-    for (int s0 = 0; s0 < DB; s0 += kG3W) {
+    for (int i = threadIdx.x; i < c1 - c0; i += blockDim.x) {
+        cinfo[3 * i] = g.sptr[c0 + i] - e0;
+        cinfo[3 * i + 1] = g.sptr[c0 + i + 1] - e0;
+        cinfo[3 * i + 2] = g.self[c0 + i];
+    }
+    {
+        // the table as 16-byte words
+        constexpr int kW = sizeof(G3Ent<ST>) / 16;
+        const int4 *src = reinterpret_cast<const int4 *>(ent + e0);
+        int4 *dst = reinterpret_cast<int4 *>(tab);
+        for (int i = threadIdx.x; i < ne * kW; i += blockDim.x) dst[i] = src[i];
+    }
+    // wave w stages rows w, w + 8, ...: the next slice's values are loaded
+    // into registers before this slice's products, so that the loads are in
+    // flight while the wave computes
+    ST nxt[kG3RowsPerWave];
+    auto load_slice = [&](int s0) {
+        int tt = s0 + lane < DB ? s0 + lane : s0;  // a valid address for lanes past the row end
+#pragma unroll
+        for (int k = 0; k < kG3RowsPerWave; ++k) {
+            int r = w + k * kG3Waves;
+            if (r < nr) nxt[k] = e[(size_t)rows[r] * DB + tt];
+        }
+    };
+    load_slice(s_begin);
+    for (int s0 = s_begin; s0 < s_end; s0 += kG3W) {
         int t = s0 + lane;
         bool in = t < DB;
-        int tt = in ? t : s0;  // a valid address for lanes past the row end
-        __syncthreads();  // the previous slice's reads are done
-        for (int rb = w * kG3Batch; rb < nr; rb += kG3Waves * kG3Batch) {
-            ST v[kG3Batch];
+        __syncthreads();  // the table is in place; the previous slice's reads are done
 #pragma unroll
-            for (int k = 0; k < kG3Batch; ++k) {
-                int r = rb + k < nr ? rb + k : rb;
-                v[k] = e[(size_t)rows[r] * DB + tt];
-            }
-#pragma unroll
-            for (int k = 0; k < kG3Batch; ++k)
-                if (rb + k < nr) rowbuf[(rb + k) * kG3W + lane] = v[k];
+        for (int k = 0; k < kG3RowsPerWave; ++k) {
+            int r = w + k * kG3Waves;
+            if (r < nr) rowbuf[r * kG3W + lane] = nxt[k];
         }
         __syncthreads();
-        for (int c = c0 + w; c < c1; c += kG3Waves) {
-            int a = g.sptr[c], z = g.sptr[c + 1];
-            ST self = rowbuf[g.self[c] * kG3W + lane];
+        if (s0 + kG3W < s_end) load_slice(s0 + kG3W);
+        for (int ci = w; ci < c1 - c0; ci += kG3Waves) {
+            int a = cinfo[3 * ci], z = cinfo[3 * ci + 1];
+            ST self = rowbuf[cinfo[3 * ci + 2] * kG3W + lane];
             ST gx = 0, gy = 0, gz = 0;
-            if (g.ok[c]) {
-                for (int k = a; k < z; ++k) {
-                    G3Ent<ST> q = ent[k];
-                    ST d = rowbuf[q.loc * kG3W + lane] - self;
-                    gx += q.px * d;
-                    gy += q.py * d;
-                    gz += q.pz * d;
+            // stencils are padded to whole batches (zero weights on the
+            // cell's own row); a batch's reads are issued before its products
+            for (int k = a; k < z; k += kG3Ent) {
+                G3Ent<ST> q[kG3Ent];
+#pragma unroll
+                for (int j = 0; j < kG3Ent; ++j) q[j] = tab[k + j];
+                ST v[kG3Ent];
+#pragma unroll
+                for (int j = 0; j < kG3Ent; ++j) v[j] = rowbuf[q[j].loc * kG3W + lane];
+#pragma unroll
+                for (int j = 0; j < kG3Ent; ++j) {
+                    ST d = v[j] - self;
+                    gx += q[j].px * d;
+                    gy += q[j].py * d;
+                    gz += q[j].pz * d;
                 }
-                if (isnan(gx) || isnan(gy) || isnan(gz)) { gx = 0; gy = 0; gz = 0; }
             }
+            if (isnan(gx) || isnan(gy) || isnan(gz)) { gx = 0; gy = 0; gz = 0; }
             if (in) {
-                ST *gc = grad + (size_t)c * 3 * DB + t;
+                ST *gc = grad + (size_t)(c0 + ci) * 3 * DB + t;
                 __builtin_nontemporal_store(gx, gc);
                 __builtin_nontemporal_store(gy, gc + DB);
                 __builtin_nontemporal_store(gz, gc + 2 * DB);
             }
         }
+    }
+}
+
+// The same gradient with the neighbour rows read straight from memory (L2):
+// one wave per (cell, group of kG3GS 64-entry slices), the lanes over the
+// entries.  A batch of stencil-table entries (scalar loads) serves the whole
+// group, and each batch issues kG3Ent * kG3GS row loads before its products.
+// Waves of Morton-consecutive cells run together and read largely the same
+// neighbour rows.
+constexpr int kG3GS = 4;
+template <typename ST>
+__global__ void __launch_bounds__(256) k_grad3g(const ST *__restrict__ e, ST *__restrict__ grad,
+                                                const int *__restrict__ sptr, const int *__restrict__ gidx,
+                                                const G3Ent<ST> *__restrict__ ent, int nc, int DB, int ngroups) {
+    int wv = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    int sg = wv / nc, c = wv - sg * nc;
+    if (sg >= ngroups) return;  // whole wave
+    int a = sptr[c], z = sptr[c + 1];
+    int tb = sg * kG3GS * kG3W + lane;
+    ST self[kG3GS], gx[kG3GS], gy[kG3GS], gz[kG3GS];
+    int tt[kG3GS];
+#pragma unroll
+    for (int q = 0; q < kG3GS; ++q) {
+        int t = tb + q * kG3W;
+        tt[q] = t < DB ? t : 0;
+        self[q] = e[(size_t)c * DB + tt[q]];
+        gx[q] = 0; gy[q] = 0; gz[q] = 0;
+    }
+    for (int k = a; k < z; k += kG3Ent) {
+        G3Ent<ST> w[kG3Ent];
+        int nb[kG3Ent];
+#pragma unroll
+        for (int j = 0; j < kG3Ent; ++j) {
+            w[j] = ent[k + j];
+            nb[j] = gidx[k + j];
+        }
+        ST v[kG3Ent][kG3GS];
+#pragma unroll
+        for (int j = 0; j < kG3Ent; ++j)
+#pragma unroll
+            for (int q = 0; q < kG3GS; ++q) v[j][q] = e[(size_t)nb[j] * DB + tt[q]];
+#pragma unroll
+        for (int j = 0; j < kG3Ent; ++j)
+#pragma unroll
+            for (int q = 0; q < kG3GS; ++q) {
+                ST d = v[j][q] - self[q];
+                gx[q] += w[j].px * d;
+                gy[q] += w[j].py * d;
+                gz[q] += w[j].pz * d;
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < kG3GS; ++q) {
+        int t = tb + q * kG3W;
+        if (t >= DB) continue;
+        ST x = gx[q], y = gy[q], w = gz[q];
+        if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
+        ST *gc = grad + (size_t)c * 3 * DB + t;
+        __builtin_nontemporal_store(x, gc);
+        __builtin_nontemporal_store(y, gc + DB);
+        __builtin_nontemporal_store(w, gc + 2 * DB);
     }
 }
 
@@ -437,6 +537,11 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     double invLx = 1.0 / p.deck.Lx;
     for (int c = 0; c < nc; ++c)
         if (p.face_size[c] != 4) { err_ = "3D meshes of tetrahedra only"; return false; }
+    // LDS budgets of the gradient tiles: FP64 192 staged rows (96 KiB) and a
+    // 1536-entry table (48 KiB); FP32 320 rows (80 KiB) and 2560 entries (40 KiB)
+    if (g3_row_cap <= 0) g3_row_cap = fp32_ ? 320 : 192;
+    if (g3_row_cap > kG3Waves * kG3RowsPerWave) { err_ = "3D gradient tile row budget above the kernel's"; return false; }
+    if (g3_ent_cap <= 0) g3_ent_cap = fp32_ ? 2560 : 1536;
     // ---- source records
     {
         std::vector<double> rec((size_t)nc * kS3D, 0.0);
@@ -509,7 +614,8 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     // stencils stays within the row budget
     {
         int nnz = p.nbr_ptr[nc];
-        int cap = g3_row_cap;
+        int cap = g3_row_cap, ecap = g3_ent_cap, ccap = std::min(g3_cell_cap, kG3CellsMax);
+        auto padded = [&](int c) { return (p.nbr_ptr[c + 1] - p.nbr_ptr[c] + kG3Ent - 1) / kG3Ent * kG3Ent; };
         std::vector<int> tile0(1, 0), rptr(1, 0), rows, sptr(nc + 1, 0), self(nc, 0);
         std::vector<int> loc_of(nc, -1), tile_rows;
         std::vector<int> e_loc;
@@ -523,9 +629,14 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
         while (c < nc) {
             tile_rows.clear();
             int cstart = c;
-            while (c < nc && c - cstart < g3_cell_cap) {
-                if (c > cstart && row_count(c, tile_rows) > cap) break;
-                if (row_count(c, tile_rows) > cap) { err_ = "a 3D LSQ stencil is larger than the tile row budget"; return false; }
+            int tile_ents = 0;
+            while (c < nc && c - cstart < ccap) {
+                if (c > cstart && (row_count(c, tile_rows) > cap || tile_ents + padded(c) > ecap)) break;
+                if (row_count(c, tile_rows) > cap || padded(c) > ecap) {
+                    err_ = "a 3D LSQ stencil is larger than the tile budget";
+                    return false;
+                }
+                tile_ents += padded(c);
                 auto add_row = [&](int r) {
                     if (loc_of[r] < 0) { loc_of[r] = (int)tile_rows.size(); tile_rows.push_back(r); }
                 };
@@ -540,6 +651,12 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                     e_loc.push_back(loc_of[p.nbr_idx[k]]);
                     e_cell.push_back(k);
                 }
+                // pad to whole batches with the cell's own row at weight 0
+                // (each adds an exact zero)
+                while ((e_loc.size() - sptr[cc]) % kG3Ent) {
+                    e_loc.push_back(loc_of[cc]);
+                    e_cell.push_back(-1);
+                }
                 sptr[cc + 1] = (int)e_loc.size();
             }
             for (int r : tile_rows) loc_of[r] = -1;
@@ -549,7 +666,11 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
         }
         g3_ntiles = (int)tile0.size() - 1;
         g3_rows = 0;
-        for (int t = 0; t < g3_ntiles; ++t) g3_rows = std::max(g3_rows, rptr[t + 1] - rptr[t]);
+        g3_ents = 0;
+        for (int t = 0; t < g3_ntiles; ++t) {
+            g3_rows = std::max(g3_rows, rptr[t + 1] - rptr[t]);
+            g3_ents = std::max(g3_ents, sptr[tile0[t + 1]] - sptr[tile0[t]]);
+        }
         // validate what the kernel assumes: every local row names the cell it
         // stands for and lies inside its tile
         for (int t = 0; t < g3_ntiles; ++t)
@@ -558,7 +679,8 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 if (self[cc] < 0 || self[cc] >= nr || rows[rptr[t] + self[cc]] != cc) { err_ = "internal error: 3D tile self row"; return false; }
                 for (int k = sptr[cc]; k < sptr[cc + 1]; ++k) {
                     int l = e_loc[k];
-                    if (l < 0 || l >= nr || rows[rptr[t] + l] != p.nbr_idx[e_cell[k]]) { err_ = "internal error: 3D tile stencil row"; return false; }
+                    int want = e_cell[k] >= 0 ? p.nbr_idx[e_cell[k]] : cc;
+                    if (l < 0 || l >= nr || rows[rptr[t] + l] != want) { err_ = "internal error: 3D tile stencil row"; return false; }
                 }
             }
         size_t ne3 = e_loc.size();
@@ -567,9 +689,9 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
             for (size_t k = 0; k < ne3; ++k) {
                 int j = e_cell[k];
                 ent[k].loc = e_loc[k];
-                ent[k].px = (float)(P3[j] * invLx);
-                ent[k].py = (float)(P3[(size_t)nnz + j] * invLx);
-                ent[k].pz = (float)(P3[2 * (size_t)nnz + j] * invLx);
+                ent[k].px = j < 0 ? 0.0f : (float)(P3[j] * invLx);  // zero without a gradient (P3 = 0)
+                ent[k].py = j < 0 ? 0.0f : (float)(P3[(size_t)nnz + j] * invLx);
+                ent[k].pz = j < 0 ? 0.0f : (float)(P3[2 * (size_t)nnz + j] * invLx);
             }
             DBuf<char> &dst = d_g3_ent;
             if (!alloc(dst, ne3 * sizeof(G3Ent<float>)) ||
@@ -581,22 +703,60 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 int j = e_cell[k];
                 ent[k].loc = e_loc[k];
                 ent[k].pad = 0;
-                ent[k].px = P3[j] * invLx;
-                ent[k].py = P3[(size_t)nnz + j] * invLx;
-                ent[k].pz = P3[2 * (size_t)nnz + j] * invLx;
+                ent[k].px = j < 0 ? 0.0 : P3[j] * invLx;
+                ent[k].py = j < 0 ? 0.0 : P3[(size_t)nnz + j] * invLx;
+                ent[k].pz = j < 0 ? 0.0 : P3[2 * (size_t)nnz + j] * invLx;
             }
             DBuf<char> &dst = d_g3_ent;
             if (!alloc(dst, ne3 * sizeof(G3Ent<double>)) ||
                 !check(hipMemcpy(dst.p, ent.data(), ne3 * sizeof(G3Ent<double>), hipMemcpyHostToDevice), "H2D"))
                 return false;
         }
-        g3_lds = (size_t)g3_rows * kG3W * (fp32_ ? sizeof(float) : sizeof(double));
+        g3_lds = (size_t)g3_rows * kG3W * (fp32_ ? sizeof(float) : sizeof(double)) + 3 * kG3CellsMax * sizeof(int) +
+                 (size_t)g3_ents * (fp32_ ? sizeof(G3Ent<float>) : sizeof(G3Ent<double>));
         int dev_lds = 0;
         (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, opt.device);
         if (g3_lds > (size_t)dev_lds) { err_ = "3D gradient tiles exceed the LDS of a workgroup"; return false; }
         if (!upload(d_g3_tile, tile0) || !upload(d_g3_rptr, rptr) || !upload(d_g3_rows, rows) ||
             !upload(d_g3_sptr, sptr) || !upload(d_g3_self, self) || !upload(d_g3_ok, gok))
             return false;
+        // the same stencils for the direct-gather kernel: global neighbour
+        // rows, padded per cell to whole batches with the cell's own row
+        std::vector<int> gptr2(nc + 1, 0), gidx2;
+        std::vector<int> gpos;  // stencil position per entry (-1: padding)
+        for (int cc = 0; cc < nc; ++cc) {
+            for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) { gidx2.push_back(p.nbr_idx[k]); gpos.push_back(k); }
+            while ((gidx2.size() - gptr2[cc]) % kG3Ent) { gidx2.push_back(cc); gpos.push_back(-1); }
+            gptr2[cc + 1] = (int)gidx2.size();
+        }
+        size_t n2 = gidx2.size();
+        if (fp32_) {
+            std::vector<G3Ent<float>> e2(n2);
+            for (size_t k = 0; k < n2; ++k) {
+                int j = gpos[k];
+                e2[k].loc = gidx2[k];
+                e2[k].px = j < 0 ? 0.0f : (float)(P3[j] * invLx);
+                e2[k].py = j < 0 ? 0.0f : (float)(P3[(size_t)nnz + j] * invLx);
+                e2[k].pz = j < 0 ? 0.0f : (float)(P3[2 * (size_t)nnz + j] * invLx);
+            }
+            if (!alloc(d_g3g_ent, n2 * sizeof(e2[0])) ||
+                !check(hipMemcpy(d_g3g_ent.p, e2.data(), n2 * sizeof(e2[0]), hipMemcpyHostToDevice), "H2D"))
+                return false;
+        } else {
+            std::vector<G3Ent<double>> e2(n2);
+            for (size_t k = 0; k < n2; ++k) {
+                int j = gpos[k];
+                e2[k].loc = gidx2[k];
+                e2[k].pad = 0;
+                e2[k].px = j < 0 ? 0.0 : P3[j] * invLx;
+                e2[k].py = j < 0 ? 0.0 : P3[(size_t)nnz + j] * invLx;
+                e2[k].pz = j < 0 ? 0.0 : P3[2 * (size_t)nnz + j] * invLx;
+            }
+            if (!alloc(d_g3g_ent, n2 * sizeof(e2[0])) ||
+                !check(hipMemcpy(d_g3g_ent.p, e2.data(), n2 * sizeof(e2[0]), hipMemcpyHostToDevice), "H2D"))
+                return false;
+        }
+        if (!upload(d_g3g_ptr, gptr2) || !upload(d_g3g_idx, gidx2)) return false;
     }
     // ---- moments and the residual's face coefficients
     std::vector<std::vector<double>> FM(nf);  // per unique face, from its first side: 3 x nv
@@ -736,16 +896,29 @@ void Solver::launch_rhs3(bool macro, const double *bval, double *rhs) {
 // The 3D transport kernels of one outer iteration (and of the set-up), on
 // the solver's stream.
 void Solver::launch_grad3() {
-    G3Dev g{d_g3_tile.p, d_g3_rptr.p, d_g3_rows.p, d_g3_sptr.p, d_g3_self.p, d_g3_ok.p};
-    int grid = ((g3_ntiles + kXcd - 1) / kXcd) * kXcd;
     int DB = nD * nB;
+    if (g3_mode == 1) {
+        int ngroups = (DB + kG3GS * kG3W - 1) / (kG3GS * kG3W);
+        int grid = ((blocks_for((long)ngroups * nc, 4) + kXcd - 1) / kXcd) * kXcd;
+        if (fp32_)
+            hipLaunchKernelGGL(k_grad3g<float>, dim3(grid), dim3(256), 0, s_, reinterpret_cast<const float *>(d_e.p),
+                               reinterpret_cast<float *>(d_grad.p), d_g3g_ptr.p, d_g3g_idx.p,
+                               reinterpret_cast<const G3Ent<float> *>(d_g3g_ent.p), nc, DB, ngroups);
+        else
+            hipLaunchKernelGGL(k_grad3g<double>, dim3(grid), dim3(256), 0, s_, d_e.p, d_grad.p, d_g3g_ptr.p,
+                               d_g3g_idx.p, reinterpret_cast<const G3Ent<double> *>(d_g3g_ent.p), nc, DB, ngroups);
+        return;
+    }
+    G3Dev g{d_g3_tile.p, d_g3_rptr.p, d_g3_rows.p, d_g3_sptr.p, d_g3_self.p, d_g3_ok.p};
+    int groups = (DB + g3_group * kG3W - 1) / (g3_group * kG3W);
+    int grid = ((groups * g3_ntiles + kXcd - 1) / kXcd) * kXcd;
     if (fp32_)
         hipLaunchKernelGGL(k_grad3<float>, dim3(grid), dim3(kG3Block), g3_lds, s_, reinterpret_cast<float *>(d_e.p),
                            reinterpret_cast<float *>(d_grad.p), g,
-                           reinterpret_cast<const G3Ent<float> *>(d_g3_ent.p), g3_ntiles, DB);
+                           reinterpret_cast<const G3Ent<float> *>(d_g3_ent.p), g3_ntiles, DB, g3_rows, g3_group);
     else
         hipLaunchKernelGGL(k_grad3<double>, dim3(grid), dim3(kG3Block), g3_lds, s_, d_e.p, d_grad.p, g,
-                           reinterpret_cast<const G3Ent<double> *>(d_g3_ent.p), g3_ntiles, DB);
+                           reinterpret_cast<const G3Ent<double> *>(d_g3_ent.p), g3_ntiles, DB, g3_rows, g3_group);
 }
 
 void Solver::launch_source3(const Modes &m) {
@@ -788,6 +961,65 @@ void Solver::launch_moments3(const Modes &m) {
                        d_vwt.p, d_YV.p, nn, nc);
     hipLaunchKernelGGL(k_fin3, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, s_, d_Y.p, d_YV.p, d_f3i.p, d_f3d.p,
                        m, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc);
+}
+
+// Project autotuner for the 3D gradient: times each candidate schedule of
+// k_grad3 (LDS tiles at the built budgets, or the direct-gather form) over a
+// few launches on this mesh and keeps the fastest.  The choice and the
+// timings are kept in tune_log_ (gbte_tuning).
+bool Solver::tune3d() {
+    if (!check(hipMemsetAsync(d_e.p, 0, d_e.n * sizeof(double), s_), "hipMemsetAsync")) return false;
+    hipEvent_t a, b;
+    if (!check(hipEventCreate(&a), "hipEventCreate") || !check(hipEventCreate(&b), "hipEventCreate")) return false;
+    double best = 1e300;
+    int pick = 0;
+    char line[160];
+    for (int mode = 0; mode < 2; ++mode) {
+        g3_mode = mode;
+        launch_grad3();  // warm
+        (void)hipEventRecord(a, s_);
+        for (int r = 0; r < 3; ++r) launch_grad3();
+        (void)hipEventRecord(b, s_);
+        if (!check(hipEventSynchronize(b), "tune sync")) return false;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        double per = ms / 3.0;
+        snprintf(line, sizeof line, "k_grad3 %s: %.3f ms per launch\n",
+                 mode == 0 ? "LDS tiles" : "direct gather", per);
+        tune_log_ += line;
+        if (per < best) { best = per; pick = mode; }
+    }
+    g3_mode = pick;
+    snprintf(line, sizeof line, "k_grad3 chosen: %s\n", pick == 0 ? "LDS tiles" : "direct gather");
+    tune_log_ += line;
+    // FP64 sweeps: the wave-specialised kernel against the level-synchronous
+    // one (FP32 storage runs only the former)
+    if (use_pc && !fp32_) {
+        if (!check(hipMemsetAsync(d_S.p, 0, d_S.n * sizeof(double), s_), "hipMemsetAsync")) return false;
+        Modes m = modes();
+        double t[2] = {0, 0};
+        for (int v = 0; v < 2; ++v) {
+            use_pc = v == 0;
+            launch_sweep3(m);  // warm
+            (void)hipEventRecord(a, s_);
+            for (int r = 0; r < 3; ++r) launch_sweep3(m);
+            (void)hipEventRecord(b, s_);
+            if (!check(hipEventSynchronize(b), "tune sync")) return false;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            t[v] = ms / 3.0;
+            snprintf(line, sizeof line, "sweep %s: %.3f ms per launch\n",
+                     v == 0 ? "wave-specialised" : "level-synchronous", t[v]);
+            tune_log_ += line;
+        }
+        use_pc = t[0] <= t[1];
+        snprintf(line, sizeof line, "sweep chosen: %s\n", use_pc ? "wave-specialised" : "level-synchronous");
+        tune_log_ += line;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (opt.verbose) printf("%s", tune_log_.c_str());
+    return check(hipGetLastError(), "tune launches");
 }
 
 }  // namespace gbte

@@ -203,6 +203,11 @@ int gbte_write_fields(const gbte_problem *p, const double *T_lattice, const doub
  * function; the MATLAB check bin/examples/cross-plane/verify_kappa_crossplane.m:8
  * computes mean(q) L / dT, which equals this on full equal-area domains. */
 int gbte_keff(gbte_handle *h, int axis, double delta_T, double *keff);
+/* What the handle's set-up timed and chose among kernel variants (one line
+ * per candidate with its milliseconds per launch, then the choice; empty
+ * where nothing was tuned).  Valid until gbte_destroy.  No reference
+ * counterpart: the reference has one implementation of each loop. */
+const char *gbte_tuning_log(const gbte_handle *h);
 const char *gbte_last_error(const gbte_handle *h);
 const char *gbte_problem_error(void);
 void gbte_destroy(gbte_handle *h);
