@@ -1,26 +1,37 @@
 """Benchmark: steady StaticBTESynthetic outer iterations on one MI355X.
 
-Workload (the 2D non-gray nanoporous film): 500 nm square minus a centred
-r = 150 nm diffuse pore, seeded Delaunay mesh of about 50k triangles, 40
-Debye-like bands, 64 in-plane directions (CADOM, Nphi 16), left/right
-isothermal 301/299 K, top/bottom periodic.  Synthetic data from the in-repo
+Default workload (the 3D film): a 200 nm cube with a centred 80 nm cubic
+void, seeded Delaunay tetrahedra from a jittered 37^3 lattice (about 295k
+cells), 40 Debye-like bands, 96 directions (Gauss-Legendre, 4 polar x 24
+azimuthal), x = 0 / x = 1 faces isothermal at 301 / 299 K, the other faces
+and the void diffuse.  --workload 2d runs the 2D non-gray nanoporous film
+instead: 500 nm square minus a centred r = 150 nm diffuse pore, seeded
+Delaunay mesh of about 50k triangles, 40 bands, 64 in-plane directions,
+left/right isothermal, top/bottom periodic.  Synthetic data from the in-repo
 seeded generator (giftbte_amd/decks.py).
 
 A "step" is one outer iteration of StaticBTESynthetic::solve: every
 band x direction transport sweep, the moment reductions, the boundary update
-and the FP64 synthetic (macro) solve.  value = cell*band*direction updates/s
-of the plain FP64 solver over the timed steps (inputs resident in HBM); the
-same figures for FP32 storage of the per-system fields (float gradients;
-double sweeps, recovery, synthetic solve and reductions) are under "mixed".  The time to a
-converged k_eff (the deck's tolerances, from the Fourier warm start) is
-reported for both, with the mixed k_eff's error against FP64.
+and the FP64 synthetic (macro) solve.  Updates/s = cell*band*direction
+updates per second over the timed steps (inputs resident in HBM).  Two
+solvers run on the same deck: FP64 storage, and "mixed" (FP32 storage of
+energy densities, gradients and sources; FP64 sweep arithmetic, temperature
+and flux recovery, synthetic solve and reductions).  On the 3D workload the
+headline value is the mixed solver's rate (the precision that workload is
+defined in) with FP64 beside it; on 2D it is FP64 with mixed beside it.  For
+both, the time to a converged k_eff (the deck's tolerances, from the Fourier
+warm start) and the end-to-end time including gbte_create are reported, with
+the mixed k_eff's error against FP64.
 
-The CPU baseline is the reference itself (oracle/_ref/BTE_CPU, compiled from
-its sources) run with MPI ranks on this host on a bounded sample of the
-workload, or, where that binary is absent, the oracle's single-core transport
-loop (also reported beside it as cpu_port).
+The CPU baseline on 2D is the reference itself (oracle/_ref/BTE_CPU, compiled
+from its sources) run with MPI ranks on this host on a bounded sample, or,
+where that binary is absent, the oracle's single-core transport loop.  The
+reference cannot run 3D meshes (its 3D mesh construction is removed and
+BTEMain.cpp:218-225 exits), so on 3D the oracle's single-core transport loop
+over a bounded number of the systems is the baseline.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-converge] [--no-cpu]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3d|2d]
+                       [--no-converge] [--no-cpu] [--batch N]
 """
 import argparse
 import dataclasses
@@ -37,16 +48,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_porous50k.json")
+TRAFFIC_FILES = {"3d": os.path.join(ROOT, "profiles", "traffic_cube3d_300k.json"),
+                 "2d": os.path.join(ROOT, "profiles", "traffic_porous50k.json")}
 
 
-def make_deck(td, n_target, nbands, nphi, seed):
+def make_deck(td, a):
     from giftbte_amd import decks
-    spec = decks.porous_spec(n_target=n_target, nbands=nbands, nphi=nphi, pore_radius=0.3, seed=seed, scale=5e-7)
+    if a.workload == "3d":
+        spec = decks.cube_void_spec(m=a.lattice, nbands=a.bands, ntheta=2, nphi=6, seed=a.seed)
+        return decks.write_deck(os.path.join(td, "cube3d"), spec)
+    spec = decks.porous_spec(n_target=a.cells, nbands=a.bands, nphi=a.nphi, pore_radius=0.3, seed=a.seed, scale=5e-7)
     return decks.write_deck(os.path.join(td, "porous"), spec)
 
 
-def cpu_baseline(problem, T0, target_s=15.0):
+def cpu_baseline(problem, T0, target_s=12.0):
     """The CPU oracle's per-system transport loop on the same mesh and
     modes, timed on a bounded number of systems (single core)."""
     import ctypes as C
@@ -70,7 +85,12 @@ def cpu_baseline(problem, T0, target_s=15.0):
             "sample": f"oracle transport loop (gradient, residual + exact sweep, boundary values, vertex "
                       f"IDW, face gradients, non-Fourier residual, moments) over {n} of "
                       f"{t.n_bands * t.n_dirs} band x direction systems of one outer iteration, same mesh; "
-                      f"{sec:.1f} s"}
+                      f"{sec:.1f} s on 1 of {host_cores()} host cores"}
+
+
+def host_cores():
+    """Cores this process may run on (the host's share on a shared box)."""
+    return len(os.sched_getaffinity(0))
 
 
 def reference_baseline(td, n_target, nbands, seed, nphi=4, iters=2, timeout=900):
@@ -88,7 +108,7 @@ def reference_baseline(td, n_target, nbands, seed, nphi=4, iters=2, timeout=900)
         return None
     from giftbte_amd import decks
     nd = 4 * nphi
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores = min(16, host_cores())
     ranks = max(r for r in range(1, nd + 1) if nd % r == 0 and r <= cores)
     spec = decks.porous_spec(n_target=n_target, nbands=nbands, nphi=nphi, pore_radius=0.3, seed=seed, scale=5e-7)
     spec = dataclasses.replace(spec, max_iter=iters, tol_temp=0.0, tol_flux=0.0)
@@ -117,19 +137,35 @@ def reference_baseline(td, n_target, nbands, seed, nphi=4, iters=2, timeout=900)
 
 def kernel_bytes_per_update(dim, word=8):
     """Algorithmic HBM bytes per cell*band*dir update of each transport kernel
-    (word = 8 for FP64 storage, 4 for FP32): the source kernel reads the LSQ
-    gradient (dim values) and writes the sweep source; the sweep reads the
-    source and writes e; the face / moment kernel reads e and writes the
-    gradient.  Tables shared by all band x direction systems are excluded."""
-    return {"k_source": (dim + 1) * word, "k_sweep_pc": 2 * word, "k_tile_face": (1 + dim) * word}
+    (word = 8 for FP64 storage, 4 for FP32), tables shared by all
+    band x direction systems excluded:
+      source     reads the LSQ gradient (dim values), writes the sweep source
+      sweep      reads the source, writes e
+      2D face    k_tile_face reads e, writes the gradient (dim values)
+      3D grad    k_grad3 reads e, writes the gradient (3 values)
+      3D moments the moment kernels read e"""
+    if dim == 3:
+        return {"source": 4 * word, "sweep": 2 * word, "grad": 4 * word, "moments": word}
+    return {"source": (dim + 1) * word, "sweep": 2 * word, "face": (1 + dim) * word}
 
 
-def load_traffic(kernel, cfg, prec):
+def kernel_name(phase, dim, prec, tuning):
+    """The kernel a timed phase launches (the tuner's choice on 3D)."""
+    if dim == 2:
+        return {"source": "k_source", "sweep": "k_sweep_pc", "face": "k_tile_face"}[phase]
+    if phase == "sweep":
+        return "k_sweep_dir" if "sweep chosen: level-synchronous" in tuning else "k_sweep_pc"
+    if phase == "grad":
+        return "k_grad3" if "k_grad3 chosen: LDS tiles" in tuning else "k_grad3g"
+    return {"source": "k_source3", "moments": "k_mom3 + k_vert3 + k_fin3"}[phase]
+
+
+def load_traffic(kernel, cfg, prec, workload):
     """HBM bytes per launch of `kernel` at storage precision `prec` from the
     committed rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE scaled by a
     copy-kernel calibration), if it was measured on this workload."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILES[workload]) as f:
             tr = json.load(f)
     except (OSError, ValueError):
         return None
@@ -190,17 +226,81 @@ def run_batch(a):
         print(json.dumps(out))
 
 
+def run_precision(p, a, prec, cfg, U, T0_out):
+    """Set-up, warm-up, K timed steps and (unless --no-converge) a solve to
+    the deck tolerances from the Fourier warm start, on one handle."""
+    from giftbte_amd import BTESolver
+    t = p.tables
+    K = a.steps
+    t_c = time.time()
+    s = BTESolver(p, precision=prec)
+    create_s = time.time() - t_c
+    tuning = s.tuning_log()
+    fp = s.fourier()
+    if not T0_out:
+        T0_out.append(s.fields()["T_lattice"])
+    st_w = s.solve(a.warmup, 0.0, 0.0)
+    print(f"[bench] {prec}: gbte_create {create_s:.2f} s, fourier ({fp} passes), warmup {st_w.iterations} it "
+          f"{st_w.solve_ms:.1f} ms", file=sys.stderr, flush=True)
+    t1 = time.time()
+    st = s.solve(K, 0.0, 0.0)  # returns after the device has finished
+    wall = time.time() - t1
+    bpu = kernel_bytes_per_update(t.dim, 8 if prec == "fp64" else 4)
+    if t.dim == 3:
+        ktime = {"source": st.source_ms, "sweep": st.sweep_ms, "grad": st.grad_ms,
+                 "moments": st.face_ms - st.grad_ms}
+    else:
+        ktime = {"source": st.source_ms, "sweep": st.sweep_ms, "face": st.face_ms}
+    kernels = {}
+    for ph, ms in ktime.items():
+        per = ms / K
+        gbs = bpu[ph] * U / (per * 1e-3) / 1e9 if per > 0 else 0.0
+        kernels[kernel_name(ph, t.dim, prec, tuning)] = {
+            "ms_per_launch": per, "bytes_per_update": bpu[ph], "achieved_GBs": gbs,
+            "frac_hbm_peak": gbs / HBM_PEAK_GBS}
+    dom_ph = max(ktime, key=ktime.get)
+    dom = kernel_name(dom_ph, t.dim, prec, tuning)
+    r = {"value": U * K / wall, "ms_per_step": wall * 1e3 / K,
+         "breakdown_ms_per_step": {"source": st.source_ms / K, "sweep": st.sweep_ms / K,
+                                   "face_moments": st.face_ms / K, "gradient": st.grad_ms / K,
+                                   "transport": st.transport_ms / K, "macro": st.macro_ms / K,
+                                   "macro_passes_per_step": st.macro_passes / K,
+                                   "sweep_levels": st.sweep_levels},
+         "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"],
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac_hbm_peak"],
+                      "traffic": load_traffic(dom, cfg, prec, a.workload),
+                      "bytes_per_update": bpu[dom_ph], "updates_per_launch": U},
+         "kernels": kernels, "gbte_create_s": create_s,
+         "tuning": [ln for ln in tuning.splitlines() if ln]}
+    if not a.no_converge:
+        tc0 = time.time()
+        s.fourier()
+        stc = s.solve(t.max_iter, t.tol_temp, t.tol_flux)
+        r["time_to_converged_s"] = time.time() - tc0
+        r["end_to_end_s"] = create_s + r["time_to_converged_s"]
+        r["converged"] = bool(stc.converged)
+        r["converged_iterations"] = stc.iterations
+        r["keff_W_per_mK"] = s.keff(0, 2.0)
+        print(f"[bench] {prec}: converged={r['converged']} in {stc.iterations} it, "
+              f"{r['time_to_converged_s']:.3f} s, k_eff {r['keff_W_per_mK']:.9g}", file=sys.stderr, flush=True)
+    s.close()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cells", type=int, default=50000)
+    ap.add_argument("--workload", choices=("3d", "2d"), default="3d")
+    ap.add_argument("--lattice", type=int, default=36, help="3d: lattice cells per cube edge (36: about 295k tets)")
+    ap.add_argument("--cells", type=int, default=50000, help="2d: target triangle count")
     ap.add_argument("--bands", type=int, default=40)
-    ap.add_argument("--nphi", type=int, default=16)
+    ap.add_argument("--nphi", type=int, default=16, help="2d: directions / 4")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-converge", action="store_true", help="skip the timed solve to the deck tolerance")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--precisions", default="fp64,fp32", help="storage precisions to run (default both)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="instead of the default line: N independent porous instances solved side by side")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -210,7 +310,7 @@ def main():
         print("bench.py measures one GPU: this build runs the whole problem on a single MI355X "
               "(no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
-    from giftbte_amd import Problem, BTESolver
+    from giftbte_amd import Problem
 
     if a.batch > 0:
         run_batch(a)
@@ -228,7 +328,7 @@ def main():
 
     with tempfile.TemporaryDirectory() as td:
         t_gen = time.time()
-        deck = make_deck(td, a.cells, a.bands, a.nphi, a.seed)
+        deck = make_deck(td, a)
         p = Problem(deck)
         t = p.tables
         nc, nB, nD = t.n_cells, t.n_bands, t.n_dirs
@@ -236,77 +336,46 @@ def main():
               file=sys.stderr, flush=True)
         U = nc * nB * nD
         K = a.steps
-        cfg = {"workload": "2D porous film 500 nm, r=150 nm diffuse pore, seeded Delaunay, 40 bands, 64 dirs",
-               "cells": nc, "bands": nB, "directions": nD, "systems": nB * nD, "updates_per_step": U,
-               "parallelism": "single GPU"}
+        if a.workload == "3d":
+            wl = (f"3D film: 200 nm cube, centred 80 nm cubic void, seeded Delaunay tetrahedra, {nB} bands, "
+                  f"{nD} dirs (4 polar x 24 azimuthal)")
+        else:
+            wl = f"2D porous film 500 nm, r=150 nm diffuse pore, seeded Delaunay, {nB} bands, {nD} dirs"
+        cfg = {"workload": wl, "cells": nc, "bands": nB, "directions": nD, "systems": nB * nD,
+               "updates_per_step": U, "parallelism": "single GPU"}
         runs = {}
-        T0 = None
-        for prec in ("fp64", "fp32"):
-            t_c = time.time()
-            s = BTESolver(p, precision=prec)
-            fp = s.fourier()
-            if T0 is None:
-                T0 = s.fields()["T_lattice"]
-            st_w = s.solve(a.warmup, 0.0, 0.0)
-            print(f"[bench] {prec}: set-up + fourier ({fp} passes) {time.time() - t_c:.2f} s, warmup "
-                  f"{st_w.iterations} it {st_w.solve_ms:.1f} ms", file=sys.stderr, flush=True)
-            t1 = time.time()
-            st = s.solve(a.steps, 0.0, 0.0)  # returns after the device has finished
-            wall = time.time() - t1
-            s.close()
-            bpu = kernel_bytes_per_update(t.dim, 8 if prec == "fp64" else 4)
-            ktime = {"k_source": st.source_ms, "k_sweep_pc": st.sweep_ms, "k_tile_face": st.face_ms}
-            kernels = {}
-            for k, ms in ktime.items():
-                per = ms / K
-                gbs = bpu[k] * U / (per * 1e-3) / 1e9 if per > 0 else 0.0
-                kernels[k] = {"ms_per_launch": per, "bytes_per_update": bpu[k], "achieved_GBs": gbs,
-                              "frac_hbm_peak": gbs / HBM_PEAK_GBS}
-            dom = max(ktime, key=ktime.get)
-            r = {"value": U * K / wall, "ms_per_step": wall * 1e3 / K,
-                 "breakdown_ms_per_step": {"source": st.source_ms / K, "sweep": st.sweep_ms / K,
-                                           "face_moments": st.face_ms / K, "transport": st.transport_ms / K,
-                                           "macro": st.macro_ms / K, "macro_passes_per_step": st.macro_passes / K,
-                                           "sweep_levels": st.sweep_levels},
-                 "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"],
-                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels[dom]["frac_hbm_peak"],
-                              "traffic": load_traffic(dom, cfg, prec),
-                              "bytes_per_update": bpu[dom], "updates_per_launch": U},
-                 "kernels": kernels}
-            if not a.no_converge:
-                s2 = BTESolver(p, precision=prec)
-                tc0 = time.time()
-                s2.fourier()
-                stc = s2.solve(t.max_iter, t.tol_temp, t.tol_flux)
-                r["time_to_converged_s"] = time.time() - tc0
-                r["converged"] = bool(stc.converged)
-                r["converged_iterations"] = stc.iterations
-                r["keff_W_per_mK"] = s2.keff(0, 2.0)
-                s2.close()
-                print(f"[bench] {prec}: converged in {stc.iterations} it, {r['time_to_converged_s']:.3f} s",
-                      file=sys.stderr, flush=True)
-            runs[prec] = r
-        f64, f32 = runs["fp64"], runs["fp32"]
+        T0 = []
+        for prec in a.precisions.split(","):
+            runs[prec] = run_precision(p, a, prec, cfg, U, T0)
+        mixed_head = a.workload == "3d" and "fp32" in runs
+        head = runs["fp32"] if mixed_head else runs[next(iter(runs))]
         out = {
             "metric": "time-to-converged k_eff (1e-6 rel) & cell\u00b7band\u00b7dir updates/s, mixed vs FP64",
-            "value": f64["value"], "unit": "cell*band*dir updates/s", "n_gpus": 1, "steps": K, "warmup": a.warmup,
-            "ms_per_step": f64["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64", "data": "synthetic (seeded porous Delaunay deck, seeded Debye-like bands)",
+            "value": head["value"], "unit": "cell*band*dir updates/s", "n_gpus": 1, "steps": K, "warmup": a.warmup,
+            "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32" if mixed_head else "f64",
+            "dtype_detail": ("f32 storage of e, gradients and sources, f32 gradient arithmetic; f64 sweep "
+                             "arithmetic, recovery, synthetic solve and reductions") if mixed_head else "f64 throughout",
+            "data": f"synthetic (seeded {'tetrahedral cube-with-void' if a.workload == '3d' else 'porous Delaunay'} "
+                    f"deck, seeded Debye-like bands)",
             "config": cfg,
         }
-        out.update({k: v for k, v in f64.items() if k not in ("value", "ms_per_step")})
-        mixed = {"dtype": "f32 storage of e, gradients and sources, f32 gradient arithmetic; f64 sweeps, "
-                          "recovery, synthetic solve and reductions",
-                 "value": f32["value"], "ms_per_step": f32["ms_per_step"],
-                 "speedup_updates_vs_fp64": f32["value"] / f64["value"]}
-        mixed.update({k: v for k, v in f32.items() if k not in ("value", "ms_per_step")})
-        if "keff_W_per_mK" in f64:
-            mixed["keff_rel_err_vs_fp64"] = abs(f32["keff_W_per_mK"] - f64["keff_W_per_mK"]) / abs(f64["keff_W_per_mK"])
-            mixed["speedup_time_to_converged_vs_fp64"] = f64["time_to_converged_s"] / f32["time_to_converged_s"]
-        out["mixed"] = mixed
+        out.update({k: v for k, v in head.items() if k not in ("value", "ms_per_step")})
+        if "fp64" in runs and "fp32" in runs:
+            f64, f32 = runs["fp64"], runs["fp32"]
+            other_key, other = ("fp64", f64) if mixed_head else ("mixed", f32)
+            o = {"value": other["value"], "ms_per_step": other["ms_per_step"]}
+            o.update({k: v for k, v in other.items() if k not in ("value", "ms_per_step")})
+            out[other_key] = o
+            out["mixed_vs_fp64"] = {"speedup_updates": f32["value"] / f64["value"]}
+            if "keff_W_per_mK" in f64:
+                out["mixed_vs_fp64"].update({
+                    "keff_rel_err": abs(f32["keff_W_per_mK"] - f64["keff_W_per_mK"]) / abs(f64["keff_W_per_mK"]),
+                    "speedup_time_to_converged": f64["time_to_converged_s"] / f32["time_to_converged_s"],
+                    "speedup_end_to_end": f64["end_to_end_s"] / f32["end_to_end_s"]})
         if not a.no_cpu:
-            port = cpu_baseline(p, T0)
-            ref = reference_baseline(td, a.cells, a.bands, a.seed)
+            port = cpu_baseline(p, T0[0])
+            ref = reference_baseline(td, a.cells, a.bands, a.seed) if a.workload == "2d" else None
             out["cpu_baseline"] = ref if ref is not None else port
             if ref is not None:
                 out["cpu_port"] = port

@@ -41,6 +41,7 @@ class SolveStats:
     macro_cg_iters: int
     source_ms: float
     face_ms: float
+    grad_ms: float = 0.0
 
 
 class Problem:
@@ -134,7 +135,7 @@ class BTESolver:
         self._check(rc)
         return SolveStats(st.iterations, st.macro_passes, st.fourier_passes, bool(st.converged), st.err_temp,
                           st.err_flux, st.solve_ms, st.sweep_ms, st.transport_ms, st.macro_ms,
-                          st.sweep_levels, st.macro_cg_iters, st.source_ms, st.face_ms)
+                          st.sweep_levels, st.macro_cg_iters, st.source_ms, st.face_ms, st.grad_ms)
 
     def fields(self) -> dict:
         nc, nb = self.nc, self.nb
@@ -198,7 +199,7 @@ def solve_batch(solvers, max_iter, tol_temp, tol_flux):
         raise GBTEError(status[bad], solvers[bad].lib.gbte_last_error(solvers[bad]._h).decode())
     return [SolveStats(x.iterations, x.macro_passes, x.fourier_passes, bool(x.converged), x.err_temp, x.err_flux,
                        x.solve_ms, x.sweep_ms, x.transport_ms, x.macro_ms, x.sweep_levels, x.macro_cg_iters,
-                       x.source_ms, x.face_ms) for x in st]
+                       x.source_ms, x.face_ms, x.grad_ms) for x in st]
 
 
 def run_deck(deck_dir: str, max_iter: int | None = None, tol_temp: float | None = None,

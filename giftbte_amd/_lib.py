@@ -85,7 +85,8 @@ class Stats(C.Structure):
                 ("converged", C.c_int), ("err_temp", C.c_double), ("err_flux", C.c_double),
                 ("solve_ms", C.c_double), ("sweep_ms", C.c_double),
                 ("transport_ms", C.c_double), ("macro_ms", C.c_double), ("sweep_levels", C.c_int),
-                ("macro_cg_iters", C.c_int), ("source_ms", C.c_double), ("face_ms", C.c_double)]
+                ("macro_cg_iters", C.c_int), ("source_ms", C.c_double), ("face_ms", C.c_double),
+                ("grad_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -3273,6 +3273,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         st->macro_cg_iters = (int)cg_total;
         st->source_ms = source_ms;
         st->face_ms = face_ms;
+        st->grad_ms = 0;  // formed inside the face kernel
     }
     return status;
 }
@@ -3316,7 +3317,7 @@ int Solver::solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st) {
     int nt = 0, status = GBTE_OK, macro_total = 0;
     bool conv = false;
     double errT = 0, errQ = 0;
-    double source_ms = 0, sweep_ms = 0, face_ms = 0, transport_ms = 0, macro_ms = 0;
+    double source_ms = 0, sweep_ms = 0, face_ms = 0, grad_ms = 0, transport_ms = 0, macro_ms = 0;
     long cg_total = 0;
     for (nt = 0; nt < max_iter; ++nt) {
         hipLaunchKernelGGL(k_copy_old, dim3(red_blocks), dim3(kBlock), 0, s_, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_Told.p,
@@ -3329,6 +3330,7 @@ int Solver::solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         launch_bound_out3(m);
         (void)hipEventRecord(ev_[3], s_);
         if (!first_order) launch_grad3();
+        (void)hipEventRecord(ev_[7], s_);
         launch_moments3(m);
         (void)hipEventRecord(ev_[4], s_);
         launch_boundary();
@@ -3378,6 +3380,8 @@ int Solver::solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             sweep_ms += ms;
             (void)hipEventElapsedTime(&ms, ev_[3], ev_[4]);
             face_ms += ms;
+            (void)hipEventElapsedTime(&ms, ev_[3], ev_[7]);
+            grad_ms += ms;
             (void)hipEventElapsedTime(&ms, ev_[0], ev_[4]);
             transport_ms += ms;
             (void)hipEventElapsedTime(&ms, ev_[5], ev_[6]);
@@ -3437,6 +3441,7 @@ int Solver::solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st) {
         st->macro_cg_iters = (int)cg_total;
         st->source_ms = source_ms;
         st->face_ms = face_ms;
+        st->grad_ms = grad_ms;
     }
     return status;
 }

@@ -152,7 +152,7 @@ class Solver {
     std::string err_;
     hipStream_t s_ = nullptr;
     // before/after: source | sweep | bound_out | face kernels | boundary | macro loop
-    hipEvent_t ev_[7] = {};
+    hipEvent_t ev_[8] = {};
     // side stream: the next iteration's interior sources run there, after the
     // face kernel, beside the boundary update and the synthetic passes
     hipStream_t s2_ = nullptr;

@@ -144,6 +144,8 @@ typedef struct gbte_stats {
     int macro_cg_iters;   /* PCG iterations, summed over synthetic solves (PCG only) */
     double source_ms;     /* device time of the source kernel, summed                */
     double face_ms;       /* device time of the vertex/face/moment kernel(s), summed */
+    double grad_ms;       /* of which the LSQ gradient kernel on tetrahedral meshes (0 in 2D,
+                             where the face kernel forms the gradient) */
 } gbte_stats;
 
 /* ---- problem construction (replaces BTEMain.cpp:87-362: decks, BTEBand,
