@@ -8,6 +8,12 @@ namespace {
 thread_local std::string g_problem_error;
 }
 
+namespace gbte {
+// The calling thread's last message of a call without a handle (problem
+// construction, gbte_flux_keff*), read by gbte_problem_error.
+std::string &thread_error() { return g_problem_error; }
+}  // namespace gbte
+
 extern "C" {
 
 int gbte_problem_load(const char *deck_dir, gbte_problem **out) {

@@ -210,7 +210,17 @@ int gbte_keff(gbte_handle *h, int axis, double delta_T, double *keff);
  * where nothing was tuned).  Valid until gbte_destroy.  No reference
  * counterpart: the reference has one implementation of each loop. */
 const char *gbte_tuning_log(const gbte_handle *h);
+/* k_eff from a flux field already on the device: *keff_dev =
+ * sum_i q_dev[i] vol_dev[i] / (area delta_T) over n_cells cells (device
+ * pointers; keff_dev one double), ordered on `stream` (a hipStream_t; NULL:
+ * the null stream), in a fixed reduction order.  The partial sums live in a
+ * buffer allocated and freed inside the call, which therefore waits for the
+ * stream.  n_cells <= 16,777,216.  Errors: gbte_problem_error().  The same
+ * quantity as gbte_keff (A_perp and delta_T supplied by the caller). */
+int gbte_flux_keff(const double *q_dev, const double *vol_dev, int n_cells, double area, double delta_T,
+                   double *keff_dev, void *stream);
 const char *gbte_last_error(const gbte_handle *h);
+/* The calling thread's last error message of a call without a handle. */
 const char *gbte_problem_error(void);
 void gbte_destroy(gbte_handle *h);
 
