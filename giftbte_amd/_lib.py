@@ -126,7 +126,7 @@ def load(path: str | None = None) -> C.CDLL:
         lib.gbte_write_outputs.argtypes = [vp, vp, C.c_char_p]
         lib.gbte_solve_batch.argtypes = [C.POINTER(vp), C.c_int, _pi, _pd, _pd, C.POINTER(Stats), _pi]
     lib.gbte_write_fields.argtypes = [vp, _pd, _pd, _pd, _pd, _pd, _pd, C.c_char_p]
-    for f in ("gbte_flux_keff",):
+    for f in ("gbte_flux_keff", "gbte_flux_keff_noalloc"):
         if hasattr(lib, f):
             getattr(lib, f).argtypes = [vp, vp, C.c_int, C.c_double, C.c_double, vp, vp]
     if path is None:
@@ -138,7 +138,7 @@ EXPORTED = [
     "gbte_problem_load", "gbte_problem_tables", "gbte_problem_free", "gbte_create", "gbte_fourier_solve",
     "gbte_set_initial", "gbte_solve", "gbte_get_fields", "gbte_keff", "gbte_last_error", "gbte_problem_error",
     "gbte_destroy", "gbte_write_outputs", "gbte_write_fields", "gbte_solve_batch", "gbte_tuning_log",
-    "gbte_flux_keff",
+    "gbte_flux_keff", "gbte_flux_keff_noalloc",
 ]
 
 

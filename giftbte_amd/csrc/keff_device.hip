@@ -1,10 +1,16 @@
 // Effective conductivity reduced on the device from a caller's flux field:
 // k_eff = sum_i q_i V_i / (A_perp dT), the general form of the check in
 // bin/examples/cross-plane/verify_kappa_crossplane.m:8 (mean(q) L / dT on
-// full equal-area domains), in a fixed order (4096-cell blocks, then the
-// block sums), with the partial sums in a buffer allocated per call.
+// full equal-area domains).  Two entry points share the kernels:
+//   gbte_flux_keff          partial sums in a buffer it allocates per call
+//   gbte_flux_keff_noalloc  partial sums in a fixed static device array, for
+//                           callers that cannot allocate (graph capture,
+//                           serving loops); stream-ordered, no host sync
+// Both reduce in a fixed order (4096-cell blocks, then the block sums), so
+// they give bit-identical results.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "giftbte.h"
@@ -18,8 +24,13 @@ namespace {
 constexpr int kKeffBlock = 256;
 constexpr int kKeffPerThread = 16;
 constexpr int kKeffCells = kKeffBlock * kKeffPerThread;  // cells per first-stage block
-constexpr int kKeffMaxBlocks = 4096;                      // first-stage blocks at most
+constexpr int kKeffMaxBlocks = 4096;                      // partial sums the static array holds
 constexpr long kKeffMaxCells = (long)kKeffCells * kKeffMaxBlocks;
+constexpr int kMaxDevices = 64;
+
+// Partial sums of gbte_flux_keff_noalloc, one array per GPU (32 KiB), kept
+// until the process ends.  Calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double scratch[kKeffMaxBlocks];
 
 __device__ double block_sum(double v, double *red) {
     red[threadIdx.x] = v;
@@ -82,6 +93,28 @@ int launch(const double *q, const double *vol, int n, double area, double dT, do
     return GBTE_OK;
 }
 
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (the host-side name is not a device address).
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+int scratch_of_current(double **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(GBTE_HIP_ERROR, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GBTE_HIP_ERROR, "gbte_flux_keff_noalloc: device index too large");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GBTE_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return GBTE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -102,6 +135,20 @@ int gbte_flux_keff(const double *q_dev, const double *vol_dev, int n_cells, doub
     if (e != hipSuccess) return fail(GBTE_HIP_ERROR, std::string("gbte_flux_keff: ") + hipGetErrorString(e));
     gbte::thread_error().clear();
     return GBTE_OK;
+}
+
+int gbte_flux_keff_noalloc(const double *q_dev, const double *vol_dev, int n_cells, double area, double delta_T,
+                           double *keff_dev, void *stream) {
+    int rc = check_args(q_dev, vol_dev, n_cells, area, delta_T, keff_dev);
+    if (rc != GBTE_OK) return rc;
+    double *part = nullptr;
+    rc = scratch_of_current(&part);
+    if (rc != GBTE_OK) return rc;
+    // every partial that k_flux_final reads is written first by k_flux_partial
+    // in the same stream, so nothing is cleared
+    rc = launch(q_dev, vol_dev, n_cells, area, delta_T, keff_dev, part, static_cast<hipStream_t>(stream));
+    if (rc == GBTE_OK) gbte::thread_error().clear();
+    return rc;
 }
 
 }  // extern "C"

@@ -219,6 +219,16 @@ const char *gbte_tuning_log(const gbte_handle *h);
  * quantity as gbte_keff (A_perp and delta_T supplied by the caller). */
 int gbte_flux_keff(const double *q_dev, const double *vol_dev, int n_cells, double area, double delta_T,
                    double *keff_dev, void *stream);
+/* gbte_flux_keff that allocates nothing: the partial sums go to a fixed
+ * 32 KiB device array of the library (one per GPU, held until the process
+ * ends), so the call only enqueues two kernels on `stream` and returns; it
+ * can be captured into a HIP graph.  Same limit (n_cells <= 16,777,216,
+ * larger is GBTE_BAD_INPUT) and bit-identical results.  Calls that overlap on
+ * one GPU share the array and race: order them on one stream.  No reference
+ * counterpart: for callers that cannot allocate device memory (graph
+ * capture, serving loops). */
+int gbte_flux_keff_noalloc(const double *q_dev, const double *vol_dev, int n_cells, double area, double delta_T,
+                           double *keff_dev, void *stream);
 const char *gbte_last_error(const gbte_handle *h);
 /* The calling thread's last error message of a call without a handle. */
 const char *gbte_problem_error(void);

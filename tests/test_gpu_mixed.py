@@ -25,9 +25,12 @@ KEFF_TOL = 1e-6
 
 
 def _keff_ref(p, ref, dT):
+    """sum q_x V / (A_x dT), A_x the node extent normal to x (y, and z in 3D)."""
     vol = p.array("volume")
-    ys = p.array("node_y")
-    return float((ref["qx"] * vol).sum() / ((ys.max() - ys.min()) * dT))
+    area = np.ptp(p.array("node_y"))
+    if p.tables.dim == 3:
+        area *= np.ptp(p.array("node_z"))
+    return float((ref["qx"] * vol).sum() / (area * dT))
 
 
 @pytest.mark.parametrize("name", list(decklib.DECKS))
