@@ -151,6 +151,17 @@ class BTESolver:
         self._check(self.lib.gbte_keff(self._h, axis, delta_t, C.byref(v)))
         return v.value
 
+    def adjoint(self, axis: int = 0, delta_t: float = 2.0, max_iter: int = 100000, tol: float = 1e-13):
+        """Sensitivities of keff(axis, delta_t) at the converged state of the
+        last solve (gbte_adjoint): (dk/ds per band for tau_b -> s_b tau_b,
+        dk/dT per boundary face (isothermal faces; 0 elsewhere), stats)."""
+        ds = np.zeros(self.nb)
+        dtb = np.zeros(self.problem.tables.n_bounds)
+        st = _lib.AdjointStats()
+        self._check(self.lib.gbte_adjoint(self._h, int(axis), float(delta_t), int(max_iter), float(tol),
+                                          _lib.dptr(ds), _lib.dptr(dtb), C.byref(st)))
+        return ds, dtb, st.as_dict()
+
     def tuning_log(self) -> str:
         """The kernel variants timed at set-up, their milliseconds per launch
         and the choices made (empty where nothing was tuned)."""

@@ -92,6 +92,13 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("change", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -119,6 +126,8 @@ def load(path: str | None = None) -> C.CDLL:
         lib.gbte_keff.argtypes = [vp, C.c_int, C.c_double, _pd]
         lib.gbte_last_error.argtypes = [vp]
         lib.gbte_last_error.restype = C.c_char_p
+        lib.gbte_adjoint.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, _pd, _pd,
+                                      C.POINTER(AdjointStats)]
         lib.gbte_tuning_log.argtypes = [vp]
         lib.gbte_tuning_log.restype = C.c_char_p
         lib.gbte_destroy.argtypes = [vp]
@@ -138,7 +147,7 @@ EXPORTED = [
     "gbte_problem_load", "gbte_problem_tables", "gbte_problem_free", "gbte_create", "gbte_fourier_solve",
     "gbte_set_initial", "gbte_solve", "gbte_get_fields", "gbte_keff", "gbte_last_error", "gbte_problem_error",
     "gbte_destroy", "gbte_write_outputs", "gbte_write_fields", "gbte_solve_batch", "gbte_tuning_log",
-    "gbte_flux_keff", "gbte_flux_keff_noalloc",
+    "gbte_flux_keff", "gbte_flux_keff_noalloc", "gbte_adjoint",
 ]
 
 

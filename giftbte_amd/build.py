@@ -47,7 +47,8 @@ def build_lib() -> str:
     os.makedirs(objdir, exist_ok=True)
     objs = []
     srcs = ["problem.cpp", "bands.cpp", "outputs.cpp", "reorder.cpp", "ndchol.cpp", "tiles.cpp", "capi_problem.cpp", "capi_solver.cpp",
-            "solver.hip", "solver3d.hip", "keff_device.hip"]
+            "solver.hip", "solver3d.hip", "keff_device.hip",
+            "adjoint.hip"]
     for s in srcs:
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, s + ".o")

@@ -164,6 +164,16 @@ int gbte_keff(gbte_handle *h, int axis, double delta_T, double *keff) {
     return finish(h, h->s->keff(axis, delta_T, keff));
 }
 
+int gbte_adjoint(gbte_handle *h, int axis, double delta_T, int max_iter, double tol, double *dk_dtau_scale,
+                 double *dk_dbound_temp, gbte_adjoint_stats *stats) {
+    if (!h || !h->s || !h->s->ok()) return GBTE_HIP_ERROR;
+    if (!dk_dtau_scale) {
+        h->err = "gbte_adjoint: dk_dtau_scale is required";
+        return GBTE_BAD_INPUT;
+    }
+    return finish(h, h->s->adjoint(axis, delta_T, max_iter, tol, dk_dtau_scale, dk_dbound_temp, stats));
+}
+
 const char *gbte_last_error(const gbte_handle *h) { return h ? h->err.c_str() : "null handle"; }
 const char *gbte_tuning_log(const gbte_handle *h) { return h && h->s ? h->s->tuning_log().c_str() : ""; }
 

@@ -2631,17 +2631,13 @@ bool Solver::build() {
                        (double)hsum / n_tiles, (double)rsum / n_tiles, rmax);
             }
         }
-        std::vector<int>().swap(h_gptr);
-        std::vector<int>().swap(h_gidx);
-        std::vector<int>().swap(h_vptr);
-        std::vector<int>().swap(h_vsrc);
-        std::vector<double>().swap(h_vwt);
+        // h_gptr / h_gidx / h_gP / h_gok and h_vptr / h_vsrc / h_vwt stay:
+        // the adjoint transposes the same LSQ and inverse-distance stencils
         std::vector<int>().swap(h_fkind);
         std::vector<int>().swap(h_fother);
         std::vector<int>().swap(h_fv0);
         std::vector<int>().swap(h_fv1);
-        std::vector<int>().swap(h_gok);
-        for (auto *v : {&h_gP, &h_fga, &h_fn0x, &h_fn0y, &h_fmxm, &h_fmym}) std::vector<double>().swap(*v);
+        for (auto *v : {&h_fga, &h_fn0x, &h_fn0y, &h_fmxm, &h_fmym}) std::vector<double>().swap(*v);
     }
     // ---- state
     size_t nE = (size_t)nD * nc * nB;
@@ -3191,6 +3187,7 @@ int Solver::solve(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             (void)hipEventRecord(ev_[6], s_);
         }
         macro_total += passes;
+        last_passes_ = passes;
         auto add_times = [&]() {  // after a host sync on s_
             float ms = 0;
             if (side_src) {
@@ -3372,6 +3369,7 @@ int Solver::solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st) {
             (void)hipEventRecord(ev_[6], s_);
         }
         macro_total += passes;
+        last_passes_ = passes;
         auto add_times = [&]() {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, ev_[0], ev_[1]);

@@ -98,6 +98,9 @@ class Solver {
     int n_cells() const { return nc; }
     int n_bands() const { return nB; }
     int keff(int axis, double dT, double *out);
+    // discrete adjoint of the converged state (adjoint.hip): dk_eff / d(tau
+    // scale of each band) and dk_eff / d(temperature of each boundary face)
+    int adjoint(int axis, double dT, int max_iter, double tol, double *dk_ds, double *dk_dtb, gbte_adjoint_stats *st);
     // record the synthetic-pass graphs now (the first solve does it
     // otherwise); batched solves do this before their threads start, as a
     // capture must not overlap another thread's allocations and copies
@@ -246,6 +249,7 @@ class Solver {
     double errInner_ = 0, errTold_ = 0, errQold_ = 0;
     int errInc_ = 0;
     int fourier_passes_ = 0;
+    int last_passes_ = 0;  // synthetic passes of the last outer iteration
 };
 
 template <typename T>

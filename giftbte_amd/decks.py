@@ -20,6 +20,8 @@ import math
 import os
 from dataclasses import dataclass, field
 
+import dataclasses
+
 import numpy as np
 
 # edge entity ids used by the generators (BC index = entity + 1)
@@ -300,6 +302,21 @@ def debye_bands(nb: int, seed: int, omega_max=1e14, vmax=6400.0, mfp_min=1e-9, m
     lr = (C / tau) / (C / tau).sum()
     hr = C / C.sum()
     return Bands(v, tau, C, lr, hr)
+
+
+def scale_tau(bands: Bands, s) -> Bands:
+    """Relaxation times tau_b -> s_b tau_b with the lattice ratios following,
+    lr = (C/tau) / sum(C/tau) (the relation debye_bands uses); group
+    velocities, capacities and heat ratios unchanged."""
+    tau = bands.tau * np.asarray(s, dtype=np.float64)
+    lr = (bands.C / tau) / (bands.C / tau).sum()
+    return Bands(bands.v.copy(), tau, bands.C.copy(), lr, bands.hr.copy())
+
+
+def set_wall_temperature(spec: DeckSpec, index: int, value: float) -> DeckSpec:
+    """The spec with boundary condition `index` (type 1) at temperature value."""
+    bcs = [(i, t, value if i == index else v) for (i, t, v) in spec.bcs]
+    return dataclasses.replace(spec, bcs=bcs)
 
 
 # ------------------------------------------------------------------ writers

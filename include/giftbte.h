@@ -148,6 +148,14 @@ typedef struct gbte_stats {
                              where the face kernel forms the gradient) */
 } gbte_stats;
 
+/* Statistics of gbte_adjoint. */
+typedef struct gbte_adjoint_stats {
+    int iterations;   /* adjoint iterations run                                   */
+    int converged;    /* 1 if the relative change of the T adjoint fell to tol    */
+    double change;    /* last relative change                                     */
+    double solve_ms;  /* wall time of the call                                    */
+} gbte_adjoint_stats;
+
 /* ---- problem construction (replaces BTEMain.cpp:87-362: decks, BTEBand,
  *      BTEBoundaryCondition, DistributeMesh, BTEAngle) ---------------------- */
 
@@ -229,6 +237,23 @@ int gbte_flux_keff(const double *q_dev, const double *vol_dev, int n_cells, doub
  * capture, serving loops). */
 int gbte_flux_keff_noalloc(const double *q_dev, const double *vol_dev, int n_cells, double area, double delta_T,
                            double *keff_dev, void *stream);
+/* Sensitivities of k_eff (the gbte_keff quantity along `axis` with the
+ * nominal delta_T) at the converged state of the last gbte_solve, by the
+ * discrete adjoint of the outer iteration (StaticBTESyntheticSolver.cpp:
+ * 63-221 with its synthetic passes, or solve_DOM for Sequential): a
+ * fixed-point iteration of the transposed map, its transport part a sweep of
+ * the transposed upwind matrix in reversed upwind order on the GPU, to a
+ * relative change of tol or max_iter iterations.
+ *   dk_dtau_scale[n_bands]: d k_eff / d s_b for tau_b -> s_b tau_b, the lattice
+ *     ratios following as lr = (C/tau) / sum (C/tau) and kappa_bulk = sum
+ *     C v^2 tau / MaterialDimension, at s = 1;
+ *   dk_dbound_temp[n_bounds] (may be NULL): d k_eff / d T of each isothermal
+ *     (type 1) boundary face; 0 for the others.
+ * Covers 2D meshes of one material in FP64 storage, Order 2 or 1, both
+ * schemes, boundary types 1, 2, 3 and periodic; else GBTE_UNSUPPORTED.
+ * No reference counterpart. */
+int gbte_adjoint(gbte_handle *h, int axis, double delta_T, int max_iter, double tol, double *dk_dtau_scale,
+                 double *dk_dbound_temp, gbte_adjoint_stats *stats);
 const char *gbte_last_error(const gbte_handle *h);
 /* The calling thread's last error message of a call without a handle. */
 const char *gbte_problem_error(void);
