@@ -31,7 +31,7 @@ BTEMain.cpp:218-225 exits), so on 3D the oracle's single-core transport loop
 over a bounded number of the systems is the baseline.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3d|2d]
-                       [--no-converge] [--no-cpu] [--batch N]
+                       [--no-converge] [--no-cpu] [--batch N] [--adjoint]
 """
 import argparse
 import dataclasses
@@ -226,6 +226,71 @@ def run_batch(a):
         print(json.dumps(out))
 
 
+def run_adjoint(a):
+    """--adjoint: the 2D porous film (as --workload 2d) solved in FP64 to
+    margins of 1e-15 / 1e-13 from the Fourier warm start, then gbte_adjoint
+    for dk_eff/d(tau scale) of every band and dk_eff/dT_hot.  Checked on the
+    host against central differences (step 1e-4) of the GPU solve for the
+    --adjoint-fd bands with the largest sensitivities, and against the closed
+    form dk/dT_hot = k / (T_hot - T_cold).  Prints one JSON line."""
+    import numpy as np
+    from giftbte_amd import BTESolver, Problem, decks
+    spec0 = decks.porous_spec(n_target=a.cells, nbands=a.bands, nphi=a.nphi, pore_radius=0.3, seed=a.seed,
+                              scale=5e-7)
+    with tempfile.TemporaryDirectory() as td:
+        def solve(spec, name):
+            p = Problem(decks.write_deck(os.path.join(td, name), spec))
+            t0 = time.time()
+            s = BTESolver(p)
+            t1 = time.time()
+            s.fourier()
+            st = s.solve(100000, 1e-15, 1e-13)
+            t2 = time.time()
+            return p, s, st, t1 - t0, t2 - t1
+        p, s, st, t_create, t_solve = solve(spec0, "base")
+        t = p.tables
+        k0 = s.keff(0, 2.0)
+        print(f"[bench] adjoint: {t.n_cells} cells x {t.n_bands} bands x {t.n_dirs} dirs, forward {st.iterations} it "
+              f"{t_solve:.2f} s (create {t_create:.2f} s), k {k0:.12g}", file=sys.stderr, flush=True)
+        t0 = time.time()
+        g, gtb, ast = s.adjoint(0, 2.0, 100000, 1e-13)
+        t_adj = time.time() - t0
+        bt, btemp = p.array("bound_type"), p.array("bound_temp")
+        g_hot = float(gtb[(bt == 1) & (btemp == 301.0)].sum())
+        s.close()
+        print(f"[bench] adjoint: {ast['iterations']} it {t_adj:.2f} s, dk/dT_hot {g_hot:.10g} (k/2 {k0 / 2:.10g})",
+              file=sys.stderr, flush=True)
+        fd_bands = [int(b) for b in np.argsort(-np.abs(g))[:a.adjoint_fd]]
+        fd, t_fd = {}, time.time()
+        for b in fd_bands:
+            ks = []
+            for sgn in (1, -1):
+                sc = np.ones(t.n_bands)
+                sc[b] = 1 + sgn * 1e-4
+                _, s2, _, _, _ = solve(dataclasses.replace(spec0, bands=decks.scale_tau(spec0.bands, sc)), f"b{b}{sgn}")
+                ks.append(s2.keff(0, 2.0))
+                s2.close()
+            fd[b] = (ks[0] - ks[1]) / 2e-4
+            print(f"[bench] band {b}: adjoint {g[b]:.10g} fd {fd[b]:.10g}", file=sys.stderr, flush=True)
+        ga, gf = np.array([g[b] for b in fd_bands]), np.array([fd[b] for b in fd_bands])
+        U = t.n_cells * t.n_bands * t.n_dirs
+        out = {"metric": "adjoint dk_eff/d(tau scale) and dk_eff/dT_hot, time and agreement with finite differences",
+               "value": t_adj, "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+               "data": "synthetic (seeded porous Delaunay deck, seeded Debye-like bands)",
+               "config": {"workload": "2D porous film 500 nm, r=150 nm diffuse pore, seeded Delaunay, FP64 forward "
+                                      "and adjoint", "cells": t.n_cells, "bands": t.n_bands, "directions": t.n_dirs,
+                          "updates_per_iteration": U, "parallelism": "single GPU"},
+               "forward": {"iterations": st.iterations, "solve_s": t_solve, "create_s": t_create, "keff": k0},
+               "adjoint": {"iterations": ast["iterations"], "converged": bool(ast["converged"]), "time_s": t_adj,
+                           "ms_per_iteration": t_adj * 1e3 / max(1, ast["iterations"]),
+                           "dk_dtau_scale": g.tolist(), "dk_dT_hot": g_hot},
+               "checks": {"fd_bands": fd_bands, "fd": [fd[b] for b in fd_bands],
+                          "fd_rel_l2": float(np.linalg.norm(ga - gf) / np.linalg.norm(gf)),
+                          "dT_hot_rel_err_vs_k_over_dT": abs(g_hot - k0 / 2.0) / abs(k0 / 2.0),
+                          "fd_time_s": time.time() - t_fd}}
+        print(json.dumps(out))
+
+
 def run_precision(p, a, prec, cfg, U, T0_out):
     """Set-up, warm-up, K timed steps and (unless --no-converge) a solve to
     the deck tolerances from the Fourier warm start, on one handle."""
@@ -303,6 +368,9 @@ def main():
     ap.add_argument("--precisions", default="fp64,fp32", help="storage precisions to run (default both)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="instead of the default line: N independent porous instances solved side by side")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="instead of the default line: the adjoint sensitivities on the 2D porous workload")
+    ap.add_argument("--adjoint-fd", type=int, default=2, help="bands checked by finite differences (--adjoint)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -314,6 +382,9 @@ def main():
 
     if a.batch > 0:
         run_batch(a)
+        return
+    if a.adjoint:
+        run_adjoint(a)
         return
     if a.copy_probe > 0:
         import torch
