@@ -154,6 +154,8 @@ def kernel_name(phase, dim, prec, tuning):
     if dim == 2:
         return {"source": "k_source", "sweep": "k_sweep_pc", "face": "k_tile_face"}[phase]
     if phase == "sweep":
+        if "sweep chosen: chunked" in tuning:
+            return "k_sweep_lv"
         return "k_sweep_dir" if "sweep chosen: level-synchronous" in tuning else "k_sweep_pc"
     if phase == "grad":
         return "k_grad3" if "k_grad3 chosen: LDS tiles" in tuning else "k_grad3g"

@@ -2133,6 +2133,29 @@ bool Solver::build() {
                 rec_G[P0] = G;
             }
         }
+        if (dim == 3 && !opt.fallback_kernels) {
+            // records of the level-synchronous chunked sweep: every upwind
+            // cell lies on an earlier level of its direction (checked)
+            std::vector<LvRec> lrec(NI);
+            for (int d = 0; d < nD; ++d) {
+                const int *lp = levptr.data() + dlev[d];
+                int L = dlev[d + 1] - dlev[d] - 1;
+                for (int l = 0; l < L; ++l)
+                    for (int k = lp[l]; k < lp[l + 1]; ++k) {
+                        size_t P0 = (size_t)d * nc + k;
+                        LvRec &r = lrec[P0];
+                        r.cell = item_cell[P0];
+                        r.G = rec_G[P0];
+                        for (int j = 0; j < kMaxIn; ++j) {
+                            int qq = rec_q[P0 * kMaxIn + j];
+                            r.c[j] = qq >= 0 ? rec_c[P0 * kMaxIn + j] : -1;
+                            r.g[j] = qq >= 0 ? rec_g[P0 * kMaxIn + j] : 0.0;
+                            if (qq >= lp[l]) { err_ = "internal error: upwind cell on the same or a later level"; return false; }
+                        }
+                    }
+            }
+            if (!upload(d_lv_rec, lrec)) return false;
+        }
         // about one workgroup per CU: split the bands of a direction into
         // chunks; the ring takes 64 KiB of LDS
         sweep_nchunk = std::max(1, std::min(nB, 256 / std::max(1, nD)));
@@ -2643,9 +2666,9 @@ bool Solver::build() {
     size_t nE = (size_t)nD * nc * nB;
     // FP32 sweeps: e, its gradient, the boundary values and S in float
     // (buffers sized in doubles)
-    if (fp32_ && (opt.fallback_kernels || !use_pc || (dim == 2 && !use_tiles))) {
-        err_ = "FP32 sweeps need the tiled face kernel and the wave-specialised sweep (unsupported for this mesh or "
-               "with fallback kernels)";
+    if (fp32_ && (opt.fallback_kernels || (!use_pc && !(dim == 3 && d_lv_rec.p)) || (dim == 2 && !use_tiles))) {
+        err_ = "FP32 sweeps need the tiled face kernel and the wave-specialised sweep (2D) or a sweep kernel that "
+               "takes float storage (unsupported for this mesh or with fallback kernels)";
         return false;
     }
     auto half = [&](size_t n) { return fp32_ ? (n + 1) / 2 : n; };
@@ -2897,7 +2920,9 @@ Modes Solver::modes() const {
 // One whole-source sweep of every direction (tetrahedral meshes): the
 // wave-specialised kernel or the level-synchronous one.
 void Solver::launch_sweep3(const Modes &m) {
-    if (use_pc) {
+    if (lv_bc > 0) {
+        launch_sweep_lv(m, lv_bc);
+    } else if (use_pc) {
         PcSched ps{d_pc_rec.p, d_pc_size.p, d_pc_f0.p, d_pc_dsl.p, d_pc_far.p};
         launch_sweep_pc(ps, m, nullptr);
     } else {

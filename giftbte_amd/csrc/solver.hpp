@@ -52,6 +52,14 @@ struct PcRec {
     double g[3], G;
 };
 
+// Upwind record of one position of the level-synchronous chunked sweep
+// (k_sweep_lv, tetrahedra), 48 bytes: the cell, its interior upwind cells
+// (-1: none) with their in-face (A/V)(s.n), and G the sum over out-faces.
+struct LvRec {
+    int cell, c[3];
+    double g[3], G;
+};
+
 // The wave-specialised sweep's schedule, as device pointers.
 struct PcSched {
     const PcRec *rec;            // [dir][position]
@@ -134,6 +142,7 @@ class Solver {
     bool tune3d();
     void launch_sweep_pc(const PcSched &ps, const Modes &m, const double *tpos);
     void launch_sweep3(const Modes &m);
+    void launch_sweep_lv(const Modes &m, int bc);
     Modes modes() const;
     // macro / Fourier linear solve of (-K) x = -rhs: the direct factor solve,
     // or PCG warm-started from x
@@ -188,6 +197,12 @@ class Solver {
     DBuf<PcRec> d_pc_rec;
     DBuf<int> d_pc_f0, d_pc_dsl, d_pc_far;
     DBuf<unsigned short> d_pc_size;
+    // level-synchronous chunked sweep (k_sweep_lv, tetrahedra): records in
+    // upwind order; lv_bc bands per workgroup when chosen (0: not used).
+    // The sweep source S is then stored [dir][band chunk][position][lv_bc]
+    // (s_sh = log2 lv_bc), else [dir][position][band] (s_sh = -1)
+    DBuf<LvRec> d_lv_rec;
+    int lv_bc = 0, lv_np = 2, s_sh = -1;
     // gradient stencil
     DBuf<int> d_gptr, d_gidx, d_gok;
     DBuf<double> d_gP;

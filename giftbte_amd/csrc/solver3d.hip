@@ -247,7 +247,9 @@ __global__ void __launch_bounds__(256) k_grad3g(const ST *__restrict__ e, ST *__
 // StaticBTESynthetic.cpp:1427-1500, less the new upwind values):
 //   S = C (T_old - Tref) + Q hr tau - sum_out a (r . g_c) - sum_in,interior a (r' . g_nb)
 //       - sum_in,boundary a eb,      a = tau (A/V) (v . n),
-// written to each direction's upwind order for the sweep.  One wave per cell
+// written to each direction's upwind order for the sweep: S[dir][position][band]
+// (sh < 0), or S[dir][band chunk][position][band in chunk] with chunks of
+// 2^sh bands (the layout of k_sweep_lv).  One wave per cell
 // row; the cell's face record (normal, A/V, r from its centre, r' from the
 // neighbour's centre) sits in scalar registers, and every face issues its
 // loads (the neighbour's three gradient components or the boundary value)
@@ -259,7 +261,7 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3(const int *__restric
                                                             const double *__restrict__ Q,
                                                             const double *__restrict__ rec,
                                                             const int *__restrict__ reci, Modes m, int nc, int nB,
-                                                            int nD) {
+                                                            int nD, int sh) {
     extern __shared__ double lds[];
     double *mvg = lds, *mtau = mvg + nB, *mcap = mtau + nB, *mhr = mcap + nB, *msx = mhr + nB, *msy = msx + nD,
            *msz = msy + nD;
@@ -322,7 +324,127 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3(const int *__restric
             double ui = fint[f] ? fqx[f] * ux[f] + fqy[f] * uy[f] + fqz[f] * uz[f] : ux[f];
             s -= a * (dot >= 0 ? uo : ui);
         }
-        __builtin_nontemporal_store((ST)s, S + ((size_t)d * nc + pc[d]) * nB + b);
+        size_t si = sh < 0 ? ((size_t)d * nc + pc[d]) * nB + b
+                           : ((((size_t)d * (nB >> sh) + (b >> sh)) * nc + pc[d]) << sh) + (b & ((1 << sh) - 1));
+        __builtin_nontemporal_store((ST)s, S + si);
+    }
+}
+
+// 16-byte vector moves of N values of type T (N * sizeof(T) a multiple of 16,
+// the address 16-byte aligned)
+template <typename T, int N>
+__device__ __forceinline__ void vld(T (&v)[N], const T *p) {
+    constexpr int NV = N * (int)sizeof(T) / 16;
+    static_assert(NV * 16 == N * (int)sizeof(T), "whole 16-byte vectors");
+    int4 t[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) t[k] = reinterpret_cast<const int4 *>(p)[k];
+    __builtin_memcpy(v, t, sizeof(v));
+}
+template <typename T, int N>
+__device__ __forceinline__ void vst(T *p, const T (&v)[N]) {
+    constexpr int NV = N * (int)sizeof(T) / 16;
+    int4 t[NV];
+    __builtin_memcpy(t, v, sizeof(v));
+#pragma unroll
+    for (int k = 0; k < NV; ++k) reinterpret_cast<int4 *>(p)[k] = t[k];
+}
+
+// The per-system SparseLU solve (StaticBTESyntheticSolver.cpp:96-100 on
+// _get_coefficient :784-840) as a level-synchronous upwind sweep for meshes
+// whose levels are wide (tetrahedra: about 500 levels of about 600 cells per
+// direction):
+//   e_new = (S - sum_in,interior tau v g_f e_new_nb) / (1 + tau v G_out).
+// One 256-thread workgroup per (direction, chunk of BC bands), so that every
+// CU holds several: a thread takes whole positions of a level with all BC
+// bands of each, so a position's record is read once and its source, upwind
+// values and new values move as 16-byte vectors.  The upwind values are read
+// from e, where this workgroup wrote them on earlier levels; levels are
+// separated by workgroup barriers.
+//
+// Software pipeline: a level's first 256 NP positions have their records and
+// sources loaded during the previous level, issued after that level's upwind
+// loads, so that only the upwind loads (L2) and the arithmetic lie on the
+// chain of dependent levels (positions beyond the first 256 NP of a wide
+// level are loaded in place).  The level pointers of the direction sit in
+// LDS.  The chunks of one direction take consecutive workgroup ids, which
+// xcd_block places on one XCD, so they share its L2 for the records and for
+// the partial lines of the e rows they write.
+template <typename ST, int BC, int NP>
+__global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec, const int *__restrict__ levptr,
+                                                  const int *__restrict__ dlev, const ST *__restrict__ S,
+                                                  ST *__restrict__ e, Modes m, int nc, int nB, int nD) {
+    extern __shared__ int slp[];  // level pointers of the direction
+    int nchunk = nB / BC;
+    int id = xcd_block();
+    if (id >= nD * nchunk) return;  // whole workgroup
+    int d = id / nchunk, chunk = id - d * nchunk, b0 = chunk * BC;
+    double tv[BC];
+#pragma unroll
+    for (int i = 0; i < BC; ++i) tv[i] = m.tau[b0 + i] * m.vg[b0 + i];
+    int L = dlev[d + 1] - dlev[d] - 1;
+    for (int i = threadIdx.x; i <= L; i += 256) slp[i] = levptr[dlev[d] + i];
+    __syncthreads();
+    const LvRec *rd = rec + (size_t)d * nc;
+    const ST *Sd = S + ((size_t)d * nchunk + chunk) * nc * BC;
+    size_t DB = (size_t)nD * nB;
+    ST *ed = e + (size_t)d * nB + b0;
+    int tid = threadIdx.x;
+    auto fetch = [&](int l, LvRec(&rr)[NP], ST(&ss)[NP][BC]) {
+        int a0 = slp[l], a1 = slp[l + 1];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            int p = a0 + tid + i * 256;
+            if (p < a1) {
+                rr[i] = rd[p];
+                vld(ss[i], Sd + (size_t)p * BC);
+            }
+        }
+    };
+    auto upwind = [&](const LvRec &r, ST(&u)[3][BC]) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (r.c[j] >= 0) vld(u[j], ed + (size_t)r.c[j] * DB);
+    };
+    auto solve = [&](const LvRec &r, const ST(&s)[BC], const ST(&u)[3][BC]) {
+        ST en[BC];
+#pragma unroll
+        for (int b = 0; b < BC; ++b) {
+            double v = s[b];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (r.c[j] >= 0) v -= tv[b] * r.g[j] * (double)u[j][b];
+            en[b] = (ST)(v / (1.0 + tv[b] * r.G));
+        }
+        vst(ed + (size_t)r.cell * DB, en);
+    };
+    // level l from the buffers (rc, sc); the next level's first positions
+    // into (rn, sn)
+    auto level = [&](int l, LvRec(&rc)[NP], ST(&sc)[NP][BC], LvRec(&rn)[NP], ST(&sn)[NP][BC]) {
+        int a0 = slp[l], a1 = slp[l + 1];
+        ST u[NP][3][BC];
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+            if (a0 + tid + i * 256 < a1) upwind(rc[i], u[i]);
+        if (l + 1 < L) fetch(l + 1, rn, sn);
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+            if (a0 + tid + i * 256 < a1) solve(rc[i], sc[i], u[i]);
+        for (int p = a0 + 256 * NP + tid; p < a1; p += 256) {
+            LvRec r = rd[p];
+            ST s1[BC], u1[3][BC];
+            vld(s1, Sd + (size_t)p * BC);
+            upwind(r, u1);
+            solve(r, s1, u1);
+        }
+        __syncthreads();
+    };
+    LvRec rA[NP], rB[NP];
+    ST sA[NP][BC], sB[NP][BC];
+    if (L > 0) fetch(0, rA, sA);
+    for (int l = 0; l < L; l += 2) {
+        level(l, rA, sA, rB, sB);
+        if (l + 1 < L) level(l + 1, rB, sB, rA, sA);
     }
 }
 
@@ -927,10 +1049,36 @@ void Solver::launch_source3(const Modes &m) {
     if (fp32_)
         hipLaunchKernelGGL(k_source3<float>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
                            reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
-                           reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD);
+                           reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
     else
         hipLaunchKernelGGL(k_source3<double>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p, d_S.p, d_grad.p,
-                           d_eb.p, d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD);
+                           d_eb.p, d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
+}
+
+// The level-synchronous chunked sweep with bc bands per workgroup and lv_np
+// positions per thread in flight (the tuner's candidates).
+void Solver::launch_sweep_lv(const Modes &m, int bc) {
+    int grid = ((nD * (nB / bc) + kXcd - 1) / kXcd) * kXcd;
+    size_t lds = (size_t)(nlev + 1) * sizeof(int);
+#define GBTE_LV(ST_, BC_, NP_)                                                                                      \
+    hipLaunchKernelGGL((k_sweep_lv<ST_, BC_, NP_>), dim3(grid), dim3(256), lds, s_, d_lv_rec.p, d_levptr.p, d_dlev.p, \
+                       reinterpret_cast<const ST_ *>(d_S.p), reinterpret_cast<ST_ *>(d_e.p), m, nc, nB, nD)
+#define GBTE_LV_NP(ST_, BC_)                  \
+    do {                                      \
+        if (lv_np == 4) GBTE_LV(ST_, BC_, 4); \
+        else if (lv_np == 2) GBTE_LV(ST_, BC_, 2); \
+        else GBTE_LV(ST_, BC_, 1);            \
+    } while (0)
+    if (fp32_) {
+        if (bc == 4) GBTE_LV_NP(float, 4);
+        else GBTE_LV_NP(float, 8);
+    } else {
+        if (bc == 2) GBTE_LV_NP(double, 2);
+        else if (bc == 4) GBTE_LV_NP(double, 4);
+        else GBTE_LV_NP(double, 8);
+    }
+#undef GBTE_LV_NP
+#undef GBTE_LV
 }
 
 void Solver::launch_bound_out3(const Modes &m) {
@@ -992,14 +1140,33 @@ bool Solver::tune3d() {
     g3_mode = pick;
     snprintf(line, sizeof line, "k_grad3 chosen: %s\n", pick == 0 ? "LDS tiles" : "direct gather");
     tune_log_ += line;
-    // FP64 sweeps: the wave-specialised kernel against the level-synchronous
-    // one (FP32 storage runs only the former)
-    if (use_pc && !fp32_) {
+    // sweeps: the wave-specialised kernel (when its schedule was built), the
+    // one-workgroup-per-direction level-synchronous kernel (FP64) and the
+    // chunked level-synchronous kernel at each band chunk and positions per
+    // thread in flight that the storage type allows
+    {
         if (!check(hipMemsetAsync(d_S.p, 0, d_S.n * sizeof(double), s_), "hipMemsetAsync")) return false;
+        if (!check(hipMemsetAsync(d_e.p, 0, d_e.n * sizeof(double), s_), "hipMemsetAsync")) return false;
         Modes m = modes();
-        double t[2] = {0, 0};
-        for (int v = 0; v < 2; ++v) {
-            use_pc = v == 0;
+        struct Cand { int kind, bc, np; };  // kind 0: wave-specialised, 1: level-synchronous, 2: chunked
+        std::vector<Cand> cands;
+        bool pc_built = use_pc;
+        int want = opt.sweep_kernel;  // GBTE_SWEEP_AUTO: every family
+        if (pc_built && (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_WAVESPEC)) cands.push_back({0, 0, 0});
+        if (!fp32_ && (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_LEVEL)) cands.push_back({1, 0, 0});
+        int wsz = fp32_ ? 4 : 8;
+        for (int bc : {2, 4, 8})
+            if (d_lv_rec.p && nB % bc == 0 && (bc * wsz) % 16 == 0 && (nB * wsz) % 16 == 0 &&
+                (size_t)(nlev + 1) * sizeof(int) <= 48 * 1024 &&
+                (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_CHUNKED))
+                for (int np : {1, 2, 4})
+                    if (bc * wsz * np <= 128) cands.push_back({2, bc, np});
+        double best = 1e300;
+        Cand pick{-1, 0, 0};
+        for (const Cand &c : cands) {
+            use_pc = c.kind == 0;
+            lv_bc = c.kind == 2 ? c.bc : 0;
+            lv_np = c.kind == 2 ? c.np : 2;
             launch_sweep3(m);  // warm
             (void)hipEventRecord(a, s_);
             for (int r = 0; r < 3; ++r) launch_sweep3(m);
@@ -1007,13 +1174,27 @@ bool Solver::tune3d() {
             if (!check(hipEventSynchronize(b), "tune sync")) return false;
             float ms = 0;
             (void)hipEventElapsedTime(&ms, a, b);
-            t[v] = ms / 3.0;
-            snprintf(line, sizeof line, "sweep %s: %.3f ms per launch\n",
-                     v == 0 ? "wave-specialised" : "level-synchronous", t[v]);
+            double per = ms / 3.0;
+            if (c.kind == 2)
+                snprintf(line, sizeof line, "sweep chunked %d bands, %d positions per thread: %.3f ms per launch\n",
+                         c.bc, c.np, per);
+            else
+                snprintf(line, sizeof line, "sweep %s: %.3f ms per launch\n",
+                         c.kind == 0 ? "wave-specialised" : "level-synchronous", per);
             tune_log_ += line;
+            if (per < best) { best = per; pick = c; }
         }
-        use_pc = t[0] <= t[1];
-        snprintf(line, sizeof line, "sweep chosen: %s\n", use_pc ? "wave-specialised" : "level-synchronous");
+        if (pick.kind < 0) { err_ = "no sweep kernel takes this mesh at this storage precision"; return false; }
+        use_pc = pick.kind == 0;
+        lv_bc = pick.kind == 2 ? pick.bc : 0;
+        lv_np = pick.kind == 2 ? pick.np : 2;
+        s_sh = -1;
+        for (int k = 1; lv_bc > 0 && k <= 3; ++k)
+            if ((1 << k) == lv_bc) s_sh = k;
+        if (pick.kind == 2)
+            snprintf(line, sizeof line, "sweep chosen: chunked %d bands, %d positions per thread\n", pick.bc, pick.np);
+        else
+            snprintf(line, sizeof line, "sweep chosen: %s\n", use_pc ? "wave-specialised" : "level-synchronous");
         tune_log_ += line;
     }
     (void)hipEventDestroy(a);

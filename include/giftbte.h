@@ -128,7 +128,14 @@ typedef struct gbte_options {
     int fallback_kernels; /* 1: use the kernels meant for decks the fast ones do not
                              fit (level-synchronous sweep, untiled face kernels);
                              same results, for testing those paths */
+    int sweep_kernel;     /* tetrahedral meshes: GBTE_SWEEP_AUTO (default: the set-up
+                             tuner times every sweep kernel that takes the mesh and
+                             keeps the fastest) or one kernel family:
+                             GBTE_SWEEP_WAVESPEC, GBTE_SWEEP_LEVEL (FP64 only),
+                             GBTE_SWEEP_CHUNKED; same results to rounding */
 } gbte_options;
+
+enum { GBTE_SWEEP_AUTO = 0, GBTE_SWEEP_WAVESPEC = 1, GBTE_SWEEP_LEVEL = 2, GBTE_SWEEP_CHUNKED = 3 };
 
 typedef struct gbte_stats {
     int iterations;       /* outer iterations run                               */

@@ -24,15 +24,23 @@ def _keff_x(p, q, dT=2.0):
     return float((q * vol).sum() / ((ys.max() - ys.min()) * (zs.max() - zs.min()) * dT))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("name", DECKS3D)
-def test_3d_matches_oracle(name, precision, deck_root):
+# every sweep kernel family the set-up tuner chooses from, forced, on the
+# 8-band deck (the level-synchronous one-workgroup-per-direction kernel is
+# FP64 only)
+SWEEPS = [("cube3d_b8", "fp64", "wavespec"), ("cube3d_b8", "fp64", "level"), ("cube3d_b8", "fp64", "chunked"),
+          ("cube3d_b8", "fp32", "wavespec"), ("cube3d_b8", "fp32", "chunked")]
+
+
+@pytest.mark.parametrize("name,precision,sweep",
+                         [(n, pr, "auto") for n in DECKS3D for pr in ("fp64", "fp32")] + SWEEPS)
+def test_3d_matches_oracle(name, precision, sweep, deck_root):
     from giftbte_amd import Problem, BTESolver
     d = decklib.make(name, deck_root)
     s = decklib.settings(name)
-    print(f"\n[gpu 3d] deck={name} precision={precision} settings={s}", flush=True)
+    print(f"\n[gpu 3d] deck={name} precision={precision} sweep={sweep} settings={s}", flush=True)
     p = Problem(d)
-    sol = BTESolver(p, precision=precision)
+    sol = BTESolver(p, precision=precision, sweep_kernel=sweep)
+    print("  " + sol.tuning_log().replace("\n", "; "), flush=True)
     fp = sol.fourier()
     st = sol.solve(s["max_iter"], s["tol_temp"], s["tol_flux"])
     ref, info = _oracle.run(p.tables, s["max_iter"], s["tol_temp"], s["tol_flux"])
