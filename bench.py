@@ -139,13 +139,16 @@ def kernel_bytes_per_update(dim, word=8):
     """Algorithmic HBM bytes per cell*band*dir update of each transport kernel
     (word = 8 for FP64 storage, 4 for FP32), tables shared by all
     band x direction systems excluded:
-      source     reads the LSQ gradient (dim values), writes the sweep source
+      source     2D: reads the LSQ gradient (2 values), writes the sweep source;
+                 3D: reads one face projection of the gradient per face (4
+                 values: the cell's own for out-faces, the neighbour's for
+                 in-faces), writes the sweep source
       sweep      reads the source, writes e
-      2D face    k_tile_face reads e, writes the gradient (dim values)
-      3D grad    k_grad3 reads e, writes the gradient (3 values)
+      2D face    k_tile_face reads e, writes the gradient (2 values)
+      3D grad    k_grad3g reads e, writes the gradient's four face projections
       3D moments the moment kernels read e"""
     if dim == 3:
-        return {"source": 4 * word, "sweep": 2 * word, "grad": 4 * word, "moments": word}
+        return {"source": 5 * word, "sweep": 2 * word, "grad": 5 * word, "moments": word}
     return {"source": (dim + 1) * word, "sweep": 2 * word, "face": (1 + dim) * word}
 
 
@@ -158,7 +161,7 @@ def kernel_name(phase, dim, prec, tuning):
             return "k_sweep_lv"
         return "k_sweep_dir" if "sweep chosen: level-synchronous" in tuning else "k_sweep_pc"
     if phase == "grad":
-        return "k_grad3" if "k_grad3 chosen: LDS tiles" in tuning else "k_grad3g"
+        return "k_grad3g"
     return {"source": "k_source3", "moments": "k_mom3 + k_vert3 + k_fin3"}[phase]
 
 

@@ -2673,7 +2673,9 @@ bool Solver::build() {
     }
     auto half = [&](size_t n) { return fp32_ ? (n + 1) / 2 : n; };
     size_t nEB = (size_t)nD * ne * nB;
-    if (!alloc(d_e, half(nE)) || !alloc(d_grad, half(nE * dim)) || !alloc(d_eb, half(nEB + 2)) ||
+    // gradients: per component in 2D; on tetrahedra the projections on the
+    // four face offsets
+    if (!alloc(d_e, half(nE)) || !alloc(d_grad, half(nE * (dim == 3 ? 4 : dim))) || !alloc(d_eb, half(nEB + 2)) ||
         !alloc(d_eV, dim == 2 ? (size_t)nD * nn * nB : 0) || !alloc(d_T, nc) || !alloc(d_Told, nc) || !alloc(d_T1, nc) ||
         !alloc(d_TV, nn) || !alloc(d_E, nc) || !alloc(d_qx, nc) || !alloc(d_qy, nc) || !alloc(d_qz, nc) ||
         !alloc(d_qxo, nc) || !alloc(d_qyo, nc) || !alloc(d_qzo, nc) || !alloc(d_RM, nc) || !alloc(d_Tb, (size_t)nB * nc) ||

@@ -234,16 +234,11 @@ class Solver {
     // 3D: source records, LSQ gradient tiles, moments, residual and
     // right-hand-side coefficients (solver3d.hip)
     DBuf<double> d_s3d, d_Y, d_YV, d_f3d, d_r3d[2];
-    DBuf<int> d_s3i, d_g3_tile, d_g3_rptr, d_g3_rows, d_g3_sptr, d_g3_self, d_g3_ok, d_f3i, d_r3i[2];
-    DBuf<char> d_g3_ent, d_g3g_ent;
+    DBuf<int> d_s3i, d_f3i, d_r3i[2];
+    DBuf<char> d_g3g_ent;
     DBuf<int> d_g3g_ptr, d_g3g_idx;
-    int g3_mode = 0;        // 0: LDS tiles, 1: direct gather (chosen by tune3d)
+    int g3_gs = 4, g3_wpb = 4;  // k_grad3g: 64-entry slices per wave, waves per workgroup (chosen by tune3d)
     std::string tune_log_;  // the autotuner's candidates, timings and choices
-    // tile budgets (rows staged per slice, stencil-table entries, cells) and
-    // the largest tile built; the table sits after the rows in LDS
-    int g3_ntiles = 0, g3_rows = 0, g3_ents = 0, g3_row_cap = 0, g3_ent_cap = 0, g3_cell_cap = 64,
-        g3_group = 10;  // 64-entry slices per workgroup
-    size_t g3_lds = 0;
     DBuf<double> d_ndy, d_ndu, d_ndx;
     // state
     DBuf<double> d_e, d_grad, d_eb, d_eV, d_T, d_Told, d_T1, d_TV, d_E, d_qx, d_qy, d_qz, d_qxo, d_qyo, d_qzo,

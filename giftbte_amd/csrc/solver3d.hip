@@ -4,9 +4,11 @@
 // (src/StaticBTESynthetic/StaticBTESyntheticSolver.cpp:63-221) on tetrahedra:
 //   k_source3     the part of _get_Re that does not depend on the new upwind
 //                 values, for all systems              StaticBTESynthetic.cpp:1427-1500
-//   k_sweep_pc    the per-system SparseLU solve as an upwind sweep (solver.hip)
+//   k_sweep_lv    the per-system SparseLU solve as a level-synchronous
+//                 upwind sweep (or k_sweep_pc / k_sweep_dir of solver.hip;
+//                 the set-up tuner picks the fastest)  StaticBTESyntheticSolver.cpp:96-100
 //   k_bound_out3  _get_bound_ee                                      :1502-1539
-//   k_grad3       the next step's _get_gradient_larger (3D branch)   :1329-1371
+//   k_grad3g      the next step's _get_gradient_larger (3D branch)   :1329-1371
 //   k_mom3        per-row moments of e and of the boundary values: T, q and
 //                 the seven direction factors of the non-Fourier face terms
 //   k_vert3       _set_vertex_energydensity on those moments         :1541-1616
@@ -16,16 +18,18 @@
 //   k_boundary    _set_bound_ee_1 + _set_macro_boundary_condition (solver.hip)
 //   synthetic     _marco_solution with the 3D _get_Re_macro (k_rhs3) :2307-2407, :2692-2734
 //
-// Layout: e [cell][dir][band] as in 2D; the LSQ gradient is stored per cell
-// as three contiguous component rows, grad [cell][x|y|z][dir][band], so that
-// each component of a row is one coalesced load.
+// Layout: e [cell][dir][band] as in 2D.  The LSQ gradient is used only
+// through its projections on the cell's four face offsets (r_f . grad e, in
+// the sources and the outgoing boundary values), so it is stored as those:
+// phi [cell][face][dir][band], each face's row one coalesced load.
 //
 // The LSQ gradient of a tetrahedron reads its whole vertex-neighbour stencil
 // (about 80 cells on a Delaunay mesh), so every row is re-read by about 80
-// gradients.  k_grad3 stages the rows a tile of Morton-consecutive cells
-// needs, one 64-entry slice at a time, in LDS and forms the gradients of the
-// tile's cells from there: HBM / L2 see each staged row once per tile, and the
-// stencil arithmetic runs out of LDS.
+// gradients; k_grad3g reads them from L2, where the waves of
+// Morton-consecutive cells find largely the same rows.  (Staging the rows of
+// a tile of cells in LDS was measured slower: the tile's stencil union is
+// about 190 rows for 8 cells, and the per-entry weights then have to reach
+// every lane from LDS or scalar loads.)
 //
 // The non-Fourier residual is linear in the face values, so the per-system
 // sums over faces factor through per-row moments Y_j = sum_sys K_j(sys) e:
@@ -54,16 +58,12 @@ constexpr int kYV = 8;            // vertex moments (the seven residual factors)
 constexpr int kS3D = 40;          // source record doubles per cell: 4 faces x (n, A/V, r, r')
 constexpr int kS3I = 4;           // source record ints per cell
 constexpr int kSrc3Rows = 4;      // k_source3: cell rows (waves) per workgroup
-constexpr int kG3Block = 512;     // k_grad3: 8 waves
-constexpr int kG3Waves = kG3Block / 64;
 constexpr int kG3W = 64;          // entries per slice
 constexpr int kG3Ent = 8;         // stencil entries per batch (stencils padded to a multiple)
-constexpr int kG3CellsMax = 64;   // cells per tile at most
-constexpr int kG3RowsPerWave = 40;  // staged rows per wave at most (row budget <= 8 x 40)
 constexpr int kFin = 21;          // k_fin3 coefficients per face slot
 constexpr int kMomRows = 4;       // k_mom3: rows (waves) per workgroup
 
-// One LSQ stencil entry of k_grad3: tile-local row and the weights / L_x.
+// One LSQ stencil entry of k_grad3g: the neighbour row and the weights / L_x.
 template <typename AT>
 struct G3Ent;
 template <>
@@ -77,131 +77,35 @@ struct alignas(16) G3Ent<float> {
     float px, py, pz;
 };
 
-struct G3Dev {
-    const int *tile_cell0;  // [n_tiles + 1]: tile t owns cells [cell0[t], cell0[t+1])
-    const int *rptr, *rows;  // staged rows of tile t: rows[rptr[t] .. rptr[t+1])
-    const int *sptr;        // per cell: stencil entries [sptr[c], sptr[c+1])
-    const int *self;        // per cell: tile-local row of the cell
-    const int *ok;          // per cell: the reference computes a gradient (stencil > 3)
-};
-
 // ----------------------------------------------------------------- kernels
 
-// _get_gradient_larger, 3D (StaticBTESynthetic.cpp:1329-1371), for the cells
-// of one tile: grad = sum_j (P_j / L_x) (e_j - e_i), zero for a stencil of 3
-// or fewer cells (zero weights) or a NaN.  The tile's stencil table (row,
-// weights) is copied to LDS once; the staged rows (its cells and their
-// stencils) one 64-entry slice at a time.  Each wave then takes one cell of
-// the tile at a time, its lanes one entry each: every stencil read (the
-// table entry, a broadcast, and the row value) is an LDS read.  Arithmetic in
-// the storage type.
-template <typename ST>
-__global__ void __launch_bounds__(kG3Block) k_grad3(const ST *__restrict__ e, ST *__restrict__ grad, G3Dev g,
-                                                    const G3Ent<ST> *__restrict__ ent, int n_tiles, int DB,
-                                                    int row_cap, int group) {
-    extern __shared__ double lds_raw[];
-    ST *rowbuf = reinterpret_cast<ST *>(lds_raw);
-    int *cinfo = reinterpret_cast<int *>(rowbuf + (size_t)row_cap * kG3W);  // per tile cell: first entry, end, row
-    G3Ent<ST> *tab = reinterpret_cast<G3Ent<ST> *>(cinfo + 3 * kG3CellsMax);
-    // workgroup = (slice group, tile), tiles fastest: the workgroups an XCD
-    // runs together take Morton-consecutive tiles at the same slices, so the
-    // halo rows that neighbouring tiles share are read from its L2
-    int id = xcd_block(), sg = id / n_tiles, tile = id - sg * n_tiles;
-    int s_begin = sg * group * kG3W, s_end = min(DB, s_begin + group * kG3W);
-    if (s_begin >= DB) return;
-    int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-    int c0 = g.tile_cell0[tile], c1 = g.tile_cell0[tile + 1];
-    int r0 = g.rptr[tile], nr = g.rptr[tile + 1] - r0;
-    int e0 = g.sptr[c0], ne = g.sptr[c1] - e0;
-    const int *rows = g.rows + r0;
-    for (int i = threadIdx.x; i < c1 - c0; i += blockDim.x) {
-        cinfo[3 * i] = g.sptr[c0 + i] - e0;
-        cinfo[3 * i + 1] = g.sptr[c0 + i + 1] - e0;
-        cinfo[3 * i + 2] = g.self[c0 + i];
-    }
-    {
-        // the table as 16-byte words
-        constexpr int kW = sizeof(G3Ent<ST>) / 16;
-        const int4 *src = reinterpret_cast<const int4 *>(ent + e0);
-        int4 *dst = reinterpret_cast<int4 *>(tab);
-        for (int i = threadIdx.x; i < ne * kW; i += blockDim.x) dst[i] = src[i];
-    }
-    // wave w stages rows w, w + 8, ...: the next slice's values are loaded
-    // into registers before this slice's products, so that the loads are in
-    // flight while the wave computes
-    ST nxt[kG3RowsPerWave];
-    auto load_slice = [&](int s0) {
-        int tt = s0 + lane < DB ? s0 + lane : s0;  // a valid address for lanes past the row end
-#pragma unroll
-        for (int k = 0; k < kG3RowsPerWave; ++k) {
-            int r = w + k * kG3Waves;
-            if (r < nr) nxt[k] = e[(size_t)rows[r] * DB + tt];
-        }
-    };
-    load_slice(s_begin);
-    for (int s0 = s_begin; s0 < s_end; s0 += kG3W) {
-        int t = s0 + lane;
-        bool in = t < DB;
-        __syncthreads();  // the table is in place; the previous slice's reads are done
-#pragma unroll
-        for (int k = 0; k < kG3RowsPerWave; ++k) {
-            int r = w + k * kG3Waves;
-            if (r < nr) rowbuf[r * kG3W + lane] = nxt[k];
-        }
-        __syncthreads();
-        if (s0 + kG3W < s_end) load_slice(s0 + kG3W);
-        for (int ci = w; ci < c1 - c0; ci += kG3Waves) {
-            int a = cinfo[3 * ci], z = cinfo[3 * ci + 1];
-            ST self = rowbuf[cinfo[3 * ci + 2] * kG3W + lane];
-            ST gx = 0, gy = 0, gz = 0;
-            // stencils are padded to whole batches (zero weights on the
-            // cell's own row); a batch's reads are issued before its products
-            for (int k = a; k < z; k += kG3Ent) {
-                G3Ent<ST> q[kG3Ent];
-#pragma unroll
-                for (int j = 0; j < kG3Ent; ++j) q[j] = tab[k + j];
-                ST v[kG3Ent];
-#pragma unroll
-                for (int j = 0; j < kG3Ent; ++j) v[j] = rowbuf[q[j].loc * kG3W + lane];
-#pragma unroll
-                for (int j = 0; j < kG3Ent; ++j) {
-                    ST d = v[j] - self;
-                    gx += q[j].px * d;
-                    gy += q[j].py * d;
-                    gz += q[j].pz * d;
-                }
-            }
-            if (isnan(gx) || isnan(gy) || isnan(gz)) { gx = 0; gy = 0; gz = 0; }
-            if (in) {
-                ST *gc = grad + (size_t)(c0 + ci) * 3 * DB + t;
-                __builtin_nontemporal_store(gx, gc);
-                __builtin_nontemporal_store(gy, gc + DB);
-                __builtin_nontemporal_store(gz, gc + 2 * DB);
-            }
-        }
-    }
-}
-
-// The same gradient with the neighbour rows read straight from memory (L2):
-// one wave per (cell, group of kG3GS 64-entry slices), the lanes over the
-// entries.  A batch of stencil-table entries (scalar loads) serves the whole
-// group, and each batch issues kG3Ent * kG3GS row loads before its products.
-// Waves of Morton-consecutive cells run together and read largely the same
-// neighbour rows.
-constexpr int kG3GS = 4;
-template <typename ST>
-__global__ void __launch_bounds__(256) k_grad3g(const ST *__restrict__ e, ST *__restrict__ grad,
-                                                const int *__restrict__ sptr, const int *__restrict__ gidx,
-                                                const G3Ent<ST> *__restrict__ ent, int nc, int DB, int ngroups) {
-    int wv = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+// _get_gradient_larger, 3D (StaticBTESynthetic.cpp:1329-1371):
+// grad = sum_j (P_j / L_x) (e_j - e_i), zero for a stencil of 3 or fewer
+// cells (zero weights) or a NaN, stored as its projections on the cell's four
+// face offsets, phi_f = r_f . grad with r_f the face centre minus the cell
+// centre: the sources and the outgoing boundary values use the gradient only
+// through these (_get_Re :1427-1500, _get_bound_ee :1502-1539), so one load
+// per face replaces three.  One wave per (cell, group of GS 64-entry
+// slices), the lanes over the entries, the neighbour rows read straight from
+// memory (L2).  A batch of stencil-table entries (scalar loads) serves the
+// whole group, and each batch issues kG3Ent * GS row loads before its
+// products.  The WPB waves of a workgroup take Morton-consecutive cells, whose
+// stencils (in increasing row order) overlap, so they read largely the same
+// neighbour rows at about the same time.
+template <typename ST, int GS, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, ST *__restrict__ phi,
+                                                     const int *__restrict__ sptr, const int *__restrict__ gidx,
+                                                     const G3Ent<ST> *__restrict__ ent, const double *__restrict__ rec,
+                                                     int nc, int DB, int ngroups) {
+    int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     int sg = wv / nc, c = wv - sg * nc;
     if (sg >= ngroups) return;  // whole wave
     int a = sptr[c], z = sptr[c + 1];
-    int tb = sg * kG3GS * kG3W + lane;
-    ST self[kG3GS], gx[kG3GS], gy[kG3GS], gz[kG3GS];
-    int tt[kG3GS];
+    int tb = sg * GS * kG3W + lane;
+    ST self[GS], gx[GS], gy[GS], gz[GS];
+    int tt[GS];
 #pragma unroll
-    for (int q = 0; q < kG3GS; ++q) {
+    for (int q = 0; q < GS; ++q) {
         int t = tb + q * kG3W;
         tt[q] = t < DB ? t : 0;
         self[q] = e[(size_t)c * DB + tt[q]];
@@ -215,31 +119,32 @@ __global__ void __launch_bounds__(256) k_grad3g(const ST *__restrict__ e, ST *__
             w[j] = ent[k + j];
             nb[j] = gidx[k + j];
         }
-        ST v[kG3Ent][kG3GS];
+        ST v[kG3Ent][GS];
 #pragma unroll
         for (int j = 0; j < kG3Ent; ++j)
 #pragma unroll
-            for (int q = 0; q < kG3GS; ++q) v[j][q] = e[(size_t)nb[j] * DB + tt[q]];
+            for (int q = 0; q < GS; ++q) v[j][q] = e[(size_t)nb[j] * DB + tt[q]];
 #pragma unroll
         for (int j = 0; j < kG3Ent; ++j)
 #pragma unroll
-            for (int q = 0; q < kG3GS; ++q) {
+            for (int q = 0; q < GS; ++q) {
                 ST d = v[j][q] - self[q];
                 gx[q] += w[j].px * d;
                 gy[q] += w[j].py * d;
                 gz[q] += w[j].pz * d;
             }
     }
+    const double *rc = rec + (size_t)c * kS3D;
 #pragma unroll
-    for (int q = 0; q < kG3GS; ++q) {
+    for (int q = 0; q < GS; ++q) {
         int t = tb + q * kG3W;
         if (t >= DB) continue;
-        ST x = gx[q], y = gy[q], w = gz[q];
+        double x = gx[q], y = gy[q], w = gz[q];
         if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
-        ST *gc = grad + (size_t)c * 3 * DB + t;
-        __builtin_nontemporal_store(x, gc);
-        __builtin_nontemporal_store(y, gc + DB);
-        __builtin_nontemporal_store(w, gc + 2 * DB);
+        ST *pc = phi + (size_t)c * 4 * DB + t;
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+            __builtin_nontemporal_store((ST)(rc[f * 10 + 4] * x + rc[f * 10 + 5] * y + rc[f * 10 + 6] * w), pc + f * DB);
     }
 }
 
@@ -249,14 +154,15 @@ __global__ void __launch_bounds__(256) k_grad3g(const ST *__restrict__ e, ST *__
 //       - sum_in,boundary a eb,      a = tau (A/V) (v . n),
 // written to each direction's upwind order for the sweep: S[dir][position][band]
 // (sh < 0), or S[dir][band chunk][position][band in chunk] with chunks of
-// 2^sh bands (the layout of k_sweep_lv).  One wave per cell
-// row; the cell's face record (normal, A/V, r from its centre, r' from the
-// neighbour's centre) sits in scalar registers, and every face issues its
-// loads (the neighbour's three gradient components or the boundary value)
-// before any term is formed.
+// 2^sh bands (the layout of k_sweep_lv).  r . g_c is the cell's own face
+// projection phi_f and r' . g_nb the neighbour's projection on the shared
+// face (k_grad3g), so each face is one load, from the row its sign of
+// v . n selects.  One wave per cell row; the cell's face record (normal,
+// A/V) sits in scalar registers, and every face issues its load before any
+// term is formed.
 template <typename ST>
 __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3(const int *__restrict__ posc, ST *__restrict__ S,
-                                                            const ST *__restrict__ grad, const ST *__restrict__ eb,
+                                                            const ST *__restrict__ phi, const ST *__restrict__ eb,
                                                             const double *__restrict__ Told,
                                                             const double *__restrict__ Q,
                                                             const double *__restrict__ rec,
@@ -288,42 +194,32 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3(const int *__restric
     if (c >= nc) return;
     int DB = nD * nB;
     const double *rd = rec + (size_t)c * kS3D;
-    double fnx[4], fny[4], fnz[4], fav[4], frx[4], fry[4], frz[4], fqx[4], fqy[4], fqz[4];
-    const ST *fin[4];
-    bool fint[4];
+    double fnx[4], fny[4], fnz[4], fav[4];
+    const ST *fo[4], *fin[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         const double *r = rd + f * 10;
         fnx[f] = r[0]; fny[f] = r[1]; fnz[f] = r[2]; fav[f] = r[3];
-        frx[f] = r[4]; fry[f] = r[5]; frz[f] = r[6];
-        fqx[f] = r[7]; fqy[f] = r[8]; fqz[f] = r[9];
-        int nb = reci[(size_t)c * kS3I + f];
-        fint[f] = nb >= 0;
-        fin[f] = nb >= 0 ? grad + (size_t)nb * 3 * DB : eb + (size_t)(-nb - 1) * DB;
+        // interior: 4 nb + the neighbour's slot of the shared face;
+        // boundary: -(entry) - 1
+        int q = reci[(size_t)c * kS3I + f];
+        fo[f] = phi + ((size_t)c * 4 + f) * DB;
+        fin[f] = q >= 0 ? phi + (size_t)q * DB : eb + (size_t)(-q - 1) * DB;
     }
     double To = Told[c] - m.eref, Qc = Q[c];
-    const ST *gc = grad + (size_t)c * 3 * DB;
     const int *pc = pos + w * nD;
     for (int t = lane; t < DB; t += 64) {
         int d = t / nB, b = t - d * nB;
         double vb = mvg[b], tau = mtau[b], sx = msx[d], sy = msy[d], sz = msz[d];
+        double dot[4], u[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            dot[f] = vb * sx * fnx[f] + vb * sy * fny[f] + vb * sz * fnz[f];
+            u[f] = (dot[f] >= 0 ? fo[f] : fin[f])[t];
+        }
         double s = To * mcap[b] + Qc * mhr[b] * mtau[b];
-        double gx = gc[t], gy = gc[t + DB], gz = gc[t + 2 * DB];
-        double ux[4], uy[4], uz[4];
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            ux[f] = fin[f][t];
-            uy[f] = fint[f] ? (double)fin[f][t + DB] : 0.0;
-            uz[f] = fint[f] ? (double)fin[f][t + 2 * DB] : 0.0;
-        }
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            double dot = vb * sx * fnx[f] + vb * sy * fny[f] + vb * sz * fnz[f];
-            double a = tau * fav[f] * dot;
-            double uo = frx[f] * gx + fry[f] * gy + frz[f] * gz;
-            double ui = fint[f] ? fqx[f] * ux[f] + fqy[f] * uy[f] + fqz[f] * uz[f] : ux[f];
-            s -= a * (dot >= 0 ? uo : ui);
-        }
+        for (int f = 0; f < 4; ++f) s -= tau * fav[f] * dot[f] * u[f];
         size_t si = sh < 0 ? ((size_t)d * nc + pc[d]) * nB + b
                            : ((((size_t)d * (nB >> sh) + (b >> sh)) * nc + pc[d]) << sh) + (b & ((1 << sh) - 1));
         __builtin_nontemporal_store((ST)s, S + si);
@@ -449,27 +345,26 @@ __global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec,
 }
 
 // _get_bound_ee (StaticBTESynthetic.cpp:1502-1539): outgoing boundary values
-// e + r . grad e, one wave per boundary entry.
+// e + r . grad e, the cell's face projection phi_f of k_grad3g; one wave per
+// boundary entry.
 template <typename ST>
-__global__ void __launch_bounds__(256) k_bound_out3(const ST *__restrict__ e, const ST *__restrict__ grad,
+__global__ void __launch_bounds__(256) k_bound_out3(const ST *__restrict__ e, const ST *__restrict__ phi,
                                                     ST *__restrict__ eb, Modes m, const int *__restrict__ ent_list,
                                                     const int *__restrict__ be_cell, const int *__restrict__ be_slot,
                                                     const double *__restrict__ snx, const double *__restrict__ sny,
-                                                    const double *__restrict__ snz, const double *__restrict__ srx,
-                                                    const double *__restrict__ sry, const double *__restrict__ srz,
-                                                    int n_ent, int nB, int nD) {
+                                                    const double *__restrict__ snz, int n_ent, int nB, int nD) {
     int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
     if (k >= n_ent) return;
     int en = ent_list[k], c = be_cell[en], s = be_slot[en];
-    double nx = snx[s], ny = sny[s], nz = snz[s], rx = srx[s], ry = sry[s], rz = srz[s];
+    double nx = snx[s], ny = sny[s], nz = snz[s];
     int DB = nD * nB;
-    const ST *ec = e + (size_t)c * DB, *gc = grad + (size_t)c * 3 * DB;
+    const ST *ec = e + (size_t)c * DB, *pf = phi + ((size_t)c * 4 + (s - c * 6)) * DB;
     ST *eo = eb + (size_t)en * DB;
     for (int t = lane; t < DB; t += 64) {
         int d = t / nB, b = t - d * nB;
         double vb = m.vg[b];
         double dot = vb * m.sx[d] * nx + vb * m.sy[d] * ny + vb * m.sz[d] * nz;
-        if (dot >= 0) eo[t] = (ST)(ec[t] + (rx * gc[t] + ry * gc[t + DB] + rz * gc[t + 2 * DB]));
+        if (dot >= 0) eo[t] = (ST)((double)ec[t] + (double)pf[t]);
     }
 }
 
@@ -659,11 +554,6 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     double invLx = 1.0 / p.deck.Lx;
     for (int c = 0; c < nc; ++c)
         if (p.face_size[c] != 4) { err_ = "3D meshes of tetrahedra only"; return false; }
-    // LDS budgets of the gradient tiles: FP64 192 staged rows (96 KiB) and a
-    // 1536-entry table (48 KiB); FP32 320 rows (80 KiB) and 2560 entries (40 KiB)
-    if (g3_row_cap <= 0) g3_row_cap = fp32_ ? 320 : 192;
-    if (g3_row_cap > kG3Waves * kG3RowsPerWave) { err_ = "3D gradient tile row budget above the kernel's"; return false; }
-    if (g3_ent_cap <= 0) g3_ent_cap = fp32_ ? 2560 : 1536;
     // ---- source records
     {
         std::vector<double> rec((size_t)nc * kS3D, 0.0);
@@ -679,7 +569,12 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 if (fb == -1) {
                     int n = p.face_neighbor[s];
                     r[7] = p.face_cx[s] - p.cx[n]; r[8] = p.face_cy[s] - p.cy[n]; r[9] = p.face_cz[s] - p.cz[n];
-                    reci[(size_t)c * kS3I + f] = n;
+                    // the neighbour's slot of the shared face: its phi row
+                    int fn = -1;
+                    for (int k = 0; k < 4; ++k)
+                        if (p.face_neighbor[(size_t)n * 6 + k] == c && p.face_bound[(size_t)n * 6 + k] == -1) fn = k;
+                    if (fn < 0) { err_ = "internal error: interior face without its neighbour's side"; return false; }
+                    reci[(size_t)c * kS3I + f] = n * 4 + fn;
                 } else {
                     int ibc = -1;
                     for (int k = 0; k < 2; ++k)
@@ -732,122 +627,22 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
         qa.insert(qa.end(), q[2].begin(), q[2].end());
         if (!upload(d_gptr, qptr) || !upload(d_gidx, qidx) || !upload(d_gP, qa) || !upload(d_gok, gok)) return false;
     }
-    // ---- k_grad3 tiles: Morton-consecutive cells while the union of their
-    // stencils stays within the row budget
+    // ---- gradient stencil tables: global neighbour rows, padded per cell to
+    // whole batches with the cell's own row at weight 0 (each adds an exact
+    // zero)
     {
         int nnz = p.nbr_ptr[nc];
-        int cap = g3_row_cap, ecap = g3_ent_cap, ccap = std::min(g3_cell_cap, kG3CellsMax);
-        auto padded = [&](int c) { return (p.nbr_ptr[c + 1] - p.nbr_ptr[c] + kG3Ent - 1) / kG3Ent * kG3Ent; };
-        std::vector<int> tile0(1, 0), rptr(1, 0), rows, sptr(nc + 1, 0), self(nc, 0);
-        std::vector<int> loc_of(nc, -1), tile_rows;
-        std::vector<int> e_loc;
-        std::vector<int> e_cell;  // entry -> stencil position (for the weights)
-        auto row_count = [&](int c, const std::vector<int> &cur) {
-            int add = loc_of[c] < 0 ? 1 : 0;
-            for (int k = p.nbr_ptr[c]; k < p.nbr_ptr[c + 1]; ++k) add += loc_of[p.nbr_idx[k]] < 0 ? 1 : 0;
-            return (int)cur.size() + add;
-        };
-        int c = 0;
-        while (c < nc) {
-            tile_rows.clear();
-            int cstart = c;
-            int tile_ents = 0;
-            while (c < nc && c - cstart < ccap) {
-                if (c > cstart && (row_count(c, tile_rows) > cap || tile_ents + padded(c) > ecap)) break;
-                if (row_count(c, tile_rows) > cap || padded(c) > ecap) {
-                    err_ = "a 3D LSQ stencil is larger than the tile budget";
-                    return false;
-                }
-                tile_ents += padded(c);
-                auto add_row = [&](int r) {
-                    if (loc_of[r] < 0) { loc_of[r] = (int)tile_rows.size(); tile_rows.push_back(r); }
-                };
-                add_row(c);
-                for (int k = p.nbr_ptr[c]; k < p.nbr_ptr[c + 1]; ++k) add_row(p.nbr_idx[k]);
-                ++c;
-            }
-            // per cell of the tile: its row and its stencil entries
-            for (int cc = cstart; cc < c; ++cc) {
-                self[cc] = loc_of[cc];
-                for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) {
-                    e_loc.push_back(loc_of[p.nbr_idx[k]]);
-                    e_cell.push_back(k);
-                }
-                // pad to whole batches with the cell's own row at weight 0
-                // (each adds an exact zero)
-                while ((e_loc.size() - sptr[cc]) % kG3Ent) {
-                    e_loc.push_back(loc_of[cc]);
-                    e_cell.push_back(-1);
-                }
-                sptr[cc + 1] = (int)e_loc.size();
-            }
-            for (int r : tile_rows) loc_of[r] = -1;
-            rows.insert(rows.end(), tile_rows.begin(), tile_rows.end());
-            rptr.push_back((int)rows.size());
-            tile0.push_back(c);
-        }
-        g3_ntiles = (int)tile0.size() - 1;
-        g3_rows = 0;
-        g3_ents = 0;
-        for (int t = 0; t < g3_ntiles; ++t) {
-            g3_rows = std::max(g3_rows, rptr[t + 1] - rptr[t]);
-            g3_ents = std::max(g3_ents, sptr[tile0[t + 1]] - sptr[tile0[t]]);
-        }
-        // validate what the kernel assumes: every local row names the cell it
-        // stands for and lies inside its tile
-        for (int t = 0; t < g3_ntiles; ++t)
-            for (int cc = tile0[t]; cc < tile0[t + 1]; ++cc) {
-                int nr = rptr[t + 1] - rptr[t];
-                if (self[cc] < 0 || self[cc] >= nr || rows[rptr[t] + self[cc]] != cc) { err_ = "internal error: 3D tile self row"; return false; }
-                for (int k = sptr[cc]; k < sptr[cc + 1]; ++k) {
-                    int l = e_loc[k];
-                    int want = e_cell[k] >= 0 ? p.nbr_idx[e_cell[k]] : cc;
-                    if (l < 0 || l >= nr || rows[rptr[t] + l] != want) { err_ = "internal error: 3D tile stencil row"; return false; }
-                }
-            }
-        size_t ne3 = e_loc.size();
-        if (fp32_) {
-            std::vector<G3Ent<float>> ent(ne3);
-            for (size_t k = 0; k < ne3; ++k) {
-                int j = e_cell[k];
-                ent[k].loc = e_loc[k];
-                ent[k].px = j < 0 ? 0.0f : (float)(P3[j] * invLx);  // zero without a gradient (P3 = 0)
-                ent[k].py = j < 0 ? 0.0f : (float)(P3[(size_t)nnz + j] * invLx);
-                ent[k].pz = j < 0 ? 0.0f : (float)(P3[2 * (size_t)nnz + j] * invLx);
-            }
-            DBuf<char> &dst = d_g3_ent;
-            if (!alloc(dst, ne3 * sizeof(G3Ent<float>)) ||
-                !check(hipMemcpy(dst.p, ent.data(), ne3 * sizeof(G3Ent<float>), hipMemcpyHostToDevice), "H2D"))
-                return false;
-        } else {
-            std::vector<G3Ent<double>> ent(ne3);
-            for (size_t k = 0; k < ne3; ++k) {
-                int j = e_cell[k];
-                ent[k].loc = e_loc[k];
-                ent[k].pad = 0;
-                ent[k].px = j < 0 ? 0.0 : P3[j] * invLx;
-                ent[k].py = j < 0 ? 0.0 : P3[(size_t)nnz + j] * invLx;
-                ent[k].pz = j < 0 ? 0.0 : P3[2 * (size_t)nnz + j] * invLx;
-            }
-            DBuf<char> &dst = d_g3_ent;
-            if (!alloc(dst, ne3 * sizeof(G3Ent<double>)) ||
-                !check(hipMemcpy(dst.p, ent.data(), ne3 * sizeof(G3Ent<double>), hipMemcpyHostToDevice), "H2D"))
-                return false;
-        }
-        g3_lds = (size_t)g3_rows * kG3W * (fp32_ ? sizeof(float) : sizeof(double)) + 3 * kG3CellsMax * sizeof(int) +
-                 (size_t)g3_ents * (fp32_ ? sizeof(G3Ent<float>) : sizeof(G3Ent<double>));
-        int dev_lds = 0;
-        (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, opt.device);
-        if (g3_lds > (size_t)dev_lds) { err_ = "3D gradient tiles exceed the LDS of a workgroup"; return false; }
-        if (!upload(d_g3_tile, tile0) || !upload(d_g3_rptr, rptr) || !upload(d_g3_rows, rows) ||
-            !upload(d_g3_sptr, sptr) || !upload(d_g3_self, self) || !upload(d_g3_ok, gok))
-            return false;
-        // the same stencils for the direct-gather kernel: global neighbour
-        // rows, padded per cell to whole batches with the cell's own row
         std::vector<int> gptr2(nc + 1, 0), gidx2;
         std::vector<int> gpos;  // stencil position per entry (-1: padding)
+        std::vector<std::pair<int, int>> st;
         for (int cc = 0; cc < nc; ++cc) {
-            for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) { gidx2.push_back(p.nbr_idx[k]); gpos.push_back(k); }
+            // in increasing neighbour order: the waves of a workgroup take
+            // consecutive cells, whose stencils overlap, and then read the
+            // shared rows at about the same time (L1 hits)
+            st.clear();
+            for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) st.push_back({p.nbr_idx[k], k});
+            std::sort(st.begin(), st.end());
+            for (auto &q : st) { gidx2.push_back(q.first); gpos.push_back(q.second); }
             while ((gidx2.size() - gptr2[cc]) % kG3Ent) { gidx2.push_back(cc); gpos.push_back(-1); }
             gptr2[cc + 1] = (int)gidx2.size();
         }
@@ -1019,28 +814,28 @@ void Solver::launch_rhs3(bool macro, const double *bval, double *rhs) {
 // the solver's stream.
 void Solver::launch_grad3() {
     int DB = nD * nB;
-    if (g3_mode == 1) {
-        int ngroups = (DB + kG3GS * kG3W - 1) / (kG3GS * kG3W);
-        int grid = ((blocks_for((long)ngroups * nc, 4) + kXcd - 1) / kXcd) * kXcd;
-        if (fp32_)
-            hipLaunchKernelGGL(k_grad3g<float>, dim3(grid), dim3(256), 0, s_, reinterpret_cast<const float *>(d_e.p),
-                               reinterpret_cast<float *>(d_grad.p), d_g3g_ptr.p, d_g3g_idx.p,
-                               reinterpret_cast<const G3Ent<float> *>(d_g3g_ent.p), nc, DB, ngroups);
-        else
-            hipLaunchKernelGGL(k_grad3g<double>, dim3(grid), dim3(256), 0, s_, d_e.p, d_grad.p, d_g3g_ptr.p,
-                               d_g3g_idx.p, reinterpret_cast<const G3Ent<double> *>(d_g3g_ent.p), nc, DB, ngroups);
-        return;
-    }
-    G3Dev g{d_g3_tile.p, d_g3_rptr.p, d_g3_rows.p, d_g3_sptr.p, d_g3_self.p, d_g3_ok.p};
-    int groups = (DB + g3_group * kG3W - 1) / (g3_group * kG3W);
-    int grid = ((groups * g3_ntiles + kXcd - 1) / kXcd) * kXcd;
-    if (fp32_)
-        hipLaunchKernelGGL(k_grad3<float>, dim3(grid), dim3(kG3Block), g3_lds, s_, reinterpret_cast<float *>(d_e.p),
-                           reinterpret_cast<float *>(d_grad.p), g,
-                           reinterpret_cast<const G3Ent<float> *>(d_g3_ent.p), g3_ntiles, DB, g3_rows, g3_group);
-    else
-        hipLaunchKernelGGL(k_grad3<double>, dim3(grid), dim3(kG3Block), g3_lds, s_, d_e.p, d_grad.p, g,
-                           reinterpret_cast<const G3Ent<double> *>(d_g3_ent.p), g3_ntiles, DB, g3_rows, g3_group);
+    int ngroups = (DB + g3_gs * kG3W - 1) / (g3_gs * kG3W);
+    int grid = ((blocks_for((long)ngroups * nc, g3_wpb) + kXcd - 1) / kXcd) * kXcd;
+#define GBTE_G3(ST_, GS_, W_)                                                                                    \
+    hipLaunchKernelGGL((k_grad3g<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                               \
+                       reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<ST_ *>(d_grad.p), d_g3g_ptr.p,    \
+                       d_g3g_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_g3g_ent.p), d_s3d.p, nc, DB, ngroups)
+#define GBTE_G3_W(ST_, GS_)                         \
+    do {                                            \
+        if (g3_wpb == 16) GBTE_G3(ST_, GS_, 16);    \
+        else GBTE_G3(ST_, GS_, 4);                  \
+    } while (0)
+#define GBTE_G3_GS(ST_)                        \
+    do {                                       \
+        if (g3_gs == 1) GBTE_G3_W(ST_, 1);     \
+        else if (g3_gs == 2) GBTE_G3_W(ST_, 2); \
+        else GBTE_G3_W(ST_, 4);                \
+    } while (0)
+    if (fp32_) GBTE_G3_GS(float);
+    else GBTE_G3_GS(double);
+#undef GBTE_G3_GS
+#undef GBTE_G3_W
+#undef GBTE_G3
 }
 
 void Solver::launch_source3(const Modes &m) {
@@ -1088,12 +883,10 @@ void Solver::launch_bound_out3(const Modes &m) {
     if (fp32_)
         hipLaunchKernelGGL(k_bound_out3<float>, grid, dim3(256), 0, s_, reinterpret_cast<const float *>(d_e.p),
                            reinterpret_cast<const float *>(d_grad.p), reinterpret_cast<float *>(d_eb.p), m,
-                           d_ent_list.p, d_be_cell.p, d_be_slot.p, d_snx.p, d_sny.p, d_snz.p, d_srx.p, d_sry.p,
-                           d_srz.p, n_ent, nB, nD);
+                           d_ent_list.p, d_be_cell.p, d_be_slot.p, d_snx.p, d_sny.p, d_snz.p, n_ent, nB, nD);
     else
         hipLaunchKernelGGL(k_bound_out3<double>, grid, dim3(256), 0, s_, d_e.p, d_grad.p, d_eb.p, m, d_ent_list.p,
-                           d_be_cell.p, d_be_slot.p, d_snx.p, d_sny.p, d_snz.p, d_srx.p, d_sry.p, d_srz.p, n_ent, nB,
-                           nD);
+                           d_be_cell.p, d_be_slot.p, d_snx.p, d_sny.p, d_snz.p, n_ent, nB, nD);
 }
 
 void Solver::launch_moments3(const Modes &m) {
@@ -1119,27 +912,34 @@ bool Solver::tune3d() {
     if (!check(hipMemsetAsync(d_e.p, 0, d_e.n * sizeof(double), s_), "hipMemsetAsync")) return false;
     hipEvent_t a, b;
     if (!check(hipEventCreate(&a), "hipEventCreate") || !check(hipEventCreate(&b), "hipEventCreate")) return false;
-    double best = 1e300;
-    int pick = 0;
     char line[160];
-    for (int mode = 0; mode < 2; ++mode) {
-        g3_mode = mode;
-        launch_grad3();  // warm
-        (void)hipEventRecord(a, s_);
-        for (int r = 0; r < 3; ++r) launch_grad3();
-        (void)hipEventRecord(b, s_);
-        if (!check(hipEventSynchronize(b), "tune sync")) return false;
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, a, b);
-        double per = ms / 3.0;
-        snprintf(line, sizeof line, "k_grad3 %s: %.3f ms per launch\n",
-                 mode == 0 ? "LDS tiles" : "direct gather", per);
+    {
+        // the gradient: 64-entry slices per wave (one stencil-table batch
+        // serves them all; more slices, more loads in flight per wave)
+        double best = 1e300;
+        int pick = 4, pickw = 4;
+        for (int wpb : {4, 16})
+            for (int gs : {1, 2, 4}) {
+                g3_gs = gs;
+                g3_wpb = wpb;
+                launch_grad3();  // warm
+                (void)hipEventRecord(a, s_);
+                for (int r = 0; r < 3; ++r) launch_grad3();
+                (void)hipEventRecord(b, s_);
+                if (!check(hipEventSynchronize(b), "tune sync")) return false;
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, a, b);
+                double per = ms / 3.0;
+                snprintf(line, sizeof line, "k_grad3g %d slices per wave, %d waves per workgroup: %.3f ms per launch\n",
+                         gs, wpb, per);
+                tune_log_ += line;
+                if (per < best) { best = per; pick = gs; pickw = wpb; }
+            }
+        g3_gs = pick;
+        g3_wpb = pickw;
+        snprintf(line, sizeof line, "k_grad3g chosen: %d slices per wave, %d waves per workgroup\n", pick, pickw);
         tune_log_ += line;
-        if (per < best) { best = per; pick = mode; }
     }
-    g3_mode = pick;
-    snprintf(line, sizeof line, "k_grad3 chosen: %s\n", pick == 0 ? "LDS tiles" : "direct gather");
-    tune_log_ += line;
     // sweeps: the wave-specialised kernel (when its schedule was built), the
     // one-workgroup-per-direction level-synchronous kernel (FP64) and the
     // chunked level-synchronous kernel at each band chunk and positions per
