@@ -161,7 +161,7 @@ def kernel_name(phase, dim, prec, tuning):
             return "k_sweep_lv"
         return "k_sweep_dir" if "sweep chosen: level-synchronous" in tuning else "k_sweep_pc"
     if phase == "grad":
-        return "k_grad3g"
+        return "k_grad3m" if "gradient chosen: k_grad3m" in tuning else "k_grad3g"
     return {"source": "k_source3", "moments": "k_mom3 + k_vert3 + k_fin3"}[phase]
 
 

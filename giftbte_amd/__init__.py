@@ -97,7 +97,7 @@ class BTESolver:
 
     def __init__(self, problem: Problem, precision: str = "fp64", device: int = 0, verbose: bool = False,
                  macro_rtol: float = 0.0, linear_solver: str = "direct", fallback_kernels: bool = False,
-                 sweep_kernel: str = "auto"):
+                 sweep_kernel: str = "auto", gradient_kernel: str = "auto"):
         self.problem = problem
         self.lib = problem.lib
         t = problem.tables
@@ -109,10 +109,13 @@ class BTESolver:
         sweeps = {"auto": 0, "wavespec": 1, "level": 2, "chunked": 3}
         if sweep_kernel not in sweeps:
             raise ValueError(f"sweep_kernel must be one of {sorted(sweeps)}")
+        grads = {"auto": 0, "gather": 1, "matrix": 2}
+        if gradient_kernel not in grads:
+            raise ValueError(f"gradient_kernel must be one of {sorted(grads)}")
         opts = _lib.Options(_lib.GBTE_FP64 if precision == "fp64" else _lib.GBTE_FP32_SWEEP, device, int(verbose),
                             macro_rtol,
                             _lib.GBTE_LINSOLVE_DIRECT if linear_solver == "direct" else _lib.GBTE_LINSOLVE_PCG,
-                            int(bool(fallback_kernels)), sweeps[sweep_kernel])
+                            int(bool(fallback_kernels)), sweeps[sweep_kernel], grads[gradient_kernel])
         self._h = C.c_void_p()
         rc = self.lib.gbte_create(problem.handle, C.byref(opts), C.byref(self._h))
         if rc != 0:

@@ -238,6 +238,12 @@ class Solver {
     DBuf<char> d_g3g_ent;
     DBuf<int> d_g3g_ptr, d_g3g_idx;
     int g3_gs = 4, g3_wpb = 4;  // k_grad3g: 64-entry slices per wave, waves per workgroup (chosen by tune3d)
+    // matrix-core gradient (k_grad3m): tiles of up to four cells, their rows,
+    // cells and A' operands; used when the tuner finds it faster
+    bool gm_ok = false, use_gm = false;
+    int gm_ntiles = 0;
+    DBuf<int> d_gm_rows, d_gm_cell;
+    DBuf<char> d_gm_A;
     std::string tune_log_;  // the autotuner's candidates, timings and choices
     DBuf<double> d_ndy, d_ndu, d_ndx;
     // state

@@ -48,6 +48,7 @@
 #include <cmath>
 #include <cstdio>
 #include <limits>
+#include <type_traits>
 
 namespace gbte {
 
@@ -58,6 +59,10 @@ constexpr int kYV = 8;            // vertex moments (the seven residual factors)
 constexpr int kS3D = 40;          // source record doubles per cell: 4 faces x (n, A/V, r, r')
 constexpr int kS3I = 4;           // source record ints per cell
 constexpr int kSrc3Rows = 4;      // k_source3: cell rows (waves) per workgroup
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kG3W = 64;          // entries per slice
 constexpr int kG3Ent = 8;         // stencil entries per batch (stencils padded to a multiple)
 constexpr int kFin = 21;          // k_fin3 coefficients per face slot
@@ -145,6 +150,144 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, S
 #pragma unroll
         for (int f = 0; f < 4; ++f)
             __builtin_nontemporal_store((ST)(rc[f * 10 + 4] * x + rc[f * 10 + 5] * y + rc[f * 10 + 6] * w), pc + f * DB);
+    }
+}
+
+// The same face projections on the matrix cores.  Tiles of up to four
+// Morton-consecutive cells; the rows a tile needs (its cells and their
+// stencils, at most 4 kGmKS) are the K dimension of one 16x16x4 product per
+// four rows:
+//   phi[cell q][face f][t] = sum_k A'[f, q][k] (E[row k][t] - E[ref][t]),
+//   A'[f, q][k] = r_f(q) . W_q[k],  W_q[k] = P_k / L_x on q's stencil rows,
+//   -sum P / L_x on q's own row, 0 elsewhere,
+// so the 16 matrix rows are the 4 faces x 4 cells and the output needs no
+// further arithmetic.  Every row of A' sums to zero, so subtracting one row
+// of the tile (ref) from every staged value leaves the product unchanged
+// while keeping the operands at the size of the differences.  One wave per
+// tile, over slices of 16 NB entries: in K-step s lane l loads row
+// 4 s + l / 16 at entries NB (l % 16) .. NB (l % 16) + NB - 1 of the slice,
+// which is exactly its B operand of K-step s for each of the NB products
+// (product j takes the entries NB n + j), so the rows go from memory (L2,
+// shared by neighbouring tiles) straight into the products without LDS.
+// The A' operands stay in registers for the whole tile; in FP32 each B
+// register is reloaded with the next slice's value as soon as its product has
+// issued, so the next slice's loads are in flight behind this slice's
+// products (FP64 keeps two waves per SIMD instead).
+// Output: lane l holds cell l / 16, faces 0..3 (elements), entries
+// NB (l % 16) + j (FP64: D[4 i + l / 16][l % 16] in element i, so A' row
+// m = 4 f + q; FP32: D[4 (l / 16) + i][l % 16], so m = 4 q + f).  A NaN
+// zeroes the four projections (the reference's NaN -> 0 gradient); as in any
+// dense product, a non-finite staged value reaches every cell of the tile at
+// that entry.
+constexpr int kGmKS = 40;  // K-steps per tile (at most 160 rows)
+template <typename ST, int NB>
+__global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__restrict__ phi,
+                                               const int *__restrict__ trows, const int *__restrict__ tcell,
+                                               const ST *__restrict__ tA, int n_tiles, int DB, int spg) {
+    using BV = typename std::conditional<NB == 2, typename std::conditional<sizeof(ST) == 4, v2f, v2d>::type, ST>::type;
+    // wave = (slice group, tile), tiles fastest: the waves that run together
+    // work on the same slice group, so the rows that neighbouring tiles share
+    // are read from L2
+    int id = xcd_block(), sg = id / n_tiles, tile = id - sg * n_tiles;
+    constexpr int W = 16 * NB;  // entries per slice
+    constexpr bool kRoll = sizeof(ST) == 4;
+    int s_begin = sg * spg * W, s_end = min(DB, s_begin + spg * W);
+    if (s_begin >= DB) return;  // whole wave
+    int lane = threadIdx.x, q = lane >> 4, n = lane & 15;
+    ST a[kGmKS];
+    int row[kGmKS];
+#pragma unroll
+    for (int s = 0; s < kGmKS; ++s) {
+        a[s] = tA[((size_t)tile * kGmKS + s) * 64 + lane];
+        row[s] = trows[((size_t)tile * kGmKS + s) * 4 + q];
+    }
+    int cell = tcell[tile * 4 + q];
+    int ref = trows[(size_t)tile * kGmKS * 4];
+    // lane's first entry of a slice, clamped to a whole vector inside the row
+    auto ent0 = [&](int s0) {
+        int t = s0 + NB * n;
+        return t + NB <= DB ? t : DB - NB;
+    };
+    auto ld = [&](int r, int t) { return *reinterpret_cast<const BV *>(e + (size_t)r * DB + t); };
+    BV b[kGmKS];
+    BV rn;
+    {
+        int t0 = ent0(s_begin);
+        rn = ld(ref, t0);
+#pragma unroll
+        for (int s = 0; s < kGmKS; ++s) b[s] = ld(row[s], t0);
+    }
+    for (int s0 = s_begin; s0 < s_end; s0 += W) {
+        int tn = ent0(s0 + W < s_end ? s0 + W : s0);  // the next slice (the last one again at the end)
+        if constexpr (!kRoll) {
+            // FP64: the registers of a second set of B operands would halve
+            // the waves per SIMD; this slice's loads are issued here instead
+            if (s0 != s_begin) {
+                int tc = ent0(s0);
+                rn = ld(ref, tc);
+#pragma unroll
+                for (int s = 0; s < kGmKS; ++s) {
+                    // an empty asm keeps the 64-bit row offsets from being
+                    // hoisted out of the slice loop (80 registers)
+                    int r = row[s];
+                    asm volatile("" : "+v"(r));
+                    b[s] = ld(r, tc);
+                }
+            }
+        }
+        BV rc = rn;
+        if constexpr (kRoll) rn = ld(ref, tn);
+        double v[NB][4];
+        if constexpr (sizeof(ST) == 4) {
+            v4f acc[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) acc[j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < kGmKS; ++s) {
+                BV d = b[s] - rc;
+                if constexpr (kRoll) b[s] = ld(row[s], tn);
+                if constexpr (NB == 2) {
+                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], d[0], acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], d[1], acc[1], 0, 0, 0);
+                } else {
+                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], d, acc[0], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[j][i] = acc[j][i];
+        } else {
+            v4d acc[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < kGmKS; ++s) {
+                BV d = b[s] - rc;
+                if constexpr (kRoll) b[s] = ld(row[s], tn);
+                if constexpr (NB == 2) {
+                    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], d[0], acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], d[1], acc[1], 0, 0, 0);
+                } else {
+                    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], d, acc[0], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[j][i] = acc[j][i];
+        }
+        int t = s0 + NB * n;
+        if (cell >= 0) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (t + j >= DB) continue;
+                bool bad = isnan(v[j][0]) || isnan(v[j][1]) || isnan(v[j][2]) || isnan(v[j][3]);
+                ST *pc = phi + (size_t)cell * 4 * DB + t + j;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) __builtin_nontemporal_store((ST)(bad ? 0.0 : v[j][i]), pc + i * DB);
+            }
+        }
     }
 }
 
@@ -554,9 +697,11 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     double invLx = 1.0 / p.deck.Lx;
     for (int c = 0; c < nc; ++c)
         if (p.face_size[c] != 4) { err_ = "3D meshes of tetrahedra only"; return false; }
-    // ---- source records
+    // ---- source records (kept: the face offsets also weight the gradient's
+    // face projections)
+    std::vector<double> rec_h((size_t)nc * kS3D, 0.0);
     {
-        std::vector<double> rec((size_t)nc * kS3D, 0.0);
+        std::vector<double> &rec = rec_h;
         std::vector<int> reci((size_t)nc * kS3I, 0);
         for (int c = 0; c < nc; ++c)
             for (int f = 0; f < 4; ++f) {
@@ -674,6 +819,80 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 return false;
         }
         if (!upload(d_g3g_ptr, gptr2) || !upload(d_g3g_idx, gidx2)) return false;
+        // ---- matrix-core tiles (k_grad3m): up to four consecutive cells while
+        // the rows they need fit kGmKS K-steps
+        {
+            std::vector<int> trows, tcell;
+            std::vector<double> tA;  // [tile][K-step][lane], FP64 values
+            std::vector<int> loc_of(nc, -1), cur;
+            auto rows_with = [&](int cc) {
+                int add = loc_of[cc] < 0 ? 1 : 0;
+                for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) add += loc_of[p.nbr_idx[k]] < 0 ? 1 : 0;
+                return (int)cur.size() + add;
+            };
+            auto add_row = [&](int r) {
+                if (loc_of[r] < 0) { loc_of[r] = (int)cur.size(); cur.push_back(r); }
+            };
+            const int cap = 4 * kGmKS;
+            bool ok = true;
+            int c = 0;
+            while (c < nc && ok) {
+                cur.clear();
+                int c0 = c;
+                while (c < nc && c - c0 < 4 && (c == c0 || rows_with(c) <= cap)) {
+                    if (rows_with(c) > cap) { ok = false; break; }
+                    add_row(c);
+                    for (int k = p.nbr_ptr[c]; k < p.nbr_ptr[c + 1]; ++k) add_row(p.nbr_idx[k]);
+                    ++c;
+                }
+                if (!ok) break;
+                // W'[m][k] in double: m = 4 f + q (FP64) or 4 q + f (FP32)
+                std::vector<double> W((size_t)16 * cap, 0.0);
+                for (int qq = 0; qq < 4; ++qq) {
+                    int cc = c0 + qq;
+                    tcell.push_back(cc < c ? cc : -1);
+                    if (cc >= c || !gok[cc]) continue;
+                    const double *r = nullptr;
+                    for (int f = 0; f < 4; ++f) {
+                        int m = fp32_ ? 4 * qq + f : 4 * f + qq;
+                        r = rec_h.data() + (size_t)cc * kS3D + f * 10 + 4;
+                        double sum = 0;
+                        for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) {
+                            double w = (r[0] * P3[k] + r[1] * P3[(size_t)nnz + k] + r[2] * P3[2 * (size_t)nnz + k]) * invLx;
+                            W[(size_t)m * cap + loc_of[p.nbr_idx[k]]] += w;
+                            sum += w;
+                        }
+                        W[(size_t)m * cap + loc_of[cc]] -= sum;
+                    }
+                }
+                for (int k = 0; k < cap; ++k) trows.push_back(k < (int)cur.size() ? cur[k] : cur[0]);
+                for (int st = 0; st < kGmKS; ++st)
+                    for (int l = 0; l < 64; ++l) tA.push_back(W[(size_t)(l & 15) * cap + 4 * st + (l >> 4)]);
+                for (int r : cur) loc_of[r] = -1;
+            }
+            // validate: every tile row names a cell, every cell is in one tile
+            gm_ok = ok;
+            if (ok) {
+                for (int r : trows)
+                    if (r < 0 || r >= nc) { err_ = "internal error: matrix-core tile row"; return false; }
+                gm_ntiles = (int)tcell.size() / 4;
+                int seen = 0;
+                for (int t2 = 0; t2 < gm_ntiles; ++t2)
+                    for (int qq = 0; qq < 4; ++qq) seen += tcell[t2 * 4 + qq] >= 0;
+                if (seen != nc) { err_ = "internal error: matrix-core tiles do not cover the cells"; return false; }
+                if (!upload(d_gm_rows, trows) || !upload(d_gm_cell, tcell)) return false;
+                if (fp32_) {
+                    std::vector<float> tf(tA.begin(), tA.end());
+                    if (!alloc(d_gm_A, tf.size() * sizeof(float)) ||
+                        !check(hipMemcpy(d_gm_A.p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice), "H2D"))
+                        return false;
+                } else if (!alloc(d_gm_A, tA.size() * sizeof(double)) ||
+                           !check(hipMemcpy(d_gm_A.p, tA.data(), tA.size() * sizeof(double), hipMemcpyHostToDevice),
+                                  "H2D")) {
+                    return false;
+                }
+            }
+        }
     }
     // ---- moments and the residual's face coefficients
     std::vector<std::vector<double>> FM(nf);  // per unique face, from its first side: 3 x nv
@@ -814,6 +1033,26 @@ void Solver::launch_rhs3(bool macro, const double *bval, double *rhs) {
 // the solver's stream.
 void Solver::launch_grad3() {
     int DB = nD * nB;
+    if (use_gm) {
+        // slice groups of 256 entries
+        int nb = fp32_ && DB % 2 == 0 ? 2 : 1, spg = 256 / (16 * nb);
+        int ngroups = (DB + 255) / 256;
+        int grid = (int)((((long)gm_ntiles * ngroups) + kXcd - 1) / kXcd) * kXcd;
+        // FP32 reads pairs of entries (8-byte loads, two products per K-step)
+        // where the row length allows whole pairs
+        if (fp32_ && DB % 2 == 0)
+            hipLaunchKernelGGL((k_grad3m<float, 2>), dim3(grid), dim3(64), 0, s_, reinterpret_cast<const float *>(d_e.p),
+                               reinterpret_cast<float *>(d_grad.p), d_gm_rows.p, d_gm_cell.p,
+                               reinterpret_cast<const float *>(d_gm_A.p), gm_ntiles, DB, spg);
+        else if (fp32_)
+            hipLaunchKernelGGL((k_grad3m<float, 1>), dim3(grid), dim3(64), 0, s_, reinterpret_cast<const float *>(d_e.p),
+                               reinterpret_cast<float *>(d_grad.p), d_gm_rows.p, d_gm_cell.p,
+                               reinterpret_cast<const float *>(d_gm_A.p), gm_ntiles, DB, spg);
+        else
+            hipLaunchKernelGGL((k_grad3m<double, 1>), dim3(grid), dim3(64), 0, s_, d_e.p, d_grad.p, d_gm_rows.p,
+                               d_gm_cell.p, reinterpret_cast<const double *>(d_gm_A.p), gm_ntiles, DB, spg);
+        return;
+    }
     int ngroups = (DB + g3_gs * kG3W - 1) / (g3_gs * kG3W);
     int grid = ((blocks_for((long)ngroups * nc, g3_wpb) + kXcd - 1) / kXcd) * kXcd;
 #define GBTE_G3(ST_, GS_, W_)                                                                                    \
@@ -918,8 +1157,11 @@ bool Solver::tune3d() {
         // serves them all; more slices, more loads in flight per wave)
         double best = 1e300;
         int pick = 4, pickw = 4;
+        bool want_g = opt.gradient_kernel != GBTE_GRAD_MATRIX, want_m = opt.gradient_kernel != GBTE_GRAD_GATHER;
+        if (!want_g && !gm_ok) { err_ = "the matrix-core gradient does not take this mesh's stencils"; return false; }
         for (int wpb : {4, 16})
             for (int gs : {1, 2, 4}) {
+                if (!want_g) continue;
                 g3_gs = gs;
                 g3_wpb = wpb;
                 launch_grad3();  // warm
@@ -935,10 +1177,31 @@ bool Solver::tune3d() {
                 tune_log_ += line;
                 if (per < best) { best = per; pick = gs; pickw = wpb; }
             }
+        double best_g = best;
+        if (gm_ok && want_m) {
+            use_gm = true;
+            launch_grad3();  // warm
+            (void)hipEventRecord(a, s_);
+            for (int r = 0; r < 3; ++r) launch_grad3();
+            (void)hipEventRecord(b, s_);
+            if (!check(hipEventSynchronize(b), "tune sync")) return false;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            double per = ms / 3.0;
+            snprintf(line, sizeof line, "k_grad3m (matrix cores, %d tiles): %.3f ms per launch\n", gm_ntiles, per);
+            tune_log_ += line;
+            use_gm = per < best;
+            if (use_gm) best = per;
+        }
         g3_gs = pick;
         g3_wpb = pickw;
-        snprintf(line, sizeof line, "k_grad3g chosen: %d slices per wave, %d waves per workgroup\n", pick, pickw);
+        if (use_gm)
+            snprintf(line, sizeof line, "gradient chosen: k_grad3m (matrix cores)\n");
+        else
+            snprintf(line, sizeof line, "gradient chosen: k_grad3g, %d slices per wave, %d waves per workgroup\n",
+                     pick, pickw);
         tune_log_ += line;
+        (void)best_g;
     }
     // sweeps: the wave-specialised kernel (when its schedule was built), the
     // one-workgroup-per-direction level-synchronous kernel (FP64) and the

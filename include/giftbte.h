@@ -133,9 +133,14 @@ typedef struct gbte_options {
                              keeps the fastest) or one kernel family:
                              GBTE_SWEEP_WAVESPEC, GBTE_SWEEP_LEVEL (FP64 only),
                              GBTE_SWEEP_CHUNKED; same results to rounding */
+    int gradient_kernel;  /* tetrahedral meshes: GBTE_GRAD_AUTO (default: the set-up tuner
+                             keeps the faster) or GBTE_GRAD_GATHER (stencil gathers on the
+                             vector units) or GBTE_GRAD_MATRIX (tiles of four cells on
+                             the matrix cores); same results to rounding */
 } gbte_options;
 
 enum { GBTE_SWEEP_AUTO = 0, GBTE_SWEEP_WAVESPEC = 1, GBTE_SWEEP_LEVEL = 2, GBTE_SWEEP_CHUNKED = 3 };
+enum { GBTE_GRAD_AUTO = 0, GBTE_GRAD_GATHER = 1, GBTE_GRAD_MATRIX = 2 };
 
 typedef struct gbte_stats {
     int iterations;       /* outer iterations run                               */

@@ -27,19 +27,22 @@ def _keff_x(p, q, dT=2.0):
 # every sweep kernel family the set-up tuner chooses from, forced, on the
 # 8-band deck (the level-synchronous one-workgroup-per-direction kernel is
 # FP64 only)
-SWEEPS = [("cube3d_b8", "fp64", "wavespec"), ("cube3d_b8", "fp64", "level"), ("cube3d_b8", "fp64", "chunked"),
-          ("cube3d_b8", "fp32", "wavespec"), ("cube3d_b8", "fp32", "chunked")]
+SWEEPS = [("cube3d_b8", "fp64", "wavespec", "auto"), ("cube3d_b8", "fp64", "level", "auto"),
+          ("cube3d_b8", "fp64", "chunked", "auto"), ("cube3d_b8", "fp32", "wavespec", "auto"),
+          ("cube3d_b8", "fp32", "chunked", "auto")]
+# both gradient kernels, forced, on both decks and precisions
+GRADS = [(n, pr, "auto", g) for n in DECKS3D for pr in ("fp64", "fp32") for g in ("gather", "matrix")]
 
 
-@pytest.mark.parametrize("name,precision,sweep",
-                         [(n, pr, "auto") for n in DECKS3D for pr in ("fp64", "fp32")] + SWEEPS)
-def test_3d_matches_oracle(name, precision, sweep, deck_root):
+@pytest.mark.parametrize("name,precision,sweep,grad",
+                         [(n, pr, "auto", "auto") for n in DECKS3D for pr in ("fp64", "fp32")] + SWEEPS + GRADS)
+def test_3d_matches_oracle(name, precision, sweep, grad, deck_root):
     from giftbte_amd import Problem, BTESolver
     d = decklib.make(name, deck_root)
     s = decklib.settings(name)
-    print(f"\n[gpu 3d] deck={name} precision={precision} sweep={sweep} settings={s}", flush=True)
+    print(f"\n[gpu 3d] deck={name} precision={precision} sweep={sweep} gradient={grad} settings={s}", flush=True)
     p = Problem(d)
-    sol = BTESolver(p, precision=precision, sweep_kernel=sweep)
+    sol = BTESolver(p, precision=precision, sweep_kernel=sweep, gradient_kernel=grad)
     print("  " + sol.tuning_log().replace("\n", "; "), flush=True)
     fp = sol.fourier()
     st = sol.solve(s["max_iter"], s["tol_temp"], s["tol_flux"])
