@@ -279,14 +279,31 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
         }
         int t = s0 + NB * n;
         if (cell >= 0) {
+            bool bad[NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                if (t + j >= DB) continue;
-                bool bad = isnan(v[j][0]) || isnan(v[j][1]) || isnan(v[j][2]) || isnan(v[j][3]);
-                ST *pc = phi + (size_t)cell * 4 * DB + t + j;
+            for (int j = 0; j < NB; ++j) bad[j] = isnan(v[j][0]) || isnan(v[j][1]) || isnan(v[j][2]) || isnan(v[j][3]);
+            ST *pc = phi + (size_t)cell * 4 * DB + t;
+            bool pair = false;
+            if constexpr (NB == 2) {
+                // both entries of the pair in one store per face
+                pair = t + 1 < DB;
+                if (pair)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) __builtin_nontemporal_store((ST)(bad ? 0.0 : v[j][i]), pc + i * DB);
+                    for (int i = 0; i < 4; ++i) {
+                        BV o;
+                        o[0] = (ST)(bad[0] ? 0.0 : v[0][i]);
+                        o[1] = (ST)(bad[1] ? 0.0 : v[1][i]);
+                        __builtin_nontemporal_store(o, reinterpret_cast<BV *>(pc + i * DB));
+                    }
             }
+            if (!pair)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    if (t + j >= DB) continue;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        __builtin_nontemporal_store((ST)(bad[j] ? 0.0 : v[j][i]), pc + j + i * DB);
+                }
         }
     }
 }
