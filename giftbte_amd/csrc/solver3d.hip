@@ -386,6 +386,87 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3(const int *__restric
     }
 }
 
+// FP32 storage, an even band count: the same source with two bands of one
+// direction per lane, so every face value is one 8-byte load of a pair and
+// both sources go out in one 8-byte store; with one entry per lane the float
+// kernel issues as many memory instructions as the FP64 one for half the
+// bytes.  Each entry's arithmetic is k_source3's.  A face whose two entries
+// take different rows (one band of zero group velocity, v . n = 0 counting as
+// outgoing) loads the other row too.
+__global__ void __launch_bounds__(64 * kSrc3Rows) k_source3_f32x2(
+    const int *__restrict__ posc, float *__restrict__ S, const float *__restrict__ phi, const float *__restrict__ eb,
+    const double *__restrict__ Told, const double *__restrict__ Q, const double *__restrict__ rec,
+    const int *__restrict__ reci, Modes m, int nc, int nB, int nD, int sh) {
+    extern __shared__ double lds[];
+    double *mvg = lds, *mtau = mvg + nB, *mcap = mtau + nB, *mhr = mcap + nB, *msx = mhr + nB, *msy = msx + nD,
+           *msz = msy + nD;
+    int *pos = reinterpret_cast<int *>(msz + nD);
+    int row0 = xcd_block() * kSrc3Rows;
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+        mvg[i] = m.vg[i];
+        mtau[i] = m.tau[i];
+        mcap[i] = m.cap[i];
+        mhr[i] = m.hr[i];
+    }
+    for (int i = threadIdx.x; i < nD; i += blockDim.x) {
+        msx[i] = m.sx[i];
+        msy[i] = m.sy[i];
+        msz[i] = m.sz[i];
+    }
+    for (int i = threadIdx.x; i < kSrc3Rows * nD; i += blockDim.x) {
+        int r = row0 + i / nD;
+        pos[i] = r < nc ? posc[(size_t)r * nD + i % nD] : 0;
+    }
+    __syncthreads();
+    int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int c = row0 + w, lane = threadIdx.x & 63;
+    if (c >= nc) return;
+    int DB = nD * nB;
+    const double *rd = rec + (size_t)c * kS3D;
+    double fnx[4], fny[4], fnz[4], fav[4];
+    const float *fo[4], *fin[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const double *r = rd + f * 10;
+        fnx[f] = r[0]; fny[f] = r[1]; fnz[f] = r[2]; fav[f] = r[3];
+        int q = reci[(size_t)c * kS3I + f];
+        fo[f] = phi + ((size_t)c * 4 + f) * DB;
+        fin[f] = q >= 0 ? phi + (size_t)q * DB : eb + (size_t)(-q - 1) * DB;
+    }
+    double To = Told[c] - m.eref, Qc = Q[c];
+    const int *pc = pos + w * nD;
+    for (int t = 2 * lane; t < DB; t += 128) {
+        int d = t / nB, b = t - d * nB;
+        double vb0 = mvg[b], vb1 = mvg[b + 1], sx = msx[d], sy = msy[d], sz = msz[d];
+        double dot0[4], dot1[4];
+        float2 u[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            dot0[f] = vb0 * sx * fnx[f] + vb0 * sy * fny[f] + vb0 * sz * fnz[f];
+            dot1[f] = vb1 * sx * fnx[f] + vb1 * sy * fny[f] + vb1 * sz * fnz[f];
+            u[f] = *reinterpret_cast<const float2 *>((dot0[f] >= 0 && dot1[f] >= 0 ? fo[f] : fin[f]) + t);
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+            if ((dot0[f] >= 0) != (dot1[f] >= 0)) {
+                float2 uo = *reinterpret_cast<const float2 *>(fo[f] + t);
+                if (dot0[f] >= 0) u[f].x = uo.x;
+                else u[f].y = uo.y;
+            }
+        double s0 = To * mcap[b] + Qc * mhr[b] * mtau[b];
+        double s1 = To * mcap[b + 1] + Qc * mhr[b + 1] * mtau[b + 1];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            s0 -= mtau[b] * fav[f] * dot0[f] * (double)u[f].x;
+            s1 -= mtau[b + 1] * fav[f] * dot1[f] * (double)u[f].y;
+        }
+        size_t si = sh < 0 ? ((size_t)d * nc + pc[d]) * nB + b
+                           : ((((size_t)d * (nB >> sh) + (b >> sh)) * nc + pc[d]) << sh) + (b & ((1 << sh) - 1));
+        v2f o = {(float)s0, (float)s1};
+        __builtin_nontemporal_store(o, reinterpret_cast<v2f *>(S + si));
+    }
+}
+
 // 16-byte vector moves of N values of type T (N * sizeof(T) a multiple of 16,
 // the address 16-byte aligned)
 template <typename T, int N>
@@ -429,7 +510,7 @@ __device__ __forceinline__ void vst(T *p, const T (&v)[N]) {
 template <typename ST, int BC, int NP>
 __global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec, const int *__restrict__ levptr,
                                                   const int *__restrict__ dlev, const ST *__restrict__ S,
-                                                  ST *__restrict__ e, Modes m, int nc, int nB, int nD) {
+                                                  ST *__restrict__ e, Modes m, int nc, int nB, int nD, int smajor) {
     extern __shared__ int slp[];  // level pointers of the direction
     int nchunk = nB / BC;
     int id = xcd_block();
@@ -442,7 +523,9 @@ __global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec,
     for (int i = threadIdx.x; i <= L; i += 256) slp[i] = levptr[dlev[d] + i];
     __syncthreads();
     const LvRec *rd = rec + (size_t)d * nc;
-    const ST *Sd = S + ((size_t)d * nchunk + chunk) * nc * BC;
+    // S[dir][chunk][position][BC], or S[dir][position][band] (smajor)
+    const ST *Sd = smajor ? S + (size_t)d * nc * nB + b0 : S + ((size_t)d * nchunk + chunk) * nc * BC;
+    size_t sst = smajor ? (size_t)nB : (size_t)BC;
     size_t DB = (size_t)nD * nB;
     ST *ed = e + (size_t)d * nB + b0;
     int tid = threadIdx.x;
@@ -453,7 +536,7 @@ __global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec,
             int p = a0 + tid + i * 256;
             if (p < a1) {
                 rr[i] = rd[p];
-                vld(ss[i], Sd + (size_t)p * BC);
+                vld(ss[i], Sd + (size_t)p * sst);
             }
         }
     };
@@ -489,7 +572,7 @@ __global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec,
         for (int p = a0 + 256 * NP + tid; p < a1; p += 256) {
             LvRec r = rd[p];
             ST s1[BC], u1[3][BC];
-            vld(s1, Sd + (size_t)p * BC);
+            vld(s1, Sd + (size_t)p * sst);
             upwind(r, u1);
             solve(r, s1, u1);
         }
@@ -1097,7 +1180,11 @@ void Solver::launch_grad3() {
 void Solver::launch_source3(const Modes &m) {
     int grid = ((blocks_for(nc, kSrc3Rows) + kXcd - 1) / kXcd) * kXcd;
     size_t lds = (size_t)(4 * nB + 3 * nD) * sizeof(double) + (size_t)kSrc3Rows * nD * sizeof(int);
-    if (fp32_)
+    if (fp32_ && nB % 2 == 0)
+        hipLaunchKernelGGL(k_source3_f32x2, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
+                           reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
+                           reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
+    else if (fp32_)
         hipLaunchKernelGGL(k_source3<float>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
                            reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
                            reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
@@ -1113,7 +1200,8 @@ void Solver::launch_sweep_lv(const Modes &m, int bc) {
     size_t lds = (size_t)(nlev + 1) * sizeof(int);
 #define GBTE_LV(ST_, BC_, NP_)                                                                                      \
     hipLaunchKernelGGL((k_sweep_lv<ST_, BC_, NP_>), dim3(grid), dim3(256), lds, s_, d_lv_rec.p, d_levptr.p, d_dlev.p, \
-                       reinterpret_cast<const ST_ *>(d_S.p), reinterpret_cast<ST_ *>(d_e.p), m, nc, nB, nD)
+                       reinterpret_cast<const ST_ *>(d_S.p), reinterpret_cast<ST_ *>(d_e.p), m, nc, nB, nD,  \
+                       s_sh < 0 ? 1 : 0)
 #define GBTE_LV_NP(ST_, BC_)                  \
     do {                                      \
         if (lv_np == 4) GBTE_LV(ST_, BC_, 4); \
@@ -1276,6 +1364,34 @@ bool Solver::tune3d() {
         else
             snprintf(line, sizeof line, "sweep chosen: %s\n", use_pc ? "wave-specialised" : "level-synchronous");
         tune_log_ += line;
+        if (pick.kind == 2) {
+            // the layout of S between the source kernel and the chunked sweep:
+            // per band chunk (the sweep's loads contiguous) or per position
+            // with all bands (the source's stores contiguous); the pair is timed
+            int shc = s_sh;
+            double tl[2] = {0, 0};
+            for (int v = 0; v < 2; ++v) {
+                s_sh = v == 0 ? shc : -1;
+                launch_source3(m);
+                launch_sweep3(m);  // warm
+                (void)hipEventRecord(a, s_);
+                for (int r = 0; r < 3; ++r) {
+                    launch_source3(m);
+                    launch_sweep3(m);
+                }
+                (void)hipEventRecord(b, s_);
+                if (!check(hipEventSynchronize(b), "tune sync")) return false;
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, a, b);
+                tl[v] = ms / 3.0;
+                snprintf(line, sizeof line, "source + sweep, S %s: %.3f ms per pair\n",
+                         v == 0 ? "per band chunk" : "per position", tl[v]);
+                tune_log_ += line;
+            }
+            s_sh = tl[0] <= tl[1] ? shc : -1;
+            snprintf(line, sizeof line, "S layout chosen: %s\n", s_sh < 0 ? "per position" : "per band chunk");
+            tune_log_ += line;
+        }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
