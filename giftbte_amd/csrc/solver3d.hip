@@ -1168,7 +1168,8 @@ void Solver::launch_grad3() {
     do {                                       \
         if (g3_gs == 1) GBTE_G3_W(ST_, 1);     \
         else if (g3_gs == 2) GBTE_G3_W(ST_, 2); \
-        else GBTE_G3_W(ST_, 4);                \
+        else if (g3_gs == 4) GBTE_G3_W(ST_, 4); \
+        else GBTE_G3_W(ST_, 8);                \
     } while (0)
     if (fp32_) GBTE_G3_GS(float);
     else GBTE_G3_GS(double);
@@ -1265,8 +1266,8 @@ bool Solver::tune3d() {
         bool want_g = opt.gradient_kernel != GBTE_GRAD_MATRIX, want_m = opt.gradient_kernel != GBTE_GRAD_GATHER;
         if (!want_g && !gm_ok) { err_ = "the matrix-core gradient does not take this mesh's stencils"; return false; }
         for (int wpb : {4, 16})
-            for (int gs : {1, 2, 4}) {
-                if (!want_g) continue;
+            for (int gs : {1, 2, 4, 8}) {
+                if (!want_g || (gs == 8 && !fp32_)) continue;  // FP64 is bound by the gather at 2 slices
                 g3_gs = gs;
                 g3_wpb = wpb;
                 launch_grad3();  // warm
