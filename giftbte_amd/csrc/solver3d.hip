@@ -195,12 +195,14 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
     if (s_begin >= DB) return;  // whole wave
     int lane = threadIdx.x, q = lane >> 4, n = lane & 15;
     ST a[kGmKS];
-    int row[kGmKS];
 #pragma unroll
-    for (int s = 0; s < kGmKS; ++s) {
-        a[s] = tA[((size_t)tile * kGmKS + s) * 64 + lane];
-        row[s] = trows[((size_t)tile * kGmKS + s) * 4 + q];
-    }
+    for (int s = 0; s < kGmKS; ++s) a[s] = tA[((size_t)tile * kGmKS + s) * 64 + lane];
+    // the tile's rows in LDS (registers go to the operands): K-step s, lane
+    // group q reads srow[4 s + q]
+    __shared__ int srow[kGmKS * 4];
+    for (int i = lane; i < kGmKS * 4; i += 64) srow[i] = trows[(size_t)tile * kGmKS * 4 + i];
+    __syncthreads();
+    auto row = [&](int s) { return srow[4 * s + q]; };
     int cell = tcell[tile * 4 + q];
     int ref = trows[(size_t)tile * kGmKS * 4];
     // lane's first entry of a slice, clamped to a whole vector inside the row
@@ -215,9 +217,12 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
         int t0 = ent0(s_begin);
         rn = ld(ref, t0);
 #pragma unroll
-        for (int s = 0; s < kGmKS; ++s) b[s] = ld(row[s], t0);
+        for (int s = 0; s < kGmKS; ++s) b[s] = ld(row(s), t0);
     }
     for (int s0 = s_begin; s0 < s_end; s0 += W) {
+        // the row indices are read from LDS again for every slice: kept in
+        // registers (or as 64-bit addresses) they would halve the waves per SIMD
+        asm volatile("" ::: "memory");
         int tn = ent0(s0 + W < s_end ? s0 + W : s0);  // the next slice (the last one again at the end)
         if constexpr (!kRoll) {
             // FP64: the registers of a second set of B operands would halve
@@ -227,11 +232,7 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
                 rn = ld(ref, tc);
 #pragma unroll
                 for (int s = 0; s < kGmKS; ++s) {
-                    // an empty asm keeps the 64-bit row offsets from being
-                    // hoisted out of the slice loop (80 registers)
-                    int r = row[s];
-                    asm volatile("" : "+v"(r));
-                    b[s] = ld(r, tc);
+                    b[s] = ld(row(s), tc);
                 }
             }
         }
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
 #pragma unroll
             for (int s = 0; s < kGmKS; ++s) {
                 BV d = b[s] - rc;
-                if constexpr (kRoll) b[s] = ld(row[s], tn);
+                if constexpr (kRoll) b[s] = ld(row(s), tn);
                 if constexpr (NB == 2) {
                     acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], d[0], acc[0], 0, 0, 0);
                     acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], d[1], acc[1], 0, 0, 0);
@@ -264,7 +265,7 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
 #pragma unroll
             for (int s = 0; s < kGmKS; ++s) {
                 BV d = b[s] - rc;
-                if constexpr (kRoll) b[s] = ld(row[s], tn);
+                if constexpr (kRoll) b[s] = ld(row(s), tn);
                 if constexpr (NB == 2) {
                     acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], d[0], acc[0], 0, 0, 0);
                     acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], d[1], acc[1], 0, 0, 0);
