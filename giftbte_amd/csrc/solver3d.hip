@@ -1210,14 +1210,23 @@ void Solver::launch_sweep_lv(const Modes &m, int bc) {
         else if (lv_np == 2) GBTE_LV(ST_, BC_, 2); \
         else GBTE_LV(ST_, BC_, 1);            \
     } while (0)
+#define GBTE_LV_NP2(ST_, BC_)                      \
+    do {                                           \
+        if (lv_np == 2) GBTE_LV(ST_, BC_, 2);      \
+        else GBTE_LV(ST_, BC_, 1);                 \
+    } while (0)
     if (fp32_) {
         if (bc == 4) GBTE_LV_NP(float, 4);
-        else GBTE_LV_NP(float, 8);
+        else if (bc == 8) GBTE_LV_NP(float, 8);
+        else GBTE_LV_NP2(float, 20);
     } else {
         if (bc == 2) GBTE_LV_NP(double, 2);
         else if (bc == 4) GBTE_LV_NP(double, 4);
-        else GBTE_LV_NP(double, 8);
+        else if (bc == 8) GBTE_LV_NP(double, 8);
+        else if (bc == 10) GBTE_LV_NP2(double, 10);
+        else GBTE_LV_NP2(double, 20);
     }
+#undef GBTE_LV_NP2
 #undef GBTE_LV_NP
 #undef GBTE_LV
 }
@@ -1325,18 +1334,26 @@ bool Solver::tune3d() {
         if (pc_built && (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_WAVESPEC)) cands.push_back({0, 0, 0});
         if (!fp32_ && (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_LEVEL)) cands.push_back({1, 0, 0});
         int wsz = fp32_ ? 4 : 8;
-        for (int bc : {2, 4, 8})
+        // band chunks of whole 16-byte vectors (the non-power-of-two ones with
+        // S stored per position), up to 160 bytes of each operand per thread
+        for (int bc : {2, 4, 8, 10, 20})
             if (d_lv_rec.p && nB % bc == 0 && (bc * wsz) % 16 == 0 && (nB * wsz) % 16 == 0 &&
                 (size_t)(nlev + 1) * sizeof(int) <= 48 * 1024 &&
                 (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_CHUNKED))
                 for (int np : {1, 2, 4})
-                    if (bc * wsz * np <= 128) cands.push_back({2, bc, np});
+                    if (bc * wsz * np <= 160 && (np <= 2 || (bc & (bc - 1)) == 0)) cands.push_back({2, bc, np});
         double best = 1e300;
         Cand pick{-1, 0, 0};
+        auto chunk_sh = [](int bc) {  // S per band chunk where the chunk is a power of two
+            for (int k = 1; k <= 3; ++k)
+                if ((1 << k) == bc) return k;
+            return -1;
+        };
         for (const Cand &c : cands) {
             use_pc = c.kind == 0;
             lv_bc = c.kind == 2 ? c.bc : 0;
             lv_np = c.kind == 2 ? c.np : 2;
+            s_sh = c.kind == 2 ? chunk_sh(c.bc) : -1;
             launch_sweep3(m);  // warm
             (void)hipEventRecord(a, s_);
             for (int r = 0; r < 3; ++r) launch_sweep3(m);
@@ -1358,9 +1375,7 @@ bool Solver::tune3d() {
         use_pc = pick.kind == 0;
         lv_bc = pick.kind == 2 ? pick.bc : 0;
         lv_np = pick.kind == 2 ? pick.np : 2;
-        s_sh = -1;
-        for (int k = 1; lv_bc > 0 && k <= 3; ++k)
-            if ((1 << k) == lv_bc) s_sh = k;
+        s_sh = pick.kind == 2 ? chunk_sh(pick.bc) : -1;
         if (pick.kind == 2)
             snprintf(line, sizeof line, "sweep chosen: chunked %d bands, %d positions per thread\n", pick.bc, pick.np);
         else
