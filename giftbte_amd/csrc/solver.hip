@@ -32,6 +32,7 @@
 #include "tiles.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1763,6 +1764,22 @@ __global__ void k_fourier_flux(const double *__restrict__ T, const int *__restri
 }
 
 
+// Independent host work items 0 .. n-1 on up to 16 host threads (the set-up
+// of per-direction sweep schedules and of 3D tables); fn must not throw.
+template <typename F>
+void host_parallel(int n, F fn) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int nt = std::max(1, std::min(n, (int)std::min(16u, hw ? hw : 1u)));
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&]() {
+            for (int i; (i = next.fetch_add(1)) < n;) fn(i);
+        });
+    for (auto &x : th) x.join();
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1922,6 +1939,17 @@ bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>>
 
 bool Solver::build() {
     const Problem &p = P;
+    // host time of each set-up phase, reported with the tuner's record
+    auto tp = std::chrono::steady_clock::now();
+    std::string phases;
+    auto mark = [&](const char *what) {
+        auto t = std::chrono::steady_clock::now();
+        char b[96];
+        snprintf(b, sizeof b, "%s%s %.2f s", phases.empty() ? "" : ", ", what,
+                 std::chrono::duration<double>(t - tp).count());
+        phases += b;
+        tp = t;
+    };
     nc = p.n_cells; nb = p.n_bounds; ne = 2 * nb; nn = p.n_nodes; nB = p.n_bands; nD = p.n_dirs; nf = p.n_faces;
     dim = p.dim;
     if (p.n_matters != 1) { err_ = "only single-material problems are supported"; return false; }
@@ -2060,6 +2088,7 @@ bool Solver::build() {
         !upload(d_sarea, p.face_area) || !upload(d_srx, srx) || !upload(d_sry, sry) || !upload(d_srz, srz) ||
         !upload(d_sqx, sqx) || !upload(d_sqy, sqy) || !upload(d_sqz, sqz))
         return false;
+    mark("mode, geometry and boundary tables");
     // ---- per-direction upwind order.  Band 0 decides the upwind graph; the
     // band signs agree except for faces parallel to a direction to rounding,
     // whose coupling coefficient vanishes.
@@ -2067,19 +2096,23 @@ bool Solver::build() {
         size_t NI = (size_t)nD * nc;
         std::vector<int> item_cell(NI), posc(NI), levptr, dlev(nD + 1, 0), rec_q(NI * kMaxIn, -1), rec_c(NI * kMaxIn, -1);
         std::vector<double> rec_g(NI * kMaxIn, 0.0), rec_G(NI, 0.0);
-        std::vector<int> level(nc), indeg(nc), q, pos(nc), cnt;
         double v0 = vg[0];
-        nlev = 0;
-        for (int d = 0; d < nD; ++d) {
+        // every direction's schedule is independent host work: one host
+        // thread per direction at a time; each writes its own slices, and
+        // the level pointers are concatenated afterwards
+        std::vector<std::vector<int>> lpd(nD);
+        std::vector<std::string> errd(nD);
+        std::vector<int> posT(NI);  // [dir][cell] position, transposed into posc below
+        host_parallel(nD, [&](int d) {
             auto dotf = [&](size_t s) { return v0 * sx[d] * p.face_nx[s] + v0 * sy[d] * p.face_ny[s] + v0 * sz[d] * p.face_nz[s]; };
-            std::fill(indeg.begin(), indeg.end(), 0);
-            std::fill(level.begin(), level.end(), 0);
+            std::vector<int> level(nc, 0), indeg(nc, 0), q, cnt;
+            int *pos = posT.data() + (size_t)d * nc;
+            q.reserve(nc);
             for (int c = 0; c < nc; ++c)
                 for (int j = 0; j < p.face_size[c]; ++j) {
                     size_t s = (size_t)c * 6 + j;
                     if (snb[s] >= 0 && dotf(s) < 0) indeg[c]++;
                 }
-            q.clear();
             for (int c = 0; c < nc; ++c)
                 if (indeg[c] == 0) { q.push_back(c); level[c] = 0; }
             size_t head = 0;
@@ -2095,21 +2128,16 @@ bool Solver::build() {
                 }
             }
             if ((int)q.size() != nc) {
-                err_ = "the upwind graph of direction " + std::to_string(d) + " has a cycle; an acyclic (e.g. Delaunay) mesh is required";
-                return false;
+                errd[d] = "the upwind graph of direction " + std::to_string(d) + " has a cycle; an acyclic (e.g. Delaunay) mesh is required";
+                return;
             }
             int L = 0;
             for (int c = 0; c < nc; ++c) L = std::max(L, level[c] + 1);
-            nlev = std::max(nlev, L);
             cnt.assign(L + 1, 0);
             for (int c = 0; c < nc; ++c) cnt[level[c] + 1]++;
             for (int l = 0; l < L; ++l) cnt[l + 1] += cnt[l];
-            dlev[d + 1] = dlev[d] + L + 1;
-            for (int l = 0; l <= L; ++l) levptr.push_back(cnt[l]);
-            for (int c = 0; c < nc; ++c) {
-                pos[c] = cnt[level[c]]++;
-                posc[(size_t)c * nD + d] = pos[c];
-            }
+            lpd[d].assign(cnt.begin(), cnt.end());
+            for (int c = 0; c < nc; ++c) pos[c] = cnt[level[c]]++;
             size_t base = (size_t)d * nc;
             for (int c = 0; c < nc; ++c) {
                 size_t P0 = base + pos[c];
@@ -2123,7 +2151,7 @@ bool Solver::build() {
                     if (dotf(s) >= 0) {
                         G += gf;
                     } else if (snb[s] >= 0) {
-                        if (nin >= kMaxIn) { err_ = "a cell has more in-faces than the sweep supports"; return false; }
+                        if (nin >= kMaxIn) { errd[d] = "a cell has more in-faces than the sweep supports"; return; }
                         rec_q[P0 * kMaxIn + nin] = pos[snb[s]];
                         rec_c[P0 * kMaxIn + nin] = snb[s];
                         rec_g[P0 * kMaxIn + nin] = gf;
@@ -2132,12 +2160,27 @@ bool Solver::build() {
                 }
                 rec_G[P0] = G;
             }
+        });
+        for (int d = 0; d < nD; ++d)
+            if (!errd[d].empty()) { err_ = errd[d]; return false; }
+        nlev = 0;
+        for (int d = 0; d < nD; ++d) {
+            int L = (int)lpd[d].size() - 1;
+            nlev = std::max(nlev, L);
+            dlev[d + 1] = dlev[d] + L + 1;
+            levptr.insert(levptr.end(), lpd[d].begin(), lpd[d].end());
         }
+        host_parallel((nc + 4095) / 4096, [&](int blk) {
+            for (int c = blk * 4096; c < std::min(nc, (blk + 1) * 4096); ++c)
+                for (int d = 0; d < nD; ++d) posc[(size_t)c * nD + d] = posT[(size_t)d * nc + c];
+        });
+        std::vector<int>().swap(posT);
         if (dim == 3 && !opt.fallback_kernels) {
             // records of the level-synchronous chunked sweep: every upwind
             // cell lies on an earlier level of its direction (checked)
             std::vector<LvRec> lrec(NI);
-            for (int d = 0; d < nD; ++d) {
+            std::vector<char> badd(nD, 0);
+            host_parallel(nD, [&](int d) {
                 const int *lp = levptr.data() + dlev[d];
                 int L = dlev[d + 1] - dlev[d] - 1;
                 for (int l = 0; l < L; ++l)
@@ -2150,10 +2193,12 @@ bool Solver::build() {
                             int qq = rec_q[P0 * kMaxIn + j];
                             r.c[j] = qq >= 0 ? rec_c[P0 * kMaxIn + j] : -1;
                             r.g[j] = qq >= 0 ? rec_g[P0 * kMaxIn + j] : 0.0;
-                            if (qq >= lp[l]) { err_ = "internal error: upwind cell on the same or a later level"; return false; }
+                            if (qq >= lp[l]) badd[d] = 1;
                         }
                     }
-            }
+            });
+            for (int d = 0; d < nD; ++d)
+                if (badd[d]) { err_ = "internal error: upwind cell on the same or a later level"; return false; }
             if (!upload(d_lv_rec, lrec)) return false;
         }
         // about one workgroup per CU: split the bands of a direction into
@@ -2178,10 +2223,14 @@ bool Solver::build() {
         if (!opt.fallback_kernels && pc_bc <= kPcBc && (kPcDepth + 1) * kPcCap <= pc_ring) {
             std::vector<PcRec> prec(NI);
             std::vector<int> psl, pfp, pdsl(1, 0), pfar;
-            std::vector<int> freq(nc, 0), touched;
             bool fits = true;
             int R = pc_ring;
-            for (int d = 0; d < nD && fits; ++d) {
+            // per direction (host threads): sub-level starts, far-entry
+            // offsets local to the direction, far cells; concatenated below
+            std::vector<std::vector<int>> psl_d(nD), pfp_d(nD), pfar_d(nD);
+            std::vector<std::string> errd2(nD);
+            host_parallel(nD, [&](int d) {
+                std::vector<int> freq(nc, 0), touched;
                 const int *lp = levptr.data() + dlev[d];
                 int L = dlev[d + 1] - dlev[d] - 1;
                 size_t base = (size_t)d * nc;
@@ -2217,9 +2266,10 @@ bool Solver::build() {
                     touched.clear();
                 }
                 int Ls = (int)sub.size() - 1;
+                std::vector<int> &sl = psl_d[d], &fp = pfp_d[d], &fc = pfar_d[d];
                 for (int m = 0; m < Ls; ++m) {
-                    psl.push_back(sub[m]);
-                    pfp.push_back((int)pfar.size());
+                    sl.push_back(sub[m]);
+                    fp.push_back((int)fc.size());
                     int a1 = sub[m + 1], nf = 0;
                     for (int k = sub[m]; k < a1; ++k) {
                         size_t P0 = base + k;
@@ -2233,46 +2283,59 @@ bool Solver::build() {
                             else if (qq >= a1 - R) r.q[j] = qq;
                             else {
                                 r.q[j] = -2 - nf++;
-                                pfar.push_back(rec_c[P0 * kMaxIn + j]);
+                                fc.push_back(rec_c[P0 * kMaxIn + j]);
                             }
                         }
                     }
-                    if (nf > pc_far) { err_ = "internal error: sweep sub-level far budget"; return false; }
+                    if (nf > pc_far) { errd2[d] = "internal error: sweep sub-level far budget"; return; }
                 }
-                psl.push_back(sub[Ls]);
-                pfp.push_back((int)pfar.size());
+                sl.push_back(sub[Ls]);
+                fp.push_back((int)fc.size());
+            });
+            for (int d = 0; d < nD; ++d)
+                if (!errd2[d].empty()) { err_ = errd2[d]; return false; }
+            for (int d = 0; d < nD; ++d) {
+                int off = (int)pfar.size();
+                psl.insert(psl.end(), psl_d[d].begin(), psl_d[d].end());
+                for (int f : pfp_d[d]) pfp.push_back(off + f);
+                pfar.insert(pfar.end(), pfar_d[d].begin(), pfar_d[d].end());
                 pdsl.push_back((int)psl.size());
+                std::vector<int>().swap(psl_d[d]);
+                std::vector<int>().swap(pfp_d[d]);
+                std::vector<int>().swap(pfar_d[d]);
             }
             // validate what the kernel assumes: sub-levels cover the
             // direction in order within kPcCap / kPcFar; a ring read is still
             // in the ring and finished; a far read is finished before the
             // producers read it, kPcDepth sub-levels ahead
-            for (int d = 0; d < nD && fits; ++d) {
+            host_parallel(nD, [&](int d) {
                 const int *sl = psl.data() + pdsl[d];
                 const int *fp = pfp.data() + pdsl[d];
                 int Ls = pdsl[d + 1] - pdsl[d] - 1;
-                if (sl[0] != 0 || sl[Ls] != nc) { err_ = "internal error: sweep sub-levels"; return false; }
+                if (sl[0] != 0 || sl[Ls] != nc) { errd2[d] = "internal error: sweep sub-levels"; return; }
                 for (int m2 = 0; m2 < Ls; ++m2) {
                     int a0 = sl[m2], a1 = sl[m2 + 1], nf = fp[m2 + 1] - fp[m2];
-                    if (a1 <= a0 || a1 - a0 > kPcCap || nf < 0 || nf > pc_far) { err_ = "internal error: sweep sub-level size"; return false; }
+                    if (a1 <= a0 || a1 - a0 > kPcCap || nf < 0 || nf > pc_far) { errd2[d] = "internal error: sweep sub-level size"; return; }
                     int ready = sl[std::max(0, m2 - kPcDepth)];
                     for (int k = a0; k < a1; ++k) {
                         const PcRec &r = prec[(size_t)d * nc + k];
                         for (int j = 0; j < kMaxIn; ++j) {
                             int qq = r.q[j];
-                            if (qq >= 0 && !(qq < a0 && qq >= a1 - pc_ring)) { err_ = "internal error: sweep ring read"; return false; }
+                            if (qq >= 0 && !(qq < a0 && qq >= a1 - pc_ring)) { errd2[d] = "internal error: sweep ring read"; return; }
                             if (qq <= -2) {
                                 int f = -2 - qq;
                                 int up = rec_q[((size_t)d * nc + k) * kMaxIn + j];
                                 if (f >= nf || pfar[fp[m2] + f] != rec_c[((size_t)d * nc + k) * kMaxIn + j] || !(up < ready)) {
-                                    err_ = "internal error: sweep far read";
-                                    return false;
+                                    errd2[d] = "internal error: sweep far read";
+                                    return;
                                 }
                             }
                         }
                     }
                 }
-            }
+            });
+            for (int d = 0; d < nD; ++d)
+                if (!errd2[d].empty()) { err_ = errd2[d]; return false; }
             pfar.push_back(0);  // padding read by clamped loads
             int maxsub0 = 0;
             for (int d = 0; d < nD; ++d) maxsub0 = std::max(maxsub0, pdsl[d + 1] - pdsl[d] - 1);
@@ -2302,6 +2365,7 @@ bool Solver::build() {
             }
         }
     }
+    mark("sweep schedules");
     // ---- LSQ gradient stencil (_get_CellMatrix_larger + Get_inverse_2D,
     // StaticBTESynthetic.cpp:1000-1064, utility.cpp:401-466); 3D: build3d
     if (dim == 2) {
@@ -2590,8 +2654,10 @@ bool Solver::build() {
         if (direct) th.join();
         if (!okM || !build_matrix(rowsF, Kfour, NDfour, direct ? &Ff : nullptr, direct ? &ef : nullptr)) return false;
     }
+    mark("gradient, macro and Fourier tables and factors");
     // ---- 3D tables (solver3d.hip)
     if (dim == 3 && !build3d(be_slot)) return false;
+    if (dim == 3) mark("3D tables");
     // ---- LDS tiles of the face / vertex / moment kernel (2D)
     if (dim == 2) {
         Tiles T;
@@ -2718,7 +2784,10 @@ bool Solver::build() {
                 if (p.face_bound[(size_t)c * 6 + j] == -1) nint[c]++;
         if (!upload(d_nint, nint)) return false;
     }
+    mark("tiles and state");
     if (dim == 3 && !tune3d()) return false;
+    if (dim == 3) mark("tuner");
+    tune_log_ = "set-up (host): " + phases + "\n" + tune_log_;
     return check(hipStreamSynchronize(s_), "build sync");
 }
 
