@@ -48,6 +48,7 @@
 #include <cmath>
 #include <cstdio>
 #include <limits>
+#include <thread>
 #include <type_traits>
 
 namespace gbte {
@@ -923,8 +924,9 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
         // ---- matrix-core tiles (k_grad3m): up to four consecutive cells while
         // the rows they need fit kGmKS K-steps
         {
-            std::vector<int> trows, tcell;
-            std::vector<double> tA;  // [tile][K-step][lane], FP64 values
+            // tile boundaries and rows (sequential, greedy), then the A'
+            // operands of each tile (host threads)
+            std::vector<int> trows, tcell, tnrow;
             std::vector<int> loc_of(nc, -1), cur;
             auto rows_with = [&](int cc) {
                 int add = loc_of[cc] < 0 ? 1 : 0;
@@ -947,33 +949,14 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                     ++c;
                 }
                 if (!ok) break;
-                // W'[m][k] in double: m = 4 f + q (FP64) or 4 q + f (FP32)
-                std::vector<double> W((size_t)16 * cap, 0.0);
-                for (int qq = 0; qq < 4; ++qq) {
-                    int cc = c0 + qq;
-                    tcell.push_back(cc < c ? cc : -1);
-                    if (cc >= c || !gok[cc]) continue;
-                    const double *r = nullptr;
-                    for (int f = 0; f < 4; ++f) {
-                        int m = fp32_ ? 4 * qq + f : 4 * f + qq;
-                        r = rec_h.data() + (size_t)cc * kS3D + f * 10 + 4;
-                        double sum = 0;
-                        for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) {
-                            double w = (r[0] * P3[k] + r[1] * P3[(size_t)nnz + k] + r[2] * P3[2 * (size_t)nnz + k]) * invLx;
-                            W[(size_t)m * cap + loc_of[p.nbr_idx[k]]] += w;
-                            sum += w;
-                        }
-                        W[(size_t)m * cap + loc_of[cc]] -= sum;
-                    }
-                }
+                for (int qq = 0; qq < 4; ++qq) tcell.push_back(c0 + qq < c ? c0 + qq : -1);
                 for (int k = 0; k < cap; ++k) trows.push_back(k < (int)cur.size() ? cur[k] : cur[0]);
-                for (int st = 0; st < kGmKS; ++st)
-                    for (int l = 0; l < 64; ++l) tA.push_back(W[(size_t)(l & 15) * cap + 4 * st + (l >> 4)]);
+                tnrow.push_back((int)cur.size());
                 for (int r : cur) loc_of[r] = -1;
             }
-            // validate: every tile row names a cell, every cell is in one tile
             gm_ok = ok;
             if (ok) {
+                // validate: every tile row names a cell, every cell is in one tile
                 for (int r : trows)
                     if (r < 0 || r >= nc) { err_ = "internal error: matrix-core tile row"; return false; }
                 gm_ntiles = (int)tcell.size() / 4;
@@ -981,17 +964,53 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 for (int t2 = 0; t2 < gm_ntiles; ++t2)
                     for (int qq = 0; qq < 4; ++qq) seen += tcell[t2 * 4 + qq] >= 0;
                 if (seen != nc) { err_ = "internal error: matrix-core tiles do not cover the cells"; return false; }
+                // A'[m][k] in double, m = 4 f + q (FP64) or 4 q + f (FP32),
+                // stored [tile][K-step][lane] in the storage type
+                size_t per = (size_t)kGmKS * 64;
+                std::vector<double> tA64(fp32_ ? 0 : (size_t)gm_ntiles * per);
+                std::vector<float> tA32(fp32_ ? (size_t)gm_ntiles * per : 0);
+                int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+                std::vector<std::thread> th;
+                for (int w = 0; w < nth; ++w)
+                    th.emplace_back([&, w]() {
+                        std::vector<int> loc(nc, -1);
+                        std::vector<double> W((size_t)16 * cap);
+                        for (int t2 = w; t2 < gm_ntiles; t2 += nth) {
+                            const int *rw = trows.data() + (size_t)t2 * cap;
+                            for (int k = 0; k < tnrow[t2]; ++k) loc[rw[k]] = k;
+                            std::fill(W.begin(), W.end(), 0.0);
+                            for (int qq = 0; qq < 4; ++qq) {
+                                int cc = tcell[t2 * 4 + qq];
+                                if (cc < 0 || !gok[cc]) continue;
+                                for (int f = 0; f < 4; ++f) {
+                                    int m = fp32_ ? 4 * qq + f : 4 * f + qq;
+                                    const double *r = rec_h.data() + (size_t)cc * kS3D + f * 10 + 4;
+                                    double sum = 0;
+                                    for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) {
+                                        double wk = (r[0] * P3[k] + r[1] * P3[(size_t)nnz + k] +
+                                                     r[2] * P3[2 * (size_t)nnz + k]) * invLx;
+                                        W[(size_t)m * cap + loc[p.nbr_idx[k]]] += wk;
+                                        sum += wk;
+                                    }
+                                    W[(size_t)m * cap + loc[cc]] -= sum;
+                                }
+                            }
+                            for (int k = 0; k < tnrow[t2]; ++k) loc[rw[k]] = -1;
+                            size_t o = (size_t)t2 * per;
+                            for (int st = 0; st < kGmKS; ++st)
+                                for (int l = 0; l < 64; ++l) {
+                                    double v = W[(size_t)(l & 15) * cap + 4 * st + (l >> 4)];
+                                    if (fp32_) tA32[o + st * 64 + l] = (float)v;
+                                    else tA64[o + st * 64 + l] = v;
+                                }
+                        }
+                    });
+                for (auto &x : th) x.join();
                 if (!upload(d_gm_rows, trows) || !upload(d_gm_cell, tcell)) return false;
-                if (fp32_) {
-                    std::vector<float> tf(tA.begin(), tA.end());
-                    if (!alloc(d_gm_A, tf.size() * sizeof(float)) ||
-                        !check(hipMemcpy(d_gm_A.p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice), "H2D"))
-                        return false;
-                } else if (!alloc(d_gm_A, tA.size() * sizeof(double)) ||
-                           !check(hipMemcpy(d_gm_A.p, tA.data(), tA.size() * sizeof(double), hipMemcpyHostToDevice),
-                                  "H2D")) {
+                size_t bytes = fp32_ ? tA32.size() * sizeof(float) : tA64.size() * sizeof(double);
+                const void *src = fp32_ ? (const void *)tA32.data() : (const void *)tA64.data();
+                if (!alloc(d_gm_A, bytes) || !check(hipMemcpy(d_gm_A.p, src, bytes, hipMemcpyHostToDevice), "H2D"))
                     return false;
-                }
             }
         }
     }
