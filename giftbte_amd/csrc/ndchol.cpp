@@ -32,12 +32,16 @@ constexpr int kPanel = 32;  // pivot columns per trailing update of a front
 
 // body(j) for j in [lo, hi) on the host's cores when the work is large
 // (columns are independent); the split balances sum (len - j).
+// set on the host threads that factorize whole subtrees, which then run
+// their fronts' columns serially
+thread_local bool t_serial_cols = false;
+
 template <typename Body>
 void parallel_cols(int lo, int hi, int len, Body body) {
     long work = 0;
     for (int j = lo; j < hi; ++j) work += len - j;
     unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    int nth = (int)std::min<long>(hw, std::max<long>(1, work / 200000));
+    int nth = t_serial_cols ? 1 : (int)std::min<long>(hw, std::max<long>(1, work / 200000));
     if (nth <= 1 || hi - lo < 2 * nth) {
         for (int j = lo; j < hi; ++j) body(j);
         return;
@@ -182,11 +186,25 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
     F.rows.resize(F.rows_off[nt]);
     F.frm.assign(F.f_off[nt], 0.0);
     F.fcm.assign(F.f_off[nt], 0.0);
-    std::vector<double> Li;
+    // where each child's map of its update rows into its parent's front
+    // goes: the children of every node in postorder, in child order
     std::vector<int> map_of_node(nt, -1);
-    for (int k = 0; k < nt; ++k) {
+    {
+        int run = 0;
+        for (int k = 0; k < nt; ++k)
+            for (int q = F.child_ptr[k]; q < F.child_ptr[k + 1]; ++q) {
+                map_of_node[F.child_idx[q]] = run;
+                run += F.nrow[F.child_idx[q]];
+            }
+        F.map.assign(run, 0);
+    }
+    for (int k = 0; k < nt; ++k) F.max_front = std::max(F.max_front, F.ncol[k] + F.nrow[k]);
+    // one supernode: its front (the rows of A and the children's update
+    // matrices), partial Cholesky, F_k and its own update matrix.  A node
+    // needs only its children, so disjoint subtrees are independent.
+    auto factor_node = [&](int k) -> std::string {
+        std::vector<double> Li;
         int nc_ = F.ncol[k], nr = F.nrow[k], m = nc_ + nr, c0 = F.col0[k], hi = c0 + nc_;
-        F.max_front = std::max(F.max_front, m);
         std::copy(rows[k].begin(), rows[k].end(), F.rows.begin() + F.rows_off[k]);
         auto loc = [&](int pp) -> int {
             if (pp < hi) return pp - c0;
@@ -204,8 +222,7 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
             int nrc = F.nrow[c];
             std::vector<int> mp(nrc);
             for (int i = 0; i < nrc; ++i) mp[i] = loc(rows[c][i]);
-            map_of_node[c] = (int)F.map.size();
-            F.map.insert(F.map.end(), mp.begin(), mp.end());
+            std::copy(mp.begin(), mp.end(), F.map.begin() + map_of_node[c]);
             const std::vector<double> &Uc = U[c];
             for (int i = 0; i < nrc; ++i)
                 for (int j = 0; j <= i; ++j) {
@@ -284,6 +301,49 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
             const double *cj = &C[(size_t)(nc_ + j) * m + nc_];
             for (int i = j; i < nr; ++i) U[k][(size_t)i * nr + j] = cj[i];
         }
+        return std::string();
+    };
+    // the subtrees below the first depth with at least 64 nodes on host
+    // threads (each subtree in postorder, columns serially), then the nodes
+    // above them in postorder with the columns of each front in parallel;
+    // the arithmetic of every node is the same either way
+    {
+        std::vector<int> size(nt, 1);
+        for (int k = 0; k < nt; ++k)
+            for (int q = F.child_ptr[k]; q < F.child_ptr[k + 1]; ++q) size[k] += size[F.child_idx[q]];
+        int maxd = 0;
+        for (int k = 0; k < nt; ++k) maxd = std::max(maxd, F.depth[k]);
+        int D = maxd + 1;
+        for (int dd = 0; dd <= maxd; ++dd) {
+            int cntd = 0;
+            for (int k = 0; k < nt; ++k) cntd += F.depth[k] == dd;
+            if (cntd >= 64) { D = dd; break; }
+        }
+        std::vector<int> roots;
+        for (int k = 0; k < nt; ++k)
+            if (F.depth[k] == D) roots.push_back(k);
+        std::vector<std::string> rerr(roots.size());
+        unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        int nth = (int)std::min<size_t>(std::min(16u, hw), roots.size());
+        if (nth > 0) {
+            std::vector<std::thread> th;
+            for (int w = 0; w < nth; ++w)
+                th.emplace_back([&, w]() {
+                    t_serial_cols = true;
+                    for (size_t i = w; i < roots.size(); i += nth) {
+                        int r = roots[i];
+                        for (int k = r - size[r] + 1; k <= r && rerr[i].empty(); ++k) rerr[i] = factor_node(k);
+                    }
+                });
+            for (auto &x : th) x.join();
+        }
+        for (auto &e : rerr)
+            if (!e.empty()) return e;
+        for (int k = 0; k < nt; ++k)
+            if (F.depth[k] < D) {
+                std::string e = factor_node(k);
+                if (!e.empty()) return e;
+            }
     }
     // map offsets per node (as child of its parent)
     for (int k = 0; k < nt; ++k) F.map_off[k] = map_of_node[k] < 0 ? 0 : map_of_node[k];

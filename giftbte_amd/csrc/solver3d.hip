@@ -1278,15 +1278,40 @@ void Solver::launch_moments3(const Modes &m) {
                        m, d_T.p, d_qx.p, d_qy.p, d_qz.p, d_RM.p, nc);
 }
 
-// Project autotuner for the 3D gradient: times each candidate schedule of
-// k_grad3 (LDS tiles at the built budgets, or the direct-gather form) over a
-// few launches on this mesh and keeps the fastest.  The choice and the
-// timings are kept in tune_log_ (gbte_tuning).
+// Project autotuner for tetrahedral meshes: times each candidate shape of
+// the gradient kernels (k_grad3g slices per wave and waves per workgroup,
+// k_grad3m), of the sweeps (wave-specialised, level-synchronous, chunked at
+// each band chunk and positions per thread) and the layout of S between the
+// source and the chunked sweep, on this mesh, and keeps the fastest.  The
+// candidates, timings and choices are kept in tune_log_ (gbte_tuning_log).
 bool Solver::tune3d() {
     if (!check(hipMemsetAsync(d_e.p, 0, d_e.n * sizeof(double), s_), "hipMemsetAsync")) return false;
     hipEvent_t a, b;
     if (!check(hipEventCreate(&a), "hipEventCreate") || !check(hipEventCreate(&b), "hipEventCreate")) return false;
     char line[160];
+    // one candidate: a warm launch and a timed one, then two more timed
+    // launches only when the first is within 30% of the best so far (a
+    // clearly slower candidate costs two launches instead of four)
+    auto time_it = [&](auto launch, double best, double &per) -> bool {
+        launch();
+        (void)hipEventRecord(a, s_);
+        launch();
+        (void)hipEventRecord(b, s_);
+        if (!check(hipEventSynchronize(b), "tune sync")) return false;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        per = ms;
+        if (per <= 1.3 * best) {
+            (void)hipEventRecord(a, s_);
+            launch();
+            launch();
+            (void)hipEventRecord(b, s_);
+            if (!check(hipEventSynchronize(b), "tune sync")) return false;
+            (void)hipEventElapsedTime(&ms, a, b);
+            per = (per + ms) / 3.0;
+        }
+        return true;
+    };
     {
         // the gradient: 64-entry slices per wave (one stencil-table batch
         // serves them all; more slices, more loads in flight per wave)
@@ -1299,14 +1324,8 @@ bool Solver::tune3d() {
                 if (!want_g || (gs == 8 && !fp32_)) continue;  // FP64 is bound by the gather at 2 slices
                 g3_gs = gs;
                 g3_wpb = wpb;
-                launch_grad3();  // warm
-                (void)hipEventRecord(a, s_);
-                for (int r = 0; r < 3; ++r) launch_grad3();
-                (void)hipEventRecord(b, s_);
-                if (!check(hipEventSynchronize(b), "tune sync")) return false;
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, a, b);
-                double per = ms / 3.0;
+                double per = 0;
+                if (!time_it([&]() { launch_grad3(); }, best, per)) return false;
                 snprintf(line, sizeof line, "k_grad3g %d slices per wave, %d waves per workgroup: %.3f ms per launch\n",
                          gs, wpb, per);
                 tune_log_ += line;
@@ -1315,14 +1334,8 @@ bool Solver::tune3d() {
         double best_g = best;
         if (gm_ok && want_m) {
             use_gm = true;
-            launch_grad3();  // warm
-            (void)hipEventRecord(a, s_);
-            for (int r = 0; r < 3; ++r) launch_grad3();
-            (void)hipEventRecord(b, s_);
-            if (!check(hipEventSynchronize(b), "tune sync")) return false;
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, a, b);
-            double per = ms / 3.0;
+            double per = 0;
+            if (!time_it([&]() { launch_grad3(); }, best, per)) return false;
             snprintf(line, sizeof line, "k_grad3m (matrix cores, %d tiles): %.3f ms per launch\n", gm_ntiles, per);
             tune_log_ += line;
             use_gm = per < best;
@@ -1373,14 +1386,8 @@ bool Solver::tune3d() {
             lv_bc = c.kind == 2 ? c.bc : 0;
             lv_np = c.kind == 2 ? c.np : 2;
             s_sh = c.kind == 2 ? chunk_sh(c.bc) : -1;
-            launch_sweep3(m);  // warm
-            (void)hipEventRecord(a, s_);
-            for (int r = 0; r < 3; ++r) launch_sweep3(m);
-            (void)hipEventRecord(b, s_);
-            if (!check(hipEventSynchronize(b), "tune sync")) return false;
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, a, b);
-            double per = ms / 3.0;
+            double per = 0;
+            if (!time_it([&]() { launch_sweep3(m); }, best, per)) return false;
             if (c.kind == 2)
                 snprintf(line, sizeof line, "sweep chunked %d bands, %d positions per thread: %.3f ms per launch\n",
                          c.bc, c.np, per);
