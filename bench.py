@@ -145,7 +145,9 @@ def kernel_bytes_per_update(dim, word=8):
                  in-faces), writes the sweep source
       sweep      reads the source, writes e
       2D face    k_tile_face reads e, writes the gradient (2 values)
-      3D grad    k_grad3g reads e, writes the gradient's four face projections
+      3D grad    reads e, writes the gradient's four face projections (the
+                 vertex-sum variant's sums are intermediate traffic, counted
+                 in the measured traffic, not in the algorithmic bytes)
       3D moments the moment kernels read e"""
     if dim == 3:
         return {"source": 5 * word, "sweep": 2 * word, "grad": 5 * word, "moments": word}
@@ -161,7 +163,9 @@ def kernel_name(phase, dim, prec, tuning):
             return "k_sweep_lv"
         return "k_sweep_dir" if "sweep chosen: level-synchronous" in tuning else "k_sweep_pc"
     if phase == "grad":
-        return "k_grad3m" if "gradient chosen: k_grad3m" in tuning else "k_grad3g"
+        if "gradient chosen: k_grad3m" in tuning:
+            return "k_grad3m"
+        return "k_vsum3 + k_grad3v" if "gradient chosen: k_vsum3 + k_grad3v" in tuning else "k_grad3g"
     return {"source": "k_source3", "moments": "k_mom3 + k_vert3 + k_fin3"}[phase]
 
 

@@ -109,7 +109,7 @@ class BTESolver:
         sweeps = {"auto": 0, "wavespec": 1, "level": 2, "chunked": 3}
         if sweep_kernel not in sweeps:
             raise ValueError(f"sweep_kernel must be one of {sorted(sweeps)}")
-        grads = {"auto": 0, "gather": 1, "matrix": 2}
+        grads = {"auto": 0, "gather": 1, "matrix": 2, "vertex": 3}
         if gradient_kernel not in grads:
             raise ValueError(f"gradient_kernel must be one of {sorted(grads)}")
         opts = _lib.Options(_lib.GBTE_FP64 if precision == "fp64" else _lib.GBTE_FP32_SWEEP, device, int(verbose),

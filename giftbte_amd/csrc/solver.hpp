@@ -133,6 +133,7 @@ class Solver {
     // tetrahedral meshes (solver3d.hip): device tables, the solve loop and
     // the launches of the 3D kernels
     bool build3d(const std::vector<int> &be_slot);
+    bool build_gv(double invLx);
     int solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st);
     void launch_rhs3(bool macro, const double *bval, double *rhs);
     void launch_grad3();
@@ -244,6 +245,12 @@ class Solver {
     int gm_ntiles = 0;
     DBuf<int> d_gm_rows, d_gm_cell;
     DBuf<char> d_gm_A;
+    // vertex-sum gradient (k_vsum3 + k_grad3v): per-vertex entries, chosen
+    // cells and sums, per-cell entries and records
+    bool gv_ok = false, use_gv = false;
+    int gv_nv = 0;
+    DBuf<int> d_gv_vptr, d_gv_vidx, d_gv_vref, d_gv_ptr, d_gv_idx;
+    DBuf<char> d_gv_vent, d_gv_ent, d_gv_rec, d_gv_sums;
     std::string tune_log_;  // the autotuner's candidates, timings and choices
     DBuf<double> d_ndy, d_ndu, d_ndx;
     // state

@@ -45,6 +45,7 @@
 #include "solver.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <limits>
@@ -85,38 +86,15 @@ struct alignas(16) G3Ent<float> {
 
 // ----------------------------------------------------------------- kernels
 
-// _get_gradient_larger, 3D (StaticBTESynthetic.cpp:1329-1371):
-// grad = sum_j (P_j / L_x) (e_j - e_i), zero for a stencil of 3 or fewer
-// cells (zero weights) or a NaN, stored as its projections on the cell's four
-// face offsets, phi_f = r_f . grad with r_f the face centre minus the cell
-// centre: the sources and the outgoing boundary values use the gradient only
-// through these (_get_Re :1427-1500, _get_bound_ee :1502-1539), so one load
-// per face replaces three.  One wave per (cell, group of GS 64-entry
-// slices), the lanes over the entries, the neighbour rows read straight from
-// memory (L2).  A batch of stencil-table entries (scalar loads) serves the
-// whole group, and each batch issues kG3Ent * GS row loads before its
-// products.  The WPB waves of a workgroup take Morton-consecutive cells, whose
-// stencils (in increasing row order) overlap, so they read largely the same
-// neighbour rows at about the same time.
-template <typename ST, int GS, int WPB>
-__global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, ST *__restrict__ phi,
-                                                     const int *__restrict__ sptr, const int *__restrict__ gidx,
-                                                     const G3Ent<ST> *__restrict__ ent, const double *__restrict__ rec,
-                                                     int nc, int DB, int ngroups) {
-    int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-    int sg = wv / nc, c = wv - sg * nc;
-    if (sg >= ngroups) return;  // whole wave
-    int a = sptr[c], z = sptr[c + 1];
-    int tb = sg * GS * kG3W + lane;
-    ST self[GS], gx[GS], gy[GS], gz[GS];
-    int tt[GS];
-#pragma unroll
-    for (int q = 0; q < GS; ++q) {
-        int t = tb + q * kG3W;
-        tt[q] = t < DB ? t : 0;
-        self[q] = e[(size_t)c * DB + tt[q]];
-        gx[q] = 0; gy[q] = 0; gz[q] = 0;
-    }
+// The entry loop of the gathering LSQ kernels: g += w_j (X[row j] - self)
+// over the stencil-table entries a..z (padded to whole batches), with S0 also
+// the plain sum of the differences.  A batch of table entries (scalar loads)
+// serves all GS slices of the wave, and each batch issues kG3Ent * GS row
+// loads before its products.
+template <typename ST, int GS, bool S0>
+__device__ __forceinline__ void g3_entries(const ST *__restrict__ X, const int *__restrict__ gidx,
+                                           const G3Ent<ST> *__restrict__ ent, int a, int z, const int *tt, int DB,
+                                           const ST *self, ST *gx, ST *gy, ST *gz, ST *s0) {
     for (int k = a; k < z; k += kG3Ent) {
         G3Ent<ST> w[kG3Ent];
         int nb[kG3Ent];
@@ -129,7 +107,7 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, S
 #pragma unroll
         for (int j = 0; j < kG3Ent; ++j)
 #pragma unroll
-            for (int q = 0; q < GS; ++q) v[j][q] = e[(size_t)nb[j] * DB + tt[q]];
+            for (int q = 0; q < GS; ++q) v[j][q] = X[(size_t)nb[j] * DB + tt[q]];
 #pragma unroll
         for (int j = 0; j < kG3Ent; ++j)
 #pragma unroll
@@ -138,19 +116,163 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, S
                 gx[q] += w[j].px * d;
                 gy[q] += w[j].py * d;
                 gz[q] += w[j].pz * d;
+                if constexpr (S0) s0[q] += d;
             }
     }
+}
+
+// phi_f = r_f . g for the four faces of cell c (NaN -> 0, the reference's
+// zero gradient)
+template <typename ST>
+__device__ __forceinline__ void g3_store_phi(ST *__restrict__ phi, const double *__restrict__ rc, int c, int t, int DB,
+                                             double x, double y, double w) {
+    if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
+    ST *pc = phi + (size_t)c * 4 * DB + t;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+        __builtin_nontemporal_store((ST)(rc[f * 10 + 4] * x + rc[f * 10 + 5] * y + rc[f * 10 + 6] * w), pc + f * DB);
+}
+
+// _get_gradient_larger, 3D (StaticBTESynthetic.cpp:1329-1371):
+// grad = sum_j (P_j / L_x) (e_j - e_i), zero for a stencil of 3 or fewer
+// cells (zero weights) or a NaN, stored as its projections on the cell's four
+// face offsets, phi_f = r_f . grad with r_f the face centre minus the cell
+// centre: the sources and the outgoing boundary values use the gradient only
+// through these (_get_Re :1427-1500, _get_bound_ee :1502-1539), so one load
+// per face replaces three.  One wave per (cell, group of GS 64-entry
+// slices), the lanes over the entries, the neighbour rows read straight from
+// memory (L2).  The WPB waves of a workgroup take Morton-consecutive cells,
+// whose stencils (in increasing row order) overlap, so they read largely the
+// same neighbour rows at about the same time.
+template <typename ST, int GS, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, ST *__restrict__ phi,
+                                                     const int *__restrict__ sptr, const int *__restrict__ gidx,
+                                                     const G3Ent<ST> *__restrict__ ent, const double *__restrict__ rec,
+                                                     int nc, int DB, int ngroups) {
+    int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    int sg = wv / nc, c = wv - sg * nc;
+    if (sg >= ngroups) return;  // whole wave
+    int tb = sg * GS * kG3W + lane;
+    ST self[GS], gx[GS], gy[GS], gz[GS];
+    int tt[GS];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+        int t = tb + q * kG3W;
+        tt[q] = t < DB ? t : 0;
+        self[q] = e[(size_t)c * DB + tt[q]];
+        gx[q] = 0; gy[q] = 0; gz[q] = 0;
+    }
+    g3_entries<ST, GS, false>(e, gidx, ent, sptr[c], sptr[c + 1], tt, DB, self, gx, gy, gz, nullptr);
     const double *rc = rec + (size_t)c * kS3D;
 #pragma unroll
     for (int q = 0; q < GS; ++q) {
         int t = tb + q * kG3W;
-        if (t >= DB) continue;
-        double x = gx[q], y = gy[q], w = gz[q];
-        if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
-        ST *pc = phi + (size_t)c * 4 * DB + t;
+        if (t < DB) g3_store_phi(phi, rc, c, t, DB, gx[q], gy[q], gz[q]);
+    }
+}
+
+// The same gradient through sums over the mesh vertices, which halves the
+// rows a cell reads.  The stencil of cell i is the union of the cells C(v)
+// around its four vertices; a cell k of it is found m_ik times (the vertices
+// it shares with i: 1, 2 or 3), so with j_k = (x_k - x_i) / L_x, M the 3 x 3
+// inverse of the LSQ normal matrix and r_v one chosen cell of C(v),
+//   sum_k j_k (e_k - e_i) = sum_v (S1_v + d_v S0_v)
+//                           + sum_k c_k (e_k - e_i),  d_v = (x_v - x_i) / L_x,
+//   S0_v = sum_{k in C(v)} (e_k - e_{r_v}),
+//   S1_v = sum_{k in C(v)} ((x_k - x_v) / L_x) (e_k - e_{r_v}),
+//   c_k  = -(m_ik - 1) j_k + sum_{v: r_v = k} sum_{l in C(v)} j_l,
+// and grad = (M / L_x) times that: exact algebra, the weights c_k summing to
+// zero.  Only the cells sharing an edge with i and the chosen r_v keep an own
+// entry (about 20 against about 78), plus the four vertex sums.  Subtracting
+// e_{r_v} keeps the vertex sums at the size of differences (no cancellation
+// in FP32).
+// k_vsum3: one wave per (vertex, slice group): the quad (S1x, S1y, S1z, S0)
+// per entry, [vertex][entry][4].  Vertices are numbered in the order of
+// their first cell, so consecutive waves share cells.
+template <typename ST, int GS, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_vsum3(const ST *__restrict__ e, ST *__restrict__ vs,
+                                                    const int *__restrict__ vptr, const int *__restrict__ vidx,
+                                                    const G3Ent<ST> *__restrict__ vent, const int *__restrict__ vref,
+                                                    int nv, int DB, int ngroups) {
+    using Q4 = typename std::conditional<sizeof(ST) == 4, v4f, v4d>::type;
+    int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    int sg = wv / nv, v = wv - sg * nv;
+    if (sg >= ngroups) return;  // whole wave
+    int tb = sg * GS * kG3W + lane;
+    int r = vref[v];
+    ST self[GS], gx[GS], gy[GS], gz[GS], s0[GS];
+    int tt[GS];
 #pragma unroll
-        for (int f = 0; f < 4; ++f)
-            __builtin_nontemporal_store((ST)(rc[f * 10 + 4] * x + rc[f * 10 + 5] * y + rc[f * 10 + 6] * w), pc + f * DB);
+    for (int q = 0; q < GS; ++q) {
+        int t = tb + q * kG3W;
+        tt[q] = t < DB ? t : 0;
+        self[q] = e[(size_t)r * DB + tt[q]];
+        gx[q] = 0; gy[q] = 0; gz[q] = 0; s0[q] = 0;
+    }
+    g3_entries<ST, GS, true>(e, vidx, vent, vptr[v], vptr[v + 1], tt, DB, self, gx, gy, gz, s0);
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+        int t = tb + q * kG3W;
+        if (t < DB) *reinterpret_cast<Q4 *>(vs + ((size_t)v * DB + t) * 4) = Q4{gx[q], gy[q], gz[q], s0[q]};
+    }
+}
+
+// k_grad3v's per-cell record: the four vertices (k_vsum3 numbering),
+// a = M / L_x (row-major) and b_v = (M / L_x) d_v
+template <typename ST>
+struct alignas(16) G3VRec {
+    int v[4];
+    ST a[9];
+    ST b[12];
+};
+
+// k_grad3v: the face projections from the vertex sums and the remaining
+// entries (one wave per (cell, slice group), as k_grad3g)
+template <typename ST, int GS, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_grad3v(const ST *__restrict__ e, const ST *__restrict__ vs,
+                                                     ST *__restrict__ phi, const G3VRec<ST> *__restrict__ vrec,
+                                                     const int *__restrict__ sptr, const int *__restrict__ gidx,
+                                                     const G3Ent<ST> *__restrict__ ent, const double *__restrict__ rec,
+                                                     int nc, int DB, int ngroups) {
+    using Q4 = typename std::conditional<sizeof(ST) == 4, v4f, v4d>::type;
+    int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    int sg = wv / nc, c = wv - sg * nc;
+    if (sg >= ngroups) return;  // whole wave
+    int tb = sg * GS * kG3W + lane;
+    const G3VRec<ST> &R = vrec[c];
+    ST self[GS], gx[GS], gy[GS], gz[GS], sx[GS], sy[GS], sz[GS];
+    int tt[GS];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+        int t = tb + q * kG3W;
+        tt[q] = t < DB ? t : 0;
+        self[q] = e[(size_t)c * DB + tt[q]];
+        gx[q] = 0; gy[q] = 0; gz[q] = 0; sx[q] = 0; sy[q] = 0; sz[q] = 0;
+    }
+    // the vertex sums: S1 summed over the four vertices, S0 weighted per vertex
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        Q4 qv[GS];
+#pragma unroll
+        for (int q = 0; q < GS; ++q) qv[q] = *reinterpret_cast<const Q4 *>(vs + ((size_t)R.v[i] * DB + tt[q]) * 4);
+#pragma unroll
+        for (int q = 0; q < GS; ++q) {
+            sx[q] += qv[q][0]; sy[q] += qv[q][1]; sz[q] += qv[q][2];
+            gx[q] += R.b[3 * i] * qv[q][3]; gy[q] += R.b[3 * i + 1] * qv[q][3]; gz[q] += R.b[3 * i + 2] * qv[q][3];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+        gx[q] += R.a[0] * sx[q] + R.a[1] * sy[q] + R.a[2] * sz[q];
+        gy[q] += R.a[3] * sx[q] + R.a[4] * sy[q] + R.a[5] * sz[q];
+        gz[q] += R.a[6] * sx[q] + R.a[7] * sy[q] + R.a[8] * sz[q];
+    }
+    g3_entries<ST, GS, false>(e, gidx, ent, sptr[c], sptr[c + 1], tt, DB, self, gx, gy, gz, nullptr);
+    const double *rc = rec + (size_t)c * kS3D;
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+        int t = tb + q * kG3W;
+        if (t < DB) g3_store_phi(phi, rc, c, t, DB, gx[q], gy[q], gz[q]);
     }
 }
 
@@ -742,9 +864,9 @@ __global__ void k_rhs3(const double *__restrict__ RM, const double *__restrict__
     rhs[c] = -r;
 }
 
-// (J^T J)^{-1} J^T of a J with three columns (Get_inverse_3D,
-// src/utility/utility.cpp:488-548), row-major 3 x m.
-void lsq3(const std::vector<double> &J, int m, std::vector<double> &P) {
+// (J^T J)^{-1} of a J with three columns (Get_inverse_3D,
+// src/utility/utility.cpp:488-548: the adjugate over the determinant)
+void jtj_inv(const std::vector<double> &J, int m, double I[3][3]) {
     double A[3][3];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
@@ -753,7 +875,6 @@ void lsq3(const std::vector<double> &J, int m, std::vector<double> &P) {
         }
     double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) + A[0][1] * (A[1][2] * A[2][0] - A[1][0] * A[2][2]) +
                  A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
-    double I[3][3];
     I[0][0] = 1.0 / det * (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
     I[1][0] = 1.0 / det * (A[0][2] * A[2][1] - A[0][1] * A[2][2]);
     I[2][0] = 1.0 / det * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
@@ -763,6 +884,13 @@ void lsq3(const std::vector<double> &J, int m, std::vector<double> &P) {
     I[0][2] = 1.0 / det * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
     I[1][2] = 1.0 / det * (A[0][1] * A[2][0] - A[0][0] * A[2][1]);
     I[2][2] = 1.0 / det * (A[0][0] * A[1][1] - A[1][0] * A[0][1]);
+}
+
+// (J^T J)^{-1} J^T of a J with three columns (Get_inverse_3D,
+// src/utility/utility.cpp:488-548), row-major 3 x m.
+void lsq3(const std::vector<double> &J, int m, std::vector<double> &P) {
+    double I[3][3];
+    jtj_inv(J, m, I);
     P.assign((size_t)3 * m, 0.0);
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < m; ++j)
@@ -1014,6 +1142,10 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
             }
         }
     }
+    // ---- vertex-sum gradient (k_vsum3 + k_grad3v): the cells around each
+    // vertex, one chosen cell r_v per vertex, per-vertex and per-cell tables
+    gv_ok = build_gv(invLx);
+    if (!err_.empty()) return false;
     // ---- moments and the residual's face coefficients
     std::vector<std::vector<double>> FM(nf);  // per unique face, from its first side: 3 x nv
     for (int f = 0; f < nf; ++f) {
@@ -1143,6 +1275,215 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     return true;
 }
 
+// Tables of the vertex-sum gradient (k_vsum3 + k_grad3v; the algebra is at
+// k_vsum3).  Returns false, with err_ empty, when the mesh does not take it:
+// a cell that is not a tetrahedron, or a stencil that is not exactly the
+// cells around the cell's vertices (cells of another material).
+bool Solver::build_gv(double invLx) {
+    const Problem &p = P;
+    const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    auto par = [&](int n, auto fn) {
+        std::vector<std::thread> th;
+        for (int w = 0; w < nth; ++w)
+            th.emplace_back([&, w]() {
+                for (int i = w; i < n; i += nth) fn(i);
+            });
+        for (auto &x : th) x.join();
+    };
+    for (int c = 0; c < nc; ++c)
+        if (p.cell_nvert[c] != 4) return false;
+    // cells around each vertex, in increasing cell order
+    std::vector<int> cnt(nn + 1, 0);
+    for (int c = 0; c < nc; ++c)
+        for (int i = 0; i < 4; ++i) {
+            int v = p.cell_vert[(size_t)c * 8 + i];
+            if (v < 0 || v >= nn) { err_ = "internal error: tetrahedron vertex out of range"; return false; }
+            ++cnt[v + 1];
+        }
+    for (int v = 0; v < nn; ++v) cnt[v + 1] += cnt[v];
+    std::vector<int> v2c(cnt[nn]);
+    {
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int c = 0; c < nc; ++c)
+            for (int i = 0; i < 4; ++i) v2c[fill[p.cell_vert[(size_t)c * 8 + i]]++] = c;
+    }
+    // the used vertices, numbered in the order of their first cell
+    std::vector<int> order, newid(nn, -1);
+    for (int v = 0; v < nn; ++v)
+        if (cnt[v + 1] > cnt[v]) order.push_back(v);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return v2c[cnt[x]] != v2c[cnt[y]] ? v2c[cnt[x]] < v2c[cnt[y]] : x < y;
+    });
+    int nvu = (int)order.size();
+    for (int i = 0; i < nvu; ++i) newid[order[i]] = i;
+    // r_v: the cell of C(v) that shares an edge with the most cells of C(v)
+    // (those then need no own entry for it)
+    std::vector<int> rv(nn, -1);
+    auto shares_edge = [&](int x, int y) {
+        int n = 0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) n += p.cell_vert[(size_t)x * 8 + i] == p.cell_vert[(size_t)y * 8 + j];
+        return n >= 2;
+    };
+    par(nvu, [&](int i) {
+        int v = order[i], best = -1;
+        for (int a = cnt[v]; a < cnt[v + 1]; ++a) {
+            int s = 0;
+            for (int b = cnt[v]; b < cnt[v + 1]; ++b) s += shares_edge(v2c[a], v2c[b]);
+            if (s > best) { best = s; rv[v] = v2c[a]; }
+        }
+    });
+    // per-vertex entries: (cell, (x_k - x_v) / L_x), padded with r_v at 0
+    std::vector<int> vptr(nvu + 1, 0), vidx, vref(nvu);
+    std::vector<double> vw;
+    for (int i = 0; i < nvu; ++i) {
+        int v = order[i];
+        vref[i] = rv[v];
+        for (int a = cnt[v]; a < cnt[v + 1]; ++a) {
+            int k = v2c[a];
+            vidx.push_back(k);
+            vw.push_back(invLx * (p.cx[k] - p.node_x[v]));
+            vw.push_back(invLx * (p.cy[k] - p.node_y[v]));
+            vw.push_back(invLx * (p.cz[k] - p.node_z[v]));
+        }
+        while ((vidx.size() - vptr[i]) % kG3Ent) {
+            vidx.push_back(rv[v]);
+            vw.insert(vw.end(), {0.0, 0.0, 0.0});
+        }
+        vptr[i + 1] = (int)vidx.size();
+    }
+    // per-cell entries and records, built per cell on host threads
+    struct CellOut {
+        std::vector<int> idx;
+        std::vector<double> w;  // 3 per entry
+        double a[9], b[12];
+        int v[4];
+        bool ok = true;
+    };
+    std::vector<CellOut> out(nc);
+    par(nc, [&](int c) {
+        CellOut &o = out[c];
+        std::vector<int> L;
+        for (int i = 0; i < 4; ++i) {
+            int v = p.cell_vert[(size_t)c * 8 + i];
+            L.insert(L.end(), v2c.begin() + cnt[v], v2c.begin() + cnt[v + 1]);
+        }
+        std::sort(L.begin(), L.end());
+        // the distinct cells other than c must be exactly the stencil
+        std::vector<int> nb(p.nbr_idx.begin() + p.nbr_ptr[c], p.nbr_idx.begin() + p.nbr_ptr[c + 1]);
+        std::sort(nb.begin(), nb.end());
+        std::vector<std::pair<int, int>> mult;  // (cell, shared vertices)
+        for (size_t a = 0; a < L.size();) {
+            size_t z = a;
+            while (z < L.size() && L[z] == L[a]) ++z;
+            if (L[a] != c) mult.push_back({L[a], (int)(z - a)});
+            a = z;
+        }
+        if (mult.size() != nb.size()) { o.ok = false; return; }
+        for (size_t k = 0; k < nb.size(); ++k)
+            if (mult[k].first != nb[k]) { o.ok = false; return; }
+        int m = (int)nb.size();
+        double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (m > 3) {
+            std::vector<double> J((size_t)m * 3);
+            for (int k = 0; k < m; ++k) {
+                int n = p.nbr_idx[p.nbr_ptr[c] + k];
+                J[k * 3 + 0] = invLx * (p.cx[n] - p.cx[c]);
+                J[k * 3 + 1] = invLx * (p.cy[n] - p.cy[c]);
+                J[k * 3 + 2] = invLx * (p.cz[n] - p.cz[c]);
+            }
+            jtj_inv(J, m, M);
+        }
+        auto jv = [&](int k, double *d) {
+            d[0] = invLx * (p.cx[k] - p.cx[c]); d[1] = invLx * (p.cy[k] - p.cy[c]); d[2] = invLx * (p.cz[k] - p.cz[c]);
+        };
+        // c_k: -(m_ik - 1) j_k for the cells sharing an edge, + Q_v at r_v
+        std::vector<std::pair<int, std::array<double, 3>>> cw;
+        auto add = [&](int k, const double *d, double f) {
+            for (auto &x : cw)
+                if (x.first == k) { for (int i = 0; i < 3; ++i) x.second[i] += f * d[i]; return; }
+            cw.push_back({k, {f * d[0], f * d[1], f * d[2]}});
+        };
+        for (auto &km : mult)
+            if (km.second >= 2) {
+                double d[3];
+                jv(km.first, d);
+                add(km.first, d, -(double)(km.second - 1));
+            }
+        for (int i = 0; i < 4; ++i) {
+            int v = p.cell_vert[(size_t)c * 8 + i];
+            double Q[3] = {0, 0, 0}, d[3];
+            for (int a = cnt[v]; a < cnt[v + 1]; ++a) {
+                jv(v2c[a], d);
+                for (int j = 0; j < 3; ++j) Q[j] += d[j];
+            }
+            if (rv[v] != c) add(rv[v], Q, 1.0);
+            o.v[i] = newid[v];
+            double dv[3] = {invLx * (p.node_x[v] - p.cx[c]), invLx * (p.node_y[v] - p.cy[c]),
+                            invLx * (p.node_z[v] - p.cz[c])};
+            for (int r = 0; r < 3; ++r)
+                o.b[3 * i + r] = invLx * (M[r][0] * dv[0] + M[r][1] * dv[1] + M[r][2] * dv[2]);
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int j = 0; j < 3; ++j) o.a[3 * r + j] = invLx * M[r][j];
+        std::sort(cw.begin(), cw.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        for (auto &x : cw) {
+            o.idx.push_back(x.first);
+            for (int r = 0; r < 3; ++r)
+                o.w.push_back(invLx * (M[r][0] * x.second[0] + M[r][1] * x.second[1] + M[r][2] * x.second[2]));
+        }
+        while (o.idx.size() % kG3Ent) {
+            o.idx.push_back(c);
+            o.w.insert(o.w.end(), {0.0, 0.0, 0.0});
+        }
+    });
+    for (int c = 0; c < nc; ++c)
+        if (!out[c].ok) return false;
+    std::vector<int> gptr(nc + 1, 0), gidx;
+    std::vector<double> gw;
+    for (int c = 0; c < nc; ++c) {
+        gidx.insert(gidx.end(), out[c].idx.begin(), out[c].idx.end());
+        gw.insert(gw.end(), out[c].w.begin(), out[c].w.end());
+        gptr[c + 1] = (int)gidx.size();
+    }
+    for (int k : gidx)
+        if (k < 0 || k >= nc) { err_ = "internal error: vertex-gradient entry"; return false; }
+    for (int k : vidx)
+        if (k < 0 || k >= nc) { err_ = "internal error: vertex-sum entry"; return false; }
+    // upload in the storage type
+    auto put = [&](DBuf<char> &dst, auto tag, const std::vector<int> &idx, const std::vector<double> &w) -> bool {
+        using ST = decltype(tag);
+        std::vector<G3Ent<ST>> h(idx.size());
+        for (size_t k = 0; k < idx.size(); ++k) {
+            h[k] = G3Ent<ST>{};
+            h[k].loc = idx[k];
+            h[k].px = (ST)w[3 * k]; h[k].py = (ST)w[3 * k + 1]; h[k].pz = (ST)w[3 * k + 2];
+        }
+        return alloc(dst, h.size() * sizeof(h[0])) &&
+               check(hipMemcpy(dst.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice), "H2D");
+    };
+    auto put_rec = [&](auto tag) -> bool {
+        using ST = decltype(tag);
+        std::vector<G3VRec<ST>> h(nc);
+        for (int c = 0; c < nc; ++c) {
+            h[c] = G3VRec<ST>{};
+            for (int i = 0; i < 4; ++i) h[c].v[i] = out[c].v[i];
+            for (int i = 0; i < 9; ++i) h[c].a[i] = (ST)out[c].a[i];
+            for (int i = 0; i < 12; ++i) h[c].b[i] = (ST)out[c].b[i];
+        }
+        return alloc(d_gv_rec, h.size() * sizeof(h[0])) &&
+               check(hipMemcpy(d_gv_rec.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice), "H2D");
+    };
+    bool ok = fp32_ ? put(d_gv_vent, 0.0f, vidx, vw) && put(d_gv_ent, 0.0f, gidx, gw) && put_rec(0.0f)
+                    : put(d_gv_vent, 0.0, vidx, vw) && put(d_gv_ent, 0.0, gidx, gw) && put_rec(0.0);
+    if (!ok || !upload(d_gv_vptr, vptr) || !upload(d_gv_vidx, vidx) || !upload(d_gv_vref, vref) ||
+        !upload(d_gv_ptr, gptr) || !upload(d_gv_idx, gidx))
+        return false;
+    gv_nv = nvu;
+    size_t sums = (size_t)nvu * nD * nB * 4 * (fp32_ ? sizeof(float) : sizeof(double));
+    return alloc(d_gv_sums, sums);
+}
+
 void Solver::launch_rhs3(bool macro, const double *bval, double *rhs) {
     int set = macro ? 0 : 1;
     hipLaunchKernelGGL(k_rhs3, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, s_, macro ? d_RM.p : nullptr, d_Q.p,
@@ -1175,10 +1516,30 @@ void Solver::launch_grad3() {
     }
     int ngroups = (DB + g3_gs * kG3W - 1) / (g3_gs * kG3W);
     int grid = ((blocks_for((long)ngroups * nc, g3_wpb) + kXcd - 1) / kXcd) * kXcd;
+    int gridv = ((blocks_for((long)ngroups * gv_nv, g3_wpb) + kXcd - 1) / kXcd) * kXcd;
+    // the vertex sums, then the projections from them
+#define GBTE_G3V(ST_, GS_, W_)                                                                                   \
+    do {                                                                                                         \
+        hipLaunchKernelGGL((k_vsum3<ST_, GS_, W_>), dim3(gridv), dim3(64 * W_), 0, s_,                           \
+                           reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<ST_ *>(d_gv_sums.p),           \
+                           d_gv_vptr.p, d_gv_vidx.p, reinterpret_cast<const G3Ent<ST_> *>(d_gv_vent.p),          \
+                           d_gv_vref.p, gv_nv, DB, ngroups);                                                     \
+        hipLaunchKernelGGL((k_grad3v<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                           \
+                           reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<const ST_ *>(d_gv_sums.p),     \
+                           reinterpret_cast<ST_ *>(d_grad.p), reinterpret_cast<const G3VRec<ST_> *>(d_gv_rec.p), \
+                           d_gv_ptr.p, d_gv_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_gv_ent.p), d_s3d.p, nc, \
+                           DB, ngroups);                                                                         \
+    } while (0)
 #define GBTE_G3(ST_, GS_, W_)                                                                                    \
-    hipLaunchKernelGGL((k_grad3g<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                               \
-                       reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<ST_ *>(d_grad.p), d_g3g_ptr.p,    \
-                       d_g3g_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_g3g_ent.p), d_s3d.p, nc, DB, ngroups)
+    do {                                                                                                         \
+        if (use_gv)                                                                                              \
+            GBTE_G3V(ST_, GS_, W_);                                                                              \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_grad3g<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                       \
+                               reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<ST_ *>(d_grad.p),          \
+                               d_g3g_ptr.p, d_g3g_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_g3g_ent.p),      \
+                               d_s3d.p, nc, DB, ngroups);                                                        \
+    } while (0)
 #define GBTE_G3_W(ST_, GS_)                         \
     do {                                            \
         if (g3_wpb == 16) GBTE_G3(ST_, GS_, 16);    \
@@ -1196,6 +1557,7 @@ void Solver::launch_grad3() {
 #undef GBTE_G3_GS
 #undef GBTE_G3_W
 #undef GBTE_G3
+#undef GBTE_G3V
 }
 
 void Solver::launch_source3(const Modes &m) {
@@ -1316,22 +1678,38 @@ bool Solver::tune3d() {
         // the gradient: 64-entry slices per wave (one stencil-table batch
         // serves them all; more slices, more loads in flight per wave)
         double best = 1e300;
-        int pick = 4, pickw = 4;
-        bool want_g = opt.gradient_kernel != GBTE_GRAD_MATRIX, want_m = opt.gradient_kernel != GBTE_GRAD_GATHER;
-        if (!want_g && !gm_ok) { err_ = "the matrix-core gradient does not take this mesh's stencils"; return false; }
-        for (int wpb : {4, 16})
-            for (int gs : {1, 2, 4, 8}) {
-                if (!want_g || (gs == 8 && !fp32_)) continue;  // FP64 is bound by the gather at 2 slices
-                g3_gs = gs;
-                g3_wpb = wpb;
-                double per = 0;
-                if (!time_it([&]() { launch_grad3(); }, best, per)) return false;
-                snprintf(line, sizeof line, "k_grad3g %d slices per wave, %d waves per workgroup: %.3f ms per launch\n",
-                         gs, wpb, per);
-                tune_log_ += line;
-                if (per < best) { best = per; pick = gs; pickw = wpb; }
-            }
-        double best_g = best;
+        int pick = 4, pickw = 4, kind = 0;  // kind 0: k_grad3g, 1: k_vsum3 + k_grad3v, 2: k_grad3m
+        int want = opt.gradient_kernel;
+        if (want < GBTE_GRAD_AUTO || want > GBTE_GRAD_VERTEX) { err_ = "unknown gradient_kernel"; return false; }
+        bool want_g = want == GBTE_GRAD_AUTO || want == GBTE_GRAD_GATHER;
+        bool want_v = want == GBTE_GRAD_AUTO || want == GBTE_GRAD_VERTEX;
+        bool want_m = want == GBTE_GRAD_AUTO || want == GBTE_GRAD_MATRIX;
+        if (want == GBTE_GRAD_MATRIX && !gm_ok) {
+            err_ = "the matrix-core gradient does not take this mesh's stencils";
+            return false;
+        }
+        if (want == GBTE_GRAD_VERTEX && !gv_ok) {
+            err_ = "the vertex-sum gradient does not take this mesh (tetrahedra of one material only)";
+            return false;
+        }
+        for (int v : {0, 1}) {
+            if (v == 0 ? !want_g : !(want_v && gv_ok)) continue;
+            use_gv = v == 1;
+            for (int wpb : {4, 16})
+                for (int gs : {1, 2, 4, 8}) {
+                    if (gs == 8 && !fp32_) continue;  // FP64 is bound by the gather at 2 slices
+                    if (v == 1 && gs == 1) continue;
+                    g3_gs = gs;
+                    g3_wpb = wpb;
+                    double per = 0;
+                    if (!time_it([&]() { launch_grad3(); }, best, per)) return false;
+                    snprintf(line, sizeof line, "%s %d slices per wave, %d waves per workgroup: %.3f ms per launch\n",
+                             v ? "k_vsum3 + k_grad3v" : "k_grad3g", gs, wpb, per);
+                    tune_log_ += line;
+                    if (per < best) { best = per; pick = gs; pickw = wpb; kind = v; }
+                }
+        }
+        use_gv = false;
         if (gm_ok && want_m) {
             use_gm = true;
             double per = 0;
@@ -1339,17 +1717,17 @@ bool Solver::tune3d() {
             snprintf(line, sizeof line, "k_grad3m (matrix cores, %d tiles): %.3f ms per launch\n", gm_ntiles, per);
             tune_log_ += line;
             use_gm = per < best;
-            if (use_gm) best = per;
+            if (use_gm) { best = per; kind = 2; }
         }
         g3_gs = pick;
         g3_wpb = pickw;
+        use_gv = kind == 1;
         if (use_gm)
             snprintf(line, sizeof line, "gradient chosen: k_grad3m (matrix cores)\n");
         else
-            snprintf(line, sizeof line, "gradient chosen: k_grad3g, %d slices per wave, %d waves per workgroup\n",
-                     pick, pickw);
+            snprintf(line, sizeof line, "gradient chosen: %s, %d slices per wave, %d waves per workgroup\n",
+                     use_gv ? "k_vsum3 + k_grad3v" : "k_grad3g", pick, pickw);
         tune_log_ += line;
-        (void)best_g;
     }
     // sweeps: the wave-specialised kernel (when its schedule was built), the
     // one-workgroup-per-direction level-synchronous kernel (FP64) and the

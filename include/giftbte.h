@@ -134,13 +134,15 @@ typedef struct gbte_options {
                              GBTE_SWEEP_WAVESPEC, GBTE_SWEEP_LEVEL (FP64 only),
                              GBTE_SWEEP_CHUNKED; same results to rounding */
     int gradient_kernel;  /* tetrahedral meshes: GBTE_GRAD_AUTO (default: the set-up tuner
-                             keeps the faster) or GBTE_GRAD_GATHER (stencil gathers on the
-                             vector units) or GBTE_GRAD_MATRIX (tiles of four cells on
-                             the matrix cores); same results to rounding */
+                             keeps the fastest) or GBTE_GRAD_GATHER (stencil gathers on the
+                             vector units), GBTE_GRAD_MATRIX (tiles of four cells on the
+                             matrix cores) or GBTE_GRAD_VERTEX (sums over the mesh
+                             vertices, then the remaining stencil entries; tetrahedra of
+                             one material); same results to rounding */
 } gbte_options;
 
 enum { GBTE_SWEEP_AUTO = 0, GBTE_SWEEP_WAVESPEC = 1, GBTE_SWEEP_LEVEL = 2, GBTE_SWEEP_CHUNKED = 3 };
-enum { GBTE_GRAD_AUTO = 0, GBTE_GRAD_GATHER = 1, GBTE_GRAD_MATRIX = 2 };
+enum { GBTE_GRAD_AUTO = 0, GBTE_GRAD_GATHER = 1, GBTE_GRAD_MATRIX = 2, GBTE_GRAD_VERTEX = 3 };
 
 typedef struct gbte_stats {
     int iterations;       /* outer iterations run                               */
