@@ -2176,14 +2176,17 @@ bool Solver::build() {
         });
         std::vector<int>().swap(posT);
         if (dim == 3 && !opt.fallback_kernels) {
-            // records of the level-synchronous chunked sweep: every upwind
-            // cell lies on an earlier level of its direction (checked)
+            // records of the level-synchronous chunked sweep: every level is
+            // non-empty (the kernel's lanes past a level's end re-read its
+            // last position) and every upwind cell lies on an earlier level
+            // of its direction (both checked)
             std::vector<LvRec> lrec(NI);
             std::vector<char> badd(nD, 0);
             host_parallel(nD, [&](int d) {
                 const int *lp = levptr.data() + dlev[d];
                 int L = dlev[d + 1] - dlev[d] - 1;
-                for (int l = 0; l < L; ++l)
+                for (int l = 0; l < L; ++l) {
+                    if (lp[l + 1] <= lp[l]) badd[d] = 1;
                     for (int k = lp[l]; k < lp[l + 1]; ++k) {
                         size_t P0 = (size_t)d * nc + k;
                         LvRec &r = lrec[P0];
@@ -2196,9 +2199,10 @@ bool Solver::build() {
                             if (qq >= lp[l]) badd[d] = 1;
                         }
                     }
+                }
             });
             for (int d = 0; d < nD; ++d)
-                if (badd[d]) { err_ = "internal error: upwind cell on the same or a later level"; return false; }
+                if (badd[d]) { err_ = "internal error: empty level, or upwind cell on the same or a later level"; return false; }
             if (!upload(d_lv_rec, lrec)) return false;
         }
         // about one workgroup per CU: split the bands of a direction into

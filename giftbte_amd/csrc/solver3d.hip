@@ -653,52 +653,54 @@ __global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec,
     size_t DB = (size_t)nD * nB;
     ST *ed = e + (size_t)d * nB + b0;
     int tid = threadIdx.x;
+    // Every load of the pipelined part is unconditional (lanes past the end
+    // of a level re-read its last position, absent upwind cells read row 0,
+    // both masked afterwards): with no branch around a load, the wait before
+    // the products counts exactly this level's upwind loads and leaves the
+    // next level's records and sources in flight.
     auto fetch = [&](int l, LvRec(&rr)[NP], ST(&ss)[NP][BC]) {
         int a0 = slp[l], a1 = slp[l + 1];
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
-            int p = a0 + tid + i * 256;
-            if (p < a1) {
-                rr[i] = rd[p];
-                vld(ss[i], Sd + (size_t)p * sst);
-            }
+            int p = min(a0 + tid + i * 256, a1 - 1);
+            rr[i] = rd[p];
+            vld(ss[i], Sd + (size_t)p * sst);
         }
     };
     auto upwind = [&](const LvRec &r, ST(&u)[3][BC]) {
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            if (r.c[j] >= 0) vld(u[j], ed + (size_t)r.c[j] * DB);
+        for (int j = 0; j < 3; ++j) vld(u[j], ed + (size_t)max(r.c[j], 0) * DB);
     };
-    auto solve = [&](const LvRec &r, const ST(&s)[BC], const ST(&u)[3][BC]) {
+    auto solve = [&](const LvRec &r, const ST(&s)[BC], const ST(&u)[3][BC], bool store) {
         ST en[BC];
 #pragma unroll
         for (int b = 0; b < BC; ++b) {
             double v = s[b];
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                if (r.c[j] >= 0) v -= tv[b] * r.g[j] * (double)u[j][b];
+            for (int j = 0; j < 3; ++j) {
+                double t = tv[b] * r.g[j] * (double)u[j][b];
+                v -= r.c[j] >= 0 ? t : 0.0;  // an absent upwind cell: exactly v
+            }
             en[b] = (ST)(v / (1.0 + tv[b] * r.G));
         }
-        vst(ed + (size_t)r.cell * DB, en);
+        if (store) vst(ed + (size_t)r.cell * DB, en);
     };
     // level l from the buffers (rc, sc); the next level's first positions
-    // into (rn, sn)
+    // into (rn, sn) (the last level re-reads its own)
     auto level = [&](int l, LvRec(&rc)[NP], ST(&sc)[NP][BC], LvRec(&rn)[NP], ST(&sn)[NP][BC]) {
         int a0 = slp[l], a1 = slp[l + 1];
         ST u[NP][3][BC];
 #pragma unroll
-        for (int i = 0; i < NP; ++i)
-            if (a0 + tid + i * 256 < a1) upwind(rc[i], u[i]);
-        if (l + 1 < L) fetch(l + 1, rn, sn);
+        for (int i = 0; i < NP; ++i) upwind(rc[i], u[i]);
+        fetch(l + 1 < L ? l + 1 : l, rn, sn);
 #pragma unroll
-        for (int i = 0; i < NP; ++i)
-            if (a0 + tid + i * 256 < a1) solve(rc[i], sc[i], u[i]);
+        for (int i = 0; i < NP; ++i) solve(rc[i], sc[i], u[i], a0 + tid + i * 256 < a1);
         for (int p = a0 + 256 * NP + tid; p < a1; p += 256) {
             LvRec r = rd[p];
             ST s1[BC], u1[3][BC];
             vld(s1, Sd + (size_t)p * sst);
             upwind(r, u1);
-            solve(r, s1, u1);
+            solve(r, s1, u1, true);
         }
         __syncthreads();
     };
