@@ -106,7 +106,7 @@ class BTESolver:
             raise ValueError("linear_solver must be 'direct' or 'pcg'")
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32' (FP32 storage of the per-system fields)")
-        sweeps = {"auto": 0, "wavespec": 1, "level": 2, "chunked": 3}
+        sweeps = {"auto": 0, "wavespec": 1, "level": 2, "chunked": 3, "lanes": 4}
         if sweep_kernel not in sweeps:
             raise ValueError(f"sweep_kernel must be one of {sorted(sweeps)}")
         grads = {"auto": 0, "gather": 1, "matrix": 2, "vertex": 3}

@@ -204,6 +204,7 @@ class Solver {
     // (s_sh = log2 lv_bc), else [dir][position][band] (s_sh = -1)
     DBuf<LvRec> d_lv_rec;
     int lv_bc = 0, lv_np = 2, s_sh = -1;
+    int lv_v = 1, lv_tpb = 256;  // lanes per position (1: a lane takes the whole chunk), threads per workgroup
     // gradient stencil
     DBuf<int> d_gptr, d_gidx, d_gok;
     DBuf<double> d_gP;

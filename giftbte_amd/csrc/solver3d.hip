@@ -616,65 +616,72 @@ __device__ __forceinline__ void vst(T *p, const T (&v)[N]) {
 // whose levels are wide (tetrahedra: about 500 levels of about 600 cells per
 // direction):
 //   e_new = (S - sum_in,interior tau v g_f e_new_nb) / (1 + tau v G_out).
-// One 256-thread workgroup per (direction, chunk of BC bands), so that every
-// CU holds several: a thread takes whole positions of a level with all BC
-// bands of each, so a position's record is read once and its source, upwind
-// values and new values move as 16-byte vectors.  The upwind values are read
-// from e, where this workgroup wrote them on earlier levels; levels are
-// separated by workgroup barriers.
+// One TPB-thread workgroup per (direction, chunk of BC bands), so that every
+// CU holds several.  V lanes take one position, each lane BC / V bands of
+// it as whole 16-byte vectors: with V = 1 a lane moves the position's whole
+// chunk (its record read once), with V lanes per position the chunk's
+// upwind reads and stores are one 16-byte vector per lane, and the V lanes
+// of a position touch one cache line together instead of one lane touching
+// it V times (the L1 processes scattered reads line by line).  The upwind
+// values are read from e, where this workgroup wrote them on earlier levels;
+// levels are separated by workgroup barriers.
 //
-// Software pipeline: a level's first 256 NP positions have their records and
-// sources loaded during the previous level, issued after that level's upwind
-// loads, so that only the upwind loads (L2) and the arithmetic lie on the
-// chain of dependent levels (positions beyond the first 256 NP of a wide
-// level are loaded in place).  The level pointers of the direction sit in
-// LDS.  The chunks of one direction take consecutive workgroup ids, which
-// xcd_block places on one XCD, so they share its L2 for the records and for
-// the partial lines of the e rows they write.
-template <typename ST, int BC, int NP>
-__global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec, const int *__restrict__ levptr,
+// Software pipeline: a level's first TPB NP / V positions have their records
+// and sources loaded during the previous level, issued after that level's
+// upwind loads, so that only the upwind loads (L2) and the arithmetic lie on
+// the chain of dependent levels (positions beyond those of a wide level are
+// loaded in place).  The level pointers of the direction sit in LDS.  The
+// chunks of one direction take consecutive workgroup ids, which xcd_block
+// places on one XCD, so they share its L2 for the records and for the
+// partial lines of the e rows they write.
+template <typename ST, int BC, int NP, int TPB, int V>
+__global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec, const int *__restrict__ levptr,
                                                   const int *__restrict__ dlev, const ST *__restrict__ S,
                                                   ST *__restrict__ e, Modes m, int nc, int nB, int nD, int smajor) {
+    constexpr int BL = BC / V;   // bands per lane
+    constexpr int PP = TPB / V;  // positions per pass
+    static_assert(BC % V == 0 && TPB % V == 0 && (BL * sizeof(ST)) % 16 == 0, "whole 16-byte vectors per lane");
     extern __shared__ int slp[];  // level pointers of the direction
     int nchunk = nB / BC;
     int id = xcd_block();
     if (id >= nD * nchunk) return;  // whole workgroup
-    int d = id / nchunk, chunk = id - d * nchunk, b0 = chunk * BC;
-    double tv[BC];
+    int d = id / nchunk, chunk = id - d * nchunk;
+    int tid = (int)threadIdx.x / V, sub = (int)threadIdx.x - tid * V;  // position slot, vector of the chunk
+    int b0 = chunk * BC + sub * BL;
+    double tv[BL];
 #pragma unroll
-    for (int i = 0; i < BC; ++i) tv[i] = m.tau[b0 + i] * m.vg[b0 + i];
+    for (int i = 0; i < BL; ++i) tv[i] = m.tau[b0 + i] * m.vg[b0 + i];
     int L = dlev[d + 1] - dlev[d] - 1;
-    for (int i = threadIdx.x; i <= L; i += 256) slp[i] = levptr[dlev[d] + i];
+    for (int i = threadIdx.x; i <= L; i += TPB) slp[i] = levptr[dlev[d] + i];
     __syncthreads();
     const LvRec *rd = rec + (size_t)d * nc;
     // S[dir][chunk][position][BC], or S[dir][position][band] (smajor)
-    const ST *Sd = smajor ? S + (size_t)d * nc * nB + b0 : S + ((size_t)d * nchunk + chunk) * nc * BC;
+    const ST *Sd = smajor ? S + (size_t)d * nc * nB + b0 : S + ((size_t)d * nchunk + chunk) * nc * BC + sub * BL;
     size_t sst = smajor ? (size_t)nB : (size_t)BC;
     size_t DB = (size_t)nD * nB;
     ST *ed = e + (size_t)d * nB + b0;
-    int tid = threadIdx.x;
     // Every load of the pipelined part is unconditional (lanes past the end
     // of a level re-read its last position, absent upwind cells read row 0,
     // both masked afterwards): with no branch around a load, the wait before
     // the products counts exactly this level's upwind loads and leaves the
     // next level's records and sources in flight.
-    auto fetch = [&](int l, LvRec(&rr)[NP], ST(&ss)[NP][BC]) {
+    auto fetch = [&](int l, LvRec(&rr)[NP], ST(&ss)[NP][BL]) {
         int a0 = slp[l], a1 = slp[l + 1];
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
-            int p = min(a0 + tid + i * 256, a1 - 1);
+            int p = min(a0 + tid + i * PP, a1 - 1);
             rr[i] = rd[p];
             vld(ss[i], Sd + (size_t)p * sst);
         }
     };
-    auto upwind = [&](const LvRec &r, ST(&u)[3][BC]) {
+    auto upwind = [&](const LvRec &r, ST(&u)[3][BL]) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) vld(u[j], ed + (size_t)max(r.c[j], 0) * DB);
     };
-    auto solve = [&](const LvRec &r, const ST(&s)[BC], const ST(&u)[3][BC], bool store) {
-        ST en[BC];
+    auto solve = [&](const LvRec &r, const ST(&s)[BL], const ST(&u)[3][BL], bool store) {
+        ST en[BL];
 #pragma unroll
-        for (int b = 0; b < BC; ++b) {
+        for (int b = 0; b < BL; ++b) {
             double v = s[b];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
@@ -687,17 +694,17 @@ __global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec,
     };
     // level l from the buffers (rc, sc); the next level's first positions
     // into (rn, sn) (the last level re-reads its own)
-    auto level = [&](int l, LvRec(&rc)[NP], ST(&sc)[NP][BC], LvRec(&rn)[NP], ST(&sn)[NP][BC]) {
+    auto level = [&](int l, LvRec(&rc)[NP], ST(&sc)[NP][BL], LvRec(&rn)[NP], ST(&sn)[NP][BL]) {
         int a0 = slp[l], a1 = slp[l + 1];
-        ST u[NP][3][BC];
+        ST u[NP][3][BL];
 #pragma unroll
         for (int i = 0; i < NP; ++i) upwind(rc[i], u[i]);
         fetch(l + 1 < L ? l + 1 : l, rn, sn);
 #pragma unroll
-        for (int i = 0; i < NP; ++i) solve(rc[i], sc[i], u[i], a0 + tid + i * 256 < a1);
-        for (int p = a0 + 256 * NP + tid; p < a1; p += 256) {
+        for (int i = 0; i < NP; ++i) solve(rc[i], sc[i], u[i], a0 + tid + i * PP < a1);
+        for (int p = a0 + PP * NP + tid; p < a1; p += PP) {
             LvRec r = rd[p];
-            ST s1[BC], u1[3][BC];
+            ST s1[BL], u1[3][BL];
             vld(s1, Sd + (size_t)p * sst);
             upwind(r, u1);
             solve(r, s1, u1, true);
@@ -705,7 +712,7 @@ __global__ void __launch_bounds__(256) k_sweep_lv(const LvRec *__restrict__ rec,
         __syncthreads();
     };
     LvRec rA[NP], rB[NP];
-    ST sA[NP][BC], sB[NP][BC];
+    ST sA[NP][BL], sB[NP][BL];
     if (L > 0) fetch(0, rA, sA);
     for (int l = 0; l < L; l += 2) {
         level(l, rA, sA, rB, sB);
@@ -1583,22 +1590,38 @@ void Solver::launch_source3(const Modes &m) {
 void Solver::launch_sweep_lv(const Modes &m, int bc) {
     int grid = ((nD * (nB / bc) + kXcd - 1) / kXcd) * kXcd;
     size_t lds = (size_t)(nlev + 1) * sizeof(int);
-#define GBTE_LV(ST_, BC_, NP_)                                                                                      \
-    hipLaunchKernelGGL((k_sweep_lv<ST_, BC_, NP_>), dim3(grid), dim3(256), lds, s_, d_lv_rec.p, d_levptr.p, d_dlev.p, \
-                       reinterpret_cast<const ST_ *>(d_S.p), reinterpret_cast<ST_ *>(d_e.p), m, nc, nB, nD,  \
-                       s_sh < 0 ? 1 : 0)
-#define GBTE_LV_NP(ST_, BC_)                  \
-    do {                                      \
-        if (lv_np == 4) GBTE_LV(ST_, BC_, 4); \
-        else if (lv_np == 2) GBTE_LV(ST_, BC_, 2); \
-        else GBTE_LV(ST_, BC_, 1);            \
+#define GBTE_LV(ST_, BC_, NP_, TPB_, V_)                                                                             \
+    hipLaunchKernelGGL((k_sweep_lv<ST_, BC_, NP_, TPB_, V_>), dim3(grid), dim3(TPB_), lds, s_, d_lv_rec.p,           \
+                       d_levptr.p, d_dlev.p, reinterpret_cast<const ST_ *>(d_S.p), reinterpret_cast<ST_ *>(d_e.p), m, \
+                       nc, nB, nD, s_sh < 0 ? 1 : 0)
+#define GBTE_LV_NP(ST_, BC_)                            \
+    do {                                                \
+        if (lv_np == 4) GBTE_LV(ST_, BC_, 4, 256, 1);   \
+        else if (lv_np == 2) GBTE_LV(ST_, BC_, 2, 256, 1); \
+        else GBTE_LV(ST_, BC_, 1, 256, 1);              \
     } while (0)
-#define GBTE_LV_NP2(ST_, BC_)                      \
-    do {                                           \
-        if (lv_np == 2) GBTE_LV(ST_, BC_, 2);      \
-        else GBTE_LV(ST_, BC_, 1);                 \
+#define GBTE_LV_NP2(ST_, BC_)                           \
+    do {                                                \
+        if (lv_np == 2) GBTE_LV(ST_, BC_, 2, 256, 1);   \
+        else GBTE_LV(ST_, BC_, 1, 256, 1);              \
     } while (0)
-    if (fp32_) {
+    // one 16-byte vector per lane: V = BC * word / 16 lanes per position
+#define GBTE_LVV(ST_, BC_)                                                     \
+    do {                                                                       \
+        constexpr int V_ = BC_ * (int)sizeof(ST_) / 16;                        \
+        if (lv_tpb == 512) {                                                   \
+            if (lv_np == 4) GBTE_LV(ST_, BC_, 4, 512, V_);                     \
+            else GBTE_LV(ST_, BC_, 2, 512, V_);                                \
+        } else {                                                               \
+            if (lv_np == 4) GBTE_LV(ST_, BC_, 4, 256, V_);                     \
+            else GBTE_LV(ST_, BC_, 2, 256, V_);                                \
+        }                                                                      \
+    } while (0)
+    if (lv_v > 1) {
+        if (fp32_) GBTE_LVV(float, 8);
+        else if (bc == 4) GBTE_LVV(double, 4);
+        else GBTE_LVV(double, 8);
+    } else if (fp32_) {
         if (bc == 4) GBTE_LV_NP(float, 4);
         else if (bc == 8) GBTE_LV_NP(float, 8);
         else GBTE_LV_NP2(float, 20);
@@ -1609,6 +1632,7 @@ void Solver::launch_sweep_lv(const Modes &m, int bc) {
         else if (bc == 10) GBTE_LV_NP2(double, 10);
         else GBTE_LV_NP2(double, 20);
     }
+#undef GBTE_LVV
 #undef GBTE_LV_NP2
 #undef GBTE_LV_NP
 #undef GBTE_LV
@@ -1739,10 +1763,11 @@ bool Solver::tune3d() {
         if (!check(hipMemsetAsync(d_S.p, 0, d_S.n * sizeof(double), s_), "hipMemsetAsync")) return false;
         if (!check(hipMemsetAsync(d_e.p, 0, d_e.n * sizeof(double), s_), "hipMemsetAsync")) return false;
         Modes m = modes();
-        struct Cand { int kind, bc, np; };  // kind 0: wave-specialised, 1: level-synchronous, 2: chunked
+        struct Cand { int kind, bc, np, v = 1, tpb = 256; };  // kind 0: wave-specialised, 1: level-synchronous, 2: chunked
         std::vector<Cand> cands;
         bool pc_built = use_pc;
         int want = opt.sweep_kernel;  // GBTE_SWEEP_AUTO: every family
+        if (want < GBTE_SWEEP_AUTO || want > GBTE_SWEEP_LANES) { err_ = "unknown sweep_kernel"; return false; }
         if (pc_built && (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_WAVESPEC)) cands.push_back({0, 0, 0});
         if (!fp32_ && (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_LEVEL)) cands.push_back({1, 0, 0});
         int wsz = fp32_ ? 4 : 8;
@@ -1754,6 +1779,14 @@ bool Solver::tune3d() {
                 (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_CHUNKED))
                 for (int np : {1, 2, 4})
                     if (bc * wsz * np <= 160 && (np <= 2 || (bc & (bc - 1)) == 0)) cands.push_back({2, bc, np});
+        // several lanes per position, one 16-byte vector each (8 bands FP32;
+        // 4 or 8 bands FP64), 256 or 512 threads per workgroup
+        for (int bc : {4, 8})
+            if (d_lv_rec.p && nB % bc == 0 && (fp32_ ? bc == 8 : true) && (nB * wsz) % 16 == 0 &&
+                (size_t)(nlev + 1) * sizeof(int) <= 48 * 1024 &&
+                (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_LANES))
+                for (int tpb : {256, 512})
+                    for (int np : {2, 4}) cands.push_back({2, bc, np, bc * wsz / 16, tpb});
         double best = 1e300;
         Cand pick{-1, 0, 0};
         auto chunk_sh = [](int bc) {  // S per band chunk where the chunk is a power of two
@@ -1765,10 +1798,16 @@ bool Solver::tune3d() {
             use_pc = c.kind == 0;
             lv_bc = c.kind == 2 ? c.bc : 0;
             lv_np = c.kind == 2 ? c.np : 2;
+            lv_v = c.v;
+            lv_tpb = c.tpb;
             s_sh = c.kind == 2 ? chunk_sh(c.bc) : -1;
             double per = 0;
             if (!time_it([&]() { launch_sweep3(m); }, best, per)) return false;
-            if (c.kind == 2)
+            if (c.kind == 2 && c.v > 1)
+                snprintf(line, sizeof line,
+                         "sweep chunked %d bands, %d lanes per position, %d positions per lane, %d threads: %.3f ms per launch\n",
+                         c.bc, c.v, c.np, c.tpb, per);
+            else if (c.kind == 2)
                 snprintf(line, sizeof line, "sweep chunked %d bands, %d positions per thread: %.3f ms per launch\n",
                          c.bc, c.np, per);
             else
@@ -1781,8 +1820,14 @@ bool Solver::tune3d() {
         use_pc = pick.kind == 0;
         lv_bc = pick.kind == 2 ? pick.bc : 0;
         lv_np = pick.kind == 2 ? pick.np : 2;
+        lv_v = pick.v;
+        lv_tpb = pick.tpb;
         s_sh = pick.kind == 2 ? chunk_sh(pick.bc) : -1;
-        if (pick.kind == 2)
+        if (pick.kind == 2 && pick.v > 1)
+            snprintf(line, sizeof line,
+                     "sweep chosen: chunked %d bands, %d lanes per position, %d positions per lane, %d threads\n",
+                     pick.bc, pick.v, pick.np, pick.tpb);
+        else if (pick.kind == 2)
             snprintf(line, sizeof line, "sweep chosen: chunked %d bands, %d positions per thread\n", pick.bc, pick.np);
         else
             snprintf(line, sizeof line, "sweep chosen: %s\n", use_pc ? "wave-specialised" : "level-synchronous");

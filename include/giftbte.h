@@ -132,7 +132,9 @@ typedef struct gbte_options {
                              tuner times every sweep kernel that takes the mesh and
                              keeps the fastest) or one kernel family:
                              GBTE_SWEEP_WAVESPEC, GBTE_SWEEP_LEVEL (FP64 only),
-                             GBTE_SWEEP_CHUNKED; same results to rounding */
+                             GBTE_SWEEP_CHUNKED (a lane takes a position's band
+                             chunk), GBTE_SWEEP_LANES (several lanes per position,
+                             one 16-byte vector each); same results to rounding */
     int gradient_kernel;  /* tetrahedral meshes: GBTE_GRAD_AUTO (default: the set-up tuner
                              keeps the fastest) or GBTE_GRAD_GATHER (stencil gathers on the
                              vector units), GBTE_GRAD_MATRIX (tiles of four cells on the
@@ -141,7 +143,7 @@ typedef struct gbte_options {
                              one material); same results to rounding */
 } gbte_options;
 
-enum { GBTE_SWEEP_AUTO = 0, GBTE_SWEEP_WAVESPEC = 1, GBTE_SWEEP_LEVEL = 2, GBTE_SWEEP_CHUNKED = 3 };
+enum { GBTE_SWEEP_AUTO = 0, GBTE_SWEEP_WAVESPEC = 1, GBTE_SWEEP_LEVEL = 2, GBTE_SWEEP_CHUNKED = 3, GBTE_SWEEP_LANES = 4 };
 enum { GBTE_GRAD_AUTO = 0, GBTE_GRAD_GATHER = 1, GBTE_GRAD_MATRIX = 2, GBTE_GRAD_VERTEX = 3 };
 
 typedef struct gbte_stats {

@@ -29,7 +29,8 @@ def _keff_x(p, q, dT=2.0):
 # FP64 only)
 SWEEPS = [("cube3d_b8", "fp64", "wavespec", "auto"), ("cube3d_b8", "fp64", "level", "auto"),
           ("cube3d_b8", "fp64", "chunked", "auto"), ("cube3d_b8", "fp32", "wavespec", "auto"),
-          ("cube3d_b8", "fp32", "chunked", "auto")]
+          ("cube3d_b8", "fp32", "chunked", "auto"), ("cube3d_b8", "fp64", "lanes", "auto"),
+          ("cube3d_b8", "fp32", "lanes", "auto")]
 # each gradient kernel, forced, on both decks and precisions
 GRADS = [(n, pr, "auto", g) for n in DECKS3D for pr in ("fp64", "fp32") for g in ("gather", "matrix", "vertex")]
 
