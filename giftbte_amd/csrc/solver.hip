@@ -2752,6 +2752,10 @@ bool Solver::build() {
         !alloc(d_tmac, ne) || !alloc(d_rhs, nc) || !alloc(d_r, nc) || !alloc(d_z, nc) || !alloc(d_p, nc) ||
         !alloc(d_Ap, nc) || !alloc(d_S, half(nE)))
         return false;
+    // tetrahedra: the gradient kernels write a face projection only where it
+    // is read (the out-faces per direction); the other entries stay zero
+    if (dim == 3 && !check(hipMemsetAsync(d_grad.p, 0, d_grad.n * sizeof(double), s_), "zero face projections"))
+        return false;
     if (use_pc && dim == 2 && !alloc(d_tpos, (size_t)nD * nc)) return false;
     if (opt.linear_solver == GBTE_LINSOLVE_DIRECT &&
         (!alloc(d_ndy, nc) || !alloc(d_ndx, nc) || !alloc(d_ndu, (size_t)std::max<long>(1, std::max(NDmacro.u_total, NDfour.u_total)))))

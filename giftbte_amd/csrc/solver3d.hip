@@ -121,16 +121,28 @@ __device__ __forceinline__ void g3_entries(const ST *__restrict__ X, const int *
     }
 }
 
-// phi_f = r_f . g for the four faces of cell c (NaN -> 0, the reference's
-// zero gradient)
+// A face projection phi_f is read only where the face is an out-face of
+// the cell for the entry's direction (s . n_f >= 0: by the cell's own source
+// and outgoing boundary value, and by the downwind neighbour's source), so
+// it is written only there: about half of the four rows.  The write is
+// skipped only where s . n_f < -kInFace, where every reader's rounded
+// v s . n is negative too; the rows are zeroed once at set-up.
+constexpr double kInFace = 1e-12;
+
+// phi_f = r_f . g for the out-faces of cell c in the direction of entry t
+// (NaN -> 0, the reference's zero gradient)
 template <typename ST>
 __device__ __forceinline__ void g3_store_phi(ST *__restrict__ phi, const double *__restrict__ rc, int c, int t, int DB,
-                                             double x, double y, double w) {
+                                             int nB, const Modes &m, double x, double y, double w) {
     if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
+    int d = t / nB;
+    double sx = m.sx[d], sy = m.sy[d], sz = m.sz[d];
     ST *pc = phi + (size_t)c * 4 * DB + t;
 #pragma unroll
     for (int f = 0; f < 4; ++f)
-        __builtin_nontemporal_store((ST)(rc[f * 10 + 4] * x + rc[f * 10 + 5] * y + rc[f * 10 + 6] * w), pc + f * DB);
+        if (sx * rc[f * 10] + sy * rc[f * 10 + 1] + sz * rc[f * 10 + 2] >= -kInFace)
+            __builtin_nontemporal_store((ST)(rc[f * 10 + 4] * x + rc[f * 10 + 5] * y + rc[f * 10 + 6] * w),
+                                        pc + f * DB);
 }
 
 // _get_gradient_larger, 3D (StaticBTESynthetic.cpp:1329-1371):
@@ -148,7 +160,7 @@ template <typename ST, int GS, int WPB>
 __global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, ST *__restrict__ phi,
                                                      const int *__restrict__ sptr, const int *__restrict__ gidx,
                                                      const G3Ent<ST> *__restrict__ ent, const double *__restrict__ rec,
-                                                     int nc, int DB, int ngroups) {
+                                                     Modes m, int nc, int DB, int nB, int ngroups) {
     int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     int sg = wv / nc, c = wv - sg * nc;
     if (sg >= ngroups) return;  // whole wave
@@ -167,7 +179,7 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, S
 #pragma unroll
     for (int q = 0; q < GS; ++q) {
         int t = tb + q * kG3W;
-        if (t < DB) g3_store_phi(phi, rc, c, t, DB, gx[q], gy[q], gz[q]);
+        if (t < DB) g3_store_phi(phi, rc, c, t, DB, nB, m, gx[q], gy[q], gz[q]);
     }
 }
 
@@ -233,7 +245,7 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3v(const ST *__restrict__ e, c
                                                      ST *__restrict__ phi, const G3VRec<ST> *__restrict__ vrec,
                                                      const int *__restrict__ sptr, const int *__restrict__ gidx,
                                                      const G3Ent<ST> *__restrict__ ent, const double *__restrict__ rec,
-                                                     int nc, int DB, int ngroups) {
+                                                     Modes m, int nc, int DB, int nB, int ngroups) {
     using Q4 = typename std::conditional<sizeof(ST) == 4, v4f, v4d>::type;
     int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     int sg = wv / nc, c = wv - sg * nc;
@@ -272,7 +284,7 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3v(const ST *__restrict__ e, c
 #pragma unroll
     for (int q = 0; q < GS; ++q) {
         int t = tb + q * kG3W;
-        if (t < DB) g3_store_phi(phi, rc, c, t, DB, gx[q], gy[q], gz[q]);
+        if (t < DB) g3_store_phi(phi, rc, c, t, DB, nB, m, gx[q], gy[q], gz[q]);
     }
 }
 
@@ -306,7 +318,8 @@ constexpr int kGmKS = 40;  // K-steps per tile (at most 160 rows)
 template <typename ST, int NB>
 __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__restrict__ phi,
                                                const int *__restrict__ trows, const int *__restrict__ tcell,
-                                               const ST *__restrict__ tA, int n_tiles, int DB, int spg) {
+                                               const ST *__restrict__ tA, const double *__restrict__ rec, Modes m,
+                                               int n_tiles, int DB, int nB, int spg) {
     using BV = typename std::conditional<NB == 2, typename std::conditional<sizeof(ST) == 4, v2f, v2d>::type, ST>::type;
     // wave = (slice group, tile), tiles fastest: the waves that run together
     // work on the same slice group, so the rows that neighbouring tiles share
@@ -328,6 +341,13 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
     auto row = [&](int s) { return srow[4 * s + q]; };
     int cell = tcell[tile * 4 + q];
     int ref = trows[(size_t)tile * kGmKS * 4];
+    // the lane's cell's face normals (the out-faces per direction)
+    double fnx[4], fny[4], fnz[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const double *r = rec + (size_t)(cell >= 0 ? cell : 0) * kS3D + f * 10;
+        fnx[f] = r[0]; fny[f] = r[1]; fnz[f] = r[2];
+    }
     // lane's first entry of a slice, clamped to a whole vector inside the row
     auto ent0 = [&](int s0) {
         int t = s0 + NB * n;
@@ -407,6 +427,16 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
 #pragma unroll
             for (int j = 0; j < NB; ++j) bad[j] = isnan(v[j][0]) || isnan(v[j][1]) || isnan(v[j][2]) || isnan(v[j][3]);
             ST *pc = phi + (size_t)cell * 4 * DB + t;
+            // out-faces only (as g3_store_phi); a pair shares its direction
+            // when the bands per direction are even, else both entries'
+            bool out[NB][4];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                int d = min(t + j, DB - 1) / nB;
+                double sx = m.sx[d], sy = m.sy[d], sz = m.sz[d];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) out[j][i] = sx * fnx[i] + sy * fny[i] + sz * fnz[i] >= -kInFace;
+            }
             bool pair = false;
             if constexpr (NB == 2) {
                 // both entries of the pair in one store per face
@@ -414,6 +444,7 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
                 if (pair)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
+                        if (!out[0][i] && !out[1][i]) continue;
                         BV o;
                         o[0] = (ST)(bad[0] ? 0.0 : v[0][i]);
                         o[1] = (ST)(bad[1] ? 0.0 : v[1][i]);
@@ -426,7 +457,7 @@ __global__ void __launch_bounds__(64) k_grad3m(const ST *__restrict__ e, ST *__r
                     if (t + j >= DB) continue;
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        __builtin_nontemporal_store((ST)(bad[j] ? 0.0 : v[j][i]), pc + j + i * DB);
+                        if (out[j][i]) __builtin_nontemporal_store((ST)(bad[j] ? 0.0 : v[j][i]), pc + j + i * DB);
                 }
         }
     }
@@ -1503,6 +1534,7 @@ void Solver::launch_rhs3(bool macro, const double *bval, double *rhs) {
 // the solver's stream.
 void Solver::launch_grad3() {
     int DB = nD * nB;
+    Modes md = modes();
     if (use_gm) {
         // slice groups of 256 entries
         int nb = fp32_ && DB % 2 == 0 ? 2 : 1, spg = 256 / (16 * nb);
@@ -1513,14 +1545,15 @@ void Solver::launch_grad3() {
         if (fp32_ && DB % 2 == 0)
             hipLaunchKernelGGL((k_grad3m<float, 2>), dim3(grid), dim3(64), 0, s_, reinterpret_cast<const float *>(d_e.p),
                                reinterpret_cast<float *>(d_grad.p), d_gm_rows.p, d_gm_cell.p,
-                               reinterpret_cast<const float *>(d_gm_A.p), gm_ntiles, DB, spg);
+                               reinterpret_cast<const float *>(d_gm_A.p), d_s3d.p, md, gm_ntiles, DB, nB, spg);
         else if (fp32_)
             hipLaunchKernelGGL((k_grad3m<float, 1>), dim3(grid), dim3(64), 0, s_, reinterpret_cast<const float *>(d_e.p),
                                reinterpret_cast<float *>(d_grad.p), d_gm_rows.p, d_gm_cell.p,
-                               reinterpret_cast<const float *>(d_gm_A.p), gm_ntiles, DB, spg);
+                               reinterpret_cast<const float *>(d_gm_A.p), d_s3d.p, md, gm_ntiles, DB, nB, spg);
         else
             hipLaunchKernelGGL((k_grad3m<double, 1>), dim3(grid), dim3(64), 0, s_, d_e.p, d_grad.p, d_gm_rows.p,
-                               d_gm_cell.p, reinterpret_cast<const double *>(d_gm_A.p), gm_ntiles, DB, spg);
+                               d_gm_cell.p, reinterpret_cast<const double *>(d_gm_A.p), d_s3d.p, md, gm_ntiles, DB,
+                               nB, spg);
         return;
     }
     int ngroups = (DB + g3_gs * kG3W - 1) / (g3_gs * kG3W);
@@ -1536,8 +1569,8 @@ void Solver::launch_grad3() {
         hipLaunchKernelGGL((k_grad3v<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                           \
                            reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<const ST_ *>(d_gv_sums.p),     \
                            reinterpret_cast<ST_ *>(d_grad.p), reinterpret_cast<const G3VRec<ST_> *>(d_gv_rec.p), \
-                           d_gv_ptr.p, d_gv_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_gv_ent.p), d_s3d.p, nc, \
-                           DB, ngroups);                                                                         \
+                           d_gv_ptr.p, d_gv_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_gv_ent.p), d_s3d.p, md, \
+                           nc, DB, nB, ngroups);                                                                 \
     } while (0)
 #define GBTE_G3(ST_, GS_, W_)                                                                                    \
     do {                                                                                                         \
@@ -1547,7 +1580,7 @@ void Solver::launch_grad3() {
             hipLaunchKernelGGL((k_grad3g<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                       \
                                reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<ST_ *>(d_grad.p),          \
                                d_g3g_ptr.p, d_g3g_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_g3g_ent.p),      \
-                               d_s3d.p, nc, DB, ngroups);                                                        \
+                               d_s3d.p, md, nc, DB, nB, ngroups);                                                \
     } while (0)
 #define GBTE_G3_W(ST_, GS_)                         \
     do {                                            \
