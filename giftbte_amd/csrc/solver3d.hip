@@ -8,7 +8,10 @@
 //                 upwind sweep (or k_sweep_pc / k_sweep_dir of solver.hip;
 //                 the set-up tuner picks the fastest)  StaticBTESyntheticSolver.cpp:96-100
 //   k_bound_out3  _get_bound_ee                                      :1502-1539
-//   k_grad3g      the next step's _get_gradient_larger (3D branch)   :1329-1371
+//   k_vsum3 +     the next step's _get_gradient_larger (3D branch)   :1329-1371
+//   k_grad3v      through sums over the mesh vertices (or k_grad3g, the
+//                 direct stencil gather, or k_grad3m on the matrix cores;
+//                 the set-up tuner picks the fastest)
 //   k_mom3        per-row moments of e and of the boundary values: T, q and
 //                 the seven direction factors of the non-Fourier face terms
 //   k_vert3       _set_vertex_energydensity on those moments         :1541-1616
@@ -25,11 +28,13 @@
 //
 // The LSQ gradient of a tetrahedron reads its whole vertex-neighbour stencil
 // (about 80 cells on a Delaunay mesh), so every row is re-read by about 80
-// gradients; k_grad3g reads them from L2, where the waves of
-// Morton-consecutive cells find largely the same rows.  (Staging the rows of
-// a tile of cells in LDS was measured slower: the tile's stencil union is
-// about 190 rows for 8 cells, and the per-entry weights then have to reach
-// every lane from LDS or scalar loads.)
+// gradients.  k_grad3g gathers them from L2, where the waves of
+// Morton-consecutive cells find largely the same rows; k_vsum3 + k_grad3v
+// first sum each vertex's cells once and then read four vertex sums and
+// about 20 rows per cell.  (Staging the rows of a tile of cells in LDS was
+// measured slower: the tile's stencil union is about 190 rows for 8 cells,
+// and the per-entry weights then have to reach every lane from LDS or scalar
+// loads.)
 //
 // The non-Fourier residual is linear in the face values, so the per-system
 // sums over faces factor through per-row moments Y_j = sum_sys K_j(sys) e:
