@@ -1944,7 +1944,7 @@ bool Solver::build() {
     std::string phases;
     auto mark = [&](const char *what) {
         auto t = std::chrono::steady_clock::now();
-        char b[96];
+        char b[256];
         snprintf(b, sizeof b, "%s%s %.2f s", phases.empty() ? "" : ", ", what,
                  std::chrono::duration<double>(t - tp).count());
         phases += b;
@@ -2575,6 +2575,7 @@ bool Solver::build() {
             !upload(d_be_slot, be_slot))  // face slot per entry, for k_bound_out
             return false;
     }
+    mark("gradient, vertex, face and boundary tables");
     // ---- macro (set 0) and Fourier (set 1) right-hand sides and matrices
     {
         double kap = p.kappa_bulk[0];
@@ -2658,10 +2659,13 @@ bool Solver::build() {
         if (direct) th.join();
         if (!okM || !build_matrix(rowsF, Kfour, NDfour, direct ? &Ff : nullptr, direct ? &ef : nullptr)) return false;
     }
-    mark("gradient, macro and Fourier tables and factors");
+    mark("macro and Fourier matrices and factors");
     // ---- 3D tables (solver3d.hip)
     if (dim == 3 && !build3d(be_slot)) return false;
-    if (dim == 3) mark("3D tables");
+    if (dim == 3) {
+        std::string what = "3D tables (" + phase3d_ + ")";
+        mark(what.c_str());
+    }
     // ---- LDS tiles of the face / vertex / moment kernel (2D)
     if (dim == 2) {
         Tiles T;

@@ -253,6 +253,7 @@ class Solver {
     DBuf<int> d_gv_vptr, d_gv_vidx, d_gv_vref, d_gv_ptr, d_gv_idx;
     DBuf<char> d_gv_vent, d_gv_ent, d_gv_rec, d_gv_sums;
     std::string tune_log_;  // the autotuner's candidates, timings and choices
+    std::string phase3d_;   // build3d's host phase times
     DBuf<double> d_ndy, d_ndu, d_ndx;
     // state
     DBuf<double> d_e, d_grad, d_eb, d_eV, d_T, d_Told, d_T1, d_TV, d_E, d_qx, d_qy, d_qz, d_qxo, d_qyo, d_qzo,

@@ -51,6 +51,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <limits>
@@ -683,6 +684,7 @@ __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec,
     if (id >= nD * nchunk) return;  // whole workgroup
     int d = id / nchunk, chunk = id - d * nchunk;
     int tid = (int)threadIdx.x / V, sub = (int)threadIdx.x - tid * V;  // position slot, vector of the chunk
+    int wpos = __builtin_amdgcn_readfirstlane(tid);                      // the wave's first position slot
     int b0 = chunk * BC + sub * BL;
     double tv[BL];
 #pragma unroll
@@ -736,8 +738,11 @@ __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec,
 #pragma unroll
         for (int i = 0; i < NP; ++i) upwind(rc[i], u[i]);
         fetch(l + 1 < L ? l + 1 : l, rn, sn);
+        // a wave none of whose positions lies in the level skips the
+        // arithmetic (a uniform branch with no load in it)
 #pragma unroll
-        for (int i = 0; i < NP; ++i) solve(rc[i], sc[i], u[i], a0 + tid + i * PP < a1);
+        for (int i = 0; i < NP; ++i)
+            if (a0 + wpos + i * PP < a1) solve(rc[i], sc[i], u[i], a0 + tid + i * PP < a1);
         for (int p = a0 + PP * NP + tid; p < a1; p += PP) {
             LvRec r = rd[p];
             ST s1[BL], u1[3][BL];
@@ -968,6 +973,16 @@ void face_lsq(const Problem &p, size_t s, double invLx, std::vector<double> &P) 
 // be_slot: boundary entry -> face slot.
 bool Solver::build3d(const std::vector<int> &be_slot) {
     const Problem &p = P;
+    auto t0 = std::chrono::steady_clock::now();
+    phase3d_.clear();
+    auto mark = [&](const char *what) {
+        auto t = std::chrono::steady_clock::now();
+        char b[96];
+        snprintf(b, sizeof b, "%s%s %.2f s", phase3d_.empty() ? "" : ", ", what,
+                 std::chrono::duration<double>(t - t0).count());
+        phase3d_ += b;
+        t0 = t;
+    };
     size_t ns = (size_t)nc * 6;
     double invLx = 1.0 / p.deck.Lx;
     for (int c = 0; c < nc; ++c)
@@ -1047,6 +1062,7 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
         qa.insert(qa.end(), q[2].begin(), q[2].end());
         if (!upload(d_gptr, qptr) || !upload(d_gidx, qidx) || !upload(d_gP, qa) || !upload(d_gok, gok)) return false;
     }
+    mark("source records and LSQ weights");
     // ---- gradient stencil tables: global neighbour rows, padded per cell to
     // whole batches with the cell's own row at weight 0 (each adds an exact
     // zero)
@@ -1094,6 +1110,7 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 return false;
         }
         if (!upload(d_g3g_ptr, gptr2) || !upload(d_g3g_idx, gidx2)) return false;
+        mark("gather tables");
         // ---- matrix-core tiles (k_grad3m): up to four consecutive cells while
         // the rows they need fit kGmKS K-steps
         {
@@ -1187,10 +1204,12 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
             }
         }
     }
+    mark("matrix-core tiles");
     // ---- vertex-sum gradient (k_vsum3 + k_grad3v): the cells around each
     // vertex, one chosen cell r_v per vertex, per-vertex and per-cell tables
     gv_ok = build_gv(invLx);
     if (!err_.empty()) return false;
+    mark("vertex sums");
     // ---- moments and the residual's face coefficients
     std::vector<std::vector<double>> FM(nf);  // per unique face, from its first side: 3 x nv
     for (int f = 0; f < nf; ++f) {
@@ -1317,6 +1336,7 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
         }
     }
     (void)be_slot;
+    mark("face and right-hand-side tables");
     return true;
 }
 
