@@ -35,6 +35,18 @@ def _run(cmd, cwd=ROOT):
     subprocess.run(cmd, cwd=cwd, check=True)
 
 
+# sources with no HIP code, compiled by the host compiler alone
+HOST_ONLY = {"ndchol.cpp"}
+
+
+def host_cxx() -> str:
+    """The clang++ that hipcc drives for host code."""
+    for c in ("/opt/rocm/llvm/bin/clang++", shutil.which("clang++") or "", shutil.which("g++") or ""):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("no host C++ compiler")
+
+
 def hipcc() -> str:
     for c in ("/opt/rocm/bin/hipcc", shutil.which("hipcc") or ""):
         if c and os.path.exists(c):
@@ -52,7 +64,12 @@ def build_lib() -> str:
     for s in srcs:
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, s + ".o")
-        cmd = [hipcc()] + HIP_FLAGS + ["-I", INC, "-I", CSRC, "-c", src, "-o", obj]
+        if s in HOST_ONLY:
+            # plain host C++ (no device pass): its dense kernels are built for
+            # two host instruction sets and picked at load time
+            cmd = [host_cxx(), "-O3", "-std=c++17", "-fPIC", "-I", CSRC, "-c", src, "-o", obj]
+        else:
+            cmd = [hipcc()] + HIP_FLAGS + ["-I", INC, "-I", CSRC, "-c", src, "-o", obj]
         _run(cmd)
         objs.append(obj)
     tmp = LIB + ".tmp"

@@ -28,7 +28,10 @@ namespace gbte {
 namespace {
 
 constexpr int kLeaf = 96;
-constexpr int kPanel = 32;  // pivot columns per trailing update of a front
+constexpr int kPanel = 32;    // pivot columns per trailing update of a front
+constexpr int kGroup = 8;     // columns updated together (one pass over a pivot column); the
+                              // trailing update's 8-way kernel assumes 8
+constexpr int kRowTile = 256; // rows of -L_A Li per cache tile
 
 // body(j) for j in [lo, hi) on the host's cores when the work is large
 // (columns are independent); the split balances sum (len - j).
@@ -61,6 +64,146 @@ void parallel_cols(int lo, int hi, int len, Body body) {
                 for (int j = cut[q]; j < cut[q + 1]; ++j) body(j);
             });
     for (auto &x : th) x.join();
+}
+
+// body(g) for g in [0, n) on the host's cores when the total work (about
+// `work` multiply-adds) is large; contiguous ranges of g per thread
+template <typename Body>
+void parallel_groups(int n, double work, Body body) {
+    unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    int nth = t_serial_cols ? 1 : (int)std::min<double>(std::min<double>(hw, n), std::max(1.0, work / 200000.0));
+    if (nth <= 1) {
+        for (int g = 0; g < n; ++g) body(g);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int q = 0; q < nth; ++q)
+        th.emplace_back([&, q]() {
+            for (int g = q; g < n; g += nth) body(g);
+        });
+    for (auto &x : th) x.join();
+}
+
+// One group of kGroup trailing columns j0.. of a front (column-major, m
+// rows) updated by the pivot columns a0..a1-1: C[i][j] -= C[i][a] C[j][a],
+// in register blocks of 4 rows x kGroup columns that take all pivot columns
+// before they are stored (each entry still takes the pivots in increasing
+// order, so the result does not depend on the blocking).  Built for the
+// host's vector units where they have AVX2 / FMA-class instructions, with a
+// baseline copy chosen at load time otherwise.
+__attribute__((target_clones("arch=x86-64-v3", "default"))) void trailing_group(double *Cp, int m, int a0, int a1,
+                                                                               int j0) {
+    int nj = std::min(kGroup, m - j0), np = a1 - a0;
+    double *cj[kGroup];
+    for (int q = 0; q < kGroup; ++q) cj[q] = Cp + (size_t)(j0 + std::min(q, nj - 1)) * m;
+    if (nj < kGroup) {
+        for (int a = a0; a < a1; ++a) {
+            const double *ca = Cp + (size_t)a * m;
+            for (int q = 0; q < nj; ++q) {
+                double lj = ca[j0 + q];
+                for (int i = j0; i < m; ++i) cj[q][i] -= ca[i] * lj;
+            }
+        }
+        return;
+    }
+    // the pivots' entries on the group's rows, [a][q]
+    double lt[kPanel][kGroup];
+    for (int a = 0; a < np; ++a)
+        for (int q = 0; q < kGroup; ++q) lt[a][q] = Cp[(size_t)(a0 + a) * m + j0 + q];
+    int i = j0;
+    for (; i + 4 <= m; i += 4) {
+        double acc[4][kGroup];
+        for (int q = 0; q < kGroup; ++q)
+            for (int r = 0; r < 4; ++r) acc[r][q] = cj[q][i + r];
+        for (int a = 0; a < np; ++a) {
+            const double *ca = Cp + (size_t)(a0 + a) * m + i;
+            double v0 = ca[0], v1 = ca[1], v2 = ca[2], v3 = ca[3];
+            for (int q = 0; q < kGroup; ++q) {
+                double l = lt[a][q];
+                acc[0][q] -= v0 * l;
+                acc[1][q] -= v1 * l;
+                acc[2][q] -= v2 * l;
+                acc[3][q] -= v3 * l;
+            }
+        }
+        for (int q = 0; q < kGroup; ++q)
+            for (int r = 0; r < 4; ++r) cj[q][i + r] = acc[r][q];
+    }
+    for (; i < m; ++i)
+        for (int q = 0; q < kGroup; ++q) {
+            double c = cj[q][i];
+            for (int a = 0; a < np; ++a) c -= Cp[(size_t)(a0 + a) * m + i] * lt[a][q];
+            cj[q][i] = c;
+        }
+}
+
+// Columns col0.. (kGroup of them) of inv(L_kk) by forward substitution,
+// L_kk the first nc columns of a front (column-major, m rows); each column
+// is computed exactly as alone (entries in increasing j), the group only
+// sharing the reads of L.
+__attribute__((target_clones("arch=x86-64-v3", "default"))) void inverse_group(const double *Cp, int m, int nc,
+                                                                              double *Li, int col0) {
+    int ng = std::min(kGroup, nc - col0);
+    double *x[kGroup];
+    for (int q = 0; q < kGroup; ++q) x[q] = Li + (size_t)(col0 + std::min(q, ng - 1)) * nc;
+    for (int q = 0; q < ng; ++q) x[q][col0 + q] = 1.0;
+    for (int j = col0; j < nc; ++j) {
+        const double *cj = Cp + (size_t)j * m;
+        int act = std::min(ng, j - col0 + 1);  // columns col0 .. col0 + act - 1 have reached row j
+        double xj[kGroup];
+        for (int q = 0; q < act; ++q) {
+            double *xq = x[q];
+            xq[j] = j == col0 + q ? 1.0 / cj[j] : xq[j] / cj[j];
+            xj[q] = xq[j];
+        }
+        if (act == kGroup) {
+            double *x0 = x[0], *x1 = x[1], *x2 = x[2], *x3 = x[3], *x4 = x[4], *x5 = x[5], *x6 = x[6], *x7 = x[7];
+            for (int i = j + 1; i < nc; ++i) {
+                double c = cj[i];
+                x0[i] -= c * xj[0]; x1[i] -= c * xj[1]; x2[i] -= c * xj[2]; x3[i] -= c * xj[3];
+                x4[i] -= c * xj[4]; x5[i] -= c * xj[5]; x6[i] -= c * xj[6]; x7[i] -= c * xj[7];
+            }
+        } else {
+            for (int q = 0; q < act; ++q) {
+                double *xq = x[q];
+                for (int i = j + 1; i < nc; ++i) xq[i] -= cj[i] * xj[q];
+            }
+        }
+    }
+}
+
+// Rows nc.. of F = -L_A inv(L_kk) for the kGroup columns a0g.., in tiles of
+// kRowTile rows: each L_A column tile is read once per group, and each
+// entry sums its terms in increasing j (the zeros of inv(L_kk) above its
+// diagonal are skipped, as alone).
+__attribute__((target_clones("arch=x86-64-v3", "default"))) void lali_group(const double *Cp, int m, int nc, int nr,
+                                                                           const double *Li, double *Fr, int a0g) {
+    int na = std::min(kGroup, nc - a0g);
+    double acc[kGroup][kRowTile];
+    for (int r0 = 0; r0 < nr; r0 += kRowTile) {
+        int rn = std::min(kRowTile, nr - r0);
+        for (int q = 0; q < kGroup; ++q)
+            for (int r = 0; r < rn; ++r) acc[q][r] = 0.0;
+        for (int j = a0g; j < nc; ++j) {
+            const double *cj = Cp + (size_t)j * m + nc + r0;
+            if (na == kGroup && j >= a0g + kGroup - 1) {
+                double l[kGroup];
+                for (int q = 0; q < kGroup; ++q) l[q] = Li[(size_t)(a0g + q) * nc + j];
+                for (int r = 0; r < rn; ++r) {
+                    double c = cj[r];
+                    for (int q = 0; q < kGroup; ++q) acc[q][r] += c * l[q];
+                }
+            } else {
+                for (int q = 0; q < na; ++q) {
+                    if (j < a0g + q) continue;
+                    double l = Li[(size_t)(a0g + q) * nc + j];
+                    for (int r = 0; r < rn; ++r) acc[q][r] += cj[r] * l;
+                }
+            }
+        }
+        for (int q = 0; q < na; ++q)
+            for (int r = 0; r < rn; ++r) Fr[(size_t)(nc + r0 + r) * nc + a0g + q] = -acc[q][r];
+    }
 }
 
 struct Dissect {
@@ -223,13 +366,19 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
             std::vector<int> mp(nrc);
             for (int i = 0; i < nrc; ++i) mp[i] = loc(rows[c][i]);
             std::copy(mp.begin(), mp.end(), F.map.begin() + map_of_node[c]);
+            // the child's update matrix, lower triangle column-major; each
+            // front entry receives at most one term from a child, so the
+            // order of the loops does not change the sums
             const std::vector<double> &Uc = U[c];
-            for (int i = 0; i < nrc; ++i)
-                for (int j = 0; j <= i; ++j) {
-                    int li = mp[i], lj = mp[j];
-                    if (li >= lj) C[(size_t)lj * m + li] += Uc[(size_t)i * nrc + j];
-                    else C[(size_t)li * m + lj] += Uc[(size_t)i * nrc + j];
+            for (int j = 0; j < nrc; ++j) {
+                int lj = mp[j];
+                const double *uj = &Uc[(size_t)j * nrc];
+                for (int i = j; i < nrc; ++i) {
+                    int li = mp[i];
+                    if (li >= lj) C[(size_t)lj * m + li] += uj[i];
+                    else C[(size_t)li * m + lj] += uj[i];
                 }
+            }
             std::vector<double>().swap(U[c]);
         }
         // partial dense Cholesky of the first nc_ columns, right-looking in
@@ -251,55 +400,39 @@ std::string nd_factor(const std::vector<std::vector<std::pair<int, double>>> &A,
                     for (int i = j; i < m; ++i) cj[i] -= ca[i] * lj;
                 }
             }
-            parallel_cols(a1, m, m, [&](int j) {
-                double *cj = &C[(size_t)j * m];
-                for (int a = a0; a < a1; ++a) {
-                    const double *ca = &C[(size_t)a * m];
-                    double lj = ca[j];
-                    if (lj == 0) continue;
-                    for (int i = j; i < m; ++i) cj[i] -= ca[i] * lj;
-                }
+            // trailing columns in groups of kGroup (rows above a column's
+            // diagonal lie in the unused upper triangle)
+            parallel_groups((m - a1 + kGroup - 1) / kGroup, 0.5 * (double)(m - a1) * (m - a1) * (a1 - a0), [&](int g) {
+                trailing_group(C.data(), m, a0, a1, a1 + g * kGroup);
             });
         }
         // inverse of L_kk (lower triangular), column-major: Li[a * nc_ + i];
         // column a by forward substitution, the terms of each entry in
         // increasing order
         Li.assign((size_t)nc_ * nc_, 0.0);
-        parallel_cols(0, nc_, nc_, [&](int col) {
-            double *x = &Li[(size_t)col * nc_];
-            x[col] = 1.0;
-            for (int j = col; j < nc_; ++j) {
-                const double *cj = &C[(size_t)j * m];
-                if (j == col) {
-                    x[j] = 1.0 / cj[j];
-                } else {
-                    x[j] = x[j] / cj[j];
-                }
-                double xj = x[j];
-                for (int i = j + 1; i < nc_; ++i) x[i] -= cj[i] * xj;
-            }
-        });
+        parallel_groups((nc_ + kGroup - 1) / kGroup, (double)nc_ * nc_ * nc_ / 6.0,
+                        [&](int g) { inverse_group(C.data(), m, nc_, Li.data(), g * kGroup); });
         // F = [Li; -L_A Li]
         double *Fr = F.frm.data() + F.f_off[k];
         for (int i = 0; i < nc_; ++i)
             for (int a = 0; a <= i; ++a) Fr[(size_t)i * nc_ + a] = Li[(size_t)a * nc_ + i];
-        parallel_cols(0, nc_, nc_, [&](int a) {
-            std::vector<double> acc(nr, 0.0);
-            const double *la = &Li[(size_t)a * nc_];
-            for (int j = a; j < nc_; ++j) {
-                double lv = la[j];
-                const double *cj = &C[(size_t)j * m + nc_];
-                for (int r = 0; r < nr; ++r) acc[r] += cj[r] * lv;
-            }
-            for (int r = 0; r < nr; ++r) Fr[(size_t)(nc_ + r) * nc_ + a] = -acc[r];
-        });
+        // -L_A Li in groups of kGroup columns
+        parallel_groups((nc_ + kGroup - 1) / kGroup, 0.5 * (double)nc_ * nc_ * nr,
+                        [&](int g) { lali_group(C.data(), m, nc_, nr, Li.data(), Fr, g * kGroup); });
+        // the same F column-major, transposed in 32 x 32 tiles
         double *Fc = F.fcm.data() + F.f_off[k];
-        for (int i = 0; i < m; ++i)
-            for (int a = 0; a < nc_; ++a) Fc[(size_t)a * m + i] = Fr[(size_t)i * nc_ + a];
+        parallel_groups((m + 31) / 32, 4.0 * (double)m * nc_, [&](int ti) {
+            int i0 = ti * 32, i1 = std::min(m, i0 + 32);
+            for (int a0t = 0; a0t < nc_; a0t += 32) {
+                int a1t = std::min(nc_, a0t + 32);
+                for (int i = i0; i < i1; ++i)
+                    for (int a = a0t; a < a1t; ++a) Fc[(size_t)a * m + i] = Fr[(size_t)i * nc_ + a];
+            }
+        });
         U[k].assign((size_t)nr * nr, 0.0);
         for (int j = 0; j < nr; ++j) {
             const double *cj = &C[(size_t)(nc_ + j) * m + nc_];
-            for (int i = j; i < nr; ++i) U[k][(size_t)i * nr + j] = cj[i];
+            std::copy(cj + j, cj + nr, U[k].begin() + (size_t)j * nr + j);
         }
         return std::string();
     };
