@@ -914,6 +914,22 @@ __global__ void k_rhs3(const double *__restrict__ RM, const double *__restrict__
     rhs[c] = -r;
 }
 
+// fn(lo, hi) over [0, n) in contiguous blocks on up to 16 host threads
+template <typename F>
+void host_blocks(int n, F fn) {
+    int nth = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(), (n + 1023) / 1024}));
+    if (nth <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int w = 0; w < nth; ++w) {
+        int lo = (int)((long)n * w / nth), hi = (int)((long)n * (w + 1) / nth);
+        th.emplace_back([&fn, lo, hi]() { fn(lo, hi); });
+    }
+    for (auto &x : th) x.join();
+}
+
 // (J^T J)^{-1} of a J with three columns (Get_inverse_3D,
 // src/utility/utility.cpp:488-548: the adjugate over the determinant)
 void jtj_inv(const std::vector<double> &J, int m, double I[3][3]) {
@@ -993,7 +1009,9 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     {
         std::vector<double> &rec = rec_h;
         std::vector<int> reci((size_t)nc * kS3I, 0);
-        for (int c = 0; c < nc; ++c)
+        std::vector<char> bad(nc, 0);
+        host_blocks(nc, [&](int lo, int hi) {
+        for (int c = lo; c < hi; ++c)
             for (int f = 0; f < 4; ++f) {
                 size_t s = (size_t)c * 6 + f;
                 double *r = &rec[(size_t)c * kS3D + f * 10];
@@ -1008,15 +1026,22 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                     int fn = -1;
                     for (int k = 0; k < 4; ++k)
                         if (p.face_neighbor[(size_t)n * 6 + k] == c && p.face_bound[(size_t)n * 6 + k] == -1) fn = k;
-                    if (fn < 0) { err_ = "internal error: interior face without its neighbour's side"; return false; }
+                    if (fn < 0) { bad[c] = 1; continue; }
                     reci[(size_t)c * kS3I + f] = n * 4 + fn;
                 } else {
                     int ibc = -1;
                     for (int k = 0; k < 2; ++k)
                         if (p.bound_cell[fb * 2 + k] == c && p.bound_face[fb * 2 + k] == f) ibc = k;
-                    if (ibc < 0) { err_ = "boundary face without a boundary entry"; return false; }
+                    if (ibc < 0) { bad[c] = 2; continue; }
                     reci[(size_t)c * kS3I + f] = -(fb * 2 + ibc) - 1;
                 }
+            }
+        });
+        for (int c = 0; c < nc; ++c)
+            if (bad[c]) {
+                err_ = bad[c] == 1 ? "internal error: interior face without its neighbour's side"
+                                   : "boundary face without a boundary entry";
+                return false;
             }
         if (!upload(d_s3d, rec) || !upload(d_s3i, reci)) return false;
     }
@@ -1027,39 +1052,44 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     {
         int nnz = p.nbr_ptr[nc];
         P3.assign((size_t)3 * nnz, 0.0);
-        std::vector<double> J, Pc;
+        host_blocks(nc, [&](int lo, int hi) {
+            std::vector<double> J, Pc;
+            for (int c = lo; c < hi; ++c) {
+                int a = p.nbr_ptr[c], m = p.nbr_ptr[c + 1] - a;
+                if (m <= 3) continue;
+                J.assign((size_t)m * 3, 0.0);
+                for (int k = 0; k < m; ++k) {
+                    int n = p.nbr_idx[a + k];
+                    J[k * 3 + 0] = (invLx * (p.cx[n] - p.cx[c]));
+                    J[k * 3 + 1] = (invLx * (p.cy[n] - p.cy[c]));
+                    J[k * 3 + 2] = (invLx * (p.cz[n] - p.cz[c]));
+                }
+                lsq3(J, m, Pc);
+                for (int i = 0; i < 3; ++i)
+                    for (int k = 0; k < m; ++k) P3[(size_t)i * nnz + a + k] = Pc[(size_t)i * m + k];
+                gok[c] = 1;
+            }
+        });
+        // the Fourier flux kernel's table: each cell's stencil padded to
+        // whole batches with its own row at weight 0
+        std::vector<int> qptr(nc + 1, 0);
         for (int c = 0; c < nc; ++c) {
-            int a = p.nbr_ptr[c], m = p.nbr_ptr[c + 1] - a;
-            if (m <= 3) continue;
-            J.assign((size_t)m * 3, 0.0);
-            for (int k = 0; k < m; ++k) {
-                int n = p.nbr_idx[a + k];
-                J[k * 3 + 0] = (invLx * (p.cx[n] - p.cx[c]));
-                J[k * 3 + 1] = (invLx * (p.cy[n] - p.cy[c]));
-                J[k * 3 + 2] = (invLx * (p.cz[n] - p.cz[c]));
-            }
-            lsq3(J, m, Pc);
-            for (int i = 0; i < 3; ++i)
-                for (int k = 0; k < m; ++k) P3[(size_t)i * nnz + a + k] = Pc[(size_t)i * m + k];
-            gok[c] = 1;
-        }
-        std::vector<int> qptr(nc + 1, 0), qidx;
-        std::vector<double> q[3];
-        for (int c = 0; c < nc; ++c) {
-            for (int k = p.nbr_ptr[c]; k < p.nbr_ptr[c + 1]; ++k) {
-                qidx.push_back(p.nbr_idx[k]);
-                for (int i = 0; i < 3; ++i) q[i].push_back(P3[(size_t)i * nnz + k]);
-            }
-            while ((qidx.size() - qptr[c]) % kStencilBatch) {
-                qidx.push_back(c);
-                for (int i = 0; i < 3; ++i) q[i].push_back(0.0);
-            }
-            qptr[c + 1] = (int)qidx.size();
+            int m = p.nbr_ptr[c + 1] - p.nbr_ptr[c];
+            qptr[c + 1] = qptr[c] + (m + kStencilBatch - 1) / kStencilBatch * kStencilBatch;
         }
         gnnz = qptr[nc];
-        std::vector<double> qa(q[0]);
-        qa.insert(qa.end(), q[1].begin(), q[1].end());
-        qa.insert(qa.end(), q[2].begin(), q[2].end());
+        std::vector<int> qidx(gnnz);
+        std::vector<double> qa((size_t)3 * gnnz, 0.0);
+        host_blocks(nc, [&](int lo, int hi) {
+            for (int c = lo; c < hi; ++c) {
+                int o = qptr[c];
+                for (int k = p.nbr_ptr[c]; k < p.nbr_ptr[c + 1]; ++k, ++o) {
+                    qidx[o] = p.nbr_idx[k];
+                    for (int i = 0; i < 3; ++i) qa[(size_t)i * gnnz + o] = P3[(size_t)i * nnz + k];
+                }
+                for (; o < qptr[c + 1]; ++o) qidx[o] = c;
+            }
+        });
         if (!upload(d_gptr, qptr) || !upload(d_gidx, qidx) || !upload(d_gP, qa) || !upload(d_gok, gok)) return false;
     }
     mark("source records and LSQ weights");
@@ -1068,43 +1098,54 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     // zero)
     {
         int nnz = p.nbr_ptr[nc];
-        std::vector<int> gptr2(nc + 1, 0), gidx2;
-        std::vector<int> gpos;  // stencil position per entry (-1: padding)
-        std::vector<std::pair<int, int>> st;
+        std::vector<int> gptr2(nc + 1, 0);
         for (int cc = 0; cc < nc; ++cc) {
-            // in increasing neighbour order: the waves of a workgroup take
-            // consecutive cells, whose stencils overlap, and then read the
-            // shared rows at about the same time (L1 hits)
-            st.clear();
-            for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) st.push_back({p.nbr_idx[k], k});
-            std::sort(st.begin(), st.end());
-            for (auto &q : st) { gidx2.push_back(q.first); gpos.push_back(q.second); }
-            while ((gidx2.size() - gptr2[cc]) % kG3Ent) { gidx2.push_back(cc); gpos.push_back(-1); }
-            gptr2[cc + 1] = (int)gidx2.size();
+            int m = p.nbr_ptr[cc + 1] - p.nbr_ptr[cc];
+            gptr2[cc + 1] = gptr2[cc] + (m + kG3Ent - 1) / kG3Ent * kG3Ent;
         }
-        size_t n2 = gidx2.size();
+        size_t n2 = (size_t)gptr2[nc];
+        std::vector<int> gidx2(n2);
+        std::vector<int> gpos(n2);  // stencil position per entry (-1: padding)
+        host_blocks(nc, [&](int lo, int hi) {
+            std::vector<std::pair<int, int>> st;
+            for (int cc = lo; cc < hi; ++cc) {
+                // in increasing neighbour order: the waves of a workgroup take
+                // consecutive cells, whose stencils overlap, and then read the
+                // shared rows at about the same time (L1 hits)
+                st.clear();
+                for (int k = p.nbr_ptr[cc]; k < p.nbr_ptr[cc + 1]; ++k) st.push_back({p.nbr_idx[k], k});
+                std::sort(st.begin(), st.end());
+                int o = gptr2[cc];
+                for (auto &q : st) { gidx2[o] = q.first; gpos[o] = q.second; ++o; }
+                for (; o < gptr2[cc + 1]; ++o) { gidx2[o] = cc; gpos[o] = -1; }
+            }
+        });
         if (fp32_) {
             std::vector<G3Ent<float>> e2(n2);
-            for (size_t k = 0; k < n2; ++k) {
-                int j = gpos[k];
-                e2[k].loc = gidx2[k];
-                e2[k].px = j < 0 ? 0.0f : (float)(P3[j] * invLx);
-                e2[k].py = j < 0 ? 0.0f : (float)(P3[(size_t)nnz + j] * invLx);
-                e2[k].pz = j < 0 ? 0.0f : (float)(P3[2 * (size_t)nnz + j] * invLx);
-            }
+            host_blocks((int)n2, [&](int lo, int hi) {
+                for (int k = lo; k < hi; ++k) {
+                    int j = gpos[k];
+                    e2[k].loc = gidx2[k];
+                    e2[k].px = j < 0 ? 0.0f : (float)(P3[j] * invLx);
+                    e2[k].py = j < 0 ? 0.0f : (float)(P3[(size_t)nnz + j] * invLx);
+                    e2[k].pz = j < 0 ? 0.0f : (float)(P3[2 * (size_t)nnz + j] * invLx);
+                }
+            });
             if (!alloc(d_g3g_ent, n2 * sizeof(e2[0])) ||
                 !check(hipMemcpy(d_g3g_ent.p, e2.data(), n2 * sizeof(e2[0]), hipMemcpyHostToDevice), "H2D"))
                 return false;
         } else {
             std::vector<G3Ent<double>> e2(n2);
-            for (size_t k = 0; k < n2; ++k) {
-                int j = gpos[k];
-                e2[k].loc = gidx2[k];
-                e2[k].pad = 0;
-                e2[k].px = j < 0 ? 0.0 : P3[j] * invLx;
-                e2[k].py = j < 0 ? 0.0 : P3[(size_t)nnz + j] * invLx;
-                e2[k].pz = j < 0 ? 0.0 : P3[2 * (size_t)nnz + j] * invLx;
-            }
+            host_blocks((int)n2, [&](int lo, int hi) {
+                for (int k = lo; k < hi; ++k) {
+                    int j = gpos[k];
+                    e2[k].loc = gidx2[k];
+                    e2[k].pad = 0;
+                    e2[k].px = j < 0 ? 0.0 : P3[j] * invLx;
+                    e2[k].py = j < 0 ? 0.0 : P3[(size_t)nnz + j] * invLx;
+                    e2[k].pz = j < 0 ? 0.0 : P3[2 * (size_t)nnz + j] * invLx;
+                }
+            });
             if (!alloc(d_g3g_ent, n2 * sizeof(e2[0])) ||
                 !check(hipMemcpy(d_g3g_ent.p, e2.data(), n2 * sizeof(e2[0]), hipMemcpyHostToDevice), "H2D"))
                 return false;
@@ -1736,7 +1777,7 @@ bool Solver::tune3d() {
     if (!check(hipEventCreate(&a), "hipEventCreate") || !check(hipEventCreate(&b), "hipEventCreate")) return false;
     char line[160];
     // one candidate: a warm launch and a timed one, then two more timed
-    // launches only when the first is within 30% of the best so far (a
+    // launches only when the first is within 15% of the best so far (a
     // clearly slower candidate costs two launches instead of four)
     auto time_it = [&](auto launch, double best, double &per) -> bool {
         launch();
@@ -1747,7 +1788,7 @@ bool Solver::tune3d() {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, a, b);
         per = ms;
-        if (per <= 1.3 * best) {
+        if (per <= 1.15 * best) {
             (void)hipEventRecord(a, s_);
             launch();
             launch();
@@ -1776,7 +1817,11 @@ bool Solver::tune3d() {
             err_ = "the vertex-sum gradient does not take this mesh (tetrahedra of one material only)";
             return false;
         }
-        for (int v : {0, 1}) {
+        // the vertex sums read about half the rows of the direct gather, so
+        // the gather shapes are timed only when the vertex sums do not take
+        // the mesh or the gather is asked for
+        if (want == GBTE_GRAD_AUTO && gv_ok) want_g = false;
+        for (int v : {1, 0}) {
             if (v == 0 ? !want_g : !(want_v && gv_ok)) continue;
             use_gv = v == 1;
             for (int wpb : {4, 16})
