@@ -2175,6 +2175,7 @@ bool Solver::build() {
                 for (int d = 0; d < nD; ++d) posc[(size_t)c * nD + d] = posT[(size_t)d * nc + c];
         });
         std::vector<int>().swap(posT);
+        mark("upwind orders");
         if (dim == 3 && !opt.fallback_kernels) {
             // records of the level-synchronous chunked sweep: every level is
             // non-empty (the kernel's lanes past a level's end re-read its
@@ -2214,6 +2215,7 @@ bool Solver::build() {
         if (!upload(d_item_cell, item_cell) || !upload(d_posc, posc) || !upload(d_levptr, levptr) || !upload(d_dlev, dlev) ||
             !upload(d_rec_q, rec_q) || !upload(d_rec_c, rec_c) || !upload(d_rec_g, rec_g) || !upload(d_rec_G, rec_G))
             return false;
+        mark("level records");
         // ---- wave-specialised sweep schedule: sub-levels of at most kPcCap
         // positions and kPcFar far upwind reads (upwind positions older than
         // the ring of the last pc_ring positions when the sub-level runs)
@@ -2224,7 +2226,13 @@ bool Solver::build() {
         pc_nchunk = (nB + pc_bc - 1) / pc_bc;
         pc_ring = 1;
         while ((size_t)pc_ring * 2 * pc_bc * sizeof(double) <= 80 * 1024) pc_ring *= 2;
-        if (!opt.fallback_kernels && pc_bc <= kPcBc && (kPcDepth + 1) * kPcCap <= pc_ring) {
+        // on tetrahedra the wave-specialised kernel is a candidate only for
+        // levels about as narrow as its sub-levels: a wider level is cut into
+        // several sub-levels that run one after another (the 295k-tet deck's
+        // levels hold 590 cells on average, where the chunked kernel is
+        // faster by 1.5 to 2.5x), so the schedule is not built there
+        bool pc_fits = dim == 2 || (double)NI / std::max(1, dlev[nD] - nD) <= 2.0 * kPcCap;
+        if (!opt.fallback_kernels && pc_fits && pc_bc <= kPcBc && (kPcDepth + 1) * kPcCap <= pc_ring) {
             std::vector<PcRec> prec(NI);
             std::vector<int> psl, pfp, pdsl(1, 0), pfar;
             bool fits = true;
@@ -2369,7 +2377,7 @@ bool Solver::build() {
             }
         }
     }
-    mark("sweep schedules");
+    mark("wave-specialised schedule");
     // ---- LSQ gradient stencil (_get_CellMatrix_larger + Get_inverse_2D,
     // StaticBTESynthetic.cpp:1000-1064, utility.cpp:401-466); 3D: build3d
     if (dim == 2) {
