@@ -1093,11 +1093,22 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
         if (!upload(d_gptr, qptr) || !upload(d_gidx, qidx) || !upload(d_gP, qa) || !upload(d_gok, gok)) return false;
     }
     mark("source records and LSQ weights");
+    // ---- vertex-sum gradient (k_vsum3 + k_grad3v): the cells around each
+    // vertex, one chosen cell r_v per vertex, per-vertex and per-cell tables
+    gv_ok = build_gv(invLx);
+    if (!err_.empty()) return false;
+    mark("vertex sums");
+    // the direct gather and the matrix-core tiles read about twice the rows
+    // of the vertex sums per cell: their tables are built only when the
+    // vertex sums do not take the mesh or the kernel is asked for
+    bool need_g = !gv_ok || opt.gradient_kernel == GBTE_GRAD_GATHER;
+    bool need_m = !gv_ok || opt.gradient_kernel == GBTE_GRAD_MATRIX;
     // ---- gradient stencil tables: global neighbour rows, padded per cell to
     // whole batches with the cell's own row at weight 0 (each adds an exact
     // zero)
     {
         int nnz = p.nbr_ptr[nc];
+        if (need_g) {
         std::vector<int> gptr2(nc + 1, 0);
         for (int cc = 0; cc < nc; ++cc) {
             int m = p.nbr_ptr[cc + 1] - p.nbr_ptr[cc];
@@ -1151,10 +1162,11 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 return false;
         }
         if (!upload(d_g3g_ptr, gptr2) || !upload(d_g3g_idx, gidx2)) return false;
+        }
         mark("gather tables");
         // ---- matrix-core tiles (k_grad3m): up to four consecutive cells while
         // the rows they need fit kGmKS K-steps
-        {
+        if (need_m) {
             // tile boundaries and rows (sequential, greedy), then the A'
             // operands of each tile (host threads)
             std::vector<int> trows, tcell, tnrow;
@@ -1246,11 +1258,6 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
         }
     }
     mark("matrix-core tiles");
-    // ---- vertex-sum gradient (k_vsum3 + k_grad3v): the cells around each
-    // vertex, one chosen cell r_v per vertex, per-vertex and per-cell tables
-    gv_ok = build_gv(invLx);
-    if (!err_.empty()) return false;
-    mark("vertex sums");
     // ---- moments and the residual's face coefficients
     std::vector<std::vector<double>> FM(nf);  // per unique face, from its first side: 3 x nv
     for (int f = 0; f < nf; ++f) {
