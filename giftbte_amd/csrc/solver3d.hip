@@ -671,19 +671,26 @@ __device__ __forceinline__ void vst(T *p, const T (&v)[N]) {
 // chunks of one direction take consecutive workgroup ids, which xcd_block
 // places on one XCD, so they share its L2 for the records and for the
 // partial lines of the e rows they write.
+//
+// V = 0: chunks of up to BC bands with one 16-byte vector per lane, the lanes
+// per position set per workgroup (the last chunk of a direction may be
+// narrower: 40 FP32 bands are chunks of 16, 16 and 8, four or two lanes per
+// position), S per position.
 template <typename ST, int BC, int NP, int TPB, int V>
 __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec, const int *__restrict__ levptr,
                                                   const int *__restrict__ dlev, const ST *__restrict__ S,
                                                   ST *__restrict__ e, Modes m, int nc, int nB, int nD, int smajor) {
-    constexpr int BL = BC / V;   // bands per lane
-    constexpr int PP = TPB / V;  // positions per pass
-    static_assert(BC % V == 0 && TPB % V == 0 && (BL * sizeof(ST)) % 16 == 0, "whole 16-byte vectors per lane");
+    constexpr int BL = V > 0 ? BC / V : 16 / (int)sizeof(ST);  // bands per lane
+    static_assert((V == 0 || (BC % V == 0 && TPB % V == 0)) && BC % BL == 0 && (BL * sizeof(ST)) % 16 == 0,
+                  "whole 16-byte vectors per lane");
     extern __shared__ int slp[];  // level pointers of the direction
-    int nchunk = nB / BC;
+    int nchunk = V > 0 ? nB / BC : (nB + BC - 1) / BC;
     int id = xcd_block();
     if (id >= nD * nchunk) return;  // whole workgroup
     int d = id / nchunk, chunk = id - d * nchunk;
-    int tid = (int)threadIdx.x / V, sub = (int)threadIdx.x - tid * V;  // position slot, vector of the chunk
+    int vl = V > 0 ? V : min(BC, nB - chunk * BC) / BL;  // lanes per position
+    const int PP = TPB / vl;                                // positions per pass
+    int tid = (int)threadIdx.x / vl, sub = (int)threadIdx.x - tid * vl;  // position slot, vector of the chunk
     int wpos = __builtin_amdgcn_readfirstlane(tid);                      // the wave's first position slot
     int b0 = chunk * BC + sub * BL;
     double tv[BL];
@@ -1723,7 +1730,21 @@ void Solver::launch_sweep_lv(const Modes &m, int bc) {
             else GBTE_LV(ST_, BC_, 2, 256, V_);                                \
         }                                                                      \
     } while (0)
-    if (lv_v > 1) {
+    if (lv_v == 0) {
+        // chunks of up to 16 bands (FP32) or 16 / 8 bands (FP64), lanes per
+        // position set per chunk; the grid counts the partial last chunk
+        grid = ((nD * ((nB + bc - 1) / bc) + kXcd - 1) / kXcd) * kXcd;
+        if (fp32_) {
+            if (lv_np == 4) GBTE_LV(float, 16, 4, 512, 0);
+            else GBTE_LV(float, 16, 2, 512, 0);
+        } else if (bc == 16) {
+            if (lv_np == 4) GBTE_LV(double, 16, 4, 512, 0);
+            else GBTE_LV(double, 16, 2, 512, 0);
+        } else {
+            if (lv_np == 4) GBTE_LV(double, 8, 4, 512, 0);
+            else GBTE_LV(double, 8, 2, 512, 0);
+        }
+    } else if (lv_v > 1) {
         if (fp32_) GBTE_LVV(float, 8);
         else if (bc == 4) GBTE_LVV(double, 4);
         else GBTE_LVV(double, 8);
@@ -1897,6 +1918,17 @@ bool Solver::tune3d() {
                 (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_LANES))
                 for (int tpb : {256, 512})
                     for (int np : {2, 4}) cands.push_back({2, bc, np, bc * wsz / 16, tpb});
+        // chunks of up to 16 bands (FP32 and FP64) or 8 (FP64) where the bands
+        // split into such chunks with 1, 2, 4 or 8 lanes per position, S per
+        // position (v = 0 marks the per-chunk lane count)
+        for (int bc : {16, 8}) {
+            int bl = 16 / wsz, last = nB % bc == 0 ? bc : nB % bc;
+            bool pow2 = last % bl == 0 && ((last / bl) & (last / bl - 1)) == 0;
+            if ((fp32_ && bc != 16) || nB % bc == 0 || !pow2 || !d_lv_rec.p ||
+                (size_t)(nlev + 1) * sizeof(int) > 48 * 1024 || !(want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_LANES))
+                continue;
+            for (int np : {2, 4}) cands.push_back({2, bc, np, 0, 512});
+        }
         double best = 1e300;
         Cand pick{-1, 0, 0};
         auto chunk_sh = [](int bc) {  // S per band chunk where the chunk is a power of two
@@ -1910,10 +1942,14 @@ bool Solver::tune3d() {
             lv_np = c.kind == 2 ? c.np : 2;
             lv_v = c.v;
             lv_tpb = c.tpb;
-            s_sh = c.kind == 2 ? chunk_sh(c.bc) : -1;
+            s_sh = c.kind == 2 && c.v != 0 ? chunk_sh(c.bc) : -1;
             double per = 0;
             if (!time_it([&]() { launch_sweep3(m); }, best, per)) return false;
-            if (c.kind == 2 && c.v > 1)
+            if (c.kind == 2 && c.v == 0)
+                snprintf(line, sizeof line,
+                         "sweep chunked up to %d bands, one vector per lane, %d positions per lane, %d threads: %.3f ms per launch\n",
+                         c.bc, c.np, c.tpb, per);
+            else if (c.kind == 2 && c.v > 1)
                 snprintf(line, sizeof line,
                          "sweep chunked %d bands, %d lanes per position, %d positions per lane, %d threads: %.3f ms per launch\n",
                          c.bc, c.v, c.np, c.tpb, per);
@@ -1932,8 +1968,12 @@ bool Solver::tune3d() {
         lv_np = pick.kind == 2 ? pick.np : 2;
         lv_v = pick.v;
         lv_tpb = pick.tpb;
-        s_sh = pick.kind == 2 ? chunk_sh(pick.bc) : -1;
-        if (pick.kind == 2 && pick.v > 1)
+        s_sh = pick.kind == 2 && pick.v != 0 ? chunk_sh(pick.bc) : -1;
+        if (pick.kind == 2 && pick.v == 0)
+            snprintf(line, sizeof line,
+                     "sweep chosen: chunked up to %d bands, one vector per lane, %d positions per lane, %d threads\n",
+                     pick.bc, pick.np, pick.tpb);
+        else if (pick.kind == 2 && pick.v > 1)
             snprintf(line, sizeof line,
                      "sweep chosen: chunked %d bands, %d lanes per position, %d positions per lane, %d threads\n",
                      pick.bc, pick.v, pick.np, pick.tpb);

@@ -57,6 +57,10 @@ DECKS = {
     # isothermal x = 0 / x = 1), 8 bands, Gauss 3D directions (32)
     "cube3d_b8": (lambda: decks.cube_void_spec(m=6, nbands=8, ntheta=2, nphi=2, seed=6),
                   dict(max_iter=8, tol_temp=0.0, tol_flux=0.0)),
+    # 3D, 20 bands: a band count that 8-band chunks do not divide (the sweep
+    # then runs chunks of 16 and 4 bands with per-chunk lanes per position)
+    "cube3d_b20": (lambda: decks.cube_void_spec(m=5, nbands=20, ntheta=2, nphi=2, seed=9),
+                   dict(max_iter=6, tol_temp=0.0, tol_flux=0.0)),
     # 3D gray solid cube (no void), Sequential scheme, stopped by the tolerances
     "cube3d_gray_seq": (lambda: dataclasses.replace(
         decks.cube_void_spec(m=5, nbands=1, ntheta=1, nphi=3, void=0.0, seed=8), bands=decks.gray_band(),

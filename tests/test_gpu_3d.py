@@ -26,11 +26,13 @@ def _keff_x(p, q, dT=2.0):
 
 # every sweep kernel family the set-up tuner chooses from, forced, on the
 # 8-band deck (the level-synchronous one-workgroup-per-direction kernel is
-# FP64 only)
+# FP64 only); on the 20-band deck the FP32 lane sweep can only run chunks of
+# 16 and 4 bands (lanes per position set per chunk)
 SWEEPS = [("cube3d_b8", "fp64", "wavespec", "auto"), ("cube3d_b8", "fp64", "level", "auto"),
           ("cube3d_b8", "fp64", "chunked", "auto"), ("cube3d_b8", "fp32", "wavespec", "auto"),
           ("cube3d_b8", "fp32", "chunked", "auto"), ("cube3d_b8", "fp64", "lanes", "auto"),
-          ("cube3d_b8", "fp32", "lanes", "auto")]
+          ("cube3d_b8", "fp32", "lanes", "auto"), ("cube3d_b20", "fp32", "lanes", "auto"),
+          ("cube3d_b20", "fp64", "auto", "auto")]
 # each gradient kernel, forced, on both decks and precisions
 GRADS = [(n, pr, "auto", g) for n in DECKS3D for pr in ("fp64", "fp32") for g in ("gather", "matrix", "vertex")]
 
@@ -44,7 +46,10 @@ def test_3d_matches_oracle(name, precision, sweep, grad, deck_root):
     print(f"\n[gpu 3d] deck={name} precision={precision} sweep={sweep} gradient={grad} settings={s}", flush=True)
     p = Problem(d)
     sol = BTESolver(p, precision=precision, sweep_kernel=sweep, gradient_kernel=grad)
-    print("  " + sol.tuning_log().replace("\n", "; "), flush=True)
+    log = sol.tuning_log()
+    print("  " + log.replace("\n", "; "), flush=True)
+    if (name, precision, sweep) == ("cube3d_b20", "fp32", "lanes"):
+        assert "sweep chosen: chunked up to 16 bands" in log  # the only lane shape 20 FP32 bands admit
     fp = sol.fourier()
     st = sol.solve(s["max_iter"], s["tol_temp"], s["tol_flux"])
     ref, info = _oracle.run(p.tables, s["max_iter"], s["tol_temp"], s["tol_flux"])
