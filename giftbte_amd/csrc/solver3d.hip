@@ -663,11 +663,11 @@ __device__ __forceinline__ void vst(T *p, const T (&v)[N]) {
 // values are read from e, where this workgroup wrote them on earlier levels;
 // levels are separated by workgroup barriers.
 //
-// Software pipeline: a level's first TPB NP / V positions have their records
-// and sources loaded during the previous level, issued after that level's
+// Software pipeline: the positions are taken in chunks of TPB NP / V (a wide
+// level is several chunks, with no barrier between them), and a chunk's
+// records and sources are loaded during the previous chunk, issued after its
 // upwind loads, so that only the upwind loads (L2) and the arithmetic lie on
-// the chain of dependent levels (positions beyond those of a wide level are
-// loaded in place).  The level pointers of the direction sit in LDS.  The
+// the chain of dependent levels.  The level pointers of the direction sit in LDS.  The
 // chunks of one direction take consecutive workgroup ids, which xcd_block
 // places on one XCD, so they share its L2 for the records and for the
 // partial lines of the e rows they write.
@@ -710,15 +710,6 @@ __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec,
     // both masked afterwards): with no branch around a load, the wait before
     // the products counts exactly this level's upwind loads and leaves the
     // next level's records and sources in flight.
-    auto fetch = [&](int l, LvRec(&rr)[NP], ST(&ss)[NP][BL]) {
-        int a0 = slp[l], a1 = slp[l + 1];
-#pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            int p = min(a0 + tid + i * PP, a1 - 1);
-            rr[i] = rd[p];
-            vld(ss[i], Sd + (size_t)p * sst);
-        }
-    };
     auto upwind = [&](const LvRec &r, ST(&u)[3][BL]) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) vld(u[j], ed + (size_t)max(r.c[j], 0) * DB);
@@ -737,34 +728,52 @@ __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec,
         }
         if (store) vst(ed + (size_t)r.cell * DB, en);
     };
-    // level l from the buffers (rc, sc); the next level's first positions
-    // into (rn, sn) (the last level re-reads its own)
-    auto level = [&](int l, LvRec(&rc)[NP], ST(&sc)[NP][BL], LvRec(&rn)[NP], ST(&sn)[NP][BL]) {
-        int a0 = slp[l], a1 = slp[l + 1];
+    // The positions of a direction are taken in chunks of up to PP NP: a
+    // level is one chunk or several, and a barrier separates the levels, not
+    // the chunks of one level (whose upwind cells all lie on earlier levels).
+    // Chunk (l, p0) from the buffers (rc, sc); the next chunk's positions
+    // into (rn, sn); the last chunk re-reads its own.
+    auto next = [&](int l, int p0, int &nl, int &np0) {
+        if (p0 + PP * NP < slp[l + 1]) { nl = l; np0 = p0 + PP * NP; }
+        else { nl = l + 1; np0 = slp[l + 1]; }
+    };
+    auto fetch = [&](int l, int p0, LvRec(&rr)[NP], ST(&ss)[NP][BL]) {
+        int a1 = slp[l + 1];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            int p = min(p0 + tid + i * PP, a1 - 1);
+            rr[i] = rd[p];
+            vld(ss[i], Sd + (size_t)p * sst);
+        }
+    };
+    auto chunk_step = [&](int l, int p0, LvRec(&rc)[NP], ST(&sc)[NP][BL], LvRec(&rn)[NP], ST(&sn)[NP][BL],
+                          int &nl, int &np0) {
+        int a1 = slp[l + 1];
+        next(l, p0, nl, np0);
         ST u[NP][3][BL];
 #pragma unroll
         for (int i = 0; i < NP; ++i) upwind(rc[i], u[i]);
-        fetch(l + 1 < L ? l + 1 : l, rn, sn);
-        // a wave none of whose positions lies in the level skips the
+        fetch(nl < L ? nl : l, nl < L ? np0 : p0, rn, sn);
+        // a wave none of whose positions lies in the chunk skips the
         // arithmetic (a uniform branch with no load in it)
 #pragma unroll
         for (int i = 0; i < NP; ++i)
-            if (a0 + wpos + i * PP < a1) solve(rc[i], sc[i], u[i], a0 + tid + i * PP < a1);
-        for (int p = a0 + PP * NP + tid; p < a1; p += PP) {
-            LvRec r = rd[p];
-            ST s1[BL], u1[3][BL];
-            vld(s1, Sd + (size_t)p * sst);
-            upwind(r, u1);
-            solve(r, s1, u1, true);
-        }
-        __syncthreads();
+            if (p0 + wpos + i * PP < a1) solve(rc[i], sc[i], u[i], p0 + tid + i * PP < a1);
+        if (nl != l) __syncthreads();
     };
     LvRec rA[NP], rB[NP];
     ST sA[NP][BL], sB[NP][BL];
-    if (L > 0) fetch(0, rA, sA);
-    for (int l = 0; l < L; l += 2) {
-        level(l, rA, sA, rB, sB);
-        if (l + 1 < L) level(l + 1, rB, sB, rA, sA);
+    if (L > 0) {
+        int l = 0, p0 = slp[0], nl, np0;
+        fetch(0, p0, rA, sA);
+        for (;;) {
+            chunk_step(l, p0, rA, sA, rB, sB, nl, np0);
+            if (nl >= L) break;
+            l = nl; p0 = np0;
+            chunk_step(l, p0, rB, sB, rA, sA, nl, np0);
+            if (nl >= L) break;
+            l = nl; p0 = np0;
+        }
     }
 }
 
@@ -1722,7 +1731,9 @@ void Solver::launch_sweep_lv(const Modes &m, int bc) {
 #define GBTE_LVV(ST_, BC_)                                                     \
     do {                                                                       \
         constexpr int V_ = BC_ * (int)sizeof(ST_) / 16;                        \
-        if (lv_tpb == 512) {                                                   \
+        if (lv_tpb == 1024) {                                                  \
+            GBTE_LV(ST_, BC_, 2, 1024, V_);                                    \
+        } else if (lv_tpb == 512) {                                            \
             if (lv_np == 4) GBTE_LV(ST_, BC_, 4, 512, V_);                     \
             else GBTE_LV(ST_, BC_, 2, 512, V_);                                \
         } else {                                                               \
@@ -1916,8 +1927,9 @@ bool Solver::tune3d() {
             if (d_lv_rec.p && nB % bc == 0 && (fp32_ ? bc == 8 : true) && (nB * wsz) % 16 == 0 &&
                 (size_t)(nlev + 1) * sizeof(int) <= 48 * 1024 &&
                 (want == GBTE_SWEEP_AUTO || want == GBTE_SWEEP_LANES))
-                for (int tpb : {256, 512})
-                    for (int np : {2, 4}) cands.push_back({2, bc, np, bc * wsz / 16, tpb});
+                for (int tpb : {256, 512, 1024})
+                    for (int np : {2, 4})
+                        if (tpb < 1024 || np == 2) cands.push_back({2, bc, np, bc * wsz / 16, tpb});
         // chunks of up to 16 bands (FP32 and FP64) or 8 (FP64) where the bands
         // split into such chunks with 1, 2, 4 or 8 lanes per position, S per
         // position (v = 0 marks the per-chunk lane count)
