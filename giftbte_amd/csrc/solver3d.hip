@@ -596,6 +596,13 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3_f32x2(
     }
     double To = Told[c] - m.eref, Qc = Q[c];
     const int *pc = pos + w * nD;
+    // two entries per lane and two iterations' loads in flight; a face's row
+    // (own projection or the upwind side's) is chosen once per direction by
+    // the sign of s . n, which the two bands share (the bands' v s . n can
+    // differ in sign only where s . n vanishes to rounding, and there the
+    // term itself is that small; the projections are written where
+    // s . n >= -1e-12 on either side, so both rows hold values there)
+#pragma unroll 2
     for (int t = 2 * lane; t < DB; t += 128) {
         int d = t / nB, b = t - d * nB;
         double vb0 = mvg[b], vb1 = mvg[b + 1], sx = msx[d], sy = msy[d], sz = msz[d];
@@ -605,15 +612,9 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3_f32x2(
         for (int f = 0; f < 4; ++f) {
             dot0[f] = vb0 * sx * fnx[f] + vb0 * sy * fny[f] + vb0 * sz * fnz[f];
             dot1[f] = vb1 * sx * fnx[f] + vb1 * sy * fny[f] + vb1 * sz * fnz[f];
-            u[f] = *reinterpret_cast<const float2 *>((dot0[f] >= 0 && dot1[f] >= 0 ? fo[f] : fin[f]) + t);
+            double sn = sx * fnx[f] + sy * fny[f] + sz * fnz[f];
+            u[f] = *reinterpret_cast<const float2 *>((sn >= 0 ? fo[f] : fin[f]) + t);
         }
-#pragma unroll
-        for (int f = 0; f < 4; ++f)
-            if ((dot0[f] >= 0) != (dot1[f] >= 0)) {
-                float2 uo = *reinterpret_cast<const float2 *>(fo[f] + t);
-                if (dot0[f] >= 0) u[f].x = uo.x;
-                else u[f].y = uo.y;
-            }
         double s0 = To * mcap[b] + Qc * mhr[b] * mtau[b];
         double s1 = To * mcap[b + 1] + Qc * mhr[b + 1] * mtau[b + 1];
 #pragma unroll
