@@ -23,12 +23,14 @@ both, the time to a converged k_eff (the deck's tolerances, from the Fourier
 warm start) and the end-to-end time including gbte_create are reported, with
 the mixed k_eff's error against FP64.
 
-The CPU baseline on 2D is the reference itself (oracle/_ref/BTE_CPU, compiled
-from its sources) run with MPI ranks on this host on a bounded sample, or,
-where that binary is absent, the oracle's single-core transport loop.  The
-reference cannot run 3D meshes (its 3D mesh construction is removed and
-BTEMain.cpp:218-225 exits), so on 3D the oracle's single-core transport loop
-over a bounded number of the systems is the baseline.
+The CPU baseline is the reference itself, compiled from its sources, run with
+MPI ranks on this host on a bounded sample: on 2D its BTE_CPU on the same mesh
+with a subset of the directions; on 3D, where the reference cannot build the
+mesh (its 3D mesh construction is removed and BTEMain.cpp:218-225 exits), its
+StaticBTESynthetic solve() driven by oracle/ref_dump on our tables of a
+smaller deck of the same generator.  Where those binaries are absent, the
+oracle's single-core transport loop on the same mesh is the baseline.  The
+oracle's figure is reported beside the reference's as cpu_port.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3d|2d]
                        [--no-converge] [--no-cpu] [--batch N] [--adjoint]
@@ -86,6 +88,53 @@ def cpu_baseline(problem, T0, target_s=12.0):
                       f"IDW, face gradients, non-Fourier residual, moments) over {n} of "
                       f"{t.n_bands * t.n_dirs} band x direction systems of one outer iteration, same mesh; "
                       f"{sec:.1f} s on 1 of {host_cores()} host cores"}
+
+
+def reference_baseline_3d(td, nbands, seed, lattice=14, iters=2, timeout=600):
+    """The reference's own StaticBTESynthetic (compiled from its sources with
+    oracle/ref_dump) on tetrahedra: the reference cannot build 3D meshes
+    (BTEMain.cpp:218-225 exits), so ref_dump --tables --time hands it our
+    tables of a cube-with-void deck of the same generator and runs solve()
+    with the band x direction systems over MPI ranks on this host's cores.
+    Bounded sample: a lattice of `lattice` cells per edge (about 17k
+    tetrahedra at 14; the reference's own set-up, per rank, grows faster than
+    its iterations), the same bands, 32 directions.  Rate = updates of one
+    outer iteration / the reference's per-iteration inner-loop timer; rank
+    count as for 2D."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+    mpi = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpi)):
+        return None
+    from giftbte_amd import decks, Problem
+    spec = decks.cube_void_spec(m=lattice, nbands=nbands, ntheta=2, nphi=2, seed=seed)
+    deck = decks.write_deck(os.path.join(td, "refdeck3d"), spec)
+    p = Problem(deck)
+    p.write_tables(os.path.join(td, "reftables3d"))
+    t = p.tables
+    nd = t.n_dirs
+    cores = min(16, host_cores())
+    ranks = max(r for r in range(1, nd + 1) if nd % r == 0 and r <= cores)
+    t0 = time.time()
+    try:
+        r = subprocess.run(["timeout", str(timeout), mpi, "-np", str(ranks), exe, deck, os.path.join(td, "refout3d"),
+                            str(iters), "0", "0", "--tables", os.path.join(td, "reftables3d"), "--time"],
+                           cwd=deck, capture_output=True, text=True)
+    except OSError:
+        return None
+    wall = time.time() - t0
+    ms = [float(x) for x in re.findall(r"Time taken by inner loop: ([0-9.eE+-]+) milliseconds", r.stdout)]
+    if r.returncode != 0 or not ms:
+        print(f"[bench] reference 3D baseline failed (rc {r.returncode})", file=sys.stderr, flush=True)
+        return None
+    upd = float(t.n_cells) * t.n_bands * nd
+    sec = sum(ms) / len(ms) / 1e3
+    return {"value": upd / sec, "unit": "cell*band*dir updates/s", "cores": ranks, "kind": "reference",
+            "sample": f"the reference's StaticBTESynthetic::solve (compiled from its sources; oracle/ref_dump "
+                      f"--tables --time hands it our tetrahedral tables, its own 3D mesh reader being removed), "
+                      f"mpiexec -np {ranks}, on a {t.n_cells}-tet cube-with-void deck of the same generator, "
+                      f"{t.n_bands} bands, {nd} directions ({t.n_bands * nd} systems); mean of its per-iteration "
+                      f"inner-loop timer over {len(ms)} outer iterations, {sec * 1e3:.0f} ms; {wall:.0f} s wall "
+                      f"with set-up"}
 
 
 def host_cores():
@@ -455,7 +504,8 @@ def main():
                     "speedup_end_to_end": f64["end_to_end_s"] / f32["end_to_end_s"]})
         if not a.no_cpu:
             port = cpu_baseline(p, T0[0])
-            ref = reference_baseline(td, a.cells, a.bands, a.seed) if a.workload == "2d" else None
+            ref = (reference_baseline(td, a.cells, a.bands, a.seed) if a.workload == "2d"
+                   else reference_baseline_3d(td, a.bands, a.seed))
             out["cpu_baseline"] = ref if ref is not None else port
             if ref is not None:
                 out["cpu_port"] = port

@@ -10,6 +10,7 @@ library is missing or no GPU is present the calls raise.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -73,6 +74,28 @@ class Problem:
 
     def array(self, name: str) -> np.ndarray:
         return self.tables.array(name)
+
+    # the flattened mesh tables of a tetrahedral (or 2D) problem, as raw
+    # little-endian arrays: what a solver that takes a built mesh reads
+    # (the reference's BTEMesh public vectors, includes/BTEMesh/BTEMesh.h:27-37)
+    TABLE_BINS = ["face_size", "face_neighbor", "face_bound", "face_index", "face_nvert", "face_vert", "cell_nvert",
+                  "cell_vert", "matter", "face_area", "face_nx", "face_ny", "face_nz", "face_cx", "face_cy",
+                  "face_cz", "volume", "cx", "cy", "cz", "heat_source", "initial_temp", "node_x", "node_y", "node_z",
+                  "bound_cell", "bound_face", "bound_type", "bound_connect", "bound_temp", "fface_cell",
+                  "fface_face", "fface_bound"]
+
+    def write_tables(self, out_dir: str) -> None:
+        """Write the flattened tables to out_dir/<name>.bin (int32 or
+        float64), with sizes.bin (cells, nodes, boundaries, faces, dim) and
+        scale.bin (L_x, L_y, L_z)."""
+        os.makedirs(out_dir, exist_ok=True)
+        t = self.tables
+        for name in self.TABLE_BINS:
+            a = self.array(name)
+            a = a.astype("<i4" if a.dtype.kind == "i" else "<f8")
+            a.tofile(os.path.join(out_dir, name + ".bin"))
+        np.array([t.n_cells, t.n_nodes, t.n_bounds, t.n_faces, t.dim], "<i4").tofile(os.path.join(out_dir, "sizes.bin"))
+        np.array([t.Lx, t.Ly, t.Lz], "<f8").tofile(os.path.join(out_dir, "scale.bin"))
 
     def close(self):
         if self._h:

@@ -8,7 +8,7 @@
 // reference's own .dat outputs carry 6 significant digits only
 // (src/StaticBTESynthetic/StaticBTESynthetic.cpp:2613-2653).
 //
-// usage: ref_dump <deck_dir> <out_dir> <max_iter> <tolT> <tolQ> [stage_band stage_dir] [--tables DIR]
+// usage: ref_dump <deck_dir> <out_dir> <max_iter> <tolT> <tolQ> [stage_band stage_dir] [--tables DIR [--time]]
 //   deck_dir  a COPY of a deck (the reference writes its .dat files into the
 //             working directory on every iteration)
 //   out_dir   where the .npy files go
@@ -404,6 +404,30 @@ void mesh_from_tables(const std::string &dir, BTEMesh &m) {
     }
 }
 
+// Timing run on a mesh built from tables: the reference's StaticBTESynthetic
+// with its systems distributed over the MPI ranks (as BTEMain.cpp runs it with
+// mpiexec), from the initial temperature, for max_iter outer iterations of
+// solve() (StaticBTESyntheticSolver.cpp:12-247; rank 0 prints its
+// per-iteration "Time taken by inner loop" line).  Nothing is dumped.
+int time_from_tables(const Deck &deck, BTEBand *bands, BTEBoundaryCondition *bcs, const std::string &tables,
+                     int maxIter) {
+    int size = 1, rank = 0;
+    MPI_Comm_size(MPI_COMM_WORLD, &size);
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    BTEMesh mesh;
+    mesh_from_tables(tables, mesh);
+    BTEAngle *angles = new BTEAngle(deck.ntheta, deck.nphi, deck.dimMat, deck.angleMethod);
+    StaticBTESynthetic solver(&mesh, bcs, bands, angles, size, rank);
+    for (size_t i = 0; i < mesh.Elements.size(); ++i) solver.temperature[i] = mesh.Elements[i].initial_temperature;
+    std::fill(solver.ebound, solver.ebound + (size_t)solver.numBand * solver.numDirection * solver.numBound * 2, 0.0);
+    solver.error_old = 0;
+    solver.errorOld = 0;
+    solver.errorOld_heat = 0;
+    solver.solve(0, maxIter, 0, 0.0, 0.0);
+    MPI_Finalize();
+    return 0;
+}
+
 // The steady flow of main on a mesh built from tables: _set_initial
 // (Solution.cpp:78-124), the Fourier warm start (:236-260) and the BTE solve
 // with its hand-over (:262-414), at np=1.
@@ -506,11 +530,14 @@ int main(int argc, char **argv) {
     double tolT = atof(argv[4]), tolQ = atof(argv[5]);
     int sb = -1, sd = -1;
     std::string tables;  // --tables DIR: the mesh from flattened tables (3D decks)
+    bool timing = false; // --time (with --tables): a timing run over the MPI ranks, nothing dumped
     std::vector<std::string> pos;
     for (int k = 6; k < argc; ++k) {
         if (std::string(argv[k]) == "--tables" && k + 1 < argc) {
             tables = argv[++k];
             if (tables[0] != '/') tables = std::string(cwd0) + "/" + tables;
+        } else if (std::string(argv[k]) == "--time") {
+            timing = true;
         } else {
             pos.push_back(argv[k]);
         }
@@ -530,6 +557,7 @@ int main(int argc, char **argv) {
     std::ifstream bcFile(deck.bcFile);
     BTEBoundaryCondition *bcs = new BTEBoundaryCondition(bcFile, bands);
     bcFile.close();
+    if (!tables.empty() && timing) return time_from_tables(deck, bands, bcs, tables, maxIter);
     if (!tables.empty()) return run_from_tables(deck, bands, bcs, tables, maxIter, tolT, tolQ, sb, sd);
     std::ifstream heatfile(deck.heatFile), inittemp(""), geofile(deck.meshFile);
     DistributeMesh *dm = new DistributeMesh(deck.dimGeo, geofile, deck.Lx, deck.Ly, deck.Lz, deck.meshType, bands, bcs,

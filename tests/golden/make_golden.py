@@ -108,22 +108,8 @@ def golden_decks():
 
 # tables a 3D run hands to ref_dump (--tables): the reference's 3D mesh
 # reader and geometry were removed, so its solvers run on ours
-TABLE_BINS = ["face_size", "face_neighbor", "face_bound", "face_index", "face_nvert", "face_vert", "cell_nvert",
-              "cell_vert", "matter", "face_area", "face_nx", "face_ny", "face_nz", "face_cx", "face_cy", "face_cz",
-              "volume", "cx", "cy", "cz", "heat_source", "initial_temp", "node_x", "node_y", "node_z", "bound_cell",
-              "bound_face", "bound_type", "bound_connect", "bound_temp", "fface_cell", "fface_face", "fface_bound"]
-
-
 def write_tables(problem, out_dir):
-    """The problem's flattened tables as raw little-endian arrays."""
-    os.makedirs(out_dir, exist_ok=True)
-    t = problem.tables
-    for name in TABLE_BINS:
-        a = problem.array(name)
-        a = a.astype("<i4" if a.dtype.kind == "i" else "<f8")
-        a.tofile(os.path.join(out_dir, name + ".bin"))
-    np.array([t.n_cells, t.n_nodes, t.n_bounds, t.n_faces, t.dim], "<i4").tofile(os.path.join(out_dir, "sizes.bin"))
-    np.array([t.Lx, t.Ly, t.Lz], "<f8").tofile(os.path.join(out_dir, "scale.bin"))
+    problem.write_tables(out_dir)
 
 
 def run_one(name, factory, fmt, max_iter, tolT, tolQ, stage):
