@@ -137,6 +137,7 @@ class Solver {
     int solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st);
     void launch_rhs3(bool macro, const double *bval, double *rhs);
     void launch_grad3();
+    bool src_width_ok(int v) const;
     void launch_source3(const Modes &m);
     void launch_bound_out3(const Modes &m);
     void launch_moments3(const Modes &m);
@@ -205,6 +206,7 @@ class Solver {
     DBuf<LvRec> d_lv_rec;
     int lv_bc = 0, lv_np = 2, s_sh = -1;
     int lv_v = 1, lv_tpb = 256;  // lanes per position (1: a lane takes the whole chunk), threads per workgroup
+    int src_v = 0;  // FP32 3D source: bands per lane (2 or 4; 0: 2 where the layout takes it, else 1)
     // gradient stencil
     DBuf<int> d_gptr, d_gidx, d_gok;
     DBuf<double> d_gP;
@@ -236,6 +238,7 @@ class Solver {
     // 3D: source records, LSQ gradient tiles, moments, residual and
     // right-hand-side coefficients (solver3d.hip)
     DBuf<double> d_s3d, d_Y, d_YV, d_f3d, d_r3d[2];
+    DBuf<unsigned char> d_oflag;  // 3D out-face flags per (cell, direction), bit per face
     DBuf<int> d_s3i, d_f3i, d_r3i[2];
     DBuf<char> d_g3g_ent;
     DBuf<int> d_g3g_ptr, d_g3g_idx;

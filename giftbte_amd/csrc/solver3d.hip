@@ -72,7 +72,8 @@ typedef float v2f __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kG3W = 64;          // entries per slice
-constexpr int kG3Ent = 8;         // stencil entries per batch (stencils padded to a multiple)
+constexpr int kG3Ent = 8;         // stencil entries per batch
+constexpr int kG3Pad = 4;         // stencils padded to a multiple (a batch of 8, then one of 4)
 constexpr int kFin = 21;          // k_fin3 coefficients per face slot
 constexpr int kMomRows = 4;       // k_mom3: rows (waves) per workgroup
 
@@ -92,39 +93,76 @@ struct alignas(16) G3Ent<float> {
 
 // ----------------------------------------------------------------- kernels
 
+// Row loads of the gradient kernels: a buffer resource per row (built from
+// the wave-uniform row address by scalar instructions) and a per-lane 32-bit
+// byte offset, so that a load takes no per-lane 64-bit address arithmetic.
+// Word 3 is the gfx9 default (32-bit data format, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void *row, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T ld_row(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <>
+__device__ __forceinline__ float ld_row<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double ld_row<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ v4f ld_row<v4f>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ v4d ld_row<v4d>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    v2d lo = __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+    v2d hi = __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off + 16, 0, 0));
+    return v4d{lo[0], lo[1], hi[0], hi[1]};
+}
+
+// One batch of N stencil-table entries: the table entries by scalar loads
+// (they serve all GS slices of the wave), then N * GS row loads before the
+// products.
+template <typename ST, int GS, bool S0, int N>
+__device__ __forceinline__ void g3_batch(const ST *__restrict__ X, const int *__restrict__ gidx,
+                                         const G3Ent<ST> *__restrict__ ent, int k, const unsigned *to, size_t DB,
+                                         const ST *self, ST *gx, ST *gy, ST *gz, ST *s0) {
+    G3Ent<ST> w[N];
+    __amdgpu_buffer_rsrc_t row[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        w[j] = ent[k + j];
+        row[j] = row_rsrc(X + (size_t)gidx[k + j] * DB, (unsigned)(DB * sizeof(ST)));
+    }
+    ST v[N][GS];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int q = 0; q < GS; ++q) v[j][q] = ld_row<ST>(row[j], to[q]);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int q = 0; q < GS; ++q) {
+            ST d = v[j][q] - self[q];
+            gx[q] += w[j].px * d;
+            gy[q] += w[j].py * d;
+            gz[q] += w[j].pz * d;
+            if constexpr (S0) s0[q] += d;
+        }
+}
+
 // The entry loop of the gathering LSQ kernels: g += w_j (X[row j] - self)
-// over the stencil-table entries a..z (padded to whole batches), with S0 also
-// the plain sum of the differences.  A batch of table entries (scalar loads)
-// serves all GS slices of the wave, and each batch issues kG3Ent * GS row
-// loads before its products.
+// over the stencil-table entries a..z (padded to a multiple of kG3Pad), with
+// S0 also the plain sum of the differences; to[q] is slice q's byte offset
+// in a row.
 template <typename ST, int GS, bool S0>
 __device__ __forceinline__ void g3_entries(const ST *__restrict__ X, const int *__restrict__ gidx,
-                                           const G3Ent<ST> *__restrict__ ent, int a, int z, const int *tt, int DB,
+                                           const G3Ent<ST> *__restrict__ ent, int a, int z, const unsigned *to, int DB,
                                            const ST *self, ST *gx, ST *gy, ST *gz, ST *s0) {
-    for (int k = a; k < z; k += kG3Ent) {
-        G3Ent<ST> w[kG3Ent];
-        int nb[kG3Ent];
-#pragma unroll
-        for (int j = 0; j < kG3Ent; ++j) {
-            w[j] = ent[k + j];
-            nb[j] = gidx[k + j];
-        }
-        ST v[kG3Ent][GS];
-#pragma unroll
-        for (int j = 0; j < kG3Ent; ++j)
-#pragma unroll
-            for (int q = 0; q < GS; ++q) v[j][q] = X[(size_t)nb[j] * DB + tt[q]];
-#pragma unroll
-        for (int j = 0; j < kG3Ent; ++j)
-#pragma unroll
-            for (int q = 0; q < GS; ++q) {
-                ST d = v[j][q] - self[q];
-                gx[q] += w[j].px * d;
-                gy[q] += w[j].py * d;
-                gz[q] += w[j].pz * d;
-                if constexpr (S0) s0[q] += d;
-            }
-    }
+    int k = a;
+    for (; k + kG3Ent <= z; k += kG3Ent) g3_batch<ST, GS, S0, kG3Ent>(X, gidx, ent, k, to, DB, self, gx, gy, gz, s0);
+    if (k < z) g3_batch<ST, GS, S0, kG3Pad>(X, gidx, ent, k, to, DB, self, gx, gy, gz, s0);
 }
 
 // A face projection phi_f is read only where the face is an out-face of
@@ -135,20 +173,68 @@ __device__ __forceinline__ void g3_entries(const ST *__restrict__ X, const int *
 // v s . n is negative too; the rows are zeroed once at set-up.
 constexpr double kInFace = 1e-12;
 
-// phi_f = r_f . g for the out-faces of cell c in the direction of entry t
-// (NaN -> 0, the reference's zero gradient)
-template <typename ST>
-__device__ __forceinline__ void g3_store_phi(ST *__restrict__ phi, const double *__restrict__ rc, int c, int t, int DB,
-                                             int nB, const Modes &m, double x, double y, double w) {
-    if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
-    int d = t / nB;
+// The direction of entry t = d nB + b: floor((t + 0.5) / nB) in FP32 with
+// rnB = 1 / nB, exact while t < 2^21 (the relative error, below 2^-22, then
+// moves the quotient by less than its distance 0.5 / nB to an integer); the
+// 3D set-up rejects larger dir x band counts.
+__device__ __forceinline__ int dir_of(int t, float rnB) { return (int)(((float)t + 0.5f) * rnB); }
+
+// Out-face flags per (cell, direction), bit f where s . n_f >= -kInFace:
+// the faces whose projections the gradient kernels write (set-up; one byte
+// load per lane then replaces three direction loads and the dot products).
+__global__ void k_oflags3(const double *__restrict__ rec, Modes m, int nc, int nD, unsigned char *__restrict__ fl) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nc * nD) return;
+    int c = (int)(i / nD), d = (int)(i - (size_t)c * nD);
+    const double *rc = rec + (size_t)c * kS3D;
     double sx = m.sx[d], sy = m.sy[d], sz = m.sz[d];
-    ST *pc = phi + (size_t)c * 4 * DB + t;
+    unsigned v = 0;
 #pragma unroll
     for (int f = 0; f < 4; ++f)
-        if (sx * rc[f * 10] + sy * rc[f * 10 + 1] + sz * rc[f * 10 + 2] >= -kInFace)
-            __builtin_nontemporal_store((ST)(rc[f * 10 + 4] * x + rc[f * 10 + 5] * y + rc[f * 10 + 6] * w),
-                                        pc + f * DB);
+        if (sx * rc[f * 10] + sy * rc[f * 10 + 1] + sz * rc[f * 10 + 2] >= -kInFace) v |= 1u << f;
+    fl[i] = (unsigned char)v;
+}
+
+// phi_f = r_f . g for the out-faces of cell c in the entries tb + q kG3W
+// (q < GS, those below DB), NaN -> 0 (the reference's zero gradient), the
+// out-faces from the set-up flags.  FP64 storage takes the projections in
+// FP64, FP32 storage in FP32 with the face offsets converted once per wave.
+template <typename ST, int GS>
+__device__ __forceinline__ void g3_store_phi(ST *__restrict__ phi, const double *__restrict__ rc,
+                                             const unsigned char *__restrict__ oflag, int c, int tb, int DB, int nB,
+                                             ST *gx, ST *gy, ST *gz) {
+    float rnB = 1.0f / (float)nB;
+    int nD = DB / nB;
+    __amdgpu_buffer_rsrc_t fr_c = row_rsrc(oflag + (size_t)c * nD, (unsigned)nD);
+    ST *pc = phi + (size_t)c * 4 * DB;
+    using AT = typename std::conditional<sizeof(ST) == 4, float, double>::type;
+    AT fr[4][3];
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) fr[f][a] = (AT)rc[f * 10 + 4 + a];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+        int t = tb + q * kG3W;
+        if (t >= DB) continue;
+        AT x = gx[q], y = gy[q], w = gz[q];
+        if (isnan(x) || isnan(y) || isnan(w)) { x = 0; y = 0; w = 0; }
+        unsigned fl = __builtin_amdgcn_raw_buffer_load_b8(fr_c, dir_of(t, rnB), 0, 0);
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+            if (fl & (1u << f)) __builtin_nontemporal_store((ST)(fr[f][0] * x + fr[f][1] * y + fr[f][2] * w), pc + f * DB + t);
+    }
+}
+
+// The per-lane part of a gradient wave: slice q's entry t (clamped to 0
+// past the row's end) as a byte offset in a row.
+template <typename ST, int GS>
+__device__ __forceinline__ void g3_offsets(int tb, int DB, unsigned *to) {
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+        int t = tb + q * kG3W;
+        to[q] = (unsigned)(t < DB ? t : 0) * (unsigned)sizeof(ST);
+    }
 }
 
 // _get_gradient_larger, 3D (StaticBTESynthetic.cpp:1329-1371):
@@ -166,27 +252,23 @@ template <typename ST, int GS, int WPB>
 __global__ void __launch_bounds__(64 * WPB) k_grad3g(const ST *__restrict__ e, ST *__restrict__ phi,
                                                      const int *__restrict__ sptr, const int *__restrict__ gidx,
                                                      const G3Ent<ST> *__restrict__ ent, const double *__restrict__ rec,
-                                                     Modes m, int nc, int DB, int nB, int ngroups) {
+                                                     const unsigned char *__restrict__ oflag, int nc, int DB, int nB,
+                                                     int ngroups) {
     int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     int sg = wv / nc, c = wv - sg * nc;
     if (sg >= ngroups) return;  // whole wave
     int tb = sg * GS * kG3W + lane;
     ST self[GS], gx[GS], gy[GS], gz[GS];
-    int tt[GS];
+    unsigned to[GS];
+    g3_offsets<ST, GS>(tb, DB, to);
+    __amdgpu_buffer_rsrc_t ec = row_rsrc(e + (size_t)c * DB, (unsigned)(DB * sizeof(ST)));
 #pragma unroll
     for (int q = 0; q < GS; ++q) {
-        int t = tb + q * kG3W;
-        tt[q] = t < DB ? t : 0;
-        self[q] = e[(size_t)c * DB + tt[q]];
+        self[q] = ld_row<ST>(ec, to[q]);
         gx[q] = 0; gy[q] = 0; gz[q] = 0;
     }
-    g3_entries<ST, GS, false>(e, gidx, ent, sptr[c], sptr[c + 1], tt, DB, self, gx, gy, gz, nullptr);
-    const double *rc = rec + (size_t)c * kS3D;
-#pragma unroll
-    for (int q = 0; q < GS; ++q) {
-        int t = tb + q * kG3W;
-        if (t < DB) g3_store_phi(phi, rc, c, t, DB, nB, m, gx[q], gy[q], gz[q]);
-    }
+    g3_entries<ST, GS, false>(e, gidx, ent, sptr[c], sptr[c + 1], to, DB, self, gx, gy, gz, nullptr);
+    g3_store_phi<ST, GS>(phi, rec + (size_t)c * kS3D, oflag, c, tb, DB, nB, gx, gy, gz);
 }
 
 // The same gradient through sums over the mesh vertices, which halves the
@@ -217,21 +299,21 @@ __global__ void __launch_bounds__(64 * WPB) k_vsum3(const ST *__restrict__ e, ST
     int sg = wv / nv, v = wv - sg * nv;
     if (sg >= ngroups) return;  // whole wave
     int tb = sg * GS * kG3W + lane;
-    int r = vref[v];
     ST self[GS], gx[GS], gy[GS], gz[GS], s0[GS];
-    int tt[GS];
+    unsigned to[GS];
+    g3_offsets<ST, GS>(tb, DB, to);
+    __amdgpu_buffer_rsrc_t er = row_rsrc(e + (size_t)vref[v] * DB, (unsigned)(DB * sizeof(ST)));
 #pragma unroll
     for (int q = 0; q < GS; ++q) {
-        int t = tb + q * kG3W;
-        tt[q] = t < DB ? t : 0;
-        self[q] = e[(size_t)r * DB + tt[q]];
+        self[q] = ld_row<ST>(er, to[q]);
         gx[q] = 0; gy[q] = 0; gz[q] = 0; s0[q] = 0;
     }
-    g3_entries<ST, GS, true>(e, vidx, vent, vptr[v], vptr[v + 1], tt, DB, self, gx, gy, gz, s0);
+    g3_entries<ST, GS, true>(e, vidx, vent, vptr[v], vptr[v + 1], to, DB, self, gx, gy, gz, s0);
+    Q4 *vv = reinterpret_cast<Q4 *>(vs + (size_t)v * DB * 4);
 #pragma unroll
     for (int q = 0; q < GS; ++q) {
         int t = tb + q * kG3W;
-        if (t < DB) *reinterpret_cast<Q4 *>(vs + ((size_t)v * DB + t) * 4) = Q4{gx[q], gy[q], gz[q], s0[q]};
+        if (t < DB) vv[t] = Q4{gx[q], gy[q], gz[q], s0[q]};
     }
 }
 
@@ -251,7 +333,8 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3v(const ST *__restrict__ e, c
                                                      ST *__restrict__ phi, const G3VRec<ST> *__restrict__ vrec,
                                                      const int *__restrict__ sptr, const int *__restrict__ gidx,
                                                      const G3Ent<ST> *__restrict__ ent, const double *__restrict__ rec,
-                                                     Modes m, int nc, int DB, int nB, int ngroups) {
+                                                     const unsigned char *__restrict__ oflag, int nc, int DB, int nB,
+                                                     int ngroups) {
     using Q4 = typename std::conditional<sizeof(ST) == 4, v4f, v4d>::type;
     int wv = __builtin_amdgcn_readfirstlane(xcd_block() * WPB + (int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     int sg = wv / nc, c = wv - sg * nc;
@@ -259,20 +342,21 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3v(const ST *__restrict__ e, c
     int tb = sg * GS * kG3W + lane;
     const G3VRec<ST> &R = vrec[c];
     ST self[GS], gx[GS], gy[GS], gz[GS], sx[GS], sy[GS], sz[GS];
-    int tt[GS];
+    unsigned to[GS];
+    g3_offsets<ST, GS>(tb, DB, to);
+    __amdgpu_buffer_rsrc_t ec = row_rsrc(e + (size_t)c * DB, (unsigned)(DB * sizeof(ST)));
 #pragma unroll
     for (int q = 0; q < GS; ++q) {
-        int t = tb + q * kG3W;
-        tt[q] = t < DB ? t : 0;
-        self[q] = e[(size_t)c * DB + tt[q]];
+        self[q] = ld_row<ST>(ec, to[q]);
         gx[q] = 0; gy[q] = 0; gz[q] = 0; sx[q] = 0; sy[q] = 0; sz[q] = 0;
     }
     // the vertex sums: S1 summed over the four vertices, S0 weighted per vertex
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+        __amdgpu_buffer_rsrc_t vb = row_rsrc(vs + (size_t)R.v[i] * DB * 4, (unsigned)(DB * 4 * sizeof(ST)));
         Q4 qv[GS];
 #pragma unroll
-        for (int q = 0; q < GS; ++q) qv[q] = *reinterpret_cast<const Q4 *>(vs + ((size_t)R.v[i] * DB + tt[q]) * 4);
+        for (int q = 0; q < GS; ++q) qv[q] = ld_row<Q4>(vb, to[q] * 4);
 #pragma unroll
         for (int q = 0; q < GS; ++q) {
             sx[q] += qv[q][0]; sy[q] += qv[q][1]; sz[q] += qv[q][2];
@@ -285,13 +369,8 @@ __global__ void __launch_bounds__(64 * WPB) k_grad3v(const ST *__restrict__ e, c
         gy[q] += R.a[3] * sx[q] + R.a[4] * sy[q] + R.a[5] * sz[q];
         gz[q] += R.a[6] * sx[q] + R.a[7] * sy[q] + R.a[8] * sz[q];
     }
-    g3_entries<ST, GS, false>(e, gidx, ent, sptr[c], sptr[c + 1], tt, DB, self, gx, gy, gz, nullptr);
-    const double *rc = rec + (size_t)c * kS3D;
-#pragma unroll
-    for (int q = 0; q < GS; ++q) {
-        int t = tb + q * kG3W;
-        if (t < DB) g3_store_phi(phi, rc, c, t, DB, nB, m, gx[q], gy[q], gz[q]);
-    }
+    g3_entries<ST, GS, false>(e, gidx, ent, sptr[c], sptr[c + 1], to, DB, self, gx, gy, gz, nullptr);
+    g3_store_phi<ST, GS>(phi, rec + (size_t)c * kS3D, oflag, c, tb, DB, nB, gx, gy, gz);
 }
 
 // The same face projections on the matrix cores.  Tiles of up to four
@@ -547,17 +626,23 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3(const int *__restric
     }
 }
 
-// FP32 storage, an even band count: the same source with two bands of one
-// direction per lane, so every face value is one 8-byte load of a pair and
-// both sources go out in one 8-byte store; with one entry per lane the float
-// kernel issues as many memory instructions as the FP64 one for half the
-// bytes.  Each entry's arithmetic is k_source3's.  A face whose two entries
-// take different rows (one band of zero group velocity, v . n = 0 counting as
-// outgoing) loads the other row too.
-__global__ void __launch_bounds__(64 * kSrc3Rows) k_source3_f32x2(
+// FP32 storage, a band count divisible by V (2 or 4): the same source with V
+// bands of one direction per lane, so every face value is one 8- or 16-byte
+// load and the V sources go out in one store.  The vector memory pipeline
+// bounds this kernel, and it issues a 16-byte load per lane at about the cost
+// of an 8-byte one, so V = 4 moves twice the bytes per instruction.  Each
+// entry's arithmetic is k_source3's.  A face's row (own projection or the
+// upwind side's) is chosen once per direction by the sign of s . n, which the
+// V bands share: their v s . n can differ in sign only where s . n vanishes
+// to rounding, and there the term itself is that small; the projections are
+// written where s . n >= -1e-12 on either side, so both rows hold values
+// there.  Two iterations' loads are in flight.
+template <int V>
+__global__ void __launch_bounds__(64 * kSrc3Rows) k_source3_f32v(
     const int *__restrict__ posc, float *__restrict__ S, const float *__restrict__ phi, const float *__restrict__ eb,
     const double *__restrict__ Told, const double *__restrict__ Q, const double *__restrict__ rec,
     const int *__restrict__ reci, Modes m, int nc, int nB, int nD, int sh) {
+    using VT = typename std::conditional<V == 4, v4f, v2f>::type;
     extern __shared__ double lds[];
     double *mvg = lds, *mtau = mvg + nB, *mcap = mtau + nB, *mhr = mcap + nB, *msx = mhr + nB, *msy = msx + nD,
            *msz = msy + nD;
@@ -596,36 +681,31 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3_f32x2(
     }
     double To = Told[c] - m.eref, Qc = Q[c];
     const int *pc = pos + w * nD;
-    // two entries per lane and two iterations' loads in flight; a face's row
-    // (own projection or the upwind side's) is chosen once per direction by
-    // the sign of s . n, which the two bands share (the bands' v s . n can
-    // differ in sign only where s . n vanishes to rounding, and there the
-    // term itself is that small; the projections are written where
-    // s . n >= -1e-12 on either side, so both rows hold values there)
 #pragma unroll 2
-    for (int t = 2 * lane; t < DB; t += 128) {
+    for (int t = V * lane; t < DB; t += 64 * V) {
         int d = t / nB, b = t - d * nB;
-        double vb0 = mvg[b], vb1 = mvg[b + 1], sx = msx[d], sy = msy[d], sz = msz[d];
-        double dot0[4], dot1[4];
-        float2 u[4];
+        double sx = msx[d], sy = msy[d], sz = msz[d];
+        VT u[4];
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
-            dot0[f] = vb0 * sx * fnx[f] + vb0 * sy * fny[f] + vb0 * sz * fnz[f];
-            dot1[f] = vb1 * sx * fnx[f] + vb1 * sy * fny[f] + vb1 * sz * fnz[f];
             double sn = sx * fnx[f] + sy * fny[f] + sz * fnz[f];
-            u[f] = *reinterpret_cast<const float2 *>((sn >= 0 ? fo[f] : fin[f]) + t);
+            u[f] = *reinterpret_cast<const VT *>((sn >= 0 ? fo[f] : fin[f]) + t);
         }
-        double s0 = To * mcap[b] + Qc * mhr[b] * mtau[b];
-        double s1 = To * mcap[b + 1] + Qc * mhr[b + 1] * mtau[b + 1];
+        VT o;
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            s0 -= mtau[b] * fav[f] * dot0[f] * (double)u[f].x;
-            s1 -= mtau[b + 1] * fav[f] * dot1[f] * (double)u[f].y;
+        for (int j = 0; j < V; ++j) {
+            double vb = mvg[b + j], tb = mtau[b + j];
+            double sj = To * mcap[b + j] + Qc * mhr[b + j] * tb;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double dot = vb * sx * fnx[f] + vb * sy * fny[f] + vb * sz * fnz[f];
+                sj -= tb * fav[f] * dot * (double)u[f][j];
+            }
+            o[j] = (float)sj;
         }
         size_t si = sh < 0 ? ((size_t)d * nc + pc[d]) * nB + b
                            : ((((size_t)d * (nB >> sh) + (b >> sh)) * nc + pc[d]) << sh) + (b & ((1 << sh) - 1));
-        v2f o = {(float)s0, (float)s1};
-        __builtin_nontemporal_store(o, reinterpret_cast<v2f *>(S + si));
+        __builtin_nontemporal_store(o, reinterpret_cast<VT *>(S + si));
     }
 }
 
@@ -1020,6 +1100,11 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     double invLx = 1.0 / p.deck.Lx;
     for (int c = 0; c < nc; ++c)
         if (p.face_size[c] != 4) { err_ = "3D meshes of tetrahedra only"; return false; }
+    // the gradient kernels find an entry's direction by an FP32 quotient
+    if ((long long)nD * nB >= (1LL << 21)) {
+        err_ = "3D meshes take fewer than 2^21 direction x band systems";
+        return false;
+    }
     // ---- source records (kept: the face offsets also weight the gradient's
     // face projections)
     std::vector<double> rec_h((size_t)nc * kS3D, 0.0);
@@ -1061,6 +1146,11 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 return false;
             }
         if (!upload(d_s3d, rec) || !upload(d_s3i, reci)) return false;
+        size_t nfl = (size_t)nc * nD;
+        if (!alloc(d_oflag, nfl)) return false;
+        hipLaunchKernelGGL(k_oflags3, dim3((unsigned)((nfl + 255) / 256)), dim3(256), 0, s_, d_s3d.p, modes(), nc, nD,
+                           d_oflag.p);
+        if (!check(hipGetLastError(), "k_oflags3")) return false;
     }
     // ---- LSQ weights (_get_CellMatrix_larger + Get_inverse_3D): padded row
     // tables for the Fourier flux kernel, and the k_grad3 tiles
@@ -1121,15 +1211,15 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
     bool need_g = !gv_ok || opt.gradient_kernel == GBTE_GRAD_GATHER;
     bool need_m = !gv_ok || opt.gradient_kernel == GBTE_GRAD_MATRIX;
     // ---- gradient stencil tables: global neighbour rows, padded per cell to
-    // whole batches with the cell's own row at weight 0 (each adds an exact
-    // zero)
+    // a multiple of kG3Pad with the cell's own row at weight 0 (each adds an
+    // exact zero)
     {
         int nnz = p.nbr_ptr[nc];
         if (need_g) {
         std::vector<int> gptr2(nc + 1, 0);
         for (int cc = 0; cc < nc; ++cc) {
             int m = p.nbr_ptr[cc + 1] - p.nbr_ptr[cc];
-            gptr2[cc + 1] = gptr2[cc] + (m + kG3Ent - 1) / kG3Ent * kG3Ent;
+            gptr2[cc + 1] = gptr2[cc] + (m + kG3Pad - 1) / kG3Pad * kG3Pad;
         }
         size_t n2 = (size_t)gptr2[nc];
         std::vector<int> gidx2(n2);
@@ -1476,7 +1566,7 @@ bool Solver::build_gv(double invLx) {
             vw.push_back(invLx * (p.cy[k] - p.node_y[v]));
             vw.push_back(invLx * (p.cz[k] - p.node_z[v]));
         }
-        while ((vidx.size() - vptr[i]) % kG3Ent) {
+        while ((vidx.size() - vptr[i]) % kG3Pad) {
             vidx.push_back(rv[v]);
             vw.insert(vw.end(), {0.0, 0.0, 0.0});
         }
@@ -1562,7 +1652,7 @@ bool Solver::build_gv(double invLx) {
             for (int r = 0; r < 3; ++r)
                 o.w.push_back(invLx * (M[r][0] * x.second[0] + M[r][1] * x.second[1] + M[r][2] * x.second[2]));
         }
-        while (o.idx.size() % kG3Ent) {
+        while (o.idx.size() % kG3Pad) {
             o.idx.push_back(c);
             o.w.insert(o.w.end(), {0.0, 0.0, 0.0});
         }
@@ -1659,8 +1749,8 @@ void Solver::launch_grad3() {
         hipLaunchKernelGGL((k_grad3v<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                           \
                            reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<const ST_ *>(d_gv_sums.p),     \
                            reinterpret_cast<ST_ *>(d_grad.p), reinterpret_cast<const G3VRec<ST_> *>(d_gv_rec.p), \
-                           d_gv_ptr.p, d_gv_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_gv_ent.p), d_s3d.p, md, \
-                           nc, DB, nB, ngroups);                                                                 \
+                           d_gv_ptr.p, d_gv_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_gv_ent.p), d_s3d.p,     \
+                           d_oflag.p, nc, DB, nB, ngroups);                                                      \
     } while (0)
 #define GBTE_G3(ST_, GS_, W_)                                                                                    \
     do {                                                                                                         \
@@ -1670,7 +1760,7 @@ void Solver::launch_grad3() {
             hipLaunchKernelGGL((k_grad3g<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                       \
                                reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<ST_ *>(d_grad.p),          \
                                d_g3g_ptr.p, d_g3g_idx.p, reinterpret_cast<const G3Ent<ST_> *>(d_g3g_ent.p),      \
-                               d_s3d.p, md, nc, DB, nB, ngroups);                                                \
+                               d_s3d.p, d_oflag.p, nc, DB, nB, ngroups);                                         \
     } while (0)
 #define GBTE_G3_W(ST_, GS_)                         \
     do {                                            \
@@ -1692,11 +1782,22 @@ void Solver::launch_grad3() {
 #undef GBTE_G3V
 }
 
+// FP32 3D source: V bands per lane take a band count divisible by V and, for
+// S per band chunk, chunks of a multiple of V bands
+bool Solver::src_width_ok(int v) const {
+    return v == 1 || (fp32_ && nB % v == 0 && (s_sh < 0 || (1 << s_sh) % v == 0));
+}
+
 void Solver::launch_source3(const Modes &m) {
     int grid = ((blocks_for(nc, kSrc3Rows) + kXcd - 1) / kXcd) * kXcd;
     size_t lds = (size_t)(4 * nB + 3 * nD) * sizeof(double) + (size_t)kSrc3Rows * nD * sizeof(int);
-    if (fp32_ && nB % 2 == 0)
-        hipLaunchKernelGGL(k_source3_f32x2, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
+    int v = fp32_ ? (src_v > 0 && src_width_ok(src_v) ? src_v : src_width_ok(2) ? 2 : 1) : 1;
+    if (v == 4)
+        hipLaunchKernelGGL(k_source3_f32v<4>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
+                           reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
+                           reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
+    else if (v == 2)
+        hipLaunchKernelGGL(k_source3_f32v<2>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
                            reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
                            reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
     else if (fp32_)
@@ -2021,6 +2122,26 @@ bool Solver::tune3d() {
             }
             s_sh = tl[0] <= tl[1] ? shc : -1;
             snprintf(line, sizeof line, "S layout chosen: %s\n", s_sh < 0 ? "per position" : "per band chunk");
+            tune_log_ += line;
+        }
+    }
+    if (fp32_) {
+        // FP32 source: bands per lane (8- or 16-byte face loads)
+        Modes m = modes();
+        double best = 1e30;
+        int pick = 0;
+        for (int v : {2, 4}) {
+            if (!src_width_ok(v)) continue;
+            src_v = v;
+            double per = 0;
+            if (!time_it([&]() { launch_source3(m); }, best, per)) return false;
+            snprintf(line, sizeof line, "source %d bands per lane: %.3f ms per launch\n", v, per);
+            tune_log_ += line;
+            if (per < best) { best = per; pick = v; }
+        }
+        src_v = pick;
+        if (pick) {
+            snprintf(line, sizeof line, "source chosen: %d bands per lane\n", pick);
             tune_log_ += line;
         }
     }
