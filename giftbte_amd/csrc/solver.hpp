@@ -239,6 +239,7 @@ class Solver {
     // right-hand-side coefficients (solver3d.hip)
     DBuf<double> d_s3d, d_Y, d_YV, d_f3d, d_r3d[2];
     DBuf<unsigned char> d_oflag;  // 3D out-face flags per (cell, direction), bit per face
+    DBuf<double> d_dirf;          // 3D moments: direction factors per direction
     DBuf<int> d_s3i, d_f3i, d_r3i[2];
     DBuf<char> d_g3g_ent;
     DBuf<int> d_g3g_ptr, d_g3g_idx;

@@ -885,51 +885,41 @@ __global__ void __launch_bounds__(256) k_bound_out3(const ST *__restrict__ e, co
 // Row moments Y_0..10 (see the top of this file) of the cell rows of e and
 // of the boundary-entry rows of eb.  One wave per row with the lanes over the
 // bands: each lane sums its band over the directions with the direction
-// factors (w, w s_a s_b, w s_a), then multiplies by the band factors and the
-// wave adds the bands in a fixed order.
+// factors (w, w s_a s_b, w s_a; a set-up table, kDirF per direction, read by
+// scalar loads), then multiplies by the band factors and the wave adds the
+// bands in a fixed order.
+constexpr int kDirF = 10;
 template <typename ST>
 __global__ void __launch_bounds__(64 * kMomRows) k_mom3(const ST *__restrict__ e, const ST *__restrict__ eb,
-                                                        const int *__restrict__ ent_list, Modes m,
-                                                        double *__restrict__ Y, int nc, int n_ent, int nB, int nD) {
+                                                        const int *__restrict__ ent_list, const double *__restrict__ dirf,
+                                                        Modes m, double *__restrict__ Y, int nc, int n_ent, int nB,
+                                                        int nD) {
     int r = xcd_block() * kMomRows + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (r >= nc + n_ent) return;  // whole wave
     const ST *row = r < nc ? e + (size_t)r * nD * nB : eb + (size_t)ent_list[r - nc] * nD * nB;
-    double y[11];
+    int b = lane < nB ? lane : 0;  // nB <= 64; lanes past nB add nothing (their band factors are 0)
+    double f[kDirF];
 #pragma unroll
-    for (int j = 0; j < 11; ++j) y[j] = 0;
-    for (int b = lane; b < ((nB + 63) & ~63); b += 64) {
-        bool act = b < nB;
-        double sw = 0, sxx = 0, sxy = 0, sxz = 0, syy = 0, syz = 0, szz = 0, qx = 0, qy = 0, qz = 0;
-        if (act)
-            for (int d = 0; d < nD; ++d) {
-                double v = row[d * nB + b], wd = m.w[d], sx = m.sx[d], sy = m.sy[d], sz = m.sz[d];
-                double wv = wd * v;
-                sw += wv;
-                sxx += wv * (sx * sx);
-                sxy += wv * (sx * sy);
-                sxz += wv * (sx * sz);
-                syy += wv * (sy * sy);
-                syz += wv * (sy * sz);
-                szz += wv * (sz * sz);
-                qx += wv * sx;
-                qy += wv * sy;
-                qz += wv * sz;
-            }
-        double vb = act ? m.vg[b] : 0.0, lrc = act ? m.lrc[b] : 0.0;
-        double h = act && m.hr[b] > 0 ? 1.0 : 0.0;
-        double kk = h * m.kappa * lrc, tv = h * (act ? m.tau[b] : 0.0) * vb * vb;
-        y[0] += kk * sw;
-        y[1] += tv * sxx;
-        y[2] += tv * sxy;
-        y[3] += tv * sxz;
-        y[4] += tv * syy;
-        y[5] += tv * syz;
-        y[6] += tv * szz;
-        y[7] += lrc * sw;
-        y[8] += vb * qx;
-        y[9] += vb * qy;
-        y[10] += vb * qz;
+    for (int j = 0; j < kDirF; ++j) f[j] = 0;
+#pragma unroll 4
+    for (int d = 0; d < nD; ++d) {
+        double v = row[d * nB + b];
+        const double *df = dirf + (size_t)d * kDirF;
+#pragma unroll
+        for (int j = 0; j < kDirF; ++j) f[j] += df[j] * v;
     }
+    bool act = lane < nB;
+    double vb = act ? m.vg[b] : 0.0, lrc = act ? m.lrc[b] : 0.0;
+    double h = act && m.hr[b] > 0 ? 1.0 : 0.0;
+    double kk = h * m.kappa * lrc, tv = h * (act ? m.tau[b] : 0.0) * vb * vb;
+    double y[11];
+    y[0] = kk * f[0];
+#pragma unroll
+    for (int j = 1; j < 7; ++j) y[j] = tv * f[j];
+    y[7] = lrc * f[0];
+    y[8] = vb * f[7];
+    y[9] = vb * f[8];
+    y[10] = vb * f[9];
 #pragma unroll
     for (int j = 0; j < 11; ++j) y[j] = wave_sum(y[j]);
     // boundary entry en's moments are row nc + en
@@ -1146,6 +1136,17 @@ bool Solver::build3d(const std::vector<int> &be_slot) {
                 return false;
             }
         if (!upload(d_s3d, rec) || !upload(d_s3i, reci)) return false;
+        // the moments' direction factors: w, w s_a s_b (xx xy xz yy yz zz), w s_a
+        std::vector<double> dirf((size_t)nD * kDirF);
+        for (int d = 0; d < nD; ++d) {
+            double w = p.weight[d], x = p.dir_x[d], y = p.dir_y[d], z = p.dir_z[d];
+            double *f = &dirf[(size_t)d * kDirF];
+            f[0] = w;
+            f[1] = w * (x * x); f[2] = w * (x * y); f[3] = w * (x * z);
+            f[4] = w * (y * y); f[5] = w * (y * z); f[6] = w * (z * z);
+            f[7] = w * x; f[8] = w * y; f[9] = w * z;
+        }
+        if (!upload(d_dirf, dirf)) return false;
         size_t nfl = (size_t)nc * nD;
         if (!alloc(d_oflag, nfl)) return false;
         hipLaunchKernelGGL(k_oflags3, dim3((unsigned)((nfl + 255) / 256)), dim3(256), 0, s_, d_s3d.p, modes(), nc, nD,
@@ -1896,10 +1897,11 @@ void Solver::launch_moments3(const Modes &m) {
     int grid = ((blocks_for(nc + n_ent, kMomRows) + kXcd - 1) / kXcd) * kXcd;
     if (fp32_)
         hipLaunchKernelGGL(k_mom3<float>, dim3(grid), dim3(64 * kMomRows), 0, s_, reinterpret_cast<const float *>(d_e.p),
-                           reinterpret_cast<const float *>(d_eb.p), d_ent_list.p, m, d_Y.p, nc, n_ent, nB, nD);
+                           reinterpret_cast<const float *>(d_eb.p), d_ent_list.p, d_dirf.p, m, d_Y.p, nc, n_ent, nB,
+                           nD);
     else
-        hipLaunchKernelGGL(k_mom3<double>, dim3(grid), dim3(64 * kMomRows), 0, s_, d_e.p, d_eb.p, d_ent_list.p, m,
-                           d_Y.p, nc, n_ent, nB, nD);
+        hipLaunchKernelGGL(k_mom3<double>, dim3(grid), dim3(64 * kMomRows), 0, s_, d_e.p, d_eb.p, d_ent_list.p,
+                           d_dirf.p, m, d_Y.p, nc, n_ent, nB, nD);
     hipLaunchKernelGGL(k_vert3, dim3(blocks_for(nn, kBlock)), dim3(kBlock), 0, s_, d_Y.p, d_vptr.p, d_vsrc.p,
                        d_vwt.p, d_YV.p, nn, nc);
     hipLaunchKernelGGL(k_fin3, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, s_, d_Y.p, d_YV.p, d_f3i.p, d_f3d.p,
