@@ -1575,12 +1575,15 @@ struct NDDev {
 };
 
 constexpr int kNdRows = 64;
+constexpr int kNdWide = 256;      // levels with fewer row blocks ...
+constexpr int kNdWideLen = 1024;  // ... and reductions this long take 16 waves per block
 
-__global__ void __launch_bounds__(kBlock) k_nd_forward(const int2 *__restrict__ items, NDDev F,
+template <int W>
+__global__ void __launch_bounds__(64 * W) k_nd_forward(const int2 *__restrict__ items, NDDev F,
                                                        const double *__restrict__ b, double *__restrict__ y,
                                                        double *__restrict__ u) {
     extern __shared__ double f[];
-    __shared__ double red[kWaves][kNdRows];
+    __shared__ double red[W][kNdRows];
     int2 it = items[blockIdx.x];
     int k = it.x, r0 = it.y;
     int nc = F.ncol[k], nr = F.nrow[k], m = nc + nr, c0 = F.col0[k];
@@ -1597,7 +1600,7 @@ __global__ void __launch_bounds__(kBlock) k_nd_forward(const int2 *__restrict__ 
     int i = r0 + lane;
     bool act = i < m;
     const double *Fc = F.fcm + F.f_off[k];  // element (i, a) at a * m + i
-    int per = (nc + kWaves - 1) / kWaves, a0 = w * per, a1 = min(nc, a0 + per);
+    int per = (nc + W - 1) / W, a0 = w * per, a1 = min(nc, a0 + per);
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     if (act) {
         int a = a0;
@@ -1627,17 +1630,18 @@ __global__ void __launch_bounds__(kBlock) k_nd_forward(const int2 *__restrict__ 
     __syncthreads();
     if (w == 0 && act) {
         double t = 0;
-        for (int ww = 0; ww < kWaves; ++ww) t += red[ww][lane];
+        for (int ww = 0; ww < W; ++ww) t += red[ww][lane];
         if (i < nc) y[c0 + i] = t;
         else u[F.u_off[k] + i - nc] = f[i] + t;
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_nd_backward(const int2 *__restrict__ items, NDDev F,
+template <int W>
+__global__ void __launch_bounds__(64 * W) k_nd_backward(const int2 *__restrict__ items, NDDev F,
                                                         const double *__restrict__ y, double *__restrict__ xp,
                                                         double *__restrict__ out) {
     extern __shared__ double v[];
-    __shared__ double red[kWaves][kNdRows];
+    __shared__ double red[W][kNdRows];
     int2 it = items[blockIdx.x];
     int k = it.x, r0 = it.y;
     int nc = F.ncol[k], nr = F.nrow[k], m = nc + nr, c0 = F.col0[k];
@@ -1648,7 +1652,7 @@ __global__ void __launch_bounds__(kBlock) k_nd_backward(const int2 *__restrict__
     int a = r0 + lane;
     bool act = a < nc;
     const double *Fr = F.frm + F.f_off[k];  // element (j, a) at j * nc + a
-    int per = (m + kWaves - 1) / kWaves, j0 = w * per, j1 = min(m, j0 + per);
+    int per = (m + W - 1) / W, j0 = w * per, j1 = min(m, j0 + per);
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     if (act) {
         int j = j0;
@@ -1676,7 +1680,7 @@ __global__ void __launch_bounds__(kBlock) k_nd_backward(const int2 *__restrict__
     __syncthreads();
     if (w == 0 && act) {
         double t = 0;
-        for (int ww = 0; ww < kWaves; ++ww) t += red[ww][lane];
+        for (int ww = 0; ww < W; ++ww) t += red[ww][lane];
         xp[c0 + a] = t;
         out[F.perm[c0 + a]] = t;
     }
@@ -1913,18 +1917,28 @@ bool Solver::build_matrix(const std::vector<std::vector<std::pair<int, double>>>
     // work items (supernode, first output row) per level, 64 rows each
     std::vector<int2> fwd, bwd;
     f.fwd_ptr.assign(1, 0);
+    f.fwd_len.clear();
     for (int h = 0; h <= hmax; ++h) {
+        int len = 0;
         for (int k = 0; k < nt; ++k)
-            if (F.height[k] == h)
+            if (F.height[k] == h) {
                 for (int r = 0; r < F.ncol[k] + F.nrow[k]; r += kNdRows) fwd.push_back(make_int2(k, r));
+                len = std::max(len, F.ncol[k]);
+            }
         f.fwd_ptr.push_back((int)fwd.size());
+        f.fwd_len.push_back(len);
     }
     f.bwd_ptr.assign(1, 0);
+    f.bwd_len.clear();
     for (int d = 0; d <= dmax; ++d) {
+        int len = 0;
         for (int k = 0; k < nt; ++k)
-            if (F.depth[k] == d)
+            if (F.depth[k] == d) {
                 for (int r = 0; r < F.ncol[k]; r += kNdRows) bwd.push_back(make_int2(k, r));
+                len = std::max(len, F.ncol[k] + F.nrow[k]);
+            }
         f.bwd_ptr.push_back((int)bwd.size());
+        f.bwd_len.push_back(len);
     }
     f.n = n;
     f.max_front = std::max(F.max_front, 1);
@@ -2905,17 +2919,30 @@ bool Solver::nd_solve(const DevND &f, const double *d_b, double *d_x) {
     NDDev F{f.perm.p, f.col0.p, f.ncol.p, f.nrow.p, f.child_ptr.p, f.child_idx.p, f.map.p, f.map_off.p,
             f.rows.p, f.rows_off.p, f.u_off.p, f.f_off.p, f.frm.p, f.fcm.p};
     size_t lds = (size_t)f.max_front * sizeof(double);
+    // levels of fewer row blocks than the chip has CUs whose reductions are
+    // long (the large separators near the root) split each block's reduction
+    // over 16 waves instead of 4, for more loads in flight, where the front
+    // and the 16 waves' partial sums fit 64 KB of LDS
+    bool wide = lds + 16 * kNdRows * sizeof(double) <= 65536;
     for (size_t h = 0; h + 1 < f.fwd_ptr.size(); ++h) {
         int cnt = f.fwd_ptr[h + 1] - f.fwd_ptr[h];
         if (cnt == 0) continue;
-        hipLaunchKernelGGL(k_nd_forward, dim3(cnt), dim3(kBlock), lds, s_, f.fwd_items.p + f.fwd_ptr[h], F, d_b,
-                           d_ndy.p, d_ndu.p);
+        if (wide && cnt < kNdWide && f.fwd_len[h] >= kNdWideLen)
+            hipLaunchKernelGGL(k_nd_forward<16>, dim3(cnt), dim3(1024), lds, s_, f.fwd_items.p + f.fwd_ptr[h], F, d_b,
+                               d_ndy.p, d_ndu.p);
+        else
+            hipLaunchKernelGGL(k_nd_forward<kWaves>, dim3(cnt), dim3(kBlock), lds, s_, f.fwd_items.p + f.fwd_ptr[h], F,
+                               d_b, d_ndy.p, d_ndu.p);
     }
     for (size_t d = 0; d + 1 < f.bwd_ptr.size(); ++d) {
         int cnt = f.bwd_ptr[d + 1] - f.bwd_ptr[d];
         if (cnt == 0) continue;
-        hipLaunchKernelGGL(k_nd_backward, dim3(cnt), dim3(kBlock), lds, s_, f.bwd_items.p + f.bwd_ptr[d], F, d_ndy.p,
-                           d_ndx.p, d_x);
+        if (wide && cnt < kNdWide && f.bwd_len[d] >= kNdWideLen)
+            hipLaunchKernelGGL(k_nd_backward<16>, dim3(cnt), dim3(1024), lds, s_, f.bwd_items.p + f.bwd_ptr[d], F,
+                               d_ndy.p, d_ndx.p, d_x);
+        else
+            hipLaunchKernelGGL(k_nd_backward<kWaves>, dim3(cnt), dim3(kBlock), lds, s_, f.bwd_items.p + f.bwd_ptr[d],
+                               F, d_ndy.p, d_ndx.p, d_x);
     }
     return check(hipGetLastError(), "nested-dissection solve");
 }

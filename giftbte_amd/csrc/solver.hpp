@@ -86,6 +86,7 @@ struct DevND {
     DBuf<long> f_off;
     DBuf<double> frm, fcm;
     std::vector<int> fwd_ptr, bwd_ptr;  // level pointers into fwd_items / bwd_items
+    std::vector<int> fwd_len, bwd_len;  // per level: the longest reduction of a row block
     DBuf<int2> fwd_items, bwd_items;    // (supernode, first row) per workgroup
     long u_total = 0;
 };
