@@ -137,6 +137,7 @@ class Solver {
     bool build_gv(double invLx);
     int solve3d(int max_iter, double tolT, double tolQ, gbte_stats *st);
     void launch_rhs3(bool macro, const double *bval, double *rhs);
+    void launch_vsum3();
     void launch_grad3();
     bool src_width_ok(int v) const;
     void launch_source3(const Modes &m);
@@ -245,6 +246,7 @@ class Solver {
     DBuf<char> d_g3g_ent;
     DBuf<int> d_g3g_ptr, d_g3g_idx;
     int g3_gs = 4, g3_wpb = 4;  // k_grad3g: 64-entry slices per wave, waves per workgroup (chosen by tune3d)
+    int gv_gs = 4, gv_wpb = 4;  // k_vsum3: the same, chosen on its own
     // matrix-core gradient (k_grad3m): tiles of up to four cells, their rows,
     // cells and A' operands; used when the tuner finds it faster
     bool gm_ok = false, use_gm = false;

@@ -1711,6 +1711,35 @@ void Solver::launch_rhs3(bool macro, const double *bval, double *rhs) {
                        d_vol.p, bval, d_TV.p, d_r3i[set].p, d_r3d[set].p, rhs, nc);
 }
 
+// k_vsum3 at its own shape (gv_gs slices per wave, gv_wpb waves per
+// workgroup)
+void Solver::launch_vsum3() {
+    int DB = nD * nB;
+    int ngroups = (DB + gv_gs * kG3W - 1) / (gv_gs * kG3W);
+    int gridv = ((blocks_for((long)ngroups * gv_nv, gv_wpb) + kXcd - 1) / kXcd) * kXcd;
+#define GBTE_VS(ST_, GS_, W_)                                                                                    \
+    hipLaunchKernelGGL((k_vsum3<ST_, GS_, W_>), dim3(gridv), dim3(64 * W_), 0, s_,                               \
+                       reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<ST_ *>(d_gv_sums.p), d_gv_vptr.p,  \
+                       d_gv_vidx.p, reinterpret_cast<const G3Ent<ST_> *>(d_gv_vent.p), d_gv_vref.p, gv_nv, DB,   \
+                       ngroups)
+#define GBTE_VS_W(ST_, GS_)                      \
+    do {                                         \
+        if (gv_wpb == 16) GBTE_VS(ST_, GS_, 16); \
+        else GBTE_VS(ST_, GS_, 4);               \
+    } while (0)
+#define GBTE_VS_GS(ST_)                        \
+    do {                                       \
+        if (gv_gs == 2) GBTE_VS_W(ST_, 2);     \
+        else if (gv_gs == 4) GBTE_VS_W(ST_, 4); \
+        else GBTE_VS_W(ST_, 8);                \
+    } while (0)
+    if (fp32_) GBTE_VS_GS(float);
+    else GBTE_VS_GS(double);
+#undef GBTE_VS_GS
+#undef GBTE_VS_W
+#undef GBTE_VS
+}
+
 // The 3D transport kernels of one outer iteration (and of the set-up), on
 // the solver's stream.
 void Solver::launch_grad3() {
@@ -1737,16 +1766,12 @@ void Solver::launch_grad3() {
                                nB, spg);
         return;
     }
+    if (use_gv) launch_vsum3();
     int ngroups = (DB + g3_gs * kG3W - 1) / (g3_gs * kG3W);
     int grid = ((blocks_for((long)ngroups * nc, g3_wpb) + kXcd - 1) / kXcd) * kXcd;
-    int gridv = ((blocks_for((long)ngroups * gv_nv, g3_wpb) + kXcd - 1) / kXcd) * kXcd;
-    // the vertex sums, then the projections from them
+    // the projections from the vertex sums (launched above)
 #define GBTE_G3V(ST_, GS_, W_)                                                                                   \
     do {                                                                                                         \
-        hipLaunchKernelGGL((k_vsum3<ST_, GS_, W_>), dim3(gridv), dim3(64 * W_), 0, s_,                           \
-                           reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<ST_ *>(d_gv_sums.p),           \
-                           d_gv_vptr.p, d_gv_vidx.p, reinterpret_cast<const G3Ent<ST_> *>(d_gv_vent.p),          \
-                           d_gv_vref.p, gv_nv, DB, ngroups);                                                     \
         hipLaunchKernelGGL((k_grad3v<ST_, GS_, W_>), dim3(grid), dim3(64 * W_), 0, s_,                           \
                            reinterpret_cast<const ST_ *>(d_e.p), reinterpret_cast<const ST_ *>(d_gv_sums.p),     \
                            reinterpret_cast<ST_ *>(d_grad.p), reinterpret_cast<const G3VRec<ST_> *>(d_gv_rec.p), \
@@ -1964,6 +1989,27 @@ bool Solver::tune3d() {
         // the gather shapes are timed only when the vertex sums do not take
         // the mesh or the gather is asked for
         if (want == GBTE_GRAD_AUTO && gv_ok) want_g = false;
+        // the vertex sums first, at their own shape (the projections read
+        // them in any slice order); then the projections' shapes, each
+        // timed as the pair
+        if (want_v && gv_ok) {
+            double bv = 1e300;
+            int pgs = 4, pw = 4;
+            for (int wpb : {4, 16})
+                for (int gs : {2, 4, 8}) {
+                    if (gs == 8 && !fp32_) continue;
+                    gv_gs = gs;
+                    gv_wpb = wpb;
+                    double per = 0;
+                    if (!time_it([&]() { launch_vsum3(); }, bv, per)) return false;
+                    snprintf(line, sizeof line, "k_vsum3 %d slices per wave, %d waves per workgroup: %.3f ms per launch\n",
+                             gs, wpb, per);
+                    tune_log_ += line;
+                    if (per < bv) { bv = per; pgs = gs; pw = wpb; }
+                }
+            gv_gs = pgs;
+            gv_wpb = pw;
+        }
         for (int v : {1, 0}) {
             if (v == 0 ? !want_g : !(want_v && gv_ok)) continue;
             use_gv = v == 1;
@@ -1997,8 +2043,11 @@ bool Solver::tune3d() {
         if (use_gm)
             snprintf(line, sizeof line, "gradient chosen: k_grad3m (matrix cores)\n");
         else
-            snprintf(line, sizeof line, "gradient chosen: %s, %d slices per wave, %d waves per workgroup\n",
-                     use_gv ? "k_vsum3 + k_grad3v" : "k_grad3g", pick, pickw);
+            snprintf(line, sizeof line,
+                     use_gv ? "gradient chosen: k_vsum3 + k_grad3v, %d slices per wave, %d waves per workgroup "
+                              "(k_vsum3 %d and %d)\n"
+                            : "gradient chosen: k_grad3g, %d slices per wave, %d waves per workgroup\n",
+                     pick, pickw, gv_gs, gv_wpb);
         tune_log_ += line;
     }
     // sweeps: the wave-specialised kernel (when its schedule was built), the
