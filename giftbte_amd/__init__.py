@@ -120,7 +120,7 @@ class BTESolver:
 
     def __init__(self, problem: Problem, precision: str = "fp64", device: int = 0, verbose: bool = False,
                  macro_rtol: float = 0.0, linear_solver: str = "direct", fallback_kernels: bool = False,
-                 sweep_kernel: str = "auto", gradient_kernel: str = "auto"):
+                 sweep_kernel: str = "auto", gradient_kernel: str = "auto", source_layout: str = "auto"):
         self.problem = problem
         self.lib = problem.lib
         t = problem.tables
@@ -135,10 +135,14 @@ class BTESolver:
         grads = {"auto": 0, "gather": 1, "matrix": 2, "vertex": 3}
         if gradient_kernel not in grads:
             raise ValueError(f"gradient_kernel must be one of {sorted(grads)}")
+        layouts = {"auto": 0, "chunk": 1, "position": 2, "cell": 3}
+        if source_layout not in layouts:
+            raise ValueError(f"source_layout must be one of {sorted(layouts)}")
         opts = _lib.Options(_lib.GBTE_FP64 if precision == "fp64" else _lib.GBTE_FP32_SWEEP, device, int(verbose),
                             macro_rtol,
                             _lib.GBTE_LINSOLVE_DIRECT if linear_solver == "direct" else _lib.GBTE_LINSOLVE_PCG,
-                            int(bool(fallback_kernels)), sweeps[sweep_kernel], grads[gradient_kernel])
+                            int(bool(fallback_kernels)), sweeps[sweep_kernel], grads[gradient_kernel],
+                            layouts[source_layout])
         self._h = C.c_void_p()
         rc = self.lib.gbte_create(problem.handle, C.byref(opts), C.byref(self._h))
         if rc != 0:

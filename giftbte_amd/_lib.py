@@ -78,7 +78,7 @@ class Tables(C.Structure):
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("macro_rtol", C.c_double),
                 ("linear_solver", C.c_int), ("fallback_kernels", C.c_int), ("sweep_kernel", C.c_int),
-                ("gradient_kernel", C.c_int)]
+                ("gradient_kernel", C.c_int), ("source_layout", C.c_int)]
 
 
 class Stats(C.Structure):

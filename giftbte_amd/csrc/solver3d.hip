@@ -76,6 +76,11 @@ constexpr int kG3Ent = 8;         // stencil entries per batch
 constexpr int kG3Pad = 4;         // stencils padded to a multiple (a batch of 8, then one of 4)
 constexpr int kFin = 21;          // k_fin3 coefficients per face slot
 constexpr int kMomRows = 4;       // k_mom3: rows (waves) per workgroup
+// S layouts between the source and the chunked sweep (s_sh): >= 0
+// [dir][band chunk][position][2^s_sh], -1 [dir][position][band], kSCell
+// [cell][dir][band] (e's layout: the source's stores are whole rows, the
+// sweep reads S at the position's cell like its upwind values)
+constexpr int kSCell = -2;
 
 // One LSQ stencil entry of k_grad3g: the neighbour row and the weights / L_x.
 template <typename AT>
@@ -620,8 +625,9 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3(const int *__restric
         double s = To * mcap[b] + Qc * mhr[b] * mtau[b];
 #pragma unroll
         for (int f = 0; f < 4; ++f) s -= tau * fav[f] * dot[f] * u[f];
-        size_t si = sh < 0 ? ((size_t)d * nc + pc[d]) * nB + b
-                           : ((((size_t)d * (nB >> sh) + (b >> sh)) * nc + pc[d]) << sh) + (b & ((1 << sh) - 1));
+        size_t si = sh == kSCell ? (size_t)c * DB + t
+                    : sh < 0     ? ((size_t)d * nc + pc[d]) * nB + b
+                                 : ((((size_t)d * (nB >> sh) + (b >> sh)) * nc + pc[d]) << sh) + (b & ((1 << sh) - 1));
         __builtin_nontemporal_store((ST)s, S + si);
     }
 }
@@ -703,8 +709,9 @@ __global__ void __launch_bounds__(64 * kSrc3Rows) k_source3_f32v(
             }
             o[j] = (float)sj;
         }
-        size_t si = sh < 0 ? ((size_t)d * nc + pc[d]) * nB + b
-                           : ((((size_t)d * (nB >> sh) + (b >> sh)) * nc + pc[d]) << sh) + (b & ((1 << sh) - 1));
+        size_t si = sh == kSCell ? (size_t)c * DB + t
+                    : sh < 0     ? ((size_t)d * nc + pc[d]) * nB + b
+                                 : ((((size_t)d * (nB >> sh) + (b >> sh)) * nc + pc[d]) << sh) + (b & ((1 << sh) - 1));
         __builtin_nontemporal_store(o, reinterpret_cast<VT *>(S + si));
     }
 }
@@ -757,7 +764,7 @@ __device__ __forceinline__ void vst(T *p, const T (&v)[N]) {
 // per position set per workgroup (the last chunk of a direction may be
 // narrower: 40 FP32 bands are chunks of 16, 16 and 8, four or two lanes per
 // position), S per position.
-template <typename ST, int BC, int NP, int TPB, int V>
+template <typename ST, int BC, int NP, int TPB, int V, bool CM>
 __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec, const int *__restrict__ levptr,
                                                   const int *__restrict__ dlev, const ST *__restrict__ S,
                                                   ST *__restrict__ e, Modes m, int nc, int nB, int nD, int smajor) {
@@ -781,10 +788,13 @@ __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec,
     for (int i = threadIdx.x; i <= L; i += TPB) slp[i] = levptr[dlev[d] + i];
     __syncthreads();
     const LvRec *rd = rec + (size_t)d * nc;
-    // S[dir][chunk][position][BC], or S[dir][position][band] (smajor)
+    // S[dir][chunk][position][BC], S[dir][position][band] (smajor), or
+    // S[cell][dir][band] (CM: read at the position's cell with the upwind
+    // values, once its record has arrived)
     const ST *Sd = smajor ? S + (size_t)d * nc * nB + b0 : S + ((size_t)d * nchunk + chunk) * nc * BC + sub * BL;
     size_t sst = smajor ? (size_t)nB : (size_t)BC;
     size_t DB = (size_t)nD * nB;
+    const ST *Sc = S + (size_t)d * nB + b0;
     ST *ed = e + (size_t)d * nB + b0;
     // Every load of the pipelined part is unconditional (lanes past the end
     // of a level re-read its last position, absent upwind cells read row 0,
@@ -824,7 +834,7 @@ __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec,
         for (int i = 0; i < NP; ++i) {
             int p = min(p0 + tid + i * PP, a1 - 1);
             rr[i] = rd[p];
-            vld(ss[i], Sd + (size_t)p * sst);
+            if constexpr (!CM) vld(ss[i], Sd + (size_t)p * sst);
         }
     };
     auto chunk_step = [&](int l, int p0, LvRec(&rc)[NP], ST(&sc)[NP][BL], LvRec(&rn)[NP], ST(&sn)[NP][BL],
@@ -834,6 +844,10 @@ __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec,
         ST u[NP][3][BL];
 #pragma unroll
         for (int i = 0; i < NP; ++i) upwind(rc[i], u[i]);
+        if constexpr (CM) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i) vld(sc[i], Sc + (size_t)rc[i].cell * DB);
+        }
         fetch(nl < L ? nl : l, nl < L ? np0 : p0, rn, sn);
         // a wave none of whose positions lies in the chunk skips the
         // arithmetic (a uniform branch with no load in it)
@@ -1840,10 +1854,15 @@ void Solver::launch_source3(const Modes &m) {
 void Solver::launch_sweep_lv(const Modes &m, int bc) {
     int grid = ((nD * (nB / bc) + kXcd - 1) / kXcd) * kXcd;
     size_t lds = (size_t)(nlev + 1) * sizeof(int);
-#define GBTE_LV(ST_, BC_, NP_, TPB_, V_)                                                                             \
-    hipLaunchKernelGGL((k_sweep_lv<ST_, BC_, NP_, TPB_, V_>), dim3(grid), dim3(TPB_), lds, s_, d_lv_rec.p,           \
+#define GBTE_LV1(ST_, BC_, NP_, TPB_, V_, CM_)                                                                       \
+    hipLaunchKernelGGL((k_sweep_lv<ST_, BC_, NP_, TPB_, V_, CM_>), dim3(grid), dim3(TPB_), lds, s_, d_lv_rec.p,      \
                        d_levptr.p, d_dlev.p, reinterpret_cast<const ST_ *>(d_S.p), reinterpret_cast<ST_ *>(d_e.p), m, \
                        nc, nB, nD, s_sh < 0 ? 1 : 0)
+#define GBTE_LV(ST_, BC_, NP_, TPB_, V_)                           \
+    do {                                                           \
+        if (s_sh == kSCell) GBTE_LV1(ST_, BC_, NP_, TPB_, V_, true); \
+        else GBTE_LV1(ST_, BC_, NP_, TPB_, V_, false);             \
+    } while (0)
 #define GBTE_LV_NP(ST_, BC_)                            \
     do {                                                \
         if (lv_np == 4) GBTE_LV(ST_, BC_, 4, 256, 1);   \
@@ -1902,6 +1921,7 @@ void Solver::launch_sweep_lv(const Modes &m, int bc) {
 #undef GBTE_LV_NP2
 #undef GBTE_LV_NP
 #undef GBTE_LV
+#undef GBTE_LV1
 }
 
 void Solver::launch_bound_out3(const Modes &m) {
@@ -2147,14 +2167,32 @@ bool Solver::tune3d() {
         else
             snprintf(line, sizeof line, "sweep chosen: %s\n", use_pc ? "wave-specialised" : "level-synchronous");
         tune_log_ += line;
+        int wl = opt.source_layout;
+        if (wl < GBTE_SLAYOUT_AUTO || wl > GBTE_SLAYOUT_CELL) { err_ = "unknown source_layout"; return false; }
+        if (wl != GBTE_SLAYOUT_AUTO && pick.kind != 2) {
+            err_ = "source_layout applies to the chunked sweep only";
+            return false;
+        }
+        if (wl == GBTE_SLAYOUT_CHUNK && s_sh < 0) {
+            err_ = "S per band chunk needs a sweep chunk of a power-of-two band count";
+            return false;
+        }
         if (pick.kind == 2) {
             // the layout of S between the source kernel and the chunked sweep:
-            // per band chunk (the sweep's loads contiguous) or per position
-            // with all bands (the source's stores contiguous); the pair is timed
+            // per band chunk (the sweep's loads contiguous), per position with
+            // all bands, or per cell in e's layout (the source's stores whole
+            // rows, the sweep's loads at the positions' cells); the pair is
+            // timed
             int shc = s_sh;
-            double tl[2] = {0, 0};
-            for (int v = 0; v < 2; ++v) {
-                s_sh = v == 0 ? shc : -1;
+            double tl[3] = {0, 0, 0};
+            const int lay[3] = {shc, -1, kSCell};
+            const char *lname[3] = {"per band chunk", "per position", "per cell (e's layout)"};
+            for (int v = 0; v < 3; ++v) {
+                if ((v == 0 && shc < 0) || (wl != GBTE_SLAYOUT_AUTO && wl != v + 1)) {
+                    tl[v] = 1e30;  // per position is then the sweep's own layout, or another one is asked for
+                    continue;
+                }
+                s_sh = lay[v];
                 launch_source3(m);
                 launch_sweep3(m);  // warm
                 (void)hipEventRecord(a, s_);
@@ -2167,12 +2205,14 @@ bool Solver::tune3d() {
                 float ms = 0;
                 (void)hipEventElapsedTime(&ms, a, b);
                 tl[v] = ms / 3.0;
-                snprintf(line, sizeof line, "source + sweep, S %s: %.3f ms per pair\n",
-                         v == 0 ? "per band chunk" : "per position", tl[v]);
+                snprintf(line, sizeof line, "source + sweep, S %s: %.3f ms per pair\n", lname[v], tl[v]);
                 tune_log_ += line;
             }
-            s_sh = tl[0] <= tl[1] ? shc : -1;
-            snprintf(line, sizeof line, "S layout chosen: %s\n", s_sh < 0 ? "per position" : "per band chunk");
+            int lb = 0;
+            for (int v = 1; v < 3; ++v)
+                if (tl[v] < tl[lb]) lb = v;
+            s_sh = lay[lb];
+            snprintf(line, sizeof line, "S layout chosen: %s\n", lname[lb]);
             tune_log_ += line;
         }
     }

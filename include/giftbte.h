@@ -141,10 +141,18 @@ typedef struct gbte_options {
                              matrix cores) or GBTE_GRAD_VERTEX (sums over the mesh
                              vertices, then the remaining stencil entries; tetrahedra of
                              one material); same results to rounding */
+    int source_layout;    /* tetrahedral meshes with the chunked sweep: the layout of the
+                             sweep source between the source kernel and the sweep,
+                             GBTE_SLAYOUT_AUTO (default: the tuner times the pair in each)
+                             or GBTE_SLAYOUT_CHUNK ([dir][band chunk][position][chunk]),
+                             GBTE_SLAYOUT_POSITION ([dir][position][band]) or
+                             GBTE_SLAYOUT_CELL ([cell][dir][band], e's layout); same
+                             results */
 } gbte_options;
 
 enum { GBTE_SWEEP_AUTO = 0, GBTE_SWEEP_WAVESPEC = 1, GBTE_SWEEP_LEVEL = 2, GBTE_SWEEP_CHUNKED = 3, GBTE_SWEEP_LANES = 4 };
 enum { GBTE_GRAD_AUTO = 0, GBTE_GRAD_GATHER = 1, GBTE_GRAD_MATRIX = 2, GBTE_GRAD_VERTEX = 3 };
+enum { GBTE_SLAYOUT_AUTO = 0, GBTE_SLAYOUT_CHUNK = 1, GBTE_SLAYOUT_POSITION = 2, GBTE_SLAYOUT_CELL = 3 };
 
 typedef struct gbte_stats {
     int iterations;       /* outer iterations run                               */

@@ -37,16 +37,25 @@ SWEEPS = [("cube3d_b8", "fp64", "wavespec", "auto"), ("cube3d_b8", "fp64", "leve
 GRADS = [(n, pr, "auto", g) for n in DECKS3D for pr in ("fp64", "fp32") for g in ("gather", "matrix", "vertex")]
 
 
-@pytest.mark.parametrize("name,precision,sweep,grad",
-                         [(n, pr, "auto", "auto") for n in DECKS3D for pr in ("fp64", "fp32")] + SWEEPS + GRADS)
-def test_3d_matches_oracle(name, precision, sweep, grad, deck_root):
+# each layout of the sweep source, forced, with the chunked sweep
+LAYOUTS = [("cube3d_b8", pr, "chunked", "auto", lay) for pr in ("fp64", "fp32") for lay in ("chunk", "position", "cell")]
+
+
+@pytest.mark.parametrize("name,precision,sweep,grad,layout",
+                         [(n, pr, "auto", "auto", "auto") for n in DECKS3D for pr in ("fp64", "fp32")]
+                         + [c + ("auto",) for c in SWEEPS + GRADS] + LAYOUTS)
+def test_3d_matches_oracle(name, precision, sweep, grad, layout, deck_root):
     from giftbte_amd import Problem, BTESolver
     d = decklib.make(name, deck_root)
     s = decklib.settings(name)
-    print(f"\n[gpu 3d] deck={name} precision={precision} sweep={sweep} gradient={grad} settings={s}", flush=True)
+    print(f"\n[gpu 3d] deck={name} precision={precision} sweep={sweep} gradient={grad} S layout={layout} "
+          f"settings={s}", flush=True)
     p = Problem(d)
-    sol = BTESolver(p, precision=precision, sweep_kernel=sweep, gradient_kernel=grad)
+    sol = BTESolver(p, precision=precision, sweep_kernel=sweep, gradient_kernel=grad, source_layout=layout)
     log = sol.tuning_log()
+    if layout != "auto":
+        names = {"chunk": "per band chunk", "position": "per position", "cell": "per cell"}
+        assert "S layout chosen: " + names[layout] in log
     print("  " + log.replace("\n", "; "), flush=True)
     if (name, precision, sweep) == ("cube3d_b20", "fp32", "lanes"):
         assert "sweep chosen: chunked up to 16 bands" in log  # the only lane shape 20 FP32 bands admit
