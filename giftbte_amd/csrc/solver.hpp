@@ -139,6 +139,7 @@ class Solver {
     void launch_rhs3(bool macro, const double *bval, double *rhs);
     void launch_vsum3();
     void launch_grad3();
+    double *sbuf();
     bool src_width_ok(int v) const;
     void launch_source3(const Modes &m);
     void launch_bound_out3(const Modes &m);

@@ -766,8 +766,8 @@ __device__ __forceinline__ void vst(T *p, const T (&v)[N]) {
 // position), S per position.
 template <typename ST, int BC, int NP, int TPB, int V, bool CM>
 __global__ void __launch_bounds__(TPB) k_sweep_lv(const LvRec *__restrict__ rec, const int *__restrict__ levptr,
-                                                  const int *__restrict__ dlev, const ST *__restrict__ S,
-                                                  ST *__restrict__ e, Modes m, int nc, int nB, int nD, int smajor) {
+                                                  const int *__restrict__ dlev, const ST *S, ST *e, Modes m, int nc,
+                                                  int nB, int nD, int smajor) {
     constexpr int BL = V > 0 ? BC / V : 16 / (int)sizeof(ST);  // bands per lane
     static_assert((V == 0 || (BC % V == 0 && TPB % V == 0)) && BC % BL == 0 && (BL * sizeof(ST)) % 16 == 0,
                   "whole 16-byte vectors per lane");
@@ -1828,24 +1828,31 @@ bool Solver::src_width_ok(int v) const {
     return v == 1 || (fp32_ && nB % v == 0 && (s_sh < 0 || (1 << s_sh) % v == 0));
 }
 
+// The sweep source: in e's layout it is written into e itself, each entry
+// then read by the sweep just before the sweep overwrites it with the new e
+// (the source reads no e, and nothing reads the old e after it), so S and e
+// share their cache lines and pages
+double *Solver::sbuf() { return s_sh == kSCell ? d_e.p : d_S.p; }
+
 void Solver::launch_source3(const Modes &m) {
     int grid = ((blocks_for(nc, kSrc3Rows) + kXcd - 1) / kXcd) * kXcd;
     size_t lds = (size_t)(4 * nB + 3 * nD) * sizeof(double) + (size_t)kSrc3Rows * nD * sizeof(int);
     int v = fp32_ ? (src_v > 0 && src_width_ok(src_v) ? src_v : src_width_ok(2) ? 2 : 1) : 1;
+    double *S = sbuf();
     if (v == 4)
         hipLaunchKernelGGL(k_source3_f32v<4>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
-                           reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
+                           reinterpret_cast<float *>(S), reinterpret_cast<const float *>(d_grad.p),
                            reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
     else if (v == 2)
         hipLaunchKernelGGL(k_source3_f32v<2>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
-                           reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
+                           reinterpret_cast<float *>(S), reinterpret_cast<const float *>(d_grad.p),
                            reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
     else if (fp32_)
         hipLaunchKernelGGL(k_source3<float>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p,
-                           reinterpret_cast<float *>(d_S.p), reinterpret_cast<const float *>(d_grad.p),
+                           reinterpret_cast<float *>(S), reinterpret_cast<const float *>(d_grad.p),
                            reinterpret_cast<const float *>(d_eb.p), d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
     else
-        hipLaunchKernelGGL(k_source3<double>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p, d_S.p, d_grad.p,
+        hipLaunchKernelGGL(k_source3<double>, dim3(grid), dim3(64 * kSrc3Rows), lds, s_, d_posc.p, S, d_grad.p,
                            d_eb.p, d_Told.p, d_Q.p, d_s3d.p, d_s3i.p, m, nc, nB, nD, s_sh);
 }
 
@@ -1856,7 +1863,7 @@ void Solver::launch_sweep_lv(const Modes &m, int bc) {
     size_t lds = (size_t)(nlev + 1) * sizeof(int);
 #define GBTE_LV1(ST_, BC_, NP_, TPB_, V_, CM_)                                                                       \
     hipLaunchKernelGGL((k_sweep_lv<ST_, BC_, NP_, TPB_, V_, CM_>), dim3(grid), dim3(TPB_), lds, s_, d_lv_rec.p,      \
-                       d_levptr.p, d_dlev.p, reinterpret_cast<const ST_ *>(d_S.p), reinterpret_cast<ST_ *>(d_e.p), m, \
+                       d_levptr.p, d_dlev.p, reinterpret_cast<const ST_ *>(sbuf()), reinterpret_cast<ST_ *>(d_e.p), m, \
                        nc, nB, nD, s_sh < 0 ? 1 : 0)
 #define GBTE_LV(ST_, BC_, NP_, TPB_, V_)                           \
     do {                                                           \
