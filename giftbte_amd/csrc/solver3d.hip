@@ -2246,7 +2246,11 @@ bool Solver::tune3d() {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (opt.verbose) printf("%s", tune_log_.c_str());
-    return check(hipGetLastError(), "tune launches");
+    if (!check(hipGetLastError(), "tune launches")) return false;
+    // S lives in e from here on (the chunked sweep is the only one that
+    // takes this layout): its own buffer is freed
+    if (s_sh == kSCell) d_S.release();
+    return true;
 }
 
 }  // namespace gbte
