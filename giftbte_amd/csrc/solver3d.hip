@@ -2121,6 +2121,9 @@ bool Solver::tune3d() {
                 continue;
             for (int np : {2, 4}) cands.push_back({2, bc, np, 0, 512});
         }
+        // the lane forms first: usually the fastest, they set the bar early so
+        // that slower candidates get only their warm and one timed launch
+        std::stable_partition(cands.begin(), cands.end(), [](const Cand &c) { return c.kind == 2 && c.v != 1; });
         double best = 1e300;
         Cand pick{-1, 0, 0};
         auto chunk_sh = [](int bc) {  // S per band chunk where the chunk is a power of two
