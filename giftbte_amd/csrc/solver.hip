@@ -1575,8 +1575,8 @@ struct NDDev {
 };
 
 constexpr int kNdRows = 64;
-constexpr int kNdWide = 256;      // levels with fewer row blocks ...
-constexpr int kNdWideLen = 1024;  // ... and reductions this long take 16 waves per block
+constexpr int kNdWide = 512;      // levels with fewer row blocks ...
+constexpr int kNdWideLen = 512;   // ... and reductions this long take 16 waves per block
 
 template <int W>
 __global__ void __launch_bounds__(64 * W) k_nd_forward(const int2 *__restrict__ items, NDDev F,

@@ -915,7 +915,7 @@ __global__ void __launch_bounds__(64 * kMomRows) k_mom3(const ST *__restrict__ e
     double f[kDirF];
 #pragma unroll
     for (int j = 0; j < kDirF; ++j) f[j] = 0;
-#pragma unroll 4
+#pragma unroll 8
     for (int d = 0; d < nD; ++d) {
         double v = row[d * nB + b];
         const double *df = dirf + (size_t)d * kDirF;
