@@ -881,17 +881,29 @@ __global__ void __launch_bounds__(256) k_bound_out3(const ST *__restrict__ e, co
                                                     const int *__restrict__ be_cell, const int *__restrict__ be_slot,
                                                     const double *__restrict__ snx, const double *__restrict__ sny,
                                                     const double *__restrict__ snz, int n_ent, int nB, int nD) {
+    // the band and direction constants in LDS (read per lane there instead
+    // of from memory)
+    extern __shared__ double bo_lds[];
+    double *mvg = bo_lds, *msx = mvg + nB, *msy = msx + nD, *msz = msy + nD;
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) mvg[i] = m.vg[i];
+    for (int i = threadIdx.x; i < nD; i += blockDim.x) {
+        msx[i] = m.sx[i];
+        msy[i] = m.sy[i];
+        msz[i] = m.sz[i];
+    }
+    __syncthreads();
     int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
     if (k >= n_ent) return;
     int en = ent_list[k], c = be_cell[en], s = be_slot[en];
     double nx = snx[s], ny = sny[s], nz = snz[s];
     int DB = nD * nB;
+    float rnB = 1.0f / (float)nB;
     const ST *ec = e + (size_t)c * DB, *pf = phi + ((size_t)c * 4 + (s - c * 6)) * DB;
     ST *eo = eb + (size_t)en * DB;
     for (int t = lane; t < DB; t += 64) {
-        int d = t / nB, b = t - d * nB;
-        double vb = m.vg[b];
-        double dot = vb * m.sx[d] * nx + vb * m.sy[d] * ny + vb * m.sz[d] * nz;
+        int d = dir_of(t, rnB), b = t - d * nB;
+        double vb = mvg[b];
+        double dot = vb * msx[d] * nx + vb * msy[d] * ny + vb * msz[d] * nz;
         if (dot >= 0) eo[t] = (ST)((double)ec[t] + (double)pf[t]);
     }
 }
@@ -1935,12 +1947,13 @@ void Solver::launch_bound_out3(const Modes &m) {
     int n_ent = (int)h_ent_list.size();
     if (n_ent == 0) return;
     dim3 grid(blocks_for((long)n_ent * 64, 256));
+    size_t bo_lds = (size_t)(nB + 3 * nD) * sizeof(double);
     if (fp32_)
-        hipLaunchKernelGGL(k_bound_out3<float>, grid, dim3(256), 0, s_, reinterpret_cast<const float *>(d_e.p),
+        hipLaunchKernelGGL(k_bound_out3<float>, grid, dim3(256), bo_lds, s_, reinterpret_cast<const float *>(d_e.p),
                            reinterpret_cast<const float *>(d_grad.p), reinterpret_cast<float *>(d_eb.p), m,
                            d_ent_list.p, d_be_cell.p, d_be_slot.p, d_snx.p, d_sny.p, d_snz.p, n_ent, nB, nD);
     else
-        hipLaunchKernelGGL(k_bound_out3<double>, grid, dim3(256), 0, s_, d_e.p, d_grad.p, d_eb.p, m, d_ent_list.p,
+        hipLaunchKernelGGL(k_bound_out3<double>, grid, dim3(256), bo_lds, s_, d_e.p, d_grad.p, d_eb.p, m, d_ent_list.p,
                            d_be_cell.p, d_be_slot.p, d_snx.p, d_sny.p, d_snz.p, n_ent, nB, nD);
 }
 
